@@ -1,0 +1,199 @@
+/*
+ * louvain_hip.h -- C ABI of the MI355X (gfx950) Louvain library
+ * (liblouvain_hip.so).
+ *
+ * Drop-in for the four OpenMP entry points of
+ * puzzlef/louvain-communities-openmp-dynamic:
+ *
+ *   louvainStaticOmp(x, o)                                    inc/louvain.hxx:1285
+ *   louvainNaiveDynamicOmp(y, dels, ins, q, qvtot, qctot, o)  inc/louvain.hxx:1349
+ *   louvainDynamicDeltaScreeningOmp(...)                      inc/louvain.hxx:1542
+ *   louvainDynamicFrontierOmp(...)                            inc/louvain.hxx:1674
+ *
+ * with LouvainOptions (inc/louvain.hxx:30-63) and LouvainResult
+ * (inc/louvain.hxx:77-145) mirrored field for field.  Types are the
+ * reference's: vertex ids u32 (main.cxx:148), edge weights f32
+ * (main.cxx:19-21), K / Sigma_tot / M in f64 (inc/louvain.hxx:67).
+ *
+ * Conventions: every function returns 0 (LV_OK) or a negative lv_status;
+ * nothing throws across the ABI.  One lv_context per host thread; a context
+ * owns one HIP stream and grow-only device buffers.  The caller owns every
+ * host buffer.  Graph rows must be sorted by neighbour id with one entry per
+ * (u, v) (the order inc/_bitset.hxx:245-262 keeps); batches must be tidy:
+ * existence-filtered, sorted by (u, v), unique, both directions present
+ * (inc/batch.hxx:202-209).  Unsorted batches are rejected with
+ * LV_ERR_UNSORTED.
+ */
+#ifndef LOUVAIN_HIP_H
+#define LOUVAIN_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum lv_status {
+  LV_OK = 0,
+  LV_ERR_INVALID = -1,   /* bad argument (null pointer, size mismatch, id out of range) */
+  LV_ERR_NOMEM = -2,     /* device allocation failed */
+  LV_ERR_HIP = -3,       /* a HIP runtime call failed (see lv_last_error) */
+  LV_ERR_UNSORTED = -4,  /* batch not sorted by (u, v) / not unique */
+  LV_ERR_NODEVICE = -5,  /* no HIP device */
+  LV_ERR_LIMIT = -6      /* input exceeds a documented limit */
+} lv_status;
+
+typedef enum lv_approach {
+  LV_STATIC = 0,
+  LV_NAIVE_DYNAMIC = 1,
+  LV_DELTA_SCREENING = 2,
+  LV_DYNAMIC_FRONTIER = 3
+} lv_approach;
+
+/* LouvainOptions, inc/louvain.hxx:30-63; defaults (lv_options_default) from :60. */
+typedef struct lv_options {
+  int32_t repeat;                /* [1]    */
+  double resolution;             /* [1]    */
+  double tolerance;              /* [1e-2] */
+  double aggregation_tolerance;  /* [1.0]  */
+  double tolerance_drop;         /* [10]   */
+  int32_t max_iterations;        /* [20]   */
+  int32_t max_passes;            /* [10]   */
+} lv_options;
+
+/* A host graph: CSR (gapped if degrees != NULL) with rows sorted by key.
+ * Exactly one of weights / weights_f64 is set (f64 only for super-graphs fed
+ * to the phase entry points).  exists == NULL means every id < span exists
+ * (DiGraph::hasVertex, inc/Graph.hxx:148-150). */
+typedef struct lv_graph {
+  uint64_t span;              /* max vertex id + 1 */
+  const uint64_t* offsets;    /* span + 1 */
+  const uint32_t* degrees;    /* span, or NULL: offsets[u+1] - offsets[u] */
+  const uint32_t* keys;       /* offsets[span] */
+  const float* weights;       /* offsets[span] */
+  const double* weights_f64;  /* offsets[span] (alternative to weights) */
+  const uint8_t* exists;      /* span, or NULL */
+} lv_graph;
+
+/* A batch of edges as three parallel arrays (the reference's
+ * vector<tuple<K,K,V>>, whose in-memory layout is implementation-defined). */
+typedef struct lv_batch {
+  const uint32_t* u;
+  const uint32_t* v;
+  const float* w;
+  uint64_t size;
+} lv_batch;
+
+/* LouvainResult, inc/louvain.hxx:77-145.  The three arrays are caller-owned,
+ * span entries each.  Times are milliseconds of the reference's timed region
+ * (inc/louvain.hxx:1167-1217), averaged over repeat. */
+typedef struct lv_result {
+  uint32_t* membership;
+  double* vertex_weight;
+  double* community_weight;
+  int32_t iterations;
+  int32_t passes;
+  float time;
+  float marking_time;
+  float initialization_time;
+  float first_pass_time;
+  float local_move_time;
+  float aggregation_time;
+  uint64_t affected_vertices;
+} lv_result;
+
+typedef struct lv_context lv_context;
+
+/* Library / context ---------------------------------------------------- */
+void lv_options_default(lv_options* o);
+const char* lv_status_string(int status);
+int lv_context_create(int device, lv_context** out);
+void lv_context_destroy(lv_context* ctx);
+const char* lv_last_error(const lv_context* ctx);
+/* Path used by the local-moving phase: 0 = sweep all vertices each
+ * iteration with a host convergence check (the plain path), 1 = frontier
+ * worklists with device-side convergence (default). */
+int lv_context_set_path(lv_context* ctx, int path);
+
+/* The four entry points (host buffers in, host buffers out) ------------ */
+/* louvainStaticOmp, inc/louvain.hxx:1285-1296 */
+int lv_louvain_static(lv_context* ctx, const lv_graph* x, const lv_options* o, lv_result* res);
+/* louvainNaiveDynamicOmp, inc/louvain.hxx:1349-1366 */
+int lv_louvain_naive_dynamic(lv_context* ctx, const lv_graph* y, const lv_batch* deletions,
+                             const lv_batch* insertions, const uint32_t* q, const double* qvtot,
+                             const double* qctot, const lv_options* o, lv_result* res);
+/* louvainDynamicDeltaScreeningOmp, inc/louvain.hxx:1542-1565 */
+int lv_louvain_delta_screening(lv_context* ctx, const lv_graph* y, const lv_batch* deletions,
+                               const lv_batch* insertions, const uint32_t* q, const double* qvtot,
+                               const double* qctot, const lv_options* o, lv_result* res);
+/* louvainDynamicFrontierOmp, inc/louvain.hxx:1674-1690 */
+int lv_louvain_dynamic_frontier(lv_context* ctx, const lv_graph* y, const lv_batch* deletions,
+                                const lv_batch* insertions, const uint32_t* q, const double* qvtot,
+                                const double* qctot, const lv_options* o, lv_result* res);
+
+/* Device-resident form: graph, batch and state stay in HBM between calls.
+ * lv_run is the same algorithm as the four entry points above (approach
+ * selects which); the state holds (membership, K, Sigma_tot) in and out,
+ * the carry of main.cxx:127-135.  res->membership / vertex_weight /
+ * community_weight may be NULL (nothing is copied back). */
+typedef struct lv_dgraph lv_dgraph;
+typedef struct lv_dbatch lv_dbatch;
+typedef struct lv_dstate lv_dstate;
+int lv_dgraph_create(lv_context* ctx, const lv_graph* g, lv_dgraph** out);
+void lv_dgraph_destroy(lv_dgraph* g);
+uint64_t lv_dgraph_span(const lv_dgraph* g);
+uint64_t lv_dgraph_edges(const lv_dgraph* g);
+int lv_dbatch_create(lv_context* ctx, const lv_batch* deletions, const lv_batch* insertions, lv_dbatch** out);
+void lv_dbatch_destroy(lv_dbatch* b);
+int lv_dstate_create(lv_context* ctx, uint64_t span, lv_dstate** out);
+void lv_dstate_destroy(lv_dstate* s);
+int lv_dstate_upload(lv_context* ctx, lv_dstate* s, const uint32_t* q, const double* qvtot, const double* qctot);
+int lv_dstate_download(lv_context* ctx, const lv_dstate* s, uint32_t* q, double* qvtot, double* qctot);
+int lv_dstate_copy(lv_context* ctx, lv_dstate* dst, const lv_dstate* src);
+int lv_run(lv_context* ctx, int approach, const lv_dgraph* g, const lv_dbatch* b, lv_dstate* s,
+           const lv_options* o, lv_result* res);
+/* Modularity of a device state's membership on a device graph
+ * (modularityByOmp, inc/properties.hxx:205-233). */
+int lv_dmodularity(lv_context* ctx, const lv_dgraph* g, const lv_dstate* s, double resolution, double* out);
+
+/* Phase entry points: each hot-path function of inc/louvain.hxx on its own,
+ * host buffers in and out.  They run the same kernels as lv_run. */
+/* edgeWeightOmp, inc/properties.hxx:96-106 (2M) */
+int lv_edge_weight(lv_context* ctx, const lv_graph* g, double* out);
+/* modularityByOmp, inc/properties.hxx:205-233 */
+int lv_modularity(lv_context* ctx, const lv_graph* g, const uint32_t* membership, double resolution, double* out);
+/* louvainVertexWeightsOmpW, inc/louvain.hxx:210-218 */
+int lv_vertex_weights(lv_context* ctx, const lv_graph* g, double* vtot);
+/* louvainUpdateWeightsFromOmpU, inc/louvain.hxx:374-388 (vtot, ctot in/out) */
+int lv_update_weights(lv_context* ctx, uint64_t span, const lv_batch* deletions, const lv_batch* insertions,
+                      const uint32_t* vcom, double* vtot, double* ctot);
+/* louvainAffectedVerticesFrontierOmpW, inc/louvain.hxx:1606-1624 */
+int lv_affected_frontier(lv_context* ctx, uint64_t span, const lv_batch* deletions, const lv_batch* insertions,
+                         const uint32_t* vcom, uint8_t* vaff);
+/* louvainAffectedVerticesDeltaScreeningOmpW, inc/louvain.hxx:1441-1486 */
+int lv_affected_delta_screening(lv_context* ctx, const lv_graph* g, const lv_batch* deletions,
+                                const lv_batch* insertions, const uint32_t* vcom, const double* vtot,
+                                const double* ctot, double M, double resolution, uint8_t* vertices,
+                                uint8_t* neighbors, uint8_t* communities);
+/* louvainScanCommunitiesW + louvainChooseCommunity, inc/louvain.hxx:406-464:
+ * every vertex's best community against one fixed snapshot, no move. */
+int lv_choose_community(lv_context* ctx, const lv_graph* g, const uint32_t* vcom, const double* vtot,
+                        const double* ctot, double M, double resolution, uint32_t* best_c, double* best_e);
+/* louvainCommunityExistsOmpW + louvainRenumberCommunitiesOmpW,
+ * inc/louvain.hxx:668-682, :941-945: vcom (vcom_size entries) is renumbered
+ * in place; cext (span entries) receives the old -> new map. */
+int lv_renumber(lv_context* ctx, const lv_graph* g, uint32_t* vcom, uint64_t vcom_size, uint32_t* cext,
+                uint64_t* num_communities);
+/* louvainCommunityVerticesOmpW + louvainAggregateOmpW,
+ * inc/louvain.hxx:800-812, :987-992: the super-graph of a renumbered
+ * membership as a gapped CSR (zoff[C+1], zdeg[C], zkeys / zw of capacity
+ * offsets[span]).  Row order is unspecified (the reference's depends on
+ * thread timing); the rows as sets are exact. */
+int lv_aggregate(lv_context* ctx, const lv_graph* g, const uint32_t* vcom, uint64_t num_communities,
+                 uint64_t* zoff, uint32_t* zdeg, uint32_t* zkeys, double* zw);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LOUVAIN_HIP_H */

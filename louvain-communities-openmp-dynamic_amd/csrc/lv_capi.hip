@@ -1,0 +1,1184 @@
+// Host side of the MI355X Louvain library: the pass driver (the role of
+// louvainInvokeOmp, inc/louvain.hxx:1120-1221), the four approaches
+// (inc/louvain.hxx:1285, :1349, :1542, :1674) and the C ABI declared in
+// include/louvain_hip.h.  All device work runs on the context's stream;
+// exceptions are used internally and turned into lv_status codes at the ABI.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "lv_internal.hpp"
+
+using namespace lv;
+
+namespace {
+
+struct LvError {
+  int code;
+  std::string msg;
+};
+
+#define HIPCHK(x)                                                                              \
+  do {                                                                                         \
+    hipError_t e_ = (x);                                                                       \
+    if (e_ != hipSuccess)                                                                      \
+      throw LvError{e_ == hipErrorOutOfMemory || e_ == hipErrorMemoryAllocation ? LV_ERR_NOMEM \
+                                                                                : LV_ERR_HIP,  \
+                    std::string(#x) + ": " + hipGetErrorString(e_)};                           \
+  } while (0)
+
+void require(bool ok, int code, const std::string& msg) {
+  if (!ok) throw LvError{code, msg};
+}
+
+struct Buf {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+
+}  // namespace
+
+struct lv_context {
+  int device = 0;
+  hipStream_t st = nullptr;
+  std::string err;
+  int path = 1;
+  std::map<std::string, Buf> bufs;
+  void* pinned = nullptr;  // small pinned host buffer for scalar readbacks
+  size_t pinned_bytes = 0;
+  std::vector<hipEvent_t> events;
+  size_t ev_used = 0;
+  size_t scan_tmp_bytes = 0;
+
+  // Grow-only named device buffer; contents are unspecified.
+  template <class T>
+  T* ws(const std::string& name, size_t count) {
+    size_t bytes = std::max<size_t>(count * sizeof(T), 256);
+    Buf& b = bufs[name];
+    if (b.bytes < bytes) {
+      if (b.p) HIPCHK(hipFree(b.p));
+      b.p = nullptr;
+      b.bytes = 0;
+      size_t want = bytes + bytes / 8;  // headroom for slightly larger later calls
+      HIPCHK(hipMalloc(&b.p, want));
+      b.bytes = want;
+    }
+    return reinterpret_cast<T*>(b.p);
+  }
+  void* scan_tmp(size_t bytes) {
+    scan_tmp_bytes = std::max(scan_tmp_bytes, bytes);
+    return ws<uint8_t>("scan_tmp", scan_tmp_bytes);
+  }
+  template <class T>
+  void read(T* host, const T* dev, size_t n) {
+    size_t bytes = n * sizeof(T);
+    if (pinned_bytes < bytes) {
+      if (pinned) HIPCHK(hipHostFree(pinned));
+      pinned = nullptr;
+      HIPCHK(hipHostMalloc(&pinned, std::max<size_t>(bytes, 4096), hipHostMallocDefault));
+      pinned_bytes = std::max<size_t>(bytes, 4096);
+    }
+    HIPCHK(hipMemcpyAsync(pinned, dev, bytes, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    std::memcpy(host, pinned, bytes);
+  }
+  // n values dev[0], dev[stride], ... with one synchronisation
+  template <class T>
+  void read_strided(T* host, const T* dev, size_t n, size_t stride) {
+    if (pinned_bytes < n * sizeof(T)) {
+      T tmp;
+      read(&tmp, dev, 1);  // grows the pinned buffer
+      if (pinned_bytes < n * sizeof(T)) throw LvError{LV_ERR_NOMEM, "pinned buffer"};
+    }
+    T* pin = reinterpret_cast<T*>(pinned);
+    for (size_t i = 0; i < n; ++i)
+      HIPCHK(hipMemcpyAsync(pin + i, dev + i * stride, sizeof(T), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    std::memcpy(host, pin, n * sizeof(T));
+  }
+  template <class T>
+  T read1(const T* dev) {
+    T v;
+    read(&v, dev, 1);
+    return v;
+  }
+  hipEvent_t event() {
+    if (ev_used == events.size()) {
+      hipEvent_t e;
+      HIPCHK(hipEventCreate(&e));
+      events.push_back(e);
+    }
+    return events[ev_used++];
+  }
+  hipEvent_t record() {
+    hipEvent_t e = event();
+    HIPCHK(hipEventRecord(e, st));
+    return e;
+  }
+  void check() { HIPCHK(hipGetLastError()); }
+};
+
+struct lv_dgraph {
+  lv_context* ctx = nullptr;
+  uint64_t S = 0, E = 0, order = 0;
+  uint64_t* off = nullptr;
+  uint32_t* deg = nullptr;
+  uint32_t* keys = nullptr;
+  void* w = nullptr;
+  bool f64 = false;
+  uint8_t* ex = nullptr;
+  double M2 = 0;  // 2M, edgeWeightOmp
+  TierLists tiers;
+  GView view() const { return GView{S, off, deg, keys, w, ex}; }
+};
+
+struct lv_dbatch {
+  uint64_t nd = 0, ni = 0;
+  uint32_t *du = nullptr, *dv = nullptr, *iu = nullptr, *iv = nullptr;
+  float *dw = nullptr, *iw = nullptr;
+};
+
+struct lv_dstate {
+  uint64_t S = 0;
+  uint32_t* memb = nullptr;
+  double* vtot = nullptr;
+  double* ctot = nullptr;
+};
+
+namespace {
+
+template <class T>
+T* dalloc(size_t n) {
+  T* p = nullptr;
+  HIPCHK(hipMalloc(&p, std::max<size_t>(n * sizeof(T), 256)));
+  return p;
+}
+
+template <class T>
+void h2d(lv_context* c, T* dst, const T* src, size_t n) {
+  if (n) HIPCHK(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyHostToDevice, c->st));
+}
+
+template <class T>
+void d2h(lv_context* c, T* dst, const T* src, size_t n) {
+  if (n) HIPCHK(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyDeviceToHost, c->st));
+}
+
+template <class T>
+void d2d(lv_context* c, T* dst, const T* src, size_t n) {
+  if (n) HIPCHK(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyDeviceToDevice, c->st));
+}
+
+void scan32(lv_context* c, const uint32_t* in, uint32_t* out, uint64_t n) {
+  size_t bytes = 0;
+  HIPCHK(scan_u32(in, out, n, nullptr, bytes, c->st));
+  void* tmp = c->scan_tmp(bytes);
+  bytes = c->scan_tmp_bytes;
+  HIPCHK(scan_u32(in, out, n, tmp, bytes, c->st));
+}
+
+void scan64(lv_context* c, const uint64_t* in, uint64_t* out, uint64_t n) {
+  size_t bytes = 0;
+  HIPCHK(scan_u64(in, out, n, nullptr, bytes, c->st));
+  void* tmp = c->scan_tmp(bytes);
+  bytes = c->scan_tmp_bytes;
+  HIPCHK(scan_u64(in, out, n, tmp, bytes, c->st));
+}
+
+// Degree-tier lists of ids 0..n-1 (live only), ascending within each tier.
+void build_tiers(lv_context* c, const std::string& name, const uint64_t* off, const uint32_t* deg,
+                 const uint8_t* ex, uint64_t n, TierLists& t, uint32_t* ids_out = nullptr) {
+  uint32_t nb = (uint32_t)std::max<uint64_t>(1, (n + 255) / 256);
+  uint32_t* counts = c->ws<uint32_t>(name + ".cnt", (size_t)kTiers * nb + 1);
+  uint32_t* base = c->ws<uint32_t>(name + ".base", (size_t)kTiers * nb + 1);
+  t.ids = ids_out ? ids_out : c->ws<uint32_t>(name + ".ids", std::max<uint64_t>(n, 1));
+  k_fill_u32(counts, 0, (size_t)kTiers * nb + 1, c->st);
+  if (n) k_tier_count(off, deg, ex, n, 0, 0, counts, nb, c->st);
+  scan32(c, counts, base, (size_t)kTiers * nb + 1);
+  if (n) k_tier_scatter(off, deg, ex, n, 0, 0, base, nb, t.ids, c->st);
+  // tier starts: base[t * nb] for t <= kTiers (the last is the total)
+  uint32_t st[kTiers + 1];
+  c->read_strided(st, base, kTiers + 1, nb);
+  for (int k = 0; k <= kTiers; ++k) t.start[k] = st[k];
+}
+
+uint32_t max_degree_tier4(lv_context* c, const TierLists& t, const uint32_t* deg) {
+  uint64_t n4 = t.start[5] - t.start[4];
+  if (!n4) return 0;
+  uint32_t* out = c->ws<uint32_t>("maxdeg", 1);
+  k_fill_u32(out, 0, 1, c->st);
+  k_max_deg(t.ids + t.start[4], n4, deg, out, c->st);
+  return c->read1(out);
+}
+
+// ------------------------------------------------------------ local-moving
+struct Level {
+  GView g;
+  bool f64;
+  const TierLists* tiers;
+  uint64_t order;  // live vertices
+  uint32_t maxdeg4;
+};
+
+struct Counters {
+  unsigned long long v[CNT_N] = {0, 0, 0, 0};
+};
+
+MoveArgs base_args(lv_context* c, const Level& lv, uint32_t* vcom, const double* vtot, double* ctot, uint8_t* vaff,
+                   const uint8_t* fa, double M, double R) {
+  MoveArgs a{};
+  a.g = lv.g;
+  a.vcom = vcom;
+  a.vtot = vtot;
+  a.ctot = ctot;
+  a.vaff = vaff;
+  a.fa = fa;
+  a.M = M;
+  a.R = R;
+  a.el = c->ws<double>("el", 1);
+  a.cnt = c->ws<unsigned long long>("cnt", CNT_N);
+  uint64_t hcap = next_pow2(2 * (uint64_t)std::max<uint32_t>(lv.maxdeg4, 1));
+  a.hcap = hcap;
+  if (lv.maxdeg4) {
+    size_t slots = (size_t)hcap * 256;  // tier-4 grid is at most 256 blocks
+    a.hkey = c->ws<uint32_t>("h.key", slots);
+    a.hpos = c->ws<uint32_t>("h.pos", slots);
+    a.hval = c->ws<double>("h.val", slots);
+  }
+  return a;
+}
+
+// One sweep-mode iteration over the tier lists.
+void move_sweep(lv_context* c, MoveArgs a, const Level& lv) {
+  for (int t = 0; t < kTiers; ++t) {
+    uint64_t n = lv.tiers->start[t + 1] - lv.tiers->start[t];
+    if (!n) continue;
+    a.list = lv.tiers->ids + lv.tiers->start[t];
+    a.n = n;
+    k_move(a, t, lv.f64, c->st);
+  }
+}
+
+// louvainMoveOmpW (inc/louvain.hxx:587-609): iterate until the summed gain
+// el <= E or L iterations; returns the iterations, or 0 if the first one
+// gained nothing.
+int local_move(lv_context* c, const Level& lv, uint32_t* vcom, const double* vtot, double* ctot, uint8_t* vaff,
+               const uint8_t* fa, double M, double R, int L, double E, Counters& cnt) {
+  MoveArgs a = base_args(c, lv, vcom, vtot, ctot, vaff, fa, M, R);
+  k_fill_u64(reinterpret_cast<uint64_t*>(a.cnt), 0, CNT_N, c->st);
+  int l = 0;
+  double el = 0;
+  for (; l < L;) {
+    k_fill_f64(a.el, 0, 1, c->st);
+    move_sweep(c, a, lv);
+    c->check();
+    el = c->read1(a.el);
+    ++l;
+    if (el <= E) break;
+  }
+  unsigned long long h[CNT_N];
+  c->read(h, a.cnt, CNT_N);
+  for (int i = 0; i < CNT_N; ++i) cnt.v[i] += h[i];
+  return (l > 1 || el) ? l : 0;
+}
+
+// Worklist-mode local-moving for the first pass of the frontier-based
+// approaches: iteration k processes the vertices queued (flagged 0 -> 1)
+// during iteration k-1, binned by degree tier on the device; vaff keeps the
+// reference's meaning (set = flagged and not yet processed).
+int local_move_worklist(lv_context* c, const Level& lv, uint32_t* vcom, const double* vtot, double* ctot,
+                        uint8_t* vaff, const uint8_t* fa, double M, double R, int L, double E, uint32_t* q[2],
+                        uint32_t* qlen, Counters& cnt) {
+  MoveArgs a = base_args(c, lv, vcom, vtot, ctot, vaff, fa, M, R);
+  k_fill_u64(reinterpret_cast<uint64_t*>(a.cnt), 0, CNT_N, c->st);
+  const uint64_t stride = std::max<uint64_t>(lv.g.S, 1);
+  uint32_t* tq = c->ws<uint32_t>("wl.tq", (size_t)kTiers * stride);
+  uint32_t* tlen = c->ws<uint32_t>("wl.tlen", kTiers);
+  int l = 0;
+  double el = 0;
+  int in = 0;
+  for (; l < L;) {
+    k_fill_f64(a.el, 0, 1, c->st);
+    k_fill_u32(qlen + (1 - in), 0, 1, c->st);
+    k_fill_u32(tlen, 0, kTiers, c->st);
+    k_bin_queue(q[in], qlen + in, std::min<uint64_t>(lv.g.S, 1u << 19), lv.g.deg, tq, stride, tlen, c->st);
+    a.qnext = q[1 - in];
+    a.qnext_len = qlen + (1 - in);
+    for (int k = 0; k < kTiers; ++k) {
+      if (k == 4 && !lv.maxdeg4) continue;
+      a.list = tq + (size_t)k * stride;
+      a.n_dev = tlen + k;
+      a.n = std::min<uint64_t>(lv.g.S, 1u << 15);  // grid-size hint
+      k_move(a, k, lv.f64, c->st);
+    }
+    c->check();
+    el = c->read1(a.el);
+    in = 1 - in;
+    ++l;
+    if (el <= E) break;
+  }
+  unsigned long long h[CNT_N];
+  c->read(h, a.cnt, CNT_N);
+  for (int i = 0; i < CNT_N; ++i) cnt.v[i] += h[i];
+  return (l > 1 || el) ? l : 0;
+}
+
+// ------------------------------------------------------------ aggregation
+struct SuperGraph {
+  uint64_t* off = nullptr;
+  uint32_t* deg = nullptr;
+  uint32_t* keys = nullptr;
+  double* w = nullptr;
+};
+
+SuperGraph super_graph(lv_context* c, const char* name, uint64_t S, uint64_t E) {
+  std::string n(name);
+  SuperGraph s;
+  s.off = c->ws<uint64_t>(n + ".off", S + 1);
+  s.deg = c->ws<uint32_t>(n + ".deg", S);
+  s.keys = c->ws<uint32_t>(n + ".keys", E);
+  s.w = c->ws<double>(n + ".w", E);
+  return s;
+}
+
+// louvainCommunityExistsOmpW + louvainRenumberCommunitiesOmpW
+// (inc/louvain.hxx:668-682, :941-945), up to the lookup: map[0..S'] is the
+// old -> new id map; returns the community count.
+uint64_t count_communities(lv_context* c, const GView& g, const uint32_t* vcom, uint32_t*& map) {
+  uint32_t* flags = c->ws<uint32_t>("rn.flags", g.S + 1);
+  map = c->ws<uint32_t>("rn.map", g.S + 1);
+  k_fill_u32(flags, 0, g.S + 1, c->st);
+  k_exists(g, vcom, flags, c->st);
+  scan32(c, flags, map, g.S + 1);
+  return c->read1(map + g.S);
+}
+
+// louvainCommunityVerticesOmpW + louvainAggregateOmpW (inc/louvain.hxx:800-812,
+// :987-992) for a renumbered membership: builds super-graph z and its row
+// sums (K of the super-vertices) in ztot.
+void aggregate(lv_context* c, const Level& lv, const uint32_t* vcom, uint64_t C, SuperGraph& z, double* ztot) {
+  const GView& g = lv.g;
+  uint64_t GN = lv.order;
+  uint32_t* coff = c->ws<uint32_t>("ag.coff", C + 1);
+  uint32_t* cnt = c->ws<uint32_t>("ag.cnt", C + 1);
+  uint32_t* cdeg = c->ws<uint32_t>("ag.cdeg", C);
+  uint32_t* cedg = c->ws<uint32_t>("ag.cedg", std::max<uint64_t>(GN, 1));
+  uint64_t* mdeg = c->ws<uint64_t>("ag.mdeg", GN + 1);
+  uint64_t* mscan = c->ws<uint64_t>("ag.mscan", GN + 1);
+  k_fill_u32(cnt, 0, C + 1, c->st);
+  k_count_members(g, vcom, cnt, c->st);
+  scan32(c, cnt, coff, C + 1);
+  k_fill_u32(cdeg, 0, C, c->st);
+  k_scatter_members(g, vcom, coff, cdeg, cedg, c->st);
+  k_member_degrees(cedg, GN, g, mdeg, c->st);
+  scan64(c, mdeg, mscan, GN + 1);
+  k_gather_yoff(coff, mscan, C, z.off, c->st);
+  TierLists ct;
+  build_tiers(c, "ag.tiers", z.off, nullptr, nullptr, C, ct);
+  k_fill_u32(z.deg, 0, C, c->st);
+  k_fill_f64(ztot, 0, C, c->st);
+  AggArgs a{};
+  a.g = g;
+  a.vcom = vcom;
+  a.coff = coff;
+  a.cedg = cedg;
+  a.mscan = mscan;
+  a.yoff = z.off;
+  a.C = C;
+  a.zdeg = z.deg;
+  a.zkeys = z.keys;
+  a.zw = z.w;
+  a.ztot = ztot;
+  for (int t = 0; t < 4; ++t) {
+    a.list = ct.ids + ct.start[t];
+    a.n = ct.start[t + 1] - ct.start[t];
+    k_aggregate(a, t, lv.f64, c->st);
+  }
+  uint64_t n4 = ct.start[5] - ct.start[4];
+  if (n4) {
+    a.list = ct.ids + ct.start[4];
+    a.n = n4;
+    uint64_t* caps = c->ws<uint64_t>("ag.caps", n4 + 1);
+    uint64_t* chunks = c->ws<uint64_t>("ag.chunks", n4 + 1);
+    uint64_t* hoff = c->ws<uint64_t>("ag.hoff", n4 + 1);
+    uint64_t* choff = c->ws<uint64_t>("ag.choff", n4 + 1);
+    k_t4_sizes(z.off, a.list, n4, C, 4096, caps, chunks, c->st);
+    scan64(c, caps, hoff, n4 + 1);
+    scan64(c, chunks, choff, n4 + 1);
+    uint64_t tot[1], nch[1];
+    c->read(tot, hoff + n4, 1);
+    c->read(nch, choff + n4, 1);
+    a.hkey = c->ws<uint32_t>("ag.hkey", tot[0]);
+    a.hval = c->ws<double>("ag.hval", tot[0]);
+    a.hoff = hoff;
+    a.choff = choff;
+    a.nchunks = nch[0];
+    k_fill_u32(a.hkey, kEmpty, tot[0], c->st);
+    k_fill_f64(a.hval, 0, tot[0], c->st);
+    k_aggregate(a, 4, lv.f64, c->st);
+    k_agg_emit_hbm(a, tot[0], c->st);
+  }
+  c->check();
+}
+
+// ------------------------------------------------------------ the driver
+struct RunTimes {
+  float time = 0, mark = 0, init = 0, first = 0, lmove = 0, aggr = 0;
+};
+
+float elapsed(hipEvent_t a, hipEvent_t b) {
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, a, b));
+  return ms;
+}
+
+void validate_options(const lv_options* o) {
+  require(o != nullptr, LV_ERR_INVALID, "options is NULL");
+  require(o->repeat >= 1, LV_ERR_INVALID, "repeat must be >= 1");
+  require(o->max_iterations >= 0 && o->max_passes >= 0, LV_ERR_INVALID, "negative iteration/pass limit");
+  require(o->resolution > 0, LV_ERR_INVALID, "resolution must be > 0");
+  require(o->tolerance_drop > 0, LV_ERR_INVALID, "tolerance_drop must be > 0");
+}
+
+// The pass loop of louvainInvokeOmp (inc/louvain.hxx:1153-1218) for one
+// repeat.  ucom/utot/ctot are the state (membership, K, Sigma_tot).
+void invoke(lv_context* c, int approach, const lv_dgraph* x, const lv_dbatch* b, lv_dstate* s, const lv_options* o,
+            int& l, int& p, RunTimes& rt, Counters& cnt) {
+  const uint64_t S = x->S;
+  const double R = o->resolution;
+  const int L = o->max_iterations, P = o->max_passes;
+  const double M = x->M2 / 2;
+  uint32_t* ucom = s->memb;
+  double* utot = s->vtot;
+  double* ctot = s->ctot;
+  // buffers (allocation and resets are outside the timed region, as in the reference)
+  uint8_t* vaff = c->ws<uint8_t>("vaff", (S + 7) & ~3ull);
+  uint32_t* vcom = c->ws<uint32_t>("vcom", S);
+  double* vtot = c->ws<double>("vtot", S);
+  uint8_t *dsv = nullptr, *dsn = nullptr, *dsc = nullptr;
+  if (approach == LV_DELTA_SCREENING) {
+    dsv = c->ws<uint8_t>("ds.v", S);
+    dsn = c->ws<uint8_t>("ds.n", S);
+    dsc = c->ws<uint8_t>("ds.c", S);
+  }
+  bool worklist = c->path == 1 && (approach == LV_DYNAMIC_FRONTIER || approach == LV_DELTA_SCREENING);
+  uint32_t* q[2] = {nullptr, nullptr};
+  uint32_t* qlen = nullptr;
+  if (worklist) {
+    q[0] = c->ws<uint32_t>("wl.q0", S);
+    q[1] = c->ws<uint32_t>("wl.q1", S);
+    qlen = c->ws<uint32_t>("wl.len", 2);
+    k_fill_u32(qlen, 0, 2, c->st);
+  }
+  SuperGraph y = super_graph(c, "y", S, x->E), z = super_graph(c, "z", S, x->E);
+  k_fill_u8(vaff, 0, (S + 7) & ~3ull, c->st);
+  k_fill_u32(vcom, 0, S, c->st);
+  k_fill_f64(vtot, 0, S, c->st);
+  if (approach == LV_STATIC) {
+    k_fill_u32(ucom, 0, S, c->st);
+    k_fill_f64(utot, 0, S, c->st);
+    k_fill_f64(ctot, 0, S, c->st);
+  }
+  double E = o->tolerance;
+  hipEvent_t e0 = c->record();
+  // fi: initialisation
+  if (approach == LV_STATIC) {
+    k_vertex_weights(x->view(), x->f64, utot, c->st);
+    k_initialize(x->view(), ucom, ctot, utot, c->st);
+  } else {
+    k_update_weights(b->du, b->dw, b->nd, b->iu, b->iw, b->ni, ucom, utot, ctot, c->st);
+  }
+  hipEvent_t e1 = c->record();
+  // fm: marking
+  const uint8_t* fa = nullptr;
+  if (approach == LV_DYNAMIC_FRONTIER) {
+    k_frontier(b->du, b->dv, b->nd, b->iu, b->iv, b->ni, ucom, vaff, worklist ? q[0] : nullptr, qlen, c->st);
+  } else if (approach == LV_DELTA_SCREENING) {
+    k_fill_u8(dsv, 0, S, c->st);
+    k_fill_u8(dsn, 0, S, c->st);
+    k_fill_u8(dsc, 0, S, c->st);
+    k_ds_deletions(b->du, b->dv, b->nd, ucom, dsv, dsn, dsc, c->st);
+    if (b->ni) {
+      uint32_t* tkey = c->ws<uint32_t>("ds.tkey", 2 * b->ni);
+      double* tval = c->ws<double>("ds.tval", 2 * b->ni);
+      uint32_t* tlist = c->ws<uint32_t>("ds.tlist", b->ni);
+      k_fill_u32(tkey, kEmpty, 2 * b->ni, c->st);
+      k_ds_insertions(b->iu, b->iv, b->iw, b->ni, ucom, utot, ctot, M, R, tkey, tval, tlist, dsv, dsn, dsc, c->st);
+    }
+    k_ds_sweep(x->view(), ucom, dsn, dsc, dsv, c->st);
+    d2d(c, vaff, dsv, S);
+    fa = dsv;
+    if (worklist) k_compact_flags(dsv, S, q[0], qlen, c->st);
+  } else {
+    k_fill_u8(vaff, 1, S, c->st);
+  }
+  hipEvent_t e2 = c->record();
+  hipEvent_t t1 = e2;
+  bool t1_set = false;
+  Level lv0{x->view(), x->f64, &x->tiers, x->order, x->tiers.maxdeg};
+  Level cur = lv0;
+  TierLists ytiers;
+  uint64_t ySpan = 0;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> lm, ag;
+  for (l = 0, p = 0; M > 0 && P > 0;) {
+    if (p == 1) { t1 = c->record(); t1_set = true; }
+    bool first = p == 0;
+    int m;
+    hipEvent_t a0 = c->record();
+    if (first) {
+      if (worklist) m = local_move_worklist(c, cur, ucom, utot, ctot, vaff, fa, M, R, L, E, q, qlen, cnt);
+      else m = local_move(c, cur, ucom, utot, ctot, vaff, fa, M, R, L, E, cnt);
+    } else {
+      m = local_move(c, cur, vcom, vtot, ctot, vaff, nullptr, M, R, L, E, cnt);
+    }
+    lm.push_back({a0, c->record()});
+    l += std::max(m, 1);
+    ++p;
+    if (m <= 1 || p >= P) break;
+    uint32_t* map = nullptr;
+    uint32_t* mcur = first ? ucom : vcom;
+    uint64_t CN = count_communities(c, cur.g, mcur, map);
+    uint64_t GN = cur.order;
+    if ((double)CN / (double)GN >= o->aggregation_tolerance) break;
+    if (first) k_lookup(ucom, S, map, c->st);
+    else {
+      k_lookup(vcom, cur.g.S, map, c->st);
+      k_lookup(ucom, S, vcom, c->st);
+    }
+    hipEvent_t g0 = c->record();
+    aggregate(c, cur, mcur, CN, z, vtot);
+    ag.push_back({g0, c->record()});
+    std::swap(y, z);
+    ySpan = CN;
+    // next level: vcom[u] = u, Sigma_tot = K, every super-vertex affected
+    GView yg{ySpan, y.off, y.deg, y.keys, y.w, nullptr};
+    k_initialize(yg, vcom, ctot, vtot, c->st);
+    k_fill_u8(vaff, 1, CN, c->st);
+    build_tiers(c, "y.tiers", y.off, y.deg, nullptr, CN, ytiers);
+    uint64_t n4 = ytiers.start[5] - ytiers.start[4];
+    ytiers.maxdeg = n4 ? max_degree_tier4(c, ytiers, y.deg) : 0;
+    cur = Level{yg, true, &ytiers, CN, ytiers.maxdeg};
+    E /= o->tolerance_drop;
+  }
+  if (p > 1) k_lookup(ucom, S, vcom, c->st);
+  hipEvent_t e3 = c->record();
+  if (!t1_set) t1 = e3;
+  HIPCHK(hipEventSynchronize(e3));
+  c->check();
+  rt.time += elapsed(e0, e3);
+  rt.init += elapsed(e0, e1);
+  rt.mark += elapsed(e1, e2);
+  rt.first += elapsed(e2, t1);
+  for (auto& pr : lm) rt.lmove += elapsed(pr.first, pr.second);
+  for (auto& pr : ag) rt.aggr += elapsed(pr.first, pr.second);
+}
+
+void run(lv_context* c, int approach, const lv_dgraph* x, const lv_dbatch* b, lv_dstate* s, const lv_options* o,
+         lv_result* res) {
+  require(c && x && s, LV_ERR_INVALID, "null context, graph or state");
+  require(approach >= LV_STATIC && approach <= LV_DYNAMIC_FRONTIER, LV_ERR_INVALID, "unknown approach");
+  require(approach == LV_STATIC || b != nullptr, LV_ERR_INVALID, "dynamic approaches need a batch");
+  require(s->S == x->S, LV_ERR_INVALID, "state span differs from graph span");
+  validate_options(o);
+  HIPCHK(hipSetDevice(c->device));
+  c->ev_used = 0;
+  const uint64_t S = x->S;
+  int repeat = o->repeat;
+  // keep the caller's state for every repeat (louvainSetupInitialsW, inc/louvain.hxx:1240-1249)
+  uint32_t* bm = nullptr;
+  double *bv = nullptr, *bc = nullptr;
+  if (repeat > 1 && approach != LV_STATIC) {
+    bm = c->ws<uint32_t>("bak.m", S);
+    bv = c->ws<double>("bak.v", S);
+    bc = c->ws<double>("bak.c", S);
+    d2d(c, bm, s->memb, S);
+    d2d(c, bv, s->vtot, S);
+    d2d(c, bc, s->ctot, S);
+  }
+  RunTimes rt;
+  Counters cnt;
+  int l = 0, p = 0;
+  for (int r = 0; r < repeat; ++r) {
+    if (r > 0 && approach != LV_STATIC) {
+      d2d(c, s->memb, bm, S);
+      d2d(c, s->vtot, bv, S);
+      d2d(c, s->ctot, bc, S);
+    }
+    c->ev_used = 0;
+    invoke(c, approach, x, b, s, o, l, p, rt, cnt);
+  }
+  unsigned long long* dcount = c->ws<unsigned long long>("affected", 1);
+  k_fill_u64(reinterpret_cast<uint64_t*>(dcount), 0, 1, c->st);
+  k_count_u8(c->ws<uint8_t>("vaff", (S + 7) & ~3ull), 1, S, dcount, c->st);
+  unsigned long long aff = c->read1(dcount);
+  if (res) {
+    res->iterations = l;
+    res->passes = p;
+    res->time = rt.time / repeat;
+    res->marking_time = rt.mark / repeat;
+    res->initialization_time = rt.init / repeat;
+    res->first_pass_time = rt.first / repeat;
+    res->local_move_time = rt.lmove / repeat;
+    res->aggregation_time = rt.aggr / repeat;
+    res->affected_vertices = aff;
+    if (res->membership) d2h(c, res->membership, s->memb, S);
+    if (res->vertex_weight) d2h(c, res->vertex_weight, s->vtot, S);
+    if (res->community_weight) d2h(c, res->community_weight, s->ctot, S);
+    HIPCHK(hipStreamSynchronize(c->st));
+  }
+  // per-run kernel counters, for bandwidth accounting
+  unsigned long long* keep = c->ws<unsigned long long>("cnt.last", CNT_N);
+  h2d(c, keep, cnt.v, CNT_N);
+  HIPCHK(hipStreamSynchronize(c->st));
+}
+
+// ------------------------------------------------------------ uploads
+void graph_check(const lv_graph* g) {
+  require(g != nullptr, LV_ERR_INVALID, "graph is NULL");
+  require(g->offsets != nullptr, LV_ERR_INVALID, "graph offsets is NULL");
+  require(g->span < 0xFFFFFFFFull, LV_ERR_LIMIT, "span must fit 32-bit vertex ids");
+  uint64_t E = g->offsets[g->span];
+  require(E == 0 || (g->keys && (g->weights || g->weights_f64)), LV_ERR_INVALID, "graph keys/weights NULL");
+}
+
+lv_dgraph* upload_graph(lv_context* c, const lv_graph* g) {
+  graph_check(g);
+  HIPCHK(hipSetDevice(c->device));
+  auto* d = new lv_dgraph();
+  try {
+    d->ctx = c;
+    d->S = g->span;
+    d->E = g->offsets[g->span];
+    d->f64 = g->weights_f64 != nullptr;
+    d->off = dalloc<uint64_t>(d->S + 1);
+    d->deg = dalloc<uint32_t>(d->S);
+    d->keys = dalloc<uint32_t>(d->E);
+    d->w = d->f64 ? (void*)dalloc<double>(d->E) : (void*)dalloc<float>(d->E);
+    h2d(c, d->off, g->offsets, d->S + 1);
+    if (g->degrees) h2d(c, d->deg, g->degrees, d->S);
+    else k_degrees_from_offsets(d->off, d->deg, d->S, c->st);
+    h2d(c, d->keys, g->keys, d->E);
+    if (d->f64) h2d(c, (double*)d->w, g->weights_f64, d->E);
+    else h2d(c, (float*)d->w, g->weights, d->E);
+    d->order = d->S;
+    if (g->exists) {
+      d->ex = dalloc<uint8_t>(d->S);
+      h2d(c, d->ex, g->exists, d->S);
+      uint64_t n = 0;
+      for (uint64_t u = 0; u < d->S; ++u) n += g->exists[u] != 0;
+      d->order = n;
+    }
+    if (d->E) {
+      uint32_t* mx = c->ws<uint32_t>("maxkey", 1);
+      k_fill_u32(mx, 0, 1, c->st);
+      k_max_u32(d->keys, d->E, mx, c->st);
+      require(c->read1(mx) < d->S, LV_ERR_INVALID, "graph has a neighbour id >= span");
+    }
+    // 2M (edgeWeightOmp), computed once per graph like the reference's M
+    double* m2 = c->ws<double>("m2", 1);
+    k_fill_f64(m2, 0, 1, c->st);
+    k_edge_weight(d->view(), d->f64, m2, c->st);
+    d->M2 = c->read1(m2);
+    d->tiers.ids = dalloc<uint32_t>(std::max<uint64_t>(d->S, 1));
+    build_tiers(c, "g.tiers", d->off, d->deg, d->ex, d->S, d->tiers, d->tiers.ids);
+    d->tiers.maxdeg = max_degree_tier4(c, d->tiers, d->deg);
+    c->check();
+  } catch (...) {
+    lv_dgraph_destroy(d);
+    throw;
+  }
+  return d;
+}
+
+lv_dbatch* upload_batch(lv_context* c, const lv_batch* del, const lv_batch* ins, uint64_t S) {
+  HIPCHK(hipSetDevice(c->device));
+  auto* b = new lv_dbatch();
+  try {
+    b->nd = del ? del->size : 0;
+    b->ni = ins ? ins->size : 0;
+    require(!b->nd || (del->u && del->v && del->w), LV_ERR_INVALID, "deletions arrays NULL");
+    require(!b->ni || (ins->u && ins->v && ins->w), LV_ERR_INVALID, "insertions arrays NULL");
+    b->du = dalloc<uint32_t>(b->nd);
+    b->dv = dalloc<uint32_t>(b->nd);
+    b->dw = dalloc<float>(b->nd);
+    b->iu = dalloc<uint32_t>(b->ni);
+    b->iv = dalloc<uint32_t>(b->ni);
+    b->iw = dalloc<float>(b->ni);
+    if (b->nd) { h2d(c, b->du, del->u, b->nd); h2d(c, b->dv, del->v, b->nd); h2d(c, b->dw, del->w, b->nd); }
+    if (b->ni) { h2d(c, b->iu, ins->u, b->ni); h2d(c, b->iv, ins->v, b->ni); h2d(c, b->iw, ins->w, b->ni); }
+    if (S) {
+      int* bad = c->ws<int>("bad", 1);
+      k_fill_u32(reinterpret_cast<uint32_t*>(bad), 0, 1, c->st);
+      k_check_sorted(b->du, b->dv, b->nd, (uint32_t)S, bad, c->st);
+      int f1 = c->read1(bad);
+      k_fill_u32(reinterpret_cast<uint32_t*>(bad), 0, 1, c->st);
+      k_check_sorted(b->iu, b->iv, b->ni, (uint32_t)S, bad, c->st);
+      int f2 = c->read1(bad);
+      require(!((f1 | f2) & 1), LV_ERR_INVALID, "batch vertex id out of range");
+      require(!((f1 | f2) & 2), LV_ERR_UNSORTED, "batch not sorted by (u, v) or not unique");
+    }
+  } catch (...) {
+    lv_dbatch_destroy(b);
+    throw;
+  }
+  return b;
+}
+
+lv_dstate* new_state(uint64_t S) {
+  auto* s = new lv_dstate();
+  try {
+    s->S = S;
+    s->memb = dalloc<uint32_t>(S);
+    s->vtot = dalloc<double>(S);
+    s->ctot = dalloc<double>(S);
+  } catch (...) {
+    lv_dstate_destroy(s);
+    throw;
+  }
+  return s;
+}
+
+void check_membership(const uint32_t* q, uint64_t S) {
+  uint32_t mx = 0;
+  for (uint64_t u = 0; u < S; ++u) mx = std::max(mx, q[u]);
+  require(S == 0 || mx < S, LV_ERR_INVALID, "membership has a community id >= span");
+}
+
+template <class F>
+int guard(lv_context* c, F f) {
+  try {
+    f();
+    return LV_OK;
+  } catch (const LvError& e) {
+    if (c) c->err = e.msg;
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    if (c) c->err = "host allocation failed";
+    return LV_ERR_NOMEM;
+  } catch (const std::exception& e) {
+    if (c) c->err = e.what();
+    return LV_ERR_INVALID;
+  }
+}
+
+struct Owned {
+  lv_dgraph* g = nullptr;
+  lv_dbatch* b = nullptr;
+  lv_dstate* s = nullptr;
+  ~Owned() {
+    lv_dgraph_destroy(g);
+    lv_dbatch_destroy(b);
+    lv_dstate_destroy(s);
+  }
+};
+
+int host_entry(lv_context* c, int approach, const lv_graph* y, const lv_batch* del, const lv_batch* ins,
+               const uint32_t* q, const double* qvtot, const double* qctot, const lv_options* o, lv_result* res) {
+  return guard(c, [&] {
+    require(c != nullptr, LV_ERR_INVALID, "context is NULL");
+    require(res && res->membership && res->vertex_weight && res->community_weight, LV_ERR_INVALID,
+            "result arrays must be set");
+    Owned w;
+    w.g = upload_graph(c, y);
+    uint64_t S = w.g->S;
+    if (approach != LV_STATIC) {
+      require(q && qvtot && qctot, LV_ERR_INVALID, "previous state (q, qvtot, qctot) is NULL");
+      check_membership(q, S);
+      w.b = upload_batch(c, del, ins, S);
+    }
+    w.s = new_state(S);
+    if (approach != LV_STATIC) {
+      h2d(c, w.s->memb, q, S);
+      h2d(c, w.s->vtot, qvtot, S);
+      h2d(c, w.s->ctot, qctot, S);
+    }
+    run(c, approach, w.g, w.b, w.s, o, res);
+  });
+}
+
+}  // namespace
+
+// ==================================================================== C ABI
+extern "C" {
+
+void lv_options_default(lv_options* o) {
+  if (!o) return;
+  o->repeat = 1;
+  o->resolution = 1;
+  o->tolerance = 1e-2;
+  o->aggregation_tolerance = 1.0;
+  o->tolerance_drop = 10;
+  o->max_iterations = 20;
+  o->max_passes = 10;
+}
+
+const char* lv_status_string(int s) {
+  switch (s) {
+    case LV_OK: return "ok";
+    case LV_ERR_INVALID: return "invalid argument";
+    case LV_ERR_NOMEM: return "out of device memory";
+    case LV_ERR_HIP: return "HIP runtime error";
+    case LV_ERR_UNSORTED: return "batch not sorted/unique";
+    case LV_ERR_NODEVICE: return "no HIP device";
+    case LV_ERR_LIMIT: return "input exceeds a limit";
+    default: return "unknown status";
+  }
+}
+
+int lv_context_create(int device, lv_context** out) {
+  if (!out) return LV_ERR_INVALID;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return LV_ERR_NODEVICE;
+  if (device < 0 || device >= n) return LV_ERR_INVALID;
+  auto* c = new lv_context();
+  c->device = device;
+  int rc = guard(c, [&] {
+    HIPCHK(hipSetDevice(device));
+    HIPCHK(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
+  });
+  if (rc != LV_OK) {
+    lv_context_destroy(c);
+    return rc;
+  }
+  *out = c;
+  return LV_OK;
+}
+
+void lv_context_destroy(lv_context* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  if (c->st) (void)hipStreamSynchronize(c->st);
+  for (auto& kv : c->bufs)
+    if (kv.second.p) (void)hipFree(kv.second.p);
+  for (auto e : c->events) (void)hipEventDestroy(e);
+  if (c->pinned) (void)hipHostFree(c->pinned);
+  if (c->st) (void)hipStreamDestroy(c->st);
+  delete c;
+}
+
+const char* lv_last_error(const lv_context* c) { return c ? c->err.c_str() : "no context"; }
+
+int lv_context_set_path(lv_context* c, int path) {
+  if (!c || (path != 0 && path != 1)) return LV_ERR_INVALID;
+  c->path = path;
+  return LV_OK;
+}
+
+int lv_louvain_static(lv_context* c, const lv_graph* x, const lv_options* o, lv_result* res) {
+  return host_entry(c, LV_STATIC, x, nullptr, nullptr, nullptr, nullptr, nullptr, o, res);
+}
+int lv_louvain_naive_dynamic(lv_context* c, const lv_graph* y, const lv_batch* d, const lv_batch* i,
+                             const uint32_t* q, const double* qv, const double* qc, const lv_options* o,
+                             lv_result* res) {
+  return host_entry(c, LV_NAIVE_DYNAMIC, y, d, i, q, qv, qc, o, res);
+}
+int lv_louvain_delta_screening(lv_context* c, const lv_graph* y, const lv_batch* d, const lv_batch* i,
+                               const uint32_t* q, const double* qv, const double* qc, const lv_options* o,
+                               lv_result* res) {
+  return host_entry(c, LV_DELTA_SCREENING, y, d, i, q, qv, qc, o, res);
+}
+int lv_louvain_dynamic_frontier(lv_context* c, const lv_graph* y, const lv_batch* d, const lv_batch* i,
+                                const uint32_t* q, const double* qv, const double* qc, const lv_options* o,
+                                lv_result* res) {
+  return host_entry(c, LV_DYNAMIC_FRONTIER, y, d, i, q, qv, qc, o, res);
+}
+
+int lv_dgraph_create(lv_context* c, const lv_graph* g, lv_dgraph** out) {
+  if (!c || !out) return LV_ERR_INVALID;
+  return guard(c, [&] { *out = upload_graph(c, g); });
+}
+void lv_dgraph_destroy(lv_dgraph* g) {
+  if (!g) return;
+  (void)hipFree(g->off);
+  (void)hipFree(g->deg);
+  (void)hipFree(g->keys);
+  (void)hipFree(g->w);
+  (void)hipFree(g->ex);
+  (void)hipFree(g->tiers.ids);
+  delete g;
+}
+uint64_t lv_dgraph_span(const lv_dgraph* g) { return g ? g->S : 0; }
+uint64_t lv_dgraph_edges(const lv_dgraph* g) { return g ? g->E : 0; }
+
+int lv_dbatch_create(lv_context* c, const lv_batch* d, const lv_batch* i, lv_dbatch** out) {
+  if (!c || !out) return LV_ERR_INVALID;
+  return guard(c, [&] { *out = upload_batch(c, d, i, 0); });
+}
+void lv_dbatch_destroy(lv_dbatch* b) {
+  if (!b) return;
+  (void)hipFree(b->du);
+  (void)hipFree(b->dv);
+  (void)hipFree(b->dw);
+  (void)hipFree(b->iu);
+  (void)hipFree(b->iv);
+  (void)hipFree(b->iw);
+  delete b;
+}
+
+int lv_dstate_create(lv_context* c, uint64_t S, lv_dstate** out) {
+  if (!c || !out) return LV_ERR_INVALID;
+  return guard(c, [&] {
+    HIPCHK(hipSetDevice(c->device));
+    *out = new_state(S);
+  });
+}
+void lv_dstate_destroy(lv_dstate* s) {
+  if (!s) return;
+  (void)hipFree(s->memb);
+  (void)hipFree(s->vtot);
+  (void)hipFree(s->ctot);
+  delete s;
+}
+int lv_dstate_upload(lv_context* c, lv_dstate* s, const uint32_t* q, const double* qv, const double* qc) {
+  if (!c || !s || !q || !qv || !qc) return LV_ERR_INVALID;
+  return guard(c, [&] {
+    check_membership(q, s->S);
+    h2d(c, s->memb, q, s->S);
+    h2d(c, s->vtot, qv, s->S);
+    h2d(c, s->ctot, qc, s->S);
+    HIPCHK(hipStreamSynchronize(c->st));
+  });
+}
+int lv_dstate_download(lv_context* c, const lv_dstate* s, uint32_t* q, double* qv, double* qc) {
+  if (!c || !s) return LV_ERR_INVALID;
+  return guard(c, [&] {
+    if (q) d2h(c, q, s->memb, s->S);
+    if (qv) d2h(c, qv, s->vtot, s->S);
+    if (qc) d2h(c, qc, s->ctot, s->S);
+    HIPCHK(hipStreamSynchronize(c->st));
+  });
+}
+int lv_dstate_copy(lv_context* c, lv_dstate* dst, const lv_dstate* src) {
+  if (!c || !dst || !src || dst->S != src->S) return LV_ERR_INVALID;
+  return guard(c, [&] {
+    d2d(c, dst->memb, src->memb, src->S);
+    d2d(c, dst->vtot, src->vtot, src->S);
+    d2d(c, dst->ctot, src->ctot, src->S);
+    HIPCHK(hipStreamSynchronize(c->st));
+  });
+}
+
+int lv_run(lv_context* c, int approach, const lv_dgraph* g, const lv_dbatch* b, lv_dstate* s, const lv_options* o,
+           lv_result* res) {
+  if (!c) return LV_ERR_INVALID;
+  return guard(c, [&] {
+    if (b && g) {
+      // batch ids must lie within the graph (checked once per (batch, graph) use)
+      int* bad = c->ws<int>("bad", 1);
+      k_fill_u32(reinterpret_cast<uint32_t*>(bad), 0, 1, c->st);
+      k_check_sorted(b->du, b->dv, b->nd, (uint32_t)g->S, bad, c->st);
+      k_check_sorted(b->iu, b->iv, b->ni, (uint32_t)g->S, bad, c->st);
+      int f = c->read1(bad);
+      require(!(f & 1), LV_ERR_INVALID, "batch vertex id out of range");
+      require(!(f & 2), LV_ERR_UNSORTED, "batch not sorted by (u, v) or not unique");
+    }
+    run(c, approach, g, b, s, o, res);
+  });
+}
+
+int lv_dmodularity(lv_context* c, const lv_dgraph* g, const lv_dstate* s, double R, double* out) {
+  if (!c || !g || !s || !out) return LV_ERR_INVALID;
+  return guard(c, [&] {
+    double* cin = c->ws<double>("mod.cin", g->S);
+    double* ct = c->ws<double>("mod.ctot", g->S);
+    double* o = c->ws<double>("mod.out", 1);
+    k_fill_f64(cin, 0, g->S, c->st);
+    k_fill_f64(ct, 0, g->S, c->st);
+    k_fill_f64(o, 0, 1, c->st);
+    k_modularity_parts(g->view(), g->f64, s->memb, cin, ct, c->st);
+    k_modularity_sum(cin, ct, g->S, g->M2 / 2, R, o, c->st);
+    *out = c->read1(o);
+  });
+}
+
+// ------------------------------------------------------------ phase entry points
+int lv_edge_weight(lv_context* c, const lv_graph* g, double* out) {
+  if (!c || !out) return LV_ERR_INVALID;
+  return guard(c, [&] {
+    Owned w;
+    w.g = upload_graph(c, g);
+    *out = w.g->M2;
+  });
+}
+
+int lv_modularity(lv_context* c, const lv_graph* g, const uint32_t* memb, double R, double* out) {
+  if (!c || !memb || !out) return LV_ERR_INVALID;
+  return guard(c, [&] {
+    Owned w;
+    w.g = upload_graph(c, g);
+    check_membership(memb, w.g->S);
+    w.s = new_state(w.g->S);
+    h2d(c, w.s->memb, memb, w.g->S);
+    int rc = lv_dmodularity(c, w.g, w.s, R, out);
+    require(rc == LV_OK, rc, c->err);
+  });
+}
+
+int lv_vertex_weights(lv_context* c, const lv_graph* g, double* vtot) {
+  if (!c || !vtot) return LV_ERR_INVALID;
+  return guard(c, [&] {
+    Owned w;
+    w.g = upload_graph(c, g);
+    double* d = c->ws<double>("ph.vtot", w.g->S);
+    k_fill_f64(d, 0, w.g->S, c->st);
+    k_vertex_weights(w.g->view(), w.g->f64, d, c->st);
+    d2h(c, vtot, d, w.g->S);
+    HIPCHK(hipStreamSynchronize(c->st));
+  });
+}
+
+int lv_update_weights(lv_context* c, uint64_t S, const lv_batch* del, const lv_batch* ins, const uint32_t* vcom,
+                      double* vtot, double* ctot) {
+  if (!c || !vcom || !vtot || !ctot) return LV_ERR_INVALID;
+  return guard(c, [&] {
+    check_membership(vcom, S);
+    Owned w;
+    w.b = upload_batch(c, del, ins, S);
+    w.s = new_state(S);
+    h2d(c, w.s->memb, vcom, S);
+    h2d(c, w.s->vtot, vtot, S);
+    h2d(c, w.s->ctot, ctot, S);
+    k_update_weights(w.b->du, w.b->dw, w.b->nd, w.b->iu, w.b->iw, w.b->ni, w.s->memb, w.s->vtot, w.s->ctot, c->st);
+    d2h(c, vtot, w.s->vtot, S);
+    d2h(c, ctot, w.s->ctot, S);
+    HIPCHK(hipStreamSynchronize(c->st));
+  });
+}
+
+int lv_affected_frontier(lv_context* c, uint64_t S, const lv_batch* del, const lv_batch* ins, const uint32_t* vcom,
+                         uint8_t* vaff) {
+  if (!c || !vcom || !vaff) return LV_ERR_INVALID;
+  return guard(c, [&] {
+    check_membership(vcom, S);
+    Owned w;
+    w.b = upload_batch(c, del, ins, S);
+    w.s = new_state(S);
+    h2d(c, w.s->memb, vcom, S);
+    uint8_t* a = c->ws<uint8_t>("ph.vaff", (S + 7) & ~3ull);
+    k_fill_u8(a, 0, (S + 7) & ~3ull, c->st);
+    k_frontier(w.b->du, w.b->dv, w.b->nd, w.b->iu, w.b->iv, w.b->ni, w.s->memb, a, nullptr, nullptr, c->st);
+    d2h(c, vaff, a, S);
+    HIPCHK(hipStreamSynchronize(c->st));
+  });
+}
+
+int lv_affected_delta_screening(lv_context* c, const lv_graph* g, const lv_batch* del, const lv_batch* ins,
+                                const uint32_t* vcom, const double* vtot, const double* ctot, double M, double R,
+                                uint8_t* vertices, uint8_t* neighbors, uint8_t* communities) {
+  if (!c || !vcom || !vtot || !ctot || !vertices || !neighbors || !communities) return LV_ERR_INVALID;
+  return guard(c, [&] {
+    Owned w;
+    w.g = upload_graph(c, g);
+    uint64_t S = w.g->S;
+    check_membership(vcom, S);
+    w.b = upload_batch(c, del, ins, S);
+    w.s = new_state(S);
+    h2d(c, w.s->memb, vcom, S);
+    h2d(c, w.s->vtot, vtot, S);
+    h2d(c, w.s->ctot, ctot, S);
+    uint8_t* v = c->ws<uint8_t>("ph.dsv", S);
+    uint8_t* n = c->ws<uint8_t>("ph.dsn", S);
+    uint8_t* k = c->ws<uint8_t>("ph.dsc", S);
+    k_fill_u8(v, 0, S, c->st);
+    k_fill_u8(n, 0, S, c->st);
+    k_fill_u8(k, 0, S, c->st);
+    k_ds_deletions(w.b->du, w.b->dv, w.b->nd, w.s->memb, v, n, k, c->st);
+    if (w.b->ni) {
+      uint32_t* tkey = c->ws<uint32_t>("ds.tkey", 2 * w.b->ni);
+      double* tval = c->ws<double>("ds.tval", 2 * w.b->ni);
+      uint32_t* tlist = c->ws<uint32_t>("ds.tlist", w.b->ni);
+      k_fill_u32(tkey, kEmpty, 2 * w.b->ni, c->st);
+      k_ds_insertions(w.b->iu, w.b->iv, w.b->iw, w.b->ni, w.s->memb, w.s->vtot, w.s->ctot, M, R, tkey, tval, tlist,
+                      v, n, k, c->st);
+    }
+    k_ds_sweep(w.g->view(), w.s->memb, n, k, v, c->st);
+    d2h(c, vertices, v, S);
+    d2h(c, neighbors, n, S);
+    d2h(c, communities, k, S);
+    HIPCHK(hipStreamSynchronize(c->st));
+  });
+}
+
+int lv_choose_community(lv_context* c, const lv_graph* g, const uint32_t* vcom, const double* vtot,
+                        const double* ctot, double M, double R, uint32_t* best_c, double* best_e) {
+  if (!c || !vcom || !vtot || !ctot || !best_c || !best_e) return LV_ERR_INVALID;
+  return guard(c, [&] {
+    Owned w;
+    w.g = upload_graph(c, g);
+    uint64_t S = w.g->S;
+    check_membership(vcom, S);
+    w.s = new_state(S);
+    h2d(c, w.s->memb, vcom, S);
+    h2d(c, w.s->vtot, vtot, S);
+    h2d(c, w.s->ctot, ctot, S);
+    uint32_t* oc = c->ws<uint32_t>("ph.bc", S);
+    double* oe = c->ws<double>("ph.be", S);
+    k_fill_u32(oc, 0, S, c->st);
+    k_fill_f64(oe, 0, S, c->st);
+    Level lv{w.g->view(), w.g->f64, &w.g->tiers, w.g->order, w.g->tiers.maxdeg};
+    MoveArgs a = base_args(c, lv, w.s->memb, w.s->vtot, w.s->ctot, nullptr, nullptr, M, R);
+    a.choose_only = 1;
+    a.out_c = oc;
+    a.out_e = oe;
+    a.cnt = nullptr;
+    move_sweep(c, a, lv);
+    c->check();
+    d2h(c, best_c, oc, S);
+    d2h(c, best_e, oe, S);
+    HIPCHK(hipStreamSynchronize(c->st));
+  });
+}
+
+int lv_renumber(lv_context* c, const lv_graph* g, uint32_t* vcom, uint64_t n, uint32_t* cext, uint64_t* C) {
+  if (!c || !vcom || !cext || !C) return LV_ERR_INVALID;
+  return guard(c, [&] {
+    Owned w;
+    w.g = upload_graph(c, g);
+    uint64_t S = w.g->S;
+    require(n >= S, LV_ERR_INVALID, "vcom shorter than the graph span");
+    check_membership(vcom, n);
+    uint32_t* d = c->ws<uint32_t>("ph.vcom", n);
+    h2d(c, d, vcom, n);
+    uint32_t* map = nullptr;
+    *C = count_communities(c, w.g->view(), d, map);
+    k_lookup(d, n, map, c->st);
+    d2h(c, vcom, d, n);
+    d2h(c, cext, map, S);
+    HIPCHK(hipStreamSynchronize(c->st));
+  });
+}
+
+int lv_aggregate(lv_context* c, const lv_graph* g, const uint32_t* vcom, uint64_t C, uint64_t* zoff, uint32_t* zdeg,
+                 uint32_t* zkeys, double* zw) {
+  if (!c || !vcom || !zoff || !zdeg || !zkeys || !zw) return LV_ERR_INVALID;
+  return guard(c, [&] {
+    Owned w;
+    w.g = upload_graph(c, g);
+    uint64_t S = w.g->S, E = w.g->E;
+    check_membership(vcom, S);
+    uint32_t mx = 0;
+    for (uint64_t u = 0; u < S; ++u)
+      if (!g->exists || g->exists[u]) mx = std::max(mx, vcom[u]);
+    require(S == 0 || mx < C, LV_ERR_INVALID, "membership not renumbered below num_communities");
+    uint32_t* d = c->ws<uint32_t>("ph.vcom", S);
+    h2d(c, d, vcom, S);
+    SuperGraph z = super_graph(c, "ph.z", std::max<uint64_t>(C, 1), E);
+    double* ztot = c->ws<double>("ph.ztot", std::max<uint64_t>(C, 1));
+    Level lv{w.g->view(), w.g->f64, &w.g->tiers, w.g->order, w.g->tiers.maxdeg};
+    aggregate(c, lv, d, C, z, ztot);
+    d2h(c, zoff, z.off, C + 1);
+    d2h(c, zdeg, z.deg, C);
+    d2h(c, zkeys, z.keys, E);
+    d2h(c, zw, z.w, E);
+    HIPCHK(hipStreamSynchronize(c->st));
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,154 @@
+// Internal declarations shared by the kernels (lv_kernels.hip) and the host
+// pass driver / C ABI (lv_capi.hip).  gfx950 only.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <map>
+#include <string>
+
+#include "louvain_hip.h"
+
+namespace lv {
+
+constexpr uint32_t kEmpty = 0xFFFFFFFFu;  // empty hashtable key
+constexpr uint32_t kNoPos = 0xFFFFFFFFu;  // "not seen" first-position marker
+constexpr int kTiers = 5;                 // degree tiers of the group-per-vertex kernels
+// Upper degree bound of tiers 0..3 (LDS hashtables); tier 4 spills to HBM.
+constexpr uint32_t kTierMax[4] = {8, 32, 128, 1024};
+
+inline int tier_of(uint64_t d) {
+  for (int t = 0; t < 4; ++t)
+    if (d <= kTierMax[t]) return t;
+  return 4;
+}
+
+inline uint64_t next_pow2(uint64_t x) {
+  uint64_t p = 1;
+  while (p < x) p <<= 1;
+  return p;
+}
+
+// Device view of a (gapped) CSR graph.  Weight type is a template parameter
+// of the kernels: float for the input graph, double for super-graphs.
+struct GView {
+  uint64_t S;              // span
+  const uint64_t* off;     // S + 1
+  const uint32_t* deg;     // S (always materialised on the device)
+  const uint32_t* keys;
+  const void* w;           // float* or double*
+  const uint8_t* ex;       // S, or nullptr = every id exists
+};
+
+// Per-vertex degree-tier lists of a graph level (ascending ids per tier).
+struct TierLists {
+  uint32_t* ids = nullptr;      // concatenated lists, tier t at [start[t], start[t+1])
+  uint64_t start[kTiers + 1] = {0};
+  uint32_t maxdeg = 0;          // max degree of the tier-4 vertices
+};
+
+// Device counters of the local-moving kernels (for achieved-bandwidth figures).
+enum { CNT_VERTICES = 0, CNT_EDGES = 1, CNT_CANDIDATES = 2, CNT_MOVES = 3, CNT_N = 4 };
+
+// Arguments of the local-moving kernels.
+struct MoveArgs {
+  GView g;
+  const uint32_t* list;        // vertex ids to consider (nullptr = identity 0..n-1)
+  uint64_t n;                  // entries in list
+  const uint32_t* n_dev;       // if set, n is read from here on the device
+  uint32_t* vcom;
+  const double* vtot;
+  double* ctot;
+  uint8_t* vaff;
+  const uint8_t* fa;           // pass-1 restriction (delta-screening), or nullptr
+  double M, R;
+  int choose_only;             // 1: write best (c, e) per vertex, move nothing
+  uint32_t* out_c;
+  double* out_e;
+  double* el;                  // accumulated gain (l1-norm of delta-Q)
+  unsigned long long* cnt;     // CNT_N counters
+  uint32_t* qnext;             // worklist mode: vertices flagged 0 -> 1
+  uint32_t* qnext_len;
+  int pick_less;               // 1: only moves to a lower community id
+  // tier-4 HBM hashtables: one region of hcap slots per block
+  uint32_t* hkey;
+  uint32_t* hpos;
+  double* hval;
+  uint64_t hcap;
+  const int* done;             // device convergence flag: skip work when set
+};
+
+// Kernel launchers (lv_kernels.hip).  All run on `st`.
+void k_fill_u8(uint8_t* a, uint8_t v, uint64_t n, hipStream_t st);
+void k_fill_u32(uint32_t* a, uint32_t v, uint64_t n, hipStream_t st);
+void k_fill_f64(double* a, double v, uint64_t n, hipStream_t st);
+void k_fill_u64(uint64_t* a, uint64_t v, uint64_t n, hipStream_t st);
+void k_iota_u32(uint32_t* a, uint64_t n, hipStream_t st);
+void k_degrees_from_offsets(const uint64_t* off, uint32_t* deg, uint64_t S, hipStream_t st);
+void k_vertex_weights(GView g, bool f64, double* vtot, hipStream_t st);
+void k_edge_weight(GView g, bool f64, double* out, hipStream_t st);
+void k_initialize(GView g, uint32_t* vcom, double* ctot, const double* vtot, hipStream_t st);
+void k_update_weights(const uint32_t* du, const float* dw, uint64_t nd, const uint32_t* iu, const float* iw,
+                      uint64_t ni, const uint32_t* vcom, double* vtot, double* ctot, hipStream_t st);
+void k_frontier(const uint32_t* du, const uint32_t* dv, uint64_t nd, const uint32_t* iu, const uint32_t* iv,
+                uint64_t ni, const uint32_t* vcom, uint8_t* vaff, uint32_t* q, uint32_t* qlen, hipStream_t st);
+void k_ds_deletions(const uint32_t* du, const uint32_t* dv, uint64_t nd, const uint32_t* vcom, uint8_t* vertices,
+                    uint8_t* neighbors, uint8_t* communities, hipStream_t st);
+void k_ds_insertions(const uint32_t* iu, const uint32_t* iv, const float* iw, uint64_t ni, const uint32_t* vcom,
+                     const double* vtot, const double* ctot, double M, double R, uint32_t* tkey, double* tval,
+                     uint32_t* tlist, uint8_t* vertices, uint8_t* neighbors, uint8_t* communities, hipStream_t st);
+void k_ds_sweep(GView g, const uint32_t* vcom, const uint8_t* neighbors, const uint8_t* communities,
+                uint8_t* vertices, hipStream_t st);
+void k_count_u8(const uint8_t* a, uint8_t v, uint64_t n, unsigned long long* out, hipStream_t st);
+void k_check_sorted(const uint32_t* u, const uint32_t* v, uint64_t n, uint32_t S, int* bad, hipStream_t st);
+void k_move(const MoveArgs& a, int tier, bool f64, hipStream_t st);
+void k_exists(GView g, const uint32_t* vcom, uint32_t* flags, hipStream_t st);
+void k_lookup(uint32_t* a, uint64_t n, const uint32_t* map, hipStream_t st);
+void k_count_members(GView g, const uint32_t* vcom, uint32_t* cnt, hipStream_t st);
+void k_scatter_members(GView g, const uint32_t* vcom, const uint32_t* coff, uint32_t* cdeg, uint32_t* cedg,
+                       hipStream_t st);
+void k_member_degrees(const uint32_t* cedg, uint64_t n, GView g, uint64_t* mdeg, hipStream_t st);
+void k_gather_yoff(const uint32_t* coff, const uint64_t* mscan, uint64_t C, uint64_t* yoff, hipStream_t st);
+void k_tier_count(const uint64_t* off, const uint32_t* deg, const uint8_t* ex, uint64_t n, int caps_mode,
+                  uint64_t CN, uint32_t* block_counts, uint32_t nblocks, hipStream_t st);
+void k_tier_scatter(const uint64_t* off, const uint32_t* deg, const uint8_t* ex, uint64_t n, int caps_mode,
+                    uint64_t CN, const uint32_t* block_base, uint32_t nblocks, uint32_t* ids, hipStream_t st);
+struct AggArgs {
+  GView g;                   // graph being aggregated
+  const uint32_t* vcom;      // renumbered membership
+  const uint32_t* coff;      // community -> members offsets (C + 1)
+  const uint32_t* cedg;      // members
+  const uint64_t* mscan;     // exclusive scan of member degrees (members + 1)
+  const uint64_t* yoff;      // super-graph row offsets (C + 1)
+  uint64_t C;
+  const uint32_t* list;      // communities of this tier
+  uint64_t n;
+  uint32_t* zdeg;
+  uint32_t* zkeys;
+  double* zw;
+  double* ztot;              // row sums (K of the super-vertices), accumulated
+  // tier 4: shared HBM tables, one per community, chunked over blocks
+  uint32_t* hkey;
+  double* hval;
+  const uint64_t* hoff;      // table offset per tier-4 community (n + 1)
+  const uint64_t* choff;     // chunk offsets per tier-4 community (n + 1)
+  uint64_t nchunks;
+};
+void k_aggregate(const AggArgs& a, int tier, bool f64, hipStream_t st);
+void k_agg_emit_hbm(const AggArgs& a, uint64_t total_slots, hipStream_t st);
+void k_t4_sizes(const uint64_t* yoff, const uint32_t* list, uint64_t n, uint64_t C, uint64_t chunk,
+                uint64_t* caps, uint64_t* chunks, hipStream_t st);
+void k_bin_queue(const uint32_t* q, const uint32_t* qlen, uint64_t qcap, const uint32_t* deg, uint32_t* tq,
+                 uint64_t stride, uint32_t* tlen, hipStream_t st);
+void k_max_deg(const uint32_t* ids, uint64_t n, const uint32_t* deg, uint32_t* out, hipStream_t st);
+void k_max_u32(const uint32_t* a, uint64_t n, uint32_t* out, hipStream_t st);
+void k_compact_flags(const uint8_t* f, uint64_t n, uint32_t* q, uint32_t* qlen, hipStream_t st);
+void k_modularity_parts(GView g, bool f64, const uint32_t* memb, double* cin, double* ctot, hipStream_t st);
+void k_modularity_sum(const double* cin, const double* ctot, uint64_t n, double M, double R, double* out,
+                      hipStream_t st);
+// Exclusive scans (hipCUB) with grow-only temp storage owned by the caller.
+hipError_t scan_u32(const uint32_t* in, uint32_t* out, uint64_t n, void* tmp, size_t& tmp_bytes, hipStream_t st);
+hipError_t scan_u64(const uint64_t* in, uint64_t* out, uint64_t n, void* tmp, size_t& tmp_bytes, hipStream_t st);
+
+}  // namespace lv

@@ -1,0 +1,1021 @@
+// HIP kernels of the MI355X Louvain library (gfx950, wave64).
+//
+// Hot path: the local-moving kernel (move_kernel) scores every neighbour
+// community of a vertex with deltaModularity (inc/properties.hxx:253-256)
+// after summing edge weights per community in an open-addressing hashtable.
+// A vertex is handled by a group of G lanes; G and the table capacity come
+// from the vertex's degree tier:
+//   tier 0: deg <= 8     G = 8,    16 LDS slots   (32 vertices per block)
+//   tier 1: deg <= 32    G = 16,   64 LDS slots   (16 per block)
+//   tier 2: deg <= 128   G = 64,  256 LDS slots   (one wave per vertex)
+//   tier 3: deg <= 1024  G = 256, 2048 LDS slots  (one workgroup per vertex)
+//   tier 4: deg > 1024   G = 1024, HBM table of nextpow2(2 deg) slots
+// A slot holds (key u32, first position u32, sum f64): the first position
+// (edge index of the key's first appearance) reproduces the reference's
+// first-seen tie-break (inc/louvain.hxx:409, :461) independently of the
+// order in which lanes insert.  Aggregation reuses the same tables.
+//
+// Built with -ffp-contract=off: deltaModularity must evaluate without fused
+// multiply-adds to match the reference's x86-64 arithmetic bit for bit.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include "lv_internal.hpp"
+
+namespace lv {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+__host__ __device__ inline uint64_t grid_for(uint64_t n, uint64_t per_block, uint64_t cap = 1u << 20) {
+  uint64_t g = (n + per_block - 1) / per_block;
+  if (g < 1) g = 1;
+  return g < cap ? g : cap;
+}
+
+__device__ __forceinline__ uint32_t slot_of(uint32_t c, uint64_t cap) {
+  return (uint32_t)(((uint64_t)(c * 2654435761u) * cap) >> 32);
+}
+
+// deltaModularity, inc/properties.hxx:253-256 (same operation order).
+__device__ __forceinline__ double delta_q(double vcout, double vdout, double vtot, double ctot, double dtot,
+                                          double M, double R) {
+  return (vcout - vdout) / M - R * vtot * (vtot + ctot - dtot) / (2 * M * M);
+}
+
+__device__ __forceinline__ void tab_add(uint32_t* key, uint32_t* pos, double* val, uint64_t cap, uint32_t c,
+                                        double w, uint32_t p) {
+  uint32_t s = slot_of(c, cap);
+  while (true) {
+    uint32_t k = atomicCAS(&key[s], kEmpty, c);
+    if (k == kEmpty || k == c) {
+      unsafeAtomicAdd(&val[s], w);
+      if (pos) atomicMin(&pos[s], p);
+      return;
+    }
+    s = (uint32_t)((s + 1) & (cap - 1));
+  }
+}
+
+__device__ __forceinline__ double tab_get(const uint32_t* key, const double* val, uint64_t cap, uint32_t c) {
+  uint32_t s = slot_of(c, cap);
+  while (true) {
+    uint32_t k = key[s];
+    if (k == c) return val[s];
+    if (k == kEmpty) return 0;
+    s = (uint32_t)((s + 1) & (cap - 1));
+  }
+}
+
+// Better candidate: larger gain, then earlier first appearance.
+__device__ __forceinline__ bool better(double e, uint32_t p, double be, uint32_t bp) {
+  return e > be || (e == be && p < bp);
+}
+
+template <int W>
+__device__ __forceinline__ void group_best_shfl(double& be, uint32_t& bp, uint32_t& bc) {
+#pragma unroll
+  for (int m = W / 2; m > 0; m >>= 1) {
+    double oe = __shfl_xor(be, m, W);
+    uint32_t op = __shfl_xor(bp, m, W);
+    uint32_t oc = __shfl_xor(bc, m, W);
+    if (better(oe, op, be, bp)) { be = oe; bp = op; bc = oc; }
+  }
+}
+
+// Block-wide best (every thread of the block must call it).
+template <int BLOCK>
+__device__ void block_best(double& be, uint32_t& bp, uint32_t& bc) {
+  __shared__ double se[BLOCK / 64];
+  __shared__ uint32_t sp[BLOCK / 64], sc[BLOCK / 64];
+  group_best_shfl<64>(be, bp, bc);
+  int w = threadIdx.x / 64;
+  if (threadIdx.x % 64 == 0) { se[w] = be; sp[w] = bp; sc[w] = bc; }
+  __syncthreads();
+  be = se[0]; bp = sp[0]; bc = sc[0];
+#pragma unroll
+  for (int i = 1; i < BLOCK / 64; ++i)
+    if (better(se[i], sp[i], be, bp)) { be = se[i]; bp = sp[i]; bc = sc[i]; }
+  __syncthreads();
+}
+
+template <int BLOCK>
+__device__ double block_sum(double x) {
+  __shared__ double s[BLOCK / 64];
+#pragma unroll
+  for (int m = 32; m > 0; m >>= 1) x += __shfl_xor(x, m, 64);
+  if (threadIdx.x % 64 == 0) s[threadIdx.x / 64] = x;
+  __syncthreads();
+  double t = 0;
+#pragma unroll
+  for (int i = 0; i < BLOCK / 64; ++i) t += s[i];
+  __syncthreads();
+  return t;
+}
+
+template <int BLOCK>
+__device__ unsigned long long block_sum_u64(unsigned long long x) {
+  __shared__ unsigned long long s[BLOCK / 64];
+#pragma unroll
+  for (int m = 32; m > 0; m >>= 1) x += __shfl_xor(x, m, 64);
+  if (threadIdx.x % 64 == 0) s[threadIdx.x / 64] = x;
+  __syncthreads();
+  unsigned long long t = 0;
+#pragma unroll
+  for (int i = 0; i < BLOCK / 64; ++i) t += s[i];
+  __syncthreads();
+  return t;
+}
+
+// Set vaff[v] = 1; in worklist mode push v when its flag goes 0 -> 1.
+// Pushes are aggregated per wave (one atomic per wave).
+__device__ __forceinline__ void flag_vertex(uint8_t* vaff, uint32_t v, uint32_t* q, uint32_t* qlen) {
+  if (!q) {
+    vaff[v] = 1;
+    return;
+  }
+  uint32_t sh = (v & 3u) * 8u;
+  uint32_t old = atomicOr(reinterpret_cast<uint32_t*>(vaff) + (v >> 2), 1u << sh);
+  bool fresh = ((old >> sh) & 0xFFu) == 0;
+  unsigned long long mask = __ballot(fresh);
+  if (!mask) return;
+  int lane = __lane_id();
+  int leader = __ffsll((long long)mask) - 1;
+  uint32_t base = 0;
+  if (lane == leader) base = atomicAdd(qlen, (uint32_t)__popcll(mask));
+  base = __shfl(base, leader);
+  if (fresh) q[base + __popcll(mask & ((1ull << lane) - 1))] = v;
+}
+
+// ---------------------------------------------------------------- fills
+template <class T>
+__global__ void fill_kernel(T* a, T v, uint64_t n) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    a[i] = v;
+}
+
+__global__ void iota_kernel(uint32_t* a, uint64_t n) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    a[i] = (uint32_t)i;
+}
+
+__global__ void degrees_kernel(const uint64_t* off, uint32_t* deg, uint64_t S) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < S; i += (uint64_t)gridDim.x * blockDim.x)
+    deg[i] = (uint32_t)(off[i + 1] - off[i]);
+}
+
+// --------------------------------------------- per-vertex row sums (K, 2M)
+// louvainVertexWeightsOmpW (inc/louvain.hxx:210-218) and edgeWeightOmp
+// (inc/properties.hxx:96-106).  Rows of up to 64 edges are summed by one
+// lane in row order (the reference's order); longer rows by the whole wave.
+template <class WT, bool TOTAL>
+__global__ __launch_bounds__(kBlock) void row_sum_kernel(GView g, double* vtot, double* total) {
+  const WT* W = (const WT*)g.w;
+  double acc = 0;
+  int lane = __lane_id();
+  for (uint64_t base = (blockIdx.x * (uint64_t)kBlock) & ~63ull; base < g.S;
+       base += (uint64_t)gridDim.x * kBlock) {
+    uint64_t u = base + (threadIdx.x & ~63u) + lane;
+    bool live = u < g.S && (!g.ex || g.ex[u]);
+    uint64_t o = live ? g.off[u] : 0;
+    uint32_t d = live ? g.deg[u] : 0;
+    bool big = live && d > 64;
+    if (live && !big) {
+      double s = 0;
+      for (uint32_t i = 0; i < d; ++i) s += (double)W[o + i];
+      if (TOTAL) acc += s;
+      else vtot[u] += s;
+    }
+    unsigned long long mask = __ballot(big);
+    while (mask) {
+      int j = __ffsll((long long)mask) - 1;
+      mask &= mask - 1;
+      uint64_t oj = __shfl(o, j);
+      uint32_t dj = __shfl(d, j);
+      double s = 0;
+      for (uint32_t i = lane; i < dj; i += 64) s += (double)W[oj + i];
+#pragma unroll
+      for (int m = 32; m > 0; m >>= 1) s += __shfl_xor(s, m, 64);
+      if (lane == j) {
+        if (TOTAL) acc += s;
+        else vtot[u] += s;
+      }
+    }
+  }
+  if (TOTAL) {
+    double t = block_sum<kBlock>(acc);
+    if (threadIdx.x == 0) unsafeAtomicAdd(total, t);
+  }
+}
+
+// louvainInitializeOmpW (inc/louvain.hxx:285-293)
+__global__ void initialize_kernel(GView g, uint32_t* vcom, double* ctot, const double* vtot) {
+  for (uint64_t u = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; u < g.S; u += (uint64_t)gridDim.x * blockDim.x) {
+    if (g.ex && !g.ex[u]) continue;
+    vcom[u] = (uint32_t)u;
+    ctot[u] = vtot[u];
+  }
+}
+
+// louvainUpdateWeightsFromOmpU (inc/louvain.hxx:374-388): K[u] and
+// Sigma_tot[vcom[u]] minus deletions, plus insertions.  f64 atomics; the
+// reference's per-key batch order only matters for rounding.
+__global__ void update_weights_kernel(const uint32_t* du, const float* dw, uint64_t nd, const uint32_t* iu,
+                                      const float* iw, uint64_t ni, const uint32_t* vcom, double* vtot,
+                                      double* ctot) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < nd + ni;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t u;
+    double w;
+    if (i < nd) { u = du[i]; w = -(double)dw[i]; }
+    else { u = iu[i - nd]; w = (double)iw[i - nd]; }
+    unsafeAtomicAdd(&vtot[u], w);
+    unsafeAtomicAdd(&ctot[vcom[u]], w);
+  }
+}
+
+// louvainAffectedVerticesFrontierOmpW (inc/louvain.hxx:1606-1624); vaff must
+// be zero on entry.  With q set, each newly marked vertex is also queued.
+__global__ void frontier_kernel(const uint32_t* du, const uint32_t* dv, uint64_t nd, const uint32_t* iu,
+                                const uint32_t* iv, uint64_t ni, const uint32_t* vcom, uint8_t* vaff, uint32_t* q,
+                                uint32_t* qlen) {
+  uint64_t n = nd + ni;
+  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  // uniform trip count per wave so the queue push can use wave ballots
+  for (uint64_t b = (blockIdx.x * (uint64_t)blockDim.x) & ~63ull; b < n; b += stride) {
+    uint64_t i = b + (threadIdx.x & ~63u) + __lane_id();
+    bool mark = false;
+    uint32_t u = 0;
+    if (i < nd) {
+      u = du[i];
+      mark = vcom[u] == vcom[dv[i]];
+    } else if (i < n) {
+      u = iu[i - nd];
+      mark = vcom[u] != vcom[iv[i - nd]];
+    }
+    if (q) {
+      // flag_vertex ballots over the active lanes: keep all lanes active
+      bool fresh = false;
+      uint32_t sh = (u & 3u) * 8u;
+      if (mark) {
+        uint32_t old = atomicOr(reinterpret_cast<uint32_t*>(vaff) + (u >> 2), 1u << sh);
+        fresh = ((old >> sh) & 0xFFu) == 0;
+      }
+      unsigned long long mask = __ballot(fresh);
+      if (mask) {
+        int lane = __lane_id();
+        int leader = __ffsll((long long)mask) - 1;
+        uint32_t base = 0;
+        if (lane == leader) base = atomicAdd(qlen, (uint32_t)__popcll(mask));
+        base = __shfl(base, leader);
+        if (fresh) q[base + __popcll(mask & ((1ull << lane) - 1))] = u;
+      }
+    } else if (mark) {
+      vaff[u] = 1;
+    }
+  }
+}
+
+// -------------------------------------------------- delta-screening marking
+// louvainAffectedVerticesDeltaScreeningOmpW (inc/louvain.hxx:1441-1486).
+__global__ void ds_deletions_kernel(const uint32_t* du, const uint32_t* dv, uint64_t nd, const uint32_t* vcom,
+                                    uint8_t* vertices, uint8_t* neighbors, uint8_t* communities) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < nd; i += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t u = du[i], v = dv[i];
+    if (vcom[u] != vcom[v]) continue;
+    vertices[u] = 1;
+    neighbors[u] = 1;
+    communities[vcom[v]] = 1;
+  }
+}
+
+// One thread per insertion group (run of equal sources, in batch order),
+// scanned sequentially exactly as the reference does: communities listed in
+// first-seen order, sums in batch order, strict '>' from 0.  The group at
+// batch index i uses table slots [2i, 2i + 2 len) and list slots [i, i + len).
+__global__ void ds_insertions_kernel(const uint32_t* iu, const uint32_t* iv, const float* iw, uint64_t ni,
+                                     const uint32_t* vcom, const double* vtot, const double* ctot, double M, double R,
+                                     uint32_t* tkey, double* tval, uint32_t* tlist, uint8_t* vertices,
+                                     uint8_t* neighbors, uint8_t* communities) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < ni; i += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t u = iu[i];
+    if (i > 0 && iu[i - 1] == u) continue;  // not a group head
+    uint64_t len = 1;
+    while (i + len < ni && iu[i + len] == u) ++len;
+    uint64_t cap = 2 * len;
+    uint32_t* key = tkey + 2 * i;
+    double* val = tval + 2 * i;
+    uint32_t* list = tlist + i;
+    uint32_t d = vcom[u], cnt = 0;
+    for (uint64_t j = i; j < i + len; ++j) {
+      uint32_t c = vcom[iv[j]];
+      if (c == d) continue;
+      uint64_t s = ((uint64_t)(c * 2654435761u)) % cap;
+      while (key[s] != kEmpty && key[s] != c) s = (s + 1) % cap;
+      if (key[s] == kEmpty) { key[s] = c; val[s] = 0; }
+      if (!val[s]) list[cnt++] = c;  // louvainScanCommunityW, inc/louvain.hxx:409
+      val[s] += (double)iw[j];
+    }
+    uint32_t cmax = 0;
+    double emax = 0;
+    for (uint32_t k = 0; k < cnt; ++k) {
+      uint32_t c = list[k];
+      if (c == d) continue;
+      uint64_t s = ((uint64_t)(c * 2654435761u)) % cap;
+      while (key[s] != c) s = (s + 1) % cap;
+      double e = delta_q(val[s], 0.0, vtot[u], ctot[c], ctot[d], M, R);
+      if (e > emax) { emax = e; cmax = c; }
+    }
+    if (emax <= 0) continue;
+    vertices[u] = 1;
+    neighbors[u] = 1;
+    communities[cmax] = 1;
+  }
+}
+
+// Sweep: neighbours of flagged vertices, members of flagged communities.
+__global__ __launch_bounds__(kBlock) void ds_sweep_kernel(GView g, const uint32_t* vcom, const uint8_t* neighbors,
+                                                          const uint8_t* communities, uint8_t* vertices) {
+  int lane = __lane_id();
+  for (uint64_t base = (blockIdx.x * (uint64_t)kBlock) & ~63ull; base < g.S; base += (uint64_t)gridDim.x * kBlock) {
+    uint64_t u = base + (threadIdx.x & ~63u) + lane;
+    bool live = u < g.S && (!g.ex || g.ex[u]);
+    bool nb = live && neighbors[u];
+    if (live && communities[vcom[u]]) vertices[u] = 1;
+    unsigned long long mask = __ballot(nb);
+    uint64_t o = live ? g.off[u] : 0;
+    uint32_t d = live ? g.deg[u] : 0;
+    while (mask) {
+      int j = __ffsll((long long)mask) - 1;
+      mask &= mask - 1;
+      uint64_t oj = __shfl(o, j);
+      uint32_t dj = __shfl(d, j);
+      for (uint32_t i = lane; i < dj; i += 64) vertices[g.keys[oj + i]] = 1;
+    }
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void count_u8_kernel(const uint8_t* a, uint8_t v, uint64_t n,
+                                                          unsigned long long* out) {
+  unsigned long long c = 0;
+  for (uint64_t i = blockIdx.x * (uint64_t)kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock)
+    c += a[i] == v;
+  c = block_sum_u64<kBlock>(c);
+  if (threadIdx.x == 0 && c) atomicAdd(out, c);
+}
+
+__global__ void check_sorted_kernel(const uint32_t* u, const uint32_t* v, uint64_t n, uint32_t S, int* bad) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    if (u[i] >= S || v[i] >= S) { atomicOr(bad, 1); continue; }
+    if (i > 0 && (u[i - 1] > u[i] || (u[i - 1] == u[i] && v[i - 1] >= v[i]))) atomicOr(bad, 2);
+  }
+}
+
+// ------------------------------------------------------- local-moving
+// louvainMoveOmpW body (inc/louvain.hxx:595-605): for each affected vertex,
+// sum edge weights per neighbour community (self-loops skipped), choose
+// the best positive deltaModularity (first-seen on ties), and unless it is
+// community 0 move there (atomic Sigma_tot), flag the neighbours; clear the
+// vertex's flag and add its gain to el.  One group of G lanes per vertex.
+template <int BLOCK, int G, int CAP, class WT>
+__global__ __launch_bounds__(BLOCK) void move_kernel(MoveArgs a) {
+  constexpr int GPB = BLOCK / G;
+  constexpr bool HBM = CAP == 0;
+  constexpr int LCAP = HBM ? 1 : CAP;
+  __shared__ uint32_t s_key[GPB * LCAP];
+  __shared__ uint32_t s_pos[GPB * LCAP];
+  __shared__ double s_val[GPB * LCAP];
+  if (a.done && *a.done) return;
+  const int lane = threadIdx.x % G, grp = threadIdx.x / G;
+  const uint64_t n = a.n_dev ? (uint64_t)*a.n_dev : a.n;
+  const WT* W = (const WT*)a.g.w;
+  uint32_t *tkey, *tpos;
+  double* tval;
+  uint64_t cap;
+  if constexpr (HBM) {
+    uint64_t base = (uint64_t)blockIdx.x * a.hcap;
+    tkey = a.hkey + base;
+    tpos = a.hpos + base;
+    tval = a.hval + base;
+    cap = a.hcap;
+  } else {
+    tkey = s_key + grp * CAP;
+    tpos = s_pos + grp * CAP;
+    tval = s_val + grp * CAP;
+    cap = CAP;
+  }
+  double el_acc = 0;
+  unsigned long long c_vert = 0, c_edge = 0, c_cand = 0, c_move = 0;
+  for (uint64_t base = (uint64_t)blockIdx.x * GPB; base < n; base += (uint64_t)gridDim.x * GPB) {
+    const uint64_t idx = base + grp;
+    bool act = false;
+    uint32_t u = 0;
+    if (idx < n) {
+      u = a.list ? a.list[idx] : (uint32_t)idx;
+      act = (!a.g.ex || a.g.ex[u]) && (a.choose_only || (a.vaff[u] && (!a.fa || a.fa[u])));
+    }
+    uint64_t o = 0;
+    uint32_t d = 0, cu = 0;
+    double Ku = 0;
+    if (act) {
+      o = a.g.off[u];
+      d = a.g.deg[u];
+      cu = a.vcom[u];
+      Ku = a.vtot[u];
+      if constexpr (HBM) {
+        // this hub's table: nextpow2(2 d) <= hcap slots of the block's region
+        cap = 2;
+        while (cap < 2 * (uint64_t)d) cap <<= 1;
+      }
+      for (uint64_t s = lane; s < cap; s += G) { tkey[s] = kEmpty; tpos[s] = kNoPos; tval[s] = 0; }
+    }
+    __syncthreads();
+    if (act) {
+      for (uint32_t i = lane; i < d; i += G) {
+        uint32_t v = a.g.keys[o + i];
+        if (v == u) continue;
+        tab_add(tkey, tpos, tval, cap, a.vcom[v], (double)W[o + i], i);
+      }
+    }
+    __syncthreads();
+    double be = 0;
+    uint32_t bp = kNoPos, bc = 0;
+    if (act) {
+      const double vd = tab_get(tkey, tval, cap, cu);
+      const double Sd = a.ctot[cu];
+      for (uint64_t s = lane; s < cap; s += G) {
+        uint32_t c = tkey[s];
+        if (c == kEmpty) continue;
+        ++c_cand;
+        if (c == cu) continue;
+        if (a.pick_less && c > cu) continue;
+        double e = delta_q(tval[s], vd, Ku, a.ctot[c], Sd, a.M, a.R);
+        uint32_t p = tpos[s];
+        if (e > 0 && better(e, p, be, bp)) { be = e; bp = p; bc = c; }
+      }
+    }
+    if constexpr (G <= 64) group_best_shfl<G>(be, bp, bc);
+    else block_best<BLOCK>(be, bp, bc);
+    if (act) {
+      if (a.choose_only) {
+        if (lane == 0) { a.out_c[u] = bc; a.out_e[u] = be; }
+      } else {
+        if (bc != 0) {
+          if (lane == 0) {
+            unsafeAtomicAdd(&a.ctot[cu], -Ku);
+            unsafeAtomicAdd(&a.ctot[bc], Ku);
+            a.vcom[u] = bc;
+            ++c_move;
+          }
+          for (uint32_t i = lane; i < d; i += G) {
+            uint32_t v = a.g.keys[o + i];
+            if (v != u) flag_vertex(a.vaff, v, a.qnext, a.qnext_len);
+          }
+        }
+        if (lane == 0) {
+          a.vaff[u] = 0;
+          el_acc += be;
+          ++c_vert;
+          c_edge += d;
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (!a.choose_only) {
+    double t = block_sum<BLOCK>(el_acc);
+    if (threadIdx.x == 0 && t != 0) unsafeAtomicAdd(a.el, t);
+  }
+  if (a.cnt) {
+    unsigned long long v = block_sum_u64<BLOCK>(c_vert), e = block_sum_u64<BLOCK>(c_edge);
+    unsigned long long c = block_sum_u64<BLOCK>(c_cand), m = block_sum_u64<BLOCK>(c_move);
+    if (threadIdx.x == 0) {
+      if (v) atomicAdd(&a.cnt[CNT_VERTICES], v);
+      if (e) atomicAdd(&a.cnt[CNT_EDGES], e);
+      if (c) atomicAdd(&a.cnt[CNT_CANDIDATES], c);
+      if (m) atomicAdd(&a.cnt[CNT_MOVES], m);
+    }
+  }
+}
+
+// ------------------------------------------------------------ renumbering
+// louvainCommunityExistsOmpW (inc/louvain.hxx:668-682): flags must be zero.
+__global__ void exists_kernel(GView g, const uint32_t* vcom, uint32_t* flags) {
+  for (uint64_t u = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; u < g.S; u += (uint64_t)gridDim.x * blockDim.x) {
+    if (g.ex && !g.ex[u]) continue;
+    flags[vcom[u]] = 1;
+  }
+}
+
+// louvainLookupCommunitiesOmpU (inc/louvain.hxx:838-844)
+__global__ void lookup_kernel(uint32_t* a, uint64_t n, const uint32_t* map) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    a[i] = map[a[i]];
+}
+
+// ------------------------------------------------------------ aggregation
+// louvainCountCommunityVerticesOmpW / louvainCommunityVerticesOmpW
+// (inc/louvain.hxx:752-762, :800-812)
+__global__ void count_members_kernel(GView g, const uint32_t* vcom, uint32_t* cnt) {
+  for (uint64_t u = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; u < g.S; u += (uint64_t)gridDim.x * blockDim.x) {
+    if (g.ex && !g.ex[u]) continue;
+    atomicAdd(&cnt[vcom[u]], 1u);
+  }
+}
+
+__global__ void scatter_members_kernel(GView g, const uint32_t* vcom, const uint32_t* coff, uint32_t* cdeg,
+                                       uint32_t* cedg) {
+  for (uint64_t u = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; u < g.S; u += (uint64_t)gridDim.x * blockDim.x) {
+    if (g.ex && !g.ex[u]) continue;
+    uint32_t c = vcom[u];
+    uint32_t k = atomicAdd(&cdeg[c], 1u);
+    cedg[coff[c] + k] = (uint32_t)u;
+  }
+}
+
+// Degrees of the members in member order (+ a trailing 0): its exclusive
+// scan gives each community's flat edge range, which is also the
+// super-graph row offsets of louvainCommunityTotalDegreeOmpW + scan
+// (inc/louvain.hxx:712-722, :989-990).
+__global__ void member_degrees_kernel(const uint32_t* cedg, uint64_t n, GView g, uint64_t* mdeg) {
+  for (uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j <= n; j += (uint64_t)gridDim.x * blockDim.x)
+    mdeg[j] = j < n ? g.deg[cedg[j]] : 0;
+}
+
+__global__ void gather_yoff_kernel(const uint32_t* coff, const uint64_t* mscan, uint64_t C, uint64_t* yoff) {
+  for (uint64_t c = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; c <= C; c += (uint64_t)gridDim.x * blockDim.x)
+    yoff[c] = mscan[coff[c]];
+}
+
+// Member j of community range [m0, m1) holding flat edge f.
+__device__ __forceinline__ uint64_t member_of(const uint64_t* mscan, uint64_t m0, uint64_t m1, uint64_t f) {
+  uint64_t lo = m0, hi = m1 - 1;
+  while (lo < hi) {
+    uint64_t mid = (lo + hi + 1) >> 1;
+    if (mscan[mid] <= f) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// louvainAggregateEdgesOmpW (inc/louvain.hxx:897-912): per community, sum
+// edge weights by the neighbour's community over all members' edges
+// (self-loops included) and write the (community, weight) pairs into row c
+// of the gapped super-graph.  LDS tiers 0..3, one group per community.
+template <int BLOCK, int G, int CAP, class WT>
+__global__ __launch_bounds__(BLOCK) void aggregate_kernel(AggArgs a) {
+  constexpr int GPB = BLOCK / G;
+  __shared__ uint32_t s_key[GPB * CAP];
+  __shared__ double s_val[GPB * CAP];
+  __shared__ uint32_t s_cnt[GPB];
+  const int lane = threadIdx.x % G, grp = threadIdx.x / G;
+  const WT* W = (const WT*)a.g.w;
+  uint32_t* tkey = s_key + grp * CAP;
+  double* tval = s_val + grp * CAP;
+  for (uint64_t base = (uint64_t)blockIdx.x * GPB; base < a.n; base += (uint64_t)gridDim.x * GPB) {
+    const uint64_t idx = base + grp;
+    const bool act = idx < a.n;
+    uint32_t c = act ? a.list[idx] : 0;
+    uint64_t F0 = 0, F1 = 0, m0 = 0, m1 = 0;
+    if (act) {
+      F0 = a.yoff[c];
+      F1 = a.yoff[c + 1];
+      m0 = a.coff[c];
+      m1 = a.coff[c + 1];
+      for (int s = lane; s < CAP; s += G) { tkey[s] = kEmpty; tval[s] = 0; }
+      if (lane == 0) s_cnt[grp] = 0;
+    }
+    __syncthreads();
+    if (act) {
+      for (uint64_t f = F0 + lane; f < F1; f += G) {
+        uint64_t j = member_of(a.mscan, m0, m1, f);
+        uint32_t u = a.cedg[j];
+        uint64_t i = a.g.off[u] + (f - a.mscan[j]);
+        tab_add(tkey, nullptr, tval, CAP, a.vcom[a.g.keys[i]], (double)W[i], 0);
+      }
+    }
+    __syncthreads();
+    if (act) {
+      double sum = 0;
+      for (int s = lane; s < CAP; s += G) {
+        uint32_t k = tkey[s];
+        if (k == kEmpty) continue;
+        uint32_t at = atomicAdd(&s_cnt[grp], 1u);
+        a.zkeys[F0 + at] = k;
+        a.zw[F0 + at] = tval[s];
+        sum += tval[s];
+      }
+      if (sum != 0) unsafeAtomicAdd(&a.ztot[c], sum);
+    }
+    __syncthreads();
+    if (act && lane == 0) a.zdeg[c] = s_cnt[grp];
+  }
+}
+
+// Tier 4: communities with > 1024 member edges share one HBM table each,
+// filled by many blocks (CHUNK flat edges per block), emitted afterwards.
+constexpr uint64_t kAggChunk = 4096;
+
+template <class WT>
+__global__ __launch_bounds__(1024) void aggregate_hbm_kernel(AggArgs a) {
+  const WT* W = (const WT*)a.g.w;
+  for (uint64_t q = blockIdx.x; q < a.nchunks; q += gridDim.x) {
+    // tier-4 community k owning chunk q
+    uint64_t lo = 0, hi = a.n - 1;
+    while (lo < hi) {
+      uint64_t mid = (lo + hi + 1) >> 1;
+      if (a.choff[mid] <= q) lo = mid;
+      else hi = mid - 1;
+    }
+    uint64_t k = lo;
+    uint32_t c = a.list[k];
+    uint64_t F0 = a.yoff[c], F1 = a.yoff[c + 1];
+    uint64_t m0 = a.coff[c], m1 = a.coff[c + 1];
+    uint64_t f0 = F0 + (q - a.choff[k]) * kAggChunk;
+    uint64_t f1 = f0 + kAggChunk < F1 ? f0 + kAggChunk : F1;
+    uint64_t cap = a.hoff[k + 1] - a.hoff[k];
+    uint32_t* key = a.hkey + a.hoff[k];
+    double* val = a.hval + a.hoff[k];
+    for (uint64_t f = f0 + threadIdx.x; f < f1; f += blockDim.x) {
+      uint64_t j = member_of(a.mscan, m0, m1, f);
+      uint32_t u = a.cedg[j];
+      uint64_t i = a.g.off[u] + (f - a.mscan[j]);
+      tab_add(key, nullptr, val, cap, a.vcom[a.g.keys[i]], (double)W[i], 0);
+    }
+  }
+}
+
+__global__ void agg_emit_hbm_kernel(AggArgs a, uint64_t total) {
+  for (uint64_t s = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; s < total; s += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t key = a.hkey[s];
+    if (key == kEmpty) continue;
+    uint64_t lo = 0, hi = a.n - 1;
+    while (lo < hi) {
+      uint64_t mid = (lo + hi + 1) >> 1;
+      if (a.hoff[mid] <= s) lo = mid;
+      else hi = mid - 1;
+    }
+    uint32_t c = a.list[lo];
+    uint32_t at = atomicAdd(&a.zdeg[c], 1u);
+    uint64_t F0 = a.yoff[c];
+    a.zkeys[F0 + at] = key;
+    a.zw[F0 + at] = a.hval[s];
+    unsafeAtomicAdd(&a.ztot[c], a.hval[s]);
+  }
+}
+
+__global__ void t4_sizes_kernel(const uint64_t* yoff, const uint32_t* list, uint64_t n, uint64_t C, uint64_t chunk,
+                                uint64_t* caps, uint64_t* chunks) {
+  for (uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; k <= n; k += (uint64_t)gridDim.x * blockDim.x) {
+    if (k == n) { caps[k] = 0; chunks[k] = 0; continue; }
+    uint32_t c = list[k];
+    uint64_t td = yoff[c + 1] - yoff[c];
+    uint64_t m = td < C ? td : C;
+    uint64_t cap = 1;
+    while (cap < 2 * m) cap <<= 1;
+    caps[k] = cap;
+    chunks[k] = (td + chunk - 1) / chunk;
+  }
+}
+
+// ------------------------------------------------------------ tier lists
+// Partition ids 0..n-1 (live ones) into degree tiers, ascending within each
+// tier.  Pass 1 counts per block and tier (tier-major layout), an exclusive
+// scan of the counts gives each block's base, pass 2 scatters in order.
+__device__ __forceinline__ int dev_tier(uint64_t d) {
+  return d <= 8 ? 0 : d <= 32 ? 1 : d <= 128 ? 2 : d <= 1024 ? 3 : 4;
+}
+
+__global__ __launch_bounds__(kBlock) void tier_count_kernel(const uint64_t* off, const uint32_t* deg,
+                                                            const uint8_t* ex, uint64_t n, uint32_t* counts,
+                                                            uint32_t nblocks) {
+  __shared__ uint32_t s[kTiers];
+  if (threadIdx.x < kTiers) s[threadIdx.x] = 0;
+  __syncthreads();
+  uint64_t u = blockIdx.x * (uint64_t)kBlock + threadIdx.x;
+  if (u < n && (!ex || ex[u])) {
+    uint64_t d = deg ? deg[u] : off[u + 1] - off[u];
+    atomicAdd(&s[dev_tier(d)], 1u);
+  }
+  __syncthreads();
+  if (threadIdx.x < kTiers) counts[threadIdx.x * (uint64_t)nblocks + blockIdx.x] = s[threadIdx.x];
+}
+
+__global__ __launch_bounds__(kBlock) void tier_scatter_kernel(const uint64_t* off, const uint32_t* deg,
+                                                              const uint8_t* ex, uint64_t n, const uint32_t* base,
+                                                              uint32_t nblocks, uint32_t* ids) {
+  __shared__ uint32_t s_wave[kBlock / 64][kTiers];
+  uint64_t u = blockIdx.x * (uint64_t)kBlock + threadIdx.x;
+  int t = -1;
+  if (u < n && (!ex || ex[u])) {
+    uint64_t d = deg ? deg[u] : off[u + 1] - off[u];
+    t = dev_tier(d);
+  }
+  int lane = __lane_id(), w = threadIdx.x / 64;
+  uint32_t mine = 0;
+  for (int k = 0; k < kTiers; ++k) {
+    unsigned long long m = __ballot(t == k);
+    if (lane == 0) s_wave[w][k] = (uint32_t)__popcll(m);
+    if (t == k) mine = (uint32_t)__popcll(m & ((1ull << lane) - 1));
+  }
+  __syncthreads();
+  if (t >= 0) {
+    uint32_t before = 0;
+    for (int j = 0; j < w; ++j) before += s_wave[j][t];
+    ids[base[t * (uint64_t)nblocks + blockIdx.x] + before + mine] = (uint32_t)u;
+  }
+}
+
+// ------------------------------------------------------------ worklists
+// Split a queue into per-tier queues (tier t at tq + t * stride), by degree.
+__global__ __launch_bounds__(kBlock) void bin_queue_kernel(const uint32_t* q, const uint32_t* qlen,
+                                                           const uint32_t* deg, uint32_t* tq, uint64_t stride,
+                                                           uint32_t* tlen) {
+  const uint64_t n = *qlen;
+  for (uint64_t b = blockIdx.x * (uint64_t)kBlock; b < n; b += (uint64_t)gridDim.x * kBlock) {
+    uint64_t i = b + threadIdx.x;
+    int t = -1;
+    uint32_t u = 0;
+    if (i < n) {
+      u = q[i];
+      t = dev_tier(deg[u]);
+    }
+    int lane = __lane_id();
+    for (int k = 0; k < kTiers; ++k) {
+      unsigned long long m = __ballot(t == k);
+      if (!m) continue;
+      int leader = __ffsll((long long)m) - 1;
+      uint32_t base = 0;
+      if (lane == leader) base = atomicAdd(&tlen[k], (uint32_t)__popcll(m));
+      base = __shfl(base, leader);
+      if (t == k) tq[k * stride + base + __popcll(m & ((1ull << lane) - 1))] = u;
+    }
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void max_deg_kernel(const uint32_t* ids, uint64_t n, const uint32_t* deg,
+                                                         uint32_t* out) {
+  uint32_t m = 0;
+  for (uint64_t i = blockIdx.x * (uint64_t)kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock)
+    m = max(m, deg[ids[i]]);
+#pragma unroll
+  for (int k = 32; k > 0; k >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, k, 64));
+  if (__lane_id() == 0 && m) atomicMax(out, m);
+}
+
+__global__ __launch_bounds__(kBlock) void max_u32_kernel(const uint32_t* a, uint64_t n, uint32_t* out) {
+  uint32_t m = 0;
+  for (uint64_t i = blockIdx.x * (uint64_t)kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock)
+    m = max(m, a[i]);
+#pragma unroll
+  for (int k = 32; k > 0; k >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, k, 64));
+  if (__lane_id() == 0 && m) atomicMax(out, m);
+}
+
+// Queue every id whose flag is set (wave-aggregated appends).
+__global__ __launch_bounds__(kBlock) void compact_flags_kernel(const uint8_t* f, uint64_t n, uint32_t* q,
+                                                               uint32_t* qlen) {
+  for (uint64_t b = blockIdx.x * (uint64_t)kBlock; b < n; b += (uint64_t)gridDim.x * kBlock) {
+    uint64_t i = b + threadIdx.x;
+    bool on = i < n && f[i];
+    unsigned long long m = __ballot(on);
+    if (!m) continue;
+    int lane = __lane_id();
+    int leader = __ffsll((long long)m) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(qlen, (uint32_t)__popcll(m));
+    base = __shfl(base, leader);
+    if (on) q[base + __popcll(m & ((1ull << lane) - 1))] = (uint32_t)i;
+  }
+}
+
+// ------------------------------------------------------------ modularity
+// modularityByOmp (inc/properties.hxx:205-233)
+template <class WT>
+__global__ __launch_bounds__(kBlock) void modularity_parts_kernel(GView g, const uint32_t* memb, double* cin,
+                                                                  double* ctot) {
+  const WT* W = (const WT*)g.w;
+  int lane = __lane_id();
+  for (uint64_t base = (blockIdx.x * (uint64_t)kBlock) & ~63ull; base < g.S; base += (uint64_t)gridDim.x * kBlock) {
+    uint64_t u = base + (threadIdx.x & ~63u) + lane;
+    bool live = u < g.S && (!g.ex || g.ex[u]);
+    uint64_t o = live ? g.off[u] : 0;
+    uint32_t d = live ? g.deg[u] : 0;
+    uint32_t c = live ? memb[u] : 0;
+    bool big = live && d > 64;
+    if (live && !big) {
+      double si = 0, st = 0;
+      for (uint32_t i = 0; i < d; ++i) {
+        double w = (double)W[o + i];
+        if (memb[g.keys[o + i]] == c) si += w;
+        st += w;
+      }
+      if (si != 0) unsafeAtomicAdd(&cin[c], si);
+      if (st != 0) unsafeAtomicAdd(&ctot[c], st);
+    }
+    unsigned long long mask = __ballot(big);
+    while (mask) {
+      int j = __ffsll((long long)mask) - 1;
+      mask &= mask - 1;
+      uint64_t oj = __shfl(o, j);
+      uint32_t dj = __shfl(d, j);
+      uint32_t cj = __shfl(c, j);
+      double si = 0, st = 0;
+      for (uint32_t i = lane; i < dj; i += 64) {
+        double w = (double)W[oj + i];
+        if (memb[g.keys[oj + i]] == cj) si += w;
+        st += w;
+      }
+#pragma unroll
+      for (int m = 32; m > 0; m >>= 1) { si += __shfl_xor(si, m, 64); st += __shfl_xor(st, m, 64); }
+      if (lane == 0) {
+        if (si != 0) unsafeAtomicAdd(&cin[cj], si);
+        if (st != 0) unsafeAtomicAdd(&ctot[cj], st);
+      }
+    }
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void modularity_sum_kernel(const double* cin, const double* ctot, uint64_t n,
+                                                                double M, double R, double* out) {
+  double a = 0;
+  for (uint64_t c = blockIdx.x * (uint64_t)kBlock + threadIdx.x; c < n; c += (uint64_t)gridDim.x * kBlock) {
+    double x = ctot[c] / (2 * M);
+    a += cin[c] / (2 * M) - R * x * x;
+  }
+  a = block_sum<kBlock>(a);
+  if (threadIdx.x == 0) unsafeAtomicAdd(out, a);
+}
+
+#define LV_LAUNCH(kernel, grid, block, st, ...) hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), 0, st, __VA_ARGS__)
+
+}  // namespace
+
+// ------------------------------------------------------------ launchers
+void k_fill_u8(uint8_t* a, uint8_t v, uint64_t n, hipStream_t st) {
+  if (n) LV_LAUNCH(fill_kernel<uint8_t>, grid_for(n, kBlock * 4), kBlock, st, a, v, n);
+}
+void k_fill_u32(uint32_t* a, uint32_t v, uint64_t n, hipStream_t st) {
+  if (n) LV_LAUNCH(fill_kernel<uint32_t>, grid_for(n, kBlock * 4), kBlock, st, a, v, n);
+}
+void k_fill_f64(double* a, double v, uint64_t n, hipStream_t st) {
+  if (n) LV_LAUNCH(fill_kernel<double>, grid_for(n, kBlock * 4), kBlock, st, a, v, n);
+}
+void k_fill_u64(uint64_t* a, uint64_t v, uint64_t n, hipStream_t st) {
+  if (n) LV_LAUNCH(fill_kernel<uint64_t>, grid_for(n, kBlock * 4), kBlock, st, a, v, n);
+}
+void k_iota_u32(uint32_t* a, uint64_t n, hipStream_t st) {
+  if (n) LV_LAUNCH(iota_kernel, grid_for(n, kBlock * 4), kBlock, st, a, n);
+}
+void k_degrees_from_offsets(const uint64_t* off, uint32_t* deg, uint64_t S, hipStream_t st) {
+  if (S) LV_LAUNCH(degrees_kernel, grid_for(S, kBlock * 4), kBlock, st, off, deg, S);
+}
+void k_vertex_weights(GView g, bool f64, double* vtot, hipStream_t st) {
+  if (!g.S) return;
+  uint64_t grid = grid_for(g.S, kBlock, 1u << 16);
+  if (f64) LV_LAUNCH((row_sum_kernel<double, false>), grid, kBlock, st, g, vtot, nullptr);
+  else LV_LAUNCH((row_sum_kernel<float, false>), grid, kBlock, st, g, vtot, nullptr);
+}
+void k_edge_weight(GView g, bool f64, double* out, hipStream_t st) {
+  if (!g.S) return;
+  uint64_t grid = grid_for(g.S, kBlock, 1u << 14);
+  if (f64) LV_LAUNCH((row_sum_kernel<double, true>), grid, kBlock, st, g, nullptr, out);
+  else LV_LAUNCH((row_sum_kernel<float, true>), grid, kBlock, st, g, nullptr, out);
+}
+void k_initialize(GView g, uint32_t* vcom, double* ctot, const double* vtot, hipStream_t st) {
+  if (g.S) LV_LAUNCH(initialize_kernel, grid_for(g.S, kBlock * 4), kBlock, st, g, vcom, ctot, vtot);
+}
+void k_update_weights(const uint32_t* du, const float* dw, uint64_t nd, const uint32_t* iu, const float* iw,
+                      uint64_t ni, const uint32_t* vcom, double* vtot, double* ctot, hipStream_t st) {
+  if (nd + ni)
+    LV_LAUNCH(update_weights_kernel, grid_for(nd + ni, kBlock), kBlock, st, du, dw, nd, iu, iw, ni, vcom, vtot, ctot);
+}
+void k_frontier(const uint32_t* du, const uint32_t* dv, uint64_t nd, const uint32_t* iu, const uint32_t* iv,
+                uint64_t ni, const uint32_t* vcom, uint8_t* vaff, uint32_t* q, uint32_t* qlen, hipStream_t st) {
+  if (nd + ni)
+    LV_LAUNCH(frontier_kernel, grid_for(nd + ni, kBlock), kBlock, st, du, dv, nd, iu, iv, ni, vcom, vaff, q, qlen);
+}
+void k_ds_deletions(const uint32_t* du, const uint32_t* dv, uint64_t nd, const uint32_t* vcom, uint8_t* vertices,
+                    uint8_t* neighbors, uint8_t* communities, hipStream_t st) {
+  if (nd)
+    LV_LAUNCH(ds_deletions_kernel, grid_for(nd, kBlock), kBlock, st, du, dv, nd, vcom, vertices, neighbors,
+              communities);
+}
+void k_ds_insertions(const uint32_t* iu, const uint32_t* iv, const float* iw, uint64_t ni, const uint32_t* vcom,
+                     const double* vtot, const double* ctot, double M, double R, uint32_t* tkey, double* tval,
+                     uint32_t* tlist, uint8_t* vertices, uint8_t* neighbors, uint8_t* communities, hipStream_t st) {
+  if (ni)
+    LV_LAUNCH(ds_insertions_kernel, grid_for(ni, kBlock), kBlock, st, iu, iv, iw, ni, vcom, vtot, ctot, M, R, tkey,
+              tval, tlist, vertices, neighbors, communities);
+}
+void k_ds_sweep(GView g, const uint32_t* vcom, const uint8_t* neighbors, const uint8_t* communities,
+                uint8_t* vertices, hipStream_t st) {
+  if (g.S) LV_LAUNCH(ds_sweep_kernel, grid_for(g.S, kBlock, 1u << 16), kBlock, st, g, vcom, neighbors, communities, vertices);
+}
+void k_count_u8(const uint8_t* a, uint8_t v, uint64_t n, unsigned long long* out, hipStream_t st) {
+  if (n) LV_LAUNCH(count_u8_kernel, grid_for(n, kBlock * 8, 4096), kBlock, st, a, v, n, out);
+}
+void k_check_sorted(const uint32_t* u, const uint32_t* v, uint64_t n, uint32_t S, int* bad, hipStream_t st) {
+  if (n) LV_LAUNCH(check_sorted_kernel, grid_for(n, kBlock), kBlock, st, u, v, n, S, bad);
+}
+
+template <class WT>
+static void move_tier(const MoveArgs& a, int tier, hipStream_t st) {
+  // Grid: enough blocks for the list, capped; kernels loop grid-stride.  When
+  // the count lives on the device (worklists) the grid is fixed.
+  uint64_t n = a.n_dev ? (uint64_t)a.n : a.n;
+  switch (tier) {
+    case 0: LV_LAUNCH((move_kernel<256, 8, 16, WT>), grid_for(n, 32, 1u << 16), 256, st, a); break;
+    case 1: LV_LAUNCH((move_kernel<256, 16, 64, WT>), grid_for(n, 16, 1u << 16), 256, st, a); break;
+    case 2: LV_LAUNCH((move_kernel<256, 64, 256, WT>), grid_for(n, 4, 1u << 16), 256, st, a); break;
+    case 3: LV_LAUNCH((move_kernel<256, 256, 2048, WT>), grid_for(n, 1, 1u << 14), 256, st, a); break;
+    default: LV_LAUNCH((move_kernel<1024, 1024, 0, WT>), grid_for(n, 1, 256), 1024, st, a); break;
+  }
+}
+void k_move(const MoveArgs& a, int tier, bool f64, hipStream_t st) {
+  if (f64) move_tier<double>(a, tier, st);
+  else move_tier<float>(a, tier, st);
+}
+void k_exists(GView g, const uint32_t* vcom, uint32_t* flags, hipStream_t st) {
+  if (g.S) LV_LAUNCH(exists_kernel, grid_for(g.S, kBlock * 4), kBlock, st, g, vcom, flags);
+}
+void k_lookup(uint32_t* a, uint64_t n, const uint32_t* map, hipStream_t st) {
+  if (n) LV_LAUNCH(lookup_kernel, grid_for(n, kBlock * 4), kBlock, st, a, n, map);
+}
+void k_count_members(GView g, const uint32_t* vcom, uint32_t* cnt, hipStream_t st) {
+  if (g.S) LV_LAUNCH(count_members_kernel, grid_for(g.S, kBlock * 4), kBlock, st, g, vcom, cnt);
+}
+void k_scatter_members(GView g, const uint32_t* vcom, const uint32_t* coff, uint32_t* cdeg, uint32_t* cedg,
+                       hipStream_t st) {
+  if (g.S) LV_LAUNCH(scatter_members_kernel, grid_for(g.S, kBlock * 4), kBlock, st, g, vcom, coff, cdeg, cedg);
+}
+void k_member_degrees(const uint32_t* cedg, uint64_t n, GView g, uint64_t* mdeg, hipStream_t st) {
+  LV_LAUNCH(member_degrees_kernel, grid_for(n + 1, kBlock * 4), kBlock, st, cedg, n, g, mdeg);
+}
+void k_gather_yoff(const uint32_t* coff, const uint64_t* mscan, uint64_t C, uint64_t* yoff, hipStream_t st) {
+  LV_LAUNCH(gather_yoff_kernel, grid_for(C + 1, kBlock * 4), kBlock, st, coff, mscan, C, yoff);
+}
+void k_tier_count(const uint64_t* off, const uint32_t* deg, const uint8_t* ex, uint64_t n, int, uint64_t,
+                  uint32_t* block_counts, uint32_t nblocks, hipStream_t st) {
+  LV_LAUNCH(tier_count_kernel, nblocks, kBlock, st, off, deg, ex, n, block_counts, nblocks);
+}
+void k_tier_scatter(const uint64_t* off, const uint32_t* deg, const uint8_t* ex, uint64_t n, int, uint64_t,
+                    const uint32_t* block_base, uint32_t nblocks, uint32_t* ids, hipStream_t st) {
+  LV_LAUNCH(tier_scatter_kernel, nblocks, kBlock, st, off, deg, ex, n, block_base, nblocks, ids);
+}
+
+template <class WT>
+static void agg_tier(const AggArgs& a, int tier, hipStream_t st) {
+  switch (tier) {
+    case 0: LV_LAUNCH((aggregate_kernel<256, 8, 16, WT>), grid_for(a.n, 32, 1u << 16), 256, st, a); break;
+    case 1: LV_LAUNCH((aggregate_kernel<256, 16, 64, WT>), grid_for(a.n, 16, 1u << 16), 256, st, a); break;
+    case 2: LV_LAUNCH((aggregate_kernel<256, 64, 256, WT>), grid_for(a.n, 4, 1u << 16), 256, st, a); break;
+    case 3: LV_LAUNCH((aggregate_kernel<256, 256, 2048, WT>), grid_for(a.n, 1, 1u << 14), 256, st, a); break;
+    default: LV_LAUNCH((aggregate_hbm_kernel<WT>), grid_for(a.nchunks, 1, 1u << 14), 1024, st, a); break;
+  }
+}
+void k_aggregate(const AggArgs& a, int tier, bool f64, hipStream_t st) {
+  if (!a.n) return;
+  if (f64) agg_tier<double>(a, tier, st);
+  else agg_tier<float>(a, tier, st);
+}
+void k_agg_emit_hbm(const AggArgs& a, uint64_t total, hipStream_t st) {
+  if (total) LV_LAUNCH(agg_emit_hbm_kernel, grid_for(total, kBlock, 1u << 16), kBlock, st, a, total);
+}
+void k_t4_sizes(const uint64_t* yoff, const uint32_t* list, uint64_t n, uint64_t C, uint64_t chunk, uint64_t* caps,
+                uint64_t* chunks, hipStream_t st) {
+  LV_LAUNCH(t4_sizes_kernel, grid_for(n + 1, kBlock), kBlock, st, yoff, list, n, C, chunk, caps, chunks);
+}
+void k_bin_queue(const uint32_t* q, const uint32_t* qlen, uint64_t qcap, const uint32_t* deg, uint32_t* tq,
+                 uint64_t stride, uint32_t* tlen, hipStream_t st) {
+  LV_LAUNCH(bin_queue_kernel, grid_for(qcap, kBlock, 2048), kBlock, st, q, qlen, deg, tq, stride, tlen);
+}
+void k_max_deg(const uint32_t* ids, uint64_t n, const uint32_t* deg, uint32_t* out, hipStream_t st) {
+  if (n) LV_LAUNCH(max_deg_kernel, grid_for(n, kBlock, 1024), kBlock, st, ids, n, deg, out);
+}
+void k_max_u32(const uint32_t* a, uint64_t n, uint32_t* out, hipStream_t st) {
+  if (n) LV_LAUNCH(max_u32_kernel, grid_for(n, kBlock * 8, 4096), kBlock, st, a, n, out);
+}
+void k_compact_flags(const uint8_t* f, uint64_t n, uint32_t* q, uint32_t* qlen, hipStream_t st) {
+  if (n) LV_LAUNCH(compact_flags_kernel, grid_for(n, kBlock, 1u << 16), kBlock, st, f, n, q, qlen);
+}
+void k_modularity_parts(GView g, bool f64, const uint32_t* memb, double* cin, double* ctot, hipStream_t st) {
+  if (!g.S) return;
+  uint64_t grid = grid_for(g.S, kBlock, 1u << 16);
+  if (f64) LV_LAUNCH(modularity_parts_kernel<double>, grid, kBlock, st, g, memb, cin, ctot);
+  else LV_LAUNCH(modularity_parts_kernel<float>, grid, kBlock, st, g, memb, cin, ctot);
+}
+void k_modularity_sum(const double* cin, const double* ctot, uint64_t n, double M, double R, double* out,
+                      hipStream_t st) {
+  if (n) LV_LAUNCH(modularity_sum_kernel, grid_for(n, kBlock * 8, 4096), kBlock, st, cin, ctot, n, M, R, out);
+}
+
+hipError_t scan_u32(const uint32_t* in, uint32_t* out, uint64_t n, void* tmp, size_t& tmp_bytes, hipStream_t st) {
+  return hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, in, out, (int)n, st);
+}
+hipError_t scan_u64(const uint64_t* in, uint64_t* out, uint64_t n, void* tmp, size_t& tmp_bytes, hipStream_t st) {
+  return hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, in, out, (int)n, st);
+}
+
+}  // namespace lv

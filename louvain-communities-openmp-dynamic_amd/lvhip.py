@@ -1,0 +1,433 @@
+"""Python binding of liblouvain_hip.so (the MI355X Louvain library).
+
+Mirrors the reference's interface (puzzlef/louvain-communities-openmp-dynamic):
+
+    LouvainOptions                      inc/louvain.hxx:30-63
+    LouvainResult                       inc/louvain.hxx:77-145
+    louvainStaticHip(x, o)              <- louvainStaticOmp                inc/louvain.hxx:1285
+    louvainNaiveDynamicHip(y, d, i, q, qvtot, qctot, o)                    inc/louvain.hxx:1349
+    louvainDynamicDeltaScreeningHip(...)                                   inc/louvain.hxx:1542
+    louvainDynamicFrontierHip(...)                                         inc/louvain.hxx:1674
+
+A graph is a dict of numpy arrays (``off`` u64[S+1], ``keys`` u32, ``w`` f32
+or f64, optional ``deg`` u32 for a gapped CSR, optional ``ex`` u8 exists
+mask); rows sorted by neighbour id.  A batch is a dict of ``u``, ``v`` (u32)
+and ``w`` (f32) arrays, tidy (sorted by (u, v), unique, both directions).
+
+There is no CPU fallback: importing this module on a machine without the
+built library raises, and every call without a HIP device raises.
+"""
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liblouvain_hip.so")
+
+LV_STATIC, LV_NAIVE_DYNAMIC, LV_DELTA_SCREENING, LV_DYNAMIC_FRONTIER = 0, 1, 2, 3
+APPROACHES = {"static": LV_STATIC, "naive-dynamic": LV_NAIVE_DYNAMIC,
+              "delta-screening": LV_DELTA_SCREENING, "dynamic-frontier": LV_DYNAMIC_FRONTIER}
+PATH_SWEEP, PATH_WORKLIST = 0, 1
+
+# Every symbol include/louvain_hip.h declares.
+EXPORTS = [
+    "lv_options_default", "lv_status_string", "lv_context_create", "lv_context_destroy", "lv_last_error",
+    "lv_context_set_path", "lv_louvain_static", "lv_louvain_naive_dynamic", "lv_louvain_delta_screening",
+    "lv_louvain_dynamic_frontier", "lv_dgraph_create", "lv_dgraph_destroy", "lv_dgraph_span", "lv_dgraph_edges",
+    "lv_dbatch_create", "lv_dbatch_destroy", "lv_dstate_create", "lv_dstate_destroy", "lv_dstate_upload",
+    "lv_dstate_download", "lv_dstate_copy", "lv_run", "lv_dmodularity", "lv_edge_weight", "lv_modularity",
+    "lv_vertex_weights", "lv_update_weights", "lv_affected_frontier", "lv_affected_delta_screening",
+    "lv_choose_community", "lv_renumber", "lv_aggregate",
+]
+
+
+class LvError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"{msg} (status {status})")
+        self.status = status
+
+
+class _Options(C.Structure):
+    _fields_ = [("repeat", C.c_int32), ("resolution", C.c_double), ("tolerance", C.c_double),
+                ("aggregation_tolerance", C.c_double), ("tolerance_drop", C.c_double),
+                ("max_iterations", C.c_int32), ("max_passes", C.c_int32)]
+
+
+class _Graph(C.Structure):
+    _fields_ = [("span", C.c_uint64), ("offsets", C.c_void_p), ("degrees", C.c_void_p), ("keys", C.c_void_p),
+                ("weights", C.c_void_p), ("weights_f64", C.c_void_p), ("exists", C.c_void_p)]
+
+
+class _Batch(C.Structure):
+    _fields_ = [("u", C.c_void_p), ("v", C.c_void_p), ("w", C.c_void_p), ("size", C.c_uint64)]
+
+
+class _Result(C.Structure):
+    _fields_ = [("membership", C.c_void_p), ("vertex_weight", C.c_void_p), ("community_weight", C.c_void_p),
+                ("iterations", C.c_int32), ("passes", C.c_int32), ("time", C.c_float),
+                ("marking_time", C.c_float), ("initialization_time", C.c_float),
+                ("first_pass_time", C.c_float), ("local_move_time", C.c_float),
+                ("aggregation_time", C.c_float), ("affected_vertices", C.c_uint64)]
+
+
+def load_library(path=LIB_PATH):
+    if not os.path.exists(path):
+        raise ImportError(f"{path} is not built: run `python __graft_entry__.py build` (hipcc, gfx950)")
+    lib = C.CDLL(path)
+    lib.lv_status_string.restype = C.c_char_p
+    lib.lv_last_error.restype = C.c_char_p
+    lib.lv_dgraph_span.restype = C.c_uint64
+    lib.lv_dgraph_edges.restype = C.c_uint64
+    for name in ("lv_context_destroy", "lv_dgraph_destroy", "lv_dbatch_destroy", "lv_dstate_destroy"):
+        getattr(lib, name).restype = None
+    return lib
+
+
+_LIB = load_library()
+
+
+@dataclass
+class LouvainOptions:
+    """LouvainOptions (inc/louvain.hxx:30-63); defaults from inc/louvain.hxx:60."""
+    repeat: int = 1
+    resolution: float = 1.0
+    tolerance: float = 1e-2
+    aggregationTolerance: float = 1.0
+    toleranceDrop: float = 10.0
+    maxIterations: int = 20
+    maxPasses: int = 10
+
+    def _c(self):
+        return _Options(self.repeat, self.resolution, self.tolerance, self.aggregationTolerance,
+                        self.toleranceDrop, self.maxIterations, self.maxPasses)
+
+
+@dataclass
+class LouvainResult:
+    """LouvainResult (inc/louvain.hxx:77-145)."""
+    membership: np.ndarray = None
+    vertexWeight: np.ndarray = None
+    communityWeight: np.ndarray = None
+    iterations: int = 0
+    passes: int = 0
+    time: float = 0.0
+    markingTime: float = 0.0
+    initializationTime: float = 0.0
+    firstPassTime: float = 0.0
+    localMoveTime: float = 0.0
+    aggregationTime: float = 0.0
+    affectedVertices: int = 0
+    counters: dict = field(default_factory=dict)
+
+
+def _p(a):
+    return None if a is None else C.c_void_p(a.ctypes.data)
+
+
+def _arr(a, dt):
+    return None if a is None else np.ascontiguousarray(a, dtype=dt)
+
+
+class _GraphArgs:
+    """Keeps the typed numpy arrays alive while the C struct points at them."""
+
+    def __init__(self, g):
+        w = np.asarray(g["w"])
+        self.off = _arr(g["off"], np.uint64)
+        self.keys = _arr(g["keys"], np.uint32)
+        self.deg = _arr(g.get("deg"), np.uint32)
+        self.ex = _arr(g.get("ex"), np.uint8)
+        self.f64 = w.dtype == np.float64
+        self.w = _arr(w, np.float64 if self.f64 else np.float32)
+        self.S = len(self.off) - 1
+        self.c = _Graph(self.S, _p(self.off), _p(self.deg), _p(self.keys),
+                        None if self.f64 else _p(self.w), _p(self.w) if self.f64 else None, _p(self.ex))
+
+
+class _BatchArgs:
+    def __init__(self, b):
+        if b is None:
+            b = {"u": np.zeros(0, np.uint32), "v": np.zeros(0, np.uint32), "w": np.zeros(0, np.float32)}
+        self.u = _arr(b["u"], np.uint32)
+        self.v = _arr(b["v"], np.uint32)
+        self.w = _arr(b["w"], np.float32)
+        self.c = _Batch(_p(self.u), _p(self.v), _p(self.w), len(self.u))
+
+
+class Context:
+    """One HIP device + stream + grow-only device workspace (lv_context)."""
+
+    def __init__(self, device=0, path=PATH_WORKLIST):
+        self.lib = _LIB
+        self.h = C.c_void_p()
+        rc = self.lib.lv_context_create(int(device), C.byref(self.h))
+        if rc != 0:
+            raise LvError(rc, "lv_context_create: " + self.lib.lv_status_string(rc).decode())
+        self.set_path(path)
+
+    def close(self):
+        if self.h:
+            self.lib.lv_context_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_path(self, path):
+        self._check(self.lib.lv_context_set_path(self.h, int(path)), "lv_context_set_path")
+
+    def _check(self, rc, what):
+        if rc != 0:
+            detail = self.lib.lv_last_error(self.h).decode() if self.h else ""
+            raise LvError(rc, f"{what}: {self.lib.lv_status_string(rc).decode()}: {detail}")
+
+    # -- the four entry points ------------------------------------------------
+    def louvain(self, approach, g, deletions=None, insertions=None, q=None, qvtot=None, qctot=None,
+                o=None):
+        if isinstance(approach, str):
+            approach = APPROACHES[approach]
+        o = o or LouvainOptions()
+        ga = _GraphArgs(g)
+        S = ga.S
+        memb = np.zeros(S, np.uint32)
+        vt = np.zeros(S, np.float64)
+        ct = np.zeros(S, np.float64)
+        r = _Result(_p(memb), _p(vt), _p(ct))
+        oc = o._c()
+        if approach == LV_STATIC:
+            rc = self.lib.lv_louvain_static(self.h, C.byref(ga.c), C.byref(oc), C.byref(r))
+        else:
+            d, i = _BatchArgs(deletions), _BatchArgs(insertions)
+            qq, qv, qc = _arr(q, np.uint32), _arr(qvtot, np.float64), _arr(qctot, np.float64)
+            fn = {LV_NAIVE_DYNAMIC: self.lib.lv_louvain_naive_dynamic,
+                  LV_DELTA_SCREENING: self.lib.lv_louvain_delta_screening,
+                  LV_DYNAMIC_FRONTIER: self.lib.lv_louvain_dynamic_frontier}[approach]
+            rc = fn(self.h, C.byref(ga.c), C.byref(d.c), C.byref(i.c), _p(qq), _p(qv), _p(qc), C.byref(oc),
+                    C.byref(r))
+        self._check(rc, "louvain")
+        return _to_result(r, memb, vt, ct)
+
+    # -- phase entry points ---------------------------------------------------
+    def edge_weight(self, g):
+        ga = _GraphArgs(g)
+        out = C.c_double()
+        self._check(self.lib.lv_edge_weight(self.h, C.byref(ga.c), C.byref(out)), "lv_edge_weight")
+        return out.value
+
+    def modularity(self, g, membership, resolution=1.0):
+        ga = _GraphArgs(g)
+        m = _arr(membership, np.uint32)
+        out = C.c_double()
+        self._check(self.lib.lv_modularity(self.h, C.byref(ga.c), _p(m), C.c_double(resolution), C.byref(out)),
+                    "lv_modularity")
+        return out.value
+
+    def vertex_weights(self, g):
+        ga = _GraphArgs(g)
+        out = np.zeros(ga.S, np.float64)
+        self._check(self.lib.lv_vertex_weights(self.h, C.byref(ga.c), _p(out)), "lv_vertex_weights")
+        return out
+
+    def update_weights(self, span, deletions, insertions, vcom, vtot, ctot):
+        d, i = _BatchArgs(deletions), _BatchArgs(insertions)
+        vc = _arr(vcom, np.uint32)
+        vt = np.array(vtot, np.float64)
+        ct = np.array(ctot, np.float64)
+        self._check(self.lib.lv_update_weights(self.h, C.c_uint64(span), C.byref(d.c), C.byref(i.c), _p(vc),
+                                               _p(vt), _p(ct)), "lv_update_weights")
+        return vt, ct
+
+    def affected_frontier(self, span, deletions, insertions, vcom):
+        d, i = _BatchArgs(deletions), _BatchArgs(insertions)
+        vc = _arr(vcom, np.uint32)
+        out = np.zeros(span, np.uint8)
+        self._check(self.lib.lv_affected_frontier(self.h, C.c_uint64(span), C.byref(d.c), C.byref(i.c), _p(vc),
+                                                  _p(out)), "lv_affected_frontier")
+        return out
+
+    def affected_delta_screening(self, g, deletions, insertions, vcom, vtot, ctot, M, resolution=1.0):
+        ga = _GraphArgs(g)
+        d, i = _BatchArgs(deletions), _BatchArgs(insertions)
+        vc, vt, ct = _arr(vcom, np.uint32), _arr(vtot, np.float64), _arr(ctot, np.float64)
+        a, b, c = (np.zeros(ga.S, np.uint8) for _ in range(3))
+        self._check(self.lib.lv_affected_delta_screening(self.h, C.byref(ga.c), C.byref(d.c), C.byref(i.c), _p(vc),
+                                                         _p(vt), _p(ct), C.c_double(M), C.c_double(resolution),
+                                                         _p(a), _p(b), _p(c)), "lv_affected_delta_screening")
+        return a, b, c
+
+    def choose_community(self, g, vcom, vtot, ctot, M, resolution=1.0):
+        ga = _GraphArgs(g)
+        vc, vt, ct = _arr(vcom, np.uint32), _arr(vtot, np.float64), _arr(ctot, np.float64)
+        bc = np.zeros(ga.S, np.uint32)
+        be = np.zeros(ga.S, np.float64)
+        self._check(self.lib.lv_choose_community(self.h, C.byref(ga.c), _p(vc), _p(vt), _p(ct), C.c_double(M),
+                                                 C.c_double(resolution), _p(bc), _p(be)), "lv_choose_community")
+        return bc, be
+
+    def renumber(self, g, vcom):
+        ga = _GraphArgs(g)
+        vc = np.array(vcom, np.uint32)
+        cext = np.zeros(ga.S, np.uint32)
+        n = C.c_uint64()
+        self._check(self.lib.lv_renumber(self.h, C.byref(ga.c), _p(vc), C.c_uint64(len(vc)), _p(cext),
+                                         C.byref(n)), "lv_renumber")
+        return vc, cext, n.value
+
+    def aggregate(self, g, vcom, C_):
+        ga = _GraphArgs(g)
+        E = int(ga.off[-1])
+        vc = _arr(vcom, np.uint32)
+        zoff = np.zeros(C_ + 1, np.uint64)
+        zdeg = np.zeros(max(C_, 1), np.uint32)
+        zkeys = np.zeros(max(E, 1), np.uint32)
+        zw = np.zeros(max(E, 1), np.float64)
+        self._check(self.lib.lv_aggregate(self.h, C.byref(ga.c), _p(vc), C.c_uint64(C_), _p(zoff), _p(zdeg),
+                                          _p(zkeys), _p(zw)), "lv_aggregate")
+        return {"off": zoff, "deg": zdeg[:C_], "keys": zkeys[:E], "w": zw[:E]}
+
+    # -- device-resident form -------------------------------------------------
+    def device_graph(self, g):
+        return DeviceGraph(self, g)
+
+    def device_batch(self, deletions, insertions):
+        return DeviceBatch(self, deletions, insertions)
+
+    def device_state(self, span):
+        return DeviceState(self, span)
+
+    def run(self, approach, dgraph, dbatch, dstate, o=None, copy_back=False):
+        """lv_run on device-resident inputs; the state is updated in place."""
+        if isinstance(approach, str):
+            approach = APPROACHES[approach]
+        o = o or LouvainOptions()
+        S = dgraph.span
+        memb = vt = ct = None
+        if copy_back:
+            memb, vt, ct = np.zeros(S, np.uint32), np.zeros(S, np.float64), np.zeros(S, np.float64)
+        r = _Result(_p(memb), _p(vt), _p(ct))
+        oc = o._c()
+        self._check(self.lib.lv_run(self.h, int(approach), dgraph.h, dbatch.h if dbatch else None, dstate.h,
+                                    C.byref(oc), C.byref(r)), "lv_run")
+        return _to_result(r, memb, vt, ct)
+
+    def dmodularity(self, dgraph, dstate, resolution=1.0):
+        out = C.c_double()
+        self._check(self.lib.lv_dmodularity(self.h, dgraph.h, dstate.h, C.c_double(resolution), C.byref(out)),
+                    "lv_dmodularity")
+        return out.value
+
+
+class DeviceGraph:
+    def __init__(self, ctx, g):
+        self.ctx = ctx
+        self.h = C.c_void_p()
+        ga = _GraphArgs(g)
+        ctx._check(ctx.lib.lv_dgraph_create(ctx.h, C.byref(ga.c), C.byref(self.h)), "lv_dgraph_create")
+        self.span = ctx.lib.lv_dgraph_span(self.h)
+        self.edges = ctx.lib.lv_dgraph_edges(self.h)
+
+    def close(self):
+        if self.h:
+            self.ctx.lib.lv_dgraph_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DeviceBatch:
+    def __init__(self, ctx, deletions, insertions):
+        self.ctx = ctx
+        self.h = C.c_void_p()
+        d, i = _BatchArgs(deletions), _BatchArgs(insertions)
+        self.size = len(d.u) + len(i.u)
+        ctx._check(ctx.lib.lv_dbatch_create(ctx.h, C.byref(d.c), C.byref(i.c), C.byref(self.h)), "lv_dbatch_create")
+
+    def close(self):
+        if self.h:
+            self.ctx.lib.lv_dbatch_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DeviceState:
+    def __init__(self, ctx, span):
+        self.ctx = ctx
+        self.span = span
+        self.h = C.c_void_p()
+        ctx._check(ctx.lib.lv_dstate_create(ctx.h, C.c_uint64(span), C.byref(self.h)), "lv_dstate_create")
+
+    def upload(self, q, qvtot, qctot):
+        qq, qv, qc = _arr(q, np.uint32), _arr(qvtot, np.float64), _arr(qctot, np.float64)
+        self.ctx._check(self.ctx.lib.lv_dstate_upload(self.ctx.h, self.h, _p(qq), _p(qv), _p(qc)), "lv_dstate_upload")
+
+    def download(self):
+        q = np.zeros(self.span, np.uint32)
+        v = np.zeros(self.span, np.float64)
+        c = np.zeros(self.span, np.float64)
+        self.ctx._check(self.ctx.lib.lv_dstate_download(self.ctx.h, self.h, _p(q), _p(v), _p(c)),
+                        "lv_dstate_download")
+        return q, v, c
+
+    def copy_from(self, other):
+        self.ctx._check(self.ctx.lib.lv_dstate_copy(self.ctx.h, self.h, other.h), "lv_dstate_copy")
+
+    def close(self):
+        if self.h:
+            self.ctx.lib.lv_dstate_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _to_result(r, memb, vt, ct):
+    return LouvainResult(memb, vt, ct, r.iterations, r.passes, r.time, r.marking_time, r.initialization_time,
+                         r.first_pass_time, r.local_move_time, r.aggregation_time, r.affected_vertices)
+
+
+_DEFAULT_CTX = None
+
+
+def default_context():
+    global _DEFAULT_CTX
+    if _DEFAULT_CTX is None:
+        _DEFAULT_CTX = Context(0)
+    return _DEFAULT_CTX
+
+
+# ---- the reference's four entry points, by name -------------------------------
+def louvainStaticHip(x, o=None):
+    """louvainStaticOmp (inc/louvain.hxx:1285-1296) on the GPU."""
+    return default_context().louvain(LV_STATIC, x, o=o)
+
+
+def louvainNaiveDynamicHip(y, deletions, insertions, q, qvtot, qctot, o=None):
+    """louvainNaiveDynamicOmp (inc/louvain.hxx:1349-1366) on the GPU."""
+    return default_context().louvain(LV_NAIVE_DYNAMIC, y, deletions, insertions, q, qvtot, qctot, o)
+
+
+def louvainDynamicDeltaScreeningHip(y, deletions, insertions, q, qvtot, qctot, o=None):
+    """louvainDynamicDeltaScreeningOmp (inc/louvain.hxx:1542-1565) on the GPU."""
+    return default_context().louvain(LV_DELTA_SCREENING, y, deletions, insertions, q, qvtot, qctot, o)
+
+
+def louvainDynamicFrontierHip(y, deletions, insertions, q, qvtot, qctot, o=None):
+    """louvainDynamicFrontierOmp (inc/louvain.hxx:1674-1690) on the GPU."""
+    return default_context().louvain(LV_DYNAMIC_FRONTIER, y, deletions, insertions, q, qvtot, qctot, o)

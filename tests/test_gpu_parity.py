@@ -1,0 +1,224 @@
+"""HIP path vs the oracle, through the C ABI (liblouvain_hip.so).
+
+Bar: integer / flag / index work bit-exact; K and Sigma_tot within 1e-6
+relative (exact for integer weights); modularity of a final partition within
+1e-3 of the oracle's on the same inputs (the reference itself is only
+reproducible at one thread, so trajectories are compared through modularity).
+"""
+import numpy as np
+import pytest
+
+import fixtures as F
+from conftest import FIXTURES
+
+pytestmark = pytest.mark.gpu
+
+Q_TOL = 1e-3
+INTEGER_FIXTURES = [n for n in FIXTURES if n != "ppw"]
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    import lvhip
+    c = lvhip.Context(0)
+    yield c
+    c.close()
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+def _g(f):
+    return F.graph(f, "x"), F.graph(f, "y"), F.batch(f, "del"), F.batch(f, "ins")
+
+
+def _exact(f):
+    return f is not None and True
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_update_weights_and_frontier(ctx, name):
+    _say("case", name)
+    f = F.load(name)
+    x, y, d, i = _g(f)
+    S = len(y["off"]) - 1
+    vt, ct = ctx.update_weights(S, d, i, f["static_memb"], f["static_vtot"], f["static_ctot"])
+    if name in INTEGER_FIXTURES:
+        assert np.array_equal(vt, f["upd_vtot"]) and np.array_equal(ct, f["upd_ctot"])
+    else:
+        assert np.allclose(vt, f["upd_vtot"], rtol=1e-6, atol=1e-9)
+        assert np.allclose(ct, f["upd_ctot"], rtol=1e-6, atol=1e-9)
+    assert np.array_equal(ctx.affected_frontier(S, d, i, f["static_memb"]), f["frontier"])
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_delta_screening_flags(ctx, name):
+    _say("case", name)
+    f = F.load(name)
+    x, y, d, i = _g(f)
+    M = float(f["y_edge_weight"]) / 2
+    R = float(f["opts"][1])
+    a, b, c = ctx.affected_delta_screening(y, d, i, f["static_memb"], f["upd_vtot"], f["upd_ctot"], M, R)
+    assert np.array_equal(a, f["ds_vertices"])
+    assert np.array_equal(b, f["ds_neighbors"])
+    assert np.array_equal(c, f["ds_communities"])
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_choose_community_bit_exact(ctx, name):
+    """The local-moving kernel's decision on a fixed snapshot: the same
+    community and the same delta-Q bits as the reference."""
+    _say("case", name)
+    f = F.load(name)
+    x, y, d, i = _g(f)
+    R = float(f["opts"][1])
+    M = float(f["y_edge_weight"]) / 2
+    bc, be = ctx.choose_community(y, f["static_memb"], f["upd_vtot"], f["upd_ctot"], M, R)
+    if name in INTEGER_FIXTURES:
+        assert np.array_equal(bc, f["choose_y_c"]) and np.array_equal(be, f["choose_y_e"])
+    else:
+        assert np.allclose(be, f["choose_y_e"], rtol=1e-9, atol=1e-15)
+        assert (bc == f["choose_y_c"]).mean() > 0.99
+    Mx = float(f["x_edge_weight"]) / 2
+    S = len(x["off"]) - 1
+    k = ctx.vertex_weights(x)
+    bc, be = ctx.choose_community(x, np.arange(S, dtype=np.uint32), k, k, Mx, R)
+    if name in INTEGER_FIXTURES:
+        assert np.array_equal(bc, f["choose_x_c"]) and np.array_equal(be, f["choose_x_e"])
+    sg = F.super_graph(f)
+    CN = int(f["renum_CN"])
+    Msg = f["sg_K"].sum() / 2
+    bc, be = ctx.choose_community(sg, np.arange(CN, dtype=np.uint32), f["sg_K"], f["sg_K"], Msg, R)
+    if name in INTEGER_FIXTURES:
+        assert np.array_equal(bc, f["sg_choose_c"]) and np.array_equal(be, f["sg_choose_e"])
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_renumber_and_aggregate(ctx, name):
+    _say("case", name)
+    f = F.load(name)
+    x = F.graph(f, "x")
+    vc, cext, CN = ctx.renumber(x, f["static_memb"])
+    assert CN == int(f["renum_CN"])
+    assert np.array_equal(vc, f["renum_vcom"]) and np.array_equal(cext, f["renum_cext"])
+    a = ctx.aggregate(x, vc, CN)
+    assert np.array_equal(a["off"], f["agg_off"]) and np.array_equal(a["deg"], f["agg_deg"])
+    got = F.rows_as_maps(a["off"], a["deg"], a["keys"], a["w"])
+    want = F.rows_as_maps(f["agg_off"], f["agg_deg"], f["agg_keys"], f["agg_w"])
+    if name in INTEGER_FIXTURES:
+        assert got == want
+    else:
+        for g_, w_ in zip(got, want):
+            assert g_.keys() == w_.keys()
+            assert np.allclose([g_[k] for k in w_], [w_[k] for k in w_], rtol=1e-12)
+    # second level: f64 super-graph input
+    sg = F.super_graph(f)
+    vc2, _, CN2 = ctx.renumber(sg, f["sg_memb"])
+    assert CN2 == int(f["sg_renum_CN"]) and np.array_equal(vc2, f["sg_renum_vcom"])
+    a2 = ctx.aggregate(sg, vc2, CN2)
+    assert np.array_equal(a2["off"], f["sg_agg_off"]) and np.array_equal(a2["deg"], f["sg_agg_deg"])
+    got = F.rows_as_maps(a2["off"], a2["deg"], a2["keys"], a2["w"])
+    want = F.rows_as_maps(f["sg_agg_off"], f["sg_agg_deg"], f["sg_agg_keys"], f["sg_agg_w"])
+    for g_, w_ in zip(got, want):
+        assert g_.keys() == w_.keys()
+        assert np.allclose([g_[k] for k in w_], [w_[k] for k in w_], rtol=1e-12)
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_weights_and_modularity(ctx, oracle, name):
+    _say("case", name)
+    f = F.load(name)
+    x = F.graph(f, "x")
+    k = ctx.vertex_weights(x)
+    ex = x.get("ex")
+    live = np.ones(len(k), bool) if ex is None else ex.astype(bool)
+    if name in INTEGER_FIXTURES:
+        assert np.array_equal(k[live], f["static_vtot"][live])
+        assert ctx.edge_weight(x) == f["x_edge_weight"]
+    else:
+        assert np.allclose(k[live], f["static_vtot"][live], rtol=1e-12)
+    q = ctx.modularity(x, f["static_memb"])
+    assert abs(q - float(f["static_Q"])) < 1e-12
+
+
+@pytest.mark.parametrize("path", [0, 1])
+@pytest.mark.parametrize("name", FIXTURES)
+def test_dynamic_end_to_end(ctx, oracle, name, path):
+    """ND / DS / DF from the reference's static state: K exact, modularity
+    within 1e-3 of the reference's result on the same inputs."""
+    _say("case", name, "path", path)
+    f = F.load(name)
+    x, y, d, i = _g(f)
+    ctx.set_path(path)
+    import lvhip
+    o = lvhip.LouvainOptions(*[t(v) for t, v in zip((int, float, float, float, float, int, int), f["opts"])])
+    try:
+        for approach in ("naive-dynamic", "delta-screening", "dynamic-frontier"):
+            key = approach.replace("-", "_")
+            r = ctx.louvain(approach, y, d, i, f["static_memb"], f["static_vtot"], f["static_ctot"], o)
+            q = oracle.modularity(y, r.membership)
+            _say(f"  {approach}: Q gpu={q:.6f} ref={float(f[key + '_Q']):.6f} it={r.iterations} p={r.passes}")
+            assert abs(q - float(f[key + "_Q"])) <= Q_TOL
+            if name in INTEGER_FIXTURES:
+                assert np.array_equal(r.vertexWeight, f[key + "_vtot"])
+            else:
+                assert np.allclose(r.vertexWeight, f[key + "_vtot"], rtol=1e-6, atol=1e-9)
+            _check_state(y, r)
+    finally:
+        ctx.set_path(1)
+
+
+def _check_state(g, r):
+    """Sigma_tot at live ids sums to 2M; for one-pass results every live
+    community's Sigma_tot equals the sum of its members' K."""
+    S = len(g["off"]) - 1
+    ex = g.get("ex")
+    live = np.ones(S, bool) if ex is None else ex.astype(bool)
+    m = r.membership[live]
+    assert (m < S).all()
+    m2 = float(g["w"].astype(np.float64).sum())
+    ids = np.unique(m)
+    assert np.isclose(r.communityWeight[ids].sum(), m2, rtol=1e-9)
+    if r.passes == 1:
+        tot = np.bincount(m, weights=r.vertexWeight[live], minlength=S)
+        assert np.allclose(r.communityWeight[ids], tot[ids], rtol=1e-9, atol=1e-9)
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_static_end_to_end(ctx, oracle, name):
+    _say("case", name)
+    f = F.load(name)
+    x = F.graph(f, "x")
+    import lvhip
+    o = lvhip.LouvainOptions(*[t(v) for t, v in zip((int, float, float, float, float, int, int), f["opts"])])
+    r = ctx.louvain("static", x, o=o)
+    q = oracle.modularity(x, r.membership)
+    _say(f"  static: Q gpu={q:.6f} ref={float(f['static_Q']):.6f} it={r.iterations} p={r.passes}")
+    # tiny graphs: parallel trajectories may settle in a different local optimum
+    tol = 0.05 if name.startswith("tiny") else 0.02
+    assert q >= float(f["static_Q"]) - tol
+    assert np.array_equal(r.vertexWeight, f["static_vtot"]) or name not in INTEGER_FIXTURES
+    _check_state(x, r)
+
+
+def test_unsorted_batch_rejected(ctx):
+    import lvhip
+    f = F.load("tiny1")
+    y = F.graph(f, "y")
+    d, i = F.batch(f, "del"), F.batch(f, "ins")
+    bad = {k: v[::-1].copy() for k, v in i.items()}
+    with pytest.raises(lvhip.LvError) as e:
+        ctx.louvain("dynamic-frontier", y, d, bad, f["static_memb"], f["static_vtot"], f["static_ctot"])
+    assert e.value.status == -4
+
+
+def test_out_of_range_batch_rejected(ctx):
+    import lvhip
+    f = F.load("tiny1")
+    y = F.graph(f, "y")
+    d = F.batch(f, "del")
+    bad = {"u": np.array([1, 200], np.uint32), "v": np.array([200, 1], np.uint32), "w": np.ones(2, np.float32)}
+    with pytest.raises(lvhip.LvError) as e:
+        ctx.louvain("dynamic-frontier", y, d, bad, f["static_memb"], f["static_vtot"], f["static_ctot"])
+    assert e.value.status == -1
