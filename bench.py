@@ -1,0 +1,194 @@
+"""Benchmark: DF Louvain on R-MAT scale 22 (edge factor 16, a=.57 b=.19 c=.19,
+symmetrised, unit weights), consecutive random batches of 1e-4 |E| (80%
+insertions / 20% deletions), seed 7 -- the workload of the headline metric
+"DF-Louvain edges/s & p99 batch latency at dQ<=1e-3 vs plain GPU path".
+
+A step is one Dynamic Frontier run (louvainDynamicFrontierOmp,
+inc/louvain.hxx:1674) on one batch, with graph, batch and state resident in
+HBM when the timed region starts; the state carries from batch to batch as
+in main.cxx:127-135.  value = directed edges of the updated graphs processed
+per second of wall clock over the K timed steps.  The same chain is then run
+on the plain GPU path (sweep over all vertices every iteration), checked
+against the oracle (|dQ| <= 1e-3 per batch) and timed on the host cores
+through the oracle (cpu_baseline).
+
+    python bench.py [--steps K] [--warmup W] [--scale 22] [--no-cpu] [--no-plain]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "louvain-communities-openmp-dynamic_amd"))
+
+HBM_PEAK_GBPS = 8000.0
+METRIC = "DF-Louvain edges/s & p99 batch latency at dQ<=1e-3 vs plain GPU path"
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def percentile(xs, p):
+    import numpy as np
+    return float(np.percentile(np.asarray(xs, dtype=np.float64), p)) if len(xs) else None
+
+
+def algorithmic_bytes(st, tier):
+    """Local-moving bytes of one tier, first pass (f32 weights): 12 B per
+    visited edge (key, weight, community gather), 29 B per processed vertex
+    (offset, degree, community, K, own Sigma_tot, flag), 8 B per candidate
+    community (Sigma_tot gather), 20 B per move (two Sigma_tot updates and
+    the membership store) and 1 B per flagged neighbour."""
+    return (12 * st["edges"][tier] + 29 * st["vertices"][tier] + 8 * st["candidates"][tier]
+            + 20 * st["moves"][tier] + st["moved_edges"][tier])
+
+
+def run_chain(ctx, lvhip, dgraphs, dbatches, state, state0, W, K, path):
+    """Warm up on batches [0, W), time batches [W, W + K); returns wall
+    seconds, per-step algorithm times and edge counts."""
+    ctx.set_path(path)
+    state.copy_from(state0)
+    for k in range(W):
+        ctx.run(lvhip.LV_DYNAMIC_FRONTIER, dgraphs[k], dbatches[k], state)
+    ctx.stats_reset()
+    ctx.set_profiling(True)
+    lat, edges = [], 0
+    t0 = time.perf_counter()
+    for k in range(W, W + K):
+        r = ctx.run(lvhip.LV_DYNAMIC_FRONTIER, dgraphs[k], dbatches[k], state)
+        lat.append(r.time)
+        edges += dgraphs[k].edges
+    t1 = time.perf_counter()
+    ctx.set_profiling(False)
+    return t1 - t0, lat, edges, ctx.stats()
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--scale", type=int, default=22)
+    ap.add_argument("--edge-factor", type=int, default=16)
+    ap.add_argument("--batch-fraction", type=float, default=1e-4)
+    ap.add_argument("--seed", type=int, default=7)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the oracle check and CPU baseline")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain-path comparison")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              "run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+
+    import lvgraph
+    import lvhip
+
+    W, K = args.warmup, args.steps
+    n_und = args.edge_factor << args.scale
+    B = int(round(args.batch_fraction * n_und))
+    n_ins = int(round(0.8 * B))
+    n_del = B - n_ins
+    t = time.time()
+    graphs = [lvgraph.rmat_native(args.scale, args.edge_factor, 0.57, 0.19, 0.19, args.seed)]
+    log(f"rmat scale {args.scale}: S={graphs[0].span} E={graphs[0].edges} ({time.time() - t:.1f}s)")
+    batches = []
+    t = time.time()
+    for k in range(W + K):
+        d, i = graphs[-1].random_batch(n_del, n_ins, args.seed * 1000 + k + 1)
+        batches.append((d, i))
+        graphs.append(graphs[-1].apply(d, i))
+    log(f"{W + K} batches of {B} updates ({n_del} del / {n_ins} ins undirected) applied ({time.time() - t:.1f}s)")
+
+    ctx = lvhip.Context(0)
+    t = time.time()
+    dgraphs = [ctx.device_graph(g.view()) for g in graphs]
+    dbatches = [ctx.device_batch(d, i) for d, i in batches]
+    S = graphs[0].span
+    state0 = ctx.device_state(S)
+    state = ctx.device_state(S)
+    log(f"uploaded {len(dgraphs)} graphs + {len(dbatches)} batches ({time.time() - t:.1f}s)")
+    t = time.time()
+    rs = ctx.run(lvhip.LV_STATIC, dgraphs[0], None, state0)
+    q_static = ctx.dmodularity(dgraphs[0], state0)
+    log(f"static: Q={q_static:.6f} iters={rs.iterations} passes={rs.passes} time={rs.time:.1f} ms "
+        f"(wall {time.time() - t:.1f}s)")
+
+    # Quality chain (untimed): the GPU state before every batch, for the oracle.
+    ctx.set_path(lvhip.PATH_WORKLIST)
+    state.copy_from(state0)
+    prev, q_gpu = [], []
+    for k in range(W + K):
+        prev.append(state.download())
+        ctx.run(lvhip.LV_DYNAMIC_FRONTIER, dgraphs[k + 1], dbatches[k], state)
+        q_gpu.append(ctx.dmodularity(dgraphs[k + 1], state))
+
+    # Timed: optimized (worklist) path, then the plain path.
+    dg = dgraphs[1:]
+    wall, lat, edges, st = run_chain(ctx, lvhip, dg, dbatches, state, state0, W, K, lvhip.PATH_WORKLIST)
+    value = edges / wall
+    log(f"DF worklist: {K} steps {wall * 1e3:.2f} ms wall, {value:.4g} edges/s, "
+        f"p50 {percentile(lat, 50):.3f} ms p99 {percentile(lat, 99):.3f} ms")
+    plain = None
+    if not args.no_plain:
+        pw, plat, pedges, _ = run_chain(ctx, lvhip, dg, dbatches, state, state0, W, K, lvhip.PATH_SWEEP)
+        plain = {"value": pedges / pw, "ms_per_step": pw * 1e3 / K, "p50_ms": percentile(plat, 50),
+                 "p99_ms": percentile(plat, 99), "speedup": (pedges / pw) and value / (pedges / pw)}
+        log(f"DF plain: {pw * 1e3:.2f} ms wall, {plain['value']:.4g} edges/s, speedup {plain['speedup']:.2f}x")
+        ctx.set_path(lvhip.PATH_WORKLIST)
+
+    # Roofline of the dominant local-moving kernel (the tier with most time).
+    tier = int(np.argmax(st["move_ms"]))
+    launches = max(st["move_launches"][tier], 1)
+    ms = st["move_ms"][tier]
+    bytes_total = algorithmic_bytes(st, tier)
+    achieved = (bytes_total / (ms * 1e-3) / 1e9) if ms > 0 else 0.0
+    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
+                "kernel": f"move_kernel tier {tier}", "avg_launch_us": ms * 1e3 / launches,
+                "bytes_per_launch": bytes_total / launches, "launches": launches}
+
+    cpu = None
+    dq_max = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        from oracle import Oracle
+        orc = Oracle()
+        cores = len(os.sched_getaffinity(0))
+        orc.set_threads(cores)
+        dqs, ctime, cedges = [], 0.0, 0
+        for k in range(W, W + K):
+            q, vt, ct = prev[k]
+            d, i = batches[k]
+            y = graphs[k + 1].view()
+            r = orc.louvain("dynamic-frontier", y, d, i, q, vt, ct)
+            q_ref = orc.modularity(y, r.membership)
+            dqs.append(abs(q_gpu[k] - q_ref))
+            ctime += r.time
+            cedges += graphs[k + 1].edges
+        dq_max = max(dqs)
+        cpu = {"value": cedges / (ctime * 1e-3), "unit": "edges/s", "cores": cores, "kind": "port",
+               "sample": f"oracle DF (C + OpenMP) on the {K} timed batches, same inputs and GPU carry-in "
+                         f"state, reference timed region; {ctime:.1f} ms total"}
+        log(f"oracle: max |dQ| = {dq_max:.3g}; cpu {cpu['value']:.4g} edges/s on {cores} threads")
+
+    out = {
+        "metric": METRIC, "value": value, "unit": "edges/s", "n_gpus": 1, "steps": K, "warmup": W,
+        "ms_per_step": wall * 1e3 / K, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"DF Louvain, R-MAT scale {args.scale} edge factor {args.edge_factor} "
+                               f"(a=.57 b=.19 c=.19) symmetrised unit weights, batches of "
+                               f"{args.batch_fraction:g}|E| 80/20 insert/delete, seed {args.seed}",
+                   "vertices": S, "edges": graphs[0].edges, "batch_updates": B, "parallelism": "1 GPU"},
+        "p50_batch_ms": percentile(lat, 50), "p99_batch_ms": percentile(lat, 99),
+        "dq_max": dq_max, "static_modularity": q_static, "plain_gpu_path": plain,
+        "roofline": roofline, "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

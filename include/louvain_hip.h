@@ -115,6 +115,36 @@ const char* lv_last_error(const lv_context* ctx);
  * iteration with a host convergence check (the plain path), 1 = frontier
  * worklists with device-side convergence (default). */
 int lv_context_set_path(lv_context* ctx, int path);
+/* Scheduling of the local-moving kernels.  Each launch runs
+ * streams = clamp(stream_fraction * live vertices of the graph level,
+ * min_streams, max_streams) lane groups; each walks ceil(n / streams)
+ * consecutive entries of its vertex list sequentially, so at most `streams`
+ * vertices are in flight (the reference runs one sequential stream per
+ * OpenMP thread over 2048-vertex chunks, inc/louvain.hxx:594).  Degree
+ * tiers run highest first when hubs_first is set.
+ * Defaults: 0.002, 1, 2^20, 1. */
+int lv_context_set_schedule(lv_context* ctx, double stream_fraction, uint64_t min_streams, uint64_t max_streams,
+                            int hubs_first);
+
+/* Kernel statistics of the local-moving phase, per degree tier (0..4),
+ * accumulated over lv_run / entry-point calls since lv_stats_reset.  The
+ * counters are the algorithmic work (processed vertices, visited edges,
+ * distinct candidate communities, moves, edges of moved vertices); move_ms
+ * and move_launches are filled only while profiling is on (each launch is
+ * then bracketed by HIP events on the context's stream). */
+typedef struct lv_stats {
+  double move_ms[5];
+  uint64_t move_launches[5];
+  uint64_t vertices[5];
+  uint64_t edges[5];
+  uint64_t candidates[5];
+  uint64_t moves[5];
+  uint64_t moved_edges[5];
+  uint64_t runs;
+} lv_stats;
+int lv_context_set_profiling(lv_context* ctx, int on);
+int lv_stats_get(const lv_context* ctx, lv_stats* out);
+int lv_stats_reset(lv_context* ctx);
 
 /* The four entry points (host buffers in, host buffers out) ------------ */
 /* louvainStaticOmp, inc/louvain.hxx:1285-1296 */

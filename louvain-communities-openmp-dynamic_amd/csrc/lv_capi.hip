@@ -49,6 +49,18 @@ struct lv_context {
   hipStream_t st = nullptr;
   std::string err;
   int path = 1;
+  // Local-moving streams per launch: clamp(stream_fraction * live vertices of
+  // the level, min_streams, max_streams); each walks ceil(n / streams)
+  // consecutive list entries sequentially.
+  double stream_fraction = 0.002;
+  uint64_t min_streams = 1;
+  uint64_t max_streams = 1ull << 20;
+  int tier_order = 1;           // 1: highest degree tier first, 0: lowest first
+  uint32_t serial = 0;          // worklist iteration serial (visit stamps)
+  uint64_t stamp_span = ~0ull;
+  int profiling = 0;             // time every local-moving launch with events
+  lv_stats stats{};              // accumulated since lv_stats_reset
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> prof;  // (tier, events) of a run
   std::map<std::string, Buf> bufs;
   void* pinned = nullptr;  // small pinned host buffer for scalar readbacks
   size_t pinned_bytes = 0;
@@ -227,8 +239,17 @@ struct Level {
 };
 
 struct Counters {
-  unsigned long long v[CNT_N] = {0, 0, 0, 0};
+  unsigned long long v[kTiers][CNT_N] = {};
 };
+
+// Launch one tier of the local-moving kernel, bracketed by events when the
+// context profiles (per-launch durations on the context's own stream).
+void launch_move(lv_context* c, const MoveArgs& a, int tier, bool f64) {
+  hipEvent_t e0 = nullptr;
+  if (c->profiling) e0 = c->record();
+  k_move(a, tier, f64, c->st);
+  if (c->profiling) c->prof.push_back({tier, {e0, c->record()}});
+}
 
 MoveArgs base_args(lv_context* c, const Level& lv, uint32_t* vcom, const double* vtot, double* ctot, uint8_t* vaff,
                    const uint8_t* fa, double M, double R) {
@@ -242,7 +263,10 @@ MoveArgs base_args(lv_context* c, const Level& lv, uint32_t* vcom, const double*
   a.M = M;
   a.R = R;
   a.el = c->ws<double>("el", 1);
-  a.cnt = c->ws<unsigned long long>("cnt", CNT_N);
+  a.cnt = c->ws<unsigned long long>("cnt", (size_t)kTiers * CNT_N);
+  a.min_chunk = 1;
+  a.max_streams = std::min(c->max_streams,
+                           std::max<uint64_t>(c->min_streams, (uint64_t)(c->stream_fraction * (double)lv.order)));
   uint64_t hcap = next_pow2(2 * (uint64_t)std::max<uint32_t>(lv.maxdeg4, 1));
   a.hcap = hcap;
   if (lv.maxdeg4) {
@@ -256,13 +280,23 @@ MoveArgs base_args(lv_context* c, const Level& lv, uint32_t* vcom, const double*
 
 // One sweep-mode iteration over the tier lists.
 void move_sweep(lv_context* c, MoveArgs a, const Level& lv) {
-  for (int t = 0; t < kTiers; ++t) {
+  unsigned long long* cnt0 = a.cnt;
+  for (int k = 0; k < kTiers; ++k) {
+    int t = c->tier_order ? kTiers - 1 - k : k;
     uint64_t n = lv.tiers->start[t + 1] - lv.tiers->start[t];
     if (!n) continue;
     a.list = lv.tiers->ids + lv.tiers->start[t];
     a.n = n;
-    k_move(a, t, lv.f64, c->st);
+    if (cnt0) a.cnt = cnt0 + (size_t)t * CNT_N;
+    launch_move(c, a, t, lv.f64);
   }
+}
+
+void add_counters(lv_context* c, const unsigned long long* dcnt, Counters& cnt) {
+  unsigned long long h[kTiers * CNT_N];
+  c->read(h, dcnt, (size_t)kTiers * CNT_N);
+  for (int t = 0; t < kTiers; ++t)
+    for (int i = 0; i < CNT_N; ++i) cnt.v[t][i] += h[t * CNT_N + i];
 }
 
 // louvainMoveOmpW (inc/louvain.hxx:587-609): iterate until the summed gain
@@ -271,7 +305,8 @@ void move_sweep(lv_context* c, MoveArgs a, const Level& lv) {
 int local_move(lv_context* c, const Level& lv, uint32_t* vcom, const double* vtot, double* ctot, uint8_t* vaff,
                const uint8_t* fa, double M, double R, int L, double E, Counters& cnt) {
   MoveArgs a = base_args(c, lv, vcom, vtot, ctot, vaff, fa, M, R);
-  k_fill_u64(reinterpret_cast<uint64_t*>(a.cnt), 0, CNT_N, c->st);
+  k_fill_u64(reinterpret_cast<uint64_t*>(a.cnt), 0, (size_t)kTiers * CNT_N, c->st);
+  unsigned long long* cnt0 = a.cnt;
   int l = 0;
   double el = 0;
   for (; l < L;) {
@@ -282,50 +317,73 @@ int local_move(lv_context* c, const Level& lv, uint32_t* vcom, const double* vto
     ++l;
     if (el <= E) break;
   }
-  unsigned long long h[CNT_N];
-  c->read(h, a.cnt, CNT_N);
-  for (int i = 0; i < CNT_N; ++i) cnt.v[i] += h[i];
+  add_counters(c, cnt0, cnt);
   return (l > 1 || el) ? l : 0;
 }
 
 // Worklist-mode local-moving for the first pass of the frontier-based
-// approaches: iteration k processes the vertices queued (flagged 0 -> 1)
-// during iteration k-1, binned by degree tier on the device; vaff keeps the
-// reference's meaning (set = flagged and not yet processed).
+// approaches.  Iteration k visits its queue; a vertex flagged 0 -> 1 by a
+// lower-id vertex and not yet visited in iteration k joins a follow-up
+// sub-round of the same iteration (the reference's ascending sweep reaches it
+// later in the same iteration), any other flagged vertex waits for
+// iteration k+1.  Sub-rounds run until no vertex is left for iteration k;
+// the gain el of iteration k sums over its sub-rounds.  Queues are binned by
+// degree tier on the device.  q[0..3]: iteration input / next iteration /
+// two sub-round buffers; qlen[0..3] their lengths.
 int local_move_worklist(lv_context* c, const Level& lv, uint32_t* vcom, const double* vtot, double* ctot,
-                        uint8_t* vaff, const uint8_t* fa, double M, double R, int L, double E, uint32_t* q[2],
+                        uint8_t* vaff, const uint8_t* fa, double M, double R, int L, double E, uint32_t* q[4],
                         uint32_t* qlen, Counters& cnt) {
   MoveArgs a = base_args(c, lv, vcom, vtot, ctot, vaff, fa, M, R);
-  k_fill_u64(reinterpret_cast<uint64_t*>(a.cnt), 0, CNT_N, c->st);
+  k_fill_u64(reinterpret_cast<uint64_t*>(a.cnt), 0, (size_t)kTiers * CNT_N, c->st);
+  unsigned long long* cnt0 = a.cnt;
   const uint64_t stride = std::max<uint64_t>(lv.g.S, 1);
   uint32_t* tq = c->ws<uint32_t>("wl.tq", (size_t)kTiers * stride);
   uint32_t* tlen = c->ws<uint32_t>("wl.tlen", kTiers);
+  a.stamp = c->ws<uint32_t>("wl.stamp", stride);
+  if (c->stamp_span != lv.g.S) {  // fresh stamps for a new span
+    k_fill_u32(a.stamp, 0, stride, c->st);
+    c->stamp_span = lv.g.S;
+    c->serial = 0;
+  }
   int l = 0;
   double el = 0;
-  int in = 0;
+  int in = 0, nx = 1;
   for (; l < L;) {
-    k_fill_f64(a.el, 0, 1, c->st);
-    k_fill_u32(qlen + (1 - in), 0, 1, c->st);
-    k_fill_u32(tlen, 0, kTiers, c->st);
-    k_bin_queue(q[in], qlen + in, std::min<uint64_t>(lv.g.S, 1u << 19), lv.g.deg, tq, stride, tlen, c->st);
-    a.qnext = q[1 - in];
-    a.qnext_len = qlen + (1 - in);
-    for (int k = 0; k < kTiers; ++k) {
-      if (k == 4 && !lv.maxdeg4) continue;
-      a.list = tq + (size_t)k * stride;
-      a.n_dev = tlen + k;
-      a.n = std::min<uint64_t>(lv.g.S, 1u << 15);  // grid-size hint
-      k_move(a, k, lv.f64, c->st);
+    if (++c->serial == 0) {  // wrapped: reset the stamps
+      k_fill_u32(a.stamp, 0, stride, c->st);
+      c->serial = 1;
     }
-    c->check();
+    a.serial = c->serial;
+    k_fill_f64(a.el, 0, 1, c->st);
+    k_fill_u32(qlen + nx, 0, 1, c->st);
+    a.qnext = q[nx];
+    a.qnext_len = qlen + nx;
+    int cur = in, sub = 2;
+    for (int round = 0;; ++round) {
+      k_fill_u32(tlen, 0, kTiers, c->st);
+      k_fill_u32(qlen + sub, 0, 1, c->st);
+      k_bin_queue(q[cur], qlen + cur, std::min<uint64_t>(lv.g.S, 1u << 19), lv.g.deg, tq, stride, tlen, c->st);
+      a.qsame = q[sub];
+      a.qsame_len = qlen + sub;
+      for (int k = (c->tier_order ? kTiers - 1 : 0); k >= 0 && k < kTiers; k += (c->tier_order ? -1 : 1)) {
+        if (k == 4 && !lv.maxdeg4) continue;
+        a.list = tq + (size_t)k * stride;
+        a.n_dev = tlen + k;
+        a.cnt = cnt0 + (size_t)k * CNT_N;
+        launch_move(c, a, k, lv.f64);
+      }
+      c->check();
+      uint32_t nsame = c->read1(qlen + sub);
+      if (nsame == 0) break;
+      cur = sub;
+      sub = sub == 2 ? 3 : 2;
+    }
     el = c->read1(a.el);
-    in = 1 - in;
+    std::swap(in, nx);
     ++l;
     if (el <= E) break;
   }
-  unsigned long long h[CNT_N];
-  c->read(h, a.cnt, CNT_N);
-  for (int i = 0; i < CNT_N; ++i) cnt.v[i] += h[i];
+  add_counters(c, cnt0, cnt);
   return (l > 1 || el) ? l : 0;
 }
 
@@ -468,13 +526,12 @@ void invoke(lv_context* c, int approach, const lv_dgraph* x, const lv_dbatch* b,
     dsc = c->ws<uint8_t>("ds.c", S);
   }
   bool worklist = c->path == 1 && (approach == LV_DYNAMIC_FRONTIER || approach == LV_DELTA_SCREENING);
-  uint32_t* q[2] = {nullptr, nullptr};
+  uint32_t* q[4] = {nullptr, nullptr, nullptr, nullptr};
   uint32_t* qlen = nullptr;
   if (worklist) {
-    q[0] = c->ws<uint32_t>("wl.q0", S);
-    q[1] = c->ws<uint32_t>("wl.q1", S);
-    qlen = c->ws<uint32_t>("wl.len", 2);
-    k_fill_u32(qlen, 0, 2, c->st);
+    for (int k = 0; k < 4; ++k) q[k] = c->ws<uint32_t>("wl.q" + std::to_string(k), S);
+    qlen = c->ws<uint32_t>("wl.len", 4);
+    k_fill_u32(qlen, 0, 4, c->st);
   }
   SuperGraph y = super_graph(c, "y", S, x->E), z = super_graph(c, "z", S, x->E);
   k_fill_u8(vaff, 0, (S + 7) & ~3ull, c->st);
@@ -611,7 +668,12 @@ void run(lv_context* c, int approach, const lv_dgraph* x, const lv_dbatch* b, lv
       d2d(c, s->ctot, bc, S);
     }
     c->ev_used = 0;
+    c->prof.clear();
     invoke(c, approach, x, b, s, o, l, p, rt, cnt);
+    for (auto& pr : c->prof) {
+      c->stats.move_ms[pr.first] += elapsed(pr.second.first, pr.second.second);
+      c->stats.move_launches[pr.first] += 1;
+    }
   }
   unsigned long long* dcount = c->ws<unsigned long long>("affected", 1);
   k_fill_u64(reinterpret_cast<uint64_t*>(dcount), 0, 1, c->st);
@@ -632,10 +694,15 @@ void run(lv_context* c, int approach, const lv_dgraph* x, const lv_dbatch* b, lv
     if (res->community_weight) d2h(c, res->community_weight, s->ctot, S);
     HIPCHK(hipStreamSynchronize(c->st));
   }
-  // per-run kernel counters, for bandwidth accounting
-  unsigned long long* keep = c->ws<unsigned long long>("cnt.last", CNT_N);
-  h2d(c, keep, cnt.v, CNT_N);
-  HIPCHK(hipStreamSynchronize(c->st));
+  // kernel statistics for bandwidth accounting (lv_stats_get)
+  for (int t = 0; t < kTiers; ++t) {
+    c->stats.vertices[t] += cnt.v[t][CNT_VERTICES];
+    c->stats.edges[t] += cnt.v[t][CNT_EDGES];
+    c->stats.candidates[t] += cnt.v[t][CNT_CANDIDATES];
+    c->stats.moves[t] += cnt.v[t][CNT_MOVES];
+    c->stats.moved_edges[t] += cnt.v[t][CNT_MOVED_EDGES];
+  }
+  c->stats.runs += 1;
 }
 
 // ------------------------------------------------------------ uploads
@@ -868,6 +935,34 @@ const char* lv_last_error(const lv_context* c) { return c ? c->err.c_str() : "no
 int lv_context_set_path(lv_context* c, int path) {
   if (!c || (path != 0 && path != 1)) return LV_ERR_INVALID;
   c->path = path;
+  return LV_OK;
+}
+
+int lv_context_set_profiling(lv_context* c, int on) {
+  if (!c) return LV_ERR_INVALID;
+  c->profiling = on ? 1 : 0;
+  return LV_OK;
+}
+
+int lv_stats_get(const lv_context* c, lv_stats* out) {
+  if (!c || !out) return LV_ERR_INVALID;
+  *out = c->stats;
+  return LV_OK;
+}
+
+int lv_stats_reset(lv_context* c) {
+  if (!c) return LV_ERR_INVALID;
+  c->stats = lv_stats{};
+  return LV_OK;
+}
+
+int lv_context_set_schedule(lv_context* c, double stream_fraction, uint64_t min_streams, uint64_t max_streams,
+                            int hubs_first) {
+  if (!c || !(stream_fraction > 0) || min_streams < 1 || max_streams < min_streams) return LV_ERR_INVALID;
+  c->stream_fraction = stream_fraction;
+  c->min_streams = min_streams;
+  c->max_streams = max_streams;
+  c->tier_order = hubs_first ? 1 : 0;
   return LV_OK;
 }
 
@@ -1125,6 +1220,8 @@ int lv_choose_community(lv_context* c, const lv_graph* g, const uint32_t* vcom, 
     Level lv{w.g->view(), w.g->f64, &w.g->tiers, w.g->order, w.g->tiers.maxdeg};
     MoveArgs a = base_args(c, lv, w.s->memb, w.s->vtot, w.s->ctot, nullptr, nullptr, M, R);
     a.choose_only = 1;
+    a.min_chunk = 1;
+    a.max_streams = 1ull << 30;
     a.out_c = oc;
     a.out_e = oe;
     a.cnt = nullptr;

@@ -49,7 +49,7 @@ struct TierLists {
 };
 
 // Device counters of the local-moving kernels (for achieved-bandwidth figures).
-enum { CNT_VERTICES = 0, CNT_EDGES = 1, CNT_CANDIDATES = 2, CNT_MOVES = 3, CNT_N = 4 };
+enum { CNT_VERTICES = 0, CNT_EDGES = 1, CNT_CANDIDATES = 2, CNT_MOVES = 3, CNT_MOVED_EDGES = 4, CNT_N = 5 };
 
 // Arguments of the local-moving kernels.
 struct MoveArgs {
@@ -68,9 +68,15 @@ struct MoveArgs {
   double* out_e;
   double* el;                  // accumulated gain (l1-norm of delta-Q)
   unsigned long long* cnt;     // CNT_N counters
-  uint32_t* qnext;             // worklist mode: vertices flagged 0 -> 1
+  uint32_t* qnext;             // worklist mode: vertices flagged 0 -> 1, next iteration
   uint32_t* qnext_len;
+  uint32_t* qsame;             // worklist mode: flagged by a lower id and not yet
+  uint32_t* qsame_len;         //   processed this iteration -> same iteration
+  uint32_t* stamp;             // worklist mode: iteration serial of the last visit
+  uint32_t serial;
   int pick_less;               // 1: only moves to a lower community id
+  uint64_t min_chunk;          // sequential streams: chunk >= min_chunk entries
+  uint64_t max_streams;        //   and at most max_streams streams in flight
   // tier-4 HBM hashtables: one region of hcap slots per block
   uint32_t* hkey;
   uint32_t* hpos;

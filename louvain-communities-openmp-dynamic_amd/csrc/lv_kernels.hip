@@ -20,6 +20,8 @@
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
+#include <algorithm>
+
 #include "lv_internal.hpp"
 
 namespace lv {
@@ -128,24 +130,35 @@ __device__ unsigned long long block_sum_u64(unsigned long long x) {
   return t;
 }
 
-// Set vaff[v] = 1; in worklist mode push v when its flag goes 0 -> 1.
-// Pushes are aggregated per wave (one atomic per wave).
-__device__ __forceinline__ void flag_vertex(uint8_t* vaff, uint32_t v, uint32_t* q, uint32_t* qlen) {
-  if (!q) {
-    vaff[v] = 1;
-    return;
-  }
-  uint32_t sh = (v & 3u) * 8u;
-  uint32_t old = atomicOr(reinterpret_cast<uint32_t*>(vaff) + (v >> 2), 1u << sh);
-  bool fresh = ((old >> sh) & 0xFFu) == 0;
-  unsigned long long mask = __ballot(fresh);
+// Wave-aggregated append of the lanes with `on` set (one atomic per wave).
+__device__ __forceinline__ void wave_push(bool on, uint32_t v, uint32_t* q, uint32_t* qlen) {
+  unsigned long long mask = __ballot(on);
   if (!mask) return;
   int lane = __lane_id();
   int leader = __ffsll((long long)mask) - 1;
   uint32_t base = 0;
   if (lane == leader) base = atomicAdd(qlen, (uint32_t)__popcll(mask));
   base = __shfl(base, leader);
-  if (fresh) q[base + __popcll(mask & ((1ull << lane) - 1))] = v;
+  if (on) q[base + __popcll(mask & ((1ull << lane) - 1))] = v;
+}
+
+// vaff[v] = 1, as a moved vertex u flags its neighbour v (inc/louvain.hxx:602).
+// Worklist mode queues v when its flag goes 0 -> 1: into the same iteration
+// when v > u and v was not visited yet this iteration (the reference's
+// ascending sweep reaches such a v later in the same iteration), otherwise
+// into the next iteration.
+__device__ __forceinline__ void flag_vertex(uint8_t* vaff, uint32_t v, uint32_t u, const MoveArgs& a) {
+  if (!a.qnext) {
+    __hip_atomic_store(&vaff[v], (uint8_t)1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return;
+  }
+  uint32_t sh = (v & 3u) * 8u;
+  uint32_t old = atomicOr(reinterpret_cast<uint32_t*>(vaff) + (v >> 2), 1u << sh);
+  bool fresh = ((old >> sh) & 0xFFu) == 0;
+  bool same = fresh && a.qsame && v > u &&
+              __hip_atomic_load(&a.stamp[v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != a.serial;
+  wave_push(same, v, a.qsame, a.qsame_len);
+  wave_push(fresh && !same, v, a.qnext, a.qnext_len);
 }
 
 // ---------------------------------------------------------------- fills
@@ -373,11 +386,29 @@ __global__ void check_sorted_kernel(const uint32_t* u, const uint32_t* v, uint64
 }
 
 // ------------------------------------------------------- local-moving
+// Shared mutable state (memberships, Sigma_tot, flags) is read and written
+// at device scope so that a stream sees the moves other streams made.
+template <class T>
+__device__ __forceinline__ T ld_dev(const T* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class T>
+__device__ __forceinline__ void st_dev(T* p, T v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // louvainMoveOmpW body (inc/louvain.hxx:595-605): for each affected vertex,
 // sum edge weights per neighbour community (self-loops skipped), choose
 // the best positive deltaModularity (first-seen on ties), and unless it is
 // community 0 move there (atomic Sigma_tot), flag the neighbours; clear the
-// vertex's flag and add its gain to el.  One group of G lanes per vertex.
+// vertex's flag and add its gain to el.
+//
+// Scheduling follows the reference's OpenMP loop (schedule(dynamic, 2048),
+// inc/louvain.hxx:594): the list is cut into chunks of consecutive entries
+// and each group of G lanes walks its chunk SEQUENTIALLY, one vertex at a
+// time, so a vertex sees every earlier move of its own stream and the moves
+// other streams have published.  chunk = max(min_chunk, ceil(n / max_streams)):
+// at most max_streams vertices are in flight at once.
 template <int BLOCK, int G, int CAP, class WT>
 __global__ __launch_bounds__(BLOCK) void move_kernel(MoveArgs a) {
   constexpr int GPB = BLOCK / G;
@@ -389,6 +420,11 @@ __global__ __launch_bounds__(BLOCK) void move_kernel(MoveArgs a) {
   if (a.done && *a.done) return;
   const int lane = threadIdx.x % G, grp = threadIdx.x / G;
   const uint64_t n = a.n_dev ? (uint64_t)*a.n_dev : a.n;
+  const uint64_t streams = a.max_streams ? a.max_streams : 1;
+  uint64_t chunk = (n + streams - 1) / streams;
+  if (chunk < a.min_chunk) chunk = a.min_chunk;
+  if (chunk < 1) chunk = 1;
+  const uint64_t nchunks = (n + chunk - 1) / chunk;
   const WT* W = (const WT*)a.g.w;
   uint32_t *tkey, *tpos;
   double* tval;
@@ -406,82 +442,87 @@ __global__ __launch_bounds__(BLOCK) void move_kernel(MoveArgs a) {
     cap = CAP;
   }
   double el_acc = 0;
-  unsigned long long c_vert = 0, c_edge = 0, c_cand = 0, c_move = 0;
-  for (uint64_t base = (uint64_t)blockIdx.x * GPB; base < n; base += (uint64_t)gridDim.x * GPB) {
-    const uint64_t idx = base + grp;
-    bool act = false;
-    uint32_t u = 0;
-    if (idx < n) {
-      u = a.list ? a.list[idx] : (uint32_t)idx;
-      act = (!a.g.ex || a.g.ex[u]) && (a.choose_only || (a.vaff[u] && (!a.fa || a.fa[u])));
-    }
-    uint64_t o = 0;
-    uint32_t d = 0, cu = 0;
-    double Ku = 0;
-    if (act) {
-      o = a.g.off[u];
-      d = a.g.deg[u];
-      cu = a.vcom[u];
-      Ku = a.vtot[u];
-      if constexpr (HBM) {
-        // this hub's table: nextpow2(2 d) <= hcap slots of the block's region
-        cap = 2;
-        while (cap < 2 * (uint64_t)d) cap <<= 1;
+  unsigned long long c_vert = 0, c_edge = 0, c_cand = 0, c_move = 0, c_medge = 0;
+  for (uint64_t cb = (uint64_t)blockIdx.x * GPB; cb < nchunks; cb += (uint64_t)gridDim.x * GPB) {
+    const uint64_t ch = cb + grp;
+    for (uint64_t j = 0; j < chunk; ++j) {
+      const uint64_t idx = ch * chunk + j;
+      bool act = false;
+      uint32_t u = 0;
+      if (ch < nchunks && idx < n) {
+        u = a.list ? a.list[idx] : (uint32_t)idx;
+        act = (!a.g.ex || a.g.ex[u]) && (a.choose_only || (ld_dev(&a.vaff[u]) && (!a.fa || a.fa[u])));
+        if (act && a.stamp && lane == 0) st_dev(&a.stamp[u], a.serial);
       }
-      for (uint64_t s = lane; s < cap; s += G) { tkey[s] = kEmpty; tpos[s] = kNoPos; tval[s] = 0; }
-    }
-    __syncthreads();
-    if (act) {
-      for (uint32_t i = lane; i < d; i += G) {
-        uint32_t v = a.g.keys[o + i];
-        if (v == u) continue;
-        tab_add(tkey, tpos, tval, cap, a.vcom[v], (double)W[o + i], i);
+      uint64_t o = 0;
+      uint32_t d = 0, cu = 0;
+      double Ku = 0;
+      if (act) {
+        o = a.g.off[u];
+        d = a.g.deg[u];
+        cu = ld_dev(&a.vcom[u]);
+        Ku = a.vtot[u];
+        if constexpr (HBM) {
+          // this hub's table: nextpow2(2 d) <= hcap slots of the block's region
+          cap = 2;
+          while (cap < 2 * (uint64_t)d) cap <<= 1;
+        }
+        for (uint64_t s = lane; s < cap; s += G) { tkey[s] = kEmpty; tpos[s] = kNoPos; tval[s] = 0; }
       }
-    }
-    __syncthreads();
-    double be = 0;
-    uint32_t bp = kNoPos, bc = 0;
-    if (act) {
-      const double vd = tab_get(tkey, tval, cap, cu);
-      const double Sd = a.ctot[cu];
-      for (uint64_t s = lane; s < cap; s += G) {
-        uint32_t c = tkey[s];
-        if (c == kEmpty) continue;
-        ++c_cand;
-        if (c == cu) continue;
-        if (a.pick_less && c > cu) continue;
-        double e = delta_q(tval[s], vd, Ku, a.ctot[c], Sd, a.M, a.R);
-        uint32_t p = tpos[s];
-        if (e > 0 && better(e, p, be, bp)) { be = e; bp = p; bc = c; }
+      __syncthreads();
+      if (act) {
+        for (uint32_t i = lane; i < d; i += G) {
+          uint32_t v = a.g.keys[o + i];
+          if (v == u) continue;
+          tab_add(tkey, tpos, tval, cap, ld_dev(&a.vcom[v]), (double)W[o + i], i);
+        }
       }
-    }
-    if constexpr (G <= 64) group_best_shfl<G>(be, bp, bc);
-    else block_best<BLOCK>(be, bp, bc);
-    if (act) {
-      if (a.choose_only) {
-        if (lane == 0) { a.out_c[u] = bc; a.out_e[u] = be; }
-      } else {
-        if (bc != 0) {
+      __syncthreads();
+      double be = 0;
+      uint32_t bp = kNoPos, bc = 0;
+      if (act) {
+        const double vd = tab_get(tkey, tval, cap, cu);
+        const double Sd = ld_dev(&a.ctot[cu]);
+        for (uint64_t s = lane; s < cap; s += G) {
+          uint32_t c = tkey[s];
+          if (c == kEmpty) continue;
+          ++c_cand;
+          if (c == cu) continue;
+          if (a.pick_less && c > cu) continue;
+          double e = delta_q(tval[s], vd, Ku, ld_dev(&a.ctot[c]), Sd, a.M, a.R);
+          uint32_t p = tpos[s];
+          if (e > 0 && better(e, p, be, bp)) { be = e; bp = p; bc = c; }
+        }
+      }
+      if constexpr (G <= 64) group_best_shfl<G>(be, bp, bc);
+      else block_best<BLOCK>(be, bp, bc);
+      if (act) {
+        if (a.choose_only) {
+          if (lane == 0) { a.out_c[u] = bc; a.out_e[u] = be; }
+        } else {
+          if (bc != 0) {
+            if (lane == 0) {
+              unsafeAtomicAdd(&a.ctot[cu], -Ku);
+              unsafeAtomicAdd(&a.ctot[bc], Ku);
+              st_dev(&a.vcom[u], bc);
+              ++c_move;
+              c_medge += d;
+            }
+            for (uint32_t i = lane; i < d; i += G) {
+              uint32_t v = a.g.keys[o + i];
+              if (v != u) flag_vertex(a.vaff, v, u, a);
+            }
+          }
           if (lane == 0) {
-            unsafeAtomicAdd(&a.ctot[cu], -Ku);
-            unsafeAtomicAdd(&a.ctot[bc], Ku);
-            a.vcom[u] = bc;
-            ++c_move;
+            st_dev(&a.vaff[u], (uint8_t)0);
+            el_acc += be;
+            ++c_vert;
+            c_edge += d;
           }
-          for (uint32_t i = lane; i < d; i += G) {
-            uint32_t v = a.g.keys[o + i];
-            if (v != u) flag_vertex(a.vaff, v, a.qnext, a.qnext_len);
-          }
-        }
-        if (lane == 0) {
-          a.vaff[u] = 0;
-          el_acc += be;
-          ++c_vert;
-          c_edge += d;
         }
       }
+      __syncthreads();
     }
-    __syncthreads();
   }
   if (!a.choose_only) {
     double t = block_sum<BLOCK>(el_acc);
@@ -490,11 +531,13 @@ __global__ __launch_bounds__(BLOCK) void move_kernel(MoveArgs a) {
   if (a.cnt) {
     unsigned long long v = block_sum_u64<BLOCK>(c_vert), e = block_sum_u64<BLOCK>(c_edge);
     unsigned long long c = block_sum_u64<BLOCK>(c_cand), m = block_sum_u64<BLOCK>(c_move);
+    unsigned long long me = block_sum_u64<BLOCK>(c_medge);
     if (threadIdx.x == 0) {
       if (v) atomicAdd(&a.cnt[CNT_VERTICES], v);
       if (e) atomicAdd(&a.cnt[CNT_EDGES], e);
       if (c) atomicAdd(&a.cnt[CNT_CANDIDATES], c);
       if (m) atomicAdd(&a.cnt[CNT_MOVES], m);
+      if (me) atomicAdd(&a.cnt[CNT_MOVED_EDGES], me);
     }
   }
 }
@@ -922,15 +965,22 @@ void k_check_sorted(const uint32_t* u, const uint32_t* v, uint64_t n, uint32_t S
 
 template <class WT>
 static void move_tier(const MoveArgs& a, int tier, hipStream_t st) {
-  // Grid: enough blocks for the list, capped; kernels loop grid-stride.  When
-  // the count lives on the device (worklists) the grid is fixed.
-  uint64_t n = a.n_dev ? (uint64_t)a.n : a.n;
+  // One group per chunk (grid-stride over chunks).  With a device-side count
+  // the host sizes the grid for max_streams groups.
+  uint64_t streams = a.max_streams ? a.max_streams : 1;
+  uint64_t groups;
+  if (a.n_dev) {
+    groups = streams;
+  } else {
+    uint64_t chunk = std::max<uint64_t>(std::max<uint64_t>(a.min_chunk, 1), (a.n + streams - 1) / streams);
+    groups = (a.n + chunk - 1) / chunk;
+  }
   switch (tier) {
-    case 0: LV_LAUNCH((move_kernel<256, 8, 16, WT>), grid_for(n, 32, 1u << 16), 256, st, a); break;
-    case 1: LV_LAUNCH((move_kernel<256, 16, 64, WT>), grid_for(n, 16, 1u << 16), 256, st, a); break;
-    case 2: LV_LAUNCH((move_kernel<256, 64, 256, WT>), grid_for(n, 4, 1u << 16), 256, st, a); break;
-    case 3: LV_LAUNCH((move_kernel<256, 256, 2048, WT>), grid_for(n, 1, 1u << 14), 256, st, a); break;
-    default: LV_LAUNCH((move_kernel<1024, 1024, 0, WT>), grid_for(n, 1, 256), 1024, st, a); break;
+    case 0: LV_LAUNCH((move_kernel<256, 8, 16, WT>), grid_for(groups, 32, 1u << 16), 256, st, a); break;
+    case 1: LV_LAUNCH((move_kernel<256, 16, 64, WT>), grid_for(groups, 16, 1u << 16), 256, st, a); break;
+    case 2: LV_LAUNCH((move_kernel<256, 64, 256, WT>), grid_for(groups, 4, 1u << 16), 256, st, a); break;
+    case 3: LV_LAUNCH((move_kernel<256, 256, 2048, WT>), grid_for(groups, 1, 1u << 14), 256, st, a); break;
+    default: LV_LAUNCH((move_kernel<1024, 1024, 0, WT>), grid_for(groups, 1, 256), 1024, st, a); break;
   }
 }
 void k_move(const MoveArgs& a, int tier, bool f64, hipStream_t st) {

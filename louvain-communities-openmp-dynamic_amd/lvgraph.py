@@ -72,6 +72,40 @@ class _Handle:
             raise RuntimeError("lvgraph: generation/reading failed")
         self.h = C.c_void_p(h)
 
+    @property
+    def span(self):
+        return _L.lvg_span(self.h)
+
+    @property
+    def edges(self):
+        return _L.lvg_edges(self.h)
+
+    def view(self):
+        """Zero-copy numpy views of the native arrays (valid while self lives)."""
+        S, E = self.span, self.edges
+        def v(ptr, n, dt):
+            if n == 0 or not ptr:
+                return np.zeros(0, dt)
+            return np.ctypeslib.as_array(C.cast(ptr, C.POINTER(np.ctypeslib.as_ctypes_type(dt))), shape=(n,))
+        g = {"off": v(_L.lvg_offsets(self.h), S + 1, np.uint64), "keys": v(_L.lvg_keys(self.h), E, np.uint32),
+             "w": v(_L.lvg_weights(self.h), E, np.float32)}
+        ex = _L.lvg_exists(self.h)
+        if ex:
+            g["ex"] = v(ex, S, np.uint8)
+        return g
+
+    def random_batch(self, n_del, n_ins, seed, weight=1.0):
+        return random_batch(self, n_del, n_ins, seed, weight)
+
+    def apply(self, dels, ins):
+        """New native graph with the (tidy) batch applied."""
+        d, i = _batch_arrays(dels, ins)
+        bh = _tidy_handle(self, d, i)
+        try:
+            return _Handle(_L.lvg_apply_batch(self.h, C.c_void_p(bh)))
+        finally:
+            _L.lvg_batch_free(C.c_void_p(bh))
+
     def to_dict(self):
         S = _L.lvg_span(self.h)
         E = _L.lvg_edges(self.h)
@@ -103,6 +137,15 @@ def _handle_of(g):
 
 def rmat(scale, edge_factor=16, a=0.57, b=0.19, c=0.19, seed=7):
     return _Handle(_L.lvg_rmat(scale, edge_factor, a, b, c, seed)).to_dict()
+
+
+def rmat_native(scale, edge_factor=16, a=0.57, b=0.19, c=0.19, seed=7):
+    """R-MAT as a native handle (no copy); .view() gives numpy views."""
+    return _Handle(_L.lvg_rmat(scale, edge_factor, a, b, c, seed))
+
+
+def planted_partition_native(n, blocks, avg_degree, p_inter, seed, exponent=2.0, one_indexed=True):
+    return _Handle(_L.lvg_planted_partition(n, blocks, exponent, avg_degree, p_inter, seed, int(one_indexed)))
 
 
 def planted_partition(n, blocks, avg_degree, p_inter, seed, exponent=2.0, one_indexed=True):

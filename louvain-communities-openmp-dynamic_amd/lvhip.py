@@ -34,7 +34,8 @@ PATH_SWEEP, PATH_WORKLIST = 0, 1
 # Every symbol include/louvain_hip.h declares.
 EXPORTS = [
     "lv_options_default", "lv_status_string", "lv_context_create", "lv_context_destroy", "lv_last_error",
-    "lv_context_set_path", "lv_louvain_static", "lv_louvain_naive_dynamic", "lv_louvain_delta_screening",
+    "lv_context_set_path", "lv_context_set_schedule", "lv_context_set_profiling", "lv_stats_get",
+    "lv_stats_reset", "lv_louvain_static", "lv_louvain_naive_dynamic", "lv_louvain_delta_screening",
     "lv_louvain_dynamic_frontier", "lv_dgraph_create", "lv_dgraph_destroy", "lv_dgraph_span", "lv_dgraph_edges",
     "lv_dbatch_create", "lv_dbatch_destroy", "lv_dstate_create", "lv_dstate_destroy", "lv_dstate_upload",
     "lv_dstate_download", "lv_dstate_copy", "lv_run", "lv_dmodularity", "lv_edge_weight", "lv_modularity",
@@ -62,6 +63,12 @@ class _Graph(C.Structure):
 
 class _Batch(C.Structure):
     _fields_ = [("u", C.c_void_p), ("v", C.c_void_p), ("w", C.c_void_p), ("size", C.c_uint64)]
+
+
+class _Stats(C.Structure):
+    _fields_ = [("move_ms", C.c_double * 5), ("move_launches", C.c_uint64 * 5), ("vertices", C.c_uint64 * 5),
+                ("edges", C.c_uint64 * 5), ("candidates", C.c_uint64 * 5), ("moves", C.c_uint64 * 5),
+                ("moved_edges", C.c_uint64 * 5), ("runs", C.c_uint64)]
 
 
 class _Result(C.Structure):
@@ -180,6 +187,23 @@ class Context:
 
     def set_path(self, path):
         self._check(self.lib.lv_context_set_path(self.h, int(path)), "lv_context_set_path")
+
+    def set_schedule(self, stream_fraction=0.002, min_streams=1, max_streams=1 << 20, hubs_first=True):
+        self._check(self.lib.lv_context_set_schedule(self.h, C.c_double(stream_fraction), C.c_uint64(min_streams),
+                                                     C.c_uint64(max_streams), int(bool(hubs_first))),
+                    "lv_context_set_schedule")
+
+    def set_profiling(self, on=True):
+        self._check(self.lib.lv_context_set_profiling(self.h, int(bool(on))), "lv_context_set_profiling")
+
+    def stats_reset(self):
+        self._check(self.lib.lv_stats_reset(self.h), "lv_stats_reset")
+
+    def stats(self):
+        """Local-moving kernel statistics per degree tier (lv_stats)."""
+        st = _Stats()
+        self._check(self.lib.lv_stats_get(self.h, C.byref(st)), "lv_stats_get")
+        return {f: (list(getattr(st, f)) if f != "runs" else st.runs) for f, _ in _Stats._fields_}
 
     def _check(self, rc, what):
         if rc != 0:
