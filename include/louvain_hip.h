@@ -115,16 +115,36 @@ const char* lv_last_error(const lv_context* ctx);
  * iteration with a host convergence check (the plain path), 1 = frontier
  * worklists with device-side convergence (default). */
 int lv_context_set_path(lv_context* ctx, int path);
-/* Scheduling of the local-moving kernels.  Each launch runs
+/* Scheduling of the local-moving kernels, the role of the reference's
+ * schedule(dynamic, 2048) loop (inc/louvain.hxx:594).  Each launch runs
  * streams = clamp(stream_fraction * live vertices of the graph level,
- * min_streams, max_streams) lane groups; each walks ceil(n / streams)
- * consecutive entries of its vertex list sequentially, so at most `streams`
- * vertices are in flight (the reference runs one sequential stream per
- * OpenMP thread over 2048-vertex chunks, inc/louvain.hxx:594).  Degree
- * tiers run highest first when hubs_first is set.
- * Defaults: 0.002, 1, 2^20, 1. */
+ * min_streams, max_streams) lane groups; chunks of `chunk` consecutive
+ * entries of the vertex list are handed out in ascending order, and a group
+ * walks its chunk sequentially, so the list is swept in ascending order by a
+ * window of about streams x chunk entries.  order: 0 = degree tiers lowest
+ * first, 1 = highest first, 2 = no tiers: ascending ids through one kernel
+ * (vertices of degree > 4096 run first through the hub kernel); at one
+ * stream, order 2 is the reference's sequential visit order.  chunk == 0
+ * hands group g the g-th of `streams` equal parts of the list instead, so
+ * the vertices in flight lie far apart in the list.
+ * Defaults: 0.002, 1, 2^20, 0, 2. */
 int lv_context_set_schedule(lv_context* ctx, double stream_fraction, uint64_t min_streams, uint64_t max_streams,
-                            int hubs_first);
+                            uint64_t chunk, int order);
+/* The first `iterations` local-moving iterations of every pass only move a
+ * vertex to a community with a lower id than its own (no two vertices can
+ * then swap communities in one iteration).  Default 0 (off). */
+int lv_context_set_pick_less(lv_context* ctx, int iterations);
+/* Id-order sweeps (order 2) cut the ids into `slabs` consecutive ranges
+ * launched one after another, each seeing every move of the ranges before
+ * it; the streams spread within a range.  Default 1. */
+int lv_context_set_slabs(lv_context* ctx, uint64_t slabs);
+/* Hubs (degree > 4096) are processed by the hub kernel, at most `streams`
+ * (1..256) at once, before the other vertices of each sweep.  Default 256. */
+int lv_context_set_hub_streams(lv_context* ctx, uint64_t streams);
+/* Degree above which a vertex runs in the hub pre-pass instead of in id
+ * order (>= 4096; a larger value keeps more hubs in the ascending sweep at
+ * the cost of larger per-stream HBM tables).  Default 4096. */
+int lv_context_set_hub_threshold(lv_context* ctx, uint32_t degree);
 
 /* Kernel statistics of the local-moving phase, per degree tier (0..4),
  * accumulated over lv_run / entry-point calls since lv_stats_reset.  The

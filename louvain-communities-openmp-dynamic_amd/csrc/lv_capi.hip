@@ -50,12 +50,17 @@ struct lv_context {
   std::string err;
   int path = 1;
   // Local-moving streams per launch: clamp(stream_fraction * live vertices of
-  // the level, min_streams, max_streams); each walks ceil(n / streams)
+  // the level, min_streams, max_streams) lane groups, each walking dynamically
   // consecutive list entries sequentially.
   double stream_fraction = 0.002;
   uint64_t min_streams = 1;
   uint64_t max_streams = 1ull << 20;
-  int tier_order = 1;           // 1: highest degree tier first, 0: lowest first
+  uint64_t chunk = 0;           // entries per dynamically assigned chunk (0: static spread)
+  int pick_less_iters = 0;      // first iterations of each pass that only move to lower ids
+  uint64_t slabs = 1;           // id-order sweeps: consecutive id slabs launched in turn
+  uint64_t hub_streams = kHubBlocks;  // hubs processed at once
+  uint32_t hub_min = kHubMin;         // degrees above run in the hub pre-pass
+  int tier_order = 2;           // 2: ascending ids (hubs first), 1: degree tiers highest first, 0: lowest first
   uint32_t serial = 0;          // worklist iteration serial (visit stamps)
   uint64_t stamp_span = ~0ull;
   int profiling = 0;             // time every local-moving launch with events
@@ -146,7 +151,8 @@ struct lv_dgraph {
   bool f64 = false;
   uint8_t* ex = nullptr;
   double M2 = 0;  // 2M, edgeWeightOmp
-  TierLists tiers;
+  TierLists tiers;  // five degree tiers
+  TierLists hubs;   // tier 4 = degree > kHubMin (id-order mode), ascending
   GView view() const { return GView{S, off, deg, keys, w, ex}; }
 };
 
@@ -205,15 +211,15 @@ void scan64(lv_context* c, const uint64_t* in, uint64_t* out, uint64_t n) {
 
 // Degree-tier lists of ids 0..n-1 (live only), ascending within each tier.
 void build_tiers(lv_context* c, const std::string& name, const uint64_t* off, const uint32_t* deg,
-                 const uint8_t* ex, uint64_t n, TierLists& t, uint32_t* ids_out = nullptr) {
+                 const uint8_t* ex, uint64_t n, TierLists& t, uint32_t* ids_out = nullptr, uint32_t thr = 0) {
   uint32_t nb = (uint32_t)std::max<uint64_t>(1, (n + 255) / 256);
   uint32_t* counts = c->ws<uint32_t>(name + ".cnt", (size_t)kTiers * nb + 1);
   uint32_t* base = c->ws<uint32_t>(name + ".base", (size_t)kTiers * nb + 1);
   t.ids = ids_out ? ids_out : c->ws<uint32_t>(name + ".ids", std::max<uint64_t>(n, 1));
   k_fill_u32(counts, 0, (size_t)kTiers * nb + 1, c->st);
-  if (n) k_tier_count(off, deg, ex, n, 0, 0, counts, nb, c->st);
+  if (n) k_tier_count(off, deg, ex, n, thr, counts, nb, c->st);
   scan32(c, counts, base, (size_t)kTiers * nb + 1);
-  if (n) k_tier_scatter(off, deg, ex, n, 0, 0, base, nb, t.ids, c->st);
+  if (n) k_tier_scatter(off, deg, ex, n, thr, base, nb, t.ids, c->st);
   // tier starts: base[t * nb] for t <= kTiers (the last is the total)
   uint32_t st[kTiers + 1];
   c->read_strided(st, base, kTiers + 1, nb);
@@ -233,9 +239,10 @@ uint32_t max_degree_tier4(lv_context* c, const TierLists& t, const uint32_t* deg
 struct Level {
   GView g;
   bool f64;
-  const TierLists* tiers;
-  uint64_t order;  // live vertices
-  uint32_t maxdeg4;
+  const TierLists* tiers;  // five degree tiers (tier orders 0, 1)
+  uint64_t order;          // live vertices
+  uint32_t maxdeg4;        // max degree in tier 4 of `tiers`
+  const TierLists* hubs;   // degree > kHubMin, ascending (id order)
 };
 
 struct Counters {
@@ -245,10 +252,16 @@ struct Counters {
 // Launch one tier of the local-moving kernel, bracketed by events when the
 // context profiles (per-launch durations on the context's own stream).
 void launch_move(lv_context* c, const MoveArgs& a, int tier, bool f64) {
+  k_fill_u64(reinterpret_cast<uint64_t*>(a.work), 0, 1, c->st);
   hipEvent_t e0 = nullptr;
   if (c->profiling) e0 = c->record();
   k_move(a, tier, f64, c->st);
   if (c->profiling) c->prof.push_back({tier, {e0, c->record()}});
+}
+
+uint64_t level_streams(const lv_context* c, uint64_t order) {
+  uint64_t want = (uint64_t)(c->stream_fraction * (double)order);
+  return std::min(c->max_streams, std::max<uint64_t>(c->min_streams, want));
 }
 
 MoveArgs base_args(lv_context* c, const Level& lv, uint32_t* vcom, const double* vtot, double* ctot, uint8_t* vaff,
@@ -264,23 +277,63 @@ MoveArgs base_args(lv_context* c, const Level& lv, uint32_t* vcom, const double*
   a.R = R;
   a.el = c->ws<double>("el", 1);
   a.cnt = c->ws<unsigned long long>("cnt", (size_t)kTiers * CNT_N);
-  a.min_chunk = 1;
-  a.max_streams = std::min(c->max_streams,
-                           std::max<uint64_t>(c->min_streams, (uint64_t)(c->stream_fraction * (double)lv.order)));
-  uint64_t hcap = next_pow2(2 * (uint64_t)std::max<uint32_t>(lv.maxdeg4, 1));
-  a.hcap = hcap;
-  if (lv.maxdeg4) {
-    size_t slots = (size_t)hcap * 256;  // tier-4 grid is at most 256 blocks
+  a.min_chunk = 0;
+  a.max_streams = level_streams(c, lv.order);
+  a.chunk = c->chunk;
+  a.work = c->ws<unsigned long long>("work", 1);
+  a.hub_min = c->hub_min;
+  a.hub_streams = c->hub_streams;
+  // hub kernel: one HBM region per block, sized for the largest hub
+  uint32_t md = c->tier_order == 2 ? lv.hubs->maxdeg : lv.maxdeg4;
+  if (md) {
+    a.hcap = next_pow2(2 * (uint64_t)md);
+    size_t slots = (size_t)a.hcap * kHubBlocks;
     a.hkey = c->ws<uint32_t>("h.key", slots);
     a.hpos = c->ws<uint32_t>("h.pos", slots);
     a.hval = c->ws<double>("h.val", slots);
   }
+  if (c->tier_order == 2) {
+    // id-order kernel: one HBM region per stream holding nextpow2(2 deg)
+    // slots for every degree it runs (<= hub_min)
+    a.gblocks = std::min<uint64_t>(a.max_streams, 1u << 16);
+    uint64_t dmax = kHubMin;
+    if (c->hub_min > kHubMin) dmax = std::max<uint64_t>(dmax, std::min<uint64_t>(c->hub_min, lv.hubs->maxdeg));
+    a.gcap = std::max<uint64_t>(kHybridHbm, next_pow2(2 * dmax));
+    size_t slots = (size_t)a.gcap * a.gblocks;
+    a.gkey = c->ws<uint32_t>("g.key", slots);
+    a.gpos = c->ws<uint32_t>("g.pos", slots);
+    a.gval = c->ws<double>("g.val", slots);
+  }
   return a;
 }
 
-// One sweep-mode iteration over the tier lists.
+// One sweep-mode iteration over the tier lists (or, in id-order mode, over
+// all ids ascending through the general HBM-table kernel).
 void move_sweep(lv_context* c, MoveArgs a, const Level& lv) {
   unsigned long long* cnt0 = a.cnt;
+  if (c->tier_order == 2) {
+    // hubs (degree > kHubMin, ascending) first, then every other id ascending
+    uint64_t nh = lv.hubs->start[5] - lv.hubs->start[4];
+    if (nh && c->hub_min < lv.hubs->maxdeg) {
+      a.list = lv.hubs->ids + lv.hubs->start[4];
+      a.n = nh;
+      if (cnt0) a.cnt = cnt0 + (size_t)4 * CNT_N;
+      launch_move(c, a, 4, lv.f64);
+    }
+    // the id range in `slabs` consecutive slabs, one launch each: a slab sees
+    // every move of the slabs before it
+    if (cnt0) a.cnt = cnt0;
+    a.list = nullptr;
+    uint64_t P = std::max<uint64_t>(1, std::min<uint64_t>(c->slabs, lv.g.S));
+    for (uint64_t k = 0; k < P; ++k) {
+      uint64_t lo = lv.g.S * k / P, hi = lv.g.S * (k + 1) / P;
+      if (hi <= lo) continue;
+      a.base = lo;
+      a.n = hi - lo;
+      launch_move(c, a, kTierHybrid, lv.f64);
+    }
+    return;
+  }
   for (int k = 0; k < kTiers; ++k) {
     int t = c->tier_order ? kTiers - 1 - k : k;
     uint64_t n = lv.tiers->start[t + 1] - lv.tiers->start[t];
@@ -311,6 +364,7 @@ int local_move(lv_context* c, const Level& lv, uint32_t* vcom, const double* vto
   double el = 0;
   for (; l < L;) {
     k_fill_f64(a.el, 0, 1, c->st);
+    a.pick_less = l < c->pick_less_iters;
     move_sweep(c, a, lv);
     c->check();
     el = c->read1(a.el);
@@ -354,6 +408,7 @@ int local_move_worklist(lv_context* c, const Level& lv, uint32_t* vcom, const do
       c->serial = 1;
     }
     a.serial = c->serial;
+    a.pick_less = l < c->pick_less_iters;
     k_fill_f64(a.el, 0, 1, c->st);
     k_fill_u32(qlen + nx, 0, 1, c->st);
     a.qnext = q[nx];
@@ -362,10 +417,24 @@ int local_move_worklist(lv_context* c, const Level& lv, uint32_t* vcom, const do
     for (int round = 0;; ++round) {
       k_fill_u32(tlen, 0, kTiers, c->st);
       k_fill_u32(qlen + sub, 0, 1, c->st);
-      k_bin_queue(q[cur], qlen + cur, std::min<uint64_t>(lv.g.S, 1u << 19), lv.g.deg, tq, stride, tlen, c->st);
       a.qsame = q[sub];
       a.qsame_len = qlen + sub;
-      for (int k = (c->tier_order ? kTiers - 1 : 0); k >= 0 && k < kTiers; k += (c->tier_order ? -1 : 1)) {
+      uint32_t thr = c->tier_order == 2 ? c->hub_min : 0;
+      k_bin_queue(q[cur], qlen + cur, std::min<uint64_t>(lv.g.S, 1u << 19), lv.g.deg, thr, tq, stride, tlen, c->st);
+      if (c->tier_order == 2) {  // id-order mode: hubs, then the rest through the id-order kernel
+        if (lv.hubs->maxdeg > c->hub_min) {
+          a.list = tq + (size_t)4 * stride;
+          a.n_dev = tlen + 4;
+          a.cnt = cnt0 + (size_t)4 * CNT_N;
+          launch_move(c, a, 4, lv.f64);
+        }
+        a.list = tq;
+        a.n_dev = tlen;
+        a.cnt = cnt0;
+        launch_move(c, a, kTierHybrid, lv.f64);
+      }
+      for (int k = (c->tier_order ? kTiers - 1 : 0); c->tier_order != 2 && k >= 0 && k < kTiers;
+           k += (c->tier_order ? -1 : 1)) {
         if (k == 4 && !lv.maxdeg4) continue;
         a.list = tq + (size_t)k * stride;
         a.n_dev = tlen + k;
@@ -578,9 +647,9 @@ void invoke(lv_context* c, int approach, const lv_dgraph* x, const lv_dbatch* b,
   hipEvent_t e2 = c->record();
   hipEvent_t t1 = e2;
   bool t1_set = false;
-  Level lv0{x->view(), x->f64, &x->tiers, x->order, x->tiers.maxdeg};
+  Level lv0{x->view(), x->f64, &x->tiers, x->order, x->tiers.maxdeg, &x->hubs};
   Level cur = lv0;
-  TierLists ytiers;
+  TierLists ytiers, yhubs;
   uint64_t ySpan = 0;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> lm, ag;
   for (l = 0, p = 0; M > 0 && P > 0;) {
@@ -617,10 +686,14 @@ void invoke(lv_context* c, int approach, const lv_dgraph* x, const lv_dbatch* b,
     GView yg{ySpan, y.off, y.deg, y.keys, y.w, nullptr};
     k_initialize(yg, vcom, ctot, vtot, c->st);
     k_fill_u8(vaff, 1, CN, c->st);
-    build_tiers(c, "y.tiers", y.off, y.deg, nullptr, CN, ytiers);
-    uint64_t n4 = ytiers.start[5] - ytiers.start[4];
-    ytiers.maxdeg = n4 ? max_degree_tier4(c, ytiers, y.deg) : 0;
-    cur = Level{yg, true, &ytiers, CN, ytiers.maxdeg};
+    if (c->tier_order == 2) {
+      build_tiers(c, "y.hubs", y.off, y.deg, nullptr, CN, yhubs, nullptr, kHubMin);
+      yhubs.maxdeg = max_degree_tier4(c, yhubs, y.deg);
+    } else {
+      build_tiers(c, "y.tiers", y.off, y.deg, nullptr, CN, ytiers);
+      ytiers.maxdeg = max_degree_tier4(c, ytiers, y.deg);
+    }
+    cur = Level{yg, true, &ytiers, CN, ytiers.maxdeg, &yhubs};
     E /= o->tolerance_drop;
   }
   if (p > 1) k_lookup(ucom, S, vcom, c->st);
@@ -755,6 +828,9 @@ lv_dgraph* upload_graph(lv_context* c, const lv_graph* g) {
     d->tiers.ids = dalloc<uint32_t>(std::max<uint64_t>(d->S, 1));
     build_tiers(c, "g.tiers", d->off, d->deg, d->ex, d->S, d->tiers, d->tiers.ids);
     d->tiers.maxdeg = max_degree_tier4(c, d->tiers, d->deg);
+    d->hubs.ids = dalloc<uint32_t>(std::max<uint64_t>(d->S, 1));
+    build_tiers(c, "g.hubs", d->off, d->deg, d->ex, d->S, d->hubs, d->hubs.ids, kHubMin);
+    d->hubs.maxdeg = max_degree_tier4(c, d->hubs, d->deg);
     c->check();
   } catch (...) {
     lv_dgraph_destroy(d);
@@ -938,6 +1014,30 @@ int lv_context_set_path(lv_context* c, int path) {
   return LV_OK;
 }
 
+int lv_context_set_slabs(lv_context* c, uint64_t slabs) {
+  if (!c || slabs < 1) return LV_ERR_INVALID;
+  c->slabs = slabs;
+  return LV_OK;
+}
+
+int lv_context_set_hub_streams(lv_context* c, uint64_t streams) {
+  if (!c || streams < 1 || streams > kHubBlocks) return LV_ERR_INVALID;
+  c->hub_streams = streams;
+  return LV_OK;
+}
+
+int lv_context_set_hub_threshold(lv_context* c, uint32_t degree) {
+  if (!c || degree < kHubMin) return LV_ERR_INVALID;
+  c->hub_min = degree;
+  return LV_OK;
+}
+
+int lv_context_set_pick_less(lv_context* c, int iterations) {
+  if (!c || iterations < 0) return LV_ERR_INVALID;
+  c->pick_less_iters = iterations;
+  return LV_OK;
+}
+
 int lv_context_set_profiling(lv_context* c, int on) {
   if (!c) return LV_ERR_INVALID;
   c->profiling = on ? 1 : 0;
@@ -957,12 +1057,14 @@ int lv_stats_reset(lv_context* c) {
 }
 
 int lv_context_set_schedule(lv_context* c, double stream_fraction, uint64_t min_streams, uint64_t max_streams,
-                            int hubs_first) {
+                            uint64_t chunk, int hubs_first) {
   if (!c || !(stream_fraction > 0) || min_streams < 1 || max_streams < min_streams) return LV_ERR_INVALID;
+  c->chunk = chunk;
   c->stream_fraction = stream_fraction;
   c->min_streams = min_streams;
   c->max_streams = max_streams;
-  c->tier_order = hubs_first ? 1 : 0;
+  if (hubs_first < 0 || hubs_first > 2) return LV_ERR_INVALID;
+  c->tier_order = hubs_first;
   return LV_OK;
 }
 
@@ -997,6 +1099,7 @@ void lv_dgraph_destroy(lv_dgraph* g) {
   (void)hipFree(g->w);
   (void)hipFree(g->ex);
   (void)hipFree(g->tiers.ids);
+  (void)hipFree(g->hubs.ids);
   delete g;
 }
 uint64_t lv_dgraph_span(const lv_dgraph* g) { return g ? g->S : 0; }
@@ -1217,10 +1320,10 @@ int lv_choose_community(lv_context* c, const lv_graph* g, const uint32_t* vcom, 
     double* oe = c->ws<double>("ph.be", S);
     k_fill_u32(oc, 0, S, c->st);
     k_fill_f64(oe, 0, S, c->st);
-    Level lv{w.g->view(), w.g->f64, &w.g->tiers, w.g->order, w.g->tiers.maxdeg};
+    Level lv{w.g->view(), w.g->f64, &w.g->tiers, w.g->order, w.g->tiers.maxdeg, &w.g->hubs};
     MoveArgs a = base_args(c, lv, w.s->memb, w.s->vtot, w.s->ctot, nullptr, nullptr, M, R);
     a.choose_only = 1;
-    a.min_chunk = 1;
+    a.chunk = 1;
     a.max_streams = 1ull << 30;
     a.out_c = oc;
     a.out_e = oe;
@@ -1268,7 +1371,7 @@ int lv_aggregate(lv_context* c, const lv_graph* g, const uint32_t* vcom, uint64_
     h2d(c, d, vcom, S);
     SuperGraph z = super_graph(c, "ph.z", std::max<uint64_t>(C, 1), E);
     double* ztot = c->ws<double>("ph.ztot", std::max<uint64_t>(C, 1));
-    Level lv{w.g->view(), w.g->f64, &w.g->tiers, w.g->order, w.g->tiers.maxdeg};
+    Level lv{w.g->view(), w.g->f64, &w.g->tiers, w.g->order, w.g->tiers.maxdeg, &w.g->hubs};
     aggregate(c, lv, d, C, z, ztot);
     d2h(c, zoff, z.off, C + 1);
     d2h(c, zdeg, z.deg, C);

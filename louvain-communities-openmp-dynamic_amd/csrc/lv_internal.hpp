@@ -17,6 +17,14 @@ constexpr uint32_t kNoPos = 0xFFFFFFFFu;  // "not seen" first-position marker
 constexpr int kTiers = 5;                 // degree tiers of the group-per-vertex kernels
 // Upper degree bound of tiers 0..3 (LDS hashtables); tier 4 spills to HBM.
 constexpr uint32_t kTierMax[4] = {8, 32, 128, 1024};
+// Id-order kernel (kernel "tier" 5): one wave per stream, LDS table of
+// kHybridLds slots when 2 deg fits, else an HBM region of kHybridHbm slots;
+// degrees above kHubMin go to the hub kernel (tier 4, at most kHubBlocks blocks).
+constexpr int kHybridLds = 512;
+constexpr uint64_t kHybridHbm = 8192;
+constexpr uint32_t kHubMin = 4096;
+constexpr uint64_t kHubBlocks = 256;
+constexpr int kTierHybrid = 5;
 
 inline int tier_of(uint64_t d) {
   for (int t = 0; t < 4; ++t)
@@ -54,7 +62,8 @@ enum { CNT_VERTICES = 0, CNT_EDGES = 1, CNT_CANDIDATES = 2, CNT_MOVES = 3, CNT_M
 // Arguments of the local-moving kernels.
 struct MoveArgs {
   GView g;
-  const uint32_t* list;        // vertex ids to consider (nullptr = identity 0..n-1)
+  const uint32_t* list;        // vertex ids to consider (nullptr = ids base .. base+n-1)
+  uint64_t base;               // first id when list is nullptr
   uint64_t n;                  // entries in list
   const uint32_t* n_dev;       // if set, n is read from here on the device
   uint32_t* vcom;
@@ -75,13 +84,23 @@ struct MoveArgs {
   uint32_t* stamp;             // worklist mode: iteration serial of the last visit
   uint32_t serial;
   int pick_less;               // 1: only moves to a lower community id
-  uint64_t min_chunk;          // sequential streams: chunk >= min_chunk entries
-  uint64_t max_streams;        //   and at most max_streams streams in flight
-  // tier-4 HBM hashtables: one region of hcap slots per block
+  uint64_t min_chunk;          // unused (kept zero)
+  uint64_t max_streams;        // lane groups launched (streams walking at once)
+  uint64_t chunk;              // list entries per dynamically assigned chunk
+  unsigned long long* work;    // chunk counter, zero before each launch
+  // hub kernel (tier 4) HBM hashtables: one region of hcap slots per block
   uint32_t* hkey;
   uint32_t* hpos;
   double* hval;
   uint64_t hcap;
+  // id-order kernel (tier 5): one region of gcap slots per block (stream)
+  uint32_t* gkey;
+  uint32_t* gpos;
+  double* gval;
+  uint64_t gcap;
+  uint64_t gblocks;            // regions allocated = grid cap
+  uint32_t hub_min;            // degrees above this go to the hub kernel
+  uint64_t hub_streams;        // hub kernel: hubs processed at once (<= kHubBlocks)
   const int* done;             // device convergence flag: skip work when set
 };
 
@@ -116,10 +135,12 @@ void k_scatter_members(GView g, const uint32_t* vcom, const uint32_t* coff, uint
                        hipStream_t st);
 void k_member_degrees(const uint32_t* cedg, uint64_t n, GView g, uint64_t* mdeg, hipStream_t st);
 void k_gather_yoff(const uint32_t* coff, const uint64_t* mscan, uint64_t C, uint64_t* yoff, hipStream_t st);
-void k_tier_count(const uint64_t* off, const uint32_t* deg, const uint8_t* ex, uint64_t n, int caps_mode,
-                  uint64_t CN, uint32_t* block_counts, uint32_t nblocks, hipStream_t st);
-void k_tier_scatter(const uint64_t* off, const uint32_t* deg, const uint8_t* ex, uint64_t n, int caps_mode,
-                    uint64_t CN, const uint32_t* block_base, uint32_t nblocks, uint32_t* ids, hipStream_t st);
+// Degree partition of ids 0..n-1 (live only), ascending within each part:
+// thr == 0 the five tiers, thr > 0 two parts (tier 0: degree <= thr, tier 4: > thr).
+void k_tier_count(const uint64_t* off, const uint32_t* deg, const uint8_t* ex, uint64_t n, uint32_t thr,
+                  uint32_t* block_counts, uint32_t nblocks, hipStream_t st);
+void k_tier_scatter(const uint64_t* off, const uint32_t* deg, const uint8_t* ex, uint64_t n, uint32_t thr,
+                    const uint32_t* block_base, uint32_t nblocks, uint32_t* ids, hipStream_t st);
 struct AggArgs {
   GView g;                   // graph being aggregated
   const uint32_t* vcom;      // renumbered membership
@@ -145,8 +166,8 @@ void k_aggregate(const AggArgs& a, int tier, bool f64, hipStream_t st);
 void k_agg_emit_hbm(const AggArgs& a, uint64_t total_slots, hipStream_t st);
 void k_t4_sizes(const uint64_t* yoff, const uint32_t* list, uint64_t n, uint64_t C, uint64_t chunk,
                 uint64_t* caps, uint64_t* chunks, hipStream_t st);
-void k_bin_queue(const uint32_t* q, const uint32_t* qlen, uint64_t qcap, const uint32_t* deg, uint32_t* tq,
-                 uint64_t stride, uint32_t* tlen, hipStream_t st);
+void k_bin_queue(const uint32_t* q, const uint32_t* qlen, uint64_t qcap, const uint32_t* deg, uint32_t thr,
+                 uint32_t* tq, uint64_t stride, uint32_t* tlen, hipStream_t st);
 void k_max_deg(const uint32_t* ids, uint64_t n, const uint32_t* deg, uint32_t* out, hipStream_t st);
 void k_max_u32(const uint32_t* a, uint64_t n, uint32_t* out, hipStream_t st);
 void k_compact_flags(const uint8_t* f, uint64_t n, uint32_t* q, uint32_t* qlen, hipStream_t st);

@@ -397,6 +397,45 @@ __device__ __forceinline__ void st_dev(T* p, T v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// One vertex's neighbour-community table: phases of louvainClearScanW,
+// louvainScanCommunitiesW and louvainChooseCommunity (inc/louvain.hxx:406-464)
+// for a group of G lanes.  Called with LDS or HBM table pointers; after
+// inlining each call site keeps native ds_* or global_* instructions.
+template <int G>
+__device__ __forceinline__ void table_clear(uint32_t* key, uint32_t* pos, double* val, uint64_t cap, int lane) {
+  for (uint64_t s = lane; s < cap; s += G) { key[s] = kEmpty; pos[s] = kNoPos; val[s] = 0; }
+}
+
+template <int G, class WT>
+__device__ __forceinline__ void table_scan(const MoveArgs& a, const WT* W, uint32_t u, uint64_t o, uint32_t d,
+                                           uint32_t* key, uint32_t* pos, double* val, uint64_t cap, int lane) {
+  for (uint32_t i = lane; i < d; i += G) {
+    uint32_t v = a.g.keys[o + i];
+    if (v == u) continue;  // self-loops are skipped (louvainScanCommunityW<SELF=false>)
+    tab_add(key, pos, val, cap, ld_dev(&a.vcom[v]), (double)W[o + i], i);
+  }
+}
+
+template <int G>
+__device__ __forceinline__ void table_best(const MoveArgs& a, uint32_t cu, double Ku, const uint32_t* key,
+                                           const uint32_t* pos, const double* val, uint64_t cap, int lane, double& be,
+                                           uint32_t& bp, uint32_t& bc, unsigned long long& c_cand) {
+  const double vd = tab_get(key, val, cap, cu);
+  const double Sd = ld_dev(&a.ctot[cu]);
+  for (uint64_t s = lane; s < cap; s += G) {
+    uint32_t c = key[s];
+    if (c == kEmpty) continue;
+    ++c_cand;
+    if (c == cu) continue;
+    if (a.pick_less && c > cu) continue;
+    double e = delta_q(val[s], vd, Ku, ld_dev(&a.ctot[c]), Sd, a.M, a.R);
+    uint32_t p = pos[s];
+    if (e > 0 && better(e, p, be, bp)) { be = e; bp = p; bc = c; }
+  }
+}
+
+enum { MOVE_LDS = 0, MOVE_HBM = 1, MOVE_HYBRID = 2 };
+
 // louvainMoveOmpW body (inc/louvain.hxx:595-605): for each affected vertex,
 // sum edge weights per neighbour community (self-loops skipped), choose
 // the best positive deltaModularity (first-seen on ties), and unless it is
@@ -404,95 +443,109 @@ __device__ __forceinline__ void st_dev(T* p, T v) {
 // vertex's flag and add its gain to el.
 //
 // Scheduling follows the reference's OpenMP loop (schedule(dynamic, 2048),
-// inc/louvain.hxx:594): the list is cut into chunks of consecutive entries
-// and each group of G lanes walks its chunk SEQUENTIALLY, one vertex at a
-// time, so a vertex sees every earlier move of its own stream and the moves
-// other streams have published.  chunk = max(min_chunk, ceil(n / max_streams)):
-// at most max_streams vertices are in flight at once.
-template <int BLOCK, int G, int CAP, class WT>
+// inc/louvain.hxx:594): the list is cut into chunks of a.chunk consecutive
+// entries handed out in ascending order from a device counter (a.work); each
+// group of G lanes walks its chunk SEQUENTIALLY, one vertex at a time, then
+// takes the next free chunk.  The list is therefore swept in ascending order
+// by a window of (resident groups x chunk) entries, and a vertex sees the
+// moves of nearly every entry before it.
+//
+// MODE: MOVE_LDS    per-group LDS table of CAP slots (degree <= CAP / 2)
+//       MOVE_HBM    per-block HBM table of nextpow2(2 deg) <= hcap slots (hubs)
+//       MOVE_HYBRID one wave per stream: LDS table when 2 deg <= CAP, else the
+//                   block's HBM region of gcap slots; vertices with degree above
+//                   hub_min are left to the MOVE_HBM pre-pass.
+template <int BLOCK, int G, int CAP, class WT, int MODE>
 __global__ __launch_bounds__(BLOCK) void move_kernel(MoveArgs a) {
   constexpr int GPB = BLOCK / G;
-  constexpr bool HBM = CAP == 0;
-  constexpr int LCAP = HBM ? 1 : CAP;
+  constexpr int LCAP = MODE == MOVE_HBM ? 1 : CAP;
   __shared__ uint32_t s_key[GPB * LCAP];
   __shared__ uint32_t s_pos[GPB * LCAP];
   __shared__ double s_val[GPB * LCAP];
+  __shared__ uint64_t s_take;
   if (a.done && *a.done) return;
   const int lane = threadIdx.x % G, grp = threadIdx.x / G;
   const uint64_t n = a.n_dev ? (uint64_t)*a.n_dev : a.n;
+  // a.chunk > 0: chunks of a.chunk entries handed out in ascending order;
+  // a.chunk == 0: ceil(n / streams) entries per chunk, chunk g to group g,
+  // so the groups walk far-apart parts of the list at the same time.
+  const bool dyn = a.chunk != 0;
   const uint64_t streams = a.max_streams ? a.max_streams : 1;
-  uint64_t chunk = (n + streams - 1) / streams;
-  if (chunk < a.min_chunk) chunk = a.min_chunk;
-  if (chunk < 1) chunk = 1;
+  const uint64_t chunk = dyn ? a.chunk : ((n + streams - 1) / streams ? (n + streams - 1) / streams : 1);
   const uint64_t nchunks = (n + chunk - 1) / chunk;
+  uint64_t next = (uint64_t)blockIdx.x * GPB;
   const WT* W = (const WT*)a.g.w;
-  uint32_t *tkey, *tpos;
-  double* tval;
-  uint64_t cap;
-  if constexpr (HBM) {
+  // LDS table of this group; HBM region of this block
+  uint32_t* lkey = s_key + grp * LCAP;
+  uint32_t* lpos = s_pos + grp * LCAP;
+  double* lval = s_val + grp * LCAP;
+  uint32_t *hkey = nullptr, *hpos = nullptr;
+  double* hval = nullptr;
+  if constexpr (MODE == MOVE_HBM) {
     uint64_t base = (uint64_t)blockIdx.x * a.hcap;
-    tkey = a.hkey + base;
-    tpos = a.hpos + base;
-    tval = a.hval + base;
-    cap = a.hcap;
-  } else {
-    tkey = s_key + grp * CAP;
-    tpos = s_pos + grp * CAP;
-    tval = s_val + grp * CAP;
-    cap = CAP;
+    hkey = a.hkey + base;
+    hpos = a.hpos + base;
+    hval = a.hval + base;
+  } else if constexpr (MODE == MOVE_HYBRID) {
+    uint64_t base = (uint64_t)blockIdx.x * a.gcap;
+    hkey = a.gkey + base;
+    hpos = a.gpos + base;
+    hval = a.gval + base;
   }
   double el_acc = 0;
   unsigned long long c_vert = 0, c_edge = 0, c_cand = 0, c_move = 0, c_medge = 0;
-  for (uint64_t cb = (uint64_t)blockIdx.x * GPB; cb < nchunks; cb += (uint64_t)gridDim.x * GPB) {
+  for (;;) {
+    // the block takes GPB consecutive chunks (one per group)
+    uint64_t cb = next;
+    if (dyn) {
+      if (threadIdx.x == 0) s_take = atomicAdd(a.work, (unsigned long long)GPB);
+      __syncthreads();
+      cb = s_take;
+      __syncthreads();
+    } else {
+      next += (uint64_t)gridDim.x * GPB;
+    }
+    if (cb >= nchunks) break;
     const uint64_t ch = cb + grp;
     for (uint64_t j = 0; j < chunk; ++j) {
       const uint64_t idx = ch * chunk + j;
       bool act = false;
       uint32_t u = 0;
       if (ch < nchunks && idx < n) {
-        u = a.list ? a.list[idx] : (uint32_t)idx;
+        u = a.list ? a.list[idx] : (uint32_t)(a.base + idx);
         act = (!a.g.ex || a.g.ex[u]) && (a.choose_only || (ld_dev(&a.vaff[u]) && (!a.fa || a.fa[u])));
+        if constexpr (MODE == MOVE_HYBRID) act = act && a.g.deg[u] <= a.hub_min;
         if (act && a.stamp && lane == 0) st_dev(&a.stamp[u], a.serial);
       }
       uint64_t o = 0;
       uint32_t d = 0, cu = 0;
       double Ku = 0;
+      bool hbm = MODE == MOVE_HBM;
+      uint64_t cap = CAP;
       if (act) {
         o = a.g.off[u];
         d = a.g.deg[u];
         cu = ld_dev(&a.vcom[u]);
         Ku = a.vtot[u];
-        if constexpr (HBM) {
-          // this hub's table: nextpow2(2 d) <= hcap slots of the block's region
-          cap = 2;
+        if constexpr (MODE == MOVE_HYBRID) hbm = 2 * (uint64_t)d > CAP;
+        if (hbm) {
+          cap = 2;  // nextpow2(2 d) slots of the block's region
           while (cap < 2 * (uint64_t)d) cap <<= 1;
         }
-        for (uint64_t s = lane; s < cap; s += G) { tkey[s] = kEmpty; tpos[s] = kNoPos; tval[s] = 0; }
+        if (hbm) table_clear<G>(hkey, hpos, hval, cap, lane);
+        else table_clear<G>(lkey, lpos, lval, cap, lane);
       }
       __syncthreads();
       if (act) {
-        for (uint32_t i = lane; i < d; i += G) {
-          uint32_t v = a.g.keys[o + i];
-          if (v == u) continue;
-          tab_add(tkey, tpos, tval, cap, ld_dev(&a.vcom[v]), (double)W[o + i], i);
-        }
+        if (hbm) table_scan<G, WT>(a, W, u, o, d, hkey, hpos, hval, cap, lane);
+        else table_scan<G, WT>(a, W, u, o, d, lkey, lpos, lval, cap, lane);
       }
       __syncthreads();
       double be = 0;
       uint32_t bp = kNoPos, bc = 0;
       if (act) {
-        const double vd = tab_get(tkey, tval, cap, cu);
-        const double Sd = ld_dev(&a.ctot[cu]);
-        for (uint64_t s = lane; s < cap; s += G) {
-          uint32_t c = tkey[s];
-          if (c == kEmpty) continue;
-          ++c_cand;
-          if (c == cu) continue;
-          if (a.pick_less && c > cu) continue;
-          double e = delta_q(tval[s], vd, Ku, ld_dev(&a.ctot[c]), Sd, a.M, a.R);
-          uint32_t p = tpos[s];
-          if (e > 0 && better(e, p, be, bp)) { be = e; bp = p; bc = c; }
-        }
+        if (hbm) table_best<G>(a, cu, Ku, hkey, hpos, hval, cap, lane, be, bp, bc, c_cand);
+        else table_best<G>(a, cu, Ku, lkey, lpos, lval, cap, lane, be, bp, bc, c_cand);
       }
       if constexpr (G <= 64) group_best_shfl<G>(be, bp, bc);
       else block_best<BLOCK>(be, bp, bc);
@@ -726,34 +779,36 @@ __global__ void t4_sizes_kernel(const uint64_t* yoff, const uint32_t* list, uint
 // Partition ids 0..n-1 (live ones) into degree tiers, ascending within each
 // tier.  Pass 1 counts per block and tier (tier-major layout), an exclusive
 // scan of the counts gives each block's base, pass 2 scatters in order.
-__device__ __forceinline__ int dev_tier(uint64_t d) {
+// thr == 0: the five degree tiers; thr > 0: two classes, tier 4 = degree > thr.
+__device__ __forceinline__ int dev_tier(uint64_t d, uint32_t thr) {
+  if (thr) return d > thr ? 4 : 0;
   return d <= 8 ? 0 : d <= 32 ? 1 : d <= 128 ? 2 : d <= 1024 ? 3 : 4;
 }
 
 __global__ __launch_bounds__(kBlock) void tier_count_kernel(const uint64_t* off, const uint32_t* deg,
-                                                            const uint8_t* ex, uint64_t n, uint32_t* counts,
-                                                            uint32_t nblocks) {
+                                                            const uint8_t* ex, uint64_t n, uint32_t thr,
+                                                            uint32_t* counts, uint32_t nblocks) {
   __shared__ uint32_t s[kTiers];
   if (threadIdx.x < kTiers) s[threadIdx.x] = 0;
   __syncthreads();
   uint64_t u = blockIdx.x * (uint64_t)kBlock + threadIdx.x;
   if (u < n && (!ex || ex[u])) {
     uint64_t d = deg ? deg[u] : off[u + 1] - off[u];
-    atomicAdd(&s[dev_tier(d)], 1u);
+    atomicAdd(&s[dev_tier(d, thr)], 1u);
   }
   __syncthreads();
   if (threadIdx.x < kTiers) counts[threadIdx.x * (uint64_t)nblocks + blockIdx.x] = s[threadIdx.x];
 }
 
 __global__ __launch_bounds__(kBlock) void tier_scatter_kernel(const uint64_t* off, const uint32_t* deg,
-                                                              const uint8_t* ex, uint64_t n, const uint32_t* base,
-                                                              uint32_t nblocks, uint32_t* ids) {
+                                                              const uint8_t* ex, uint64_t n, uint32_t thr,
+                                                              const uint32_t* base, uint32_t nblocks, uint32_t* ids) {
   __shared__ uint32_t s_wave[kBlock / 64][kTiers];
   uint64_t u = blockIdx.x * (uint64_t)kBlock + threadIdx.x;
   int t = -1;
   if (u < n && (!ex || ex[u])) {
     uint64_t d = deg ? deg[u] : off[u + 1] - off[u];
-    t = dev_tier(d);
+    t = dev_tier(d, thr);
   }
   int lane = __lane_id(), w = threadIdx.x / 64;
   uint32_t mine = 0;
@@ -773,8 +828,8 @@ __global__ __launch_bounds__(kBlock) void tier_scatter_kernel(const uint64_t* of
 // ------------------------------------------------------------ worklists
 // Split a queue into per-tier queues (tier t at tq + t * stride), by degree.
 __global__ __launch_bounds__(kBlock) void bin_queue_kernel(const uint32_t* q, const uint32_t* qlen,
-                                                           const uint32_t* deg, uint32_t* tq, uint64_t stride,
-                                                           uint32_t* tlen) {
+                                                           const uint32_t* deg, uint32_t thr, uint32_t* tq,
+                                                           uint64_t stride, uint32_t* tlen) {
   const uint64_t n = *qlen;
   for (uint64_t b = blockIdx.x * (uint64_t)kBlock; b < n; b += (uint64_t)gridDim.x * kBlock) {
     uint64_t i = b + threadIdx.x;
@@ -782,7 +837,7 @@ __global__ __launch_bounds__(kBlock) void bin_queue_kernel(const uint32_t* q, co
     uint32_t u = 0;
     if (i < n) {
       u = q[i];
-      t = dev_tier(deg[u]);
+      t = dev_tier(deg[u], thr);
     }
     int lane = __lane_id();
     for (int k = 0; k < kTiers; ++k) {
@@ -965,22 +1020,31 @@ void k_check_sorted(const uint32_t* u, const uint32_t* v, uint64_t n, uint32_t S
 
 template <class WT>
 static void move_tier(const MoveArgs& a, int tier, hipStream_t st) {
-  // One group per chunk (grid-stride over chunks).  With a device-side count
-  // the host sizes the grid for max_streams groups.
+  // max_streams groups (no more than there are chunks); chunks are handed
+  // out dynamically, so the grid only sets how many walk at once.
   uint64_t streams = a.max_streams ? a.max_streams : 1;
-  uint64_t groups;
-  if (a.n_dev) {
-    groups = streams;
-  } else {
-    uint64_t chunk = std::max<uint64_t>(std::max<uint64_t>(a.min_chunk, 1), (a.n + streams - 1) / streams);
-    groups = (a.n + chunk - 1) / chunk;
+  uint64_t groups = streams;
+  if (!a.n_dev) {
+    uint64_t chunk = a.chunk ? a.chunk : std::max<uint64_t>(1, (a.n + streams - 1) / streams);
+    groups = std::min<uint64_t>(streams, (a.n + chunk - 1) / chunk);
   }
   switch (tier) {
-    case 0: LV_LAUNCH((move_kernel<256, 8, 16, WT>), grid_for(groups, 32, 1u << 16), 256, st, a); break;
-    case 1: LV_LAUNCH((move_kernel<256, 16, 64, WT>), grid_for(groups, 16, 1u << 16), 256, st, a); break;
-    case 2: LV_LAUNCH((move_kernel<256, 64, 256, WT>), grid_for(groups, 4, 1u << 16), 256, st, a); break;
-    case 3: LV_LAUNCH((move_kernel<256, 256, 2048, WT>), grid_for(groups, 1, 1u << 14), 256, st, a); break;
-    default: LV_LAUNCH((move_kernel<1024, 1024, 0, WT>), grid_for(groups, 1, 256), 1024, st, a); break;
+    case 0: LV_LAUNCH((move_kernel<256, 8, 16, WT, MOVE_LDS>), grid_for(groups, 32, 1u << 16), 256, st, a); break;
+    case 1: LV_LAUNCH((move_kernel<256, 16, 64, WT, MOVE_LDS>), grid_for(groups, 16, 1u << 16), 256, st, a); break;
+    case 2: LV_LAUNCH((move_kernel<256, 64, 256, WT, MOVE_LDS>), grid_for(groups, 4, 1u << 16), 256, st, a); break;
+    case 3: LV_LAUNCH((move_kernel<256, 256, 2048, WT, MOVE_LDS>), grid_for(groups, 1, 1u << 14), 256, st, a); break;
+    case 4: {
+      // hubs walk in at most hub_streams streams (blocks)
+      MoveArgs h = a;
+      uint64_t hs = std::max<uint64_t>(1, std::min<uint64_t>(a.hub_streams ? a.hub_streams : kHubBlocks, kHubBlocks));
+      h.max_streams = std::min<uint64_t>(a.max_streams ? a.max_streams : 1, hs);
+      uint64_t hg = h.n_dev ? h.max_streams : std::min<uint64_t>(groups, h.max_streams);
+      LV_LAUNCH((move_kernel<1024, 1024, 0, WT, MOVE_HBM>), grid_for(hg, 1, kHubBlocks), 1024, st, h);
+      break;
+    }
+    default:
+      LV_LAUNCH((move_kernel<64, 64, kHybridLds, WT, MOVE_HYBRID>), grid_for(groups, 1, a.gblocks), 64, st, a);
+      break;
   }
 }
 void k_move(const MoveArgs& a, int tier, bool f64, hipStream_t st) {
@@ -1006,13 +1070,13 @@ void k_member_degrees(const uint32_t* cedg, uint64_t n, GView g, uint64_t* mdeg,
 void k_gather_yoff(const uint32_t* coff, const uint64_t* mscan, uint64_t C, uint64_t* yoff, hipStream_t st) {
   LV_LAUNCH(gather_yoff_kernel, grid_for(C + 1, kBlock * 4), kBlock, st, coff, mscan, C, yoff);
 }
-void k_tier_count(const uint64_t* off, const uint32_t* deg, const uint8_t* ex, uint64_t n, int, uint64_t,
+void k_tier_count(const uint64_t* off, const uint32_t* deg, const uint8_t* ex, uint64_t n, uint32_t thr,
                   uint32_t* block_counts, uint32_t nblocks, hipStream_t st) {
-  LV_LAUNCH(tier_count_kernel, nblocks, kBlock, st, off, deg, ex, n, block_counts, nblocks);
+  LV_LAUNCH(tier_count_kernel, nblocks, kBlock, st, off, deg, ex, n, thr, block_counts, nblocks);
 }
-void k_tier_scatter(const uint64_t* off, const uint32_t* deg, const uint8_t* ex, uint64_t n, int, uint64_t,
+void k_tier_scatter(const uint64_t* off, const uint32_t* deg, const uint8_t* ex, uint64_t n, uint32_t thr,
                     const uint32_t* block_base, uint32_t nblocks, uint32_t* ids, hipStream_t st) {
-  LV_LAUNCH(tier_scatter_kernel, nblocks, kBlock, st, off, deg, ex, n, block_base, nblocks, ids);
+  LV_LAUNCH(tier_scatter_kernel, nblocks, kBlock, st, off, deg, ex, n, thr, block_base, nblocks, ids);
 }
 
 template <class WT>
@@ -1037,9 +1101,9 @@ void k_t4_sizes(const uint64_t* yoff, const uint32_t* list, uint64_t n, uint64_t
                 uint64_t* chunks, hipStream_t st) {
   LV_LAUNCH(t4_sizes_kernel, grid_for(n + 1, kBlock), kBlock, st, yoff, list, n, C, chunk, caps, chunks);
 }
-void k_bin_queue(const uint32_t* q, const uint32_t* qlen, uint64_t qcap, const uint32_t* deg, uint32_t* tq,
-                 uint64_t stride, uint32_t* tlen, hipStream_t st) {
-  LV_LAUNCH(bin_queue_kernel, grid_for(qcap, kBlock, 2048), kBlock, st, q, qlen, deg, tq, stride, tlen);
+void k_bin_queue(const uint32_t* q, const uint32_t* qlen, uint64_t qcap, const uint32_t* deg, uint32_t thr,
+                 uint32_t* tq, uint64_t stride, uint32_t* tlen, hipStream_t st) {
+  LV_LAUNCH(bin_queue_kernel, grid_for(qcap, kBlock, 2048), kBlock, st, q, qlen, deg, thr, tq, stride, tlen);
 }
 void k_max_deg(const uint32_t* ids, uint64_t n, const uint32_t* deg, uint32_t* out, hipStream_t st) {
   if (n) LV_LAUNCH(max_deg_kernel, grid_for(n, kBlock, 1024), kBlock, st, ids, n, deg, out);

@@ -34,7 +34,8 @@ PATH_SWEEP, PATH_WORKLIST = 0, 1
 # Every symbol include/louvain_hip.h declares.
 EXPORTS = [
     "lv_options_default", "lv_status_string", "lv_context_create", "lv_context_destroy", "lv_last_error",
-    "lv_context_set_path", "lv_context_set_schedule", "lv_context_set_profiling", "lv_stats_get",
+    "lv_context_set_path", "lv_context_set_schedule", "lv_context_set_pick_less", "lv_context_set_slabs", "lv_context_set_hub_streams", "lv_context_set_hub_threshold",
+    "lv_context_set_profiling", "lv_stats_get",
     "lv_stats_reset", "lv_louvain_static", "lv_louvain_naive_dynamic", "lv_louvain_delta_screening",
     "lv_louvain_dynamic_frontier", "lv_dgraph_create", "lv_dgraph_destroy", "lv_dgraph_span", "lv_dgraph_edges",
     "lv_dbatch_create", "lv_dbatch_destroy", "lv_dstate_create", "lv_dstate_destroy", "lv_dstate_upload",
@@ -188,9 +189,22 @@ class Context:
     def set_path(self, path):
         self._check(self.lib.lv_context_set_path(self.h, int(path)), "lv_context_set_path")
 
-    def set_schedule(self, stream_fraction=0.002, min_streams=1, max_streams=1 << 20, hubs_first=True):
+    def set_hub_streams(self, streams=256):
+        self._check(self.lib.lv_context_set_hub_streams(self.h, C.c_uint64(streams)), "lv_context_set_hub_streams")
+
+    def set_hub_threshold(self, degree=4096):
+        self._check(self.lib.lv_context_set_hub_threshold(self.h, C.c_uint32(degree)), "lv_context_set_hub_threshold")
+
+    def set_slabs(self, slabs=1):
+        self._check(self.lib.lv_context_set_slabs(self.h, C.c_uint64(slabs)), "lv_context_set_slabs")
+
+    def set_pick_less(self, iterations=0):
+        self._check(self.lib.lv_context_set_pick_less(self.h, int(iterations)), "lv_context_set_pick_less")
+
+    def set_schedule(self, stream_fraction=0.002, min_streams=1, max_streams=1 << 20, chunk=0, order=2):
+        """order: 0 degree tiers lowest first, 1 highest first, 2 ascending ids, no tiers."""
         self._check(self.lib.lv_context_set_schedule(self.h, C.c_double(stream_fraction), C.c_uint64(min_streams),
-                                                     C.c_uint64(max_streams), int(bool(hubs_first))),
+                                                     C.c_uint64(max_streams), C.c_uint64(chunk), int(order)),
                     "lv_context_set_schedule")
 
     def set_profiling(self, on=True):

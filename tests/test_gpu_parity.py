@@ -259,3 +259,43 @@ def test_out_of_range_batch_rejected(ctx):
     with pytest.raises(lvhip.LvError) as e:
         ctx.louvain("dynamic-frontier", y, d, bad, f["static_memb"], f["static_vtot"], f["static_ctot"])
     assert e.value.status == -1
+
+
+@pytest.mark.parametrize("name", INTEGER_FIXTURES)
+def test_single_stream_reproduces_reference_bit_exact(ctx, name):
+    """One stream, ascending ids, every vertex in the id-order kernel: the
+    GPU's first local-moving pass is the reference's sequential sweep
+    (inc/louvain.hxx:594-605 at one thread), so membership, K, Sigma_tot,
+    iterations and affected count equal the golden vectors bit for bit."""
+    import lvhip
+    _say("case", name)
+    f = F.load(name)
+    x, y, d, i = _g(f)
+    o = lvhip.LouvainOptions(*[t(v) for t, v in zip((int, float, float, float, float, int, int), f["opts"])])
+    o.maxPasses = 1
+    from oracle import Options
+    oo = Options(o.repeat, o.resolution, o.tolerance, o.aggregationTolerance, o.toleranceDrop, o.maxIterations, 1)
+    from oracle import Oracle
+    orc = Oracle()
+    orc.set_threads(1)
+    ctx.set_schedule(1e-12, 1, 1, 16, 2)
+    ctx.set_hub_threshold(1 << 30)
+    try:
+        for approach in ("static", "naive-dynamic", "dynamic-frontier"):
+            if approach == "static":
+                r = ctx.louvain(approach, x, o=o)
+                ref = orc.louvain(approach, x, options=oo)
+            else:
+                ctx.set_path(0)
+                r = ctx.louvain(approach, y, d, i, f["static_memb"], f["static_vtot"], f["static_ctot"], o)
+                ref = orc.louvain(approach, y, d, i, f["static_memb"], f["static_vtot"], f["static_ctot"], oo)
+            _say(f"  {approach}: it={r.iterations}/{ref.iterations} aff={r.affectedVertices}/{ref.affected_vertices}")
+            assert np.array_equal(r.membership, ref.membership), approach
+            assert np.array_equal(r.vertexWeight, ref.vertex_weight), approach
+            assert np.array_equal(r.communityWeight, ref.community_weight), approach
+            assert (r.iterations, r.passes, r.affectedVertices) == (ref.iterations, ref.passes,
+                                                                      ref.affected_vertices), approach
+    finally:
+        ctx.set_schedule()
+        ctx.set_hub_threshold(4096)
+        ctx.set_path(1)
