@@ -111,11 +111,25 @@ def main():
     state0 = ctx.device_state(S)
     state = ctx.device_state(S)
     log(f"uploaded {len(dgraphs)} graphs + {len(dbatches)} batches ({time.time() - t:.1f}s)")
+    # GPU Static Louvain on the original graph (reported), and the starting
+    # state of the DF chain: the reference's own Static result, as in
+    # main.cxx:89 (louvainStaticOmp), from the oracle on the host cores.
     t = time.time()
     rs = ctx.run(lvhip.LV_STATIC, dgraphs[0], None, state0)
     q_static = ctx.dmodularity(dgraphs[0], state0)
-    log(f"static: Q={q_static:.6f} iters={rs.iterations} passes={rs.passes} time={rs.time:.1f} ms "
+    log(f"GPU static: Q={q_static:.6f} iters={rs.iterations} passes={rs.passes} time={rs.time:.1f} ms "
         f"(wall {time.time() - t:.1f}s)")
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from oracle import Oracle
+    orc = Oracle()
+    cores = len(os.sched_getaffinity(0))
+    orc.set_threads(cores)
+    t = time.time()
+    r0 = orc.louvain("static", graphs[0].view())
+    state0.upload(r0.membership, r0.vertex_weight, r0.community_weight)
+    q_static_ref = ctx.dmodularity(dgraphs[0], state0)
+    log(f"reference static (oracle, {cores} threads): Q={q_static_ref:.6f} iters={r0.iterations} "
+        f"passes={r0.passes} time={r0.time:.1f} ms (wall {time.time() - t:.1f}s)")
 
     # Quality chain (untimed): the GPU state before every batch, for the oracle.
     ctx.set_path(lvhip.PATH_WORKLIST)
@@ -154,11 +168,6 @@ def main():
     cpu = None
     dq_max = None
     if not args.no_cpu:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        from oracle import Oracle
-        orc = Oracle()
-        cores = len(os.sched_getaffinity(0))
-        orc.set_threads(cores)
         dqs, ctime, cedges = [], 0.0, 0
         for k in range(W, W + K):
             q, vt, ct = prev[k]
@@ -166,10 +175,11 @@ def main():
             y = graphs[k + 1].view()
             r = orc.louvain("dynamic-frontier", y, d, i, q, vt, ct)
             q_ref = orc.modularity(y, r.membership)
-            dqs.append(abs(q_gpu[k] - q_ref))
+            dqs.append(q_gpu[k] - q_ref)
             ctime += r.time
             cedges += graphs[k + 1].edges
-        dq_max = max(dqs)
+        dq_max = max(abs(x) for x in dqs)
+        log("dQ per batch (GPU - reference): " + " ".join(f"{x:+.2e}" for x in dqs))
         cpu = {"value": cedges / (ctime * 1e-3), "unit": "edges/s", "cores": cores, "kind": "port",
                "sample": f"oracle DF (C + OpenMP) on the {K} timed batches, same inputs and GPU carry-in "
                          f"state, reference timed region; {ctime:.1f} ms total"}
@@ -184,7 +194,10 @@ def main():
                                f"{args.batch_fraction:g}|E| 80/20 insert/delete, seed {args.seed}",
                    "vertices": S, "edges": graphs[0].edges, "batch_updates": B, "parallelism": "1 GPU"},
         "p50_batch_ms": percentile(lat, 50), "p99_batch_ms": percentile(lat, 99),
-        "dq_max": dq_max, "static_modularity": q_static, "plain_gpu_path": plain,
+        "dq_max": dq_max, "static": {"gpu_modularity": q_static, "gpu_ms": rs.time,
+                                     "reference_modularity": q_static_ref, "reference_ms": r0.time,
+                                     "reference_threads": cores},
+        "plain_gpu_path": plain,
         "roofline": roofline, "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -153,6 +153,7 @@ struct lv_dgraph {
   double M2 = 0;  // 2M, edgeWeightOmp
   TierLists tiers;  // five degree tiers
   TierLists hubs;   // tier 4 = degree > kHubMin (id-order mode), ascending
+  uint64_t hub_slots = 0;
   GView view() const { return GView{S, off, deg, keys, w, ex}; }
 };
 
@@ -235,6 +236,23 @@ uint32_t max_degree_tier4(lv_context* c, const TierLists& t, const uint32_t* deg
   return c->read1(out);
 }
 
+// Sum over the hubs (tier 4 of t) of nextpow2(2 deg): the table slots a hub
+// batch over all of them needs.
+uint64_t hub_slot_total(lv_context* c, const TierLists& t, const uint32_t* deg) {
+  uint64_t nh = t.start[5] - t.start[4];
+  if (!nh) return 0;
+  std::vector<uint32_t> ids(nh), d(nh);
+  c->read(ids.data(), t.ids + t.start[4], nh);
+  uint32_t* gd = c->ws<uint32_t>("hub.deg", nh);
+  uint32_t* gi = c->ws<uint32_t>("hub.ids", nh);
+  h2d(c, gi, ids.data(), nh);
+  k_gather_u32(deg, gi, nh, gd, c->st);
+  c->read(d.data(), gd, nh);
+  uint64_t tot = 0;
+  for (uint64_t k = 0; k < nh; ++k) tot += next_pow2(2 * (uint64_t)d[k]);
+  return tot;
+}
+
 // ------------------------------------------------------------ local-moving
 struct Level {
   GView g;
@@ -243,6 +261,7 @@ struct Level {
   uint64_t order;          // live vertices
   uint32_t maxdeg4;        // max degree in tier 4 of `tiers`
   const TierLists* hubs;   // degree > kHubMin, ascending (id order)
+  uint64_t hub_slots;      // hub-batch table slots for all of them
 };
 
 struct Counters {
@@ -251,12 +270,15 @@ struct Counters {
 
 // Launch one tier of the local-moving kernel, bracketed by events when the
 // context profiles (per-launch durations on the context's own stream).
+constexpr int kTierHubBatch = 6;  // launch_move: hubs through k_hub_batch (stats slot 4)
+
 void launch_move(lv_context* c, const MoveArgs& a, int tier, bool f64) {
   k_fill_u64(reinterpret_cast<uint64_t*>(a.work), 0, 1, c->st);
   hipEvent_t e0 = nullptr;
   if (c->profiling) e0 = c->record();
-  k_move(a, tier, f64, c->st);
-  if (c->profiling) c->prof.push_back({tier, {e0, c->record()}});
+  if (tier == kTierHubBatch) k_hub_batch(a, c->st);
+  else k_move(a, tier, f64, c->st);
+  if (c->profiling) c->prof.push_back({tier == kTierHubBatch ? 4 : (tier == kTierHybrid ? 0 : tier), {e0, c->record()}});
 }
 
 uint64_t level_streams(const lv_context* c, uint64_t order) {
@@ -292,6 +314,22 @@ MoveArgs base_args(lv_context* c, const Level& lv, uint32_t* vcom, const double*
     a.hpos = c->ws<uint32_t>("h.pos", slots);
     a.hval = c->ws<double>("h.val", slots);
   }
+  uint64_t nh = lv.hubs ? lv.hubs->start[5] - lv.hubs->start[4] : 0;
+  if (c->tier_order == 2 && nh) {
+    // hub batches: per-hub scratch and a table slab for every hub of the level
+    a.f64_weights = lv.f64;
+    a.hb_cap = c->ws<uint64_t>("hb.cap", nh + 1);
+    a.hb_chunks = c->ws<uint64_t>("hb.chunks", nh + 1);
+    a.hb_tot = c->ws<uint64_t>("hb.tot", 2);
+    a.hb_cu = c->ws<uint32_t>("hb.cu", nh);
+    a.hb_vd = c->ws<double>("hb.vd", nh);
+    a.hb_sd = c->ws<double>("hb.sd", nh);
+    a.hb_emax = c->ws<unsigned long long>("hb.emax", nh);
+    a.hb_bpos = c->ws<uint32_t>("hb.bpos", nh);
+    a.hb_key = c->ws<uint32_t>("hb.key", lv.hub_slots);
+    a.hb_pos = c->ws<uint32_t>("hb.pos", lv.hub_slots);
+    a.hb_val = c->ws<double>("hb.val", lv.hub_slots);
+  }
   if (c->tier_order == 2) {
     // id-order kernel: one HBM region per stream holding nextpow2(2 deg)
     // slots for every degree it runs (<= hub_min)
@@ -318,7 +356,7 @@ void move_sweep(lv_context* c, MoveArgs a, const Level& lv) {
       a.list = lv.hubs->ids + lv.hubs->start[4];
       a.n = nh;
       if (cnt0) a.cnt = cnt0 + (size_t)4 * CNT_N;
-      launch_move(c, a, 4, lv.f64);
+      launch_move(c, a, kTierHubBatch, lv.f64);
     }
     // the id range in `slabs` consecutive slabs, one launch each: a slab sees
     // every move of the slabs before it
@@ -426,7 +464,7 @@ int local_move_worklist(lv_context* c, const Level& lv, uint32_t* vcom, const do
           a.list = tq + (size_t)4 * stride;
           a.n_dev = tlen + 4;
           a.cnt = cnt0 + (size_t)4 * CNT_N;
-          launch_move(c, a, 4, lv.f64);
+          launch_move(c, a, kTierHubBatch, lv.f64);
         }
         a.list = tq;
         a.n_dev = tlen;
@@ -647,7 +685,7 @@ void invoke(lv_context* c, int approach, const lv_dgraph* x, const lv_dbatch* b,
   hipEvent_t e2 = c->record();
   hipEvent_t t1 = e2;
   bool t1_set = false;
-  Level lv0{x->view(), x->f64, &x->tiers, x->order, x->tiers.maxdeg, &x->hubs};
+  Level lv0{x->view(), x->f64, &x->tiers, x->order, x->tiers.maxdeg, &x->hubs, x->hub_slots};
   Level cur = lv0;
   TierLists ytiers, yhubs;
   uint64_t ySpan = 0;
@@ -686,14 +724,16 @@ void invoke(lv_context* c, int approach, const lv_dgraph* x, const lv_dbatch* b,
     GView yg{ySpan, y.off, y.deg, y.keys, y.w, nullptr};
     k_initialize(yg, vcom, ctot, vtot, c->st);
     k_fill_u8(vaff, 1, CN, c->st);
+    uint64_t yslots = 0;
     if (c->tier_order == 2) {
       build_tiers(c, "y.hubs", y.off, y.deg, nullptr, CN, yhubs, nullptr, kHubMin);
       yhubs.maxdeg = max_degree_tier4(c, yhubs, y.deg);
+      yslots = hub_slot_total(c, yhubs, y.deg);
     } else {
       build_tiers(c, "y.tiers", y.off, y.deg, nullptr, CN, ytiers);
       ytiers.maxdeg = max_degree_tier4(c, ytiers, y.deg);
     }
-    cur = Level{yg, true, &ytiers, CN, ytiers.maxdeg, &yhubs};
+    cur = Level{yg, true, &ytiers, CN, ytiers.maxdeg, &yhubs, yslots};
     E /= o->tolerance_drop;
   }
   if (p > 1) k_lookup(ucom, S, vcom, c->st);
@@ -831,6 +871,7 @@ lv_dgraph* upload_graph(lv_context* c, const lv_graph* g) {
     d->hubs.ids = dalloc<uint32_t>(std::max<uint64_t>(d->S, 1));
     build_tiers(c, "g.hubs", d->off, d->deg, d->ex, d->S, d->hubs, d->hubs.ids, kHubMin);
     d->hubs.maxdeg = max_degree_tier4(c, d->hubs, d->deg);
+    d->hub_slots = hub_slot_total(c, d->hubs, d->deg);
     c->check();
   } catch (...) {
     lv_dgraph_destroy(d);
@@ -1320,7 +1361,7 @@ int lv_choose_community(lv_context* c, const lv_graph* g, const uint32_t* vcom, 
     double* oe = c->ws<double>("ph.be", S);
     k_fill_u32(oc, 0, S, c->st);
     k_fill_f64(oe, 0, S, c->st);
-    Level lv{w.g->view(), w.g->f64, &w.g->tiers, w.g->order, w.g->tiers.maxdeg, &w.g->hubs};
+    Level lv{w.g->view(), w.g->f64, &w.g->tiers, w.g->order, w.g->tiers.maxdeg, &w.g->hubs, w.g->hub_slots};
     MoveArgs a = base_args(c, lv, w.s->memb, w.s->vtot, w.s->ctot, nullptr, nullptr, M, R);
     a.choose_only = 1;
     a.chunk = 1;
@@ -1371,7 +1412,7 @@ int lv_aggregate(lv_context* c, const lv_graph* g, const uint32_t* vcom, uint64_
     h2d(c, d, vcom, S);
     SuperGraph z = super_graph(c, "ph.z", std::max<uint64_t>(C, 1), E);
     double* ztot = c->ws<double>("ph.ztot", std::max<uint64_t>(C, 1));
-    Level lv{w.g->view(), w.g->f64, &w.g->tiers, w.g->order, w.g->tiers.maxdeg, &w.g->hubs};
+    Level lv{w.g->view(), w.g->f64, &w.g->tiers, w.g->order, w.g->tiers.maxdeg, &w.g->hubs, w.g->hub_slots};
     aggregate(c, lv, d, C, z, ztot);
     d2h(c, zoff, z.off, C + 1);
     d2h(c, zdeg, z.deg, C);

@@ -102,7 +102,25 @@ struct MoveArgs {
   uint32_t hub_min;            // degrees above this go to the hub kernel
   uint64_t hub_streams;        // hub kernel: hubs processed at once (<= kHubBlocks)
   const int* done;             // device convergence flag: skip work when set
+  // hub batches (k_hub_batch): per-hub scratch, nh_max entries each
+  uint64_t* hb_cap;            // table slots per hub, then exclusive scan (nh_max + 1)
+  uint64_t* hb_chunks;         // edge chunks per hub, then exclusive scan (nh_max + 1)
+  uint64_t* hb_tot;            // [0] total slots, [1] total chunks of the batch
+  uint32_t* hb_cu;             // hub's community
+  double* hb_vd;               // hub's weight to its own community
+  double* hb_sd;               // Sigma_tot of the hub's community
+  unsigned long long* hb_emax; // best gain (bits of a positive double)
+  uint32_t* hb_bpos;           // first position of the best community
+  int f64_weights;             // hub batch: the graph's weights are f64
+  uint32_t* hb_key;            // table slab
+  uint32_t* hb_pos;
+  double* hb_val;
 };
+
+// Hub batch: every listed vertex (hub) at once, each with its own HBM table
+// and its edges split over many blocks; the reference's per-vertex steps in
+// separate device passes (table build, best gain, move, neighbour flags).
+void k_hub_batch(const MoveArgs& a, hipStream_t st);
 
 // Kernel launchers (lv_kernels.hip).  All run on `st`.
 void k_fill_u8(uint8_t* a, uint8_t v, uint64_t n, hipStream_t st);
@@ -170,6 +188,7 @@ void k_bin_queue(const uint32_t* q, const uint32_t* qlen, uint64_t qcap, const u
                  uint32_t* tq, uint64_t stride, uint32_t* tlen, hipStream_t st);
 void k_max_deg(const uint32_t* ids, uint64_t n, const uint32_t* deg, uint32_t* out, hipStream_t st);
 void k_max_u32(const uint32_t* a, uint64_t n, uint32_t* out, hipStream_t st);
+void k_gather_u32(const uint32_t* a, const uint32_t* idx, uint64_t n, uint32_t* out, hipStream_t st);
 void k_compact_flags(const uint8_t* f, uint64_t n, uint32_t* q, uint32_t* qlen, hipStream_t st);
 void k_modularity_parts(GView g, bool f64, const uint32_t* memb, double* cin, double* ctot, hipStream_t st);
 void k_modularity_sum(const double* cin, const double* ctot, uint64_t n, double M, double R, double* out,

@@ -595,6 +595,215 @@ __global__ __launch_bounds__(BLOCK) void move_kernel(MoveArgs a) {
   }
 }
 
+// ------------------------------------------------------- hub batches
+// Steps of louvainMoveOmpW's body (inc/louvain.hxx:599-605) for a batch of
+// high-degree vertices processed together: each hub has its own HBM table
+// of nextpow2(2 deg) slots and its edges are cut into kHubChunk-edge chunks
+// spread over the grid.  The best community is the largest positive gain,
+// ties to the earliest first position (inc/louvain.hxx:461), found by two
+// passes (max gain bits, then min position); its id is the community of the
+// neighbour at that position.
+constexpr uint64_t kHubChunk = 2048;
+constexpr int kHubPrepThreads = 1024;
+
+// One block: which hubs are active, their table sizes and chunk counts,
+// exclusive scans of both, and the batch totals.
+__global__ __launch_bounds__(kHubPrepThreads) void hub_prep_kernel(MoveArgs a) {
+  __shared__ uint64_t s_cap[kHubPrepThreads], s_chk[kHubPrepThreads];
+  const uint64_t n = a.n_dev ? (uint64_t)*a.n_dev : a.n;
+  const uint64_t per = (n + kHubPrepThreads - 1) / kHubPrepThreads;
+  const uint64_t lo = threadIdx.x * per, hi = min(n, lo + per);
+  uint64_t cap_sum = 0, chk_sum = 0;
+  for (uint64_t k = lo; k < hi; ++k) {
+    uint32_t u = a.list[k];
+    uint64_t d = a.g.deg[u], cap = 0, chk = 0;
+    bool act = d > a.hub_min && (!a.g.ex || a.g.ex[u]) &&
+               (a.choose_only || (ld_dev(&a.vaff[u]) && (!a.fa || a.fa[u])));
+    if (act) {
+      cap = 2;
+      while (cap < 2 * d) cap <<= 1;
+      chk = (d + kHubChunk - 1) / kHubChunk;
+      if (a.stamp) st_dev(&a.stamp[u], a.serial);
+    }
+    a.hb_cap[k] = cap;
+    a.hb_chunks[k] = chk;
+    cap_sum += cap;
+    chk_sum += chk;
+  }
+  s_cap[threadIdx.x] = cap_sum;
+  s_chk[threadIdx.x] = chk_sum;
+  __syncthreads();
+  if (threadIdx.x == 0) {  // scan of the per-thread sums (1024 entries)
+    uint64_t c0 = 0, c1 = 0;
+    for (int t = 0; t < kHubPrepThreads; ++t) {
+      uint64_t x0 = s_cap[t], x1 = s_chk[t];
+      s_cap[t] = c0;
+      s_chk[t] = c1;
+      c0 += x0;
+      c1 += x1;
+    }
+    a.hb_tot[0] = c0;
+    a.hb_tot[1] = c1;
+    a.hb_cap[n] = c0;
+    a.hb_chunks[n] = c1;
+  }
+  __syncthreads();
+  uint64_t c0 = s_cap[threadIdx.x], c1 = s_chk[threadIdx.x];
+  for (uint64_t k = lo; k < hi; ++k) {
+    uint64_t x0 = a.hb_cap[k], x1 = a.hb_chunks[k];
+    a.hb_cap[k] = c0;
+    a.hb_chunks[k] = c1;
+    c0 += x0;
+    c1 += x1;
+  }
+}
+
+__device__ __forceinline__ uint64_t find_owner(const uint64_t* scan, uint64_t n, uint64_t x) {
+  uint64_t lo = 0, hi = n - 1;  // largest k with scan[k] <= x
+  while (lo < hi) {
+    uint64_t mid = (lo + hi + 1) >> 1;
+    if (scan[mid] <= x) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+__global__ void hub_clear_kernel(MoveArgs a) {
+  const uint64_t total = a.hb_tot[0];
+  for (uint64_t s = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; s < total; s += (uint64_t)gridDim.x * blockDim.x) {
+    a.hb_key[s] = kEmpty;
+    a.hb_pos[s] = kNoPos;
+    a.hb_val[s] = 0;
+  }
+}
+
+template <class WT>
+__global__ __launch_bounds__(kBlock) void hub_insert_kernel(MoveArgs a) {
+  const uint64_t n = a.n_dev ? (uint64_t)*a.n_dev : a.n;
+  const uint64_t nchunks = a.hb_tot[1];
+  const WT* W = (const WT*)a.g.w;
+  for (uint64_t q = blockIdx.x; q < nchunks; q += gridDim.x) {
+    uint64_t k = find_owner(a.hb_chunks, n, q);
+    uint32_t u = a.list[k];
+    uint64_t o = a.g.off[u], d = a.g.deg[u];
+    uint64_t i0 = (q - a.hb_chunks[k]) * kHubChunk, i1 = min(d, i0 + kHubChunk);
+    uint64_t base = a.hb_cap[k], cap = a.hb_cap[k + 1] - base;
+    for (uint64_t i = i0 + threadIdx.x; i < i1; i += kBlock) {
+      uint32_t v = a.g.keys[o + i];
+      if (v == u) continue;
+      tab_add(a.hb_key + base, a.hb_pos + base, a.hb_val + base, cap, ld_dev(&a.vcom[v]), (double)W[o + i],
+              (uint32_t)i);
+    }
+  }
+}
+
+// Per hub: own community, its weight in the table, Sigma_tot; reset the best.
+__global__ void hub_self_kernel(MoveArgs a) {
+  const uint64_t n = a.n_dev ? (uint64_t)*a.n_dev : a.n;
+  for (uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; k < n; k += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t base = a.hb_cap[k], cap = a.hb_cap[k + 1] - base;
+    a.hb_emax[k] = 0;
+    a.hb_bpos[k] = kNoPos;
+    if (!cap) continue;
+    uint32_t u = a.list[k];
+    uint32_t cu = ld_dev(&a.vcom[u]);
+    a.hb_cu[k] = cu;
+    a.hb_vd[k] = tab_get(a.hb_key + base, a.hb_val + base, cap, cu);
+    a.hb_sd[k] = ld_dev(&a.ctot[cu]);
+  }
+}
+
+// pass 0: largest positive gain per hub; pass 1: earliest position among them.
+template <int PASS>
+__global__ void hub_best_kernel(MoveArgs a) {
+  const uint64_t n = a.n_dev ? (uint64_t)*a.n_dev : a.n;
+  const uint64_t total = a.hb_tot[0];
+  for (uint64_t s = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; s < total; s += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t c = a.hb_key[s];
+    if (c == kEmpty) continue;
+    uint64_t k = find_owner(a.hb_cap, n, s);
+    uint32_t cu = a.hb_cu[k];
+    if (c == cu) continue;
+    if (a.pick_less && c > cu) continue;
+    uint32_t u = a.list[k];
+    double e = delta_q(a.hb_val[s], a.hb_vd[k], a.vtot[u], ld_dev(&a.ctot[c]), a.hb_sd[k], a.M, a.R);
+    if (!(e > 0)) continue;
+    unsigned long long bits = (unsigned long long)__double_as_longlong(e);
+    if (PASS == 0) atomicMax(&a.hb_emax[k], bits);
+    else if (bits == a.hb_emax[k]) atomicMin(&a.hb_bpos[k], a.hb_pos[s]);
+  }
+}
+
+// Per hub: the move (or the choice, in choose-only mode) and its gain.
+__global__ void hub_apply_kernel(MoveArgs a) {
+  const uint64_t n = a.n_dev ? (uint64_t)*a.n_dev : a.n;
+  double el = 0;
+  unsigned long long cv = 0, ce = 0, cm = 0, cme = 0;
+  for (uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; k < n; k += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t cap = a.hb_cap[k + 1] - a.hb_cap[k];
+    if (!cap) continue;
+    uint32_t u = a.list[k];
+    double e = a.hb_bpos[k] == kNoPos ? 0.0 : __longlong_as_double((long long)a.hb_emax[k]);
+    uint32_t bc = 0;
+    if (a.hb_bpos[k] != kNoPos) bc = ld_dev(&a.vcom[a.g.keys[a.g.off[u] + a.hb_bpos[k]]]);
+    uint32_t d = a.g.deg[u];
+    if (a.choose_only) {
+      a.out_c[u] = bc;
+      a.out_e[u] = e;
+      continue;
+    }
+    if (bc != 0) {
+      uint32_t cu = a.hb_cu[k];
+      double Ku = a.vtot[u];
+      unsafeAtomicAdd(&a.ctot[cu], -Ku);
+      unsafeAtomicAdd(&a.ctot[bc], Ku);
+      st_dev(&a.vcom[u], bc);
+      ++cm;
+      cme += d;
+    } else {
+      a.hb_bpos[k] = kNoPos;  // tells the flag pass there was no move
+    }
+    st_dev(&a.vaff[u], (uint8_t)0);
+    el += e;
+    ++cv;
+    ce += d;
+  }
+  if (a.choose_only) return;
+  double t = block_sum<kBlock>(el);
+  if (threadIdx.x == 0 && t != 0) unsafeAtomicAdd(a.el, t);
+  if (a.cnt) {
+    unsigned long long v = block_sum_u64<kBlock>(cv), e2 = block_sum_u64<kBlock>(ce);
+    unsigned long long m = block_sum_u64<kBlock>(cm), me = block_sum_u64<kBlock>(cme);
+    if (threadIdx.x == 0) {
+      if (v) atomicAdd(&a.cnt[CNT_VERTICES], v);
+      if (e2) atomicAdd(&a.cnt[CNT_EDGES], e2);
+      if (m) atomicAdd(&a.cnt[CNT_MOVES], m);
+      if (me) atomicAdd(&a.cnt[CNT_MOVED_EDGES], me);
+    }
+  }
+}
+
+// Neighbours of the hubs that moved are flagged (chunks spread over the grid).
+__global__ __launch_bounds__(kBlock) void hub_flag_kernel(MoveArgs a) {
+  const uint64_t n = a.n_dev ? (uint64_t)*a.n_dev : a.n;
+  const uint64_t nchunks = a.hb_tot[1];
+  unsigned long long cand = 0;
+  for (uint64_t q = blockIdx.x; q < nchunks; q += gridDim.x) {
+    uint64_t k = find_owner(a.hb_chunks, n, q);
+    if (a.hb_bpos[k] == kNoPos) continue;  // uniform per block
+    uint32_t u = a.list[k];
+    uint64_t o = a.g.off[u], d = a.g.deg[u];
+    uint64_t i0 = (q - a.hb_chunks[k]) * kHubChunk, i1 = min(d, i0 + kHubChunk);
+    for (uint64_t b = i0; b < i1; b += kBlock) {  // uniform trip count: wave ballots in flag_vertex
+      uint64_t i = b + threadIdx.x;
+      bool on = i < i1;
+      uint32_t v = on ? a.g.keys[o + i] : u;
+      if (on && v != u) flag_vertex(a.vaff, v, u, a);
+    }
+  }
+  (void)cand;
+}
+
 // ------------------------------------------------------------ renumbering
 // louvainCommunityExistsOmpW (inc/louvain.hxx:668-682): flags must be zero.
 __global__ void exists_kernel(GView g, const uint32_t* vcom, uint32_t* flags) {
@@ -871,6 +1080,11 @@ __global__ __launch_bounds__(kBlock) void max_u32_kernel(const uint32_t* a, uint
   if (__lane_id() == 0 && m) atomicMax(out, m);
 }
 
+__global__ void gather_u32_kernel(const uint32_t* a, const uint32_t* idx, uint64_t n, uint32_t* out) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    out[i] = a[idx[i]];
+}
+
 // Queue every id whose flag is set (wave-aggregated appends).
 __global__ __launch_bounds__(kBlock) void compact_flags_kernel(const uint8_t* f, uint64_t n, uint32_t* q,
                                                                uint32_t* qlen) {
@@ -1051,6 +1265,18 @@ void k_move(const MoveArgs& a, int tier, bool f64, hipStream_t st) {
   if (f64) move_tier<double>(a, tier, st);
   else move_tier<float>(a, tier, st);
 }
+void k_hub_batch(const MoveArgs& a, hipStream_t st) {
+  const uint32_t grid = 2048;
+  LV_LAUNCH(hub_prep_kernel, 1, kHubPrepThreads, st, a);
+  LV_LAUNCH(hub_clear_kernel, grid, kBlock, st, a);
+  if (a.g.w && a.f64_weights) LV_LAUNCH(hub_insert_kernel<double>, grid, kBlock, st, a);
+  else LV_LAUNCH(hub_insert_kernel<float>, grid, kBlock, st, a);
+  LV_LAUNCH(hub_self_kernel, 64, kBlock, st, a);
+  LV_LAUNCH(hub_best_kernel<0>, grid, kBlock, st, a);
+  LV_LAUNCH(hub_best_kernel<1>, grid, kBlock, st, a);
+  LV_LAUNCH(hub_apply_kernel, 16, kBlock, st, a);
+  if (!a.choose_only) LV_LAUNCH(hub_flag_kernel, grid, kBlock, st, a);
+}
 void k_exists(GView g, const uint32_t* vcom, uint32_t* flags, hipStream_t st) {
   if (g.S) LV_LAUNCH(exists_kernel, grid_for(g.S, kBlock * 4), kBlock, st, g, vcom, flags);
 }
@@ -1110,6 +1336,9 @@ void k_max_deg(const uint32_t* ids, uint64_t n, const uint32_t* deg, uint32_t* o
 }
 void k_max_u32(const uint32_t* a, uint64_t n, uint32_t* out, hipStream_t st) {
   if (n) LV_LAUNCH(max_u32_kernel, grid_for(n, kBlock * 8, 4096), kBlock, st, a, n, out);
+}
+void k_gather_u32(const uint32_t* a, const uint32_t* idx, uint64_t n, uint32_t* out, hipStream_t st) {
+  if (n) LV_LAUNCH(gather_u32_kernel, grid_for(n, kBlock, 1024), kBlock, st, a, idx, n, out);
 }
 void k_compact_flags(const uint8_t* f, uint64_t n, uint32_t* q, uint32_t* qlen, hipStream_t st) {
   if (n) LV_LAUNCH(compact_flags_kernel, grid_for(n, kBlock, 1u << 16), kBlock, st, f, n, q, qlen);
