@@ -141,10 +141,12 @@ int lv_context_set_slabs(lv_context* ctx, uint64_t slabs);
 /* Hubs (degree > 4096) are processed by the hub kernel, at most `streams`
  * (1..256) at once, before the other vertices of each sweep.  Default 256. */
 int lv_context_set_hub_streams(lv_context* ctx, uint64_t streams);
-/* Degree above which a vertex runs in the hub pre-pass instead of in id
- * order (>= 4096; a larger value keeps more hubs in the ascending sweep at
- * the cost of larger per-stream HBM tables).  Default 4096. */
-int lv_context_set_hub_threshold(lv_context* ctx, uint32_t degree);
+/* Degree above which a vertex runs in the hub batch (every such vertex at
+ * once, edges split over many blocks) instead of in id order: sweep_degree
+ * (>= 4096; larger keeps more hubs in the ascending sweep at the cost of
+ * larger per-stream HBM tables) for sweeps, worklist_degree (256..4096) for
+ * frontier worklists.  Defaults 4096, 256. */
+int lv_context_set_hub_threshold(lv_context* ctx, uint32_t sweep_degree, uint32_t worklist_degree);
 
 /* Kernel statistics of the local-moving phase, per degree tier (0..4),
  * accumulated over lv_run / entry-point calls since lv_stats_reset.  The

@@ -59,7 +59,12 @@ struct lv_context {
   int pick_less_iters = 0;      // first iterations of each pass that only move to lower ids
   uint64_t slabs = 1;           // id-order sweeps: consecutive id slabs launched in turn
   uint64_t hub_streams = kHubBlocks;  // hubs processed at once
-  uint32_t hub_min = kHubMin;         // degrees above run in the hub pre-pass
+  uint32_t hub_min = kHubMin;         // sweeps: degrees above run in the hub batch
+  uint32_t hub_min_wl = 256;          // worklists: degrees above run in the hub batch
+  // worklist hub-table slots of the last graph seen (its degree array, span, threshold)
+  const uint32_t* wl_key_deg = nullptr;
+  uint64_t wl_key_S = 0, wl_slots = 0;
+  uint32_t wl_key_thr = 0;
   int tier_order = 2;           // 2: ascending ids (hubs first), 1: degree tiers highest first, 0: lowest first
   uint32_t serial = 0;          // worklist iteration serial (visit stamps)
   uint64_t stamp_span = ~0ull;
@@ -287,7 +292,7 @@ uint64_t level_streams(const lv_context* c, uint64_t order) {
 }
 
 MoveArgs base_args(lv_context* c, const Level& lv, uint32_t* vcom, const double* vtot, double* ctot, uint8_t* vaff,
-                   const uint8_t* fa, double M, double R) {
+                   const uint8_t* fa, double M, double R, bool worklist = false) {
   MoveArgs a{};
   a.g = lv.g;
   a.vcom = vcom;
@@ -303,7 +308,7 @@ MoveArgs base_args(lv_context* c, const Level& lv, uint32_t* vcom, const double*
   a.max_streams = level_streams(c, lv.order);
   a.chunk = c->chunk;
   a.work = c->ws<unsigned long long>("work", 1);
-  a.hub_min = c->hub_min;
+  a.hub_min = worklist ? c->hub_min_wl : c->hub_min;
   a.hub_streams = c->hub_streams;
   // hub kernel: one HBM region per block, sized for the largest hub
   uint32_t md = c->tier_order == 2 ? lv.hubs->maxdeg : lv.maxdeg4;
@@ -315,6 +320,22 @@ MoveArgs base_args(lv_context* c, const Level& lv, uint32_t* vcom, const double*
     a.hval = c->ws<double>("h.val", slots);
   }
   uint64_t nh = lv.hubs ? lv.hubs->start[5] - lv.hubs->start[4] : 0;
+  uint64_t slots = lv.hub_slots;
+  if (worklist && c->tier_order == 2) {
+    // a queue may hold any vertex above the worklist threshold: scratch for
+    // all of them, table slots for the sum over them
+    nh = lv.g.S;
+    if (c->wl_key_deg != lv.g.deg || c->wl_key_S != lv.g.S || c->wl_key_thr != c->hub_min_wl) {
+      unsigned long long* tot = c->ws<unsigned long long>("hb.need", 1);
+      k_fill_u64(reinterpret_cast<uint64_t*>(tot), 0, 1, c->st);
+      k_caps_sum(lv.g.deg, lv.g.S, c->hub_min_wl, tot, c->st);
+      c->wl_slots = c->read1(tot);
+      c->wl_key_deg = lv.g.deg;
+      c->wl_key_S = lv.g.S;
+      c->wl_key_thr = c->hub_min_wl;
+    }
+    slots = c->wl_slots;
+  }
   if (c->tier_order == 2 && nh) {
     // hub batches: per-hub scratch and a table slab for every hub of the level
     a.f64_weights = lv.f64;
@@ -326,9 +347,9 @@ MoveArgs base_args(lv_context* c, const Level& lv, uint32_t* vcom, const double*
     a.hb_sd = c->ws<double>("hb.sd", nh);
     a.hb_emax = c->ws<unsigned long long>("hb.emax", nh);
     a.hb_bpos = c->ws<uint32_t>("hb.bpos", nh);
-    a.hb_key = c->ws<uint32_t>("hb.key", lv.hub_slots);
-    a.hb_pos = c->ws<uint32_t>("hb.pos", lv.hub_slots);
-    a.hb_val = c->ws<double>("hb.val", lv.hub_slots);
+    a.hb_key = c->ws<uint32_t>("hb.key", slots);
+    a.hb_pos = c->ws<uint32_t>("hb.pos", slots);
+    a.hb_val = c->ws<double>("hb.val", slots);
   }
   if (c->tier_order == 2) {
     // id-order kernel: one HBM region per stream holding nextpow2(2 deg)
@@ -393,9 +414,13 @@ void add_counters(lv_context* c, const unsigned long long* dcnt, Counters& cnt) 
 // louvainMoveOmpW (inc/louvain.hxx:587-609): iterate until the summed gain
 // el <= E or L iterations; returns the iterations, or 0 if the first one
 // gained nothing.
+// ascending: hand out chunks in ascending order even if the context spreads
+// them -- a sparse frontier needs a vertex flagged by a lower id to be reached
+// later in the same sweep (inc/louvain.hxx:602 then :598).
 int local_move(lv_context* c, const Level& lv, uint32_t* vcom, const double* vtot, double* ctot, uint8_t* vaff,
-               const uint8_t* fa, double M, double R, int L, double E, Counters& cnt) {
+               const uint8_t* fa, double M, double R, int L, double E, Counters& cnt, bool ascending = false) {
   MoveArgs a = base_args(c, lv, vcom, vtot, ctot, vaff, fa, M, R);
+  if (ascending && !a.chunk) a.chunk = 16;
   k_fill_u64(reinterpret_cast<uint64_t*>(a.cnt), 0, (size_t)kTiers * CNT_N, c->st);
   unsigned long long* cnt0 = a.cnt;
   int l = 0;
@@ -425,7 +450,7 @@ int local_move(lv_context* c, const Level& lv, uint32_t* vcom, const double* vto
 int local_move_worklist(lv_context* c, const Level& lv, uint32_t* vcom, const double* vtot, double* ctot,
                         uint8_t* vaff, const uint8_t* fa, double M, double R, int L, double E, uint32_t* q[4],
                         uint32_t* qlen, Counters& cnt) {
-  MoveArgs a = base_args(c, lv, vcom, vtot, ctot, vaff, fa, M, R);
+  MoveArgs a = base_args(c, lv, vcom, vtot, ctot, vaff, fa, M, R, true);
   k_fill_u64(reinterpret_cast<uint64_t*>(a.cnt), 0, (size_t)kTiers * CNT_N, c->st);
   unsigned long long* cnt0 = a.cnt;
   const uint64_t stride = std::max<uint64_t>(lv.g.S, 1);
@@ -457,10 +482,10 @@ int local_move_worklist(lv_context* c, const Level& lv, uint32_t* vcom, const do
       k_fill_u32(qlen + sub, 0, 1, c->st);
       a.qsame = q[sub];
       a.qsame_len = qlen + sub;
-      uint32_t thr = c->tier_order == 2 ? c->hub_min : 0;
+      uint32_t thr = c->tier_order == 2 ? c->hub_min_wl : 0;
       k_bin_queue(q[cur], qlen + cur, std::min<uint64_t>(lv.g.S, 1u << 19), lv.g.deg, thr, tq, stride, tlen, c->st);
       if (c->tier_order == 2) {  // id-order mode: hubs, then the rest through the id-order kernel
-        if (lv.hubs->maxdeg > c->hub_min) {
+        if (lv.g.S) {  // an empty bin costs the batch's few launches only
           a.list = tq + (size_t)4 * stride;
           a.n_dev = tlen + 4;
           a.cnt = cnt0 + (size_t)4 * CNT_N;
@@ -697,7 +722,8 @@ void invoke(lv_context* c, int approach, const lv_dgraph* x, const lv_dbatch* b,
     hipEvent_t a0 = c->record();
     if (first) {
       if (worklist) m = local_move_worklist(c, cur, ucom, utot, ctot, vaff, fa, M, R, L, E, q, qlen, cnt);
-      else m = local_move(c, cur, ucom, utot, ctot, vaff, fa, M, R, L, E, cnt);
+      else m = local_move(c, cur, ucom, utot, ctot, vaff, fa, M, R, L, E, cnt,
+                          approach == LV_DYNAMIC_FRONTIER || approach == LV_DELTA_SCREENING);
     } else {
       m = local_move(c, cur, vcom, vtot, ctot, vaff, nullptr, M, R, L, E, cnt);
     }
@@ -1067,9 +1093,11 @@ int lv_context_set_hub_streams(lv_context* c, uint64_t streams) {
   return LV_OK;
 }
 
-int lv_context_set_hub_threshold(lv_context* c, uint32_t degree) {
-  if (!c || degree < kHubMin) return LV_ERR_INVALID;
-  c->hub_min = degree;
+int lv_context_set_hub_threshold(lv_context* c, uint32_t sweep_degree, uint32_t worklist_degree) {
+  if (!c || sweep_degree < kHubMin || worklist_degree < kHybridLds / 2 || worklist_degree > kHubMin)
+    return LV_ERR_INVALID;
+  c->hub_min = sweep_degree;
+  c->hub_min_wl = worklist_degree;
   return LV_OK;
 }
 

@@ -1085,6 +1085,21 @@ __global__ void gather_u32_kernel(const uint32_t* a, const uint32_t* idx, uint64
     out[i] = a[idx[i]];
 }
 
+// Sum of nextpow2(2 deg) over the ids with degree above thr (hub table slots).
+__global__ __launch_bounds__(kBlock) void caps_sum_kernel(const uint32_t* deg, uint64_t n, uint32_t thr,
+                                                          unsigned long long* out) {
+  unsigned long long acc = 0;
+  for (uint64_t u = blockIdx.x * (uint64_t)kBlock + threadIdx.x; u < n; u += (uint64_t)gridDim.x * kBlock) {
+    uint64_t d = deg[u];
+    if (d <= thr) continue;
+    uint64_t cap = 2;
+    while (cap < 2 * d) cap <<= 1;
+    acc += cap;
+  }
+  acc = block_sum_u64<kBlock>(acc);
+  if (threadIdx.x == 0 && acc) atomicAdd(out, acc);
+}
+
 // Queue every id whose flag is set (wave-aggregated appends).
 __global__ __launch_bounds__(kBlock) void compact_flags_kernel(const uint8_t* f, uint64_t n, uint32_t* q,
                                                                uint32_t* qlen) {
@@ -1339,6 +1354,9 @@ void k_max_u32(const uint32_t* a, uint64_t n, uint32_t* out, hipStream_t st) {
 }
 void k_gather_u32(const uint32_t* a, const uint32_t* idx, uint64_t n, uint32_t* out, hipStream_t st) {
   if (n) LV_LAUNCH(gather_u32_kernel, grid_for(n, kBlock, 1024), kBlock, st, a, idx, n, out);
+}
+void k_caps_sum(const uint32_t* deg, uint64_t n, uint32_t thr, unsigned long long* out, hipStream_t st) {
+  if (n) LV_LAUNCH(caps_sum_kernel, grid_for(n, kBlock * 8, 4096), kBlock, st, deg, n, thr, out);
 }
 void k_compact_flags(const uint8_t* f, uint64_t n, uint32_t* q, uint32_t* qlen, hipStream_t st) {
   if (n) LV_LAUNCH(compact_flags_kernel, grid_for(n, kBlock, 1u << 16), kBlock, st, f, n, q, qlen);

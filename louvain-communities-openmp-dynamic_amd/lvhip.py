@@ -192,8 +192,10 @@ class Context:
     def set_hub_streams(self, streams=256):
         self._check(self.lib.lv_context_set_hub_streams(self.h, C.c_uint64(streams)), "lv_context_set_hub_streams")
 
-    def set_hub_threshold(self, degree=4096):
-        self._check(self.lib.lv_context_set_hub_threshold(self.h, C.c_uint32(degree)), "lv_context_set_hub_threshold")
+    def set_hub_threshold(self, sweep_degree=4096, worklist_degree=256):
+        self._check(self.lib.lv_context_set_hub_threshold(self.h, C.c_uint32(sweep_degree),
+                                                          C.c_uint32(worklist_degree)),
+                    "lv_context_set_hub_threshold")
 
     def set_slabs(self, slabs=1):
         self._check(self.lib.lv_context_set_slabs(self.h, C.c_uint64(slabs)), "lv_context_set_slabs")

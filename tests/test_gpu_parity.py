@@ -279,7 +279,7 @@ def test_single_stream_reproduces_reference_bit_exact(ctx, name):
     orc = Oracle()
     orc.set_threads(1)
     ctx.set_schedule(1e-12, 1, 1, 16, 2)
-    ctx.set_hub_threshold(1 << 30)
+    ctx.set_hub_threshold(1 << 30, 4096)
     try:
         for approach in ("static", "naive-dynamic", "dynamic-frontier"):
             if approach == "static":
@@ -297,5 +297,5 @@ def test_single_stream_reproduces_reference_bit_exact(ctx, name):
                                                                       ref.affected_vertices), approach
     finally:
         ctx.set_schedule()
-        ctx.set_hub_threshold(4096)
+        ctx.set_hub_threshold()
         ctx.set_path(1)
