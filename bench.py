@@ -55,14 +55,20 @@ def run_chain(ctx, lvhip, dgraphs, dbatches, state, state0, W, K, path):
         ctx.run(lvhip.LV_DYNAMIC_FRONTIER, dgraphs[k], dbatches[k], state)
     ctx.stats_reset()
     ctx.set_profiling(True)
-    lat, edges = [], 0
+    lat, edges, parts = [], 0, []
     t0 = time.perf_counter()
     for k in range(W, W + K):
         r = ctx.run(lvhip.LV_DYNAMIC_FRONTIER, dgraphs[k], dbatches[k], state)
         lat.append(r.time)
+        parts.append((r.initializationTime, r.markingTime, r.firstPassTime, r.localMoveTime, r.aggregationTime,
+                      r.iterations, r.passes, r.affectedVertices))
         edges += dgraphs[k].edges
     t1 = time.perf_counter()
     ctx.set_profiling(False)
+    m = [sum(x[j] for x in parts) / len(parts) for j in range(8)]
+    log(f"  path {path} per batch (mean): update {m[0]:.3f} ms, marking {m[1]:.3f} ms, first pass {m[2]:.3f} ms, "
+        f"local-moving {m[3]:.3f} ms, aggregation {m[4]:.3f} ms; iterations {m[5]:.1f}, passes {m[6]:.1f}, "
+        f"affected {m[7]:.0f}")
     return t1 - t0, lat, edges, ctx.stats()
 
 
@@ -77,6 +83,13 @@ def main():
     ap.add_argument("--seed", type=int, default=7)
     ap.add_argument("--no-cpu", action="store_true", help="skip the oracle check and CPU baseline")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-path comparison")
+    ap.add_argument("--no-static", action="store_true", help="skip the (reported) GPU Static run")
+    ap.add_argument("--stream-fraction", type=float, default=None,
+                    help="streams per live vertex of a level (lv_context_set_schedule)")
+    ap.add_argument("--worklist-hub-degree", type=int, default=None,
+                    help="degree above which worklist vertices run in the hub batch (256..4096)")
+    ap.add_argument("--path", choices=("frontier", "worklist", "plain"), default="frontier",
+                    help="local-moving path of the timed chain (the others are timed beside it)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -104,6 +117,10 @@ def main():
     log(f"{W + K} batches of {B} updates ({n_del} del / {n_ins} ins undirected) applied ({time.time() - t:.1f}s)")
 
     ctx = lvhip.Context(0)
+    if args.stream_fraction is not None:
+        ctx.set_schedule(stream_fraction=args.stream_fraction)
+    if args.worklist_hub_degree is not None:
+        ctx.set_hub_threshold(worklist_degree=args.worklist_hub_degree)
     t = time.time()
     dgraphs = [ctx.device_graph(g.view()) for g in graphs]
     dbatches = [ctx.device_batch(d, i) for d, i in batches]
@@ -114,11 +131,13 @@ def main():
     # GPU Static Louvain on the original graph (reported), and the starting
     # state of the DF chain: the reference's own Static result, as in
     # main.cxx:89 (louvainStaticOmp), from the oracle on the host cores.
-    t = time.time()
-    rs = ctx.run(lvhip.LV_STATIC, dgraphs[0], None, state0)
-    q_static = ctx.dmodularity(dgraphs[0], state0)
-    log(f"GPU static: Q={q_static:.6f} iters={rs.iterations} passes={rs.passes} time={rs.time:.1f} ms "
-        f"(wall {time.time() - t:.1f}s)")
+    rs, q_static = None, None
+    if not args.no_static:
+        t = time.time()
+        rs = ctx.run(lvhip.LV_STATIC, dgraphs[0], None, state0)
+        q_static = ctx.dmodularity(dgraphs[0], state0)
+        log(f"GPU static: Q={q_static:.6f} iters={rs.iterations} passes={rs.passes} time={rs.time:.1f} ms "
+            f"(wall {time.time() - t:.1f}s)")
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     from oracle import Oracle
     orc = Oracle()
@@ -132,27 +151,34 @@ def main():
         f"passes={r0.passes} time={r0.time:.1f} ms (wall {time.time() - t:.1f}s)")
 
     # Quality chain (untimed): the GPU state before every batch, for the oracle.
-    ctx.set_path(lvhip.PATH_WORKLIST)
-    state.copy_from(state0)
+    ctx.set_path({"frontier": lvhip.PATH_FRONTIER, "worklist": lvhip.PATH_WORKLIST,
+                  "plain": lvhip.PATH_SWEEP}[args.path])
     prev, q_gpu = [], []
-    for k in range(W + K):
-        prev.append(state.download())
-        ctx.run(lvhip.LV_DYNAMIC_FRONTIER, dgraphs[k + 1], dbatches[k], state)
-        q_gpu.append(ctx.dmodularity(dgraphs[k + 1], state))
+    if not args.no_cpu:
+        state.copy_from(state0)
+        for k in range(W + K):
+            prev.append(state.download())
+            ctx.run(lvhip.LV_DYNAMIC_FRONTIER, dgraphs[k + 1], dbatches[k], state)
+            q_gpu.append(ctx.dmodularity(dgraphs[k + 1], state))
 
-    # Timed: optimized (worklist) path, then the plain path.
+    # Timed: the chosen path, then the others beside it.
+    paths = {"frontier": lvhip.PATH_FRONTIER, "worklist": lvhip.PATH_WORKLIST, "plain": lvhip.PATH_SWEEP}
     dg = dgraphs[1:]
-    wall, lat, edges, st = run_chain(ctx, lvhip, dg, dbatches, state, state0, W, K, lvhip.PATH_WORKLIST)
+    wall, lat, edges, st = run_chain(ctx, lvhip, dg, dbatches, state, state0, W, K, paths[args.path])
     value = edges / wall
-    log(f"DF worklist: {K} steps {wall * 1e3:.2f} ms wall, {value:.4g} edges/s, "
+    log(f"DF {args.path}: {K} steps {wall * 1e3:.2f} ms wall, {value:.4g} edges/s, "
         f"p50 {percentile(lat, 50):.3f} ms p99 {percentile(lat, 99):.3f} ms")
-    plain = None
+    others = {}
     if not args.no_plain:
-        pw, plat, pedges, _ = run_chain(ctx, lvhip, dg, dbatches, state, state0, W, K, lvhip.PATH_SWEEP)
-        plain = {"value": pedges / pw, "ms_per_step": pw * 1e3 / K, "p50_ms": percentile(plat, 50),
-                 "p99_ms": percentile(plat, 99), "speedup": (pedges / pw) and value / (pedges / pw)}
-        log(f"DF plain: {pw * 1e3:.2f} ms wall, {plain['value']:.4g} edges/s, speedup {plain['speedup']:.2f}x")
-        ctx.set_path(lvhip.PATH_WORKLIST)
+        for name, path in paths.items():
+            if name == args.path:
+                continue
+            pw, plat, pedges, _ = run_chain(ctx, lvhip, dg, dbatches, state, state0, W, K, path)
+            others[name] = {"value": pedges / pw, "ms_per_step": pw * 1e3 / K, "p50_ms": percentile(plat, 50),
+                            "p99_ms": percentile(plat, 99), "speedup": (pedges / pw) and value / (pedges / pw)}
+            log(f"DF {name}: {pw * 1e3:.2f} ms wall, {others[name]['value']:.4g} edges/s, "
+                f"{args.path} path {others[name]['speedup']:.2f}x faster")
+    ctx.set_path(paths[args.path])
 
     # Roofline of the dominant local-moving kernel (the tier with most time).
     tier = int(np.argmax(st["move_ms"]))
@@ -194,10 +220,11 @@ def main():
                                f"{args.batch_fraction:g}|E| 80/20 insert/delete, seed {args.seed}",
                    "vertices": S, "edges": graphs[0].edges, "batch_updates": B, "parallelism": "1 GPU"},
         "p50_batch_ms": percentile(lat, 50), "p99_batch_ms": percentile(lat, 99),
-        "dq_max": dq_max, "static": {"gpu_modularity": q_static, "gpu_ms": rs.time,
+        "dq_max": dq_max, "static": {"gpu_modularity": q_static, "gpu_ms": rs and rs.time,
                                      "reference_modularity": q_static_ref, "reference_ms": r0.time,
                                      "reference_threads": cores},
-        "plain_gpu_path": plain,
+        "path": args.path, "plain_gpu_path": others.get("plain"),
+        "other_gpu_paths": {k: v for k, v in others.items() if k != "plain"},
         "roofline": roofline, "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

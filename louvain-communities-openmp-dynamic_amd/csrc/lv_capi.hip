@@ -78,11 +78,13 @@ struct lv_context {
   size_t ev_used = 0;
   size_t scan_tmp_bytes = 0;
 
-  // Grow-only named device buffer; contents are unspecified.
+  // Grow-only named device buffer; contents are unspecified (*fresh is set
+  // when it was just allocated).
   template <class T>
-  T* ws(const std::string& name, size_t count) {
+  T* ws(const std::string& name, size_t count, bool* fresh = nullptr) {
     size_t bytes = std::max<size_t>(count * sizeof(T), 256);
     Buf& b = bufs[name];
+    if (fresh) *fresh = b.bytes < bytes;
     if (b.bytes < bytes) {
       if (b.p) HIPCHK(hipFree(b.p));
       b.p = nullptr;
@@ -93,6 +95,7 @@ struct lv_context {
     }
     return reinterpret_cast<T*>(b.p);
   }
+  size_t ws_bytes(const std::string& name) { return bufs[name].bytes; }
   void* scan_tmp(size_t bytes) {
     scan_tmp_bytes = std::max(scan_tmp_bytes, bytes);
     return ws<uint8_t>("scan_tmp", scan_tmp_bytes);
@@ -276,6 +279,7 @@ struct Counters {
 // Launch one tier of the local-moving kernel, bracketed by events when the
 // context profiles (per-launch durations on the context's own stream).
 constexpr int kTierHubBatch = 6;  // launch_move: hubs through k_hub_batch (stats slot 4)
+constexpr int kSubRounds = 4;      // worklist sub-rounds enqueued per host check
 
 void launch_move(lv_context* c, const MoveArgs& a, int tier, bool f64) {
   k_fill_u64(reinterpret_cast<uint64_t*>(a.work), 0, 1, c->st);
@@ -325,14 +329,15 @@ MoveArgs base_args(lv_context* c, const Level& lv, uint32_t* vcom, const double*
     // a queue may hold any vertex above the worklist threshold: scratch for
     // all of them, table slots for the sum over them
     nh = lv.g.S;
-    if (c->wl_key_deg != lv.g.deg || c->wl_key_S != lv.g.S || c->wl_key_thr != c->hub_min_wl) {
+    const uint32_t thr = c->hub_min_wl;
+    if (c->wl_key_deg != lv.g.deg || c->wl_key_S != lv.g.S || c->wl_key_thr != thr) {
       unsigned long long* tot = c->ws<unsigned long long>("hb.need", 1);
       k_fill_u64(reinterpret_cast<uint64_t*>(tot), 0, 1, c->st);
-      k_caps_sum(lv.g.deg, lv.g.S, c->hub_min_wl, tot, c->st);
+      k_caps_sum(lv.g.deg, lv.g.S, thr, tot, c->st);
       c->wl_slots = c->read1(tot);
       c->wl_key_deg = lv.g.deg;
       c->wl_key_S = lv.g.S;
-      c->wl_key_thr = c->hub_min_wl;
+      c->wl_key_thr = thr;
     }
     slots = c->wl_slots;
   }
@@ -345,11 +350,19 @@ MoveArgs base_args(lv_context* c, const Level& lv, uint32_t* vcom, const double*
     a.hb_cu = c->ws<uint32_t>("hb.cu", nh);
     a.hb_vd = c->ws<double>("hb.vd", nh);
     a.hb_sd = c->ws<double>("hb.sd", nh);
-    a.hb_emax = c->ws<unsigned long long>("hb.emax", nh);
     a.hb_bpos = c->ws<uint32_t>("hb.bpos", nh);
-    a.hb_key = c->ws<uint32_t>("hb.key", slots);
-    a.hb_pos = c->ws<uint32_t>("hb.pos", slots);
-    a.hb_val = c->ws<double>("hb.val", slots);
+    a.hb_ce = c->ws<double>("hb.ce", slots / kHubTabChunk + 1);
+    a.hb_cp = c->ws<uint32_t>("hb.cp", slots / kHubTabChunk + 1);
+    a.hb_bsum = c->ws<uint64_t>("hb.bsum", 2 * kHubScanBlocks);
+    // the batch's best pass empties every slot it used, so a slab is
+    // filled once, when it is allocated
+    bool fk = false, fp = false, fv = false;
+    a.hb_key = c->ws<uint32_t>("hb.key", slots, &fk);
+    a.hb_pos = c->ws<uint32_t>("hb.pos", slots, &fp);
+    a.hb_val = c->ws<double>("hb.val", slots, &fv);
+    if (fk) k_fill_u32(a.hb_key, kEmpty, c->ws_bytes("hb.key") / sizeof(uint32_t), c->st);
+    if (fp) k_fill_u32(a.hb_pos, kNoPos, c->ws_bytes("hb.pos") / sizeof(uint32_t), c->st);
+    if (fv) k_fill_f64(a.hb_val, 0, c->ws_bytes("hb.val") / sizeof(double), c->st);
   }
   if (c->tier_order == 2) {
     // id-order kernel: one HBM region per stream holding nextpow2(2 deg)
@@ -420,7 +433,8 @@ void add_counters(lv_context* c, const unsigned long long* dcnt, Counters& cnt) 
 int local_move(lv_context* c, const Level& lv, uint32_t* vcom, const double* vtot, double* ctot, uint8_t* vaff,
                const uint8_t* fa, double M, double R, int L, double E, Counters& cnt, bool ascending = false) {
   MoveArgs a = base_args(c, lv, vcom, vtot, ctot, vaff, fa, M, R);
-  if (ascending && !a.chunk) a.chunk = 16;
+  a.sparse = c->path == 1;
+  if (ascending && !a.chunk) a.chunk = a.sparse ? 64 : 16;
   k_fill_u64(reinterpret_cast<uint64_t*>(a.cnt), 0, (size_t)kTiers * CNT_N, c->st);
   unsigned long long* cnt0 = a.cnt;
   int l = 0;
@@ -448,23 +462,36 @@ int local_move(lv_context* c, const Level& lv, uint32_t* vcom, const double* vto
 // degree tier on the device.  q[0..3]: iteration input / next iteration /
 // two sub-round buffers; qlen[0..3] their lengths.
 int local_move_worklist(lv_context* c, const Level& lv, uint32_t* vcom, const double* vtot, double* ctot,
-                        uint8_t* vaff, const uint8_t* fa, double M, double R, int L, double E, uint32_t* q[4],
-                        uint32_t* qlen, Counters& cnt) {
+                        uint8_t* vaff, const uint8_t* fa, double M, double R, int L, double E, Counters& cnt) {
   MoveArgs a = base_args(c, lv, vcom, vtot, ctot, vaff, fa, M, R, true);
+  a.sparse = 1;  // queue entries read 64 at a time
   k_fill_u64(reinterpret_cast<uint64_t*>(a.cnt), 0, (size_t)kTiers * CNT_N, c->st);
   unsigned long long* cnt0 = a.cnt;
-  const uint64_t stride = std::max<uint64_t>(lv.g.S, 1);
-  uint32_t* tq = c->ws<uint32_t>("wl.tq", (size_t)kTiers * stride);
-  uint32_t* tlen = c->ws<uint32_t>("wl.tlen", kTiers);
+  const uint64_t S = lv.g.S, stride = std::max<uint64_t>(S, 1), fbytes = (S + 7) & ~3ull;
+  uint32_t* ql = c->ws<uint32_t>("wl.low", stride);
+  uint32_t* qh = c->ws<uint32_t>("wl.hub", stride);
+  uint32_t* lens = c->ws<uint32_t>("wl.len", 2);
+  uint64_t* bsum = c->ws<uint64_t>("wl.bsum", 2 * kWlScanBlocks);
+  // next-sub-round flags: all zero between calls (each compaction resets
+  // the flags it reads, and a sub-round loop ends on an empty compaction)
+  bool fresh = false;
+  a.fsame = c->ws<uint8_t>("wl.same", fbytes, &fresh);
+  if (fresh) k_fill_u8(a.fsame, 0, c->ws_bytes("wl.same"), c->st);
+  uint32_t* tq = nullptr;
+  uint32_t* tlen = nullptr;
+  if (c->tier_order != 2) {
+    tq = c->ws<uint32_t>("wl.tq", (size_t)kTiers * stride);
+    tlen = c->ws<uint32_t>("wl.tlen", kTiers);
+  }
   a.stamp = c->ws<uint32_t>("wl.stamp", stride);
-  if (c->stamp_span != lv.g.S) {  // fresh stamps for a new span
+  if (c->stamp_span != S) {  // fresh stamps for a new span
     k_fill_u32(a.stamp, 0, stride, c->st);
-    c->stamp_span = lv.g.S;
+    c->stamp_span = S;
     c->serial = 0;
   }
+  const uint32_t thr = c->tier_order == 2 ? c->hub_min_wl : 0;
   int l = 0;
   double el = 0;
-  int in = 0, nx = 1;
   for (; l < L;) {
     if (++c->serial == 0) {  // wrapped: reset the stamps
       k_fill_u32(a.stamp, 0, stride, c->st);
@@ -473,45 +500,42 @@ int local_move_worklist(lv_context* c, const Level& lv, uint32_t* vcom, const do
     a.serial = c->serial;
     a.pick_less = l < c->pick_less_iters;
     k_fill_f64(a.el, 0, 1, c->st);
-    k_fill_u32(qlen + nx, 0, 1, c->st);
-    a.qnext = q[nx];
-    a.qnext_len = qlen + nx;
-    int cur = in, sub = 2;
+    // the iteration's queue: every flagged id, ascending
+    k_wl_queue(vaff, false, S, lv.g.deg, thr, ql, qh, lens, bsum, c->st);
+    // Sub-rounds are enqueued kSubRounds at a time (their kernels read the
+    // queue lengths on the device; an empty round costs its launches only),
+    // with one host check of the last compaction per group.
     for (int round = 0;; ++round) {
-      k_fill_u32(tlen, 0, kTiers, c->st);
-      k_fill_u32(qlen + sub, 0, 1, c->st);
-      a.qsame = q[sub];
-      a.qsame_len = qlen + sub;
-      uint32_t thr = c->tier_order == 2 ? c->hub_min_wl : 0;
-      k_bin_queue(q[cur], qlen + cur, std::min<uint64_t>(lv.g.S, 1u << 19), lv.g.deg, thr, tq, stride, tlen, c->st);
       if (c->tier_order == 2) {  // id-order mode: hubs, then the rest through the id-order kernel
-        if (lv.g.S) {  // an empty bin costs the batch's few launches only
-          a.list = tq + (size_t)4 * stride;
-          a.n_dev = tlen + 4;
+        if (S) {
+          a.list = qh;
+          a.n_dev = lens + 1;
           a.cnt = cnt0 + (size_t)4 * CNT_N;
           launch_move(c, a, kTierHubBatch, lv.f64);
         }
-        a.list = tq;
-        a.n_dev = tlen;
+        a.list = ql;
+        a.n_dev = lens;
         a.cnt = cnt0;
         launch_move(c, a, kTierHybrid, lv.f64);
+      } else {
+        k_fill_u32(tlen, 0, kTiers, c->st);
+        k_bin_queue(ql, lens, std::min<uint64_t>(S, 1u << 19), lv.g.deg, 0, tq, stride, tlen, c->st);
+        for (int k = (c->tier_order ? kTiers - 1 : 0); k >= 0 && k < kTiers; k += (c->tier_order ? -1 : 1)) {
+          if (k == 4 && !lv.maxdeg4) continue;
+          a.list = tq + (size_t)k * stride;
+          a.n_dev = tlen + k;
+          a.cnt = cnt0 + (size_t)k * CNT_N;
+          launch_move(c, a, k, lv.f64);
+        }
       }
-      for (int k = (c->tier_order ? kTiers - 1 : 0); c->tier_order != 2 && k >= 0 && k < kTiers;
-           k += (c->tier_order ? -1 : 1)) {
-        if (k == 4 && !lv.maxdeg4) continue;
-        a.list = tq + (size_t)k * stride;
-        a.n_dev = tlen + k;
-        a.cnt = cnt0 + (size_t)k * CNT_N;
-        launch_move(c, a, k, lv.f64);
-      }
+      k_wl_queue(a.fsame, true, S, lv.g.deg, thr, ql, qh, lens, bsum, c->st);
+      if ((round + 1) % kSubRounds) continue;
       c->check();
-      uint32_t nsame = c->read1(qlen + sub);
-      if (nsame == 0) break;
-      cur = sub;
-      sub = sub == 2 ? 3 : 2;
+      uint32_t n2[2];
+      c->read(n2, lens, 2);
+      if (n2[0] + n2[1] == 0) break;
     }
     el = c->read1(a.el);
-    std::swap(in, nx);
     ++l;
     if (el <= E) break;
   }
@@ -657,14 +681,7 @@ void invoke(lv_context* c, int approach, const lv_dgraph* x, const lv_dbatch* b,
     dsn = c->ws<uint8_t>("ds.n", S);
     dsc = c->ws<uint8_t>("ds.c", S);
   }
-  bool worklist = c->path == 1 && (approach == LV_DYNAMIC_FRONTIER || approach == LV_DELTA_SCREENING);
-  uint32_t* q[4] = {nullptr, nullptr, nullptr, nullptr};
-  uint32_t* qlen = nullptr;
-  if (worklist) {
-    for (int k = 0; k < 4; ++k) q[k] = c->ws<uint32_t>("wl.q" + std::to_string(k), S);
-    qlen = c->ws<uint32_t>("wl.len", 4);
-    k_fill_u32(qlen, 0, 4, c->st);
-  }
+  bool worklist = c->path == 2 && (approach == LV_DYNAMIC_FRONTIER || approach == LV_DELTA_SCREENING);
   SuperGraph y = super_graph(c, "y", S, x->E), z = super_graph(c, "z", S, x->E);
   k_fill_u8(vaff, 0, (S + 7) & ~3ull, c->st);
   k_fill_u32(vcom, 0, S, c->st);
@@ -687,7 +704,7 @@ void invoke(lv_context* c, int approach, const lv_dgraph* x, const lv_dbatch* b,
   // fm: marking
   const uint8_t* fa = nullptr;
   if (approach == LV_DYNAMIC_FRONTIER) {
-    k_frontier(b->du, b->dv, b->nd, b->iu, b->iv, b->ni, ucom, vaff, worklist ? q[0] : nullptr, qlen, c->st);
+    k_frontier(b->du, b->dv, b->nd, b->iu, b->iv, b->ni, ucom, vaff, c->st);
   } else if (approach == LV_DELTA_SCREENING) {
     k_fill_u8(dsv, 0, S, c->st);
     k_fill_u8(dsn, 0, S, c->st);
@@ -703,7 +720,6 @@ void invoke(lv_context* c, int approach, const lv_dgraph* x, const lv_dbatch* b,
     k_ds_sweep(x->view(), ucom, dsn, dsc, dsv, c->st);
     d2d(c, vaff, dsv, S);
     fa = dsv;
-    if (worklist) k_compact_flags(dsv, S, q[0], qlen, c->st);
   } else {
     k_fill_u8(vaff, 1, S, c->st);
   }
@@ -721,7 +737,7 @@ void invoke(lv_context* c, int approach, const lv_dgraph* x, const lv_dbatch* b,
     int m;
     hipEvent_t a0 = c->record();
     if (first) {
-      if (worklist) m = local_move_worklist(c, cur, ucom, utot, ctot, vaff, fa, M, R, L, E, q, qlen, cnt);
+      if (worklist) m = local_move_worklist(c, cur, ucom, utot, ctot, vaff, fa, M, R, L, E, cnt);
       else m = local_move(c, cur, ucom, utot, ctot, vaff, fa, M, R, L, E, cnt,
                           approach == LV_DYNAMIC_FRONTIER || approach == LV_DELTA_SCREENING);
     } else {
@@ -1076,7 +1092,7 @@ void lv_context_destroy(lv_context* c) {
 const char* lv_last_error(const lv_context* c) { return c ? c->err.c_str() : "no context"; }
 
 int lv_context_set_path(lv_context* c, int path) {
-  if (!c || (path != 0 && path != 1)) return LV_ERR_INVALID;
+  if (!c || path < 0 || path > 2) return LV_ERR_INVALID;
   c->path = path;
   return LV_OK;
 }
@@ -1330,7 +1346,7 @@ int lv_affected_frontier(lv_context* c, uint64_t S, const lv_batch* del, const l
     h2d(c, w.s->memb, vcom, S);
     uint8_t* a = c->ws<uint8_t>("ph.vaff", (S + 7) & ~3ull);
     k_fill_u8(a, 0, (S + 7) & ~3ull, c->st);
-    k_frontier(w.b->du, w.b->dv, w.b->nd, w.b->iu, w.b->iv, w.b->ni, w.s->memb, a, nullptr, nullptr, c->st);
+    k_frontier(w.b->du, w.b->dv, w.b->nd, w.b->iu, w.b->iv, w.b->ni, w.s->memb, a, c->st);
     d2h(c, vaff, a, S);
     HIPCHK(hipStreamSynchronize(c->st));
   });

@@ -24,6 +24,12 @@ constexpr int kHybridLds = 512;
 constexpr uint64_t kHybridHbm = 8192;
 constexpr uint32_t kHubMin = 4096;
 constexpr uint64_t kHubBlocks = 256;
+// hub batch: table slots per best-pass chunk (every hub table is a multiple),
+// blocks of the table-size scan
+constexpr uint64_t kHubTabChunk = 1024;
+constexpr int kHubScanBlocks = 256;
+// worklist queue compaction: blocks (contiguous id segments) per pass
+constexpr int kWlScanBlocks = 1024;
 constexpr int kTierHybrid = 5;
 
 inline int tier_of(uint64_t d) {
@@ -77,16 +83,15 @@ struct MoveArgs {
   double* out_e;
   double* el;                  // accumulated gain (l1-norm of delta-Q)
   unsigned long long* cnt;     // CNT_N counters
-  uint32_t* qnext;             // worklist mode: vertices flagged 0 -> 1, next iteration
-  uint32_t* qnext_len;
-  uint32_t* qsame;             // worklist mode: flagged by a lower id and not yet
-  uint32_t* qsame_len;         //   processed this iteration -> same iteration
+  uint8_t* fsame;              // worklist mode: flagged by a lower id and not yet
+                               //   visited this iteration -> next sub-round
   uint32_t* stamp;             // worklist mode: iteration serial of the last visit
   uint32_t serial;
   int pick_less;               // 1: only moves to a lower community id
   uint64_t min_chunk;          // unused (kept zero)
   uint64_t max_streams;        // lane groups launched (streams walking at once)
   uint64_t chunk;              // list entries per dynamically assigned chunk
+  int sparse;                  // id-order kernel: scan 64-entry windows by ballot, visit flagged only
   unsigned long long* work;    // chunk counter, zero before each launch
   // hub kernel (tier 4) HBM hashtables: one region of hcap slots per block
   uint32_t* hkey;
@@ -109,8 +114,10 @@ struct MoveArgs {
   uint32_t* hb_cu;             // hub's community
   double* hb_vd;               // hub's weight to its own community
   double* hb_sd;               // Sigma_tot of the hub's community
-  unsigned long long* hb_emax; // best gain (bits of a positive double)
-  uint32_t* hb_bpos;           // first position of the best community
+  uint32_t* hb_bpos;           // first position of the best community (kNoPos: no move)
+  double* hb_ce;               // best gain per kHubTabChunk-slot table chunk
+  uint32_t* hb_cp;             // its first position (kNoPos: none)
+  uint64_t* hb_bsum;           // 2 * kHubScanBlocks block totals of the size scan
   int f64_weights;             // hub batch: the graph's weights are f64
   uint32_t* hb_key;            // table slab
   uint32_t* hb_pos;
@@ -135,7 +142,7 @@ void k_initialize(GView g, uint32_t* vcom, double* ctot, const double* vtot, hip
 void k_update_weights(const uint32_t* du, const float* dw, uint64_t nd, const uint32_t* iu, const float* iw,
                       uint64_t ni, const uint32_t* vcom, double* vtot, double* ctot, hipStream_t st);
 void k_frontier(const uint32_t* du, const uint32_t* dv, uint64_t nd, const uint32_t* iu, const uint32_t* iv,
-                uint64_t ni, const uint32_t* vcom, uint8_t* vaff, uint32_t* q, uint32_t* qlen, hipStream_t st);
+                uint64_t ni, const uint32_t* vcom, uint8_t* vaff, hipStream_t st);
 void k_ds_deletions(const uint32_t* du, const uint32_t* dv, uint64_t nd, const uint32_t* vcom, uint8_t* vertices,
                     uint8_t* neighbors, uint8_t* communities, hipStream_t st);
 void k_ds_insertions(const uint32_t* iu, const uint32_t* iv, const float* iw, uint64_t ni, const uint32_t* vcom,
@@ -190,7 +197,12 @@ void k_max_deg(const uint32_t* ids, uint64_t n, const uint32_t* deg, uint32_t* o
 void k_max_u32(const uint32_t* a, uint64_t n, uint32_t* out, hipStream_t st);
 void k_gather_u32(const uint32_t* a, const uint32_t* idx, uint64_t n, uint32_t* out, hipStream_t st);
 void k_caps_sum(const uint32_t* deg, uint64_t n, uint32_t thr, unsigned long long* out, hipStream_t st);
-void k_compact_flags(const uint8_t* f, uint64_t n, uint32_t* q, uint32_t* qlen, hipStream_t st);
+// Ordered worklist queues: ids with f[id] set, ascending, degree <= thr to
+// ql and above to qh (thr == 0: all to ql), counts to lens[0..1]; clear
+// resets the flags read.  f holds S bytes rounded up to a multiple of 4;
+// bsum holds 2 * kWlScanBlocks entries.
+void k_wl_queue(uint8_t* f, bool clear, uint64_t S, const uint32_t* deg, uint32_t thr, uint32_t* ql, uint32_t* qh,
+                uint32_t* lens, uint64_t* bsum, hipStream_t st);
 void k_modularity_parts(GView g, bool f64, const uint32_t* memb, double* cin, double* ctot, hipStream_t st);
 void k_modularity_sum(const double* cin, const double* ctot, uint64_t n, double M, double R, double* out,
                       hipStream_t st);

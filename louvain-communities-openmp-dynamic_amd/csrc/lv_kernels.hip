@@ -148,17 +148,12 @@ __device__ __forceinline__ void wave_push(bool on, uint32_t v, uint32_t* q, uint
 // ascending sweep reaches such a v later in the same iteration), otherwise
 // into the next iteration.
 __device__ __forceinline__ void flag_vertex(uint8_t* vaff, uint32_t v, uint32_t u, const MoveArgs& a) {
-  if (!a.qnext) {
-    __hip_atomic_store(&vaff[v], (uint8_t)1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    return;
-  }
-  uint32_t sh = (v & 3u) * 8u;
-  uint32_t old = atomicOr(reinterpret_cast<uint32_t*>(vaff) + (v >> 2), 1u << sh);
-  bool fresh = ((old >> sh) & 0xFFu) == 0;
-  bool same = fresh && a.qsame && v > u &&
-              __hip_atomic_load(&a.stamp[v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != a.serial;
-  wave_push(same, v, a.qsame, a.qsame_len);
-  wave_push(fresh && !same, v, a.qnext, a.qnext_len);
+  __hip_atomic_store(&vaff[v], (uint8_t)1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  // worklist mode: a vertex flagged by a lower id and not visited in this
+  // iteration yet joins the next sub-round (its turn in the reference's
+  // ascending sweep is still to come, inc/louvain.hxx:602 then :598)
+  if (a.fsame && v > u && __hip_atomic_load(&a.stamp[v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != a.serial)
+    __hip_atomic_store(&a.fsame[v], (uint8_t)1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // ---------------------------------------------------------------- fills
@@ -251,41 +246,13 @@ __global__ void update_weights_kernel(const uint32_t* du, const float* dw, uint6
 // louvainAffectedVerticesFrontierOmpW (inc/louvain.hxx:1606-1624); vaff must
 // be zero on entry.  With q set, each newly marked vertex is also queued.
 __global__ void frontier_kernel(const uint32_t* du, const uint32_t* dv, uint64_t nd, const uint32_t* iu,
-                                const uint32_t* iv, uint64_t ni, const uint32_t* vcom, uint8_t* vaff, uint32_t* q,
-                                uint32_t* qlen) {
-  uint64_t n = nd + ni;
-  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  // uniform trip count per wave so the queue push can use wave ballots
-  for (uint64_t b = (blockIdx.x * (uint64_t)blockDim.x) & ~63ull; b < n; b += stride) {
-    uint64_t i = b + (threadIdx.x & ~63u) + __lane_id();
-    bool mark = false;
-    uint32_t u = 0;
+                                const uint32_t* iv, uint64_t ni, const uint32_t* vcom, uint8_t* vaff) {
+  const uint64_t n = nd + ni;
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
     if (i < nd) {
-      u = du[i];
-      mark = vcom[u] == vcom[dv[i]];
-    } else if (i < n) {
-      u = iu[i - nd];
-      mark = vcom[u] != vcom[iv[i - nd]];
-    }
-    if (q) {
-      // flag_vertex ballots over the active lanes: keep all lanes active
-      bool fresh = false;
-      uint32_t sh = (u & 3u) * 8u;
-      if (mark) {
-        uint32_t old = atomicOr(reinterpret_cast<uint32_t*>(vaff) + (u >> 2), 1u << sh);
-        fresh = ((old >> sh) & 0xFFu) == 0;
-      }
-      unsigned long long mask = __ballot(fresh);
-      if (mask) {
-        int lane = __lane_id();
-        int leader = __ffsll((long long)mask) - 1;
-        uint32_t base = 0;
-        if (lane == leader) base = atomicAdd(qlen, (uint32_t)__popcll(mask));
-        base = __shfl(base, leader);
-        if (fresh) q[base + __popcll(mask & ((1ull << lane) - 1))] = u;
-      }
-    } else if (mark) {
-      vaff[u] = 1;
+      if (vcom[du[i]] == vcom[dv[i]]) vaff[du[i]] = 1;
+    } else if (vcom[iu[i - nd]] != vcom[iv[i - nd]]) {
+      vaff[iu[i - nd]] = 1;
     }
   }
 }
@@ -507,31 +474,19 @@ __global__ __launch_bounds__(BLOCK) void move_kernel(MoveArgs a) {
     }
     if (cb >= nchunks) break;
     const uint64_t ch = cb + grp;
-    for (uint64_t j = 0; j < chunk; ++j) {
-      const uint64_t idx = ch * chunk + j;
-      bool act = false;
-      uint32_t u = 0;
-      if (ch < nchunks && idx < n) {
-        u = a.list ? a.list[idx] : (uint32_t)(a.base + idx);
-        act = (!a.g.ex || a.g.ex[u]) && (a.choose_only || (ld_dev(&a.vaff[u]) && (!a.fa || a.fa[u])));
-        if constexpr (MODE == MOVE_HYBRID) act = act && a.g.deg[u] <= a.hub_min;
-        if (act && a.stamp && lane == 0) st_dev(&a.stamp[u], a.serial);
-      }
-      uint64_t o = 0;
-      uint32_t d = 0, cu = 0;
-      double Ku = 0;
+    // One vertex (offset o, degree d, community cu, weight Ku) through the
+    // group; every thread of the block calls it.
+    auto visit = [&](bool act, uint32_t u, uint64_t o, uint32_t d, uint32_t cu, double Ku) {
+      if (act && a.stamp && lane == 0) st_dev(&a.stamp[u], a.serial);
       bool hbm = MODE == MOVE_HBM;
       uint64_t cap = CAP;
       if (act) {
-        o = a.g.off[u];
-        d = a.g.deg[u];
-        cu = ld_dev(&a.vcom[u]);
-        Ku = a.vtot[u];
         if constexpr (MODE == MOVE_HYBRID) hbm = 2 * (uint64_t)d > CAP;
-        if (hbm) {
-          cap = 2;  // nextpow2(2 d) slots of the block's region
-          while (cap < 2 * (uint64_t)d) cap <<= 1;
-        }
+        // nextpow2(2 d) slots: of the block's HBM region, or (at least 16)
+        // of the group's LDS table
+        uint64_t want = hbm ? 2 : 16;
+        while (want < 2 * (uint64_t)d) want <<= 1;
+        cap = hbm ? want : (want < CAP ? want : CAP);
         if (hbm) table_clear<G>(hkey, hpos, hval, cap, lane);
         else table_clear<G>(lkey, lpos, lval, cap, lane);
       }
@@ -575,6 +530,66 @@ __global__ __launch_bounds__(BLOCK) void move_kernel(MoveArgs a) {
         }
       }
       __syncthreads();
+    };
+    if constexpr (MODE == MOVE_HYBRID) {
+      if (a.sparse) {
+        // 64-entry windows (one wave per block): the flags of a window are
+        // read at once and only the flagged entries are visited, in order;
+        // after each visit the later flags are read again, so a vertex
+        // flagged by an earlier one of the window is still visited in this
+        // sweep (inc/louvain.hxx:602 then :598).
+        for (uint64_t j0 = 0; j0 < chunk; j0 += 64) {
+          // (a flagged entry's offset, degree, community and K are loaded
+          // with the window; its community changes only when it is visited)
+          const uint64_t idx = ch * chunk + j0 + lane;
+          const bool in = ch < nchunks && j0 + lane < chunk && idx < n;
+          uint32_t ul = 0, dl = 0, cl = 0;
+          uint64_t ol = 0;
+          double kl = 0;
+          bool ok = false, pre = false;
+          if (in) {
+            ul = a.list ? a.list[idx] : (uint32_t)(a.base + idx);
+            dl = a.g.deg[ul];
+            ok = (!a.g.ex || a.g.ex[ul]) && (a.choose_only || !a.fa || a.fa[ul]) && dl <= a.hub_min;
+          }
+          bool live = ok && (a.choose_only || ld_dev(&a.vaff[ul]));
+          unsigned long long mask = __ballot(live);
+          while (mask) {
+            if (live && !pre) {
+              ol = a.g.off[ul];
+              cl = ld_dev(&a.vcom[ul]);
+              kl = a.vtot[ul];
+              pre = true;
+            }
+            const int b = __ffsll((long long)mask) - 1;
+            visit(true, (uint32_t)__shfl(ul, b, 64), (uint64_t)__shfl(ol, b, 64), (uint32_t)__shfl(dl, b, 64),
+                  (uint32_t)__shfl(cl, b, 64), __shfl(kl, b, 64));
+            live = ok && lane > b && (a.choose_only || ld_dev(&a.vaff[ul]));
+            mask = __ballot(live);
+          }
+        }
+        continue;
+      }
+    }
+    for (uint64_t j = 0; j < chunk; ++j) {
+      const uint64_t idx = ch * chunk + j;
+      bool act = false;
+      uint32_t u = 0;
+      if (ch < nchunks && idx < n) {
+        u = a.list ? a.list[idx] : (uint32_t)(a.base + idx);
+        act = (!a.g.ex || a.g.ex[u]) && (a.choose_only || (ld_dev(&a.vaff[u]) && (!a.fa || a.fa[u])));
+        if constexpr (MODE == MOVE_HYBRID) act = act && a.g.deg[u] <= a.hub_min;
+      }
+      uint64_t o = 0;
+      uint32_t d = 0, cu = 0;
+      double Ku = 0;
+      if (act) {
+        o = a.g.off[u];
+        d = a.g.deg[u];
+        cu = ld_dev(&a.vcom[u]);
+        Ku = a.vtot[u];
+      }
+      visit(act, u, o, d, cu, Ku);
     }
   }
   if (!a.choose_only) {
@@ -597,64 +612,129 @@ __global__ __launch_bounds__(BLOCK) void move_kernel(MoveArgs a) {
 
 // ------------------------------------------------------- hub batches
 // Steps of louvainMoveOmpW's body (inc/louvain.hxx:599-605) for a batch of
-// high-degree vertices processed together: each hub has its own HBM table
-// of nextpow2(2 deg) slots and its edges are cut into kHubChunk-edge chunks
-// spread over the grid.  The best community is the largest positive gain,
-// ties to the earliest first position (inc/louvain.hxx:461), found by two
-// passes (max gain bits, then min position); its id is the community of the
-// neighbour at that position.
-constexpr uint64_t kHubChunk = 2048;
-constexpr int kHubPrepThreads = 1024;
+// high-degree vertices (hubs) processed together: each with its own HBM table
+// of nextpow2(2 deg) slots.  Its edges are cut into kHubChunk-edge chunks
+// spread over the grid; each chunk is summed per community in an LDS table
+// first, so a community that many edges share costs one HBM atomic per chunk.
+// The best community is the largest positive gain, ties to the earliest
+// first position (inc/louvain.hxx:461): one pass over the tables reduces
+// every kTabChunk-slot chunk (and empties its slots, so the slab is clean
+// for the next batch), the apply pass reduces a hub's chunk results; the
+// chosen id is the community of the neighbour at that position.
+constexpr uint64_t kHubChunk = 1024;          // edges per insert / flag chunk
+constexpr uint32_t kHubLdsTab = 2 * kHubChunk;  // LDS slots for one chunk's communities
+constexpr uint64_t kTabChunk = kHubTabChunk;  // table slots per best-pass chunk
+constexpr int kHubPrepBlocks = kHubScanBlocks;
+static_assert(kHubPrepBlocks == kBlock, "hub_scan_kernel scans one block total per thread");
 
-// One block: which hubs are active, their table sizes and chunk counts,
-// exclusive scans of both, and the batch totals.
-__global__ __launch_bounds__(kHubPrepThreads) void hub_prep_kernel(MoveArgs a) {
-  __shared__ uint64_t s_cap[kHubPrepThreads], s_chk[kHubPrepThreads];
-  const uint64_t n = a.n_dev ? (uint64_t)*a.n_dev : a.n;
-  const uint64_t per = (n + kHubPrepThreads - 1) / kHubPrepThreads;
-  const uint64_t lo = threadIdx.x * per, hi = min(n, lo + per);
-  uint64_t cap_sum = 0, chk_sum = 0;
-  for (uint64_t k = lo; k < hi; ++k) {
-    uint32_t u = a.list[k];
-    uint64_t d = a.g.deg[u], cap = 0, chk = 0;
-    bool act = d > a.hub_min && (!a.g.ex || a.g.ex[u]) &&
-               (a.choose_only || (ld_dev(&a.vaff[u]) && (!a.fa || a.fa[u])));
-    if (act) {
-      cap = 2;
-      while (cap < 2 * d) cap <<= 1;
-      chk = (d + kHubChunk - 1) / kHubChunk;
-      if (a.stamp) st_dev(&a.stamp[u], a.serial);
-    }
-    a.hb_cap[k] = cap;
-    a.hb_chunks[k] = chk;
-    cap_sum += cap;
-    chk_sum += chk;
+// Block-wide exclusive scan of one value per thread; `total` gets the sum.
+template <int BLOCK>
+__device__ uint64_t block_scan_excl(uint64_t x, uint64_t& total) {
+  __shared__ uint64_t s[BLOCK / 64];
+  const int lane = threadIdx.x % 64, w = threadIdx.x / 64;
+  uint64_t inc = x;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    uint64_t y = __shfl_up(inc, d, 64);
+    if (lane >= d) inc += y;
   }
-  s_cap[threadIdx.x] = cap_sum;
-  s_chk[threadIdx.x] = chk_sum;
+  if (lane == 63) s[w] = inc;
   __syncthreads();
-  if (threadIdx.x == 0) {  // scan of the per-thread sums (1024 entries)
-    uint64_t c0 = 0, c1 = 0;
-    for (int t = 0; t < kHubPrepThreads; ++t) {
-      uint64_t x0 = s_cap[t], x1 = s_chk[t];
-      s_cap[t] = c0;
-      s_chk[t] = c1;
-      c0 += x0;
-      c1 += x1;
-    }
-    a.hb_tot[0] = c0;
-    a.hb_tot[1] = c1;
-    a.hb_cap[n] = c0;
-    a.hb_chunks[n] = c1;
+  uint64_t before = 0;
+  total = 0;
+#pragma unroll
+  for (int i = 0; i < BLOCK / 64; ++i) {
+    if (i < w) before += s[i];
+    total += s[i];
   }
   __syncthreads();
-  uint64_t c0 = s_cap[threadIdx.x], c1 = s_chk[threadIdx.x];
-  for (uint64_t k = lo; k < hi; ++k) {
-    uint64_t x0 = a.hb_cap[k], x1 = a.hb_chunks[k];
-    a.hb_cap[k] = c0;
-    a.hb_chunks[k] = c1;
-    c0 += x0;
-    c1 += x1;
+  return before + inc - x;
+}
+
+// Insert into an HBM table whose keys, once set, never change: a key that
+// is already there is found without a compare-and-swap.
+__device__ __forceinline__ void tab_merge(uint32_t* key, uint32_t* pos, double* val, uint64_t cap, uint32_t c,
+                                          double w, uint32_t p) {
+  uint32_t s = slot_of(c, cap);
+  while (true) {
+    uint32_t k = __hip_atomic_load(&key[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (k == kEmpty) k = atomicCAS(&key[s], kEmpty, c);
+    if (k == kEmpty || k == c) {
+      unsafeAtomicAdd(&val[s], w);
+      atomicMin(&pos[s], p);
+      return;
+    }
+    s = (uint32_t)((s + 1) & (cap - 1));
+  }
+}
+
+__device__ __forceinline__ uint64_t list_len(const MoveArgs& a) { return a.n_dev ? (uint64_t)*a.n_dev : a.n; }
+
+// Per listed vertex: active or not, table slots (nextpow2(2 deg), at least
+// kTabChunk) and edge chunks.  Each block scans a contiguous segment of the
+// list; its totals go to bsum[block] and bsum[kBlock + block].
+__global__ __launch_bounds__(kBlock) void hub_sizes_kernel(MoveArgs a, uint64_t* bsum) {
+  const uint64_t n = list_len(a);
+  const uint64_t seg = (n + gridDim.x - 1) / gridDim.x;
+  const uint64_t lo = blockIdx.x * seg, hi = min(n, lo + seg);
+  uint64_t run_cap = 0, run_chk = 0;
+  for (uint64_t b = lo; b < hi; b += kBlock) {  // block-uniform trip count
+    uint64_t k = b + threadIdx.x, cap = 0, chk = 0;
+    if (k < hi) {
+      uint32_t u = a.list[k];
+      uint64_t d = a.g.deg[u];
+      bool act = d > a.hub_min && (!a.g.ex || a.g.ex[u]) &&
+                 (a.choose_only || (ld_dev(&a.vaff[u]) && (!a.fa || a.fa[u])));
+      if (act) {
+        cap = kTabChunk;
+        while (cap < 2 * d) cap <<= 1;
+        chk = (d + kHubChunk - 1) / kHubChunk;
+        if (a.stamp) st_dev(&a.stamp[u], a.serial);
+      }
+    }
+    uint64_t tc, tk;
+    uint64_t ec = block_scan_excl<kBlock>(cap, tc);
+    uint64_t ek = block_scan_excl<kBlock>(chk, tk);
+    if (k < hi) {
+      a.hb_cap[k] = run_cap + ec;
+      a.hb_chunks[k] = run_chk + ek;
+    }
+    run_cap += tc;
+    run_chk += tk;
+  }
+  if (threadIdx.x == 0) {
+    bsum[blockIdx.x] = run_cap;
+    bsum[kBlock + blockIdx.x] = run_chk;
+  }
+}
+
+// One block: exclusive scan of the block totals, batch totals.
+__global__ __launch_bounds__(kBlock) void hub_scan_kernel(MoveArgs a, uint64_t* bsum) {
+  const uint64_t n = list_len(a);
+  uint64_t tc, tk;
+  uint64_t c = bsum[threadIdx.x], k = bsum[kBlock + threadIdx.x];
+  uint64_t ec = block_scan_excl<kBlock>(c, tc);
+  uint64_t ek = block_scan_excl<kBlock>(k, tk);
+  bsum[threadIdx.x] = ec;
+  bsum[kBlock + threadIdx.x] = ek;
+  if (threadIdx.x == 0) {
+    a.hb_tot[0] = tc;
+    a.hb_tot[1] = tk;
+    a.hb_cap[n] = tc;
+    a.hb_chunks[n] = tk;
+  }
+}
+
+// Segment offsets added: hb_cap / hb_chunks become global exclusive scans.
+__global__ __launch_bounds__(kBlock) void hub_offsets_kernel(MoveArgs a, const uint64_t* bsum) {
+  const uint64_t n = list_len(a);
+  const uint64_t seg = (n + gridDim.x - 1) / gridDim.x;
+  const uint64_t lo = blockIdx.x * seg, hi = min(n, lo + seg);
+  const uint64_t oc = bsum[blockIdx.x], ok = bsum[kBlock + blockIdx.x];
+  if (!oc && !ok) return;
+  for (uint64_t k = lo + threadIdx.x; k < hi; k += kBlock) {
+    a.hb_cap[k] += oc;
+    a.hb_chunks[k] += ok;
   }
 }
 
@@ -668,41 +748,44 @@ __device__ __forceinline__ uint64_t find_owner(const uint64_t* scan, uint64_t n,
   return lo;
 }
 
-__global__ void hub_clear_kernel(MoveArgs a) {
-  const uint64_t total = a.hb_tot[0];
-  for (uint64_t s = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; s < total; s += (uint64_t)gridDim.x * blockDim.x) {
-    a.hb_key[s] = kEmpty;
-    a.hb_pos[s] = kNoPos;
-    a.hb_val[s] = 0;
-  }
-}
-
 template <class WT>
 __global__ __launch_bounds__(kBlock) void hub_insert_kernel(MoveArgs a) {
-  const uint64_t n = a.n_dev ? (uint64_t)*a.n_dev : a.n;
+  __shared__ uint32_t s_key[kHubLdsTab], s_pos[kHubLdsTab];
+  __shared__ double s_val[kHubLdsTab];
+  const uint64_t n = list_len(a);
   const uint64_t nchunks = a.hb_tot[1];
   const WT* W = (const WT*)a.g.w;
   for (uint64_t q = blockIdx.x; q < nchunks; q += gridDim.x) {
+    for (uint32_t s = threadIdx.x; s < kHubLdsTab; s += kBlock) {
+      s_key[s] = kEmpty;
+      s_pos[s] = kNoPos;
+      s_val[s] = 0;
+    }
+    __syncthreads();
     uint64_t k = find_owner(a.hb_chunks, n, q);
     uint32_t u = a.list[k];
     uint64_t o = a.g.off[u], d = a.g.deg[u];
     uint64_t i0 = (q - a.hb_chunks[k]) * kHubChunk, i1 = min(d, i0 + kHubChunk);
-    uint64_t base = a.hb_cap[k], cap = a.hb_cap[k + 1] - base;
     for (uint64_t i = i0 + threadIdx.x; i < i1; i += kBlock) {
       uint32_t v = a.g.keys[o + i];
       if (v == u) continue;
-      tab_add(a.hb_key + base, a.hb_pos + base, a.hb_val + base, cap, ld_dev(&a.vcom[v]), (double)W[o + i],
-              (uint32_t)i);
+      tab_add(s_key, s_pos, s_val, kHubLdsTab, ld_dev(&a.vcom[v]), (double)W[o + i], (uint32_t)i);
     }
+    __syncthreads();
+    uint64_t base = a.hb_cap[k], cap = a.hb_cap[k + 1] - base;
+    for (uint32_t s = threadIdx.x; s < kHubLdsTab; s += kBlock) {
+      uint32_t c = s_key[s];
+      if (c != kEmpty) tab_merge(a.hb_key + base, a.hb_pos + base, a.hb_val + base, cap, c, s_val[s], s_pos[s]);
+    }
+    __syncthreads();
   }
 }
 
-// Per hub: own community, its weight in the table, Sigma_tot; reset the best.
+// Per hub: own community, its weight in the table, Sigma_tot.
 __global__ void hub_self_kernel(MoveArgs a) {
-  const uint64_t n = a.n_dev ? (uint64_t)*a.n_dev : a.n;
+  const uint64_t n = list_len(a);
   for (uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; k < n; k += (uint64_t)gridDim.x * blockDim.x) {
     uint64_t base = a.hb_cap[k], cap = a.hb_cap[k + 1] - base;
-    a.hb_emax[k] = 0;
     a.hb_bpos[k] = kNoPos;
     if (!cap) continue;
     uint32_t u = a.list[k];
@@ -713,39 +796,68 @@ __global__ void hub_self_kernel(MoveArgs a) {
   }
 }
 
-// pass 0: largest positive gain per hub; pass 1: earliest position among them.
-template <int PASS>
-__global__ void hub_best_kernel(MoveArgs a) {
-  const uint64_t n = a.n_dev ? (uint64_t)*a.n_dev : a.n;
-  const uint64_t total = a.hb_tot[0];
-  for (uint64_t s = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; s < total; s += (uint64_t)gridDim.x * blockDim.x) {
-    uint32_t c = a.hb_key[s];
-    if (c == kEmpty) continue;
-    uint64_t k = find_owner(a.hb_cap, n, s);
-    uint32_t cu = a.hb_cu[k];
-    if (c == cu) continue;
-    if (a.pick_less && c > cu) continue;
-    uint32_t u = a.list[k];
-    double e = delta_q(a.hb_val[s], a.hb_vd[k], a.vtot[u], ld_dev(&a.ctot[c]), a.hb_sd[k], a.M, a.R);
-    if (!(e > 0)) continue;
-    unsigned long long bits = (unsigned long long)__double_as_longlong(e);
-    if (PASS == 0) atomicMax(&a.hb_emax[k], bits);
-    else if (bits == a.hb_emax[k]) atomicMin(&a.hb_bpos[k], a.hb_pos[s]);
+// Best (gain, first position) of each kTabChunk-slot chunk of the tables;
+// the slots are emptied as they are read.
+__global__ __launch_bounds__(kBlock) void hub_best_kernel(MoveArgs a) {
+  const uint64_t n = list_len(a);
+  const uint64_t nt = a.hb_tot[0] / kTabChunk;
+  for (uint64_t t = blockIdx.x; t < nt; t += gridDim.x) {
+    const uint64_t s0 = t * kTabChunk;
+    const uint64_t k = find_owner(a.hb_cap, n, s0);
+    const uint32_t cu = a.hb_cu[k], u = a.list[k];
+    const double vd = a.hb_vd[k], sd = a.hb_sd[k], Ku = a.vtot[u];
+    double be = 0;
+    uint32_t bp = kNoPos, bc = 0;
+    for (uint64_t s = s0 + threadIdx.x; s < s0 + kTabChunk; s += kBlock) {
+      uint32_t c = a.hb_key[s];
+      if (c == kEmpty) continue;
+      uint32_t p = a.hb_pos[s];
+      double w = a.hb_val[s];
+      a.hb_key[s] = kEmpty;
+      a.hb_pos[s] = kNoPos;
+      a.hb_val[s] = 0;
+      if (c == cu || (a.pick_less && c > cu)) continue;
+      double e = delta_q(w, vd, Ku, ld_dev(&a.ctot[c]), sd, a.M, a.R);
+      if (e > 0 && better(e, p, be, bp)) {
+        be = e;
+        bp = p;
+      }
+    }
+    block_best<kBlock>(be, bp, bc);
+    if (threadIdx.x == 0) {
+      a.hb_ce[t] = be;
+      a.hb_cp[t] = bp;
+    }
   }
 }
 
-// Per hub: the move (or the choice, in choose-only mode) and its gain.
-__global__ void hub_apply_kernel(MoveArgs a) {
-  const uint64_t n = a.n_dev ? (uint64_t)*a.n_dev : a.n;
+// Per hub (one wave each): best over its chunks, then the move (or the
+// choice, in choose-only mode) and its gain.
+__global__ __launch_bounds__(kBlock) void hub_apply_kernel(MoveArgs a) {
+  const uint64_t n = list_len(a);
+  const int lane = threadIdx.x % 64;
+  const uint64_t nw = (uint64_t)gridDim.x * (kBlock / 64);
   double el = 0;
   unsigned long long cv = 0, ce = 0, cm = 0, cme = 0;
-  for (uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; k < n; k += (uint64_t)gridDim.x * blockDim.x) {
-    uint64_t cap = a.hb_cap[k + 1] - a.hb_cap[k];
-    if (!cap) continue;
+  for (uint64_t k = blockIdx.x * (uint64_t)(kBlock / 64) + threadIdx.x / 64; k < n; k += nw) {
+    const uint64_t c0 = a.hb_cap[k], c1 = a.hb_cap[k + 1];
+    if (c0 == c1) continue;  // wave-uniform
+    double be = 0;
+    uint32_t bp = kNoPos, bcx = 0;
+    for (uint64_t t = c0 / kTabChunk + lane; t < c1 / kTabChunk; t += 64) {
+      double e = a.hb_ce[t];
+      uint32_t p = a.hb_cp[t];
+      if (p != kNoPos && better(e, p, be, bp)) {
+        be = e;
+        bp = p;
+      }
+    }
+    group_best_shfl<64>(be, bp, bcx);
+    if (lane != 0) continue;
     uint32_t u = a.list[k];
-    double e = a.hb_bpos[k] == kNoPos ? 0.0 : __longlong_as_double((long long)a.hb_emax[k]);
+    double e = bp == kNoPos ? 0.0 : be;
     uint32_t bc = 0;
-    if (a.hb_bpos[k] != kNoPos) bc = ld_dev(&a.vcom[a.g.keys[a.g.off[u] + a.hb_bpos[k]]]);
+    if (bp != kNoPos) bc = ld_dev(&a.vcom[a.g.keys[a.g.off[u] + bp]]);
     uint32_t d = a.g.deg[u];
     if (a.choose_only) {
       a.out_c[u] = bc;
@@ -758,10 +870,9 @@ __global__ void hub_apply_kernel(MoveArgs a) {
       unsafeAtomicAdd(&a.ctot[cu], -Ku);
       unsafeAtomicAdd(&a.ctot[bc], Ku);
       st_dev(&a.vcom[u], bc);
+      a.hb_bpos[k] = bp;  // tells the flag pass this hub moved
       ++cm;
       cme += d;
-    } else {
-      a.hb_bpos[k] = kNoPos;  // tells the flag pass there was no move
     }
     st_dev(&a.vaff[u], (uint8_t)0);
     el += e;
@@ -785,9 +896,8 @@ __global__ void hub_apply_kernel(MoveArgs a) {
 
 // Neighbours of the hubs that moved are flagged (chunks spread over the grid).
 __global__ __launch_bounds__(kBlock) void hub_flag_kernel(MoveArgs a) {
-  const uint64_t n = a.n_dev ? (uint64_t)*a.n_dev : a.n;
+  const uint64_t n = list_len(a);
   const uint64_t nchunks = a.hb_tot[1];
-  unsigned long long cand = 0;
   for (uint64_t q = blockIdx.x; q < nchunks; q += gridDim.x) {
     uint64_t k = find_owner(a.hb_chunks, n, q);
     if (a.hb_bpos[k] == kNoPos) continue;  // uniform per block
@@ -801,7 +911,6 @@ __global__ __launch_bounds__(kBlock) void hub_flag_kernel(MoveArgs a) {
       if (on && v != u) flag_vertex(a.vaff, v, u, a);
     }
   }
-  (void)cand;
 }
 
 // ------------------------------------------------------------ renumbering
@@ -1100,20 +1209,99 @@ __global__ __launch_bounds__(kBlock) void caps_sum_kernel(const uint32_t* deg, u
   if (threadIdx.x == 0 && acc) atomicAdd(out, acc);
 }
 
-// Queue every id whose flag is set (wave-aggregated appends).
-__global__ __launch_bounds__(kBlock) void compact_flags_kernel(const uint8_t* f, uint64_t n, uint32_t* q,
-                                                               uint32_t* qlen) {
-  for (uint64_t b = blockIdx.x * (uint64_t)kBlock; b < n; b += (uint64_t)gridDim.x * kBlock) {
-    uint64_t i = b + threadIdx.x;
-    bool on = i < n && f[i];
-    unsigned long long m = __ballot(on);
-    if (!m) continue;
-    int lane = __lane_id();
-    int leader = __ffsll((long long)m) - 1;
-    uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(qlen, (uint32_t)__popcll(m));
-    base = __shfl(base, leader);
-    if (on) q[base + __popcll(m & ((1ull << lane) - 1))] = (uint32_t)i;
+// Ordered worklist queues (inc/louvain.hxx:594-598 visits ids in ascending
+// order): the ids whose flag byte is set, ascending, split at degree thr
+// into the low queue and the hub queue (thr == 0: all low); with clear, the
+// flags read are reset.  Counts per contiguous block segment, a one-block
+// scan of the counts, then an ordered scatter; flags are read a 4-byte word
+// per thread.
+constexpr uint64_t kWlTile = 4 * kBlock;
+static_assert(kWlScanBlocks % kBlock == 0, "wl_scan_kernel: whole block counts per thread");
+
+__device__ __forceinline__ uint64_t wl_segment(uint64_t S) {
+  uint64_t seg = (S + gridDim.x - 1) / gridDim.x;
+  return (seg + kWlTile - 1) / kWlTile * kWlTile;
+}
+
+__device__ __forceinline__ uint32_t wl_bits(const uint8_t* f, const uint32_t* deg, uint64_t S, uint32_t thr,
+                                            uint64_t id0, uint32_t& hubm) {
+  uint32_t lowm = 0;
+  hubm = 0;
+  uint32_t w = reinterpret_cast<const uint32_t*>(f)[id0 >> 2];
+  if (!w) return 0;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    if (id0 + j < S && ((w >> (8 * j)) & 0xFFu)) {
+      if (thr && deg[id0 + j] > thr) hubm |= 1u << j;
+      else lowm |= 1u << j;
+    }
+  }
+  return lowm;
+}
+
+__global__ __launch_bounds__(kBlock) void wl_count_kernel(const uint8_t* f, uint64_t S, const uint32_t* deg,
+                                                          uint32_t thr, uint64_t* bsum) {
+  const uint64_t seg = wl_segment(S), lo = blockIdx.x * seg, hi = min(S, lo + seg);
+  unsigned long long cl = 0, ch = 0;
+  for (uint64_t id0 = lo + 4 * (uint64_t)threadIdx.x; id0 < hi; id0 += kWlTile) {
+    uint32_t hubm;
+    uint32_t lowm = wl_bits(f, deg, S, thr, id0, hubm);
+    cl += __popc(lowm);
+    ch += __popc(hubm);
+  }
+  cl = block_sum_u64<kBlock>(cl);
+  ch = block_sum_u64<kBlock>(ch);
+  if (threadIdx.x == 0) {
+    bsum[blockIdx.x] = cl;
+    bsum[kWlScanBlocks + blockIdx.x] = ch;
+  }
+}
+
+// One block of kBlock threads, kWlScanBlocks / kBlock block counts each.
+__global__ __launch_bounds__(kBlock) void wl_scan_kernel(uint64_t* bsum, uint32_t* lens) {
+  constexpr int P = kWlScanBlocks / kBlock;
+  uint64_t* a = bsum + (uint64_t)threadIdx.x * P;
+  uint64_t* b = bsum + kWlScanBlocks + (uint64_t)threadIdx.x * P;
+  uint64_t sa = 0, sb = 0;
+  for (int k = 0; k < P; ++k) {
+    sa += a[k];
+    sb += b[k];
+  }
+  uint64_t ta, tb;
+  uint64_t ea = block_scan_excl<kBlock>(sa, ta), eb = block_scan_excl<kBlock>(sb, tb);
+  for (int k = 0; k < P; ++k) {
+    uint64_t x = a[k], y = b[k];
+    a[k] = ea;
+    b[k] = eb;
+    ea += x;
+    eb += y;
+  }
+  if (threadIdx.x == 0) {
+    lens[0] = (uint32_t)ta;
+    lens[1] = (uint32_t)tb;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void wl_scatter_kernel(uint8_t* f, int clear, uint64_t S, const uint32_t* deg,
+                                                            uint32_t thr, const uint64_t* bsum, uint32_t* ql,
+                                                            uint32_t* qh) {
+  const uint64_t seg = wl_segment(S), lo = blockIdx.x * seg, hi = min(S, lo + seg);
+  uint64_t rl = bsum[blockIdx.x], rh = bsum[kWlScanBlocks + blockIdx.x];
+  for (uint64_t t0 = lo; t0 < hi; t0 += kWlTile) {  // block-uniform trip count
+    const uint64_t id0 = t0 + 4 * (uint64_t)threadIdx.x;
+    uint32_t lowm = 0, hubm = 0;
+    if (id0 < hi) lowm = wl_bits(f, deg, S, thr, id0, hubm);
+    uint64_t tl, th;
+    uint64_t el = block_scan_excl<kBlock>(__popc(lowm), tl);
+    uint64_t eh = block_scan_excl<kBlock>(__popc(hubm), th);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      if (lowm & (1u << j)) ql[rl + el++] = (uint32_t)(id0 + j);
+      if (hubm & (1u << j)) qh[rh + eh++] = (uint32_t)(id0 + j);
+    }
+    if (clear && (lowm | hubm)) reinterpret_cast<uint32_t*>(f)[id0 >> 2] = 0;
+    rl += tl;
+    rh += th;
   }
 }
 
@@ -1219,9 +1407,8 @@ void k_update_weights(const uint32_t* du, const float* dw, uint64_t nd, const ui
     LV_LAUNCH(update_weights_kernel, grid_for(nd + ni, kBlock), kBlock, st, du, dw, nd, iu, iw, ni, vcom, vtot, ctot);
 }
 void k_frontier(const uint32_t* du, const uint32_t* dv, uint64_t nd, const uint32_t* iu, const uint32_t* iv,
-                uint64_t ni, const uint32_t* vcom, uint8_t* vaff, uint32_t* q, uint32_t* qlen, hipStream_t st) {
-  if (nd + ni)
-    LV_LAUNCH(frontier_kernel, grid_for(nd + ni, kBlock), kBlock, st, du, dv, nd, iu, iv, ni, vcom, vaff, q, qlen);
+                uint64_t ni, const uint32_t* vcom, uint8_t* vaff, hipStream_t st) {
+  if (nd + ni) LV_LAUNCH(frontier_kernel, grid_for(nd + ni, kBlock), kBlock, st, du, dv, nd, iu, iv, ni, vcom, vaff);
 }
 void k_ds_deletions(const uint32_t* du, const uint32_t* dv, uint64_t nd, const uint32_t* vcom, uint8_t* vertices,
                     uint8_t* neighbors, uint8_t* communities, hipStream_t st) {
@@ -1282,14 +1469,14 @@ void k_move(const MoveArgs& a, int tier, bool f64, hipStream_t st) {
 }
 void k_hub_batch(const MoveArgs& a, hipStream_t st) {
   const uint32_t grid = 2048;
-  LV_LAUNCH(hub_prep_kernel, 1, kHubPrepThreads, st, a);
-  LV_LAUNCH(hub_clear_kernel, grid, kBlock, st, a);
+  LV_LAUNCH(hub_sizes_kernel, kHubPrepBlocks, kBlock, st, a, a.hb_bsum);
+  LV_LAUNCH(hub_scan_kernel, 1, kBlock, st, a, a.hb_bsum);
+  LV_LAUNCH(hub_offsets_kernel, kHubPrepBlocks, kBlock, st, a, a.hb_bsum);
   if (a.g.w && a.f64_weights) LV_LAUNCH(hub_insert_kernel<double>, grid, kBlock, st, a);
   else LV_LAUNCH(hub_insert_kernel<float>, grid, kBlock, st, a);
   LV_LAUNCH(hub_self_kernel, 64, kBlock, st, a);
-  LV_LAUNCH(hub_best_kernel<0>, grid, kBlock, st, a);
-  LV_LAUNCH(hub_best_kernel<1>, grid, kBlock, st, a);
-  LV_LAUNCH(hub_apply_kernel, 16, kBlock, st, a);
+  LV_LAUNCH(hub_best_kernel, grid, kBlock, st, a);
+  LV_LAUNCH(hub_apply_kernel, 64, kBlock, st, a);
   if (!a.choose_only) LV_LAUNCH(hub_flag_kernel, grid, kBlock, st, a);
 }
 void k_exists(GView g, const uint32_t* vcom, uint32_t* flags, hipStream_t st) {
@@ -1358,8 +1545,11 @@ void k_gather_u32(const uint32_t* a, const uint32_t* idx, uint64_t n, uint32_t* 
 void k_caps_sum(const uint32_t* deg, uint64_t n, uint32_t thr, unsigned long long* out, hipStream_t st) {
   if (n) LV_LAUNCH(caps_sum_kernel, grid_for(n, kBlock * 8, 4096), kBlock, st, deg, n, thr, out);
 }
-void k_compact_flags(const uint8_t* f, uint64_t n, uint32_t* q, uint32_t* qlen, hipStream_t st) {
-  if (n) LV_LAUNCH(compact_flags_kernel, grid_for(n, kBlock, 1u << 16), kBlock, st, f, n, q, qlen);
+void k_wl_queue(uint8_t* f, bool clear, uint64_t S, const uint32_t* deg, uint32_t thr, uint32_t* ql, uint32_t* qh,
+                uint32_t* lens, uint64_t* bsum, hipStream_t st) {
+  LV_LAUNCH(wl_count_kernel, kWlScanBlocks, kBlock, st, f, S, deg, thr, bsum);
+  LV_LAUNCH(wl_scan_kernel, 1, kBlock, st, bsum, lens);
+  LV_LAUNCH(wl_scatter_kernel, kWlScanBlocks, kBlock, st, f, (int)clear, S, deg, thr, bsum, ql, qh);
 }
 void k_modularity_parts(GView g, bool f64, const uint32_t* memb, double* cin, double* ctot, hipStream_t st) {
   if (!g.S) return;

@@ -29,7 +29,7 @@ LIB_PATH = os.path.join(HERE, "liblouvain_hip.so")
 LV_STATIC, LV_NAIVE_DYNAMIC, LV_DELTA_SCREENING, LV_DYNAMIC_FRONTIER = 0, 1, 2, 3
 APPROACHES = {"static": LV_STATIC, "naive-dynamic": LV_NAIVE_DYNAMIC,
               "delta-screening": LV_DELTA_SCREENING, "dynamic-frontier": LV_DYNAMIC_FRONTIER}
-PATH_SWEEP, PATH_WORKLIST = 0, 1
+PATH_SWEEP, PATH_FRONTIER, PATH_WORKLIST = 0, 1, 2
 
 # Every symbol include/louvain_hip.h declares.
 EXPORTS = [
@@ -167,7 +167,7 @@ class _BatchArgs:
 class Context:
     """One HIP device + stream + grow-only device workspace (lv_context)."""
 
-    def __init__(self, device=0, path=PATH_WORKLIST):
+    def __init__(self, device=0, path=PATH_FRONTIER):
         self.lib = _LIB
         self.h = C.c_void_p()
         rc = self.lib.lv_context_create(int(device), C.byref(self.h))

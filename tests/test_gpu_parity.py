@@ -142,7 +142,7 @@ def test_weights_and_modularity(ctx, oracle, name):
     assert abs(q - float(f["static_Q"])) < 1e-12
 
 
-@pytest.mark.parametrize("path", [0, 1])
+@pytest.mark.parametrize("path", [0, 1, 2])
 @pytest.mark.parametrize("name", FIXTURES)
 def test_dynamic_end_to_end(ctx, oracle, name, path):
     """ND / DS / DF from the reference's static state: K exact, modularity
@@ -158,7 +158,7 @@ def test_dynamic_end_to_end(ctx, oracle, name, path):
             key = approach.replace("-", "_")
             r = ctx.louvain(approach, y, d, i, f["static_memb"], f["static_vtot"], f["static_ctot"], o)
             q = oracle.modularity(y, r.membership)
-            two = approach == "dynamic-frontier" and path == 1
+            two = approach == "dynamic-frontier" and path in (0, 1)
             q_ref = float(f[key + "_Q"]) if two else _reference_band(oracle, approach, y, d, i, f, o)
             _say(f"  {approach}: Q gpu={q:.6f} ref={q_ref:.6f} (1-thread {float(f[key + '_Q']):.6f}) "
                  f"it={r.iterations} p={r.passes}")
@@ -191,15 +191,15 @@ def _reference_band(oracle, approach, y, d, i, f, o):
 
 
 def _check_q(q_gpu, q_ref, two_sided):
-    """DF on the worklist path (one pass from a given state, with the
-    reference's rule for which flagged vertices join the current iteration)
-    must match the reference within 1e-3 both ways.  The plain sweep path
-    picks up every vertex flagged before its turn, and the multi-pass
-    approaches re-optimise the whole graph, where the reference's own result
-    moves by more than 1e-3 with its thread count (Static on a 100k planted
-    partition: Q 0.518 at one thread, 0.574 at eight); for those the bar is
-    "not worse than the lowest of the reference's 1/2/4/8-thread results by
-    more than 1e-3"."""
+    """DF on the sweep paths (ascending ids; a vertex flagged by a lower one
+    is visited in the same sweep, as in the reference) must match the
+    reference within 1e-3 both ways.  ND and DS re-optimise the whole graph,
+    where the reference's own result moves by more than 1e-3 with its thread
+    count (Static on a 100k planted partition: Q 0.518 at one thread, 0.574
+    at eight), and the worklist path visits the vertices that a sub-round
+    flags after the rest of that sub-round; for those the bar is "not worse
+    than the lowest of the reference's 1/2/4/8-thread results by more than
+    1e-3"."""
     if two_sided:
         assert abs(q_gpu - q_ref) <= Q_TOL, (q_gpu, q_ref)
     else:
@@ -261,14 +261,17 @@ def test_out_of_range_batch_rejected(ctx):
     assert e.value.status == -1
 
 
+@pytest.mark.parametrize("path", [0, 1])
 @pytest.mark.parametrize("name", INTEGER_FIXTURES)
-def test_single_stream_reproduces_reference_bit_exact(ctx, name):
+def test_single_stream_reproduces_reference_bit_exact(ctx, name, path):
     """One stream, ascending ids, every vertex in the id-order kernel: the
     GPU's first local-moving pass is the reference's sequential sweep
     (inc/louvain.hxx:594-605 at one thread), so membership, K, Sigma_tot,
-    iterations and affected count equal the golden vectors bit for bit."""
+    iterations and affected count equal the golden vectors bit for bit --
+    on the plain path (every id checked) and on the frontier path (flags
+    read 64 ids at a time, re-read after every visit)."""
     import lvhip
-    _say("case", name)
+    _say("case", name, "path", path)
     f = F.load(name)
     x, y, d, i = _g(f)
     o = lvhip.LouvainOptions(*[t(v) for t, v in zip((int, float, float, float, float, int, int), f["opts"])])
@@ -280,13 +283,13 @@ def test_single_stream_reproduces_reference_bit_exact(ctx, name):
     orc.set_threads(1)
     ctx.set_schedule(1e-12, 1, 1, 16, 2)
     ctx.set_hub_threshold(1 << 30, 4096)
+    ctx.set_path(path)
     try:
         for approach in ("static", "naive-dynamic", "dynamic-frontier"):
             if approach == "static":
                 r = ctx.louvain(approach, x, o=o)
                 ref = orc.louvain(approach, x, options=oo)
             else:
-                ctx.set_path(0)
                 r = ctx.louvain(approach, y, d, i, f["static_memb"], f["static_vtot"], f["static_ctot"], o)
                 ref = orc.louvain(approach, y, d, i, f["static_memb"], f["static_vtot"], f["static_ctot"], oo)
             _say(f"  {approach}: it={r.iterations}/{ref.iterations} aff={r.affectedVertices}/{ref.affected_vertices}")
