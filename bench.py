@@ -88,7 +88,7 @@ def main():
                     help="streams per live vertex of a level (lv_context_set_schedule)")
     ap.add_argument("--worklist-hub-degree", type=int, default=None,
                     help="degree above which worklist vertices run in the hub batch (256..4096)")
-    ap.add_argument("--path", choices=("frontier", "worklist", "plain"), default="frontier",
+    ap.add_argument("--path", choices=("frontier", "worklist", "plain"), default="worklist",
                     help="local-moving path of the timed chain (the others are timed beside it)")
     args = ap.parse_args()
     if args.gpus != 1:

@@ -112,11 +112,11 @@ int lv_context_create(int device, lv_context** out);
 void lv_context_destroy(lv_context* ctx);
 const char* lv_last_error(const lv_context* ctx);
 /* Path used by the local-moving phase.  0 (plain): each iteration is one
- * sweep in which every vertex is checked by its wave.  1 (frontier,
- * default): sweeps read the flags of 64 ids at once and visit only the
- * flagged ones.  2 (worklist): DF / DS first passes run over queues of the
- * flagged vertices, in sub-rounds until no vertex is flagged for the
- * current iteration. */
+ * sweep in which every vertex is checked by its wave.  1 (frontier):
+ * sweeps read the flags of 64 ids at once and visit only the flagged ones.
+ * 2 (worklist, default): as 1, and DF / DS first passes run over ascending
+ * queues of the flagged vertices, in sub-rounds until no vertex is flagged
+ * by a lower id that it has not been visited after. */
 int lv_context_set_path(lv_context* ctx, int path);
 /* Scheduling of the local-moving kernels, the role of the reference's
  * schedule(dynamic, 2048) loop (inc/louvain.hxx:594).  Each launch runs

@@ -48,7 +48,7 @@ struct lv_context {
   int device = 0;
   hipStream_t st = nullptr;
   std::string err;
-  int path = 1;
+  int path = 2;
   // Local-moving streams per launch: clamp(stream_fraction * live vertices of
   // the level, min_streams, max_streams) lane groups, each walking dynamically
   // consecutive list entries sequentially.

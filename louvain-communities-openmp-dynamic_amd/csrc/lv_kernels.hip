@@ -130,23 +130,7 @@ __device__ unsigned long long block_sum_u64(unsigned long long x) {
   return t;
 }
 
-// Wave-aggregated append of the lanes with `on` set (one atomic per wave).
-__device__ __forceinline__ void wave_push(bool on, uint32_t v, uint32_t* q, uint32_t* qlen) {
-  unsigned long long mask = __ballot(on);
-  if (!mask) return;
-  int lane = __lane_id();
-  int leader = __ffsll((long long)mask) - 1;
-  uint32_t base = 0;
-  if (lane == leader) base = atomicAdd(qlen, (uint32_t)__popcll(mask));
-  base = __shfl(base, leader);
-  if (on) q[base + __popcll(mask & ((1ull << lane) - 1))] = v;
-}
-
 // vaff[v] = 1, as a moved vertex u flags its neighbour v (inc/louvain.hxx:602).
-// Worklist mode queues v when its flag goes 0 -> 1: into the same iteration
-// when v > u and v was not visited yet this iteration (the reference's
-// ascending sweep reaches such a v later in the same iteration), otherwise
-// into the next iteration.
 __device__ __forceinline__ void flag_vertex(uint8_t* vaff, uint32_t v, uint32_t u, const MoveArgs& a) {
   __hip_atomic_store(&vaff[v], (uint8_t)1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   // worklist mode: a vertex flagged by a lower id and not visited in this
@@ -244,7 +228,7 @@ __global__ void update_weights_kernel(const uint32_t* du, const float* dw, uint6
 }
 
 // louvainAffectedVerticesFrontierOmpW (inc/louvain.hxx:1606-1624); vaff must
-// be zero on entry.  With q set, each newly marked vertex is also queued.
+// be zero on entry.
 __global__ void frontier_kernel(const uint32_t* du, const uint32_t* dv, uint64_t nd, const uint32_t* iu,
                                 const uint32_t* iv, uint64_t ni, const uint32_t* vcom, uint8_t* vaff) {
   const uint64_t n = nd + ni;

@@ -167,7 +167,7 @@ class _BatchArgs:
 class Context:
     """One HIP device + stream + grow-only device workspace (lv_context)."""
 
-    def __init__(self, device=0, path=PATH_FRONTIER):
+    def __init__(self, device=0, path=PATH_WORKLIST):
         self.lib = _LIB
         self.h = C.c_void_p()
         rc = self.lib.lv_context_create(int(device), C.byref(self.h))

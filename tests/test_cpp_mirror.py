@@ -1,0 +1,115 @@
+"""The C++ mirror (include/louvain_hip.hpp): the reference's four entry-point
+names with a Hip suffix, a DiGraph-like graph and vector<tuple<K, K, V>>
+batches, compiled with g++ against liblouvain_hip.so and driven the way
+main.cxx drives inc/louvain.hxx."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import fixtures as F
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PKG = os.path.join(ROOT, "louvain-communities-openmp-dynamic_amd")
+NODEVICE = -5
+
+
+def _build(tmp):
+    exe = os.path.join(str(tmp), "mirror")
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"),
+           os.path.join(ROOT, "tests", "cpp", "mirror_main.cpp"), "-L", PKG, "-llouvain_hip",
+           "-Wl,-rpath," + PKG, "-o", exe]
+    subprocess.run(cmd, check=True, capture_output=True, text=True)
+    return exe
+
+
+def _graph_bytes(g, S):
+    off = np.asarray(g["off"], np.uint64)
+    ex = np.asarray(g.get("ex", np.ones(S, np.uint8)), np.uint8)
+    return b"".join([np.uint64(off[-1]).tobytes(), off.tobytes(), np.asarray(g["keys"], np.uint32).tobytes(),
+                     np.asarray(g["w"], np.float32).tobytes(), ex.tobytes()])
+
+
+def _batch_bytes(b):
+    n = len(b["u"])
+    return b"".join([np.uint64(n).tobytes(), np.asarray(b["u"], np.uint32).tobytes(),
+                     np.asarray(b["v"], np.uint32).tobytes(), np.asarray(b["w"], np.float32).tobytes()])
+
+
+def _write_input(path, f):
+    x, y = F.graph(f, "x"), F.graph(f, "y")
+    S = len(x["off"]) - 1
+    with open(path, "wb") as fh:
+        fh.write(np.uint64(S).tobytes())
+        fh.write(_graph_bytes(x, S))
+        fh.write(_graph_bytes(y, S))
+        fh.write(_batch_bytes(F.batch(f, "del")))
+        fh.write(_batch_bytes(F.batch(f, "ins")))
+        fh.write(np.asarray(f["static_memb"], np.uint32).tobytes())
+        fh.write(np.asarray(f["static_vtot"], np.float64).tobytes())
+        fh.write(np.asarray(f["static_ctot"], np.float64).tobytes())
+    return S
+
+
+def _read_output(path, S):
+    raw = open(path, "rb").read()
+    out, o = [], 0
+    for _ in range(4):
+        it, ps = np.frombuffer(raw, np.int32, 2, o)
+        o += 8
+        memb = np.frombuffer(raw, np.uint32, S, o)
+        o += 4 * S
+        vt = np.frombuffer(raw, np.float64, S, o)
+        o += 8 * S
+        ct = np.frombuffer(raw, np.float64, S, o)
+        o += 8 * S
+        out.append({"iterations": int(it), "passes": int(ps), "membership": memb, "vtot": vt, "ctot": ct})
+    assert o == len(raw)
+    return out
+
+
+def test_mirror_compiles_and_reports_status(tmp_path):
+    """Compiles warning-free against the header and the library; without a
+    GPU the first call throws louvain_hip::Error carrying LV_ERR_NODEVICE."""
+    exe = _build(tmp_path)
+    f = F.load("tiny1")
+    _write_input(tmp_path / "in.bin", f)
+    r = subprocess.run([exe, str(tmp_path / "in.bin"), str(tmp_path / "out.bin")], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode in (0, 3), r.stderr
+    if r.returncode == 3:
+        assert r.stdout.startswith(f"lv_status {NODEVICE}:"), r.stdout
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["pp2k", "rmat10", "tiny1"])
+def test_mirror_end_to_end(tmp_path, oracle, name):
+    """Static on x and ND / DS / DF on y through the mirror: the same bars as
+    the C-ABI end-to-end tests (K exact, DF modularity within 1e-3 of the
+    reference's, the others not worse than its lowest by more than 1e-3)."""
+    from test_gpu_parity import _reference_band, Q_TOL
+    import lvhip
+    exe = _build(tmp_path)
+    f = F.load(name)
+    S = _write_input(tmp_path / "in.bin", f)
+    print("case", name, flush=True)
+    r = subprocess.run([exe, str(tmp_path / "in.bin"), str(tmp_path / "out.bin")], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    res = _read_output(tmp_path / "out.bin", S)
+    x, y, d, i = F.graph(f, "x"), F.graph(f, "y"), F.batch(f, "del"), F.batch(f, "ins")
+    q = oracle.modularity(x, res[0]["membership"])
+    tol = 0.05 if name.startswith("tiny") else 0.02
+    assert q >= float(f["static_Q"]) - tol
+    assert np.array_equal(res[0]["vtot"], f["static_vtot"])
+    o = lvhip.LouvainOptions()
+    for k, approach in ((1, "naive-dynamic"), (2, "delta-screening"), (3, "dynamic-frontier")):
+        key = approach.replace("-", "_")
+        q = oracle.modularity(y, res[k]["membership"])
+        if approach == "dynamic-frontier":
+            ref = oracle.louvain(approach, y, d, i, f["static_memb"], f["static_vtot"], f["static_ctot"])
+            assert abs(q - oracle.modularity(y, ref.membership)) <= Q_TOL
+        else:
+            assert q >= _reference_band(oracle, approach, y, d, i, f, o) - Q_TOL
+        assert np.array_equal(res[k]["vtot"], f[key + "_vtot"])
