@@ -24,6 +24,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "louvain-communities-openmp-dynamic_amd"))
 
 HBM_PEAK_GBPS = 8000.0
+KERNEL_NAME = "move_kernel<64, 64, 512, float, 2>"
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "move_kernel_df_rmat22_pmc.txt")
 METRIC = "DF-Louvain edges/s & p99 batch latency at dQ<=1e-3 vs plain GPU path"
 
 
@@ -44,6 +46,20 @@ def algorithmic_bytes(st, tier):
     the membership store) and 1 B per flagged neighbour."""
     return (12 * st["edges"][tier] + 29 * st["vertices"][tier] + 8 * st["candidates"][tier]
             + 20 * st["moves"][tier] + st["moved_edges"][tier])
+
+
+def pmc_traffic(kernel):
+    """HBM bytes per launch of `kernel` on this workload, from the committed
+    FETCH_SIZE / WRITE_SIZE summary (two rocprofv3 --pmc runs of this script,
+    each scaled by the counter's reading for a copy of known size); None if
+    the summary is absent."""
+    if not os.path.exists(PMC_SUMMARY):
+        return None
+    for line in open(PMC_SUMMARY):
+        parts = line.split("\t")
+        if len(parts) >= 2 and parts[0] == "traffic_bytes_per_launch " + kernel:
+            return {"bytes_per_launch": float(parts[1]), "source": os.path.relpath(PMC_SUMMARY, ROOT)}
+    return None
 
 
 def run_chain(ctx, lvhip, dgraphs, dbatches, state, state0, W, K, path):
@@ -180,16 +196,27 @@ def main():
                 f"{args.path} path {others[name]['speedup']:.2f}x faster")
     ctx.set_path(paths[args.path])
 
-    # Roofline of the dominant local-moving kernel (the tier with most time).
-    tier = int(np.argmax(st["move_ms"]))
-    launches = max(st["move_launches"][tier], 1)
-    ms = st["move_ms"][tier]
-    bytes_total = algorithmic_bytes(st, tier)
-    achieved = (bytes_total / (ms * 1e-3) / 1e9) if ms > 0 else 0.0
-    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
-                "kernel": f"move_kernel tier {tier}", "avg_launch_us": ms * 1e3 / launches,
-                "bytes_per_launch": bytes_total / launches, "launches": launches}
+    # Roofline of the dominant local-moving kernel.  Stats slots: 0 = the
+    # id-order kernel (one launch per sub-round / sweep), 4 = the hub batch
+    # (eight kernels per launch); 1-3 are only used by the degree-tier modes.
+    def kernel_line(slot, name):
+        launches = max(st["move_launches"][slot], 1)
+        ms = st["move_ms"][slot]
+        b = algorithmic_bytes(st, slot)
+        gbps = (b / (ms * 1e-3) / 1e9) if ms > 0 else 0.0
+        return {"kernel": name, "achieved": gbps, "avg_launch_us": ms * 1e3 / launches,
+                "bytes_per_launch": b / launches, "launches": launches}
+    main_k = kernel_line(0, KERNEL_NAME)
+    hub_k = kernel_line(4, "hub batch (hub_sizes/scan/offsets/insert/self/best/apply/flag)")
+    traffic = pmc_traffic(KERNEL_NAME)
+    roofline = {"bound": "hbm", "achieved": main_k["achieved"], "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                "frac": main_k["achieved"] / HBM_PEAK_GBPS, "traffic": traffic and traffic["bytes_per_launch"],
+                "kernel": main_k["kernel"], "avg_launch_us": main_k["avg_launch_us"],
+                "bytes_per_launch": main_k["bytes_per_launch"], "launches": main_k["launches"],
+                "traffic_source": traffic and traffic["source"], "hub_batch": hub_k}
+    log(f"roofline: {main_k['kernel']}: {main_k['bytes_per_launch'] / 1e6:.2f} MB algorithmic per launch, "
+        f"{main_k['avg_launch_us']:.1f} us -> {main_k['achieved']:.1f} GB/s; hub batch {hub_k['avg_launch_us']:.1f} us, "
+        f"{hub_k['achieved']:.1f} GB/s")
 
     cpu = None
     dq_max = None

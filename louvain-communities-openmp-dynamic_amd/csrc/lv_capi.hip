@@ -867,6 +867,16 @@ void graph_check(const lv_graph* g) {
   require(g->span < 0xFFFFFFFFull, LV_ERR_LIMIT, "span must fit 32-bit vertex ids");
   uint64_t E = g->offsets[g->span];
   require(E == 0 || (g->keys && (g->weights || g->weights_f64)), LV_ERR_INVALID, "graph keys/weights NULL");
+  // rows must lie inside the edge arrays: offsets ascending from 0, a gapped
+  // row's degree within its capacity (the kernels index keys[off[u] + i])
+  require(g->offsets[0] == 0, LV_ERR_INVALID, "graph offsets[0] must be 0");
+  bool ascending = true, within = true;
+  for (uint64_t u = 0; u < g->span; ++u) {
+    ascending &= g->offsets[u] <= g->offsets[u + 1];
+    within &= !g->degrees || g->degrees[u] <= g->offsets[u + 1] - g->offsets[u];
+  }
+  require(ascending, LV_ERR_INVALID, "graph offsets not ascending");
+  require(within, LV_ERR_INVALID, "graph degree exceeds its row's capacity");
 }
 
 lv_dgraph* upload_graph(lv_context* c, const lv_graph* g) {
@@ -897,10 +907,15 @@ lv_dgraph* upload_graph(lv_context* c, const lv_graph* g) {
       d->order = n;
     }
     if (d->E) {
-      uint32_t* mx = c->ws<uint32_t>("maxkey", 1);
-      k_fill_u32(mx, 0, 1, c->st);
-      k_max_u32(d->keys, d->E, mx, c->st);
-      require(c->read1(mx) < d->S, LV_ERR_INVALID, "graph has a neighbour id >= span");
+      int* bad = c->ws<int>("rows.bad", 1);
+      k_fill_u32(reinterpret_cast<uint32_t*>(bad), 0, 1, c->st);
+      if (g->degrees) k_check_rows(d->off, d->deg, d->keys, d->S, bad, c->st);
+      else k_check_edges(d->off, d->keys, d->S, d->E, c->ws<uint32_t>("rows.starts", d->E / 32 + 1), bad, c->st);
+      int f = c->read1(bad);
+      require(!(f & 1), LV_ERR_INVALID, "graph has a neighbour id >= span");
+      // input graphs keep DiGraph's sorted rows; super-graph rows (f64
+      // weights) are in the aggregation's first-seen order
+      require(d->f64 || !(f & 2), LV_ERR_UNSORTED, "graph row not sorted by neighbour id, or a repeated neighbour");
     }
     // 2M (edgeWeightOmp), computed once per graph like the reference's M
     double* m2 = c->ws<double>("m2", 1);

@@ -152,6 +152,11 @@ void k_ds_sweep(GView g, const uint32_t* vcom, const uint8_t* neighbors, const u
                 uint8_t* vertices, hipStream_t st);
 void k_count_u8(const uint8_t* a, uint8_t v, uint64_t n, unsigned long long* out, hipStream_t st);
 void k_check_sorted(const uint32_t* u, const uint32_t* v, uint64_t n, uint32_t S, int* bad, hipStream_t st);
+void k_check_rows(const uint64_t* off, const uint32_t* deg, const uint32_t* keys, uint64_t S, int* bad,
+                  hipStream_t st);
+// Ungapped CSR (offsets[S] edges): starts holds E / 32 + 1 words of scratch.
+void k_check_edges(const uint64_t* off, const uint32_t* keys, uint64_t S, uint64_t E, uint32_t* starts, int* bad,
+                   hipStream_t st);
 void k_move(const MoveArgs& a, int tier, bool f64, hipStream_t st);
 void k_exists(GView g, const uint32_t* vcom, uint32_t* flags, hipStream_t st);
 void k_lookup(uint32_t* a, uint64_t n, const uint32_t* map, hipStream_t st);

@@ -336,6 +336,46 @@ __global__ void check_sorted_kernel(const uint32_t* u, const uint32_t* v, uint64
   }
 }
 
+// Graph rows (the live part of a gapped row): neighbour ids < S (bit 0) and
+// strictly ascending (bit 1), the order DiGraph keeps (inc/_bitset.hxx:245-262).
+// One wave per row.
+__global__ __launch_bounds__(kBlock) void check_rows_kernel(const uint64_t* off, const uint32_t* deg,
+                                                            const uint32_t* keys, uint64_t S, int* bad) {
+  const int lane = threadIdx.x % 64;
+  const uint64_t nw = (uint64_t)gridDim.x * (kBlock / 64);
+  for (uint64_t u = blockIdx.x * (uint64_t)(kBlock / 64) + threadIdx.x / 64; u < S; u += nw) {
+    const uint64_t o = off[u];
+    const uint32_t d = deg[u];
+    int b = 0;
+    for (uint32_t i = lane; i < d; i += 64) {
+      uint32_t k = keys[o + i];
+      if (k >= S) b |= 1;
+      if (i + 1 < d && k >= keys[o + i + 1]) b |= 2;
+    }
+    if (b) atomicOr(bad, b);
+  }
+}
+
+// Ungapped CSR: the same checks edge-parallel.  Bit i of `starts` is set when
+// a row starts at edge i, so edge i and i + 1 share a row unless bit i + 1
+// is set.
+__global__ void row_starts_kernel(const uint64_t* off, uint64_t S, uint32_t* starts) {
+  for (uint64_t u = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; u < S; u += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t o = off[u];
+    if (off[u + 1] > o) atomicOr(&starts[o >> 5], 1u << (o & 31));
+  }
+}
+
+__global__ void check_edges_kernel(const uint32_t* keys, uint64_t E, uint64_t S, const uint32_t* starts, int* bad) {
+  int b = 0;
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < E; i += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t k = keys[i];
+    if (k >= S) b |= 1;
+    if (i + 1 < E && !((starts[(i + 1) >> 5] >> ((i + 1) & 31)) & 1u) && k >= keys[i + 1]) b |= 2;
+  }
+  if (__ballot(b != 0) && b) atomicOr(bad, b);
+}
+
 // ------------------------------------------------------- local-moving
 // Shared mutable state (memberships, Sigma_tot, flags) is read and written
 // at device scope so that a stream sees the moves other streams made.
@@ -1416,6 +1456,17 @@ void k_count_u8(const uint8_t* a, uint8_t v, uint64_t n, unsigned long long* out
 }
 void k_check_sorted(const uint32_t* u, const uint32_t* v, uint64_t n, uint32_t S, int* bad, hipStream_t st) {
   if (n) LV_LAUNCH(check_sorted_kernel, grid_for(n, kBlock), kBlock, st, u, v, n, S, bad);
+}
+void k_check_rows(const uint64_t* off, const uint32_t* deg, const uint32_t* keys, uint64_t S, int* bad,
+                  hipStream_t st) {
+  if (S) LV_LAUNCH(check_rows_kernel, grid_for(S, kBlock / 64, 1u << 16), kBlock, st, off, deg, keys, S, bad);
+}
+void k_check_edges(const uint64_t* off, const uint32_t* keys, uint64_t S, uint64_t E, uint32_t* starts, int* bad,
+                   hipStream_t st) {
+  if (!S || !E) return;
+  LV_LAUNCH(fill_kernel<uint32_t>, grid_for(E / 32 + 1, kBlock * 4), kBlock, st, starts, 0u, E / 32 + 1);
+  LV_LAUNCH(row_starts_kernel, grid_for(S, kBlock, 1u << 16), kBlock, st, off, S, starts);
+  LV_LAUNCH(check_edges_kernel, grid_for(E, kBlock * 8, 1u << 16), kBlock, st, keys, E, S, starts, bad);
 }
 
 template <class WT>

@@ -249,6 +249,29 @@ def test_unsorted_batch_rejected(ctx):
     assert e.value.status == -4
 
 
+@pytest.mark.parametrize("defect", ["unsorted", "repeated", "out_of_range", "offsets"])
+def test_bad_graph_rows_rejected(ctx, defect):
+    """Input rows must be DiGraph's: neighbour ids < span, strictly ascending,
+    inside the edge arrays."""
+    import lvhip
+    f = F.load("pp2k")
+    x = {k: v.copy() for k, v in F.graph(f, "x").items()}
+    off, keys = x["off"], x["keys"]
+    u = int(np.argmax(np.diff(off) >= 3))  # a row with at least three entries
+    o = int(off[u])
+    if defect == "unsorted":
+        keys[o], keys[o + 1] = keys[o + 1], keys[o]
+    elif defect == "repeated":
+        keys[o + 1] = keys[o]
+    elif defect == "offsets":
+        off[u + 1] = off[u] - 1
+    else:
+        keys[o + 2] = len(off) - 1 + 5
+    with pytest.raises(lvhip.LvError) as e:
+        ctx.louvain("static", x)
+    assert e.value.status == (-4 if defect in ("unsorted", "repeated") else -1)
+
+
 def test_out_of_range_batch_rejected(ctx):
     import lvhip
     f = F.load("tiny1")
