@@ -205,7 +205,8 @@ def main():
         b = algorithmic_bytes(st, slot)
         gbps = (b / (ms * 1e-3) / 1e9) if ms > 0 else 0.0
         return {"kernel": name, "achieved": gbps, "avg_launch_us": ms * 1e3 / launches,
-                "bytes_per_launch": b / launches, "launches": launches}
+                "bytes_per_launch": b / launches, "launches": launches,
+                "per_launch": {k: st[k][slot] / launches for k in ("vertices", "edges", "candidates", "moves")}}
     main_k = kernel_line(0, KERNEL_NAME)
     hub_k = kernel_line(4, "hub batch (hub_sizes/scan/offsets/insert/self/best/apply/flag)")
     traffic = pmc_traffic(KERNEL_NAME)
@@ -216,7 +217,8 @@ def main():
                 "traffic_source": traffic and traffic["source"], "hub_batch": hub_k}
     log(f"roofline: {main_k['kernel']}: {main_k['bytes_per_launch'] / 1e6:.2f} MB algorithmic per launch, "
         f"{main_k['avg_launch_us']:.1f} us -> {main_k['achieved']:.1f} GB/s; hub batch {hub_k['avg_launch_us']:.1f} us, "
-        f"{hub_k['achieved']:.1f} GB/s")
+        f"{hub_k['achieved']:.1f} GB/s; per launch " + ", ".join(
+            f"{k} {main_k['per_launch'][k]:.0f}/{hub_k['per_launch'][k]:.0f}" for k in main_k["per_launch"]))
 
     cpu = None
     dq_max = None

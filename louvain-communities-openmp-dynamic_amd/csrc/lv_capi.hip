@@ -280,6 +280,7 @@ struct Counters {
 // context profiles (per-launch durations on the context's own stream).
 constexpr int kTierHubBatch = 6;  // launch_move: hubs through k_hub_batch (stats slot 4)
 constexpr int kSubRounds = 4;      // worklist sub-rounds enqueued per host check
+constexpr uint64_t kRefChunk = 2048;  // the reference's OpenMP chunk (inc/louvain.hxx:594)
 
 void launch_move(lv_context* c, const MoveArgs& a, int tier, bool f64) {
   k_fill_u64(reinterpret_cast<uint64_t*>(a.work), 0, 1, c->st);
@@ -290,7 +291,12 @@ void launch_move(lv_context* c, const MoveArgs& a, int tier, bool f64) {
   if (c->profiling) c->prof.push_back({tier == kTierHubBatch ? 4 : (tier == kTierHybrid ? 0 : tier), {e0, c->record()}});
 }
 
+// Streams of a level: stream_fraction of its live vertices, clamped.  A
+// level of at most one reference chunk (2048 ids, the schedule(dynamic,
+// 2048) of inc/louvain.hxx:594) is one thread's work in the reference, so it
+// runs as one stream.
 uint64_t level_streams(const lv_context* c, uint64_t order) {
+  if (order <= kRefChunk) return 1;
   uint64_t want = (uint64_t)(c->stream_fraction * (double)order);
   return std::min(c->max_streams, std::max<uint64_t>(c->min_streams, want));
 }

@@ -257,7 +257,7 @@ def test_bad_graph_rows_rejected(ctx, defect):
     f = F.load("pp2k")
     x = {k: v.copy() for k, v in F.graph(f, "x").items()}
     off, keys = x["off"], x["keys"]
-    u = int(np.argmax(np.diff(off) >= 3))  # a row with at least three entries
+    u = int(np.argmax((np.diff(off) >= 3) & (off[:-1] > 0)))  # a row of at least three entries, not the first
     o = int(off[u])
     if defect == "unsorted":
         keys[o], keys[o + 1] = keys[o + 1], keys[o]
