@@ -198,9 +198,32 @@ typedef struct lv_dstate lv_dstate;
 int lv_dgraph_create(lv_context* ctx, const lv_graph* g, lv_dgraph** out);
 void lv_dgraph_destroy(lv_dgraph* g);
 uint64_t lv_dgraph_span(const lv_dgraph* g);
+/* Directed edges (sum of the row degrees). */
 uint64_t lv_dgraph_edges(const lv_dgraph* g);
+/* Edge slots (offsets[span]): rows may carry slack after lv_dgraph_update. */
+uint64_t lv_dgraph_capacity(const lv_dgraph* g);
+/* Batch tidy and graph update on the device: tidyBatchUpdateU
+ * (inc/batch.hxx:202-209) then applyBatchUpdateU (inc/batch.hxx:225-236).
+ * Deletions are kept when the edge exists in g, insertions when it does not;
+ * each list is sorted by (u, v) and made unique (the first of repeated
+ * entries, in input order, is kept).  g is then updated in place: rows stay
+ * sorted by neighbour id, an insertion's endpoints become existing vertices,
+ * 2M and the degree lists follow.  A row that outgrows its slack moves every
+ * row to a new CSR with slack max(4, degree / 8).  *tidy receives the tidy
+ * batch, ready for lv_run on the updated graph.  Ids must be < span (the
+ * span does not grow: LV_ERR_INVALID); only input graphs (f32 weights). */
+int lv_dgraph_update(lv_context* ctx, lv_dgraph* g, const lv_batch* deletions, const lv_batch* insertions,
+                     lv_dbatch** tidy);
+/* The device graph's CSR: offsets[span + 1], degrees[span], keys and weights
+ * of lv_dgraph_capacity entries (row u is keys[offsets[u] ..
+ * offsets[u] + degrees[u])), exists[span] (may be NULL). */
+int lv_dgraph_download(lv_context* ctx, const lv_dgraph* g, uint64_t* offsets, uint32_t* degrees, uint32_t* keys,
+                       float* weights, uint8_t* exists);
 int lv_dbatch_create(lv_context* ctx, const lv_batch* deletions, const lv_batch* insertions, lv_dbatch** out);
 void lv_dbatch_destroy(lv_dbatch* b);
+int lv_dbatch_sizes(const lv_dbatch* b, uint64_t* deletions, uint64_t* insertions);
+int lv_dbatch_download(lv_context* ctx, const lv_dbatch* b, uint32_t* du, uint32_t* dv, float* dw, uint32_t* iu,
+                       uint32_t* iv, float* iw);
 int lv_dstate_create(lv_context* ctx, uint64_t span, lv_dstate** out);
 void lv_dstate_destroy(lv_dstate* s);
 int lv_dstate_upload(lv_context* ctx, lv_dstate* s, const uint32_t* q, const double* qvtot, const double* qctot);

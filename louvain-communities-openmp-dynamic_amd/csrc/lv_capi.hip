@@ -151,7 +151,8 @@ struct lv_context {
 
 struct lv_dgraph {
   lv_context* ctx = nullptr;
-  uint64_t S = 0, E = 0, order = 0;
+  uint64_t S = 0, E = 0, order = 0;  // span, edge capacity (offsets[S]), existing vertices
+  uint64_t live = 0;                  // directed edges (sum of degrees)
   uint64_t* off = nullptr;
   uint32_t* deg = nullptr;
   uint32_t* keys = nullptr;
@@ -905,6 +906,11 @@ lv_dgraph* upload_graph(lv_context* c, const lv_graph* g) {
     if (d->f64) h2d(c, (double*)d->w, g->weights_f64, d->E);
     else h2d(c, (float*)d->w, g->weights, d->E);
     d->order = d->S;
+    d->live = d->E;
+    if (g->degrees) {
+      d->live = 0;
+      for (uint64_t u = 0; u < d->S; ++u) d->live += g->degrees[u];
+    }
     if (g->exists) {
       d->ex = dalloc<uint8_t>(d->S);
       h2d(c, d->ex, g->exists, d->S);
@@ -995,6 +1001,155 @@ void check_membership(const uint32_t* q, uint64_t S) {
   uint32_t mx = 0;
   for (uint64_t u = 0; u < S; ++u) mx = std::max(mx, q[u]);
   require(S == 0 || mx < S, LV_ERR_INVALID, "membership has a community id >= span");
+}
+
+// ------------------------------------------------------------ graph update
+// One list of a raw batch tidied on the device (tidyBatchUpdateU,
+// inc/batch.hxx:202-209): kept by the edge's existence in d, sorted by
+// (u, v) (stable), unique; the m kept entries go to new arrays.
+uint64_t tidy_list(lv_context* c, const lv_dgraph* d, const lv_batch* b, bool keep_existing, uint32_t*& ou,
+                   uint32_t*& ov, float*& ow, const std::string& tag) {
+  const uint64_t n = b ? b->size : 0;
+  ou = ov = nullptr;
+  ow = nullptr;
+  if (!n) return 0;
+  require(b->u && b->v && b->w, LV_ERR_INVALID, "batch arrays NULL");
+  require(n < 0x7FFFFFFFull, LV_ERR_LIMIT, "batch longer than 2^31 - 1 entries");
+  uint32_t* u = c->ws<uint32_t>(tag + ".u", n);
+  uint32_t* v = c->ws<uint32_t>(tag + ".v", n);
+  float* w = c->ws<float>(tag + ".w", n);
+  h2d(c, u, b->u, n);
+  h2d(c, v, b->v, n);
+  h2d(c, w, b->w, n);
+  uint64_t* key = c->ws<uint64_t>("up.key", n);
+  uint64_t* key2 = c->ws<uint64_t>("up.key2", n);
+  uint32_t* idx = c->ws<uint32_t>("up.idx", n);
+  uint32_t* idx2 = c->ws<uint32_t>("up.idx2", n);
+  uint8_t* flags = c->ws<uint8_t>("up.flags", n);
+  uint32_t* sel = c->ws<uint32_t>("up.sel", n);
+  uint32_t* cnt = c->ws<uint32_t>("up.cnt", 1);
+  int* bad = c->ws<int>("up.bad", 1);
+  k_fill_u32(reinterpret_cast<uint32_t*>(bad), 0, 1, c->st);
+  k_tidy_keys(d->view(), u, v, n, keep_existing, key, idx, bad, c->st);
+  size_t bytes = 0;
+  HIPCHK(sort_pairs_u64(key, key2, idx, idx2, n, nullptr, bytes, c->st));
+  void* tmp = c->scan_tmp(bytes);
+  bytes = c->scan_tmp_bytes;
+  HIPCHK(sort_pairs_u64(key, key2, idx, idx2, n, tmp, bytes, c->st));
+  k_tidy_flags(key2, n, flags, c->st);
+  bytes = 0;
+  HIPCHK(select_flagged_u32(idx2, flags, sel, cnt, n, nullptr, bytes, c->st));
+  tmp = c->scan_tmp(bytes);
+  bytes = c->scan_tmp_bytes;
+  HIPCHK(select_flagged_u32(idx2, flags, sel, cnt, n, tmp, bytes, c->st));
+  require(c->read1(bad) == 0, LV_ERR_INVALID, "batch vertex id >= span");
+  const uint64_t m = c->read1(cnt);
+  ou = dalloc<uint32_t>(m);
+  ov = dalloc<uint32_t>(m);
+  ow = dalloc<float>(m);
+  k_tidy_gather(sel, m, u, v, w, ou, ov, ow, c->st);
+  return m;
+}
+
+// applyBatchUpdateU (inc/batch.hxx:225-236) of a tidy device batch on d.
+void apply_update(lv_context* c, lv_dgraph* d, const lv_dbatch* b) {
+  const uint64_t S = d->S, fb = (S + 7) & ~3ull;
+  if (!b->nd && !b->ni) return;
+  bool fresh = false;
+  int* delta = c->ws<int>("up.delta", S, &fresh);  // zero between calls
+  if (fresh) k_fill_u32(reinterpret_cast<uint32_t*>(delta), 0, c->ws_bytes("up.delta") / 4, c->st);
+  uint8_t* touched = c->ws<uint8_t>("up.touched", fb, &fresh);  // zero between calls
+  if (fresh) k_fill_u8(touched, 0, c->ws_bytes("up.touched"), c->st);
+  k_upd_delta(b->du, b->nd, -1, delta, touched, c->st);
+  k_upd_delta(b->iu, b->ni, +1, delta, touched, c->st);
+  // touched rows, ascending
+  uint32_t* T = c->ws<uint32_t>("up.T", S);
+  uint32_t* Thub = c->ws<uint32_t>("up.Thub", 1);  // no hub split (threshold 0)
+  uint32_t* lens = c->ws<uint32_t>("up.lens", 3);
+  uint64_t* bsum = c->ws<uint64_t>("up.bsum", 2 * kWlScanBlocks);
+  k_wl_queue(touched, false, S, d->deg, 0, T, Thub, lens, bsum, c->st);
+  const uint64_t R = c->read1(lens);
+  uint64_t* so = c->ws<uint64_t>("up.so", R + 1);
+  uint64_t* sn = c->ws<uint64_t>("up.sn", R + 1);
+  uint64_t* bo = c->ws<uint64_t>("up.bo", R + 1);
+  uint64_t* bn = c->ws<uint64_t>("up.bn", R + 1);
+  int* bad = c->ws<int>("up.bad", 1);
+  k_fill_u32(reinterpret_cast<uint32_t*>(bad), 0, 1, c->st);
+  k_upd_sizes(T, R, d->off, d->deg, delta, so, sn, bad, c->st);
+  scan64(c, so, bo, R + 1);
+  scan64(c, sn, bn, R + 1);
+  const uint64_t total_old = c->read1(bo + R), total_new = c->read1(bn + R);
+  const bool overflow = (c->read1(bad) & 2) != 0;
+  // survivors of the touched rows and their ranks within each row
+  uint32_t* keep = c->ws<uint32_t>("up.keep", total_old + 1);
+  uint32_t* rank = c->ws<uint32_t>("up.rank", total_old + 1);
+  k_fill_u32(keep, 1, total_old, c->st);
+  k_fill_u32(keep + total_old, 0, 1, c->st);
+  float* delw = c->ws<float>("up.delw", std::max<uint64_t>(b->nd, 1));
+  k_upd_tomb(b->du, b->dv, b->nd, T, R, bo, d->off, d->deg, d->keys, static_cast<const float*>(d->w), keep, delw,
+             c->st);
+  scan32(c, keep, rank, total_old + 1);
+  // merged rows in scratch, then into the graph
+  uint32_t* yk = c->ws<uint32_t>("up.yk", std::max<uint64_t>(total_new, 1));
+  float* yw = c->ws<float>("up.yw", std::max<uint64_t>(total_new, 1));
+  k_upd_scatter(T, R, bo, bn, total_old, d->off, d->deg, d->keys, static_cast<const float*>(d->w), keep, rank, b->iu,
+                b->iv, b->iw, b->ni, yk, yw, c->st);
+  if (!overflow) {
+    k_upd_copy_back(T, R, bn, total_new, d->off, yk, yw, d->keys, static_cast<float*>(d->w), c->st);
+  } else {
+    // every row to a new CSR with slack: capacity = degree + max(4, degree / 8)
+    uint64_t* cap = c->ws<uint64_t>("up.cap", S + 1);
+    k_upd_caps(d->deg, delta, S, cap, c->st);
+    uint64_t* noff = dalloc<uint64_t>(S + 1);
+    uint32_t* nkeys = nullptr;
+    float* nw = nullptr;
+    try {
+      scan64(c, cap, noff, S + 1);
+      const uint64_t E2 = c->read1(noff + S);
+      nkeys = dalloc<uint32_t>(E2);
+      nw = dalloc<float>(E2);
+      k_upd_copy_rows(S, d->off, d->deg, d->keys, static_cast<const float*>(d->w), touched, noff, nkeys, nw, c->st);
+      k_upd_copy_back(T, R, bn, total_new, noff, yk, yw, nkeys, nw, c->st);
+      c->check();
+      HIPCHK(hipStreamSynchronize(c->st));
+      (void)hipFree(d->off);
+      (void)hipFree(d->keys);
+      (void)hipFree(d->w);
+      d->off = noff;
+      d->keys = nkeys;
+      d->w = nw;
+      d->E = E2;
+    } catch (...) {
+      (void)hipFree(noff);
+      (void)hipFree(nkeys);
+      (void)hipFree(nw);
+      throw;
+    }
+  }
+  k_upd_degrees(T, R, delta, d->deg, touched, c->st);
+  if (d->ex) {
+    k_upd_exists(b->iu, b->iv, b->ni, d->ex, c->st);
+    unsigned long long* n = c->ws<unsigned long long>("up.order", 1);
+    k_fill_u64(reinterpret_cast<uint64_t*>(n), 0, 1, c->st);
+    k_count_u8(d->ex, 1, S, n, c->st);
+    d->order = c->read1(n);
+  }
+  // 2M after the batch: the old total minus the deleted edges' weights plus
+  // the insertions'
+  double* m2 = c->ws<double>("up.m2", 1);
+  k_fill_f64(m2, d->M2, 1, c->st);
+  k_upd_wsum(b->iw, b->ni, 1.0, m2, c->st);
+  k_upd_wsum(delw, b->nd, -1.0, m2, c->st);
+  d->M2 = c->read1(m2);
+  d->live = d->live + b->ni - b->nd;
+  // degree lists of the new degrees
+  build_tiers(c, "g.tiers", d->off, d->deg, d->ex, S, d->tiers, d->tiers.ids);
+  d->tiers.maxdeg = max_degree_tier4(c, d->tiers, d->deg);
+  build_tiers(c, "g.hubs", d->off, d->deg, d->ex, S, d->hubs, d->hubs.ids, kHubMin);
+  d->hubs.maxdeg = max_degree_tier4(c, d->hubs, d->deg);
+  d->hub_slots = hub_slot_total(c, d->hubs, d->deg);
+  c->wl_key_deg = nullptr;  // worklist hub-table sizes depend on the degrees
+  c->check();
 }
 
 template <class F>
@@ -1209,7 +1364,66 @@ void lv_dgraph_destroy(lv_dgraph* g) {
   delete g;
 }
 uint64_t lv_dgraph_span(const lv_dgraph* g) { return g ? g->S : 0; }
-uint64_t lv_dgraph_edges(const lv_dgraph* g) { return g ? g->E : 0; }
+uint64_t lv_dgraph_edges(const lv_dgraph* g) { return g ? g->live : 0; }
+uint64_t lv_dgraph_capacity(const lv_dgraph* g) { return g ? g->E : 0; }
+
+int lv_dgraph_update(lv_context* c, lv_dgraph* g, const lv_batch* deletions, const lv_batch* insertions,
+                     lv_dbatch** tidy) {
+  if (!c || !g || !tidy) return LV_ERR_INVALID;
+  *tidy = nullptr;
+  return guard(c, [&] {
+    require(!g->f64, LV_ERR_INVALID, "only input graphs (f32 weights) can be updated");
+    HIPCHK(hipSetDevice(c->device));
+    auto* b = new lv_dbatch();
+    try {
+      b->nd = tidy_list(c, g, deletions, true, b->du, b->dv, b->dw, "up.del");
+      b->ni = tidy_list(c, g, insertions, false, b->iu, b->iv, b->iw, "up.ins");
+      apply_update(c, g, b);
+    } catch (...) {
+      lv_dbatch_destroy(b);
+      throw;
+    }
+    *tidy = b;
+  });
+}
+
+int lv_dgraph_download(lv_context* c, const lv_dgraph* g, uint64_t* offsets, uint32_t* degrees, uint32_t* keys,
+                       float* weights, uint8_t* exists) {
+  if (!c || !g || !offsets || !degrees || !keys || !weights) return LV_ERR_INVALID;
+  return guard(c, [&] {
+    require(!g->f64, LV_ERR_INVALID, "graph has f64 weights");
+    d2h(c, offsets, g->off, g->S + 1);
+    d2h(c, degrees, g->deg, g->S);
+    d2h(c, keys, g->keys, g->E);
+    d2h(c, weights, static_cast<const float*>(g->w), g->E);
+    if (exists) {
+      if (g->ex) d2h(c, exists, g->ex, g->S);
+      else std::memset(exists, 1, g->S);
+    }
+    HIPCHK(hipStreamSynchronize(c->st));
+  });
+}
+
+int lv_dbatch_sizes(const lv_dbatch* b, uint64_t* deletions, uint64_t* insertions) {
+  if (!b || !deletions || !insertions) return LV_ERR_INVALID;
+  *deletions = b->nd;
+  *insertions = b->ni;
+  return LV_OK;
+}
+
+int lv_dbatch_download(lv_context* c, const lv_dbatch* b, uint32_t* du, uint32_t* dv, float* dw, uint32_t* iu,
+                       uint32_t* iv, float* iw) {
+  if (!c || !b || (b->nd && (!du || !dv || !dw)) || (b->ni && (!iu || !iv || !iw))) return LV_ERR_INVALID;
+  return guard(c, [&] {
+    d2h(c, du, b->du, b->nd);
+    d2h(c, dv, b->dv, b->nd);
+    d2h(c, dw, b->dw, b->nd);
+    d2h(c, iu, b->iu, b->ni);
+    d2h(c, iv, b->iv, b->ni);
+    d2h(c, iw, b->iw, b->ni);
+    HIPCHK(hipStreamSynchronize(c->st));
+  });
+}
 
 int lv_dbatch_create(lv_context* c, const lv_batch* d, const lv_batch* i, lv_dbatch** out) {
   if (!c || !out) return LV_ERR_INVALID;

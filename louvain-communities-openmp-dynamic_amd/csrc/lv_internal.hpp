@@ -212,6 +212,35 @@ void k_modularity_parts(GView g, bool f64, const uint32_t* memb, double* cin, do
 void k_modularity_sum(const double* cin, const double* ctot, uint64_t n, double M, double R, double* out,
                       hipStream_t st);
 // Exclusive scans (hipCUB) with grow-only temp storage owned by the caller.
+// Device batch tidy and graph update (lv_update.hip).
+void k_tidy_keys(GView g, const uint32_t* u, const uint32_t* v, uint64_t n, bool keep_existing, uint64_t* key,
+                 uint32_t* idx, int* bad, hipStream_t st);
+hipError_t sort_pairs_u64(const uint64_t* kin, uint64_t* kout, const uint32_t* vin, uint32_t* vout, uint64_t n,
+                          void* tmp, size_t& bytes, hipStream_t st);
+void k_tidy_flags(const uint64_t* key, uint64_t n, uint8_t* flag, hipStream_t st);
+hipError_t select_flagged_u32(const uint32_t* in, const uint8_t* flags, uint32_t* out, uint32_t* count, uint64_t n,
+                              void* tmp, size_t& bytes, hipStream_t st);
+void k_tidy_gather(const uint32_t* sel, uint64_t m, const uint32_t* u, const uint32_t* v, const float* w,
+                   uint32_t* ou, uint32_t* ov, float* ow, hipStream_t st);
+void k_upd_delta(const uint32_t* u, uint64_t n, int s, int* delta, uint8_t* touched, hipStream_t st);
+void k_upd_sizes(const uint32_t* T, uint64_t R, const uint64_t* off, const uint32_t* deg, const int* delta,
+                 uint64_t* so, uint64_t* sn, int* bad, hipStream_t st);
+void k_upd_tomb(const uint32_t* du, const uint32_t* dv, uint64_t nd, const uint32_t* T, uint64_t R,
+                const uint64_t* bo, const uint64_t* off, const uint32_t* deg, const uint32_t* keys, const float* w,
+                uint32_t* keep, float* delw, hipStream_t st);
+void k_upd_scatter(const uint32_t* T, uint64_t R, const uint64_t* bo, const uint64_t* bn, uint64_t total_old,
+                   const uint64_t* off, const uint32_t* deg, const uint32_t* keys, const float* w,
+                   const uint32_t* keep, const uint32_t* rank, const uint32_t* iu, const uint32_t* iv,
+                   const float* iw, uint64_t ni, uint32_t* yk, float* yw, hipStream_t st);
+void k_upd_copy_back(const uint32_t* T, uint64_t R, const uint64_t* bn, uint64_t total, const uint64_t* dst_off,
+                     const uint32_t* yk, const float* yw, uint32_t* keys, float* w, hipStream_t st);
+void k_upd_degrees(const uint32_t* T, uint64_t R, int* delta, uint32_t* deg, uint8_t* touched, hipStream_t st);
+void k_upd_caps(const uint32_t* deg, const int* delta, uint64_t S, uint64_t* cap, hipStream_t st);
+void k_upd_copy_rows(uint64_t S, const uint64_t* off, const uint32_t* deg, const uint32_t* keys, const float* w,
+                     const uint8_t* touched, const uint64_t* noff, uint32_t* nkeys, float* nw, hipStream_t st);
+void k_upd_exists(const uint32_t* iu, const uint32_t* iv, uint64_t ni, uint8_t* ex, hipStream_t st);
+void k_upd_wsum(const float* w, uint64_t n, double sign, double* out, hipStream_t st);
+
 hipError_t scan_u32(const uint32_t* in, uint32_t* out, uint64_t n, void* tmp, size_t& tmp_bytes, hipStream_t st);
 hipError_t scan_u64(const uint64_t* in, uint64_t* out, uint64_t n, void* tmp, size_t& tmp_bytes, hipStream_t st);
 

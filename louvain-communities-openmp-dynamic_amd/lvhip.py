@@ -40,6 +40,7 @@ EXPORTS = [
     "lv_louvain_dynamic_frontier", "lv_dgraph_create", "lv_dgraph_destroy", "lv_dgraph_span", "lv_dgraph_edges",
     "lv_dbatch_create", "lv_dbatch_destroy", "lv_dstate_create", "lv_dstate_destroy", "lv_dstate_upload",
     "lv_dstate_download", "lv_dstate_copy", "lv_run", "lv_dmodularity", "lv_edge_weight", "lv_modularity",
+    "lv_dgraph_capacity", "lv_dgraph_update", "lv_dgraph_download", "lv_dbatch_sizes", "lv_dbatch_download",
     "lv_vertex_weights", "lv_update_weights", "lv_affected_frontier", "lv_affected_delta_screening",
     "lv_choose_community", "lv_renumber", "lv_aggregate",
 ]
@@ -88,6 +89,7 @@ def load_library(path=LIB_PATH):
     lib.lv_last_error.restype = C.c_char_p
     lib.lv_dgraph_span.restype = C.c_uint64
     lib.lv_dgraph_edges.restype = C.c_uint64
+    lib.lv_dgraph_capacity.restype = C.c_uint64
     for name in ("lv_context_destroy", "lv_dgraph_destroy", "lv_dbatch_destroy", "lv_dstate_destroy"):
         getattr(lib, name).restype = None
     return lib
@@ -371,6 +373,29 @@ class DeviceGraph:
         self.span = ctx.lib.lv_dgraph_span(self.h)
         self.edges = ctx.lib.lv_dgraph_edges(self.h)
 
+    def update(self, deletions, insertions):
+        """lv_dgraph_update: tidy the raw batch on the device, apply it to this
+        graph in place; returns the tidy batch as a DeviceBatch."""
+        d, i = _BatchArgs(deletions), _BatchArgs(insertions)
+        b = DeviceBatch.__new__(DeviceBatch)
+        b.ctx, b.h = self.ctx, C.c_void_p()
+        self.ctx._check(self.ctx.lib.lv_dgraph_update(self.ctx.h, self.h, C.byref(d.c), C.byref(i.c), C.byref(b.h)),
+                        "lv_dgraph_update")
+        nd, ni = C.c_uint64(), C.c_uint64()
+        self.ctx.lib.lv_dbatch_sizes(b.h, C.byref(nd), C.byref(ni))
+        b.size = nd.value + ni.value
+        self.edges = self.ctx.lib.lv_dgraph_edges(self.h)
+        return b
+
+    def download(self):
+        """The CSR as dict(off, deg, keys, w, ex) (rows may carry slack)."""
+        S, E = self.span, self.ctx.lib.lv_dgraph_capacity(self.h)
+        off, deg = np.zeros(S + 1, np.uint64), np.zeros(S, np.uint32)
+        keys, w, ex = np.zeros(max(E, 1), np.uint32), np.zeros(max(E, 1), np.float32), np.zeros(S, np.uint8)
+        self.ctx._check(self.ctx.lib.lv_dgraph_download(self.ctx.h, self.h, _p(off), _p(deg), _p(keys), _p(w), _p(ex)),
+                        "lv_dgraph_download")
+        return {"off": off, "deg": deg, "keys": keys[:E], "w": w[:E], "ex": ex}
+
     def close(self):
         if self.h:
             self.ctx.lib.lv_dgraph_destroy(self.h)
@@ -390,6 +415,17 @@ class DeviceBatch:
         d, i = _BatchArgs(deletions), _BatchArgs(insertions)
         self.size = len(d.u) + len(i.u)
         ctx._check(ctx.lib.lv_dbatch_create(ctx.h, C.byref(d.c), C.byref(i.c), C.byref(self.h)), "lv_dbatch_create")
+
+    def download(self):
+        """(deletions, insertions) as dicts of u, v, w arrays."""
+        nd, ni = C.c_uint64(), C.c_uint64()
+        self.ctx.lib.lv_dbatch_sizes(self.h, C.byref(nd), C.byref(ni))
+        nd, ni = nd.value, ni.value
+        du, dv, dw = np.zeros(max(nd, 1), np.uint32), np.zeros(max(nd, 1), np.uint32), np.zeros(max(nd, 1), np.float32)
+        iu, iv, iw = np.zeros(max(ni, 1), np.uint32), np.zeros(max(ni, 1), np.uint32), np.zeros(max(ni, 1), np.float32)
+        self.ctx._check(self.ctx.lib.lv_dbatch_download(self.ctx.h, self.h, _p(du), _p(dv), _p(dw), _p(iu), _p(iv),
+                                                        _p(iw)), "lv_dbatch_download")
+        return ({"u": du[:nd], "v": dv[:nd], "w": dw[:nd]}, {"u": iu[:ni], "v": iv[:ni], "w": iw[:ni]})
 
     def close(self):
         if self.h:
