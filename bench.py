@@ -104,6 +104,8 @@ def main():
                     help="streams per live vertex of a level (lv_context_set_schedule)")
     ap.add_argument("--worklist-hub-degree", type=int, default=None,
                     help="degree above which worklist vertices run in the hub batch (256..4096)")
+    ap.add_argument("--log-lines", default=None,
+                    help="also write the DF chain's results as the reference driver's log lines to this file")
     ap.add_argument("--path", choices=("frontier", "worklist", "plain"), default="worklist",
                     help="local-moving path of the timed chain (the others are timed beside it)")
     args = ap.parse_args()
@@ -169,13 +171,26 @@ def main():
     # Quality chain (untimed): the GPU state before every batch, for the oracle.
     ctx.set_path({"frontier": lvhip.PATH_FRONTIER, "worklist": lvhip.PATH_WORKLIST,
                   "plain": lvhip.PATH_SWEEP}[args.path])
-    prev, q_gpu = [], []
-    if not args.no_cpu:
+    prev, q_gpu, res_gpu = [], [], []
+    if not args.no_cpu or args.log_lines:
         state.copy_from(state0)
         for k in range(W + K):
             prev.append(state.download())
-            ctx.run(lvhip.LV_DYNAMIC_FRONTIER, dgraphs[k + 1], dbatches[k], state)
+            res_gpu.append(ctx.run(lvhip.LV_DYNAMIC_FRONTIER, dgraphs[k + 1], dbatches[k], state))
             q_gpu.append(ctx.dmodularity(dgraphs[k + 1], state))
+    if args.log_lines:
+        # the reference driver's log format (main.cxx:78-84), readable by its process.js
+        import lvlog
+        with open(args.log_lines, "w") as fh:
+            fh.write(lvlog.graph_line(f"synthetic/rmat-{args.scale}.txt") + "\n")
+            fh.write(lvlog.order_line(S, graphs[0].edges) + "\n")
+            if rs is not None:
+                fh.write(lvlog.result_line(rs, "louvainStaticHip", 1, q_static) + "\n")
+            for k in range(W + K):
+                fh.write(lvlog.order_line(S, graphs[k + 1].edges, insertions=len(batches[k][1]["u"])) + "\n")
+                fh.write(lvlog.result_line(res_gpu[k], "louvainDynamicFrontierHip", 1, q_gpu[k], 0.0,
+                                           args.batch_fraction) + "\n")
+        log(f"log lines written to {args.log_lines}")
 
     # Timed: the chosen path, then the others beside it.
     paths = {"frontier": lvhip.PATH_FRONTIER, "worklist": lvhip.PATH_WORKLIST, "plain": lvhip.PATH_SWEEP}
@@ -195,6 +210,30 @@ def main():
             log(f"DF {name}: {pw * 1e3:.2f} ms wall, {others[name]['value']:.4g} edges/s, "
                 f"{args.path} path {others[name]['speedup']:.2f}x faster")
     ctx.set_path(paths[args.path])
+
+    # End to end: each batch's H2D, device tidy and in-place graph update
+    # (lv_dgraph_update, untimed in the reference), then DF on the updated
+    # resident graph.  The first batch moves the packed upload to a CSR with
+    # slack; it is a warm-up batch.
+    ctx.set_path(paths[args.path])
+    dg_e2e = ctx.device_graph(graphs[0].view())
+    state.copy_from(state0)
+    e2e, upd = [], []
+    for k in range(W + K):
+        t0 = time.perf_counter()
+        tb = dg_e2e.update(*batches[k])
+        t1 = time.perf_counter()
+        ctx.run(lvhip.LV_DYNAMIC_FRONTIER, dg_e2e, tb, state)
+        t2 = time.perf_counter()
+        assert dg_e2e.edges == graphs[k + 1].edges, "device update diverged from the host update"
+        if k >= W:
+            e2e.append((t2 - t0) * 1e3)
+            upd.append((t1 - t0) * 1e3)
+    end_to_end = {"p50_ms": percentile(e2e, 50), "p99_ms": percentile(e2e, 99),
+                  "update_p50_ms": percentile(upd, 50), "update_p99_ms": percentile(upd, 99)}
+    log(f"end to end (H2D + device tidy + graph update + DF): p50 {end_to_end['p50_ms']:.3f} ms, "
+        f"p99 {end_to_end['p99_ms']:.3f} ms; update alone p50 {end_to_end['update_p50_ms']:.3f} ms")
+    dg_e2e.close()
 
     # Roofline of the dominant local-moving kernel.  Stats slots: 0 = the
     # id-order kernel (one launch per sub-round / sweep), 4 = the hub batch
@@ -248,7 +287,7 @@ def main():
                                f"(a=.57 b=.19 c=.19) symmetrised unit weights, batches of "
                                f"{args.batch_fraction:g}|E| 80/20 insert/delete, seed {args.seed}",
                    "vertices": S, "edges": graphs[0].edges, "batch_updates": B, "parallelism": "1 GPU"},
-        "p50_batch_ms": percentile(lat, 50), "p99_batch_ms": percentile(lat, 99),
+        "p50_batch_ms": percentile(lat, 50), "p99_batch_ms": percentile(lat, 99), "end_to_end": end_to_end,
         "dq_max": dq_max, "static": {"gpu_modularity": q_static, "gpu_ms": rs and rs.time,
                                      "reference_modularity": q_static_ref, "reference_ms": r0.time,
                                      "reference_threads": cores},
