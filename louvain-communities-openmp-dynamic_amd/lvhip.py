@@ -373,18 +373,20 @@ class DeviceGraph:
         self.span = ctx.lib.lv_dgraph_span(self.h)
         self.edges = ctx.lib.lv_dgraph_edges(self.h)
 
-    def update(self, deletions, insertions):
+    def update(self, deletions, insertions, ctx=None):
         """lv_dgraph_update: tidy the raw batch on the device, apply it to this
-        graph in place; returns the tidy batch as a DeviceBatch."""
+        graph in place (on ctx's stream, default the creating context);
+        returns the tidy batch as a DeviceBatch."""
+        ctx = ctx or self.ctx
         d, i = _BatchArgs(deletions), _BatchArgs(insertions)
         b = DeviceBatch.__new__(DeviceBatch)
-        b.ctx, b.h = self.ctx, C.c_void_p()
-        self.ctx._check(self.ctx.lib.lv_dgraph_update(self.ctx.h, self.h, C.byref(d.c), C.byref(i.c), C.byref(b.h)),
-                        "lv_dgraph_update")
+        b.ctx, b.h = ctx, C.c_void_p()
+        ctx._check(ctx.lib.lv_dgraph_update(ctx.h, self.h, C.byref(d.c), C.byref(i.c), C.byref(b.h)),
+                   "lv_dgraph_update")
         nd, ni = C.c_uint64(), C.c_uint64()
-        self.ctx.lib.lv_dbatch_sizes(b.h, C.byref(nd), C.byref(ni))
+        ctx.lib.lv_dbatch_sizes(b.h, C.byref(nd), C.byref(ni))
         b.size = nd.value + ni.value
-        self.edges = self.ctx.lib.lv_dgraph_edges(self.h)
+        self.edges = ctx.lib.lv_dgraph_edges(self.h)
         return b
 
     def download(self):
@@ -450,12 +452,14 @@ class DeviceState:
         qq, qv, qc = _arr(q, np.uint32), _arr(qvtot, np.float64), _arr(qctot, np.float64)
         self.ctx._check(self.ctx.lib.lv_dstate_upload(self.ctx.h, self.h, _p(qq), _p(qv), _p(qc)), "lv_dstate_upload")
 
-    def download(self):
+    def download(self, ctx=None):
+        """(membership, K, Sigma_tot), copied on ctx's stream (default the
+        creating context)."""
+        ctx = ctx or self.ctx
         q = np.zeros(self.span, np.uint32)
         v = np.zeros(self.span, np.float64)
         c = np.zeros(self.span, np.float64)
-        self.ctx._check(self.ctx.lib.lv_dstate_download(self.ctx.h, self.h, _p(q), _p(v), _p(c)),
-                        "lv_dstate_download")
+        ctx._check(ctx.lib.lv_dstate_download(ctx.h, self.h, _p(q), _p(v), _p(c)), "lv_dstate_download")
         return q, v, c
 
     def copy_from(self, other):
