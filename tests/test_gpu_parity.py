@@ -324,3 +324,34 @@ def test_single_stream_reproduces_reference_bit_exact(ctx, name, path):
         ctx.set_schedule()
         ctx.set_hub_threshold()
         ctx.set_path(2)
+
+
+def test_static_planted_partition_100k_mtx(ctx, tmp_path, oracle):
+    """The static parity workload: a 100k-vertex planted partition (100
+    power-law blocks, average degree 16, 30% inter-block edges, unit f32
+    weights, seed 42) written as a 1-indexed MTX and read back; Static with
+    tol 1e-2, drop 10, L=20, P=10.  Modularity not worse than the lowest of
+    the reference's 1/2/4/8-thread results by more than 1e-3; K exact."""
+    import lvgraph
+    import lvhip
+    from oracle import Options
+    g = lvgraph.planted_partition(100000, 100, 16, 0.3, 42, one_indexed=True)
+    p = str(tmp_path / "pp100k.mtx")
+    lvgraph.write_mtx(g, p)
+    x = lvgraph.read_mtx(p)
+    assert np.array_equal(x["off"], g["off"]) and np.array_equal(x["keys"], g["keys"])
+    r = ctx.louvain("static", x, o=lvhip.LouvainOptions())
+    q = oracle.modularity(x, r.membership)
+    qs = []
+    try:
+        for t in (1, 2, 4, 8):
+            oracle.set_threads(t)
+            ref = oracle.louvain("static", x, options=Options())
+            qs.append(oracle.modularity(x, ref.membership))
+    finally:
+        oracle.set_threads(1)
+    _say(f"  static pp100k: Q gpu={q:.6f} ref(1/2/4/8)={['%.6f' % v for v in qs]} it={r.iterations} p={r.passes} "
+         f"t={r.time:.1f} ms")
+    assert q >= min(qs) - Q_TOL
+    assert np.array_equal(r.vertexWeight, ref.vertex_weight)
+    _check_state(x, r)
