@@ -231,6 +231,20 @@ int lv_dstate_download(lv_context* ctx, const lv_dstate* s, uint32_t* q, double*
 int lv_dstate_copy(lv_context* ctx, lv_dstate* dst, const lv_dstate* src);
 int lv_run(lv_context* ctx, int approach, const lv_dgraph* g, const lv_dbatch* b, lv_dstate* s,
            const lv_options* o, lv_result* res);
+/* Batched form: G graphs side by side in one device graph (graph k owns
+ * vertex ids [voff[k], voff[k+1]), no edges between graphs), one batch and
+ * one state over all of them (graph k's community ids within its range).
+ * Static, ND or DF (not DS) runs on every graph at once -- each kernel
+ * launch covers all graphs -- with each graph's own M, tolerance, iteration
+ * count, pass count and aggregation decision, as lv_run on that graph alone
+ * would (inc/louvain.hxx:1120-1221 per graph).  res holds num_graphs results
+ * with graph-local membership ids (arrays optional); the state keeps the
+ * union ids, each graph's re-based into its own range, ready for the next
+ * batch.  Membership labels (not the partition) of a graph that finishes
+ * before others may differ from a run alone.  No counterpart in the
+ * reference, which runs one graph per call. */
+int lv_run_batched(lv_context* ctx, int approach, const lv_dgraph* g, const lv_dbatch* b, lv_dstate* s,
+                   const uint64_t* voff, uint64_t num_graphs, const lv_options* o, lv_result* res);
 /* Modularity of a device state's membership on a device graph
  * (modularityByOmp, inc/properties.hxx:205-233). */
 int lv_dmodularity(lv_context* ctx, const lv_dgraph* g, const lv_dstate* s, double resolution, double* out);

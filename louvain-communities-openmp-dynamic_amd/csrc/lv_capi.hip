@@ -288,7 +288,10 @@ void launch_move(lv_context* c, const MoveArgs& a, int tier, bool f64) {
   hipEvent_t e0 = nullptr;
   if (c->profiling) e0 = c->record();
   if (tier == kTierHubBatch) k_hub_batch(a, c->st);
-  else k_move(a, tier, f64, c->st);
+  else {
+    if (tier == kTierHybrid && a.ctab) k_graph_chunks(a, c->st);
+    k_move(a, tier, f64, c->st);
+  }
   if (c->profiling) c->prof.push_back({tier == kTierHubBatch ? 4 : (tier == kTierHybrid ? 0 : tier), {e0, c->record()}});
 }
 
@@ -300,6 +303,107 @@ uint64_t level_streams(const lv_context* c, uint64_t order) {
   if (order <= kRefChunk) return 1;
   uint64_t want = (uint64_t)(c->stream_fraction * (double)order);
   return std::min(c->max_streams, std::max<uint64_t>(c->min_streams, want));
+}
+
+// Per-graph control of the batched form (lv_run_batched): graph of every
+// vertex of the level and per graph its M, community-0 id, gain and skip
+// flag on the device; tolerance, iteration count and last gain on the host.
+struct Batched {
+  uint64_t G = 0;
+  uint32_t* gid = nullptr;
+  double* gM = nullptr;
+  uint32_t* gzero = nullptr;
+  double* gel = nullptr;
+  uint8_t* gmask = nullptr;
+  std::vector<double> E, el;
+  std::vector<int> it;
+  std::vector<uint8_t> skip, done;
+  // streams of the level per graph (id-order kernel with spread chunks):
+  // level id ranges, chunk starts (G + 1 each) and the chunk table
+  uint64_t* glo = nullptr;
+  uint64_t* gcs = nullptr;
+  uint64_t* ctab = nullptr;
+  uint64_t nct = 0;
+  std::vector<uint64_t> hlo, hcs;
+};
+
+void batched_args(MoveArgs& a, const Batched* bt) {
+  if (!bt) return;
+  a.gid = bt->gid;
+  a.gM = bt->gM;
+  a.gzero = bt->gzero;
+  a.gel = bt->gel;
+  a.gmask = bt->gmask;
+}
+
+// Level schedule: graph k walks its own id range in level_streams(its live
+// vertices) streams, as it would alone (so a small graph is one stream
+// beside a large one); a finished graph walks none.
+void batched_level(lv_context* c, Batched* bt, const std::vector<uint64_t>& lo, const std::vector<uint64_t>& GN) {
+  const uint64_t G = bt->G;
+  bt->hlo = lo;
+  bt->hcs.assign(G + 1, 0);
+  for (uint64_t k = 0; k < G; ++k) bt->hcs[k + 1] = bt->hcs[k] + (bt->done[k] ? 0 : level_streams(c, GN[k]));
+  bt->nct = bt->hcs[G];
+  bt->glo = c->ws<uint64_t>("bt.glo", G + 1);
+  bt->gcs = c->ws<uint64_t>("bt.gcs", G + 1);
+  bt->ctab = c->ws<uint64_t>("bt.ctab", 2 * std::max<uint64_t>(bt->nct, 1));
+  h2d(c, bt->glo, bt->hlo.data(), G + 1);
+  h2d(c, bt->gcs, bt->hcs.data(), G + 1);
+}
+
+void id_order_regions(lv_context* c, MoveArgs& a, const Level& lv);
+
+// The level schedule applies to the id-order kernel when chunks are spread
+// (chunk 0); chunks handed out in ascending order run over the union.
+void batched_schedule(lv_context* c, MoveArgs& a, const Level& lv, const Batched* bt) {
+  if (!bt || a.chunk || c->tier_order != 2 || !bt->nct) return;
+  a.ctab = bt->ctab;
+  a.nct = bt->nct;
+  a.glo = bt->glo;
+  a.gcs = bt->gcs;
+  a.ng = (uint32_t)bt->G;
+  a.max_streams = bt->nct;
+  id_order_regions(c, a, lv);
+}
+
+// Start of a pass: every unfinished graph runs.
+void batched_begin(lv_context* c, Batched* bt) {
+  for (uint64_t k = 0; k < bt->G; ++k) {
+    bt->skip[k] = bt->done[k];
+    bt->it[k] = 0;
+    bt->el[k] = 0;
+  }
+  h2d(c, bt->gmask, bt->skip.data(), bt->G);
+}
+
+// After an iteration: each running graph counts it and stops when its gain
+// is within its tolerance or it reached L (inc/louvain.hxx:598-606); returns
+// whether any graph runs on.
+bool batched_step(lv_context* c, Batched* bt, int L) {
+  c->read(bt->el.data(), bt->gel, bt->G);
+  bool any = false;
+  for (uint64_t k = 0; k < bt->G; ++k) {
+    if (bt->skip[k]) continue;
+    ++bt->it[k];
+    if (bt->el[k] <= bt->E[k] || bt->it[k] >= L) bt->skip[k] = 1;
+    else any = true;
+  }
+  if (any) h2d(c, bt->gmask, bt->skip.data(), bt->G);
+  return any;
+}
+
+// id-order kernel: one HBM region per stream (up to 2^16) holding
+// nextpow2(2 deg) slots for every degree it runs (<= hub_min)
+void id_order_regions(lv_context* c, MoveArgs& a, const Level& lv) {
+  a.gblocks = std::min<uint64_t>(a.max_streams, 1u << 16);
+  uint64_t dmax = kHubMin;
+  if (c->hub_min > kHubMin) dmax = std::max<uint64_t>(dmax, std::min<uint64_t>(c->hub_min, lv.hubs->maxdeg));
+  a.gcap = std::max<uint64_t>(kHybridHbm, next_pow2(2 * dmax));
+  size_t slots = (size_t)a.gcap * a.gblocks;
+  a.gkey = c->ws<uint32_t>("g.key", slots);
+  a.gpos = c->ws<uint32_t>("g.pos", slots);
+  a.gval = c->ws<double>("g.val", slots);
 }
 
 MoveArgs base_args(lv_context* c, const Level& lv, uint32_t* vcom, const double* vtot, double* ctot, uint8_t* vaff,
@@ -371,18 +475,7 @@ MoveArgs base_args(lv_context* c, const Level& lv, uint32_t* vcom, const double*
     if (fp) k_fill_u32(a.hb_pos, kNoPos, c->ws_bytes("hb.pos") / sizeof(uint32_t), c->st);
     if (fv) k_fill_f64(a.hb_val, 0, c->ws_bytes("hb.val") / sizeof(double), c->st);
   }
-  if (c->tier_order == 2) {
-    // id-order kernel: one HBM region per stream holding nextpow2(2 deg)
-    // slots for every degree it runs (<= hub_min)
-    a.gblocks = std::min<uint64_t>(a.max_streams, 1u << 16);
-    uint64_t dmax = kHubMin;
-    if (c->hub_min > kHubMin) dmax = std::max<uint64_t>(dmax, std::min<uint64_t>(c->hub_min, lv.hubs->maxdeg));
-    a.gcap = std::max<uint64_t>(kHybridHbm, next_pow2(2 * dmax));
-    size_t slots = (size_t)a.gcap * a.gblocks;
-    a.gkey = c->ws<uint32_t>("g.key", slots);
-    a.gpos = c->ws<uint32_t>("g.pos", slots);
-    a.gval = c->ws<double>("g.val", slots);
-  }
+  if (c->tier_order == 2) id_order_regions(c, a, lv);
   return a;
 }
 
@@ -437,22 +530,33 @@ void add_counters(lv_context* c, const unsigned long long* dcnt, Counters& cnt) 
 // ascending: hand out chunks in ascending order even if the context spreads
 // them -- a sparse frontier needs a vertex flagged by a lower id to be reached
 // later in the same sweep (inc/louvain.hxx:602 then :598).
+// Batched form (bt): every graph iterates under its own tolerance; the
+// per-graph results are in bt->it / bt->el.
 int local_move(lv_context* c, const Level& lv, uint32_t* vcom, const double* vtot, double* ctot, uint8_t* vaff,
-               const uint8_t* fa, double M, double R, int L, double E, Counters& cnt, bool ascending = false) {
+               const uint8_t* fa, double M, double R, int L, double E, Counters& cnt, bool ascending = false,
+               Batched* bt = nullptr) {
   MoveArgs a = base_args(c, lv, vcom, vtot, ctot, vaff, fa, M, R);
+  batched_args(a, bt);
   a.sparse = c->path == 1;
   if (ascending && !a.chunk) a.chunk = a.sparse ? 64 : 16;
+  batched_schedule(c, a, lv, bt);
   k_fill_u64(reinterpret_cast<uint64_t*>(a.cnt), 0, (size_t)kTiers * CNT_N, c->st);
   unsigned long long* cnt0 = a.cnt;
   int l = 0;
   double el = 0;
+  if (bt) batched_begin(c, bt);
   for (; l < L;) {
-    k_fill_f64(a.el, 0, 1, c->st);
+    if (bt) k_fill_f64(bt->gel, 0, bt->G, c->st);
+    else k_fill_f64(a.el, 0, 1, c->st);
     a.pick_less = l < c->pick_less_iters;
     move_sweep(c, a, lv);
     c->check();
-    el = c->read1(a.el);
     ++l;
+    if (bt) {
+      if (!batched_step(c, bt, L)) break;
+      continue;
+    }
+    el = c->read1(a.el);
     if (el <= E) break;
   }
   add_counters(c, cnt0, cnt);
@@ -469,8 +573,12 @@ int local_move(lv_context* c, const Level& lv, uint32_t* vcom, const double* vto
 // degree tier on the device.  q[0..3]: iteration input / next iteration /
 // two sub-round buffers; qlen[0..3] their lengths.
 int local_move_worklist(lv_context* c, const Level& lv, uint32_t* vcom, const double* vtot, double* ctot,
-                        uint8_t* vaff, const uint8_t* fa, double M, double R, int L, double E, Counters& cnt) {
+                        uint8_t* vaff, const uint8_t* fa, double M, double R, int L, double E, Counters& cnt,
+                        Batched* bt = nullptr) {
   MoveArgs a = base_args(c, lv, vcom, vtot, ctot, vaff, fa, M, R, true);
+  batched_args(a, bt);
+  batched_schedule(c, a, lv, bt);
+  if (bt) batched_begin(c, bt);
   a.sparse = 1;  // queue entries read 64 at a time
   k_fill_u64(reinterpret_cast<uint64_t*>(a.cnt), 0, (size_t)kTiers * CNT_N, c->st);
   unsigned long long* cnt0 = a.cnt;
@@ -506,7 +614,8 @@ int local_move_worklist(lv_context* c, const Level& lv, uint32_t* vcom, const do
     }
     a.serial = c->serial;
     a.pick_less = l < c->pick_less_iters;
-    k_fill_f64(a.el, 0, 1, c->st);
+    if (bt) k_fill_f64(bt->gel, 0, bt->G, c->st);
+    else k_fill_f64(a.el, 0, 1, c->st);
     // the iteration's queue: every flagged id, ascending
     k_wl_queue(vaff, false, S, lv.g.deg, thr, ql, qh, lens, bsum, c->st);
     // Sub-rounds are enqueued kSubRounds at a time (their kernels read the
@@ -542,8 +651,12 @@ int local_move_worklist(lv_context* c, const Level& lv, uint32_t* vcom, const do
       c->read(n2, lens, 2);
       if (n2[0] + n2[1] == 0) break;
     }
-    el = c->read1(a.el);
     ++l;
+    if (bt) {
+      if (!batched_step(c, bt, L)) break;
+      continue;
+    }
+    el = c->read1(a.el);
     if (el <= E) break;
   }
   add_counters(c, cnt0, cnt);
@@ -857,6 +970,283 @@ void run(lv_context* c, int approach, const lv_dgraph* x, const lv_dbatch* b, lv
     HIPCHK(hipStreamSynchronize(c->st));
   }
   // kernel statistics for bandwidth accounting (lv_stats_get)
+  for (int t = 0; t < kTiers; ++t) {
+    c->stats.vertices[t] += cnt.v[t][CNT_VERTICES];
+    c->stats.edges[t] += cnt.v[t][CNT_EDGES];
+    c->stats.candidates[t] += cnt.v[t][CNT_CANDIDATES];
+    c->stats.moves[t] += cnt.v[t][CNT_MOVES];
+    c->stats.moved_edges[t] += cnt.v[t][CNT_MOVED_EDGES];
+  }
+  c->stats.runs += 1;
+}
+
+// ------------------------------------------------------------ batched form
+// louvainInvokeOmp (inc/louvain.hxx:1120-1221) for G graphs laid side by
+// side in x (graph k owns vertex ids [voff[k], voff[k+1]), no edges between
+// graphs): every launch covers all graphs, and each graph keeps its own M,
+// tolerance, iteration count, pass count and aggregation decision, so it
+// runs as it would alone; a graph that has finished is skipped while the
+// others go on.  Community ids are re-based at the end so that graph k's
+// final ids lie in its own vertex range.
+void invoke_batched(lv_context* c, int approach, const lv_dgraph* x, const lv_dbatch* b, lv_dstate* s,
+                    const lv_options* o, const std::vector<uint64_t>& voff, std::vector<int>& l_out,
+                    std::vector<int>& p_out, RunTimes& rt, Counters& cnt) {
+  const uint64_t S = x->S, G = voff.size() - 1;
+  const double R = o->resolution;
+  const int L = o->max_iterations, P = o->max_passes;
+  uint32_t* ucom = s->memb;
+  double* utot = s->vtot;
+  double* ctot = s->ctot;
+  uint8_t* vaff = c->ws<uint8_t>("vaff", (S + 7) & ~3ull);
+  uint32_t* vcom = c->ws<uint32_t>("vcom", S);
+  double* vtot = c->ws<double>("vtot", S);
+  bool worklist = c->path == 2 && approach == LV_DYNAMIC_FRONTIER;
+  SuperGraph y = super_graph(c, "y", S, x->E), z = super_graph(c, "z", S, x->E);
+  // per-graph data: graph of every vertex (level 0 and the current level),
+  // M from the graph's own edge weights (edgeWeightOmp of each graph)
+  uint64_t* dvoff = c->ws<uint64_t>("bt.voff", G + 1);
+  h2d(c, dvoff, voff.data(), G + 1);
+  uint32_t* gid0 = c->ws<uint32_t>("bt.gid0", S);
+  uint32_t* gidA = c->ws<uint32_t>("bt.gidA", S);
+  uint32_t* gidB = c->ws<uint32_t>("bt.gidB", S);
+  k_gid_fill(dvoff, G, S, gid0, c->st);
+  double* kk = c->ws<double>("bt.k", S);
+  double* m2 = c->ws<double>("bt.m2", G);
+  k_fill_f64(kk, 0, S, c->st);
+  k_fill_f64(m2, 0, G, c->st);
+  k_vertex_weights(x->view(), x->f64, kk, c->st);
+  k_group_sum(kk, gid0, S, m2, c->st);
+  std::vector<double> M2(G);
+  c->read(M2.data(), m2, G);
+  Batched bt;
+  bt.G = G;
+  bt.gid = gid0;
+  bt.gM = c->ws<double>("bt.gM", G);
+  bt.gzero = c->ws<uint32_t>("bt.gzero", G);
+  bt.gel = c->ws<double>("bt.gel", G);
+  bt.gmask = c->ws<uint8_t>("bt.gmask", G);
+  bt.E.assign(G, o->tolerance);
+  bt.el.assign(G, 0);
+  bt.it.assign(G, 0);
+  bt.skip.assign(G, 0);
+  bt.done.assign(G, 0);
+  std::vector<double> gM(G);
+  std::vector<uint32_t> gzero(G);
+  std::vector<uint64_t> lo(voff), lo_next(G + 1);
+  for (uint64_t k = 0; k < G; ++k) {
+    gM[k] = M2[k] / 2;
+    gzero[k] = (uint32_t)voff[k];
+    bt.done[k] = !(gM[k] > 0 && P > 0);
+  }
+  h2d(c, bt.gM, gM.data(), G);
+  h2d(c, bt.gzero, gzero.data(), G);
+  // live vertices per graph at level 0 (GN of the first aggregation test)
+  std::vector<uint64_t> GN(G);
+  if (x->ex) {
+    unsigned long long* gn = c->ws<unsigned long long>("bt.gn", G);
+    k_fill_u64(reinterpret_cast<uint64_t*>(gn), 0, G, c->st);
+    k_group_count(x->ex, gid0, S, gn, c->st);
+    std::vector<unsigned long long> h(G);
+    c->read(h.data(), gn, G);
+    for (uint64_t k = 0; k < G; ++k) GN[k] = h[k];
+  } else {
+    for (uint64_t k = 0; k < G; ++k) GN[k] = voff[k + 1] - voff[k];
+  }
+  l_out.assign(G, 0);
+  p_out.assign(G, 0);
+  k_fill_u8(vaff, 0, (S + 7) & ~3ull, c->st);
+  k_fill_u32(vcom, 0, S, c->st);
+  k_fill_f64(vtot, 0, S, c->st);
+  if (approach == LV_STATIC) {
+    k_fill_u32(ucom, 0, S, c->st);
+    k_fill_f64(utot, 0, S, c->st);
+    k_fill_f64(ctot, 0, S, c->st);
+  }
+  hipEvent_t e0 = c->record();
+  if (approach == LV_STATIC) {
+    k_vertex_weights(x->view(), x->f64, utot, c->st);
+    k_initialize(x->view(), ucom, ctot, utot, c->st);
+  } else {
+    k_update_weights(b->du, b->dw, b->nd, b->iu, b->iw, b->ni, ucom, utot, ctot, c->st);
+  }
+  hipEvent_t e1 = c->record();
+  if (approach == LV_DYNAMIC_FRONTIER) k_frontier(b->du, b->dv, b->nd, b->iu, b->iv, b->ni, ucom, vaff, c->st);
+  else k_fill_u8(vaff, 1, S, c->st);
+  hipEvent_t e2 = c->record();
+  hipEvent_t t1 = e2;
+  bool t1_set = false;
+  Level cur{x->view(), x->f64, &x->tiers, x->order, x->tiers.maxdeg, &x->hubs, x->hub_slots};
+  TierLists ytiers, yhubs;
+  uint32_t* gidcur = gid0;
+  uint64_t levelS = S;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> lm, ag;
+  auto all_done = [&] {
+    for (uint64_t k = 0; k < G; ++k)
+      if (!bt.done[k]) return false;
+    return true;
+  };
+  int p = 0;
+  while (!all_done()) {
+    if (p == 1) { t1 = c->record(); t1_set = true; }
+    const bool first = p == 0;
+    bt.gid = gidcur;
+    batched_level(c, &bt, lo, GN);
+    hipEvent_t a0 = c->record();
+    if (first) {
+      if (worklist) local_move_worklist(c, cur, ucom, utot, ctot, vaff, nullptr, 0, R, L, 0, cnt, &bt);
+      else local_move(c, cur, ucom, utot, ctot, vaff, nullptr, 0, R, L, 0, cnt,
+                      approach == LV_DYNAMIC_FRONTIER, &bt);
+    } else {
+      local_move(c, cur, vcom, vtot, ctot, vaff, nullptr, 0, R, L, 0, cnt, false, &bt);
+    }
+    lm.push_back({a0, c->record()});
+    ++p;
+    // per graph: l += max(m, 1); stop passing when m <= 1 or p >= P
+    // (inc/louvain.hxx:1184-1186)
+    for (uint64_t k = 0; k < G; ++k) {
+      if (bt.done[k]) continue;
+      const int m = (bt.it[k] > 1 || bt.el[k]) ? bt.it[k] : 0;
+      l_out[k] += std::max(m, 1);
+      p_out[k] += 1;
+      if (m <= 1 || p_out[k] >= P) bt.done[k] = 1;
+    }
+    if (all_done()) break;
+    uint32_t* map = nullptr;
+    uint32_t* mcur = first ? ucom : vcom;
+    count_communities(c, cur.g, mcur, map);
+    // communities per graph: the renumbering map at the graph range bounds
+    uint32_t* lo32 = c->ws<uint32_t>("bt.lo32", G + 1);
+    uint32_t* mlo = c->ws<uint32_t>("bt.mlo", G + 1);
+    std::vector<uint32_t> lo_h(G + 1), mlo_h(G + 1);
+    for (uint64_t k = 0; k <= G; ++k) lo_h[k] = (uint32_t)lo[k];
+    h2d(c, lo32, lo_h.data(), G + 1);
+    k_gather_u32(map, lo32, G + 1, mlo, c->st);
+    c->read(mlo_h.data(), mlo, G + 1);
+    bool any = false;
+    for (uint64_t k = 0; k < G; ++k) {
+      const uint64_t CN = mlo_h[k + 1] - mlo_h[k];
+      if (!bt.done[k] && (double)CN / (double)GN[k] >= o->aggregation_tolerance) bt.done[k] = 1;
+      any = any || !bt.done[k];
+      lo_next[k] = mlo_h[k];
+      GN[k] = CN;
+    }
+    lo_next[G] = mlo_h[G];
+    if (!any) break;
+    // the union's communities become the next level (finished graphs ride
+    // along, skipped)
+    uint32_t* gidnext = gidcur == gidA ? gidB : gidA;
+    k_gid_next(mcur, gidcur, map, first ? x->ex : nullptr, levelS, gidnext, c->st);
+    if (first) k_lookup(ucom, S, map, c->st);
+    else {
+      k_lookup(vcom, cur.g.S, map, c->st);
+      k_lookup(ucom, S, vcom, c->st);
+    }
+    const uint64_t CN = lo_next[G];
+    hipEvent_t g0 = c->record();
+    aggregate(c, cur, mcur, CN, z, vtot);
+    ag.push_back({g0, c->record()});
+    std::swap(y, z);
+    GView yg{CN, y.off, y.deg, y.keys, y.w, nullptr};
+    k_initialize(yg, vcom, ctot, vtot, c->st);
+    k_fill_u8(vaff, 1, CN, c->st);
+    uint64_t yslots = 0;
+    if (c->tier_order == 2) {
+      build_tiers(c, "y.hubs", y.off, y.deg, nullptr, CN, yhubs, nullptr, kHubMin);
+      yhubs.maxdeg = max_degree_tier4(c, yhubs, y.deg);
+      yslots = hub_slot_total(c, yhubs, y.deg);
+    } else {
+      build_tiers(c, "y.tiers", y.off, y.deg, nullptr, CN, ytiers);
+      ytiers.maxdeg = max_degree_tier4(c, ytiers, y.deg);
+    }
+    cur = Level{yg, true, &ytiers, CN, ytiers.maxdeg, &yhubs, yslots};
+    gidcur = gidnext;
+    levelS = CN;
+    lo = lo_next;
+    for (uint64_t k = 0; k < G; ++k) {
+      gzero[k] = (uint32_t)lo[k];
+      if (!bt.done[k]) bt.E[k] /= o->tolerance_drop;
+    }
+    h2d(c, bt.gzero, gzero.data(), G);
+  }
+  if (p > 1) {
+    k_lookup(ucom, S, vcom, c->st);
+    // final ids of graph k: from its range of the last level to its vertex range
+    uint64_t* dlo = c->ws<uint64_t>("bt.lo", G + 1);
+    h2d(c, dlo, lo.data(), G + 1);
+    double* tmp = c->ws<double>("bt.tot", S);
+    d2d(c, tmp, ctot, S);
+    k_rebase(ucom, gid0, x->ex, S, ctot, levelS, dlo, G, dvoff, tmp, c->st);
+    d2d(c, ctot, tmp, S);
+  }
+  hipEvent_t e3 = c->record();
+  if (!t1_set) t1 = e3;
+  HIPCHK(hipEventSynchronize(e3));
+  c->check();
+  rt.time += elapsed(e0, e3);
+  rt.init += elapsed(e0, e1);
+  rt.mark += elapsed(e1, e2);
+  rt.first += elapsed(e2, t1);
+  for (auto& pr : lm) rt.lmove += elapsed(pr.first, pr.second);
+  for (auto& pr : ag) rt.aggr += elapsed(pr.first, pr.second);
+}
+
+void run_batched(lv_context* c, int approach, const lv_dgraph* x, const lv_dbatch* b, lv_dstate* s,
+                 const uint64_t* voff, uint64_t G, const lv_options* o, lv_result* res) {
+  require(c && x && s && voff && G > 0, LV_ERR_INVALID, "null context, graph, state or graph ranges");
+  require(approach == LV_STATIC || approach == LV_NAIVE_DYNAMIC || approach == LV_DYNAMIC_FRONTIER, LV_ERR_INVALID,
+          "the batched form runs Static, Naive-dynamic and Dynamic Frontier");
+  require(approach == LV_STATIC || b != nullptr, LV_ERR_INVALID, "dynamic approaches need a batch");
+  require(s->S == x->S, LV_ERR_INVALID, "state span differs from graph span");
+  require(voff[0] == 0 && voff[G] == x->S, LV_ERR_INVALID, "graph ranges must cover the span from 0");
+  for (uint64_t k = 0; k < G; ++k) require(voff[k] <= voff[k + 1], LV_ERR_INVALID, "graph ranges not ascending");
+  validate_options(o);
+  require(o->repeat == 1, LV_ERR_INVALID, "the batched form runs one repeat");
+  HIPCHK(hipSetDevice(c->device));
+  c->ev_used = 0;
+  c->prof.clear();
+  RunTimes rt;
+  Counters cnt;
+  std::vector<uint64_t> vo(voff, voff + G + 1);
+  std::vector<int> l, p;
+  invoke_batched(c, approach, x, b, s, o, vo, l, p, rt, cnt);
+  for (auto& pr : c->prof) {
+    c->stats.move_ms[pr.first] += elapsed(pr.second.first, pr.second.second);
+    c->stats.move_launches[pr.first] += 1;
+  }
+  const uint64_t S = x->S;
+  unsigned long long* aff = c->ws<unsigned long long>("bt.aff", G);
+  k_fill_u64(reinterpret_cast<uint64_t*>(aff), 0, G, c->st);
+  k_group_count(c->ws<uint8_t>("vaff", (S + 7) & ~3ull), c->ws<uint32_t>("bt.gid0", S), S, aff, c->st);
+  std::vector<unsigned long long> affh(G);
+  c->read(affh.data(), aff, G);
+  if (res) {
+    for (uint64_t k = 0; k < G; ++k) {
+      lv_result& r = res[k];
+      const uint64_t n = voff[k + 1] - voff[k];
+      r.iterations = l[k];
+      r.passes = p[k];
+      r.time = rt.time;
+      r.marking_time = rt.mark;
+      r.initialization_time = rt.init;
+      r.first_pass_time = rt.first;
+      r.local_move_time = rt.lmove;
+      r.aggregation_time = rt.aggr;
+      r.affected_vertices = affh[k];
+      if (r.membership) d2h(c, r.membership, s->memb + voff[k], n);
+      if (r.vertex_weight) d2h(c, r.vertex_weight, s->vtot + voff[k], n);
+      if (r.community_weight) d2h(c, r.community_weight, s->ctot + voff[k], n);
+    }
+    HIPCHK(hipStreamSynchronize(c->st));
+    // graph-local community ids (absent vertices, whose entries carry no
+    // community, read 0)
+    for (uint64_t k = 0; k < G; ++k) {
+      if (!res[k].membership) continue;
+      for (uint64_t u = 0; u < voff[k + 1] - voff[k]; ++u) {
+        const uint64_t m = res[k].membership[u];
+        res[k].membership[u] = (m >= voff[k] && m < voff[k + 1]) ? (uint32_t)(m - voff[k]) : 0u;
+      }
+    }
+  }
   for (int t = 0; t < kTiers; ++t) {
     c->stats.vertices[t] += cnt.v[t][CNT_VERTICES];
     c->stats.edges[t] += cnt.v[t][CNT_EDGES];
@@ -1498,6 +1888,23 @@ int lv_run(lv_context* c, int approach, const lv_dgraph* g, const lv_dbatch* b, 
       require(!(f & 2), LV_ERR_UNSORTED, "batch not sorted by (u, v) or not unique");
     }
     run(c, approach, g, b, s, o, res);
+  });
+}
+
+int lv_run_batched(lv_context* c, int approach, const lv_dgraph* g, const lv_dbatch* b, lv_dstate* s,
+                   const uint64_t* voff, uint64_t num_graphs, const lv_options* o, lv_result* res) {
+  if (!c) return LV_ERR_INVALID;
+  return guard(c, [&] {
+    if (b && g) {
+      int* bad = c->ws<int>("bad", 1);
+      k_fill_u32(reinterpret_cast<uint32_t*>(bad), 0, 1, c->st);
+      k_check_sorted(b->du, b->dv, b->nd, (uint32_t)g->S, bad, c->st);
+      k_check_sorted(b->iu, b->iv, b->ni, (uint32_t)g->S, bad, c->st);
+      int f = c->read1(bad);
+      require(!(f & 1), LV_ERR_INVALID, "batch vertex id out of range");
+      require(!(f & 2), LV_ERR_UNSORTED, "batch not sorted by (u, v) or not unique");
+    }
+    run_batched(c, approach, g, b, s, voff, num_graphs, o, res);
   });
 }
 

@@ -83,7 +83,24 @@ struct MoveArgs {
   double* out_e;
   double* el;                  // accumulated gain (l1-norm of delta-Q)
   unsigned long long* cnt;     // CNT_N counters
-  uint8_t* fsame;              // worklist mode: flagged by a lower id and not yet
+  // batched form (gid != nullptr): graph of every vertex of the level, and
+  // per graph its M, the id of its community 0 at this level, its gain
+  // accumulator and a skip flag (converged this pass, or finished)
+  const uint32_t* gid;
+  const double* gM;
+  const uint32_t* gzero;
+  double* gel;
+  const uint8_t* gmask;
+  // batched form, id-order kernel with spread chunks (chunk == 0): each
+  // graph's share of the list cut into its own streams, as it would be alone.
+  // Chunk t of nct covers list entries [ctab[2t], ctab[2t+1]) of graph k,
+  // gcs[k] <= t < gcs[k+1]; graph k holds level ids [glo[k], glo[k+1]).
+  uint64_t* ctab;
+  uint64_t nct;
+  const uint64_t* glo;
+  const uint64_t* gcs;
+  uint32_t ng;
+  uint8_t* fsame;            // worklist mode: flagged by a lower id and not yet
                                //   visited this iteration -> next sub-round
   uint32_t* stamp;             // worklist mode: iteration serial of the last visit
   uint32_t serial;
@@ -212,6 +229,23 @@ void k_modularity_parts(GView g, bool f64, const uint32_t* memb, double* cin, do
 void k_modularity_sum(const double* cin, const double* ctot, uint64_t n, double M, double R, double* out,
                       hipStream_t st);
 // Exclusive scans (hipCUB) with grow-only temp storage owned by the caller.
+// Batched form helpers (lv_update.hip): graph of every vertex from the
+// vertex ranges voff (G + 1 entries), per-graph sums and counts, graph of
+// the next level's super-vertices, and the final re-basing of each graph's
+// community ids from its last-level range lo to its vertex range.
+void k_gid_fill(const uint64_t* voff, uint64_t G, uint64_t S, uint32_t* gid, hipStream_t st);
+void k_group_sum(const double* x, const uint32_t* gid, uint64_t S, double* out, hipStream_t st);
+void k_gid_next(const uint32_t* vcom, const uint32_t* gid, const uint32_t* map, const uint8_t* ex, uint64_t S,
+                uint32_t* gid_next, hipStream_t st);
+void k_rebase(uint32_t* memb, const uint32_t* gid, const uint8_t* ex, uint64_t S, const double* tot, uint64_t SL,
+              const uint64_t* lo, uint64_t G, const uint64_t* voff, double* tot_out, hipStream_t st);
+void k_group_count(const uint8_t* f, const uint32_t* gid, uint64_t S, unsigned long long* out, hipStream_t st);
+// The chunk table a.ctab (a.nct chunks) of the list the id-order kernel is
+// about to walk (a.list with a.n_dev / a.n entries, or ids a.base.. when
+// a.list is null): graph k's entries in a.gcs[k+1] - a.gcs[k] chunks of
+// ceil(len / streams) entries.
+void k_graph_chunks(const MoveArgs& a, hipStream_t st);
+
 // Device batch tidy and graph update (lv_update.hip).
 void k_tidy_keys(GView g, const uint32_t* u, const uint32_t* v, uint64_t n, bool keep_existing, uint64_t* key,
                  uint32_t* idx, int* bad, hipStream_t st);

@@ -46,6 +46,11 @@ __device__ __forceinline__ double delta_q(double vcout, double vdout, double vto
   return (vcout - vdout) / M - R * vtot * (vtot + ctot - dtot) / (2 * M * M);
 }
 
+// Batched form: the M, community-0 id and skip flag of vertex u's graph.
+__device__ __forceinline__ double vertex_M(const MoveArgs& a, uint32_t u) { return a.gid ? a.gM[a.gid[u]] : a.M; }
+__device__ __forceinline__ uint32_t vertex_zero(const MoveArgs& a, uint32_t u) { return a.gid ? a.gzero[a.gid[u]] : 0u; }
+__device__ __forceinline__ bool vertex_skipped(const MoveArgs& a, uint32_t u) { return a.gid && a.gmask[a.gid[u]]; }
+
 __device__ __forceinline__ void tab_add(uint32_t* key, uint32_t* pos, double* val, uint64_t cap, uint32_t c,
                                         double w, uint32_t p) {
   uint32_t s = slot_of(c, cap);
@@ -408,7 +413,7 @@ __device__ __forceinline__ void table_scan(const MoveArgs& a, const WT* W, uint3
 }
 
 template <int G>
-__device__ __forceinline__ void table_best(const MoveArgs& a, uint32_t cu, double Ku, const uint32_t* key,
+__device__ __forceinline__ void table_best(const MoveArgs& a, uint32_t cu, double Ku, double M, const uint32_t* key,
                                            const uint32_t* pos, const double* val, uint64_t cap, int lane, double& be,
                                            uint32_t& bp, uint32_t& bc, unsigned long long& c_cand) {
   const double vd = tab_get(key, val, cap, cu);
@@ -419,7 +424,7 @@ __device__ __forceinline__ void table_best(const MoveArgs& a, uint32_t cu, doubl
     ++c_cand;
     if (c == cu) continue;
     if (a.pick_less && c > cu) continue;
-    double e = delta_q(val[s], vd, Ku, ld_dev(&a.ctot[c]), Sd, a.M, a.R);
+    double e = delta_q(val[s], vd, Ku, ld_dev(&a.ctot[c]), Sd, M, a.R);
     uint32_t p = pos[s];
     if (e > 0 && better(e, p, be, bp)) { be = e; bp = p; bc = c; }
   }
@@ -463,7 +468,8 @@ __global__ __launch_bounds__(BLOCK) void move_kernel(MoveArgs a) {
   const bool dyn = a.chunk != 0;
   const uint64_t streams = a.max_streams ? a.max_streams : 1;
   const uint64_t chunk = dyn ? a.chunk : ((n + streams - 1) / streams ? (n + streams - 1) / streams : 1);
-  const uint64_t nchunks = (n + chunk - 1) / chunk;
+  // a.ctab (id-order kernel, one group per block): chunk bounds per chunk
+  const uint64_t nchunks = a.ctab ? a.nct : (n + chunk - 1) / chunk;
   uint64_t next = (uint64_t)blockIdx.x * GPB;
   const WT* W = (const WT*)a.g.w;
   // LDS table of this group; HBM region of this block
@@ -498,6 +504,12 @@ __global__ __launch_bounds__(BLOCK) void move_kernel(MoveArgs a) {
     }
     if (cb >= nchunks) break;
     const uint64_t ch = cb + grp;
+    // entries [c0, c0 + clen) of the list
+    uint64_t c0 = ch * chunk, clen = chunk;
+    if (a.ctab) {
+      c0 = ch < nchunks ? a.ctab[2 * ch] : 0;
+      clen = ch < nchunks ? a.ctab[2 * ch + 1] - c0 : 0;
+    }
     // One vertex (offset o, degree d, community cu, weight Ku) through the
     // group; every thread of the block calls it.
     auto visit = [&](bool act, uint32_t u, uint64_t o, uint32_t d, uint32_t cu, double Ku) {
@@ -523,8 +535,9 @@ __global__ __launch_bounds__(BLOCK) void move_kernel(MoveArgs a) {
       double be = 0;
       uint32_t bp = kNoPos, bc = 0;
       if (act) {
-        if (hbm) table_best<G>(a, cu, Ku, hkey, hpos, hval, cap, lane, be, bp, bc, c_cand);
-        else table_best<G>(a, cu, Ku, lkey, lpos, lval, cap, lane, be, bp, bc, c_cand);
+        const double Mu = vertex_M(a, u);
+        if (hbm) table_best<G>(a, cu, Ku, Mu, hkey, hpos, hval, cap, lane, be, bp, bc, c_cand);
+        else table_best<G>(a, cu, Ku, Mu, lkey, lpos, lval, cap, lane, be, bp, bc, c_cand);
       }
       if constexpr (G <= 64) group_best_shfl<G>(be, bp, bc);
       else block_best<BLOCK>(be, bp, bc);
@@ -532,7 +545,9 @@ __global__ __launch_bounds__(BLOCK) void move_kernel(MoveArgs a) {
         if (a.choose_only) {
           if (lane == 0) { a.out_c[u] = bc; a.out_e[u] = be; }
         } else {
-          if (bc != 0) {
+          // a candidate exists (bp set) and it is not community 0 of u's
+          // graph (with no candidate bc is 0, as the reference's (0, 0))
+          if (bp != kNoPos && bc != vertex_zero(a, u)) {
             if (lane == 0) {
               unsafeAtomicAdd(&a.ctot[cu], -Ku);
               unsafeAtomicAdd(&a.ctot[bc], Ku);
@@ -547,7 +562,8 @@ __global__ __launch_bounds__(BLOCK) void move_kernel(MoveArgs a) {
           }
           if (lane == 0) {
             st_dev(&a.vaff[u], (uint8_t)0);
-            el_acc += be;
+            if (a.gel) unsafeAtomicAdd(&a.gel[a.gid[u]], be);
+            else el_acc += be;
             ++c_vert;
             c_edge += d;
           }
@@ -562,11 +578,11 @@ __global__ __launch_bounds__(BLOCK) void move_kernel(MoveArgs a) {
         // after each visit the later flags are read again, so a vertex
         // flagged by an earlier one of the window is still visited in this
         // sweep (inc/louvain.hxx:602 then :598).
-        for (uint64_t j0 = 0; j0 < chunk; j0 += 64) {
+        for (uint64_t j0 = 0; j0 < clen; j0 += 64) {
           // (a flagged entry's offset, degree, community and K are loaded
           // with the window; its community changes only when it is visited)
-          const uint64_t idx = ch * chunk + j0 + lane;
-          const bool in = ch < nchunks && j0 + lane < chunk && idx < n;
+          const uint64_t idx = c0 + j0 + lane;
+          const bool in = ch < nchunks && j0 + lane < clen && idx < n;
           uint32_t ul = 0, dl = 0, cl = 0;
           uint64_t ol = 0;
           double kl = 0;
@@ -574,7 +590,8 @@ __global__ __launch_bounds__(BLOCK) void move_kernel(MoveArgs a) {
           if (in) {
             ul = a.list ? a.list[idx] : (uint32_t)(a.base + idx);
             dl = a.g.deg[ul];
-            ok = (!a.g.ex || a.g.ex[ul]) && (a.choose_only || !a.fa || a.fa[ul]) && dl <= a.hub_min;
+            ok = (!a.g.ex || a.g.ex[ul]) && (a.choose_only || !a.fa || a.fa[ul]) && dl <= a.hub_min &&
+                 !vertex_skipped(a, ul);
           }
           bool live = ok && (a.choose_only || ld_dev(&a.vaff[ul]));
           unsigned long long mask = __ballot(live);
@@ -595,14 +612,15 @@ __global__ __launch_bounds__(BLOCK) void move_kernel(MoveArgs a) {
         continue;
       }
     }
-    for (uint64_t j = 0; j < chunk; ++j) {
-      const uint64_t idx = ch * chunk + j;
+    for (uint64_t j = 0; j < clen; ++j) {
+      const uint64_t idx = c0 + j;
       bool act = false;
       uint32_t u = 0;
       if (ch < nchunks && idx < n) {
         u = a.list ? a.list[idx] : (uint32_t)(a.base + idx);
         act = (!a.g.ex || a.g.ex[u]) && (a.choose_only || (ld_dev(&a.vaff[u]) && (!a.fa || a.fa[u])));
         if constexpr (MODE == MOVE_HYBRID) act = act && a.g.deg[u] <= a.hub_min;
+        act = act && !vertex_skipped(a, u);
       }
       uint64_t o = 0;
       uint32_t d = 0, cu = 0;
@@ -707,7 +725,7 @@ __global__ __launch_bounds__(kBlock) void hub_sizes_kernel(MoveArgs a, uint64_t*
     if (k < hi) {
       uint32_t u = a.list[k];
       uint64_t d = a.g.deg[u];
-      bool act = d > a.hub_min && (!a.g.ex || a.g.ex[u]) &&
+      bool act = d > a.hub_min && (!a.g.ex || a.g.ex[u]) && !vertex_skipped(a, u) &&
                  (a.choose_only || (ld_dev(&a.vaff[u]) && (!a.fa || a.fa[u])));
       if (act) {
         cap = kTabChunk;
@@ -829,7 +847,7 @@ __global__ __launch_bounds__(kBlock) void hub_best_kernel(MoveArgs a) {
     const uint64_t s0 = t * kTabChunk;
     const uint64_t k = find_owner(a.hb_cap, n, s0);
     const uint32_t cu = a.hb_cu[k], u = a.list[k];
-    const double vd = a.hb_vd[k], sd = a.hb_sd[k], Ku = a.vtot[u];
+    const double vd = a.hb_vd[k], sd = a.hb_sd[k], Ku = a.vtot[u], Mu = vertex_M(a, u);
     double be = 0;
     uint32_t bp = kNoPos, bc = 0;
     for (uint64_t s = s0 + threadIdx.x; s < s0 + kTabChunk; s += kBlock) {
@@ -841,7 +859,7 @@ __global__ __launch_bounds__(kBlock) void hub_best_kernel(MoveArgs a) {
       a.hb_pos[s] = kNoPos;
       a.hb_val[s] = 0;
       if (c == cu || (a.pick_less && c > cu)) continue;
-      double e = delta_q(w, vd, Ku, ld_dev(&a.ctot[c]), sd, a.M, a.R);
+      double e = delta_q(w, vd, Ku, ld_dev(&a.ctot[c]), sd, Mu, a.R);
       if (e > 0 && better(e, p, be, bp)) {
         be = e;
         bp = p;
@@ -888,7 +906,7 @@ __global__ __launch_bounds__(kBlock) void hub_apply_kernel(MoveArgs a) {
       a.out_e[u] = e;
       continue;
     }
-    if (bc != 0) {
+    if (bp != kNoPos && bc != vertex_zero(a, u)) {
       uint32_t cu = a.hb_cu[k];
       double Ku = a.vtot[u];
       unsafeAtomicAdd(&a.ctot[cu], -Ku);
@@ -899,7 +917,8 @@ __global__ __launch_bounds__(kBlock) void hub_apply_kernel(MoveArgs a) {
       cme += d;
     }
     st_dev(&a.vaff[u], (uint8_t)0);
-    el += e;
+    if (a.gel) unsafeAtomicAdd(&a.gel[a.gid[u]], e);
+    else el += e;
     ++cv;
     ce += d;
   }
@@ -1475,7 +1494,7 @@ static void move_tier(const MoveArgs& a, int tier, hipStream_t st) {
   // out dynamically, so the grid only sets how many walk at once.
   uint64_t streams = a.max_streams ? a.max_streams : 1;
   uint64_t groups = streams;
-  if (!a.n_dev) {
+  if (!a.n_dev && !a.ctab) {
     uint64_t chunk = a.chunk ? a.chunk : std::max<uint64_t>(1, (a.n + streams - 1) / streams);
     groups = std::min<uint64_t>(streams, (a.n + chunk - 1) / chunk);
   }

@@ -1,4 +1,5 @@
-// Device-side batch tidy and graph update (gfx950).
+// Device-side batch tidy and graph update, and the per-graph helpers of the
+// batched form (gfx950).
 //
 // tidyBatchUpdateU (inc/batch.hxx:202-209): deletions are kept when the edge
 // exists, insertions when it does not (filterEdgesByExistenceU, :154-162),
@@ -257,7 +258,113 @@ __global__ void upd_wsum_kernel(const float* w, uint64_t n, double sign, double*
   }
 }
 
+// ---------------------------------------------------------- batched form
+// Graph of every vertex: the k with voff[k] <= u < voff[k + 1].
+__global__ void gid_fill_kernel(const uint64_t* voff, uint64_t G, uint64_t S, uint32_t* gid) {
+  for (uint64_t u = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; u < S; u += (uint64_t)gridDim.x * blockDim.x)
+    gid[u] = (uint32_t)owner_of(voff, G, u);
+}
+
+// out[gid[u]] += x[u] (f64), one atomic per run of equal graphs in a wave.
+__global__ void group_sum_kernel(const double* x, const uint32_t* gid, uint64_t S, double* out) {
+  const int lane = threadIdx.x % 64;
+  for (uint64_t b = (blockIdx.x * (uint64_t)blockDim.x) & ~63ull; b < S; b += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t u = b + (threadIdx.x & ~63u) + lane;
+    const bool on = u < S;
+    const uint32_t gk = on ? gid[u] : 0xFFFFFFFFu;
+    double v = on ? x[u] : 0.0;
+    const uint32_t g0 = __shfl(gk, 0, 64);
+    if (__all(gk == g0 || !on)) {  // the whole wave in one graph
+      for (int m = 32; m > 0; m >>= 1) v += __shfl_xor(v, m, 64);
+      if (lane == 0 && g0 != 0xFFFFFFFFu) unsafeAtomicAdd(&out[g0], v);
+    } else if (on) {
+      unsafeAtomicAdd(&out[gk], v);
+    }
+  }
+}
+
+// Graph of the next level's super-vertices: gid_next[map[vcom[u]]] = gid[u].
+__global__ void gid_next_kernel(const uint32_t* vcom, const uint32_t* gid, const uint32_t* map, const uint8_t* ex,
+                                uint64_t S, uint32_t* gid_next) {
+  for (uint64_t u = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; u < S; u += (uint64_t)gridDim.x * blockDim.x)
+    if (!ex || ex[u]) gid_next[map[vcom[u]]] = gid[u];
+}
+
+// Final community ids of each graph re-based from its range of the last
+// level (lo) to its own vertex range (voff): membership of the original
+// vertices, and Sigma_tot moved to the re-based ids (tmp holds a copy).
+__global__ void rebase_memb_kernel(uint32_t* memb, const uint32_t* gid, const uint8_t* ex, uint64_t S,
+                                   const uint64_t* lo, const uint64_t* voff) {
+  for (uint64_t u = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; u < S; u += (uint64_t)gridDim.x * blockDim.x) {
+    if (ex && !ex[u]) continue;
+    const uint32_t k = gid[u];
+    memb[u] = (uint32_t)(memb[u] - lo[k] + voff[k]);
+  }
+}
+__global__ void rebase_tot_kernel(const double* tot, uint64_t SL, const uint64_t* lo, uint64_t G, const uint64_t* voff,
+                                  double* out) {
+  for (uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j < SL; j += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t k = owner_of(lo, G, j);
+    out[voff[k] + (j - lo[k])] = tot[j];
+  }
+}
+
+// Flagged vertices per graph.
+__global__ void group_count_kernel(const uint8_t* f, const uint32_t* gid, uint64_t S, unsigned long long* out) {
+  for (uint64_t u = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; u < S; u += (uint64_t)gridDim.x * blockDim.x)
+    if (f[u]) atomicAdd(&out[gid[u]], 1ull);
+}
+
+// Chunk t of the batched id-order schedule: graph k owns chunks
+// gcs[k] .. gcs[k+1]-1 and list entries [b0, b1) (ids glo[k] .. glo[k+1]-1;
+// an ascending list is searched, an id range is offset by base); its entries
+// are cut as the kernel cuts a list alone: ceil(len / streams) per chunk.
+__global__ void graph_chunks_kernel(MoveArgs a) {
+  const uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  if (t >= a.nct) return;
+  const uint64_t n = a.n_dev ? (uint64_t)*a.n_dev : a.n;
+  const uint64_t k = owner_of(a.gcs, a.ng, t);
+  auto bound = [&](uint64_t id) -> uint64_t {  // first entry with id >= `id`
+    if (!a.list) return id <= a.base ? 0 : min(n, id - a.base);
+    uint64_t lo = 0, hi = n;
+    while (lo < hi) {
+      const uint64_t mid = (lo + hi) >> 1;
+      if (a.list[mid] < id) lo = mid + 1;
+      else hi = mid;
+    }
+    return lo;
+  };
+  const uint64_t b0 = bound(a.glo[k]), b1 = bound(a.glo[k + 1]);
+  const uint64_t len = b1 - b0, s = a.gcs[k + 1] - a.gcs[k], j = t - a.gcs[k];
+  const uint64_t chunk = max<uint64_t>(1, (len + s - 1) / s);
+  a.ctab[2 * t] = b0 + min(len, j * chunk);
+  a.ctab[2 * t + 1] = b0 + min(len, (j + 1) * chunk);
+}
+
 }  // namespace
+
+void k_gid_fill(const uint64_t* voff, uint64_t G, uint64_t S, uint32_t* gid, hipStream_t st) {
+  if (S) LV_UPD_LAUNCH(gid_fill_kernel, blocks_for(S, kB), kB, st, voff, G, S, gid);
+}
+void k_group_sum(const double* x, const uint32_t* gid, uint64_t S, double* out, hipStream_t st) {
+  if (S) LV_UPD_LAUNCH(group_sum_kernel, blocks_for(S, kB), kB, st, x, gid, S, out);
+}
+void k_gid_next(const uint32_t* vcom, const uint32_t* gid, const uint32_t* map, const uint8_t* ex, uint64_t S,
+                uint32_t* gid_next, hipStream_t st) {
+  if (S) LV_UPD_LAUNCH(gid_next_kernel, blocks_for(S, kB), kB, st, vcom, gid, map, ex, S, gid_next);
+}
+void k_rebase(uint32_t* memb, const uint32_t* gid, const uint8_t* ex, uint64_t S, const double* tot, uint64_t SL,
+              const uint64_t* lo, uint64_t G, const uint64_t* voff, double* tot_out, hipStream_t st) {
+  if (S) LV_UPD_LAUNCH(rebase_memb_kernel, blocks_for(S, kB), kB, st, memb, gid, ex, S, lo, voff);
+  if (SL) LV_UPD_LAUNCH(rebase_tot_kernel, blocks_for(SL, kB), kB, st, tot, SL, lo, G, voff, tot_out);
+}
+void k_group_count(const uint8_t* f, const uint32_t* gid, uint64_t S, unsigned long long* out, hipStream_t st) {
+  if (S) LV_UPD_LAUNCH(group_count_kernel, blocks_for(S, kB), kB, st, f, gid, S, out);
+}
+
+void k_graph_chunks(const MoveArgs& a, hipStream_t st) {
+  if (a.nct) LV_UPD_LAUNCH(graph_chunks_kernel, blocks_for(a.nct, kB), kB, st, a);
+}
 
 void k_tidy_keys(GView g, const uint32_t* u, const uint32_t* v, uint64_t n, bool keep_existing, uint64_t* key,
                  uint32_t* idx, int* bad, hipStream_t st) {

@@ -41,6 +41,7 @@ EXPORTS = [
     "lv_dbatch_create", "lv_dbatch_destroy", "lv_dstate_create", "lv_dstate_destroy", "lv_dstate_upload",
     "lv_dstate_download", "lv_dstate_copy", "lv_run", "lv_dmodularity", "lv_edge_weight", "lv_modularity",
     "lv_dgraph_capacity", "lv_dgraph_update", "lv_dgraph_download", "lv_dbatch_sizes", "lv_dbatch_download",
+    "lv_run_batched",
     "lv_vertex_weights", "lv_update_weights", "lv_affected_frontier", "lv_affected_delta_screening",
     "lv_choose_community", "lv_renumber", "lv_aggregate",
 ]
@@ -357,6 +358,29 @@ class Context:
                                     C.byref(oc), C.byref(r)), "lv_run")
         return _to_result(r, memb, vt, ct)
 
+    def run_batched(self, approach, dgraph, dbatch, dstate, voff, o=None, copy_back=True):
+        """lv_run_batched: every graph of the union at once; returns one
+        LouvainResult per graph (graph-local ids when copy_back)."""
+        if isinstance(approach, str):
+            approach = APPROACHES[approach]
+        o = o or LouvainOptions()
+        voff = np.ascontiguousarray(voff, np.uint64)
+        G = len(voff) - 1
+        res = (_Result * G)()
+        keep = []
+        for k in range(G):
+            if copy_back:
+                n = int(voff[k + 1] - voff[k])
+                arrs = (np.zeros(n, np.uint32), np.zeros(n, np.float64), np.zeros(n, np.float64))
+                keep.append(arrs)
+                res[k] = _Result(_p(arrs[0]), _p(arrs[1]), _p(arrs[2]))
+            else:
+                keep.append((None, None, None))
+        oc = o._c()
+        self._check(self.lib.lv_run_batched(self.h, int(approach), dgraph.h, dbatch.h if dbatch else None, dstate.h,
+                                            _p(voff), C.c_uint64(G), C.byref(oc), res), "lv_run_batched")
+        return [_to_result(res[k], *keep[k]) for k in range(G)]
+
     def dmodularity(self, dgraph, dstate, resolution=1.0):
         out = C.c_double()
         self._check(self.lib.lv_dmodularity(self.h, dgraph.h, dstate.h, C.c_double(resolution), C.byref(out)),
@@ -511,3 +535,41 @@ def louvainDynamicDeltaScreeningHip(y, deletions, insertions, q, qvtot, qctot, o
 def louvainDynamicFrontierHip(y, deletions, insertions, q, qvtot, qctot, o=None):
     """louvainDynamicFrontierOmp (inc/louvain.hxx:1674-1690) on the GPU."""
     return default_context().louvain(LV_DYNAMIC_FRONTIER, y, deletions, insertions, q, qvtot, qctot, o)
+
+
+# ---------------------------------------------------------------- batched form
+def union_graphs(graphs):
+    """Graphs (numpy CSR dicts) side by side: (union dict, voff), graph k
+    owning ids [voff[k], voff[k+1])."""
+    spans = [len(g["off"]) - 1 for g in graphs]
+    voff = np.concatenate([[0], np.cumsum(spans)]).astype(np.uint64)
+    eoff = np.concatenate([[0], np.cumsum([int(g["off"][-1]) for g in graphs])]).astype(np.uint64)
+    off = np.concatenate([np.asarray(g["off"][:-1], np.uint64) + eoff[k] for k, g in enumerate(graphs)]
+                         + [eoff[-1:]]).astype(np.uint64)
+    keys = np.concatenate([np.asarray(g["keys"], np.uint32) + np.uint32(voff[k]) for k, g in enumerate(graphs)])
+    w = np.concatenate([np.asarray(g["w"], np.float32) for g in graphs])
+    u = {"off": off, "keys": keys, "w": w}
+    if any("ex" in g for g in graphs):
+        u["ex"] = np.concatenate([np.asarray(g.get("ex", np.ones(n, np.uint8)), np.uint8)
+                                  for g, n in zip(graphs, spans)])
+    return u, voff
+
+
+def union_batch(batches, voff):
+    """Per-graph batches (dicts u, v, w) as one batch over the union."""
+    return {"u": np.concatenate([np.asarray(b["u"], np.uint32) + np.uint32(voff[k]) for k, b in enumerate(batches)]),
+            "v": np.concatenate([np.asarray(b["v"], np.uint32) + np.uint32(voff[k]) for k, b in enumerate(batches)]),
+            "w": np.concatenate([np.asarray(b["w"], np.float32) for b in batches])}
+
+
+def union_state(states, voff):
+    """Per-graph (membership, K, Sigma_tot) as one union state: graph k's
+    community ids shifted into its range."""
+    S = int(voff[-1])
+    q, v, c = np.zeros(S, np.uint32), np.zeros(S, np.float64), np.zeros(S, np.float64)
+    for k, (qk, vk, ck) in enumerate(states):
+        a, b = int(voff[k]), int(voff[k + 1])
+        q[a:b] = np.asarray(qk, np.uint32) + np.uint32(a)
+        v[a:b] = vk
+        c[a:b] = ck
+    return q, v, c

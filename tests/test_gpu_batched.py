@@ -1,0 +1,115 @@
+"""Batched form (lv_run_batched): several graphs side by side in one device
+graph, every launch covering all of them, each graph keeping its own M,
+tolerance, iterations, passes and aggregation decisions.  Bar: with one
+stream (deterministic) every graph's partition, K, iteration and pass counts
+equal its run alone (lv_run); with the default schedule each graph's
+modularity is within 1e-3 of the reference's."""
+import numpy as np
+import pytest
+
+import fixtures as F
+
+pytestmark = pytest.mark.gpu
+
+NAMES = ["tiny1", "pp2k", "rmat10", "ppw", "tiny0"]
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    import lvhip
+    c = lvhip.Context(0)
+    yield c
+    c.close()
+
+
+def _canon(m, ex=None):
+    """Membership as a canonical labelling (first-appearance order)."""
+    m = np.asarray(m, np.int64).copy()
+    if ex is not None:
+        m[~ex.astype(bool)] = -1
+    _, inv = np.unique(m, return_inverse=True)
+    order = {}
+    out = np.empty(len(m), np.int64)
+    for i, x in enumerate(inv):
+        out[i] = order.setdefault(x, len(order))
+    return out
+
+
+def _inputs(which):
+    fs = [F.load(n) for n in NAMES]
+    gs = [F.graph(f, which) for f in fs]
+    bs = [(F.batch(f, "del"), F.batch(f, "ins")) for f in fs]
+    st = [(f["static_memb"], f["static_vtot"], f["static_ctot"]) for f in fs]
+    return fs, gs, bs, st
+
+
+def _run_both(ctx, approach, single_stream, passes=10):
+    import lvhip
+    fs, gs, bs, st = _inputs("x" if approach == "static" else "y")
+    u, voff = lvhip.union_graphs(gs)
+    o = lvhip.LouvainOptions(maxPasses=passes)
+    if single_stream:
+        ctx.set_schedule(1e-12, 1, 1, 16, 2)
+        ctx.set_hub_threshold(1 << 30, 4096)
+    try:
+        dg = ctx.device_graph(u)
+        db = None
+        if approach != "static":
+            db = ctx.device_batch(lvhip.union_batch([b[0] for b in bs], voff),
+                                  lvhip.union_batch([b[1] for b in bs], voff))
+        ds = ctx.device_state(int(voff[-1]))
+        if approach != "static":
+            ds.upload(*lvhip.union_state(st, voff))
+        batched = ctx.run_batched(approach, dg, db, ds, voff, o)
+        alone = []
+        for g, (d, i), s in zip(gs, bs, st):
+            if approach == "static":
+                alone.append(ctx.louvain("static", g, o=o))
+            else:
+                alone.append(ctx.louvain(approach, g, d, i, *s, o=o))
+    finally:
+        ctx.set_schedule()
+        ctx.set_hub_threshold()
+    return fs, gs, batched, alone
+
+
+@pytest.mark.parametrize("approach", ["static", "naive-dynamic", "dynamic-frontier"])
+def test_batched_first_pass_equals_alone(ctx, approach):
+    """One stream, one pass: the level-0 sweep of each graph in the union is
+    its sweep alone (own M, community 0, tolerance and iteration count), so
+    partition, K and iteration count are identical."""
+    _, gs, batched, alone = _run_both(ctx, approach, True, passes=1)
+    for name, g, b, a in zip(NAMES, gs, batched, alone):
+        print(name, approach, "it/p", b.iterations, b.passes, "alone", a.iterations, a.passes, flush=True)
+        ex = g.get("ex")
+        assert (b.iterations, b.passes) == (a.iterations, a.passes), name
+        assert np.array_equal(_canon(b.membership, ex), _canon(a.membership, ex)), name
+        assert np.array_equal(b.vertexWeight, a.vertexWeight), name
+
+
+@pytest.mark.parametrize("approach", ["static", "naive-dynamic", "dynamic-frontier"])
+def test_batched_all_passes_match_alone(ctx, oracle, approach):
+    """One stream, all passes: each graph's modularity within 1e-3 of its
+    run alone (later passes run on super-graphs whose row order follows the
+    hashtables, which the union's shifted ids change), K identical."""
+    _, gs, batched, alone = _run_both(ctx, approach, True)
+    for name, g, b, a in zip(NAMES, gs, batched, alone):
+        qb, qa = oracle.modularity(g, b.membership), oracle.modularity(g, a.membership)
+        print(name, approach, f"Q batched {qb:.6f} alone {qa:.6f} it/p {b.iterations}/{b.passes} "
+              f"alone {a.iterations}/{a.passes}", flush=True)
+        assert abs(qb - qa) <= 1e-3, name
+        assert np.array_equal(b.vertexWeight, a.vertexWeight), name
+
+
+@pytest.mark.parametrize("approach", ["static", "dynamic-frontier"])
+def test_batched_default_schedule_quality(ctx, oracle, approach):
+    fs, gs, batched, alone = _run_both(ctx, approach, False)
+    for name, f, g, b in zip(NAMES, fs, gs, batched):
+        q = oracle.modularity(g, b.membership)
+        key = "static_Q" if approach == "static" else "dynamic_frontier_Q"
+        ref = float(f[key])
+        print(name, approach, f"Q batched {q:.6f} reference {ref:.6f}", flush=True)
+        if approach == "static":
+            assert q >= ref - (0.05 if name.startswith("tiny") else 0.02), name
+        else:
+            assert abs(q - ref) <= 1e-3, name
