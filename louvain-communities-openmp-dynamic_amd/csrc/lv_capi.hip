@@ -742,7 +742,7 @@ void aggregate(lv_context* c, const Level& lv, const uint32_t* vcom, uint64_t C,
     uint64_t* chunks = c->ws<uint64_t>("ag.chunks", n4 + 1);
     uint64_t* hoff = c->ws<uint64_t>("ag.hoff", n4 + 1);
     uint64_t* choff = c->ws<uint64_t>("ag.choff", n4 + 1);
-    k_t4_sizes(z.off, a.list, n4, C, 4096, caps, chunks, c->st);
+    k_t4_sizes(z.off, a.list, n4, C, kAggHbmChunk, caps, chunks, c->st);
     scan64(c, caps, hoff, n4 + 1);
     scan64(c, chunks, choff, n4 + 1);
     uint64_t tot[1], nch[1];
@@ -1085,6 +1085,34 @@ void invoke_batched(lv_context* c, int approach, const lv_dgraph* x, const lv_db
       if (!bt.done[k]) return false;
     return true;
   };
+  // A graph is finalised at the level where it finishes: its membership and
+  // Sigma_tot go to its own vertex range (Sigma_tot into ftot), and the levels
+  // after that hold only the graphs still running.
+  double* ftot = c->ws<double>("bt.tot", S);
+  k_fill_f64(ftot, 0, S, c->st);
+  uint64_t* dlo = c->ws<uint64_t>("bt.lo", G + 1);
+  uint8_t* dfin = c->ws<uint8_t>("bt.fin", G);
+  uint8_t* dnew = c->ws<uint8_t>("bt.new", G);
+  uint8_t* ddone = c->ws<uint8_t>("bt.done", G);
+  uint8_t* lmask = c->ws<uint8_t>("bt.mask", S);
+  std::vector<uint8_t> fin(G, 0), fresh(G, 0);
+  // no graph is finalised yet (the flags of an earlier call are cleared:
+  // the lookups below leave a flagged graph's entries unmapped)
+  k_fill_u8(dfin, 0, G, c->st);
+  auto finalize = [&](bool first_level) {
+    bool any_new = false;
+    for (uint64_t k = 0; k < G; ++k) {
+      fresh[k] = bt.done[k] && !fin[k];
+      any_new = any_new || fresh[k];
+    }
+    if (!any_new) return;
+    h2d(c, dnew, fresh.data(), G);
+    h2d(c, dlo, lo.data(), G + 1);
+    k_finalize(ucom, first_level ? nullptr : vcom, gid0, x->ex, S, ctot, levelS, dlo, G, dnew, dvoff, ftot, c->st);
+    for (uint64_t k = 0; k < G; ++k) fin[k] = fin[k] || fresh[k];
+    h2d(c, dfin, fin.data(), G);
+  };
+  if (all_done()) finalize(true);
   int p = 0;
   while (!all_done()) {
     if (p == 1) { t1 = c->record(); t1_set = true; }
@@ -1110,40 +1138,55 @@ void invoke_batched(lv_context* c, int approach, const lv_dgraph* x, const lv_db
       p_out[k] += 1;
       if (m <= 1 || p_out[k] >= P) bt.done[k] = 1;
     }
-    if (all_done()) break;
-    uint32_t* map = nullptr;
     uint32_t* mcur = first ? ucom : vcom;
-    count_communities(c, cur.g, mcur, map);
-    // communities per graph: the renumbering map at the graph range bounds
+    uint32_t* map = nullptr;
     uint32_t* lo32 = c->ws<uint32_t>("bt.lo32", G + 1);
     uint32_t* mlo = c->ws<uint32_t>("bt.mlo", G + 1);
     std::vector<uint32_t> lo_h(G + 1), mlo_h(G + 1);
     for (uint64_t k = 0; k <= G; ++k) lo_h[k] = (uint32_t)lo[k];
-    h2d(c, lo32, lo_h.data(), G + 1);
-    k_gather_u32(map, lo32, G + 1, mlo, c->st);
-    c->read(mlo_h.data(), mlo, G + 1);
-    bool any = false;
-    for (uint64_t k = 0; k < G; ++k) {
-      const uint64_t CN = mlo_h[k + 1] - mlo_h[k];
-      if (!bt.done[k] && (double)CN / (double)GN[k] >= o->aggregation_tolerance) bt.done[k] = 1;
-      any = any || !bt.done[k];
-      lo_next[k] = mlo_h[k];
-      GN[k] = CN;
+    // communities per graph: the renumbering map at the graph range bounds
+    auto range_counts = [&] {
+      h2d(c, lo32, lo_h.data(), G + 1);
+      k_gather_u32(map, lo32, G + 1, mlo, c->st);
+      c->read(mlo_h.data(), mlo, G + 1);
+    };
+    const std::vector<uint64_t> GNlevel = GN;
+    if (!all_done()) {
+      count_communities(c, cur.g, mcur, map);
+      range_counts();
+      for (uint64_t k = 0; k < G; ++k) {
+        const uint64_t CN = mlo_h[k + 1] - mlo_h[k];
+        if (!bt.done[k] && (double)CN / (double)GN[k] >= o->aggregation_tolerance) bt.done[k] = 1;
+        GN[k] = CN;
+      }
     }
-    lo_next[G] = mlo_h[G];
-    if (!any) break;
-    // the union's communities become the next level (finished graphs ride
-    // along, skipped)
+    finalize(first);
+    if (all_done()) break;
+    // the next level: communities of the graphs still running
+    h2d(c, ddone, bt.done.data(), G);
+    k_active_mask(gidcur, levelS, first ? x->ex : nullptr, ddone, lmask, c->st);
+    GView act = cur.g;
+    act.ex = lmask;
+    uint64_t live = 0;
+    for (uint64_t k = 0; k < G; ++k)
+      if (!bt.done[k]) live += GNlevel[k];
+    count_communities(c, act, mcur, map);
+    range_counts();
+    for (uint64_t k = 0; k <= G; ++k) lo_next[k] = mlo_h[k];
     uint32_t* gidnext = gidcur == gidA ? gidB : gidA;
-    k_gid_next(mcur, gidcur, map, first ? x->ex : nullptr, levelS, gidnext, c->st);
-    if (first) k_lookup(ucom, S, map, c->st);
-    else {
+    k_gid_next(mcur, gidcur, map, lmask, levelS, gidnext, c->st);
+    if (first) {
+      k_lookup_active(ucom, S, map, gid0, dfin, c->st);
+    } else {
       k_lookup(vcom, cur.g.S, map, c->st);
-      k_lookup(ucom, S, vcom, c->st);
+      k_lookup_active(ucom, S, vcom, gid0, dfin, c->st);
     }
     const uint64_t CN = lo_next[G];
     hipEvent_t g0 = c->record();
-    aggregate(c, cur, mcur, CN, z, vtot);
+    Level agl = cur;
+    agl.g = act;
+    agl.order = live;
+    aggregate(c, agl, mcur, CN, z, vtot);
     ag.push_back({g0, c->record()});
     std::swap(y, z);
     GView yg{CN, y.off, y.deg, y.keys, y.w, nullptr};
@@ -1168,16 +1211,7 @@ void invoke_batched(lv_context* c, int approach, const lv_dgraph* x, const lv_db
     }
     h2d(c, bt.gzero, gzero.data(), G);
   }
-  if (p > 1) {
-    k_lookup(ucom, S, vcom, c->st);
-    // final ids of graph k: from its range of the last level to its vertex range
-    uint64_t* dlo = c->ws<uint64_t>("bt.lo", G + 1);
-    h2d(c, dlo, lo.data(), G + 1);
-    double* tmp = c->ws<double>("bt.tot", S);
-    d2d(c, tmp, ctot, S);
-    k_rebase(ucom, gid0, x->ex, S, ctot, levelS, dlo, G, dvoff, tmp, c->st);
-    d2d(c, ctot, tmp, S);
-  }
+  d2d(c, ctot, ftot, S);
   hipEvent_t e3 = c->record();
   if (!t1_set) t1 = e3;
   HIPCHK(hipEventSynchronize(e3));

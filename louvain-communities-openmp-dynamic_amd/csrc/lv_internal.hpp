@@ -23,6 +23,7 @@ constexpr uint32_t kTierMax[4] = {8, 32, 128, 1024};
 constexpr int kHybridLds = 512;
 constexpr uint64_t kHybridHbm = 8192;
 constexpr uint32_t kHubMin = 4096;
+constexpr uint64_t kAggHbmChunk = 1024;  // flat member edges per block chunk, tier-4 aggregation
 constexpr uint64_t kHubBlocks = 256;
 // hub batch: table slots per best-pass chunk (every hub table is a multiple),
 // blocks of the table-size scan
@@ -100,7 +101,7 @@ struct MoveArgs {
   const uint64_t* glo;
   const uint64_t* gcs;
   uint32_t ng;
-  uint8_t* fsame;            // worklist mode: flagged by a lower id and not yet
+  uint8_t* fsame;              // worklist mode: flagged by a lower id and not yet
                                //   visited this iteration -> next sub-round
   uint32_t* stamp;             // worklist mode: iteration serial of the last visit
   uint32_t serial;
@@ -231,14 +232,21 @@ void k_modularity_sum(const double* cin, const double* ctot, uint64_t n, double 
 // Exclusive scans (hipCUB) with grow-only temp storage owned by the caller.
 // Batched form helpers (lv_update.hip): graph of every vertex from the
 // vertex ranges voff (G + 1 entries), per-graph sums and counts, graph of
-// the next level's super-vertices, and the final re-basing of each graph's
-// community ids from its last-level range lo to its vertex range.
+// the next level's super-vertices, the level vertices of the graphs still
+// running, lookups that leave finished graphs alone, and the final ids of
+// the graphs flagged in fin: re-based from their range lo of the current
+// level to their vertex range, Sigma_tot into ftot.
 void k_gid_fill(const uint64_t* voff, uint64_t G, uint64_t S, uint32_t* gid, hipStream_t st);
 void k_group_sum(const double* x, const uint32_t* gid, uint64_t S, double* out, hipStream_t st);
 void k_gid_next(const uint32_t* vcom, const uint32_t* gid, const uint32_t* map, const uint8_t* ex, uint64_t S,
                 uint32_t* gid_next, hipStream_t st);
-void k_rebase(uint32_t* memb, const uint32_t* gid, const uint8_t* ex, uint64_t S, const double* tot, uint64_t SL,
-              const uint64_t* lo, uint64_t G, const uint64_t* voff, double* tot_out, hipStream_t st);
+void k_active_mask(const uint32_t* gid, uint64_t SL, const uint8_t* ex, const uint8_t* done, uint8_t* mask,
+                   hipStream_t st);
+void k_lookup_active(uint32_t* a, uint64_t n, const uint32_t* map, const uint32_t* gid, const uint8_t* skip,
+                     hipStream_t st);
+void k_finalize(uint32_t* ucom, const uint32_t* vcom, const uint32_t* gid0, const uint8_t* ex, uint64_t S,
+                const double* ctot, uint64_t SL, const uint64_t* lo, uint64_t G, const uint8_t* fin,
+                const uint64_t* voff, double* ftot, hipStream_t st);
 void k_group_count(const uint8_t* f, const uint32_t* gid, uint64_t S, unsigned long long* out, hipStream_t st);
 // The chunk table a.ctab (a.nct chunks) of the list the id-order kernel is
 // about to walk (a.list with a.n_dev / a.n entries, or ids a.base.. when

@@ -703,7 +703,7 @@ __device__ __forceinline__ void tab_merge(uint32_t* key, uint32_t* pos, double* 
     if (k == kEmpty) k = atomicCAS(&key[s], kEmpty, c);
     if (k == kEmpty || k == c) {
       unsafeAtomicAdd(&val[s], w);
-      atomicMin(&pos[s], p);
+      if (pos) atomicMin(&pos[s], p);
       return;
     }
     s = (uint32_t)((s + 1) & (cap - 1));
@@ -974,20 +974,43 @@ __global__ void lookup_kernel(uint32_t* a, uint64_t n, const uint32_t* map) {
 // ------------------------------------------------------------ aggregation
 // louvainCountCommunityVerticesOmpW / louvainCommunityVerticesOmpW
 // (inc/louvain.hxx:752-762, :800-812)
+// Atomic increment of x[c] for each active lane, aggregated over the wave:
+// the lanes sharing the first active lane's c add once (the others add
+// alone); returns the value the lane's increment took, so lanes sharing c
+// take consecutive values in lane order.
+__device__ __forceinline__ uint32_t wave_inc(uint32_t* x, bool on, uint32_t c) {
+  const unsigned long long act = __ballot(on);
+  if (!act) return 0;
+  const int lane = threadIdx.x % 64;
+  const int lead = __ffsll((long long)act) - 1;
+  const uint32_t c0 = __shfl(c, lead, 64);
+  const bool same = on && c == c0;
+  const unsigned long long grp = __ballot(same);
+  uint32_t base = 0;
+  if (lane == lead) base = atomicAdd(&x[c0], (uint32_t)__popcll(grp));
+  base = __shfl(base, lead, 64);
+  if (same) return base + (uint32_t)__popcll(grp & ((1ull << lane) - 1));
+  return on ? atomicAdd(&x[c], 1u) : 0;
+}
+
+// (block-uniform trip counts: the waves of a block stay together for the
+// wave-wide steps)
 __global__ void count_members_kernel(GView g, const uint32_t* vcom, uint32_t* cnt) {
-  for (uint64_t u = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; u < g.S; u += (uint64_t)gridDim.x * blockDim.x) {
-    if (g.ex && !g.ex[u]) continue;
-    atomicAdd(&cnt[vcom[u]], 1u);
+  for (uint64_t b = blockIdx.x * (uint64_t)blockDim.x; b < g.S; b += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t u = b + threadIdx.x;
+    const bool on = u < g.S && (!g.ex || g.ex[u]);
+    wave_inc(cnt, on, on ? vcom[u] : 0);
   }
 }
 
 __global__ void scatter_members_kernel(GView g, const uint32_t* vcom, const uint32_t* coff, uint32_t* cdeg,
                                        uint32_t* cedg) {
-  for (uint64_t u = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; u < g.S; u += (uint64_t)gridDim.x * blockDim.x) {
-    if (g.ex && !g.ex[u]) continue;
-    uint32_t c = vcom[u];
-    uint32_t k = atomicAdd(&cdeg[c], 1u);
-    cedg[coff[c] + k] = (uint32_t)u;
+  for (uint64_t b = blockIdx.x * (uint64_t)blockDim.x; b < g.S; b += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t u = b + threadIdx.x;
+    const bool on = u < g.S && (!g.ex || g.ex[u]);
+    const uint32_t c = on ? vcom[u] : 0;
+    const uint32_t k = wave_inc(cdeg, on, c);
+    if (on) cedg[coff[c] + k] = (uint32_t)u;
   }
 }
 
@@ -1071,35 +1094,103 @@ __global__ __launch_bounds__(BLOCK) void aggregate_kernel(AggArgs a) {
 }
 
 // Tier 4: communities with > 1024 member edges share one HBM table each,
-// filled by many blocks (CHUNK flat edges per block), emitted afterwards.
-constexpr uint64_t kAggChunk = 4096;
+// filled by many blocks (kAggHbmChunk flat member edges per block chunk).
+// A chunk is summed per neighbour community in an LDS table first (the
+// lanes of a wave that share the wave's first community add it once), then
+// merged into the HBM table with one atomic per community of the chunk: a
+// community whose edges mostly stay inside it costs one HBM atomic per
+// chunk, not one per edge.  The chunk's members (their flat edge starts and
+// row offsets) are staged in LDS, so an edge finds its member there.
+constexpr uint32_t kAggLdsTab = 2 * kAggHbmChunk;
+constexpr uint32_t kAggMembers = kAggHbmChunk;
+
+__device__ __forceinline__ void tab_add_wave(uint32_t* key, double* val, uint64_t cap, bool ok, uint32_t c,
+                                             double w) {
+  const unsigned long long act = __ballot(ok);
+  if (!act) return;
+  const int lane = threadIdx.x % 64;
+  const int lead = __ffsll((long long)act) - 1;
+  const uint32_t c0 = __shfl(c, lead, 64);
+  const bool same = ok && c == c0;
+  double sum = same ? w : 0.0;
+  for (int m = 32; m > 0; m >>= 1) sum += __shfl_xor(sum, m, 64);
+  if (lane == lead) tab_add(key, nullptr, val, cap, c0, sum, 0);
+  else if (ok && !same) tab_add(key, nullptr, val, cap, c, w, 0);
+}
 
 template <class WT>
-__global__ __launch_bounds__(1024) void aggregate_hbm_kernel(AggArgs a) {
+__global__ __launch_bounds__(kBlock) void aggregate_hbm_kernel(AggArgs a) {
+  __shared__ uint32_t s_key[kAggLdsTab];
+  __shared__ double s_val[kAggLdsTab];
+  __shared__ uint64_t s_ms[kAggMembers];   // flat edge start of each member of the chunk
+  __shared__ uint64_t s_off[kAggMembers];  // its row offset
+  __shared__ uint64_t s_k, s_j0, s_nm;
   const WT* W = (const WT*)a.g.w;
   for (uint64_t q = blockIdx.x; q < a.nchunks; q += gridDim.x) {
-    // tier-4 community k owning chunk q
-    uint64_t lo = 0, hi = a.n - 1;
-    while (lo < hi) {
-      uint64_t mid = (lo + hi + 1) >> 1;
-      if (a.choff[mid] <= q) lo = mid;
-      else hi = mid - 1;
+    for (uint32_t s = threadIdx.x; s < kAggLdsTab; s += kBlock) {
+      s_key[s] = kEmpty;
+      s_val[s] = 0;
     }
-    uint64_t k = lo;
-    uint32_t c = a.list[k];
-    uint64_t F0 = a.yoff[c], F1 = a.yoff[c + 1];
-    uint64_t m0 = a.coff[c], m1 = a.coff[c + 1];
-    uint64_t f0 = F0 + (q - a.choff[k]) * kAggChunk;
-    uint64_t f1 = f0 + kAggChunk < F1 ? f0 + kAggChunk : F1;
-    uint64_t cap = a.hoff[k + 1] - a.hoff[k];
-    uint32_t* key = a.hkey + a.hoff[k];
-    double* val = a.hval + a.hoff[k];
-    for (uint64_t f = f0 + threadIdx.x; f < f1; f += blockDim.x) {
-      uint64_t j = member_of(a.mscan, m0, m1, f);
-      uint32_t u = a.cedg[j];
-      uint64_t i = a.g.off[u] + (f - a.mscan[j]);
-      tab_add(key, nullptr, val, cap, a.vcom[a.g.keys[i]], (double)W[i], 0);
+    if (threadIdx.x == 0) {
+      // tier-4 community owning chunk q, the chunk's first and last member
+      const uint64_t k = find_owner(a.choff, a.n, q);
+      const uint32_t c = a.list[k];
+      const uint64_t F1 = a.yoff[c + 1], m0 = a.coff[c], m1 = a.coff[c + 1];
+      const uint64_t f0 = a.yoff[c] + (q - a.choff[k]) * kAggHbmChunk;
+      const uint64_t f1 = min(F1, f0 + kAggHbmChunk);
+      s_k = k;
+      s_j0 = member_of(a.mscan, m0, m1, f0);
+      s_nm = member_of(a.mscan, m0, m1, f1 - 1) - s_j0 + 1;
     }
+    __syncthreads();
+    const uint64_t k = s_k, j0 = s_j0, nm = s_nm;
+    const uint32_t c = a.list[k];
+    const uint64_t F1 = a.yoff[c + 1], m0 = a.coff[c], m1 = a.coff[c + 1];
+    const uint64_t f0 = a.yoff[c] + (q - a.choff[k]) * kAggHbmChunk;
+    const uint64_t f1 = min(F1, f0 + kAggHbmChunk);
+    // (a chunk spanning more members than LDS holds -- members without
+    // edges in between -- searches the member scan in HBM)
+    const bool staged = nm <= kAggMembers;
+    if (staged) {
+      for (uint64_t t = threadIdx.x; t < nm; t += kBlock) {
+        s_ms[t] = a.mscan[j0 + t];
+        s_off[t] = a.g.off[a.cedg[j0 + t]];
+      }
+    }
+    __syncthreads();
+    for (uint64_t fb = f0; fb < f1; fb += kBlock) {  // block-uniform trip count
+      const uint64_t f = fb + threadIdx.x;
+      const bool ok = f < f1;
+      uint32_t key = 0;
+      double w = 0;
+      if (ok) {
+        uint64_t i;
+        if (staged) {
+          uint64_t lo = 0, hi = nm - 1;  // last member starting at or before f
+          while (lo < hi) {
+            const uint64_t mid = (lo + hi + 1) >> 1;
+            if (s_ms[mid] <= f) lo = mid;
+            else hi = mid - 1;
+          }
+          i = s_off[lo] + (f - s_ms[lo]);
+        } else {
+          const uint64_t j = member_of(a.mscan, m0, m1, f);
+          i = a.g.off[a.cedg[j]] + (f - a.mscan[j]);
+        }
+        key = a.vcom[a.g.keys[i]];
+        w = (double)W[i];
+      }
+      tab_add_wave(s_key, s_val, kAggLdsTab, ok, key, w);
+    }
+    __syncthreads();
+    const uint64_t cap = a.hoff[k + 1] - a.hoff[k];
+    uint32_t* hkey = a.hkey + a.hoff[k];
+    double* hval = a.hval + a.hoff[k];
+    for (uint32_t s = threadIdx.x; s < kAggLdsTab; s += kBlock) {
+      const uint32_t ck = s_key[s];
+      if (ck != kEmpty) tab_merge(hkey, nullptr, hval, cap, ck, s_val[s], 0);
+    }
+    __syncthreads();
   }
 }
 
@@ -1568,7 +1659,7 @@ static void agg_tier(const AggArgs& a, int tier, hipStream_t st) {
     case 1: LV_LAUNCH((aggregate_kernel<256, 16, 64, WT>), grid_for(a.n, 16, 1u << 16), 256, st, a); break;
     case 2: LV_LAUNCH((aggregate_kernel<256, 64, 256, WT>), grid_for(a.n, 4, 1u << 16), 256, st, a); break;
     case 3: LV_LAUNCH((aggregate_kernel<256, 256, 2048, WT>), grid_for(a.n, 1, 1u << 14), 256, st, a); break;
-    default: LV_LAUNCH((aggregate_hbm_kernel<WT>), grid_for(a.nchunks, 1, 1u << 14), 1024, st, a); break;
+    default: LV_LAUNCH((aggregate_hbm_kernel<WT>), grid_for(a.nchunks, 1, 1u << 16), kBlock, st, a); break;
   }
 }
 void k_aggregate(const AggArgs& a, int tier, bool f64, hipStream_t st) {

@@ -290,29 +290,59 @@ __global__ void gid_next_kernel(const uint32_t* vcom, const uint32_t* gid, const
     if (!ex || ex[u]) gid_next[map[vcom[u]]] = gid[u];
 }
 
-// Final community ids of each graph re-based from its range of the last
-// level (lo) to its own vertex range (voff): membership of the original
-// vertices, and Sigma_tot moved to the re-based ids (tmp holds a copy).
-__global__ void rebase_memb_kernel(uint32_t* memb, const uint32_t* gid, const uint8_t* ex, uint64_t S,
-                                   const uint64_t* lo, const uint64_t* voff) {
+// Final community ids of the graphs flagged in fin, re-based from their
+// range of the current level (lo) to their own vertex range (voff): the
+// membership of their original vertices (the level id of u is vcom[ucom[u]],
+// or ucom[u] at the first level) and Sigma_tot of their communities
+// (ftot[voff + j - lo] = ctot[j] for level id j).
+__global__ void finalize_memb_kernel(uint32_t* ucom, const uint32_t* vcom, const uint32_t* gid, const uint8_t* ex,
+                                     uint64_t S, const uint8_t* fin, const uint64_t* lo, const uint64_t* voff) {
   for (uint64_t u = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; u < S; u += (uint64_t)gridDim.x * blockDim.x) {
     if (ex && !ex[u]) continue;
     const uint32_t k = gid[u];
-    memb[u] = (uint32_t)(memb[u] - lo[k] + voff[k]);
+    if (!fin[k]) continue;
+    const uint32_t id = vcom ? vcom[ucom[u]] : ucom[u];
+    ucom[u] = (uint32_t)(id - lo[k] + voff[k]);
   }
 }
-__global__ void rebase_tot_kernel(const double* tot, uint64_t SL, const uint64_t* lo, uint64_t G, const uint64_t* voff,
-                                  double* out) {
+__global__ void finalize_tot_kernel(const double* ctot, uint64_t SL, const uint64_t* lo, uint64_t G,
+                                    const uint8_t* fin, const uint64_t* voff, double* ftot) {
   for (uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j < SL; j += (uint64_t)gridDim.x * blockDim.x) {
     const uint64_t k = owner_of(lo, G, j);
-    out[voff[k] + (j - lo[k])] = tot[j];
+    if (fin[k]) ftot[voff[k] + (j - lo[k])] = ctot[j];
   }
 }
 
+// a[i] = map[a[i]] except for entries of the graphs flagged in skip.
+__global__ void lookup_active_kernel(uint32_t* a, uint64_t n, const uint32_t* map, const uint32_t* gid,
+                                     const uint8_t* skip) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    if (!skip[gid[i]]) a[i] = map[a[i]];
+}
+
+// Level vertices of the graphs still running (and existing, when ex is set).
+__global__ void active_mask_kernel(const uint32_t* gid, uint64_t SL, const uint8_t* ex, const uint8_t* done,
+                                   uint8_t* mask) {
+  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < SL; v += (uint64_t)gridDim.x * blockDim.x)
+    mask[v] = (!ex || ex[v]) && !done[gid[v]];
+}
+
 // Flagged vertices per graph.
+// (the lanes of a wave in the first flagged lane's graph add once)
 __global__ void group_count_kernel(const uint8_t* f, const uint32_t* gid, uint64_t S, unsigned long long* out) {
-  for (uint64_t u = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; u < S; u += (uint64_t)gridDim.x * blockDim.x)
-    if (f[u]) atomicAdd(&out[gid[u]], 1ull);
+  const int lane = threadIdx.x % 64;
+  for (uint64_t b = blockIdx.x * (uint64_t)blockDim.x; b < S; b += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t u = b + threadIdx.x;
+    const bool on = u < S && f[u];
+    const uint32_t k = on ? gid[u] : 0;
+    const unsigned long long act = __ballot(on);
+    if (!act) continue;
+    const int lead = __ffsll((long long)act) - 1;
+    const uint32_t k0 = __shfl(k, lead, 64);
+    const unsigned long long grp = __ballot(on && k == k0);
+    if (lane == lead) atomicAdd(&out[k0], (unsigned long long)__popcll(grp));
+    else if (on && k != k0) atomicAdd(&out[k], 1ull);
+  }
 }
 
 // Chunk t of the batched id-order schedule: graph k owns chunks
@@ -353,10 +383,19 @@ void k_gid_next(const uint32_t* vcom, const uint32_t* gid, const uint32_t* map, 
                 uint32_t* gid_next, hipStream_t st) {
   if (S) LV_UPD_LAUNCH(gid_next_kernel, blocks_for(S, kB), kB, st, vcom, gid, map, ex, S, gid_next);
 }
-void k_rebase(uint32_t* memb, const uint32_t* gid, const uint8_t* ex, uint64_t S, const double* tot, uint64_t SL,
-              const uint64_t* lo, uint64_t G, const uint64_t* voff, double* tot_out, hipStream_t st) {
-  if (S) LV_UPD_LAUNCH(rebase_memb_kernel, blocks_for(S, kB), kB, st, memb, gid, ex, S, lo, voff);
-  if (SL) LV_UPD_LAUNCH(rebase_tot_kernel, blocks_for(SL, kB), kB, st, tot, SL, lo, G, voff, tot_out);
+void k_finalize(uint32_t* ucom, const uint32_t* vcom, const uint32_t* gid0, const uint8_t* ex, uint64_t S,
+                const double* ctot, uint64_t SL, const uint64_t* lo, uint64_t G, const uint8_t* fin,
+                const uint64_t* voff, double* ftot, hipStream_t st) {
+  if (S) LV_UPD_LAUNCH(finalize_memb_kernel, blocks_for(S, kB), kB, st, ucom, vcom, gid0, ex, S, fin, lo, voff);
+  if (SL) LV_UPD_LAUNCH(finalize_tot_kernel, blocks_for(SL, kB), kB, st, ctot, SL, lo, G, fin, voff, ftot);
+}
+void k_lookup_active(uint32_t* a, uint64_t n, const uint32_t* map, const uint32_t* gid, const uint8_t* skip,
+                     hipStream_t st) {
+  if (n) LV_UPD_LAUNCH(lookup_active_kernel, blocks_for(n, kB), kB, st, a, n, map, gid, skip);
+}
+void k_active_mask(const uint32_t* gid, uint64_t SL, const uint8_t* ex, const uint8_t* done, uint8_t* mask,
+                   hipStream_t st) {
+  if (SL) LV_UPD_LAUNCH(active_mask_kernel, blocks_for(SL, kB), kB, st, gid, SL, ex, done, mask);
 }
 void k_group_count(const uint8_t* f, const uint32_t* gid, uint64_t S, unsigned long long* out, hipStream_t st) {
   if (S) LV_UPD_LAUNCH(group_count_kernel, blocks_for(S, kB), kB, st, f, gid, S, out);
