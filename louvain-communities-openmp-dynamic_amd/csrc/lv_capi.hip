@@ -163,6 +163,16 @@ struct lv_dgraph {
   TierLists tiers;  // five degree tiers
   TierLists hubs;   // tier 4 = degree > kHubMin (id-order mode), ascending
   uint64_t hub_slots = 0;
+  uint64_t version = 0;  // counts the updates applied to the graph
+  // Batched form: 2M and existing vertices of each graph range voff, as of
+  // `version` (a property of the resident graph, kept between runs)
+  struct Groups {
+    uint64_t version = ~0ull;
+    std::vector<uint64_t> voff;
+    std::vector<double> M2;
+    std::vector<uint64_t> order;
+  };
+  mutable Groups groups;
   GView view() const { return GView{S, off, deg, keys, w, ex}; }
 };
 
@@ -1010,14 +1020,31 @@ void invoke_batched(lv_context* c, int approach, const lv_dgraph* x, const lv_db
   uint32_t* gidA = c->ws<uint32_t>("bt.gidA", S);
   uint32_t* gidB = c->ws<uint32_t>("bt.gidB", S);
   k_gid_fill(dvoff, G, S, gid0, c->st);
-  double* kk = c->ws<double>("bt.k", S);
-  double* m2 = c->ws<double>("bt.m2", G);
-  k_fill_f64(kk, 0, S, c->st);
-  k_fill_f64(m2, 0, G, c->st);
-  k_vertex_weights(x->view(), x->f64, kk, c->st);
-  k_group_sum(kk, gid0, S, m2, c->st);
-  std::vector<double> M2(G);
-  c->read(M2.data(), m2, G);
+  lv_dgraph::Groups& gr = x->groups;
+  if (gr.version != x->version || gr.voff != voff) {
+    double* kk = c->ws<double>("bt.k", S);
+    double* m2 = c->ws<double>("bt.m2", G);
+    k_fill_f64(kk, 0, S, c->st);
+    k_fill_f64(m2, 0, G, c->st);
+    k_vertex_weights(x->view(), x->f64, kk, c->st);
+    k_group_sum(kk, gid0, S, m2, c->st);
+    gr.M2.assign(G, 0);
+    c->read(gr.M2.data(), m2, G);
+    // existing vertices per graph (GN of the first aggregation test)
+    gr.order.assign(G, 0);
+    if (x->ex) {
+      unsigned long long* gn = c->ws<unsigned long long>("bt.gn", G);
+      k_range_count(x->ex, dvoff, G, gn, c->st);
+      std::vector<unsigned long long> h(G);
+      c->read(h.data(), gn, G);
+      for (uint64_t k = 0; k < G; ++k) gr.order[k] = h[k];
+    } else {
+      for (uint64_t k = 0; k < G; ++k) gr.order[k] = voff[k + 1] - voff[k];
+    }
+    gr.voff = voff;
+    gr.version = x->version;
+  }
+  const std::vector<double>& M2 = gr.M2;
   Batched bt;
   bt.G = G;
   bt.gid = gid0;
@@ -1040,18 +1067,7 @@ void invoke_batched(lv_context* c, int approach, const lv_dgraph* x, const lv_db
   }
   h2d(c, bt.gM, gM.data(), G);
   h2d(c, bt.gzero, gzero.data(), G);
-  // live vertices per graph at level 0 (GN of the first aggregation test)
-  std::vector<uint64_t> GN(G);
-  if (x->ex) {
-    unsigned long long* gn = c->ws<unsigned long long>("bt.gn", G);
-    k_fill_u64(reinterpret_cast<uint64_t*>(gn), 0, G, c->st);
-    k_group_count(x->ex, gid0, S, gn, c->st);
-    std::vector<unsigned long long> h(G);
-    c->read(h.data(), gn, G);
-    for (uint64_t k = 0; k < G; ++k) GN[k] = h[k];
-  } else {
-    for (uint64_t k = 0; k < G; ++k) GN[k] = voff[k + 1] - voff[k];
-  }
+  std::vector<uint64_t> GN = gr.order;
   l_out.assign(G, 0);
   p_out.assign(G, 0);
   k_fill_u8(vaff, 0, (S + 7) & ~3ull, c->st);
@@ -1249,8 +1265,7 @@ void run_batched(lv_context* c, int approach, const lv_dgraph* x, const lv_dbatc
   }
   const uint64_t S = x->S;
   unsigned long long* aff = c->ws<unsigned long long>("bt.aff", G);
-  k_fill_u64(reinterpret_cast<uint64_t*>(aff), 0, G, c->st);
-  k_group_count(c->ws<uint8_t>("vaff", (S + 7) & ~3ull), c->ws<uint32_t>("bt.gid0", S), S, aff, c->st);
+  k_range_count(c->ws<uint8_t>("vaff", (S + 7) & ~3ull), c->ws<uint64_t>("bt.voff", G + 1), G, aff, c->st);
   std::vector<unsigned long long> affh(G);
   c->read(affh.data(), aff, G);
   if (res) {
@@ -1479,6 +1494,7 @@ uint64_t tidy_list(lv_context* c, const lv_dgraph* d, const lv_batch* b, bool ke
 void apply_update(lv_context* c, lv_dgraph* d, const lv_dbatch* b) {
   const uint64_t S = d->S, fb = (S + 7) & ~3ull;
   if (!b->nd && !b->ni) return;
+  ++d->version;
   bool fresh = false;
   int* delta = c->ws<int>("up.delta", S, &fresh);  // zero between calls
   if (fresh) k_fill_u32(reinterpret_cast<uint32_t*>(delta), 0, c->ws_bytes("up.delta") / 4, c->st);

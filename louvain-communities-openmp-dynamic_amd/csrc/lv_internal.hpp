@@ -231,7 +231,7 @@ void k_modularity_sum(const double* cin, const double* ctot, uint64_t n, double 
                       hipStream_t st);
 // Exclusive scans (hipCUB) with grow-only temp storage owned by the caller.
 // Batched form helpers (lv_update.hip): graph of every vertex from the
-// vertex ranges voff (G + 1 entries), per-graph sums and counts, graph of
+// vertex ranges voff (G + 1 entries), per-graph sums, flag counts per range, graph of
 // the next level's super-vertices, the level vertices of the graphs still
 // running, lookups that leave finished graphs alone, and the final ids of
 // the graphs flagged in fin: re-based from their range lo of the current
@@ -247,7 +247,7 @@ void k_lookup_active(uint32_t* a, uint64_t n, const uint32_t* map, const uint32_
 void k_finalize(uint32_t* ucom, const uint32_t* vcom, const uint32_t* gid0, const uint8_t* ex, uint64_t S,
                 const double* ctot, uint64_t SL, const uint64_t* lo, uint64_t G, const uint8_t* fin,
                 const uint64_t* voff, double* ftot, hipStream_t st);
-void k_group_count(const uint8_t* f, const uint32_t* gid, uint64_t S, unsigned long long* out, hipStream_t st);
+void k_range_count(const uint8_t* f, const uint64_t* voff, uint64_t G, unsigned long long* out, hipStream_t st);
 // The chunk table a.ctab (a.nct chunks) of the list the id-order kernel is
 // about to walk (a.list with a.n_dev / a.n entries, or ids a.base.. when
 // a.list is null): graph k's entries in a.gcs[k+1] - a.gcs[k] chunks of

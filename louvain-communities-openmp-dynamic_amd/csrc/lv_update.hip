@@ -327,21 +327,24 @@ __global__ void active_mask_kernel(const uint32_t* gid, uint64_t SL, const uint8
     mask[v] = (!ex || ex[v]) && !done[gid[v]];
 }
 
-// Flagged vertices per graph.
-// (the lanes of a wave in the first flagged lane's graph add once)
-__global__ void group_count_kernel(const uint8_t* f, const uint32_t* gid, uint64_t S, unsigned long long* out) {
-  const int lane = threadIdx.x % 64;
-  for (uint64_t b = blockIdx.x * (uint64_t)blockDim.x; b < S; b += (uint64_t)gridDim.x * blockDim.x) {
-    const uint64_t u = b + threadIdx.x;
-    const bool on = u < S && f[u];
-    const uint32_t k = on ? gid[u] : 0;
-    const unsigned long long act = __ballot(on);
-    if (!act) continue;
-    const int lead = __ffsll((long long)act) - 1;
-    const uint32_t k0 = __shfl(k, lead, 64);
-    const unsigned long long grp = __ballot(on && k == k0);
-    if (lane == lead) atomicAdd(&out[k0], (unsigned long long)__popcll(grp));
-    else if (on && k != k0) atomicAdd(&out[k], 1ull);
+// Flagged entries of each graph range [voff[k], voff[k+1]): one block per
+// graph, a block-wide sum, one store.
+__global__ __launch_bounds__(kB) void range_count_kernel(const uint8_t* f, const uint64_t* voff, uint64_t G,
+                                                         unsigned long long* out) {
+  __shared__ unsigned long long s_part[kB / 64];
+  const int lane = threadIdx.x % 64, wv = threadIdx.x / 64;
+  for (uint64_t k = blockIdx.x; k < G; k += gridDim.x) {
+    unsigned long long n = 0;
+    for (uint64_t u = voff[k] + threadIdx.x; u < voff[k + 1]; u += kB) n += f[u] != 0;
+    for (int m = 32; m > 0; m >>= 1) n += __shfl_xor(n, m, 64);
+    if (lane == 0) s_part[wv] = n;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      unsigned long long t = 0;
+      for (int i = 0; i < kB / 64; ++i) t += s_part[i];
+      out[k] = t;
+    }
+    __syncthreads();
   }
 }
 
@@ -397,8 +400,8 @@ void k_active_mask(const uint32_t* gid, uint64_t SL, const uint8_t* ex, const ui
                    hipStream_t st) {
   if (SL) LV_UPD_LAUNCH(active_mask_kernel, blocks_for(SL, kB), kB, st, gid, SL, ex, done, mask);
 }
-void k_group_count(const uint8_t* f, const uint32_t* gid, uint64_t S, unsigned long long* out, hipStream_t st) {
-  if (S) LV_UPD_LAUNCH(group_count_kernel, blocks_for(S, kB), kB, st, f, gid, S, out);
+void k_range_count(const uint8_t* f, const uint64_t* voff, uint64_t G, unsigned long long* out, hipStream_t st) {
+  if (G) LV_UPD_LAUNCH(range_count_kernel, blocks_for(G, 1), kB, st, f, voff, G, out);
 }
 
 void k_graph_chunks(const MoveArgs& a, hipStream_t st) {
