@@ -237,12 +237,14 @@ int lv_run(lv_context* ctx, int approach, const lv_dgraph* g, const lv_dbatch* b
  * Static, ND or DF (not DS) runs on every graph at once -- each kernel
  * launch covers all graphs -- with each graph's own M, tolerance, iteration
  * count, pass count and aggregation decision, as lv_run on that graph alone
- * would (inc/louvain.hxx:1120-1221 per graph).  res holds num_graphs results
- * with graph-local membership ids (arrays optional); the state keeps the
- * union ids, each graph's re-based into its own range, ready for the next
- * batch.  Membership labels (not the partition) of a graph that finishes
- * before others may differ from a run alone.  No counterpart in the
- * reference, which runs one graph per call. */
+ * would (inc/louvain.hxx:1120-1221 per graph), and in as many concurrent
+ * streams as it would get alone.  A graph that finishes leaves the later
+ * levels.  res holds num_graphs results with graph-local membership ids
+ * (arrays optional); the state keeps the union ids, each graph's re-based
+ * into its own range, ready for the next batch.  community_weight holds
+ * Sigma_tot at each graph's community ids and 0 at ids that are no
+ * community.  No counterpart in the reference, which runs one graph per
+ * call. */
 int lv_run_batched(lv_context* ctx, int approach, const lv_dgraph* g, const lv_dbatch* b, lv_dstate* s,
                    const uint64_t* voff, uint64_t num_graphs, const lv_options* o, lv_result* res);
 /* Modularity of a device state's membership on a device graph
