@@ -711,15 +711,27 @@ void aggregate(lv_context* c, const Level& lv, const uint32_t* vcom, uint64_t C,
   uint64_t GN = lv.order;
   uint32_t* coff = c->ws<uint32_t>("ag.coff", C + 1);
   uint32_t* cnt = c->ws<uint32_t>("ag.cnt", C + 1);
-  uint32_t* cdeg = c->ws<uint32_t>("ag.cdeg", C);
-  uint32_t* cedg = c->ws<uint32_t>("ag.cedg", std::max<uint64_t>(GN, 1));
   uint64_t* mdeg = c->ws<uint64_t>("ag.mdeg", GN + 1);
   uint64_t* mscan = c->ws<uint64_t>("ag.mscan", GN + 1);
   k_fill_u32(cnt, 0, C + 1, c->st);
   k_count_members(g, vcom, cnt, c->st);
   scan32(c, cnt, coff, C + 1);
-  k_fill_u32(cdeg, 0, C, c->st);
-  k_scatter_members(g, vcom, coff, cdeg, cedg, c->st);
+  // members of each community in ascending id order: a stable sort of
+  // (community, id) over the level's ids, the absent ones (key C) last
+  uint32_t* mk = c->ws<uint32_t>("ag.mk", g.S);
+  uint32_t* mk2 = c->ws<uint32_t>("ag.mk2", g.S);
+  uint32_t* mid = c->ws<uint32_t>("ag.mid", g.S);
+  uint32_t* cedg = c->ws<uint32_t>("ag.cedg", g.S);
+  k_member_keys(g, vcom, (uint32_t)C, mk, mid, c->st);
+  int bits = 1;
+  while (bits < 32 && (C >> bits)) ++bits;
+  {
+    size_t bytes = 0;
+    HIPCHK(sort_pairs_u32(mk, mk2, mid, cedg, g.S, bits, nullptr, bytes, c->st));
+    void* tmp = c->scan_tmp(bytes);
+    bytes = c->scan_tmp_bytes;
+    HIPCHK(sort_pairs_u32(mk, mk2, mid, cedg, g.S, bits, tmp, bytes, c->st));
+  }
   k_member_degrees(cedg, GN, g, mdeg, c->st);
   scan64(c, mdeg, mscan, GN + 1);
   k_gather_yoff(coff, mscan, C, z.off, c->st);
@@ -759,14 +771,25 @@ void aggregate(lv_context* c, const Level& lv, const uint32_t* vcom, uint64_t C,
     c->read(tot, hoff + n4, 1);
     c->read(nch, choff + n4, 1);
     a.hkey = c->ws<uint32_t>("ag.hkey", tot[0]);
+    a.hpos = c->ws<uint32_t>("ag.hpos", tot[0]);
     a.hval = c->ws<double>("ag.hval", tot[0]);
     a.hoff = hoff;
     a.choff = choff;
     a.nchunks = nch[0];
     k_fill_u32(a.hkey, kEmpty, tot[0], c->st);
+    k_fill_u32(a.hpos, kNoPos, tot[0], c->st);
     k_fill_f64(a.hval, 0, tot[0], c->st);
     k_aggregate(a, 4, lv.f64, c->st);
-    k_agg_emit_hbm(a, tot[0], c->st);
+    // rows in first-seen order: a bitmap of 32 words per 1024-edge chunk
+    static_assert(kAggHbmChunk == 32 * 32, "32 bitmap words per tier-4 chunk");
+    const uint64_t words = nch[0] * 32;
+    uint32_t* bm = c->ws<uint32_t>("ag.bm", words);
+    uint32_t* wcnt = c->ws<uint32_t>("ag.wcnt", words + 1);
+    uint32_t* bsc = c->ws<uint32_t>("ag.bsc", words + 1);
+    k_fill_u32(bm, 0, words, c->st);
+    k_agg_mark_hbm(a, tot[0], bm, words, wcnt, c->st);
+    scan32(c, wcnt, bsc, words + 1);
+    k_agg_emit_hbm(a, tot[0], bm, bsc, c->st);
   }
   c->check();
 }

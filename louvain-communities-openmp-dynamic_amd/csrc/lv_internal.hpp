@@ -179,8 +179,6 @@ void k_move(const MoveArgs& a, int tier, bool f64, hipStream_t st);
 void k_exists(GView g, const uint32_t* vcom, uint32_t* flags, hipStream_t st);
 void k_lookup(uint32_t* a, uint64_t n, const uint32_t* map, hipStream_t st);
 void k_count_members(GView g, const uint32_t* vcom, uint32_t* cnt, hipStream_t st);
-void k_scatter_members(GView g, const uint32_t* vcom, const uint32_t* coff, uint32_t* cdeg, uint32_t* cedg,
-                       hipStream_t st);
 void k_member_degrees(const uint32_t* cedg, uint64_t n, GView g, uint64_t* mdeg, hipStream_t st);
 void k_gather_yoff(const uint32_t* coff, const uint64_t* mscan, uint64_t C, uint64_t* yoff, hipStream_t st);
 // Degree partition of ids 0..n-1 (live only), ascending within each part:
@@ -203,15 +201,28 @@ struct AggArgs {
   uint32_t* zkeys;
   double* zw;
   double* ztot;              // row sums (K of the super-vertices), accumulated
-  // tier 4: shared HBM tables, one per community, chunked over blocks
+  // tier 4: shared HBM tables, one per community, chunked over blocks; a
+  // slot's position is its community's first member edge relative to the row
   uint32_t* hkey;
+  uint32_t* hpos;
   double* hval;
   const uint64_t* hoff;      // table offset per tier-4 community (n + 1)
   const uint64_t* choff;     // chunk offsets per tier-4 community (n + 1)
   uint64_t nchunks;
 };
 void k_aggregate(const AggArgs& a, int tier, bool f64, hipStream_t st);
-void k_agg_emit_hbm(const AggArgs& a, uint64_t total_slots, hipStream_t st);
+// Tier-4 rows in first-seen order: mark each slot's first position in the
+// bitmap bm (32 words per chunk, zeroed), popcounts of the words into cnt
+// (words + 1 entries, the last 0); after an exclusive scan of cnt into bsc,
+// emit the rows and their degrees.
+void k_agg_mark_hbm(const AggArgs& a, uint64_t total_slots, uint32_t* bm, uint64_t words, uint32_t* cnt,
+                    hipStream_t st);
+void k_agg_emit_hbm(const AggArgs& a, uint64_t total_slots, const uint32_t* bm, const uint32_t* bsc, hipStream_t st);
+// Members in ascending id order per community: sort keys (community, or C
+// for a vertex that does not exist) and ids, then a stable radix sort.
+void k_member_keys(GView g, const uint32_t* vcom, uint32_t C, uint32_t* key, uint32_t* id, hipStream_t st);
+hipError_t sort_pairs_u32(const uint32_t* kin, uint32_t* kout, const uint32_t* vin, uint32_t* vout, uint64_t n,
+                          int end_bit, void* tmp, size_t& tmp_bytes, hipStream_t st);
 void k_t4_sizes(const uint64_t* yoff, const uint32_t* list, uint64_t n, uint64_t C, uint64_t chunk,
                 uint64_t* caps, uint64_t* chunks, hipStream_t st);
 void k_bin_queue(const uint32_t* q, const uint32_t* qlen, uint64_t qcap, const uint32_t* deg, uint32_t thr,

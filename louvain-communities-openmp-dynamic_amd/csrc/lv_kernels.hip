@@ -1003,17 +1003,6 @@ __global__ void count_members_kernel(GView g, const uint32_t* vcom, uint32_t* cn
   }
 }
 
-__global__ void scatter_members_kernel(GView g, const uint32_t* vcom, const uint32_t* coff, uint32_t* cdeg,
-                                       uint32_t* cedg) {
-  for (uint64_t b = blockIdx.x * (uint64_t)blockDim.x; b < g.S; b += (uint64_t)gridDim.x * blockDim.x) {
-    const uint64_t u = b + threadIdx.x;
-    const bool on = u < g.S && (!g.ex || g.ex[u]);
-    const uint32_t c = on ? vcom[u] : 0;
-    const uint32_t k = wave_inc(cdeg, on, c);
-    if (on) cedg[coff[c] + k] = (uint32_t)u;
-  }
-}
-
 // Degrees of the members in member order (+ a trailing 0): its exclusive
 // scan gives each community's flat edge range, which is also the
 // super-graph row offsets of louvainCommunityTotalDegreeOmpW + scan
@@ -1043,16 +1032,27 @@ __device__ __forceinline__ uint64_t member_of(const uint64_t* mscan, uint64_t m0
 // edge weights by the neighbour's community over all members' edges
 // (self-loops included) and write the (community, weight) pairs into row c
 // of the gapped super-graph.  LDS tiers 0..3, one group per community.
+//
+// Row order is the reference's: communities in the order they are first
+// seen over the members (ascending ids) and their edges (louvainScanCommunitiesW
+// appends a community when first seen).  Each table slot keeps the first
+// flat member-edge index r at which its community appeared; a bitmap over
+// the community's member edges marks those r, and a slot's place in the row
+// is the number of marked bits below its r.
 template <int BLOCK, int G, int CAP, class WT>
 __global__ __launch_bounds__(BLOCK) void aggregate_kernel(AggArgs a) {
   constexpr int GPB = BLOCK / G;
+  constexpr int BW = CAP / 64 > 0 ? CAP / 64 : 1;  // bitmap words (member edges <= CAP / 2)
   __shared__ uint32_t s_key[GPB * CAP];
+  __shared__ uint32_t s_pos[GPB * CAP];
   __shared__ double s_val[GPB * CAP];
-  __shared__ uint32_t s_cnt[GPB];
+  __shared__ uint32_t s_bm[GPB * BW];
   const int lane = threadIdx.x % G, grp = threadIdx.x / G;
   const WT* W = (const WT*)a.g.w;
   uint32_t* tkey = s_key + grp * CAP;
+  uint32_t* tpos = s_pos + grp * CAP;
   double* tval = s_val + grp * CAP;
+  uint32_t* bm = s_bm + grp * BW;
   for (uint64_t base = (uint64_t)blockIdx.x * GPB; base < a.n; base += (uint64_t)gridDim.x * GPB) {
     const uint64_t idx = base + grp;
     const bool act = idx < a.n;
@@ -1063,8 +1063,8 @@ __global__ __launch_bounds__(BLOCK) void aggregate_kernel(AggArgs a) {
       F1 = a.yoff[c + 1];
       m0 = a.coff[c];
       m1 = a.coff[c + 1];
-      for (int s = lane; s < CAP; s += G) { tkey[s] = kEmpty; tval[s] = 0; }
-      if (lane == 0) s_cnt[grp] = 0;
+      for (int s = lane; s < CAP; s += G) { tkey[s] = kEmpty; tpos[s] = kNoPos; tval[s] = 0; }
+      for (int w = lane; w < BW; w += G) bm[w] = 0;
     }
     __syncthreads();
     if (act) {
@@ -1072,8 +1072,13 @@ __global__ __launch_bounds__(BLOCK) void aggregate_kernel(AggArgs a) {
         uint64_t j = member_of(a.mscan, m0, m1, f);
         uint32_t u = a.cedg[j];
         uint64_t i = a.g.off[u] + (f - a.mscan[j]);
-        tab_add(tkey, nullptr, tval, CAP, a.vcom[a.g.keys[i]], (double)W[i], 0);
+        tab_add(tkey, tpos, tval, CAP, a.vcom[a.g.keys[i]], (double)W[i], (uint32_t)(f - F0));
       }
+    }
+    __syncthreads();
+    if (act) {
+      for (int s = lane; s < CAP; s += G)
+        if (tkey[s] != kEmpty) atomicOr(&bm[tpos[s] / 32], 1u << (tpos[s] % 32));
     }
     __syncthreads();
     if (act) {
@@ -1081,15 +1086,21 @@ __global__ __launch_bounds__(BLOCK) void aggregate_kernel(AggArgs a) {
       for (int s = lane; s < CAP; s += G) {
         uint32_t k = tkey[s];
         if (k == kEmpty) continue;
-        uint32_t at = atomicAdd(&s_cnt[grp], 1u);
+        const uint32_t r = tpos[s];
+        uint32_t at = __popc(bm[r / 32] & ((1u << (r % 32)) - 1u));
+        for (uint32_t w = 0; w < r / 32; ++w) at += __popc(bm[w]);
         a.zkeys[F0 + at] = k;
         a.zw[F0 + at] = tval[s];
         sum += tval[s];
       }
       if (sum != 0) unsafeAtomicAdd(&a.ztot[c], sum);
+      if (lane == 0) {
+        uint32_t n = 0;
+        for (int w = 0; w < BW; ++w) n += __popc(bm[w]);
+        a.zdeg[c] = n;
+      }
     }
     __syncthreads();
-    if (act && lane == 0) a.zdeg[c] = s_cnt[grp];
   }
 }
 
@@ -1104,8 +1115,10 @@ __global__ __launch_bounds__(BLOCK) void aggregate_kernel(AggArgs a) {
 constexpr uint32_t kAggLdsTab = 2 * kAggHbmChunk;
 constexpr uint32_t kAggMembers = kAggHbmChunk;
 
-__device__ __forceinline__ void tab_add_wave(uint32_t* key, double* val, uint64_t cap, bool ok, uint32_t c,
-                                             double w) {
+// (lanes hold ascending positions p, so the first lane holding c0 has the
+// smallest position of the lanes that share it)
+__device__ __forceinline__ void tab_add_wave(uint32_t* key, uint32_t* pos, double* val, uint64_t cap, bool ok,
+                                             uint32_t c, double w, uint32_t p) {
   const unsigned long long act = __ballot(ok);
   if (!act) return;
   const int lane = threadIdx.x % 64;
@@ -1114,13 +1127,14 @@ __device__ __forceinline__ void tab_add_wave(uint32_t* key, double* val, uint64_
   const bool same = ok && c == c0;
   double sum = same ? w : 0.0;
   for (int m = 32; m > 0; m >>= 1) sum += __shfl_xor(sum, m, 64);
-  if (lane == lead) tab_add(key, nullptr, val, cap, c0, sum, 0);
-  else if (ok && !same) tab_add(key, nullptr, val, cap, c, w, 0);
+  if (lane == lead) tab_add(key, pos, val, cap, c0, sum, p);
+  else if (ok && !same) tab_add(key, pos, val, cap, c, w, p);
 }
 
 template <class WT>
 __global__ __launch_bounds__(kBlock) void aggregate_hbm_kernel(AggArgs a) {
   __shared__ uint32_t s_key[kAggLdsTab];
+  __shared__ uint32_t s_pos[kAggLdsTab];
   __shared__ double s_val[kAggLdsTab];
   __shared__ uint64_t s_ms[kAggMembers];   // flat edge start of each member of the chunk
   __shared__ uint64_t s_off[kAggMembers];  // its row offset
@@ -1129,6 +1143,7 @@ __global__ __launch_bounds__(kBlock) void aggregate_hbm_kernel(AggArgs a) {
   for (uint64_t q = blockIdx.x; q < a.nchunks; q += gridDim.x) {
     for (uint32_t s = threadIdx.x; s < kAggLdsTab; s += kBlock) {
       s_key[s] = kEmpty;
+      s_pos[s] = kNoPos;
       s_val[s] = 0;
     }
     if (threadIdx.x == 0) {
@@ -1180,36 +1195,70 @@ __global__ __launch_bounds__(kBlock) void aggregate_hbm_kernel(AggArgs a) {
         key = a.vcom[a.g.keys[i]];
         w = (double)W[i];
       }
-      tab_add_wave(s_key, s_val, kAggLdsTab, ok, key, w);
+      // (position relative to the community's first member edge)
+      tab_add_wave(s_key, s_pos, s_val, kAggLdsTab, ok, key, w, (uint32_t)(f - a.yoff[c]));
     }
     __syncthreads();
     const uint64_t cap = a.hoff[k + 1] - a.hoff[k];
     uint32_t* hkey = a.hkey + a.hoff[k];
+    uint32_t* hpos = a.hpos + a.hoff[k];
     double* hval = a.hval + a.hoff[k];
     for (uint32_t s = threadIdx.x; s < kAggLdsTab; s += kBlock) {
       const uint32_t ck = s_key[s];
-      if (ck != kEmpty) tab_merge(hkey, nullptr, hval, cap, ck, s_val[s], 0);
+      if (ck != kEmpty) tab_merge(hkey, hpos, hval, cap, ck, s_val[s], s_pos[s]);
     }
     __syncthreads();
   }
 }
 
-__global__ void agg_emit_hbm_kernel(AggArgs a, uint64_t total) {
+// Tier-4 rows in first-seen order: each community's member edges are covered
+// by a bitmap of 32 words per chunk (bm, from word a.choff[k] * 32); a table
+// slot marks the bit of its first position.  After an exclusive scan of the
+// words' popcounts (bsc), a slot's place in the row is the number of marked
+// bits of its community below its position.
+__global__ void agg_mark_hbm_kernel(AggArgs a, uint64_t total, uint32_t* bm) {
   for (uint64_t s = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; s < total; s += (uint64_t)gridDim.x * blockDim.x) {
-    uint32_t key = a.hkey[s];
+    if (a.hkey[s] == kEmpty) continue;
+    const uint64_t k = find_owner(a.hoff, a.n, s);
+    const uint32_t r = a.hpos[s];
+    atomicOr(&bm[a.choff[k] * 32 + r / 32], 1u << (r % 32));
+  }
+}
+
+__global__ void popc_words_kernel(const uint32_t* bm, uint64_t W, uint32_t* cnt) {
+  for (uint64_t w = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; w <= W; w += (uint64_t)gridDim.x * blockDim.x)
+    cnt[w] = w < W ? (uint32_t)__popc(bm[w]) : 0u;
+}
+
+__global__ void agg_emit_hbm_kernel(AggArgs a, uint64_t total, const uint32_t* bm, const uint32_t* bsc) {
+  for (uint64_t s = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; s < total; s += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t key = a.hkey[s];
     if (key == kEmpty) continue;
-    uint64_t lo = 0, hi = a.n - 1;
-    while (lo < hi) {
-      uint64_t mid = (lo + hi + 1) >> 1;
-      if (a.hoff[mid] <= s) lo = mid;
-      else hi = mid - 1;
-    }
-    uint32_t c = a.list[lo];
-    uint32_t at = atomicAdd(&a.zdeg[c], 1u);
-    uint64_t F0 = a.yoff[c];
+    const uint64_t k = find_owner(a.hoff, a.n, s);
+    const uint32_t c = a.list[k];
+    const uint32_t r = a.hpos[s];
+    const uint64_t w0 = a.choff[k] * 32, w = w0 + r / 32;
+    const uint32_t at = bsc[w] - bsc[w0] + (uint32_t)__popc(bm[w] & ((1u << (r % 32)) - 1u));
+    const uint64_t F0 = a.yoff[c];
     a.zkeys[F0 + at] = key;
     a.zw[F0 + at] = a.hval[s];
     unsafeAtomicAdd(&a.ztot[c], a.hval[s]);
+  }
+}
+
+__global__ void agg_deg_hbm_kernel(AggArgs a, const uint32_t* bsc) {
+  for (uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; k < a.n; k += (uint64_t)gridDim.x * blockDim.x)
+    a.zdeg[a.list[k]] = bsc[a.choff[k + 1] * 32] - bsc[a.choff[k] * 32];
+}
+
+// Sort keys of the members: community (C for a vertex that does not exist)
+// and id; a stable sort by community leaves each community's members in
+// ascending id order, as louvainCommunityVerticesOmpW lists them with one
+// thread (inc/louvain.hxx:800-812).
+__global__ void member_keys_kernel(GView g, const uint32_t* vcom, uint32_t C, uint32_t* key, uint32_t* id) {
+  for (uint64_t u = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; u < g.S; u += (uint64_t)gridDim.x * blockDim.x) {
+    key[u] = (!g.ex || g.ex[u]) ? vcom[u] : C;
+    id[u] = (uint32_t)u;
   }
 }
 
@@ -1633,10 +1682,6 @@ void k_lookup(uint32_t* a, uint64_t n, const uint32_t* map, hipStream_t st) {
 void k_count_members(GView g, const uint32_t* vcom, uint32_t* cnt, hipStream_t st) {
   if (g.S) LV_LAUNCH(count_members_kernel, grid_for(g.S, kBlock * 4), kBlock, st, g, vcom, cnt);
 }
-void k_scatter_members(GView g, const uint32_t* vcom, const uint32_t* coff, uint32_t* cdeg, uint32_t* cedg,
-                       hipStream_t st) {
-  if (g.S) LV_LAUNCH(scatter_members_kernel, grid_for(g.S, kBlock * 4), kBlock, st, g, vcom, coff, cdeg, cedg);
-}
 void k_member_degrees(const uint32_t* cedg, uint64_t n, GView g, uint64_t* mdeg, hipStream_t st) {
   LV_LAUNCH(member_degrees_kernel, grid_for(n + 1, kBlock * 4), kBlock, st, cedg, n, g, mdeg);
 }
@@ -1667,8 +1712,20 @@ void k_aggregate(const AggArgs& a, int tier, bool f64, hipStream_t st) {
   if (f64) agg_tier<double>(a, tier, st);
   else agg_tier<float>(a, tier, st);
 }
-void k_agg_emit_hbm(const AggArgs& a, uint64_t total, hipStream_t st) {
-  if (total) LV_LAUNCH(agg_emit_hbm_kernel, grid_for(total, kBlock, 1u << 16), kBlock, st, a, total);
+void k_agg_mark_hbm(const AggArgs& a, uint64_t total, uint32_t* bm, uint64_t words, uint32_t* cnt, hipStream_t st) {
+  if (total) LV_LAUNCH(agg_mark_hbm_kernel, grid_for(total, kBlock, 1u << 16), kBlock, st, a, total, bm);
+  LV_LAUNCH(popc_words_kernel, grid_for(words + 1, kBlock, 1u << 16), kBlock, st, bm, words, cnt);
+}
+void k_agg_emit_hbm(const AggArgs& a, uint64_t total, const uint32_t* bm, const uint32_t* bsc, hipStream_t st) {
+  if (total) LV_LAUNCH(agg_emit_hbm_kernel, grid_for(total, kBlock, 1u << 16), kBlock, st, a, total, bm, bsc);
+  if (a.n) LV_LAUNCH(agg_deg_hbm_kernel, grid_for(a.n, kBlock, 1u << 16), kBlock, st, a, bsc);
+}
+void k_member_keys(GView g, const uint32_t* vcom, uint32_t C, uint32_t* key, uint32_t* id, hipStream_t st) {
+  if (g.S) LV_LAUNCH(member_keys_kernel, grid_for(g.S, kBlock * 4), kBlock, st, g, vcom, C, key, id);
+}
+hipError_t sort_pairs_u32(const uint32_t* kin, uint32_t* kout, const uint32_t* vin, uint32_t* vout, uint64_t n,
+                          int end_bit, void* tmp, size_t& tmp_bytes, hipStream_t st) {
+  return hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, kin, kout, vin, vout, (int)n, 0, end_bit, st);
 }
 void k_t4_sizes(const uint64_t* yoff, const uint32_t* list, uint64_t n, uint64_t C, uint64_t chunk, uint64_t* caps,
                 uint64_t* chunks, hipStream_t st) {

@@ -94,6 +94,16 @@ def test_choose_community_bit_exact(ctx, name):
         assert np.array_equal(bc, f["sg_choose_c"]) and np.array_equal(be, f["sg_choose_e"])
 
 
+def _same_row_order(a, f, prefix):
+    """Each super-graph row lists its communities in the reference's order:
+    first seen over the members in ascending id order and their edges
+    (inc/louvain.hxx:800-812, :897-912 at one thread)."""
+    off, deg = f[prefix + "off"], f[prefix + "deg"]
+    for c in range(len(deg)):
+        o, d = int(off[c]), int(deg[c])
+        assert np.array_equal(a["keys"][o:o + d], f[prefix + "keys"][o:o + d]), c
+
+
 @pytest.mark.parametrize("name", FIXTURES)
 def test_renumber_and_aggregate(ctx, name):
     _say("case", name)
@@ -104,6 +114,7 @@ def test_renumber_and_aggregate(ctx, name):
     assert np.array_equal(vc, f["renum_vcom"]) and np.array_equal(cext, f["renum_cext"])
     a = ctx.aggregate(x, vc, CN)
     assert np.array_equal(a["off"], f["agg_off"]) and np.array_equal(a["deg"], f["agg_deg"])
+    _same_row_order(a, f, "agg_")
     got = F.rows_as_maps(a["off"], a["deg"], a["keys"], a["w"])
     want = F.rows_as_maps(f["agg_off"], f["agg_deg"], f["agg_keys"], f["agg_w"])
     if name in INTEGER_FIXTURES:
@@ -118,6 +129,7 @@ def test_renumber_and_aggregate(ctx, name):
     assert CN2 == int(f["sg_renum_CN"]) and np.array_equal(vc2, f["sg_renum_vcom"])
     a2 = ctx.aggregate(sg, vc2, CN2)
     assert np.array_equal(a2["off"], f["sg_agg_off"]) and np.array_equal(a2["deg"], f["sg_agg_deg"])
+    _same_row_order(a2, f, "sg_agg_")
     got = F.rows_as_maps(a2["off"], a2["deg"], a2["keys"], a2["w"])
     want = F.rows_as_maps(f["sg_agg_off"], f["sg_agg_deg"], f["sg_agg_keys"], f["sg_agg_w"])
     for g_, w_ in zip(got, want):
@@ -281,6 +293,45 @@ def test_out_of_range_batch_rejected(ctx):
     with pytest.raises(lvhip.LvError) as e:
         ctx.louvain("dynamic-frontier", y, d, bad, f["static_memb"], f["static_vtot"], f["static_ctot"])
     assert e.value.status == -1
+
+
+@pytest.mark.parametrize("name", INTEGER_FIXTURES)
+def test_single_stream_all_passes_bit_exact(ctx, name):
+    """One stream, every pass: super-graph rows in the reference's order
+    (members ascending, communities first seen), so each later pass sweeps
+    the same rows in the same order as the reference at one thread, and the
+    final membership, K, iterations and passes are equal bit for bit."""
+    import lvhip
+    _say("case", name)
+    f = F.load(name)
+    x, y, d, i = _g(f)
+    o = lvhip.LouvainOptions(*[t(v) for t, v in zip((int, float, float, float, float, int, int), f["opts"])])
+    from oracle import Options, Oracle
+    oo = Options(o.repeat, o.resolution, o.tolerance, o.aggregationTolerance, o.toleranceDrop, o.maxIterations,
+                 o.maxPasses)
+    orc = Oracle()
+    orc.set_threads(1)
+    ctx.set_schedule(1e-12, 1, 1, 16, 2)
+    ctx.set_hub_threshold(1 << 30, 4096)
+    ctx.set_path(0)
+    try:
+        for approach in ("static", "naive-dynamic", "dynamic-frontier"):
+            if approach == "static":
+                r = ctx.louvain(approach, x, o=o)
+                ref = orc.louvain(approach, x, options=oo)
+            else:
+                r = ctx.louvain(approach, y, d, i, f["static_memb"], f["static_vtot"], f["static_ctot"], o)
+                ref = orc.louvain(approach, y, d, i, f["static_memb"], f["static_vtot"], f["static_ctot"], oo)
+            _say(f"  {approach}: it={r.iterations}/{ref.iterations} passes={r.passes}/{ref.passes}")
+            assert np.array_equal(r.membership, ref.membership), approach
+            assert np.array_equal(r.vertexWeight, ref.vertex_weight), approach
+            live = np.unique(r.membership)
+            assert np.array_equal(r.communityWeight[live], ref.community_weight[live]), approach
+            assert (r.iterations, r.passes) == (ref.iterations, ref.passes), approach
+    finally:
+        ctx.set_schedule()
+        ctx.set_hub_threshold()
+        ctx.set_path(2)
 
 
 @pytest.mark.parametrize("path", [0, 1])
