@@ -9,8 +9,10 @@ HBM when the timed region starts; the state carries from batch to batch as
 in main.cxx:127-135.  value = directed edges of the updated graphs processed
 per second of wall clock over the K timed steps.  The same chain is then run
 on the plain GPU path (sweep over all vertices every iteration), checked
-against the oracle (|dQ| <= 1e-3 per batch) and timed on the host cores
-through the oracle (cpu_baseline).
+against the reference on the same inputs (|dQ| <= 1e-3 per batch), and the
+reference's own DF is timed on the host cores (cpu_baseline): its code
+compiled in place (oracle/_ref) where that was built, else the oracle
+restatement.
 
     python bench.py [--steps K] [--warmup W] [--scale 22] [--no-cpu] [--no-plain]
 """
@@ -22,6 +24,37 @@ import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "louvain-communities-openmp-dynamic_amd"))
+# CPUs this process may use, read before any OpenMP runtime loads.  The
+# CPU-baseline threads stay unbound: OMP_PROC_BIND would pin the process's
+# main thread, which also drives the GPU, to one core.
+HOST_CORES = len(os.sched_getaffinity(0))
+
+
+def cpu_checker():
+    """The CPU side of the bench: the reference's own code compiled in place
+    (oracle/_ref/libref.so, kind "reference") where it was built, else the
+    oracle restatement (kind "port").  Both time the reference's timed
+    region (inc/louvain.hxx:1167-1217) themselves."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    if os.path.exists(oracle.REF_SO):
+        return oracle.RefLib(), "reference"
+    return oracle.Oracle(), "port"
+
+
+def host_info():
+    """CPU model, sockets and CPUs of the host, and the CPUs this process may use."""
+    model, sockets = None, set()
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name") and model is None:
+                    model = ln.split(":", 1)[1].strip()
+                elif ln.startswith("physical id"):
+                    sockets.add(ln.split(":", 1)[1].strip())
+    except OSError:
+        pass
+    return {"cpu": model, "sockets": len(sockets) or None, "nproc": os.cpu_count(), "affinity": HOST_CORES}
 
 HBM_PEAK_GBPS = 8000.0
 KERNEL_NAME = "move_kernel<64, 64, 512, float, 2>"
@@ -221,10 +254,8 @@ def batched_main(args):
     cpu = None
     dq = over = None
     if not args.no_cpu:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        from oracle import Oracle
-        orc = Oracle()
-        cores = len(os.sched_getaffinity(0))
+        orc, kind = cpu_checker()
+        cores = HOST_CORES
         # The reference's own DF result moves with its thread count (on a
         # weak start, one thread may run several iterations and passes where
         # sixteen stop after one), so dQ is the distance from the GPU's
@@ -247,9 +278,10 @@ def batched_main(args):
         log(f"worst graph {worst[0]} (n {worst[1]}): Q {worst[2]:.5f}, reference 1 thread {worst[3]:.5f} / "
             f"{cores} threads {worst[4]:.5f}; passes {worst[5]} vs {worst[6]} / {worst[7]}; "
             f"{over} of {G} graphs beyond 1e-3 of the band")
-        cpu = {"value": E_tot / (ctime * 1e-3), "unit": "edges/s", "cores": cores, "kind": "port",
-               "sample": f"oracle DF on all {G} graphs, one after another, {ctime:.1f} ms in all"}
-        log(f"oracle: {ctime:.1f} ms for all graphs on {cores} threads; max distance of the GPU's Q from the "
+        cpu = {"value": E_tot / (ctime * 1e-3), "unit": "edges/s", "cores": cores, "kind": kind,
+               "sample": f"{kind} DF on all {G} graphs, one after another, {ctime:.1f} ms in all",
+               "host": host_info()}
+        log(f"{kind}: {ctime:.1f} ms for all graphs on {cores} threads; max distance of the GPU's Q from the "
             f"reference's 1/{cores}-thread band {dq:.3g}")
     out = {"metric": "batched DF: graph edges/s over 256 planted-partition graphs", "value": E_tot / wall,
            "unit": "edges/s", "higher_is_better": True, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -263,6 +295,92 @@ def batched_main(args):
     for d in (dy, db, ds):
         d.close()
     ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+def sweep_main(args):
+    """R-MAT scale 24 sweep (a=.57 b=.19 c=.19, edge factor 16, symmetrised,
+    unit weights, seed 24): Static on the graph, then for each batch fraction
+    1e-7 .. 1e-1 of |E| undirected (80% insertions / 20% deletions, each
+    applied to the original graph) ND, DS and DF from the reference's Static
+    state, on the GPU and with the reference on the host cores, on the same
+    inputs.  Per fraction and approach: time (the reference's timed region)
+    and modularity; GPU runs are timed on their second call (the first
+    allocates the context's workspace)."""
+    import numpy as np
+
+    import lvgraph
+    import lvhip
+
+    t = time.time()
+    g = lvgraph.rmat_native(args.sweep_scale, args.edge_factor, 0.57, 0.19, 0.19, args.sweep_seed)
+    x = g.view()
+    S, E = len(x["off"]) - 1, g.edges
+    log(f"R-MAT scale {args.sweep_scale}: {S} vertices, {E} directed edges ({time.time() - t:.1f}s)")
+    ctx = lvhip.Context(0)
+    dx = ctx.device_graph(x)
+    st = ctx.device_state(S)
+    ctx.run(lvhip.LV_STATIC, dx, None, st)
+    rs = ctx.run(lvhip.LV_STATIC, dx, None, st)
+    q_static = ctx.dmodularity(dx, st)
+    orc, kind = cpu_checker()
+    cores = HOST_CORES
+    orc.set_threads(cores)
+    r0 = orc.louvain("static", x)
+    q0 = orc.modularity(x, r0.membership)
+    log(f"Static: GPU {rs.time:.1f} ms Q {q_static:.6f}; {kind} ({cores} threads) {r0.time:.1f} ms Q {q0:.6f}")
+    static = {"gpu_ms": rs.time, "gpu_modularity": q_static, "cpu_ms": r0.time, "cpu_modularity": q0}
+    dx.close()
+    rows = []
+    und = E // 2
+    names = {"naive-dynamic": lvhip.LV_NAIVE_DYNAMIC, "delta-screening": lvhip.LV_DELTA_SCREENING,
+             "dynamic-frontier": lvhip.LV_DYNAMIC_FRONTIER}
+    gpu_df_ms, df_edges = 0.0, 0
+    for e in range(-7, 0):
+        frac = 10.0 ** e
+        b = max(1, int(round(frac * und)))
+        n_ins = int(round(0.8 * b))
+        t = time.time()
+        d, i = g.random_batch(b - n_ins, n_ins, args.sweep_seed * 100 + 7 + e)
+        td, ti = lvgraph.tidy_batch(x, d, i)
+        y = lvgraph.apply_batch(x, d, i)
+        Ey = len(y["keys"]) if "deg" not in y else int(np.sum(y["deg"]))
+        dy = ctx.device_graph(y)
+        db = ctx.device_batch(td, ti)
+        row = {"fraction": frac, "updates": b, "edges": Ey}
+        prep = time.time() - t
+        for name, a in names.items():
+            for _ in range(2):
+                st.upload(r0.membership, r0.vertex_weight, r0.community_weight)
+                r = ctx.run(a, dy, db, st)
+            q = ctx.dmodularity(dy, st)
+            ref = orc.louvain(name, y, td, ti, r0.membership, r0.vertex_weight, r0.community_weight)
+            qr = orc.modularity(y, ref.membership)
+            row[name] = {"gpu_ms": r.time, "cpu_ms": ref.time, "speedup": ref.time / r.time if r.time else None,
+                         "gpu_modularity": q, "cpu_modularity": qr, "dq": q - qr,
+                         "gpu_iterations": r.iterations, "gpu_passes": r.passes,
+                         "cpu_iterations": ref.iterations, "cpu_passes": ref.passes}
+            if name == "dynamic-frontier":
+                gpu_df_ms += r.time
+                df_edges += Ey
+        dy.close()
+        db.close()
+        rows.append(row)
+        log(f"1e{e}: {b} updates (prep {prep:.1f}s); " + "; ".join(
+            f"{n.split('-')[0][:5]} GPU {row[n]['gpu_ms']:.2f} ms / {kind} {row[n]['cpu_ms']:.1f} ms, "
+            f"dQ {row[n]['dq']:+.1e}" for n in names))
+    st.close()
+    ctx.close()
+    dq = {n: max(abs(r[n]["dq"]) for r in rows) for n in names}
+    out = {"metric": "R-MAT 24 sweep: ND / DS / DF time and dQ vs the reference per batch fraction",
+           "value": df_edges / (gpu_df_ms * 1e-3), "unit": "edges/s", "higher_is_better": True, "n_gpus": 1,
+           "dtype": "f64", "data": "synthetic", "dq_max": dq, "static": static, "fractions": rows,
+           "config": {"workload": f"R-MAT scale {args.sweep_scale} edge factor {args.edge_factor} (a=.57 b=.19 "
+                                  f"c=.19) symmetrised unit weights; batch fractions 1e-7..1e-1 of |E| undirected, "
+                                  f"80/20, each on the original graph; seed {args.sweep_seed}",
+                      "vertices": S, "edges": E},
+           "cpu_baseline": {"kind": kind, "cores": cores, "host": host_info(), "unit": "ms",
+                            "sample": "the reference on every fraction and approach, same inputs"}}
     print(json.dumps(out), flush=True)
 
 
@@ -296,6 +414,9 @@ def main():
     ap.add_argument("--batched", action="store_true", help="run the batched form (many graphs at once)")
     ap.add_argument("--batched-graphs", type=int, default=256)
     ap.add_argument("--batched-seed", type=int, default=256)
+    ap.add_argument("--sweep", action="store_true", help="run the R-MAT 24 batch-fraction sweep")
+    ap.add_argument("--sweep-scale", type=int, default=24)
+    ap.add_argument("--sweep-seed", type=int, default=24)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -305,6 +426,8 @@ def main():
         return serve_main(args)
     if args.batched:
         return batched_main(args)
+    if args.sweep:
+        return sweep_main(args)
     import numpy as np
 
     import lvgraph
@@ -348,16 +471,14 @@ def main():
         q_static = ctx.dmodularity(dgraphs[0], state0)
         log(f"GPU static: Q={q_static:.6f} iters={rs.iterations} passes={rs.passes} time={rs.time:.1f} ms "
             f"(wall {time.time() - t:.1f}s)")
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    from oracle import Oracle
-    orc = Oracle()
-    cores = len(os.sched_getaffinity(0))
+    orc, kind = cpu_checker()
+    cores = HOST_CORES
     orc.set_threads(cores)
     t = time.time()
     r0 = orc.louvain("static", graphs[0].view())
     state0.upload(r0.membership, r0.vertex_weight, r0.community_weight)
     q_static_ref = ctx.dmodularity(dgraphs[0], state0)
-    log(f"reference static (oracle, {cores} threads): Q={q_static_ref:.6f} iters={r0.iterations} "
+    log(f"reference static ({kind}, {cores} threads): Q={q_static_ref:.6f} iters={r0.iterations} "
         f"passes={r0.passes} time={r0.time:.1f} ms (wall {time.time() - t:.1f}s)")
 
     # Quality chain (untimed): the GPU state before every batch, for the oracle.
@@ -466,10 +587,11 @@ def main():
             cedges += graphs[k + 1].edges
         dq_max = max(abs(x) for x in dqs)
         log("dQ per batch (GPU - reference): " + " ".join(f"{x:+.2e}" for x in dqs))
-        cpu = {"value": cedges / (ctime * 1e-3), "unit": "edges/s", "cores": cores, "kind": "port",
-               "sample": f"oracle DF (C + OpenMP) on the {K} timed batches, same inputs and GPU carry-in "
-                         f"state, reference timed region; {ctime:.1f} ms total"}
-        log(f"oracle: max |dQ| = {dq_max:.3g}; cpu {cpu['value']:.4g} edges/s on {cores} threads")
+        what = "the reference's DF compiled in place" if kind == "reference" else "oracle DF (C + OpenMP)"
+        cpu = {"value": cedges / (ctime * 1e-3), "unit": "edges/s", "cores": cores, "kind": kind,
+               "sample": f"{what} on the {K} timed batches, same inputs and GPU carry-in state, reference "
+                         f"timed region; {ctime:.1f} ms total", "host": host_info()}
+        log(f"{kind}: max |dQ| = {dq_max:.3g}; cpu {cpu['value']:.4g} edges/s on {cores} threads")
 
     out = {
         "metric": METRIC, "value": value, "unit": "edges/s", "n_gpus": 1, "steps": K, "warmup": W,
