@@ -294,7 +294,9 @@ constexpr int kSubRounds = 4;      // worklist sub-rounds enqueued per host chec
 constexpr uint64_t kRefChunk = 2048;  // the reference's OpenMP chunk (inc/louvain.hxx:594)
 
 void launch_move(lv_context* c, const MoveArgs& a, int tier, bool f64) {
-  k_fill_u64(reinterpret_cast<uint64_t*>(a.work), 0, 1, c->st);
+  // the chunk counter of chunks handed out in order (the hub batch and
+  // spread chunks do not use it)
+  if (tier != kTierHubBatch && a.chunk) k_fill_u64(reinterpret_cast<uint64_t*>(a.work), 0, 1, c->st);
   hipEvent_t e0 = nullptr;
   if (c->profiling) e0 = c->record();
   if (tier == kTierHubBatch) k_hub_batch(a, c->st);
@@ -474,7 +476,6 @@ MoveArgs base_args(lv_context* c, const Level& lv, uint32_t* vcom, const double*
     a.hb_bpos = c->ws<uint32_t>("hb.bpos", nh);
     a.hb_ce = c->ws<double>("hb.ce", slots / kHubTabChunk + 1);
     a.hb_cp = c->ws<uint32_t>("hb.cp", slots / kHubTabChunk + 1);
-    a.hb_bsum = c->ws<uint64_t>("hb.bsum", 2 * kHubScanBlocks);
     // the batch's best pass empties every slot it used, so a slab is
     // filled once, when it is allocated
     bool fk = false, fp = false, fv = false;

@@ -25,10 +25,8 @@ constexpr uint64_t kHybridHbm = 8192;
 constexpr uint32_t kHubMin = 4096;
 constexpr uint64_t kAggHbmChunk = 1024;  // flat member edges per block chunk, tier-4 aggregation
 constexpr uint64_t kHubBlocks = 256;
-// hub batch: table slots per best-pass chunk (every hub table is a multiple),
-// blocks of the table-size scan
+// hub batch: table slots per best-pass chunk (every hub table is a multiple)
 constexpr uint64_t kHubTabChunk = 1024;
-constexpr int kHubScanBlocks = 256;
 // worklist queue compaction: blocks (contiguous id segments) per pass
 constexpr int kWlScanBlocks = 1024;
 constexpr int kTierHybrid = 5;
@@ -135,7 +133,6 @@ struct MoveArgs {
   uint32_t* hb_bpos;           // first position of the best community (kNoPos: no move)
   double* hb_ce;               // best gain per kHubTabChunk-slot table chunk
   uint32_t* hb_cp;             // its first position (kNoPos: none)
-  uint64_t* hb_bsum;           // 2 * kHubScanBlocks block totals of the size scan
   int f64_weights;             // hub batch: the graph's weights are f64
   uint32_t* hb_key;            // table slab
   uint32_t* hb_pos;

@@ -666,8 +666,6 @@ __global__ __launch_bounds__(BLOCK) void move_kernel(MoveArgs a) {
 constexpr uint64_t kHubChunk = 1024;          // edges per insert / flag chunk
 constexpr uint32_t kHubLdsTab = 2 * kHubChunk;  // LDS slots for one chunk's communities
 constexpr uint64_t kTabChunk = kHubTabChunk;  // table slots per best-pass chunk
-constexpr int kHubPrepBlocks = kHubScanBlocks;
-static_assert(kHubPrepBlocks == kBlock, "hub_scan_kernel scans one block total per thread");
 
 // Block-wide exclusive scan of one value per thread; `total` gets the sum.
 template <int BLOCK>
@@ -712,71 +710,67 @@ __device__ __forceinline__ void tab_merge(uint32_t* key, uint32_t* pos, double* 
 
 __device__ __forceinline__ uint64_t list_len(const MoveArgs& a) { return a.n_dev ? (uint64_t)*a.n_dev : a.n; }
 
-// Per listed vertex: active or not, table slots (nextpow2(2 deg), at least
-// kTabChunk) and edge chunks.  Each block scans a contiguous segment of the
-// list; its totals go to bsum[block] and bsum[kBlock + block].
-__global__ __launch_bounds__(kBlock) void hub_sizes_kernel(MoveArgs a, uint64_t* bsum) {
+// One block: per listed vertex, active or not, its table slots
+// (nextpow2(2 deg), at least kTabChunk) and edge chunks, scanned into
+// hb_cap / hb_chunks (n + 1 entries, batch totals in hb_tot); an active hub's
+// community, that community's Sigma_tot and a zero own-community weight
+// (summed by the insert pass); every hub starts as "did not move".  The list
+// is walked in tiles of kPrepBlock x kPrepItems entries, each thread taking
+// kPrepItems consecutive ones.
+constexpr int kPrepBlock = 1024, kPrepItems = 8;
+
+__global__ __launch_bounds__(kPrepBlock) void hub_prep_kernel(MoveArgs a) {
   const uint64_t n = list_len(a);
-  const uint64_t seg = (n + gridDim.x - 1) / gridDim.x;
-  const uint64_t lo = blockIdx.x * seg, hi = min(n, lo + seg);
+  constexpr uint64_t tile = (uint64_t)kPrepBlock * kPrepItems;
   uint64_t run_cap = 0, run_chk = 0;
-  for (uint64_t b = lo; b < hi; b += kBlock) {  // block-uniform trip count
-    uint64_t k = b + threadIdx.x, cap = 0, chk = 0;
-    if (k < hi) {
-      uint32_t u = a.list[k];
-      uint64_t d = a.g.deg[u];
-      bool act = d > a.hub_min && (!a.g.ex || a.g.ex[u]) && !vertex_skipped(a, u) &&
-                 (a.choose_only || (ld_dev(&a.vaff[u]) && (!a.fa || a.fa[u])));
+  for (uint64_t b = 0; b < n; b += tile) {  // block-uniform trip count
+    const uint64_t k0 = b + (uint64_t)threadIdx.x * kPrepItems;
+    uint64_t cap[kPrepItems], chk[kPrepItems], tc = 0, tk = 0;
+#pragma unroll
+    for (int j = 0; j < kPrepItems; ++j) {
+      const uint64_t k = k0 + j;
+      cap[j] = chk[j] = 0;
+      if (k >= n) continue;
+      const uint32_t u = a.list[k];
+      const uint64_t d = a.g.deg[u];
+      const bool act = d > a.hub_min && (!a.g.ex || a.g.ex[u]) && !vertex_skipped(a, u) &&
+                       (a.choose_only || (ld_dev(&a.vaff[u]) && (!a.fa || a.fa[u])));
+      a.hb_bpos[k] = kNoPos;
       if (act) {
-        cap = kTabChunk;
-        while (cap < 2 * d) cap <<= 1;
-        chk = (d + kHubChunk - 1) / kHubChunk;
+        uint64_t c = kTabChunk;
+        while (c < 2 * d) c <<= 1;
+        cap[j] = c;
+        chk[j] = (d + kHubChunk - 1) / kHubChunk;
         if (a.stamp) st_dev(&a.stamp[u], a.serial);
+        const uint32_t cu = ld_dev(&a.vcom[u]);
+        a.hb_cu[k] = cu;
+        a.hb_sd[k] = ld_dev(&a.ctot[cu]);
+        a.hb_vd[k] = 0;
       }
+      tc += cap[j];
+      tk += chk[j];
     }
-    uint64_t tc, tk;
-    uint64_t ec = block_scan_excl<kBlock>(cap, tc);
-    uint64_t ek = block_scan_excl<kBlock>(chk, tk);
-    if (k < hi) {
-      a.hb_cap[k] = run_cap + ec;
-      a.hb_chunks[k] = run_chk + ek;
+    uint64_t totc, totk;
+    uint64_t pc = run_cap + block_scan_excl<kPrepBlock>(tc, totc);
+    uint64_t pk = run_chk + block_scan_excl<kPrepBlock>(tk, totk);
+#pragma unroll
+    for (int j = 0; j < kPrepItems; ++j) {
+      const uint64_t k = k0 + j;
+      if (k < n) {
+        a.hb_cap[k] = pc;
+        a.hb_chunks[k] = pk;
+      }
+      pc += cap[j];
+      pk += chk[j];
     }
-    run_cap += tc;
-    run_chk += tk;
+    run_cap += totc;
+    run_chk += totk;
   }
   if (threadIdx.x == 0) {
-    bsum[blockIdx.x] = run_cap;
-    bsum[kBlock + blockIdx.x] = run_chk;
-  }
-}
-
-// One block: exclusive scan of the block totals, batch totals.
-__global__ __launch_bounds__(kBlock) void hub_scan_kernel(MoveArgs a, uint64_t* bsum) {
-  const uint64_t n = list_len(a);
-  uint64_t tc, tk;
-  uint64_t c = bsum[threadIdx.x], k = bsum[kBlock + threadIdx.x];
-  uint64_t ec = block_scan_excl<kBlock>(c, tc);
-  uint64_t ek = block_scan_excl<kBlock>(k, tk);
-  bsum[threadIdx.x] = ec;
-  bsum[kBlock + threadIdx.x] = ek;
-  if (threadIdx.x == 0) {
-    a.hb_tot[0] = tc;
-    a.hb_tot[1] = tk;
-    a.hb_cap[n] = tc;
-    a.hb_chunks[n] = tk;
-  }
-}
-
-// Segment offsets added: hb_cap / hb_chunks become global exclusive scans.
-__global__ __launch_bounds__(kBlock) void hub_offsets_kernel(MoveArgs a, const uint64_t* bsum) {
-  const uint64_t n = list_len(a);
-  const uint64_t seg = (n + gridDim.x - 1) / gridDim.x;
-  const uint64_t lo = blockIdx.x * seg, hi = min(n, lo + seg);
-  const uint64_t oc = bsum[blockIdx.x], ok = bsum[kBlock + blockIdx.x];
-  if (!oc && !ok) return;
-  for (uint64_t k = lo + threadIdx.x; k < hi; k += kBlock) {
-    a.hb_cap[k] += oc;
-    a.hb_chunks[k] += ok;
+    a.hb_tot[0] = run_cap;
+    a.hb_tot[1] = run_chk;
+    a.hb_cap[n] = run_cap;
+    a.hb_chunks[n] = run_chk;
   }
 }
 
@@ -815,26 +809,14 @@ __global__ __launch_bounds__(kBlock) void hub_insert_kernel(MoveArgs a) {
     }
     __syncthreads();
     uint64_t base = a.hb_cap[k], cap = a.hb_cap[k + 1] - base;
+    const uint32_t cu = a.hb_cu[k];
     for (uint32_t s = threadIdx.x; s < kHubLdsTab; s += kBlock) {
       uint32_t c = s_key[s];
-      if (c != kEmpty) tab_merge(a.hb_key + base, a.hb_pos + base, a.hb_val + base, cap, c, s_val[s], s_pos[s]);
+      if (c == kEmpty) continue;
+      tab_merge(a.hb_key + base, a.hb_pos + base, a.hb_val + base, cap, c, s_val[s], s_pos[s]);
+      if (c == cu) unsafeAtomicAdd(&a.hb_vd[k], s_val[s]);  // weight to the hub's own community
     }
     __syncthreads();
-  }
-}
-
-// Per hub: own community, its weight in the table, Sigma_tot.
-__global__ void hub_self_kernel(MoveArgs a) {
-  const uint64_t n = list_len(a);
-  for (uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; k < n; k += (uint64_t)gridDim.x * blockDim.x) {
-    uint64_t base = a.hb_cap[k], cap = a.hb_cap[k + 1] - base;
-    a.hb_bpos[k] = kNoPos;
-    if (!cap) continue;
-    uint32_t u = a.list[k];
-    uint32_t cu = ld_dev(&a.vcom[u]);
-    a.hb_cu[k] = cu;
-    a.hb_vd[k] = tab_get(a.hb_key + base, a.hb_val + base, cap, cu);
-    a.hb_sd[k] = ld_dev(&a.ctot[cu]);
   }
 }
 
@@ -1663,12 +1645,9 @@ void k_move(const MoveArgs& a, int tier, bool f64, hipStream_t st) {
 }
 void k_hub_batch(const MoveArgs& a, hipStream_t st) {
   const uint32_t grid = 2048;
-  LV_LAUNCH(hub_sizes_kernel, kHubPrepBlocks, kBlock, st, a, a.hb_bsum);
-  LV_LAUNCH(hub_scan_kernel, 1, kBlock, st, a, a.hb_bsum);
-  LV_LAUNCH(hub_offsets_kernel, kHubPrepBlocks, kBlock, st, a, a.hb_bsum);
+  LV_LAUNCH(hub_prep_kernel, 1, kPrepBlock, st, a);
   if (a.g.w && a.f64_weights) LV_LAUNCH(hub_insert_kernel<double>, grid, kBlock, st, a);
   else LV_LAUNCH(hub_insert_kernel<float>, grid, kBlock, st, a);
-  LV_LAUNCH(hub_self_kernel, 64, kBlock, st, a);
   LV_LAUNCH(hub_best_kernel, grid, kBlock, st, a);
   LV_LAUNCH(hub_apply_kernel, 64, kBlock, st, a);
   if (!a.choose_only) LV_LAUNCH(hub_flag_kernel, grid, kBlock, st, a);
