@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <map>
@@ -62,8 +63,8 @@ struct lv_context {
   uint32_t hub_min = kHubMin;         // sweeps: degrees above run in the hub batch
   uint32_t hub_min_wl = 256;          // worklists: degrees above run in the hub batch
   // worklist hub-table slots of the last graph seen (its degree array, span, threshold)
-  const uint32_t* wl_key_deg = nullptr;
-  uint64_t wl_key_S = 0, wl_slots = 0;
+  uint64_t wl_key_stamp = 0;  // graph stamp wl_slots was computed for (0: none)
+  uint64_t wl_slots = 0;
   uint32_t wl_key_thr = 0;
   int tier_order = 2;           // 2: ascending ids (hubs first), 1: degree tiers highest first, 0: lowest first
   uint32_t serial = 0;          // worklist iteration serial (visit stamps)
@@ -164,6 +165,12 @@ struct lv_dgraph {
   TierLists hubs;   // tier 4 = degree > kHubMin (id-order mode), ascending
   uint64_t hub_slots = 0;
   uint64_t version = 0;  // counts the updates applied to the graph
+  // process-unique stamp of the current degrees, new at upload and at every
+  // update: contexts key what they derive from the degrees on it
+  uint64_t stamp = 0;
+  // worklist hub-batch table slots: sum of nextpow2(2 deg) over degrees
+  // above kMidSlots / 2 (the hubs that may run in the multi-block batch)
+  uint64_t slab_slots = 0;
   // Batched form: 2M and existing vertices of each graph range voff, as of
   // `version` (a property of the resident graph, kept between runs)
   struct Groups {
@@ -272,6 +279,14 @@ uint64_t hub_slot_total(lv_context* c, const TierLists& t, const uint32_t* deg) 
   return tot;
 }
 
+// Sum of nextpow2(2 deg) over the degrees above thr (hub-batch table slots).
+uint64_t caps_total(lv_context* c, const uint32_t* deg, uint64_t S, uint32_t thr) {
+  unsigned long long* tot = c->ws<unsigned long long>("hb.need", 1);
+  k_fill_u64(reinterpret_cast<uint64_t*>(tot), 0, 1, c->st);
+  k_caps_sum(deg, S, thr, tot, c->st);
+  return c->read1(tot);
+}
+
 // ------------------------------------------------------------ local-moving
 struct Level {
   GView g;
@@ -281,6 +296,8 @@ struct Level {
   uint32_t maxdeg4;        // max degree in tier 4 of `tiers`
   const TierLists* hubs;   // degree > kHubMin, ascending (id order)
   uint64_t hub_slots;      // hub-batch table slots for all of them
+  uint64_t stamp = 0;      // resident graph's degree stamp (0: a super-graph)
+  uint64_t slab_slots = 0; // resident graph: hub-batch slots above kMidSlots / 2 edges
 };
 
 struct Counters {
@@ -290,6 +307,7 @@ struct Counters {
 // Launch one tier of the local-moving kernel, bracketed by events when the
 // context profiles (per-launch durations on the context's own stream).
 constexpr int kTierHubBatch = 6;  // launch_move: hubs through k_hub_batch (stats slot 4)
+constexpr int kTierMidHub = 7;    // launch_move: worklist hubs through k_hub_mid (stats slot 3)
 constexpr int kSubRounds = 4;      // worklist sub-rounds enqueued per host check
 constexpr uint64_t kRefChunk = 2048;  // the reference's OpenMP chunk (inc/louvain.hxx:594)
 
@@ -300,12 +318,23 @@ void launch_move(lv_context* c, const MoveArgs& a, int tier, bool f64) {
   hipEvent_t e0 = nullptr;
   if (c->profiling) e0 = c->record();
   if (tier == kTierHubBatch) k_hub_batch(a, c->st);
+  else if (tier == kTierMidHub) k_hub_mid(a, c->st);
   else {
     if (tier == kTierHybrid && a.ctab) k_graph_chunks(a, c->st);
     k_move(a, tier, f64, c->st);
   }
-  if (c->profiling) c->prof.push_back({tier == kTierHubBatch ? 4 : (tier == kTierHybrid ? 0 : tier), {e0, c->record()}});
+  const int slot = tier == kTierHubBatch ? 4 : (tier == kTierMidHub ? 3 : (tier == kTierHybrid ? 0 : tier));
+  if (c->profiling) c->prof.push_back({slot, {e0, c->record()}});
 }
+
+// Worklist queues: degrees above hub_min_wl are hubs; up to this one they
+// run a block each (LDS table), above it in the multi-block hub batch.
+uint32_t wl_big_hub(const lv_context* c) { return std::max<uint32_t>(c->hub_min_wl, kMidHubMax); }
+
+// Hubs that may reach the multi-block batch: above wl_big_hub, or marked
+// there after their capped LDS table (hubs above kMidSlots / 2 edges)
+// overflowed.  The batch's table slab covers all of them.
+uint32_t wl_slab_hub(const lv_context* c) { return std::max<uint32_t>(c->hub_min_wl, kMidSlots / 2); }
 
 // Streams of a level: stream_fraction of its live vertices, clamped.  A
 // level of at most one reference chunk (2048 ids, the schedule(dynamic,
@@ -429,8 +458,8 @@ MoveArgs base_args(lv_context* c, const Level& lv, uint32_t* vcom, const double*
   a.fa = fa;
   a.M = M;
   a.R = R;
-  a.el = c->ws<double>("el", 1);
-  a.cnt = c->ws<unsigned long long>("cnt", (size_t)kTiers * CNT_N);
+  a.el = c->ws<double>("el", (size_t)kSpread * kElStride);
+  a.cnt = c->ws<unsigned long long>("cnt", (size_t)kSpread * kCntStride);
   a.min_chunk = 0;
   a.max_streams = level_streams(c, lv.order);
   a.chunk = c->chunk;
@@ -449,17 +478,17 @@ MoveArgs base_args(lv_context* c, const Level& lv, uint32_t* vcom, const double*
   uint64_t nh = lv.hubs ? lv.hubs->start[5] - lv.hubs->start[4] : 0;
   uint64_t slots = lv.hub_slots;
   if (worklist && c->tier_order == 2) {
-    // a queue may hold any vertex above the worklist threshold: scratch for
-    // all of them, table slots for the sum over them
+    // the big-hub queue may hold any vertex above its threshold: scratch
+    // for all of them, table slots for the sum over them
     nh = lv.g.S;
-    const uint32_t thr = c->hub_min_wl;
-    if (c->wl_key_deg != lv.g.deg || c->wl_key_S != lv.g.S || c->wl_key_thr != thr) {
-      unsigned long long* tot = c->ws<unsigned long long>("hb.need", 1);
-      k_fill_u64(reinterpret_cast<uint64_t*>(tot), 0, 1, c->st);
-      k_caps_sum(lv.g.deg, lv.g.S, thr, tot, c->st);
-      c->wl_slots = c->read1(tot);
-      c->wl_key_deg = lv.g.deg;
-      c->wl_key_S = lv.g.S;
+    const uint32_t thr = wl_slab_hub(c);
+    if (thr == kMidSlots / 2 && lv.stamp) {
+      c->wl_slots = lv.slab_slots;  // the resident graph's, for the default threshold
+      c->wl_key_stamp = lv.stamp;
+      c->wl_key_thr = thr;
+    } else if (!lv.stamp || c->wl_key_stamp != lv.stamp || c->wl_key_thr != thr) {
+      c->wl_slots = caps_total(c, lv.g.deg, lv.g.S, thr);
+      c->wl_key_stamp = lv.stamp;
       c->wl_key_thr = thr;
     }
     slots = c->wl_slots;
@@ -474,6 +503,20 @@ MoveArgs base_args(lv_context* c, const Level& lv, uint32_t* vcom, const double*
     a.hb_vd = c->ws<double>("hb.vd", nh);
     a.hb_sd = c->ws<double>("hb.sd", nh);
     a.hb_bpos = c->ws<uint32_t>("hb.bpos", nh);
+    if (worklist) {
+      // capped tables: overflow flags per queue entry, and the vertices that
+      // need a full table (zeroed once; a stale mark only costs a full table)
+      a.hb_over = c->ws<uint8_t>("hb.over", nh);
+      bool fb = false;
+      a.hb_big = c->ws<uint8_t>("hb.big", lv.g.S, &fb);
+      if (fb) k_fill_u8(a.hb_big, 0, c->ws_bytes("hb.big"), c->st);
+    }
+    a.hb_off = c->ws<uint64_t>("hb.off", nh);
+    a.hb_deg = c->ws<uint32_t>("hb.deg", nh);
+    // edge chunks: every table has >= 2 deg slots, so the chunks number at
+    // most slots / (2 kHubChunk) + one partial chunk per hub
+    a.hb_cown = c->ws<uint32_t>("hb.cown", slots / (2 * kHubChunkEdges) + nh + 1);
+    a.hb_town = c->ws<uint32_t>("hb.town", slots / kHubTabChunk + 1);
     a.hb_ce = c->ws<double>("hb.ce", slots / kHubTabChunk + 1);
     a.hb_cp = c->ws<uint32_t>("hb.cp", slots / kHubTabChunk + 1);
     // the batch's best pass empties every slot it used, so a slab is
@@ -528,11 +571,25 @@ void move_sweep(lv_context* c, MoveArgs a, const Level& lv) {
   }
 }
 
+// Counters of the kSpread slots of dcnt, added into cnt.
 void add_counters(lv_context* c, const unsigned long long* dcnt, Counters& cnt) {
-  unsigned long long h[kTiers * CNT_N];
-  c->read(h, dcnt, (size_t)kTiers * CNT_N);
-  for (int t = 0; t < kTiers; ++t)
-    for (int i = 0; i < CNT_N; ++i) cnt.v[t][i] += h[t * CNT_N + i];
+  std::vector<unsigned long long> h((size_t)kSpread * kCntStride);
+  c->read(h.data(), dcnt, h.size());
+  for (int s = 0; s < kSpread; ++s)
+    for (int t = 0; t < kTiers; ++t)
+      for (int i = 0; i < CNT_N; ++i) cnt.v[t][i] += h[(size_t)s * kCntStride + t * CNT_N + i];
+}
+
+// Zero the spread gain slots of el (kSpread x kElStride doubles).
+void clear_el(lv_context* c, double* el) { k_fill_f64(el, 0, (size_t)kSpread * kElStride, c->st); }
+
+// The gain of a launch sequence: its spread slots added in slot order.
+double read_el(lv_context* c, const double* el) {
+  double h[kSpread * kElStride];
+  c->read(h, el, (size_t)kSpread * kElStride);
+  double t = 0;
+  for (int s = 0; s < kSpread; ++s) t += h[(size_t)s * kElStride];
+  return t;
 }
 
 // louvainMoveOmpW (inc/louvain.hxx:587-609): iterate until the summed gain
@@ -551,14 +608,14 @@ int local_move(lv_context* c, const Level& lv, uint32_t* vcom, const double* vto
   a.sparse = c->path == 1;
   if (ascending && !a.chunk) a.chunk = a.sparse ? 64 : 16;
   batched_schedule(c, a, lv, bt);
-  k_fill_u64(reinterpret_cast<uint64_t*>(a.cnt), 0, (size_t)kTiers * CNT_N, c->st);
+  k_fill_u64(reinterpret_cast<uint64_t*>(a.cnt), 0, (size_t)kSpread * kCntStride, c->st);
   unsigned long long* cnt0 = a.cnt;
   int l = 0;
   double el = 0;
   if (bt) batched_begin(c, bt);
   for (; l < L;) {
     if (bt) k_fill_f64(bt->gel, 0, bt->G, c->st);
-    else k_fill_f64(a.el, 0, 1, c->st);
+    else clear_el(c, a.el);
     a.pick_less = l < c->pick_less_iters;
     move_sweep(c, a, lv);
     c->check();
@@ -567,7 +624,7 @@ int local_move(lv_context* c, const Level& lv, uint32_t* vcom, const double* vto
       if (!batched_step(c, bt, L)) break;
       continue;
     }
-    el = c->read1(a.el);
+    el = read_el(c, a.el);
     if (el <= E) break;
   }
   add_counters(c, cnt0, cnt);
@@ -591,13 +648,14 @@ int local_move_worklist(lv_context* c, const Level& lv, uint32_t* vcom, const do
   batched_schedule(c, a, lv, bt);
   if (bt) batched_begin(c, bt);
   a.sparse = 1;  // queue entries read 64 at a time
-  k_fill_u64(reinterpret_cast<uint64_t*>(a.cnt), 0, (size_t)kTiers * CNT_N, c->st);
+  k_fill_u64(reinterpret_cast<uint64_t*>(a.cnt), 0, (size_t)kSpread * kCntStride, c->st);
   unsigned long long* cnt0 = a.cnt;
   const uint64_t S = lv.g.S, stride = std::max<uint64_t>(S, 1), fbytes = (S + 7) & ~3ull;
   uint32_t* ql = c->ws<uint32_t>("wl.low", stride);
+  uint32_t* qm = c->ws<uint32_t>("wl.mid", stride);
   uint32_t* qh = c->ws<uint32_t>("wl.hub", stride);
-  uint32_t* lens = c->ws<uint32_t>("wl.len", 2);
-  uint64_t* bsum = c->ws<uint64_t>("wl.bsum", 2 * kWlScanBlocks);
+  uint32_t* lens = c->ws<uint32_t>("wl.len", 3);
+  uint64_t* bsum = c->ws<uint64_t>("wl.bsum", 3 * kWlScanBlocks);
   // next-sub-round flags: all zero between calls (each compaction resets
   // the flags it reads, and a sub-round loop ends on an empty compaction)
   bool fresh = false;
@@ -616,6 +674,8 @@ int local_move_worklist(lv_context* c, const Level& lv, uint32_t* vcom, const do
     c->serial = 0;
   }
   const uint32_t thr = c->tier_order == 2 ? c->hub_min_wl : 0;
+  const uint32_t thr2 = wl_big_hub(c);
+  const uint8_t* big = c->tier_order == 2 ? a.hb_big : nullptr;  // hubs marked for the batch
   int l = 0;
   double el = 0;
   for (; l < L;) {
@@ -626,19 +686,25 @@ int local_move_worklist(lv_context* c, const Level& lv, uint32_t* vcom, const do
     a.serial = c->serial;
     a.pick_less = l < c->pick_less_iters;
     if (bt) k_fill_f64(bt->gel, 0, bt->G, c->st);
-    else k_fill_f64(a.el, 0, 1, c->st);
+    else clear_el(c, a.el);
     // the iteration's queue: every flagged id, ascending
-    k_wl_queue(vaff, false, S, lv.g.deg, thr, ql, qh, lens, bsum, c->st);
+    k_wl_queue(vaff, false, S, lv.g.deg, thr, thr2, big, ql, qm, qh, lens, bsum, c->st);
     // Sub-rounds are enqueued kSubRounds at a time (their kernels read the
     // queue lengths on the device; an empty round costs its launches only),
     // with one host check of the last compaction per group.
     for (int round = 0;; ++round) {
-      if (c->tier_order == 2) {  // id-order mode: hubs, then the rest through the id-order kernel
+      if (c->tier_order == 2) {
+        // id-order mode: the largest hubs (multi-block batch), the other
+        // hubs (a block each), then the rest through the id-order kernel
         if (S) {
           a.list = qh;
-          a.n_dev = lens + 1;
+          a.n_dev = lens + 2;
           a.cnt = cnt0 + (size_t)4 * CNT_N;
           launch_move(c, a, kTierHubBatch, lv.f64);
+          a.list = qm;
+          a.n_dev = lens + 1;
+          a.cnt = cnt0 + (size_t)3 * CNT_N;
+          launch_move(c, a, kTierMidHub, lv.f64);
         }
         a.list = ql;
         a.n_dev = lens;
@@ -655,19 +721,19 @@ int local_move_worklist(lv_context* c, const Level& lv, uint32_t* vcom, const do
           launch_move(c, a, k, lv.f64);
         }
       }
-      k_wl_queue(a.fsame, true, S, lv.g.deg, thr, ql, qh, lens, bsum, c->st);
+      k_wl_queue(a.fsame, true, S, lv.g.deg, thr, thr2, big, ql, qm, qh, lens, bsum, c->st);
       if ((round + 1) % kSubRounds) continue;
       c->check();
-      uint32_t n2[2];
-      c->read(n2, lens, 2);
-      if (n2[0] + n2[1] == 0) break;
+      uint32_t n3[3];
+      c->read(n3, lens, 3);
+      if (n3[0] + n3[1] + n3[2] == 0) break;
     }
     ++l;
     if (bt) {
       if (!batched_step(c, bt, L)) break;
       continue;
     }
-    el = c->read1(a.el);
+    el = read_el(c, a.el);
     if (el <= E) break;
   }
   add_counters(c, cnt0, cnt);
@@ -880,7 +946,8 @@ void invoke(lv_context* c, int approach, const lv_dgraph* x, const lv_dbatch* b,
   hipEvent_t e2 = c->record();
   hipEvent_t t1 = e2;
   bool t1_set = false;
-  Level lv0{x->view(), x->f64, &x->tiers, x->order, x->tiers.maxdeg, &x->hubs, x->hub_slots};
+  Level lv0{x->view(), x->f64, &x->tiers, x->order, x->tiers.maxdeg, &x->hubs, x->hub_slots, x->stamp,
+            x->slab_slots};
   Level cur = lv0;
   TierLists ytiers, yhubs;
   uint64_t ySpan = 0;
@@ -1115,7 +1182,8 @@ void invoke_batched(lv_context* c, int approach, const lv_dgraph* x, const lv_db
   hipEvent_t e2 = c->record();
   hipEvent_t t1 = e2;
   bool t1_set = false;
-  Level cur{x->view(), x->f64, &x->tiers, x->order, x->tiers.maxdeg, &x->hubs, x->hub_slots};
+  Level cur{x->view(), x->f64, &x->tiers, x->order, x->tiers.maxdeg, &x->hubs, x->hub_slots, x->stamp,
+            x->slab_slots};
   TierLists ytiers, yhubs;
   uint32_t* gidcur = gid0;
   uint64_t levelS = S;
@@ -1349,12 +1417,19 @@ void graph_check(const lv_graph* g) {
   require(within, LV_ERR_INVALID, "graph degree exceeds its row's capacity");
 }
 
+// A stamp no graph of this process has had (never 0).
+uint64_t new_graph_stamp() {
+  static std::atomic<uint64_t> last{0};
+  return ++last;
+}
+
 lv_dgraph* upload_graph(lv_context* c, const lv_graph* g) {
   graph_check(g);
   HIPCHK(hipSetDevice(c->device));
   auto* d = new lv_dgraph();
   try {
     d->ctx = c;
+    d->stamp = new_graph_stamp();
     d->S = g->span;
     d->E = g->offsets[g->span];
     d->f64 = g->weights_f64 != nullptr;
@@ -1404,6 +1479,7 @@ lv_dgraph* upload_graph(lv_context* c, const lv_graph* g) {
     build_tiers(c, "g.hubs", d->off, d->deg, d->ex, d->S, d->hubs, d->hubs.ids, kHubMin);
     d->hubs.maxdeg = max_degree_tier4(c, d->hubs, d->deg);
     d->hub_slots = hub_slot_total(c, d->hubs, d->deg);
+    d->slab_slots = caps_total(c, d->deg, d->S, kMidSlots / 2);
     c->check();
   } catch (...) {
     lv_dgraph_destroy(d);
@@ -1528,10 +1604,9 @@ void apply_update(lv_context* c, lv_dgraph* d, const lv_dbatch* b) {
   k_upd_delta(b->iu, b->ni, +1, delta, touched, c->st);
   // touched rows, ascending
   uint32_t* T = c->ws<uint32_t>("up.T", S);
-  uint32_t* Thub = c->ws<uint32_t>("up.Thub", 1);  // no hub split (threshold 0)
   uint32_t* lens = c->ws<uint32_t>("up.lens", 3);
-  uint64_t* bsum = c->ws<uint64_t>("up.bsum", 2 * kWlScanBlocks);
-  k_wl_queue(touched, false, S, d->deg, 0, T, Thub, lens, bsum, c->st);
+  uint64_t* bsum = c->ws<uint64_t>("up.bsum", 3 * kWlScanBlocks);
+  k_wl_queue(touched, false, S, d->deg, 0, 0, nullptr, T, nullptr, nullptr, lens, bsum, c->st);  // no degree split
   const uint64_t R = c->read1(lens);
   uint64_t* so = c->ws<uint64_t>("up.so", R + 1);
   uint64_t* sn = c->ws<uint64_t>("up.sn", R + 1);
@@ -1612,7 +1687,8 @@ void apply_update(lv_context* c, lv_dgraph* d, const lv_dbatch* b) {
   build_tiers(c, "g.hubs", d->off, d->deg, d->ex, S, d->hubs, d->hubs.ids, kHubMin);
   d->hubs.maxdeg = max_degree_tier4(c, d->hubs, d->deg);
   d->hub_slots = hub_slot_total(c, d->hubs, d->deg);
-  c->wl_key_deg = nullptr;  // worklist hub-table sizes depend on the degrees
+  d->slab_slots = caps_total(c, d->deg, d->S, kMidSlots / 2);
+  d->stamp = new_graph_stamp();  // worklist hub-table sizes depend on the degrees
   c->check();
 }
 

@@ -27,6 +27,22 @@ constexpr uint64_t kAggHbmChunk = 1024;  // flat member edges per block chunk, t
 constexpr uint64_t kHubBlocks = 256;
 // hub batch: table slots per best-pass chunk (every hub table is a multiple)
 constexpr uint64_t kHubTabChunk = 1024;
+// hub batch: edges per insert / flag chunk
+constexpr uint64_t kHubChunkEdges = 1024;
+// worklist hub batches: table slots a hub starts with (a multiple of kHubTabChunk)
+constexpr uint64_t kHubTabCap = 8192;
+// Worklist hubs up to kMidHubMax edges: one block of kMidBlock threads per
+// hub with its table (nextpow2(2 deg) slots, at most kMidSlots = 64 KB) in
+// LDS, kScanBatch x kMidBlock edges loaded at a time; a grid of kMidGrid
+// blocks (two per CU) walks the queue.  Larger hubs, and those with more
+// distinct neighbour communities than the table holds, go to the
+// multi-block hub batch.
+constexpr uint32_t kMidHubMax = 16384;
+constexpr uint32_t kMidSlots = 4096;
+constexpr int kMidBlock = 512;
+constexpr int kScanBatch = 8;
+constexpr uint64_t kMidGrid = 512;
+static_assert(kMidSlots > kMidBlock, "hub_mid_kernel: room for a slot per thread past the distinct limit");
 // worklist queue compaction: blocks (contiguous id segments) per pass
 constexpr int kWlScanBlocks = 1024;
 constexpr int kTierHybrid = 5;
@@ -63,6 +79,16 @@ struct TierLists {
 
 // Device counters of the local-moving kernels (for achieved-bandwidth figures).
 enum { CNT_VERTICES = 0, CNT_EDGES = 1, CNT_CANDIDATES = 2, CNT_MOVES = 3, CNT_MOVED_EDGES = 4, CNT_N = 5 };
+
+// Device atomics on one address serialise (about 12 ns each on MI355X, so
+// one per block of an 8k-block launch costs 0.1 ms).  Launch-wide sums taken
+// by one atomic per block -- the local-moving gain el and the counters -- are
+// therefore spread over kSpread slots 128 B apart, picked by block index, and
+// the host adds the slots up in slot order.
+constexpr int kSpread = 64;
+constexpr int kElStride = 16;   // doubles per gain slot (128 B)
+constexpr int kCntStride = 32;  // counters per slot: kTiers * CNT_N, rounded up to whole 128 B lines
+static_assert(kTiers * CNT_N <= kCntStride, "counter slot too small");
 
 // Arguments of the local-moving kernels.
 struct MoveArgs {
@@ -137,12 +163,24 @@ struct MoveArgs {
   uint32_t* hb_key;            // table slab
   uint32_t* hb_pos;
   double* hb_val;
+  // worklist batches (fsame set): tables capped at kHubTabCap slots; a hub
+  // whose capped table overflowed (hb_over, per list entry) is marked in
+  // hb_big (per vertex) and redone with a full table
+  uint8_t* hb_over;
+  uint8_t* hb_big;
+  uint64_t* hb_off;            // hub's row offset and degree
+  uint32_t* hb_deg;
+  uint32_t* hb_cown;           // list entry owning each edge chunk
+  uint32_t* hb_town;           // list entry owning each kHubTabChunk-slot table chunk
 };
 
 // Hub batch: every listed vertex (hub) at once, each with its own HBM table
 // and its edges split over many blocks; the reference's per-vertex steps in
 // separate device passes (table build, best gain, move, neighbour flags).
 void k_hub_batch(const MoveArgs& a, hipStream_t st);
+// Worklist hubs of degree hub_min < d <= kMidHubMax from a.list (length
+// *a.n_dev), one block each with an LDS table.
+void k_hub_mid(const MoveArgs& a, hipStream_t st);
 
 // Kernel launchers (lv_kernels.hip).  All run on `st`.
 void k_fill_u8(uint8_t* a, uint8_t v, uint64_t n, hipStream_t st);
@@ -229,11 +267,13 @@ void k_max_u32(const uint32_t* a, uint64_t n, uint32_t* out, hipStream_t st);
 void k_gather_u32(const uint32_t* a, const uint32_t* idx, uint64_t n, uint32_t* out, hipStream_t st);
 void k_caps_sum(const uint32_t* deg, uint64_t n, uint32_t thr, unsigned long long* out, hipStream_t st);
 // Ordered worklist queues: ids with f[id] set, ascending, degree <= thr to
-// ql and above to qh (thr == 0: all to ql), counts to lens[0..1]; clear
-// resets the flags read.  f holds S bytes rounded up to a multiple of 4;
-// bsum holds 2 * kWlScanBlocks entries.
-void k_wl_queue(uint8_t* f, bool clear, uint64_t S, const uint32_t* deg, uint32_t thr, uint32_t* ql, uint32_t* qh,
-                uint32_t* lens, uint64_t* bsum, hipStream_t st);
+// q0, thr < degree <= thr2 to q1 and above thr2 to q2, as are the ids above
+// thr marked in big (nullable) (thr == 0: all to q0),
+// counts to lens[0..2]; clear resets the flags read.  f holds S bytes
+// rounded up to a multiple of 4; bsum holds 3 * kWlScanBlocks entries.
+void k_wl_queue(uint8_t* f, bool clear, uint64_t S, const uint32_t* deg, uint32_t thr, uint32_t thr2,
+                const uint8_t* big, uint32_t* q0, uint32_t* q1, uint32_t* q2, uint32_t* lens, uint64_t* bsum,
+                hipStream_t st);
 void k_modularity_parts(GView g, bool f64, const uint32_t* memb, double* cin, double* ctot, hipStream_t st);
 void k_modularity_sum(const double* cin, const double* ctot, uint64_t n, double M, double R, double* out,
                       hipStream_t st);

@@ -29,6 +29,9 @@ namespace lv {
 namespace {
 
 constexpr int kBlock = 256;
+// Grid cap of the reductions that end in one same-address atomic per block
+// (those serialise, about 12 ns each): two blocks per CU still stream HBM.
+constexpr uint64_t kReduceGrid = 512;
 
 __host__ __device__ inline uint64_t grid_for(uint64_t n, uint64_t per_block, uint64_t cap = 1u << 20) {
   uint64_t g = (n + per_block - 1) / per_block;
@@ -73,6 +76,26 @@ __device__ __forceinline__ double tab_get(const uint32_t* key, const double* val
     if (k == kEmpty) return 0;
     s = (uint32_t)((s + 1) & (cap - 1));
   }
+}
+
+// tab_add for the lanes of a wave with ok set (every lane of the wave calls
+// it): the lanes whose key equals the first such lane's key c0 add their
+// summed weight once, through that lane -- one atomic instead of one per lane
+// when a community dominates -- and the others insert their own.  The lanes
+// hold ascending positions p, so the first lane holding c0 has the smallest
+// position of the lanes that share it.
+__device__ __forceinline__ void tab_add_wave(uint32_t* key, uint32_t* pos, double* val, uint64_t cap, bool ok,
+                                             uint32_t c, double w, uint32_t p) {
+  const unsigned long long act = __ballot(ok);
+  if (!act) return;
+  const int lane = threadIdx.x % 64;
+  const int lead = __ffsll((long long)act) - 1;
+  const uint32_t c0 = __shfl(c, lead, 64);
+  const bool same = ok && c == c0;
+  double sum = same ? w : 0.0;
+  for (int m = 32; m > 0; m >>= 1) sum += __shfl_xor(sum, m, 64);
+  if (lane == lead) tab_add(key, pos, val, cap, c0, sum, p);
+  else if (ok && !same) tab_add(key, pos, val, cap, c, w, p);
 }
 
 // Better candidate: larger gain, then earlier first appearance.
@@ -133,6 +156,21 @@ __device__ unsigned long long block_sum_u64(unsigned long long x) {
   for (int i = 0; i < BLOCK / 64; ++i) t += s[i];
   __syncthreads();
   return t;
+}
+
+// Slot of this block in a spread sum (kSpread slots, 128 B apart).
+__device__ __forceinline__ uint64_t spread_slot() { return blockIdx.x % kSpread; }
+
+// A block's counter totals into its spread slot (cnt: the tier's counters
+// in slot 0).
+__device__ __forceinline__ void add_counts(unsigned long long* cnt, unsigned long long v, unsigned long long e,
+                                           unsigned long long c, unsigned long long m, unsigned long long me) {
+  unsigned long long* s = cnt + spread_slot() * kCntStride;
+  if (v) atomicAdd(&s[CNT_VERTICES], v);
+  if (e) atomicAdd(&s[CNT_EDGES], e);
+  if (c) atomicAdd(&s[CNT_CANDIDATES], c);
+  if (m) atomicAdd(&s[CNT_MOVES], m);
+  if (me) atomicAdd(&s[CNT_MOVED_EDGES], me);
 }
 
 // vaff[v] = 1, as a moved vertex u flags its neighbour v (inc/louvain.hxx:602).
@@ -402,13 +440,29 @@ __device__ __forceinline__ void table_clear(uint32_t* key, uint32_t* pos, double
   for (uint64_t s = lane; s < cap; s += G) { key[s] = kEmpty; pos[s] = kNoPos; val[s] = 0; }
 }
 
+constexpr int kScanRounds = 4;
+
 template <int G, class WT>
 __device__ __forceinline__ void table_scan(const MoveArgs& a, const WT* W, uint32_t u, uint64_t o, uint32_t d,
                                            uint32_t* key, uint32_t* pos, double* val, uint64_t cap, int lane) {
-  for (uint32_t i = lane; i < d; i += G) {
-    uint32_t v = a.g.keys[o + i];
-    if (v == u) continue;  // self-loops are skipped (louvainScanCommunityW<SELF=false>)
-    tab_add(key, pos, val, cap, ld_dev(&a.vcom[v]), (double)W[o + i], i);
+  // kScanRounds rounds of G edges at a time: their key and weight loads,
+  // then their community gathers, then the inserts in round order (the
+  // order of a plain per-round loop, so the sums are the same)
+  constexpr int R = kScanRounds;
+  for (uint32_t b = 0; b < d; b += R * G) {
+    uint32_t v[R], c[R];
+    double w[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const uint32_t i = b + r * G + lane;
+      v[r] = i < d ? a.g.keys[o + i] : u;
+      w[r] = i < d ? (double)W[o + i] : 0.0;
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) c[r] = v[r] != u ? ld_dev(&a.vcom[v[r]]) : 0;
+#pragma unroll
+    for (int r = 0; r < R; ++r)  // self-loops are skipped (louvainScanCommunityW<SELF=false>)
+      if (v[r] != u) tab_add(key, pos, val, cap, c[r], w[r], b + r * G + lane);
   }
 }
 
@@ -512,7 +566,7 @@ __global__ __launch_bounds__(BLOCK) void move_kernel(MoveArgs a) {
     }
     // One vertex (offset o, degree d, community cu, weight Ku) through the
     // group; every thread of the block calls it.
-    auto visit = [&](bool act, uint32_t u, uint64_t o, uint32_t d, uint32_t cu, double Ku) {
+    auto visit = [&](bool act, uint32_t u, uint64_t o, uint32_t d, uint32_t cu, double Ku) __attribute__((always_inline)) {
       if (act && a.stamp && lane == 0) st_dev(&a.stamp[u], a.serial);
       bool hbm = MODE == MOVE_HBM;
       uint64_t cap = CAP;
@@ -636,19 +690,13 @@ __global__ __launch_bounds__(BLOCK) void move_kernel(MoveArgs a) {
   }
   if (!a.choose_only) {
     double t = block_sum<BLOCK>(el_acc);
-    if (threadIdx.x == 0 && t != 0) unsafeAtomicAdd(a.el, t);
+    if (threadIdx.x == 0 && t != 0) unsafeAtomicAdd(a.el + spread_slot() * kElStride, t);
   }
   if (a.cnt) {
     unsigned long long v = block_sum_u64<BLOCK>(c_vert), e = block_sum_u64<BLOCK>(c_edge);
     unsigned long long c = block_sum_u64<BLOCK>(c_cand), m = block_sum_u64<BLOCK>(c_move);
     unsigned long long me = block_sum_u64<BLOCK>(c_medge);
-    if (threadIdx.x == 0) {
-      if (v) atomicAdd(&a.cnt[CNT_VERTICES], v);
-      if (e) atomicAdd(&a.cnt[CNT_EDGES], e);
-      if (c) atomicAdd(&a.cnt[CNT_CANDIDATES], c);
-      if (m) atomicAdd(&a.cnt[CNT_MOVES], m);
-      if (me) atomicAdd(&a.cnt[CNT_MOVED_EDGES], me);
-    }
+    if (threadIdx.x == 0) add_counts(a.cnt, v, e, c, m, me);
   }
 }
 
@@ -663,7 +711,8 @@ __global__ __launch_bounds__(BLOCK) void move_kernel(MoveArgs a) {
 // every kTabChunk-slot chunk (and empties its slots, so the slab is clean
 // for the next batch), the apply pass reduces a hub's chunk results; the
 // chosen id is the community of the neighbour at that position.
-constexpr uint64_t kHubChunk = 1024;          // edges per insert / flag chunk
+constexpr uint64_t kHubChunk = kHubChunkEdges;  // edges per insert / flag chunk
+static_assert(kHubChunk % kBlock == 0, "hub_insert_kernel: whole edges per thread");
 constexpr uint32_t kHubLdsTab = 2 * kHubChunk;  // LDS slots for one chunk's communities
 constexpr uint64_t kTabChunk = kHubTabChunk;  // table slots per best-pass chunk
 
@@ -708,16 +757,36 @@ __device__ __forceinline__ void tab_merge(uint32_t* key, uint32_t* pos, double* 
   }
 }
 
+// tab_merge that probes at most cap slots: false if c found no slot (the
+// table is full of other communities).
+__device__ __forceinline__ bool tab_merge_bounded(uint32_t* key, uint32_t* pos, double* val, uint64_t cap,
+                                                  uint32_t c, double w, uint32_t p) {
+  uint32_t s = slot_of(c, cap);
+  for (uint64_t probe = 0; probe < cap; ++probe) {
+    uint32_t k = __hip_atomic_load(&key[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (k == kEmpty) k = atomicCAS(&key[s], kEmpty, c);
+    if (k == kEmpty || k == c) {
+      unsafeAtomicAdd(&val[s], w);
+      if (pos) atomicMin(&pos[s], p);
+      return true;
+    }
+    s = (uint32_t)((s + 1) & (cap - 1));
+  }
+  return false;
+}
+
 __device__ __forceinline__ uint64_t list_len(const MoveArgs& a) { return a.n_dev ? (uint64_t)*a.n_dev : a.n; }
 
 // One block: per listed vertex, active or not, its table slots
 // (nextpow2(2 deg), at least kTabChunk) and edge chunks, scanned into
-// hb_cap / hb_chunks (n + 1 entries, batch totals in hb_tot); an active hub's
-// community, that community's Sigma_tot and a zero own-community weight
-// (summed by the insert pass); every hub starts as "did not move".  The list
-// is walked in tiles of kPrepBlock x kPrepItems entries, each thread taking
-// kPrepItems consecutive ones.
-constexpr int kPrepBlock = 1024, kPrepItems = 8;
+// hb_cap / hb_chunks (n + 1 entries, batch totals in hb_tot), and the owner
+// of every edge chunk (hb_cown) and table chunk (hb_town); an active hub's
+// row, community, that community's Sigma_tot and a zero own-community
+// weight (summed by the insert pass); every hub starts as "did not move".
+// The list is walked in tiles of kPrepBlock x kPrepItems entries, each thread
+// taking kPrepItems consecutive ones; all of a tile's loads are issued
+// before any store, so they overlap.
+constexpr int kPrepBlock = 512, kPrepItems = 4;
 
 __global__ __launch_bounds__(kPrepBlock) void hub_prep_kernel(MoveArgs a) {
   const uint64_t n = list_len(a);
@@ -725,28 +794,42 @@ __global__ __launch_bounds__(kPrepBlock) void hub_prep_kernel(MoveArgs a) {
   uint64_t run_cap = 0, run_chk = 0;
   for (uint64_t b = 0; b < n; b += tile) {  // block-uniform trip count
     const uint64_t k0 = b + (uint64_t)threadIdx.x * kPrepItems;
+    uint32_t u[kPrepItems], d[kPrepItems], cu[kPrepItems];
+    bool act[kPrepItems], full[kPrepItems];
+    uint64_t o[kPrepItems];
+    double sd[kPrepItems];
+#pragma unroll
+    for (int j = 0; j < kPrepItems; ++j) u[j] = k0 + j < n ? a.list[k0 + j] : 0;
+#pragma unroll
+    for (int j = 0; j < kPrepItems; ++j) {
+      act[j] = full[j] = false;
+      d[j] = 0;
+      if (k0 + j >= n) continue;
+      d[j] = a.g.deg[u[j]];
+      act[j] = d[j] > a.hub_min && (!a.g.ex || a.g.ex[u[j]]) && !vertex_skipped(a, u[j]) &&
+               (a.choose_only || (ld_dev(&a.vaff[u[j]]) && (!a.fa || a.fa[u[j]])));
+      full[j] = !a.hb_over || a.hb_big[u[j]];
+    }
+#pragma unroll
+    for (int j = 0; j < kPrepItems; ++j) {
+      cu[j] = act[j] ? ld_dev(&a.vcom[u[j]]) : 0;
+      o[j] = act[j] ? a.g.off[u[j]] : 0;
+    }
+#pragma unroll
+    for (int j = 0; j < kPrepItems; ++j) sd[j] = act[j] ? ld_dev(&a.ctot[cu[j]]) : 0;
     uint64_t cap[kPrepItems], chk[kPrepItems], tc = 0, tk = 0;
 #pragma unroll
     for (int j = 0; j < kPrepItems; ++j) {
-      const uint64_t k = k0 + j;
       cap[j] = chk[j] = 0;
-      if (k >= n) continue;
-      const uint32_t u = a.list[k];
-      const uint64_t d = a.g.deg[u];
-      const bool act = d > a.hub_min && (!a.g.ex || a.g.ex[u]) && !vertex_skipped(a, u) &&
-                       (a.choose_only || (ld_dev(&a.vaff[u]) && (!a.fa || a.fa[u])));
-      a.hb_bpos[k] = kNoPos;
-      if (act) {
-        uint64_t c = kTabChunk;
-        while (c < 2 * d) c <<= 1;
-        cap[j] = c;
-        chk[j] = (d + kHubChunk - 1) / kHubChunk;
-        if (a.stamp) st_dev(&a.stamp[u], a.serial);
-        const uint32_t cu = ld_dev(&a.vcom[u]);
-        a.hb_cu[k] = cu;
-        a.hb_sd[k] = ld_dev(&a.ctot[cu]);
-        a.hb_vd[k] = 0;
-      }
+      if (!act[j]) continue;
+      uint64_t c = kTabChunk;
+      while (c < 2 * (uint64_t)d[j]) c <<= 1;
+      // worklist batches: a hub's neighbours fall in few communities, so
+      // its table starts at kHubTabCap slots; one that overflowed keeps
+      // the full nextpow2(2 deg) from then on
+      if (!full[j] && c > kHubTabCap) c = kHubTabCap;
+      cap[j] = c;
+      chk[j] = (d[j] + kHubChunk - 1) / kHubChunk;
       tc += cap[j];
       tk += chk[j];
     }
@@ -756,9 +839,20 @@ __global__ __launch_bounds__(kPrepBlock) void hub_prep_kernel(MoveArgs a) {
 #pragma unroll
     for (int j = 0; j < kPrepItems; ++j) {
       const uint64_t k = k0 + j;
-      if (k < n) {
-        a.hb_cap[k] = pc;
-        a.hb_chunks[k] = pk;
+      if (k >= n) continue;
+      a.hb_bpos[k] = kNoPos;
+      if (a.hb_over) a.hb_over[k] = 0;
+      a.hb_cap[k] = pc;
+      a.hb_chunks[k] = pk;
+      if (act[j]) {
+        if (a.stamp) st_dev(&a.stamp[u[j]], a.serial);
+        a.hb_cu[k] = cu[j];
+        a.hb_sd[k] = sd[j];
+        a.hb_vd[k] = 0;
+        a.hb_off[k] = o[j];
+        a.hb_deg[k] = d[j];
+        for (uint64_t q = 0; q < chk[j]; ++q) a.hb_cown[pk + q] = (uint32_t)k;
+        for (uint64_t t = 0; t < cap[j] / kTabChunk; ++t) a.hb_town[pc / kTabChunk + t] = (uint32_t)k;
       }
       pc += cap[j];
       pk += chk[j];
@@ -788,7 +882,6 @@ template <class WT>
 __global__ __launch_bounds__(kBlock) void hub_insert_kernel(MoveArgs a) {
   __shared__ uint32_t s_key[kHubLdsTab], s_pos[kHubLdsTab];
   __shared__ double s_val[kHubLdsTab];
-  const uint64_t n = list_len(a);
   const uint64_t nchunks = a.hb_tot[1];
   const WT* W = (const WT*)a.g.w;
   for (uint64_t q = blockIdx.x; q < nchunks; q += gridDim.x) {
@@ -798,22 +891,39 @@ __global__ __launch_bounds__(kBlock) void hub_insert_kernel(MoveArgs a) {
       s_val[s] = 0;
     }
     __syncthreads();
-    uint64_t k = find_owner(a.hb_chunks, n, q);
-    uint32_t u = a.list[k];
-    uint64_t o = a.g.off[u], d = a.g.deg[u];
+    const uint32_t k = a.hb_cown[q];
+    const uint32_t u = a.list[k];
+    const uint64_t o = a.hb_off[k], d = a.hb_deg[k];
     uint64_t i0 = (q - a.hb_chunks[k]) * kHubChunk, i1 = min(d, i0 + kHubChunk);
-    for (uint64_t i = i0 + threadIdx.x; i < i1; i += kBlock) {
-      uint32_t v = a.g.keys[o + i];
-      if (v == u) continue;
-      tab_add(s_key, s_pos, s_val, kHubLdsTab, ld_dev(&a.vcom[v]), (double)W[o + i], (uint32_t)i);
+    // the chunk's kHubChunk / kBlock edges per thread: every key and weight
+    // load first, then every community gather, then the inserts (two
+    // dependent memory round trips per chunk, not two per edge)
+    constexpr int R = kHubChunk / kBlock;
+    uint32_t v[R], c[R];
+    double w[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const uint64_t i = i0 + (uint64_t)r * kBlock + threadIdx.x;
+      v[r] = i < i1 ? a.g.keys[o + i] : u;
+      w[r] = i < i1 ? (double)W[o + i] : 0.0;
     }
+#pragma unroll
+    for (int r = 0; r < R; ++r) c[r] = v[r] != u ? ld_dev(&a.vcom[v[r]]) : 0;
+#pragma unroll
+    for (int r = 0; r < R; ++r)  // self-loops are skipped
+      tab_add_wave(s_key, s_pos, s_val, kHubLdsTab, v[r] != u, c[r], w[r],
+                   (uint32_t)(i0 + (uint64_t)r * kBlock + threadIdx.x));
     __syncthreads();
     uint64_t base = a.hb_cap[k], cap = a.hb_cap[k + 1] - base;
     const uint32_t cu = a.hb_cu[k];
     for (uint32_t s = threadIdx.x; s < kHubLdsTab; s += kBlock) {
       uint32_t c = s_key[s];
       if (c == kEmpty) continue;
-      tab_merge(a.hb_key + base, a.hb_pos + base, a.hb_val + base, cap, c, s_val[s], s_pos[s]);
+      // a capped table that is full: the hub is redone with a full one (a
+      // full table, nextpow2(2 deg) slots, always has room)
+      if (!tab_merge_bounded(a.hb_key + base, a.hb_pos + base, a.hb_val + base, cap, c, s_val[s], s_pos[s]) &&
+          a.hb_over)
+        a.hb_over[k] = 1;
       if (c == cu) unsafeAtomicAdd(&a.hb_vd[k], s_val[s]);  // weight to the hub's own community
     }
     __syncthreads();
@@ -823,11 +933,10 @@ __global__ __launch_bounds__(kBlock) void hub_insert_kernel(MoveArgs a) {
 // Best (gain, first position) of each kTabChunk-slot chunk of the tables;
 // the slots are emptied as they are read.
 __global__ __launch_bounds__(kBlock) void hub_best_kernel(MoveArgs a) {
-  const uint64_t n = list_len(a);
   const uint64_t nt = a.hb_tot[0] / kTabChunk;
   for (uint64_t t = blockIdx.x; t < nt; t += gridDim.x) {
     const uint64_t s0 = t * kTabChunk;
-    const uint64_t k = find_owner(a.hb_cap, n, s0);
+    const uint32_t k = a.hb_town[t];
     const uint32_t cu = a.hb_cu[k], u = a.list[k];
     const double vd = a.hb_vd[k], sd = a.hb_sd[k], Ku = a.vtot[u], Mu = vertex_M(a, u);
     double be = 0;
@@ -879,6 +988,13 @@ __global__ __launch_bounds__(kBlock) void hub_apply_kernel(MoveArgs a) {
     group_best_shfl<64>(be, bp, bcx);
     if (lane != 0) continue;
     uint32_t u = a.list[k];
+    if (a.hb_over && a.hb_over[k]) {
+      // its capped table overflowed: no decision; it stays flagged, gets a
+      // full table and joins the next sub-round
+      a.hb_big[u] = 1;
+      a.fsame[u] = 1;
+      continue;
+    }
     double e = bp == kNoPos ? 0.0 : be;
     uint32_t bc = 0;
     if (bp != kNoPos) bc = ld_dev(&a.vcom[a.g.keys[a.g.off[u] + bp]]);
@@ -906,28 +1022,22 @@ __global__ __launch_bounds__(kBlock) void hub_apply_kernel(MoveArgs a) {
   }
   if (a.choose_only) return;
   double t = block_sum<kBlock>(el);
-  if (threadIdx.x == 0 && t != 0) unsafeAtomicAdd(a.el, t);
+  if (threadIdx.x == 0 && t != 0) unsafeAtomicAdd(a.el + spread_slot() * kElStride, t);
   if (a.cnt) {
     unsigned long long v = block_sum_u64<kBlock>(cv), e2 = block_sum_u64<kBlock>(ce);
     unsigned long long m = block_sum_u64<kBlock>(cm), me = block_sum_u64<kBlock>(cme);
-    if (threadIdx.x == 0) {
-      if (v) atomicAdd(&a.cnt[CNT_VERTICES], v);
-      if (e2) atomicAdd(&a.cnt[CNT_EDGES], e2);
-      if (m) atomicAdd(&a.cnt[CNT_MOVES], m);
-      if (me) atomicAdd(&a.cnt[CNT_MOVED_EDGES], me);
-    }
+    if (threadIdx.x == 0) add_counts(a.cnt, v, e2, 0, m, me);
   }
 }
 
 // Neighbours of the hubs that moved are flagged (chunks spread over the grid).
 __global__ __launch_bounds__(kBlock) void hub_flag_kernel(MoveArgs a) {
-  const uint64_t n = list_len(a);
   const uint64_t nchunks = a.hb_tot[1];
   for (uint64_t q = blockIdx.x; q < nchunks; q += gridDim.x) {
-    uint64_t k = find_owner(a.hb_chunks, n, q);
+    const uint32_t k = a.hb_cown[q];
     if (a.hb_bpos[k] == kNoPos) continue;  // uniform per block
-    uint32_t u = a.list[k];
-    uint64_t o = a.g.off[u], d = a.g.deg[u];
+    const uint32_t u = a.list[k];
+    const uint64_t o = a.hb_off[k], d = a.hb_deg[k];
     uint64_t i0 = (q - a.hb_chunks[k]) * kHubChunk, i1 = min(d, i0 + kHubChunk);
     for (uint64_t b = i0; b < i1; b += kBlock) {  // uniform trip count: wave ballots in flag_vertex
       uint64_t i = b + threadIdx.x;
@@ -935,6 +1045,197 @@ __global__ __launch_bounds__(kBlock) void hub_flag_kernel(MoveArgs a) {
       uint32_t v = on ? a.g.keys[o + i] : u;
       if (on && v != u) flag_vertex(a.vaff, v, u, a);
     }
+  }
+}
+
+// louvainMoveOmpW's body (inc/louvain.hxx:595-605) for worklist hubs of
+// moderate degree: one block per hub, its neighbour-community table in LDS.
+// The blocks walk the queue a.list (length *a.n_dev) in strides of the grid.
+// Per hub: sum edge weights per neighbour community (self-loops skipped,
+// first position kept for the first-seen tie-break), pick the best positive
+// gain, and unless it is community 0 move there (atomic Sigma_tot) and flag
+// the neighbours; clear the hub's flag and add its gain to el.
+//
+// The table has nextpow2(2 deg) slots up to kMidSlots.  A hub of more than
+// kMidSlots / 2 edges may have more distinct neighbour communities than
+// that: its distinct count is kept, and past kMidDistinct (the table then
+// still has a free slot for every thread of the block) the hub is left
+// flagged, marked for a full table (hb_big) and sent to the next sub-round,
+// where it runs in the multi-block batch.
+constexpr uint32_t kMidDistinct = kMidSlots - kMidBlock;
+
+// tab_add_wave with a count of the slots it creates.
+__device__ __forceinline__ void tab_add_wave_counted(uint32_t* key, uint32_t* pos, double* val, uint32_t cap,
+                                                     bool ok, uint32_t c, double w, uint32_t p, uint32_t* made) {
+  const unsigned long long act = __ballot(ok);
+  if (!act) return;
+  const int lane = threadIdx.x % 64;
+  const int lead = __ffsll((long long)act) - 1;
+  const uint32_t c0 = __shfl(c, lead, 64);
+  const bool same = ok && c == c0;
+  double sum = same ? w : 0.0;
+  for (int m = 32; m > 0; m >>= 1) sum += __shfl_xor(sum, m, 64);
+  if (lane == lead) {
+    c = c0;
+    w = sum;
+  } else if (!ok || same) {
+    return;
+  }
+  uint32_t s = slot_of(c, cap);
+  while (true) {
+    const uint32_t k = atomicCAS(&key[s], kEmpty, c);
+    if (k == kEmpty || k == c) {
+      if (k == kEmpty) atomicAdd(made, 1u);
+      unsafeAtomicAdd(&val[s], w);
+      atomicMin(&pos[s], p);
+      return;
+    }
+    s = (s + 1) & (cap - 1);
+  }
+}
+
+template <class WT>
+__global__ __launch_bounds__(kMidBlock) void hub_mid_kernel(MoveArgs a) {
+  __shared__ uint32_t s_key[kMidSlots], s_pos[kMidSlots];
+  __shared__ double s_val[kMidSlots];
+  __shared__ uint32_t s_u, s_d, s_cu, s_made;
+  __shared__ uint64_t s_o;
+  __shared__ double s_ku, s_sd;
+  __shared__ int s_act;
+  const uint64_t n = list_len(a);
+  const WT* W = (const WT*)a.g.w;
+  double el_acc = 0;
+  unsigned long long c_vert = 0, c_edge = 0, c_cand = 0, c_move = 0, c_medge = 0;
+  for (uint64_t k = blockIdx.x; k < n; k += gridDim.x) {
+    // the hub's scalars, read once and shared (flags change under us)
+    if (threadIdx.x == 0) {
+      const uint32_t u = a.list[k];
+      const uint32_t d = a.g.deg[u];
+      int act = d > a.hub_min && d <= kMidHubMax && (!a.g.ex || a.g.ex[u]) && !vertex_skipped(a, u) &&
+                (a.choose_only || (ld_dev(&a.vaff[u]) && (!a.fa || a.fa[u])));
+      s_u = u;
+      s_d = d;
+      s_act = act;
+      s_made = 0;
+      if (act) {
+        if (a.stamp) st_dev(&a.stamp[u], a.serial);
+        const uint32_t cu = ld_dev(&a.vcom[u]);
+        s_cu = cu;
+        s_o = a.g.off[u];
+        s_ku = a.vtot[u];
+        s_sd = ld_dev(&a.ctot[cu]);
+      }
+    }
+    __syncthreads();
+    if (!s_act) {
+      __syncthreads();
+      continue;
+    }
+    const uint32_t u = s_u, d = s_d, cu = s_cu;
+    const uint64_t o = s_o;
+    const double Ku = s_ku;
+    uint32_t cap = 16;
+    while (cap < 2 * d && cap < kMidSlots) cap <<= 1;
+    const bool capped = cap < 2 * d;  // more edges than the table has room for as distinct keys
+    for (uint32_t s = threadIdx.x; s < cap; s += kMidBlock) {
+      s_key[s] = kEmpty;
+      s_pos[s] = kNoPos;
+      s_val[s] = 0;
+    }
+    __syncthreads();
+    // kScanBatch rounds of kMidBlock edges at a time: every key and weight
+    // load, then every community gather, then the inserts (a thread makes at
+    // most one new slot per round after reading the distinct count, so a
+    // capped table never fills)
+    bool over = false;
+    for (uint32_t b0 = 0; b0 < d && !over; b0 += kScanBatch * kMidBlock) {  // block-uniform
+      uint32_t v[kScanBatch], c[kScanBatch];
+      double w[kScanBatch];
+#pragma unroll
+      for (int r = 0; r < kScanBatch; ++r) {
+        const uint32_t i = b0 + r * kMidBlock + threadIdx.x;
+        v[r] = i < d ? a.g.keys[o + i] : u;
+        w[r] = i < d ? (double)W[o + i] : 0.0;
+      }
+#pragma unroll
+      for (int r = 0; r < kScanBatch; ++r) c[r] = v[r] != u ? ld_dev(&a.vcom[v[r]]) : 0;
+#pragma unroll
+      for (int r = 0; r < kScanBatch; ++r) {
+        if (b0 + r * kMidBlock < d) {  // block-uniform
+          if (capped && __hip_atomic_load(&s_made, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) >= kMidDistinct)
+            over = true;
+          // self-loops are skipped (louvainScanCommunityW<SELF=false>)
+          tab_add_wave_counted(s_key, s_pos, s_val, cap, !over && v[r] != u, c[r], w[r],
+                               b0 + r * kMidBlock + threadIdx.x, &s_made);
+        }
+      }
+      over = __syncthreads_or(over);
+    }
+    if (over) {
+      // too many distinct communities for the table: no decision here
+      if (threadIdx.x == 0) {
+        a.hb_big[u] = 1;
+        a.fsame[u] = 1;
+      }
+      __syncthreads();
+      continue;
+    }
+    const double vd = tab_get(s_key, s_val, cap, cu), Sd = s_sd, Mu = vertex_M(a, u);
+    double be = 0;
+    uint32_t bp = kNoPos, bc = 0;
+    for (uint32_t s = threadIdx.x; s < cap; s += kMidBlock) {
+      const uint32_t c = s_key[s];
+      if (c == kEmpty) continue;
+      ++c_cand;
+      if (c == cu || (a.pick_less && c > cu)) continue;
+      const double e = delta_q(s_val[s], vd, Ku, ld_dev(&a.ctot[c]), Sd, Mu, a.R);
+      const uint32_t p = s_pos[s];
+      if (e > 0 && better(e, p, be, bp)) {
+        be = e;
+        bp = p;
+        bc = c;
+      }
+    }
+    block_best<kMidBlock>(be, bp, bc);
+    if (a.choose_only) {
+      if (threadIdx.x == 0) {
+        a.out_c[u] = bc;
+        a.out_e[u] = be;
+      }
+    } else {
+      // a candidate exists and it is not community 0 of u's graph
+      const bool moves = bp != kNoPos && bc != vertex_zero(a, u);
+      if (moves) {
+        if (threadIdx.x == 0) {
+          unsafeAtomicAdd(&a.ctot[cu], -Ku);
+          unsafeAtomicAdd(&a.ctot[bc], Ku);
+          st_dev(&a.vcom[u], bc);
+          ++c_move;
+          c_medge += d;
+        }
+        for (uint32_t i = threadIdx.x; i < d; i += kMidBlock) {
+          const uint32_t v = a.g.keys[o + i];
+          if (v != u) flag_vertex(a.vaff, v, u, a);
+        }
+      }
+      if (threadIdx.x == 0) {
+        st_dev(&a.vaff[u], (uint8_t)0);
+        if (a.gel) unsafeAtomicAdd(&a.gel[a.gid[u]], be);
+        else el_acc += be;
+        ++c_vert;
+        c_edge += d;
+      }
+    }
+    __syncthreads();  // the table and the shared scalars are reused
+  }
+  if (a.choose_only) return;
+  const double t = block_sum<kMidBlock>(el_acc);
+  if (threadIdx.x == 0 && t != 0) unsafeAtomicAdd(a.el + spread_slot() * kElStride, t);
+  if (a.cnt) {
+    unsigned long long v = block_sum_u64<kMidBlock>(c_vert), e = block_sum_u64<kMidBlock>(c_edge);
+    unsigned long long c = block_sum_u64<kMidBlock>(c_cand), m = block_sum_u64<kMidBlock>(c_move);
+    unsigned long long me = block_sum_u64<kMidBlock>(c_medge);
+    if (threadIdx.x == 0) add_counts(a.cnt, v, e, c, m, me);
   }
 }
 
@@ -1096,22 +1397,6 @@ __global__ __launch_bounds__(BLOCK) void aggregate_kernel(AggArgs a) {
 // row offsets) are staged in LDS, so an edge finds its member there.
 constexpr uint32_t kAggLdsTab = 2 * kAggHbmChunk;
 constexpr uint32_t kAggMembers = kAggHbmChunk;
-
-// (lanes hold ascending positions p, so the first lane holding c0 has the
-// smallest position of the lanes that share it)
-__device__ __forceinline__ void tab_add_wave(uint32_t* key, uint32_t* pos, double* val, uint64_t cap, bool ok,
-                                             uint32_t c, double w, uint32_t p) {
-  const unsigned long long act = __ballot(ok);
-  if (!act) return;
-  const int lane = threadIdx.x % 64;
-  const int lead = __ffsll((long long)act) - 1;
-  const uint32_t c0 = __shfl(c, lead, 64);
-  const bool same = ok && c == c0;
-  double sum = same ? w : 0.0;
-  for (int m = 32; m > 0; m >>= 1) sum += __shfl_xor(sum, m, 64);
-  if (lane == lead) tab_add(key, pos, val, cap, c0, sum, p);
-  else if (ok && !same) tab_add(key, pos, val, cap, c, w, p);
-}
 
 template <class WT>
 __global__ __launch_bounds__(kBlock) void aggregate_hbm_kernel(AggArgs a) {
@@ -1335,23 +1620,35 @@ __global__ __launch_bounds__(kBlock) void bin_queue_kernel(const uint32_t* q, co
   }
 }
 
+// Maximum over a kBlock-thread block (every thread gets it).
+__device__ uint32_t block_max_u32(uint32_t m) {
+  __shared__ uint32_t s[kBlock / 64];
+#pragma unroll
+  for (int k = 32; k > 0; k >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, k, 64));
+  if (threadIdx.x % 64 == 0) s[threadIdx.x / 64] = m;
+  __syncthreads();
+  m = s[0];
+#pragma unroll
+  for (int i = 1; i < kBlock / 64; ++i) m = max(m, s[i]);
+  __syncthreads();
+  return m;
+}
+
 __global__ __launch_bounds__(kBlock) void max_deg_kernel(const uint32_t* ids, uint64_t n, const uint32_t* deg,
                                                          uint32_t* out) {
   uint32_t m = 0;
   for (uint64_t i = blockIdx.x * (uint64_t)kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock)
     m = max(m, deg[ids[i]]);
-#pragma unroll
-  for (int k = 32; k > 0; k >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, k, 64));
-  if (__lane_id() == 0 && m) atomicMax(out, m);
+  m = block_max_u32(m);
+  if (threadIdx.x == 0 && m) atomicMax(out, m);
 }
 
 __global__ __launch_bounds__(kBlock) void max_u32_kernel(const uint32_t* a, uint64_t n, uint32_t* out) {
   uint32_t m = 0;
   for (uint64_t i = blockIdx.x * (uint64_t)kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock)
     m = max(m, a[i]);
-#pragma unroll
-  for (int k = 32; k > 0; k >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, k, 64));
-  if (__lane_id() == 0 && m) atomicMax(out, m);
+  m = block_max_u32(m);
+  if (threadIdx.x == 0 && m) atomicMax(out, m);
 }
 
 __global__ void gather_u32_kernel(const uint32_t* a, const uint32_t* idx, uint64_t n, uint32_t* out) {
@@ -1375,12 +1672,14 @@ __global__ __launch_bounds__(kBlock) void caps_sum_kernel(const uint32_t* deg, u
 }
 
 // Ordered worklist queues (inc/louvain.hxx:594-598 visits ids in ascending
-// order): the ids whose flag byte is set, ascending, split at degree thr
-// into the low queue and the hub queue (thr == 0: all low); with clear, the
-// flags read are reset.  Counts per contiguous block segment, a one-block
-// scan of the counts, then an ordered scatter; flags are read a 4-byte word
-// per thread.
+// order): the ids whose flag byte is set, ascending, split by degree into
+// queue 0 (degree <= thr), queue 1 (thr < degree <= thr2) and queue 2
+// (degree > thr2); thr == 0 puts every id in queue 0.  With clear, the flags
+// read are reset.  Counts per contiguous block segment, a one-block scan of
+// the counts, then an ordered scatter; flags are read a 4-byte word per
+// thread.
 constexpr uint64_t kWlTile = 4 * kBlock;
+constexpr int kWlQueues = 3;
 static_assert(kWlScanBlocks % kBlock == 0, "wl_scan_kernel: whole block counts per thread");
 
 __device__ __forceinline__ uint64_t wl_segment(uint64_t S) {
@@ -1388,85 +1687,84 @@ __device__ __forceinline__ uint64_t wl_segment(uint64_t S) {
   return (seg + kWlTile - 1) / kWlTile * kWlTile;
 }
 
-__device__ __forceinline__ uint32_t wl_bits(const uint8_t* f, const uint32_t* deg, uint64_t S, uint32_t thr,
-                                            uint64_t id0, uint32_t& hubm) {
-  uint32_t lowm = 0;
-  hubm = 0;
+// Bit j of m[q]: id id0 + j is flagged and belongs to queue q.
+__device__ __forceinline__ bool wl_bits(const uint8_t* f, const uint32_t* deg, uint64_t S, uint32_t thr,
+                                        uint32_t thr2, const uint8_t* big, uint64_t id0,
+                                        uint32_t (&m)[kWlQueues]) {
+  m[0] = m[1] = m[2] = 0;
   uint32_t w = reinterpret_cast<const uint32_t*>(f)[id0 >> 2];
-  if (!w) return 0;
+  if (!w) return false;
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
     if (id0 + j < S && ((w >> (8 * j)) & 0xFFu)) {
-      if (thr && deg[id0 + j] > thr) hubm |= 1u << j;
-      else lowm |= 1u << j;
+      const uint32_t d = thr ? deg[id0 + j] : 0;
+      const int q = d <= thr ? 0 : ((d <= thr2 && !(big && big[id0 + j])) ? 1 : 2);
+      m[q] |= 1u << j;
     }
   }
-  return lowm;
+  return true;
 }
 
 __global__ __launch_bounds__(kBlock) void wl_count_kernel(const uint8_t* f, uint64_t S, const uint32_t* deg,
-                                                          uint32_t thr, uint64_t* bsum) {
+                                                          uint32_t thr, uint32_t thr2, const uint8_t* big,
+                                                          uint64_t* bsum) {
   const uint64_t seg = wl_segment(S), lo = blockIdx.x * seg, hi = min(S, lo + seg);
-  unsigned long long cl = 0, ch = 0;
+  unsigned long long c[kWlQueues] = {0, 0, 0};
   for (uint64_t id0 = lo + 4 * (uint64_t)threadIdx.x; id0 < hi; id0 += kWlTile) {
-    uint32_t hubm;
-    uint32_t lowm = wl_bits(f, deg, S, thr, id0, hubm);
-    cl += __popc(lowm);
-    ch += __popc(hubm);
+    uint32_t m[kWlQueues];
+    if (!wl_bits(f, deg, S, thr, thr2, big, id0, m)) continue;
+#pragma unroll
+    for (int q = 0; q < kWlQueues; ++q) c[q] += __popc(m[q]);
   }
-  cl = block_sum_u64<kBlock>(cl);
-  ch = block_sum_u64<kBlock>(ch);
-  if (threadIdx.x == 0) {
-    bsum[blockIdx.x] = cl;
-    bsum[kWlScanBlocks + blockIdx.x] = ch;
+#pragma unroll
+  for (int q = 0; q < kWlQueues; ++q) {
+    const unsigned long long t = block_sum_u64<kBlock>(c[q]);
+    if (threadIdx.x == 0) bsum[q * kWlScanBlocks + blockIdx.x] = t;
   }
 }
 
 // One block of kBlock threads, kWlScanBlocks / kBlock block counts each.
 __global__ __launch_bounds__(kBlock) void wl_scan_kernel(uint64_t* bsum, uint32_t* lens) {
   constexpr int P = kWlScanBlocks / kBlock;
-  uint64_t* a = bsum + (uint64_t)threadIdx.x * P;
-  uint64_t* b = bsum + kWlScanBlocks + (uint64_t)threadIdx.x * P;
-  uint64_t sa = 0, sb = 0;
-  for (int k = 0; k < P; ++k) {
-    sa += a[k];
-    sb += b[k];
-  }
-  uint64_t ta, tb;
-  uint64_t ea = block_scan_excl<kBlock>(sa, ta), eb = block_scan_excl<kBlock>(sb, tb);
-  for (int k = 0; k < P; ++k) {
-    uint64_t x = a[k], y = b[k];
-    a[k] = ea;
-    b[k] = eb;
-    ea += x;
-    eb += y;
-  }
-  if (threadIdx.x == 0) {
-    lens[0] = (uint32_t)ta;
-    lens[1] = (uint32_t)tb;
+#pragma unroll
+  for (int q = 0; q < kWlQueues; ++q) {
+    uint64_t* a = bsum + q * kWlScanBlocks + (uint64_t)threadIdx.x * P;
+    uint64_t sa = 0;
+    for (int k = 0; k < P; ++k) sa += a[k];
+    uint64_t ta;
+    uint64_t ea = block_scan_excl<kBlock>(sa, ta);
+    for (int k = 0; k < P; ++k) {
+      uint64_t x = a[k];
+      a[k] = ea;
+      ea += x;
+    }
+    if (threadIdx.x == 0) lens[q] = (uint32_t)ta;
   }
 }
 
 __global__ __launch_bounds__(kBlock) void wl_scatter_kernel(uint8_t* f, int clear, uint64_t S, const uint32_t* deg,
-                                                            uint32_t thr, const uint64_t* bsum, uint32_t* ql,
-                                                            uint32_t* qh) {
+                                                            uint32_t thr, uint32_t thr2, const uint8_t* big,
+                                                            const uint64_t* bsum, uint32_t* q0, uint32_t* q1,
+                                                            uint32_t* q2) {
   const uint64_t seg = wl_segment(S), lo = blockIdx.x * seg, hi = min(S, lo + seg);
-  uint64_t rl = bsum[blockIdx.x], rh = bsum[kWlScanBlocks + blockIdx.x];
+  uint32_t* const qs[kWlQueues] = {q0, q1, q2};
+  uint64_t r[kWlQueues];
+#pragma unroll
+  for (int q = 0; q < kWlQueues; ++q) r[q] = bsum[q * kWlScanBlocks + blockIdx.x];
   for (uint64_t t0 = lo; t0 < hi; t0 += kWlTile) {  // block-uniform trip count
     const uint64_t id0 = t0 + 4 * (uint64_t)threadIdx.x;
-    uint32_t lowm = 0, hubm = 0;
-    if (id0 < hi) lowm = wl_bits(f, deg, S, thr, id0, hubm);
-    uint64_t tl, th;
-    uint64_t el = block_scan_excl<kBlock>(__popc(lowm), tl);
-    uint64_t eh = block_scan_excl<kBlock>(__popc(hubm), th);
+    uint32_t m[kWlQueues] = {0, 0, 0};
+    const bool any = id0 < hi && wl_bits(f, deg, S, thr, thr2, big, id0, m);
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      if (lowm & (1u << j)) ql[rl + el++] = (uint32_t)(id0 + j);
-      if (hubm & (1u << j)) qh[rh + eh++] = (uint32_t)(id0 + j);
+    for (int q = 0; q < kWlQueues; ++q) {
+      uint64_t tq;
+      uint64_t e = block_scan_excl<kBlock>(__popc(m[q]), tq);
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        if (m[q] & (1u << j)) qs[q][r[q] + e++] = (uint32_t)(id0 + j);
+      r[q] += tq;
     }
-    if (clear && (lowm | hubm)) reinterpret_cast<uint32_t*>(f)[id0 >> 2] = 0;
-    rl += tl;
-    rh += th;
+    if (clear && any) reinterpret_cast<uint32_t*>(f)[id0 >> 2] = 0;
   }
 }
 
@@ -1559,7 +1857,7 @@ void k_vertex_weights(GView g, bool f64, double* vtot, hipStream_t st) {
 }
 void k_edge_weight(GView g, bool f64, double* out, hipStream_t st) {
   if (!g.S) return;
-  uint64_t grid = grid_for(g.S, kBlock, 1u << 14);
+  uint64_t grid = grid_for(g.S, kBlock, 2048);
   if (f64) LV_LAUNCH((row_sum_kernel<double, true>), grid, kBlock, st, g, nullptr, out);
   else LV_LAUNCH((row_sum_kernel<float, true>), grid, kBlock, st, g, nullptr, out);
 }
@@ -1593,7 +1891,7 @@ void k_ds_sweep(GView g, const uint32_t* vcom, const uint8_t* neighbors, const u
   if (g.S) LV_LAUNCH(ds_sweep_kernel, grid_for(g.S, kBlock, 1u << 16), kBlock, st, g, vcom, neighbors, communities, vertices);
 }
 void k_count_u8(const uint8_t* a, uint8_t v, uint64_t n, unsigned long long* out, hipStream_t st) {
-  if (n) LV_LAUNCH(count_u8_kernel, grid_for(n, kBlock * 8, 4096), kBlock, st, a, v, n, out);
+  if (n) LV_LAUNCH(count_u8_kernel, grid_for(n, kBlock * 8, kReduceGrid), kBlock, st, a, v, n, out);
 }
 void k_check_sorted(const uint32_t* u, const uint32_t* v, uint64_t n, uint32_t S, int* bad, hipStream_t st) {
   if (n) LV_LAUNCH(check_sorted_kernel, grid_for(n, kBlock), kBlock, st, u, v, n, S, bad);
@@ -1651,6 +1949,10 @@ void k_hub_batch(const MoveArgs& a, hipStream_t st) {
   LV_LAUNCH(hub_best_kernel, grid, kBlock, st, a);
   LV_LAUNCH(hub_apply_kernel, 64, kBlock, st, a);
   if (!a.choose_only) LV_LAUNCH(hub_flag_kernel, grid, kBlock, st, a);
+}
+void k_hub_mid(const MoveArgs& a, hipStream_t st) {
+  if (a.g.w && a.f64_weights) LV_LAUNCH(hub_mid_kernel<double>, kMidGrid, kMidBlock, st, a);
+  else LV_LAUNCH(hub_mid_kernel<float>, kMidGrid, kMidBlock, st, a);
 }
 void k_exists(GView g, const uint32_t* vcom, uint32_t* flags, hipStream_t st) {
   if (g.S) LV_LAUNCH(exists_kernel, grid_for(g.S, kBlock * 4), kBlock, st, g, vcom, flags);
@@ -1715,22 +2017,23 @@ void k_bin_queue(const uint32_t* q, const uint32_t* qlen, uint64_t qcap, const u
   LV_LAUNCH(bin_queue_kernel, grid_for(qcap, kBlock, 2048), kBlock, st, q, qlen, deg, thr, tq, stride, tlen);
 }
 void k_max_deg(const uint32_t* ids, uint64_t n, const uint32_t* deg, uint32_t* out, hipStream_t st) {
-  if (n) LV_LAUNCH(max_deg_kernel, grid_for(n, kBlock, 1024), kBlock, st, ids, n, deg, out);
+  if (n) LV_LAUNCH(max_deg_kernel, grid_for(n, kBlock, kReduceGrid), kBlock, st, ids, n, deg, out);
 }
 void k_max_u32(const uint32_t* a, uint64_t n, uint32_t* out, hipStream_t st) {
-  if (n) LV_LAUNCH(max_u32_kernel, grid_for(n, kBlock * 8, 4096), kBlock, st, a, n, out);
+  if (n) LV_LAUNCH(max_u32_kernel, grid_for(n, kBlock * 8, kReduceGrid), kBlock, st, a, n, out);
 }
 void k_gather_u32(const uint32_t* a, const uint32_t* idx, uint64_t n, uint32_t* out, hipStream_t st) {
   if (n) LV_LAUNCH(gather_u32_kernel, grid_for(n, kBlock, 1024), kBlock, st, a, idx, n, out);
 }
 void k_caps_sum(const uint32_t* deg, uint64_t n, uint32_t thr, unsigned long long* out, hipStream_t st) {
-  if (n) LV_LAUNCH(caps_sum_kernel, grid_for(n, kBlock * 8, 4096), kBlock, st, deg, n, thr, out);
+  if (n) LV_LAUNCH(caps_sum_kernel, grid_for(n, kBlock * 8, kReduceGrid), kBlock, st, deg, n, thr, out);
 }
-void k_wl_queue(uint8_t* f, bool clear, uint64_t S, const uint32_t* deg, uint32_t thr, uint32_t* ql, uint32_t* qh,
-                uint32_t* lens, uint64_t* bsum, hipStream_t st) {
-  LV_LAUNCH(wl_count_kernel, kWlScanBlocks, kBlock, st, f, S, deg, thr, bsum);
+void k_wl_queue(uint8_t* f, bool clear, uint64_t S, const uint32_t* deg, uint32_t thr, uint32_t thr2,
+                const uint8_t* big, uint32_t* q0, uint32_t* q1, uint32_t* q2, uint32_t* lens, uint64_t* bsum,
+                hipStream_t st) {
+  LV_LAUNCH(wl_count_kernel, kWlScanBlocks, kBlock, st, f, S, deg, thr, thr2, big, bsum);
   LV_LAUNCH(wl_scan_kernel, 1, kBlock, st, bsum, lens);
-  LV_LAUNCH(wl_scatter_kernel, kWlScanBlocks, kBlock, st, f, (int)clear, S, deg, thr, bsum, ql, qh);
+  LV_LAUNCH(wl_scatter_kernel, kWlScanBlocks, kBlock, st, f, (int)clear, S, deg, thr, thr2, big, bsum, q0, q1, q2);
 }
 void k_modularity_parts(GView g, bool f64, const uint32_t* memb, double* cin, double* ctot, hipStream_t st) {
   if (!g.S) return;
@@ -1740,7 +2043,7 @@ void k_modularity_parts(GView g, bool f64, const uint32_t* memb, double* cin, do
 }
 void k_modularity_sum(const double* cin, const double* ctot, uint64_t n, double M, double R, double* out,
                       hipStream_t st) {
-  if (n) LV_LAUNCH(modularity_sum_kernel, grid_for(n, kBlock * 8, 4096), kBlock, st, cin, ctot, n, M, R, out);
+  if (n) LV_LAUNCH(modularity_sum_kernel, grid_for(n, kBlock * 8, kReduceGrid), kBlock, st, cin, ctot, n, M, R, out);
 }
 
 hipError_t scan_u32(const uint32_t* in, uint32_t* out, uint64_t n, void* tmp, size_t& tmp_bytes, hipStream_t st) {
