@@ -58,6 +58,8 @@ def host_info():
 
 HBM_PEAK_GBPS = 8000.0
 KERNEL_NAME = "move_kernel<64, 64, 512, float, 2>"
+SMALL_HUB_KERNEL = "hub_mid_kernel<256, 2048u, 1024u, float>"
+MID_HUB_KERNEL = "hub_mid_kernel<512, 2048u, 16384u, float>"
 PMC_SUMMARY = os.path.join(ROOT, "profiles", "move_kernel_df_rmat22_pmc.txt")
 METRIC = "DF-Louvain edges/s & p99 batch latency at dQ<=1e-3 vs plain GPU path"
 
@@ -95,15 +97,19 @@ def pmc_traffic(kernel):
     return None
 
 
-def run_chain(ctx, lvhip, dgraphs, dbatches, state, state0, W, K, path):
+def run_chain(ctx, lvhip, dgraphs, dbatches, state, state0, W, K, path, profile=False):
     """Warm up on batches [0, W), time batches [W, W + K); returns wall
-    seconds, per-step algorithm times and edge counts."""
+    seconds, per-step algorithm times, edge counts and kernel statistics.
+    With profile, every local-moving launch is bracketed by HIP events on
+    its stream (and launched one by one, not replayed from a graph): the
+    per-kernel durations of the roofline come from such a run, the
+    throughput from one without."""
     ctx.set_path(path)
     state.copy_from(state0)
     for k in range(W):
         ctx.run(lvhip.LV_DYNAMIC_FRONTIER, dgraphs[k], dbatches[k], state)
     ctx.stats_reset()
-    ctx.set_profiling(True)
+    ctx.set_profiling(profile)
     lat, edges, parts = [], 0, []
     t0 = time.perf_counter()
     for k in range(W, W + K):
@@ -508,10 +514,14 @@ def main():
     # Timed: the chosen path, then the others beside it.
     paths = {"frontier": lvhip.PATH_FRONTIER, "worklist": lvhip.PATH_WORKLIST, "plain": lvhip.PATH_SWEEP}
     dg = dgraphs[1:]
-    wall, lat, edges, st = run_chain(ctx, lvhip, dg, dbatches, state, state0, W, K, paths[args.path])
+    wall, lat, edges, _ = run_chain(ctx, lvhip, dg, dbatches, state, state0, W, K, paths[args.path])
     value = edges / wall
     log(f"DF {args.path}: {K} steps {wall * 1e3:.2f} ms wall, {value:.4g} edges/s, "
         f"p50 {percentile(lat, 50):.3f} ms p99 {percentile(lat, 99):.3f} ms")
+    # the same chain again with every local-moving launch timed by events
+    # (launched one by one): the per-kernel durations of the roofline
+    pwall, _, _, st = run_chain(ctx, lvhip, dg, dbatches, state, state0, W, K, paths[args.path], profile=True)
+    log(f"DF {args.path}, profiled (per-launch events, no graph replay): {pwall * 1e3:.2f} ms wall")
     others = {}
     if not args.no_plain:
         for name, path in paths.items():
@@ -548,9 +558,10 @@ def main():
         f"p99 {end_to_end['p99_ms']:.3f} ms; update alone p50 {end_to_end['update_p50_ms']:.3f} ms")
     dg_e2e.close()
 
-    # Roofline of the dominant local-moving kernel.  Stats slots: 0 = the
-    # id-order kernel (one launch per sub-round / sweep), 4 = the hub batch
-    # (eight kernels per launch); 1-3 are only used by the degree-tier modes.
+    # Roofline of the dominant local-moving kernel (largest total time).
+    # Stats slots on the worklist path: 0 = the id-order kernel (one launch
+    # per sub-round), 2 / 3 = the block-per-hub kernels (small / mid hubs),
+    # 4 = the hub batch (five kernels per launch).
     def kernel_line(slot, name):
         launches = max(st["move_launches"][slot], 1)
         ms = st["move_ms"][slot]
@@ -559,14 +570,18 @@ def main():
         return {"kernel": name, "achieved": gbps, "avg_launch_us": ms * 1e3 / launches,
                 "bytes_per_launch": b / launches, "launches": launches,
                 "per_launch": {k: st[k][slot] / launches for k in ("vertices", "edges", "candidates", "moves")}}
-    main_k = kernel_line(0, KERNEL_NAME)
-    hub_k = kernel_line(4, "hub batch (hub_sizes/scan/offsets/insert/self/best/apply/flag)")
-    traffic = pmc_traffic(KERNEL_NAME)
+    lines = {0: kernel_line(0, KERNEL_NAME), 2: kernel_line(2, SMALL_HUB_KERNEL),
+             3: kernel_line(3, MID_HUB_KERNEL),
+             4: kernel_line(4, "hub batch (hub_prep/insert/best/apply/flag)")}
+    dom = max((0, 2, 3), key=lambda k: st["move_ms"][k])
+    main_k, hub_k = lines[dom], lines[4]
+    traffic = pmc_traffic(main_k["kernel"])
     roofline = {"bound": "hbm", "achieved": main_k["achieved"], "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                 "frac": main_k["achieved"] / HBM_PEAK_GBPS, "traffic": traffic and traffic["bytes_per_launch"],
                 "kernel": main_k["kernel"], "avg_launch_us": main_k["avg_launch_us"],
                 "bytes_per_launch": main_k["bytes_per_launch"], "launches": main_k["launches"],
-                "traffic_source": traffic and traffic["source"], "hub_batch": hub_k}
+                "traffic_source": traffic and traffic["source"], "hub_batch": hub_k,
+                "other_kernels": [lines[k] for k in (0, 2, 3) if k != dom]}
     log(f"roofline: {main_k['kernel']}: {main_k['bytes_per_launch'] / 1e6:.2f} MB algorithmic per launch, "
         f"{main_k['avg_launch_us']:.1f} us -> {main_k['achieved']:.1f} GB/s; hub batch {hub_k['avg_launch_us']:.1f} us, "
         f"{hub_k['achieved']:.1f} GB/s; per launch " + ", ".join(

@@ -168,6 +168,10 @@ typedef struct lv_stats {
   uint64_t runs;
 } lv_stats;
 int lv_context_set_profiling(lv_context* ctx, int on);
+/* Worklist sub-round groups (the DF / DS first pass) replayed from a captured
+ * HIP graph (on by default; off while profiling, which times every launch).
+ * No counterpart in the reference: a launch-overhead switch of this build. */
+int lv_context_set_graphs(lv_context* ctx, int on);
 int lv_stats_get(const lv_context* ctx, lv_stats* out);
 int lv_stats_reset(lv_context* ctx);
 

@@ -142,10 +142,91 @@ struct lv_context {
     }
     return events[ev_used++];
   }
-  hipEvent_t record() {
+  hipEvent_t record(hipStream_t s = nullptr) {
     hipEvent_t e = event();
-    HIPCHK(hipEventRecord(e, st));
+    HIPCHK(hipEventRecord(e, s ? s : st));
     return e;
+  }
+  // Side streams of the worklist sub-rounds (created on first use, default
+  // priority): the hub kernels run there beside the low-degree kernel on st.
+  hipStream_t side[2] = {nullptr, nullptr};
+  hipEvent_t fork_ev = nullptr, join_ev[2] = {nullptr, nullptr};
+  void make_side() {
+    if (side[0]) return;
+    for (int k = 0; k < 2; ++k) {
+      HIPCHK(hipStreamCreateWithFlags(&side[k], hipStreamNonBlocking));
+      HIPCHK(hipEventCreateWithFlags(&join_ev[k], hipEventDisableTiming));
+    }
+    HIPCHK(hipEventCreateWithFlags(&fork_ev, hipEventDisableTiming));
+  }
+  // Worklist sub-round groups replayed from captured graphs (one launch
+  // instead of some forty).  A graph is keyed by everything its kernels and
+  // their launch shapes depend on; a key is captured when an earlier call
+  // used it too (a resident graph served again), and the last kGraphCache
+  // graphs are kept.  Other groups run their launches one by one.
+  static constexpr size_t kGraphCache = 8, kSeenKeys = 32;
+  int use_graphs = 1;
+  std::vector<std::pair<std::string, hipGraphExec_t>> graphs;  // most recent last
+  std::vector<std::pair<std::string, uint64_t>> seen;  // (key, call)
+  uint64_t wl_calls = 0;                                // worklist calls so far
+  // true if the group was launched from a graph; false: enqueue it directly
+  template <class F>
+  bool replay(const std::string& key, F enqueue) {
+    for (size_t k = 0; k < graphs.size(); ++k) {
+      if (graphs[k].first != key) continue;
+      auto hit = graphs[k];
+      graphs.erase(graphs.begin() + k);
+      graphs.push_back(hit);
+      HIPCHK(hipGraphLaunch(hit.second, st));
+      return true;
+    }
+    auto it = std::find_if(seen.begin(), seen.end(), [&](const auto& e) { return e.first == key; });
+    if (it == seen.end() || it->second == wl_calls) {  // new, or only seen in this call
+      if (it == seen.end()) {
+        if (seen.size() >= kSeenKeys) seen.erase(seen.begin());
+        seen.push_back({key, wl_calls});
+      }
+      return false;
+    }
+    seen.erase(it);
+    hipGraph_t g = nullptr;
+    HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    try {
+      enqueue();
+    } catch (...) {
+      (void)hipStreamEndCapture(st, &g);
+      if (g) (void)hipGraphDestroy(g);
+      throw;
+    }
+    HIPCHK(hipStreamEndCapture(st, &g));
+    hipGraphExec_t ex = nullptr;
+    hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    HIPCHK(e);
+    if (graphs.size() >= kGraphCache) {
+      (void)hipGraphExecDestroy(graphs.front().second);
+      graphs.erase(graphs.begin());
+    }
+    graphs.push_back({key, ex});
+    HIPCHK(hipGraphLaunch(ex, st));
+    return true;
+  }
+  void drop_graph() {
+    for (auto& g : graphs) (void)hipGraphExecDestroy(g.second);
+    graphs.clear();
+    seen.clear();
+  }
+  // side streams wait for the work enqueued on st so far
+  void fork() {
+    HIPCHK(hipEventRecord(fork_ev, st));
+    for (int k = 0; k < 2; ++k) HIPCHK(hipStreamWaitEvent(side[k], fork_ev, 0));
+  }
+  // st waits for the work enqueued on the side streams so far
+  void join() {
+    for (int k = 0; k < 2; ++k) {
+      HIPCHK(hipEventRecord(join_ev[k], side[k]));
+      HIPCHK(hipStreamWaitEvent(st, join_ev[k], 0));
+    }
   }
   void check() { HIPCHK(hipGetLastError()); }
 };
@@ -307,24 +388,29 @@ struct Counters {
 // Launch one tier of the local-moving kernel, bracketed by events when the
 // context profiles (per-launch durations on the context's own stream).
 constexpr int kTierHubBatch = 6;  // launch_move: hubs through k_hub_batch (stats slot 4)
-constexpr int kTierMidHub = 7;    // launch_move: worklist hubs through k_hub_mid (stats slot 3)
+constexpr int kTierMidHub = 7;    // launch_move: worklist mid hubs through k_hub_mid (stats slot 3)
+constexpr int kTierSmallHub = 8;  // launch_move: worklist small hubs through k_hub_mid (stats slot 2)
 constexpr int kSubRounds = 4;      // worklist sub-rounds enqueued per host check
 constexpr uint64_t kRefChunk = 2048;  // the reference's OpenMP chunk (inc/louvain.hxx:594)
 
-void launch_move(lv_context* c, const MoveArgs& a, int tier, bool f64) {
+void launch_move(lv_context* c, const MoveArgs& a, int tier, bool f64, hipStream_t s = nullptr) {
+  if (!s) s = c->st;
   // the chunk counter of chunks handed out in order (the hub batch and
   // spread chunks do not use it)
-  if (tier != kTierHubBatch && a.chunk) k_fill_u64(reinterpret_cast<uint64_t*>(a.work), 0, 1, c->st);
+  if (tier != kTierHubBatch && a.chunk) k_fill_u64(reinterpret_cast<uint64_t*>(a.work), 0, 1, s);
   hipEvent_t e0 = nullptr;
-  if (c->profiling) e0 = c->record();
-  if (tier == kTierHubBatch) k_hub_batch(a, c->st);
-  else if (tier == kTierMidHub) k_hub_mid(a, c->st);
+  if (c->profiling) e0 = c->record(s);
+  if (tier == kTierHubBatch) k_hub_batch(a, s);
+  else if (tier == kTierMidHub || tier == kTierSmallHub) k_hub_mid(a, tier == kTierMidHub ? 1 : 0, s);
   else {
-    if (tier == kTierHybrid && a.ctab) k_graph_chunks(a, c->st);
-    k_move(a, tier, f64, c->st);
+    if (tier == kTierHybrid && a.ctab) k_graph_chunks(a, s);
+    k_move(a, tier, f64, s);
   }
-  const int slot = tier == kTierHubBatch ? 4 : (tier == kTierMidHub ? 3 : (tier == kTierHybrid ? 0 : tier));
-  if (c->profiling) c->prof.push_back({slot, {e0, c->record()}});
+  const int slot = tier == kTierHubBatch ? 4
+                   : tier == kTierMidHub  ? 3
+                   : tier == kTierSmallHub ? 2
+                   : (tier == kTierHybrid ? 0 : tier);
+  if (c->profiling) c->prof.push_back({slot, {e0, c->record(s)}});
 }
 
 // Worklist queues: degrees above hub_min_wl are hubs; up to this one they
@@ -449,7 +535,8 @@ void id_order_regions(lv_context* c, MoveArgs& a, const Level& lv) {
 
 MoveArgs base_args(lv_context* c, const Level& lv, uint32_t* vcom, const double* vtot, double* ctot, uint8_t* vaff,
                    const uint8_t* fa, double M, double R, bool worklist = false) {
-  MoveArgs a{};
+  MoveArgs a;
+  std::memset(&a, 0, sizeof a);  // padding too: the bytes key captured launch graphs
   a.g = lv.g;
   a.vcom = vcom;
   a.vtot = vtot;
@@ -651,11 +738,11 @@ int local_move_worklist(lv_context* c, const Level& lv, uint32_t* vcom, const do
   k_fill_u64(reinterpret_cast<uint64_t*>(a.cnt), 0, (size_t)kSpread * kCntStride, c->st);
   unsigned long long* cnt0 = a.cnt;
   const uint64_t S = lv.g.S, stride = std::max<uint64_t>(S, 1), fbytes = (S + 7) & ~3ull;
-  uint32_t* ql = c->ws<uint32_t>("wl.low", stride);
-  uint32_t* qm = c->ws<uint32_t>("wl.mid", stride);
-  uint32_t* qh = c->ws<uint32_t>("wl.hub", stride);
-  uint32_t* lens = c->ws<uint32_t>("wl.len", 3);
-  uint64_t* bsum = c->ws<uint64_t>("wl.bsum", 3 * kWlScanBlocks);
+  // queues: low degree, small hubs, mid hubs, hub batch
+  WlQueues qs;
+  for (int q = 0; q < kWlQueues; ++q) qs.q[q] = c->ws<uint32_t>("wl.q" + std::to_string(q), stride);
+  uint32_t* lens = c->ws<uint32_t>("wl.len", kWlQueues);
+  uint64_t* bsum = c->ws<uint64_t>("wl.bsum", (size_t)kWlQueues * kWlScanBlocks);
   // next-sub-round flags: all zero between calls (each compaction resets
   // the flags it reads, and a sub-round loop ends on an empty compaction)
   bool fresh = false;
@@ -673,9 +760,65 @@ int local_move_worklist(lv_context* c, const Level& lv, uint32_t* vcom, const do
     c->stamp_span = S;
     c->serial = 0;
   }
-  const uint32_t thr = c->tier_order == 2 ? c->hub_min_wl : 0;
-  const uint32_t thr2 = wl_big_hub(c);
-  const uint8_t* big = c->tier_order == 2 ? a.hb_big : nullptr;  // hubs marked for the batch
+  WlSplit sp{};
+  if (c->tier_order == 2) {
+    const uint32_t thr = c->hub_min_wl;
+    sp.t[0] = thr;
+    sp.t[1] = std::max(thr, kSmallHubMax);
+    sp.t[2] = wl_big_hub(c);
+    sp.big = a.hb_big;  // hubs marked for the batch
+  }
+  // Sub-rounds are enqueued kSubRounds at a time (their kernels read the
+  // queue lengths on the device; an empty round costs its launches only),
+  // with one host check of the last compaction per group.  Without
+  // profiling a group is replayed from a captured graph; the iteration
+  // serial is then read on the device.
+  const bool graphs = c->use_graphs && !c->profiling;
+  ++c->wl_calls;
+  uint32_t* serial_dev = c->ws<uint32_t>("wl.serial", 1);
+  if (graphs) {
+    a.serial_dev = serial_dev;
+    if (S) c->make_side();
+  }
+  // one sub-round: hubs and the rest, then the next queue
+  auto sub_round = [&]() {
+    if (c->tier_order == 2) {
+      // id-order mode, three streams at once: the largest hubs (multi-block
+      // batch), the other hubs (a block each, mid then small), and the rest
+      // through the id-order kernel.  Like the reference's threads, each
+      // sees the moves the others have made.
+      if (S) {
+        c->make_side();
+        c->fork();
+        a.list = qs.q[3];
+        a.n_dev = lens + 3;
+        a.cnt = cnt0 + (size_t)4 * CNT_N;
+        launch_move(c, a, kTierHubBatch, lv.f64, c->side[0]);
+        for (int q = 2; q >= 1; --q) {  // stats slots 3 (mid) and 2 (small)
+          a.list = qs.q[q];
+          a.n_dev = lens + q;
+          a.cnt = cnt0 + (size_t)(q + 1) * CNT_N;
+          launch_move(c, a, q == 2 ? kTierMidHub : kTierSmallHub, lv.f64, c->side[1]);
+        }
+      }
+      a.list = qs.q[0];
+      a.n_dev = lens;
+      a.cnt = cnt0;
+      launch_move(c, a, kTierHybrid, lv.f64);
+      if (S) c->join();
+    } else {
+      k_fill_u32(tlen, 0, kTiers, c->st);
+      k_bin_queue(qs.q[0], lens, std::min<uint64_t>(S, 1u << 19), lv.g.deg, 0, tq, stride, tlen, c->st);
+      for (int k = (c->tier_order ? kTiers - 1 : 0); k >= 0 && k < kTiers; k += (c->tier_order ? -1 : 1)) {
+        if (k == 4 && !lv.maxdeg4) continue;
+        a.list = tq + (size_t)k * stride;
+        a.n_dev = tlen + k;
+        a.cnt = cnt0 + (size_t)k * CNT_N;
+        launch_move(c, a, k, lv.f64);
+      }
+    }
+    k_wl_queue(a.fsame, true, S, lv.g.deg, sp, qs, lens, bsum, c->st);
+  };
   int l = 0;
   double el = 0;
   for (; l < L;) {
@@ -688,45 +831,36 @@ int local_move_worklist(lv_context* c, const Level& lv, uint32_t* vcom, const do
     if (bt) k_fill_f64(bt->gel, 0, bt->G, c->st);
     else clear_el(c, a.el);
     // the iteration's queue: every flagged id, ascending
-    k_wl_queue(vaff, false, S, lv.g.deg, thr, thr2, big, ql, qm, qh, lens, bsum, c->st);
-    // Sub-rounds are enqueued kSubRounds at a time (their kernels read the
-    // queue lengths on the device; an empty round costs its launches only),
-    // with one host check of the last compaction per group.
-    for (int round = 0;; ++round) {
-      if (c->tier_order == 2) {
-        // id-order mode: the largest hubs (multi-block batch), the other
-        // hubs (a block each), then the rest through the id-order kernel
-        if (S) {
-          a.list = qh;
-          a.n_dev = lens + 2;
-          a.cnt = cnt0 + (size_t)4 * CNT_N;
-          launch_move(c, a, kTierHubBatch, lv.f64);
-          a.list = qm;
-          a.n_dev = lens + 1;
-          a.cnt = cnt0 + (size_t)3 * CNT_N;
-          launch_move(c, a, kTierMidHub, lv.f64);
-        }
-        a.list = ql;
-        a.n_dev = lens;
-        a.cnt = cnt0;
-        launch_move(c, a, kTierHybrid, lv.f64);
-      } else {
-        k_fill_u32(tlen, 0, kTiers, c->st);
-        k_bin_queue(ql, lens, std::min<uint64_t>(S, 1u << 19), lv.g.deg, 0, tq, stride, tlen, c->st);
-        for (int k = (c->tier_order ? kTiers - 1 : 0); k >= 0 && k < kTiers; k += (c->tier_order ? -1 : 1)) {
-          if (k == 4 && !lv.maxdeg4) continue;
-          a.list = tq + (size_t)k * stride;
-          a.n_dev = tlen + k;
-          a.cnt = cnt0 + (size_t)k * CNT_N;
-          launch_move(c, a, k, lv.f64);
-        }
-      }
-      k_wl_queue(a.fsame, true, S, lv.g.deg, thr, thr2, big, ql, qm, qh, lens, bsum, c->st);
-      if ((round + 1) % kSubRounds) continue;
+    k_wl_queue(vaff, false, S, lv.g.deg, sp, qs, lens, bsum, c->st);
+    std::string key;
+    if (graphs) {
+      k_fill_u32(serial_dev, c->serial, 1, c->st);
+      // what the captured launches depend on (the serial is read on the device)
+      MoveArgs ka = a;
+      ka.serial = 0;
+      ka.list = nullptr;
+      ka.n_dev = nullptr;
+      ka.cnt = cnt0;
+      const int flags[4] = {c->tier_order, lv.f64 ? 1 : 0, (int)lv.maxdeg4, kSubRounds};
+      key.append(reinterpret_cast<const char*>(&ka), sizeof ka);
+      key.append(reinterpret_cast<const char*>(&qs), sizeof qs);
+      key.append(reinterpret_cast<const char*>(&sp), sizeof sp);
+      const void* ptrs[5] = {lens, bsum, tq, tlen, c->side[0]};
+      key.append(reinterpret_cast<const char*>(ptrs), sizeof ptrs);
+      key.append(reinterpret_cast<const char*>(flags), sizeof flags);
+      key.append(reinterpret_cast<const char*>(&S), sizeof S);
+    }
+    for (;;) {
+      auto group = [&] {
+        for (int r = 0; r < kSubRounds; ++r) sub_round();
+      };
+      if (!graphs || !c->replay(key, group)) group();
       c->check();
-      uint32_t n3[3];
-      c->read(n3, lens, 3);
-      if (n3[0] + n3[1] + n3[2] == 0) break;
+      uint32_t nq[kWlQueues];
+      c->read(nq, lens, kWlQueues);
+      uint64_t left = 0;
+      for (int q = 0; q < kWlQueues; ++q) left += nq[q];
+      if (!left) break;
     }
     ++l;
     if (bt) {
@@ -1604,9 +1738,11 @@ void apply_update(lv_context* c, lv_dgraph* d, const lv_dbatch* b) {
   k_upd_delta(b->iu, b->ni, +1, delta, touched, c->st);
   // touched rows, ascending
   uint32_t* T = c->ws<uint32_t>("up.T", S);
-  uint32_t* lens = c->ws<uint32_t>("up.lens", 3);
-  uint64_t* bsum = c->ws<uint64_t>("up.bsum", 3 * kWlScanBlocks);
-  k_wl_queue(touched, false, S, d->deg, 0, 0, nullptr, T, nullptr, nullptr, lens, bsum, c->st);  // no degree split
+  uint32_t* lens = c->ws<uint32_t>("up.lens", kWlQueues);
+  uint64_t* bsum = c->ws<uint64_t>("up.bsum", (size_t)kWlQueues * kWlScanBlocks);
+  WlQueues qs{};
+  qs.q[0] = T;
+  k_wl_queue(touched, false, S, d->deg, WlSplit{}, qs, lens, bsum, c->st);  // no degree split
   const uint64_t R = c->read1(lens);
   uint64_t* so = c->ws<uint64_t>("up.so", R + 1);
   uint64_t* sn = c->ws<uint64_t>("up.sn", R + 1);
@@ -1799,8 +1935,17 @@ void lv_context_destroy(lv_context* c) {
   if (c->st) (void)hipStreamSynchronize(c->st);
   for (auto& kv : c->bufs)
     if (kv.second.p) (void)hipFree(kv.second.p);
+  c->drop_graph();
   for (auto e : c->events) (void)hipEventDestroy(e);
   if (c->pinned) (void)hipHostFree(c->pinned);
+  for (int k = 0; k < 2; ++k) {
+    if (c->side[k]) {
+      (void)hipStreamSynchronize(c->side[k]);
+      (void)hipStreamDestroy(c->side[k]);
+    }
+    if (c->join_ev[k]) (void)hipEventDestroy(c->join_ev[k]);
+  }
+  if (c->fork_ev) (void)hipEventDestroy(c->fork_ev);
   if (c->st) (void)hipStreamDestroy(c->st);
   delete c;
 }
@@ -1836,6 +1981,13 @@ int lv_context_set_hub_threshold(lv_context* c, uint32_t sweep_degree, uint32_t 
 int lv_context_set_pick_less(lv_context* c, int iterations) {
   if (!c || iterations < 0) return LV_ERR_INVALID;
   c->pick_less_iters = iterations;
+  return LV_OK;
+}
+
+int lv_context_set_graphs(lv_context* c, int on) {
+  if (!c) return LV_ERR_INVALID;
+  c->use_graphs = on ? 1 : 0;
+  if (!on) c->drop_graph();
   return LV_OK;
 }
 

@@ -31,20 +31,36 @@ constexpr uint64_t kHubTabChunk = 1024;
 constexpr uint64_t kHubChunkEdges = 1024;
 // worklist hub batches: table slots a hub starts with (a multiple of kHubTabChunk)
 constexpr uint64_t kHubTabCap = 8192;
-// Worklist hubs up to kMidHubMax edges: one block of kMidBlock threads per
-// hub with its table (nextpow2(2 deg) slots, at most kMidSlots = 64 KB) in
-// LDS, kScanBatch x kMidBlock edges loaded at a time; a grid of kMidGrid
-// blocks (two per CU) walks the queue.  Larger hubs, and those with more
-// distinct neighbour communities than the table holds, go to the
-// multi-block hub batch.
+// Worklist hubs run one block per hub with the hub's table in LDS, in two
+// classes: up to kSmallHubMax edges (blocks of kSmallHubBlock threads,
+// nextpow2(2 deg) <= kSmallHubSlots slots = 32 KB, kSmallHubGrid blocks)
+// and up to kMidHubMax edges (kMidBlock threads, at most kMidSlots slots =
+// 32 KB, kMidGrid blocks); kScanBatch x block edges are loaded at a time.  A hub's
+// neighbours fall in few communities (tens, against thousands of edges, on
+// R-MAT after a static run), so the tables are small -- LDS is what limits
+// how many hubs run at once.  Larger hubs, and those with more distinct
+// neighbour communities than their table holds, go to the multi-block hub
+// batch.
+constexpr uint32_t kSmallHubMax = 1024;
+constexpr uint32_t kSmallHubSlots = 2 * kSmallHubMax;  // never capped
+constexpr int kSmallHubBlock = 256;
+constexpr uint64_t kSmallHubGrid = 1024;
 constexpr uint32_t kMidHubMax = 16384;
-constexpr uint32_t kMidSlots = 4096;
+constexpr uint32_t kMidSlots = 2048;
 constexpr int kMidBlock = 512;
-constexpr int kScanBatch = 8;
+constexpr int kScanBatch = 4;
 constexpr uint64_t kMidGrid = 512;
-static_assert(kMidSlots > kMidBlock, "hub_mid_kernel: room for a slot per thread past the distinct limit");
 // worklist queue compaction: blocks (contiguous id segments) per pass
 constexpr int kWlScanBlocks = 1024;
+// worklist queues: low degree, small hubs, mid hubs, hub batch
+constexpr int kWlQueues = 4;
+struct WlSplit {
+  uint32_t t[kWlQueues - 1];  // degree bounds of queues 0..2
+  const uint8_t* big;         // ids sent to the hub batch (nullable)
+};
+struct WlQueues {
+  uint32_t* q[kWlQueues];
+};
 constexpr int kTierHybrid = 5;
 
 inline int tier_of(uint64_t d) {
@@ -129,6 +145,7 @@ struct MoveArgs {
                                //   visited this iteration -> next sub-round
   uint32_t* stamp;             // worklist mode: iteration serial of the last visit
   uint32_t serial;
+  const uint32_t* serial_dev;  // if set, the serial is read from here on the device
   int pick_less;               // 1: only moves to a lower community id
   uint64_t min_chunk;          // unused (kept zero)
   uint64_t max_streams;        // lane groups launched (streams walking at once)
@@ -178,9 +195,9 @@ struct MoveArgs {
 // and its edges split over many blocks; the reference's per-vertex steps in
 // separate device passes (table build, best gain, move, neighbour flags).
 void k_hub_batch(const MoveArgs& a, hipStream_t st);
-// Worklist hubs of degree hub_min < d <= kMidHubMax from a.list (length
-// *a.n_dev), one block each with an LDS table.
-void k_hub_mid(const MoveArgs& a, hipStream_t st);
+// Worklist hubs from a.list (length *a.n_dev), one block each with an LDS
+// table: class 0 up to kSmallHubMax edges, class 1 up to kMidHubMax.
+void k_hub_mid(const MoveArgs& a, int cls, hipStream_t st);
 
 // Kernel launchers (lv_kernels.hip).  All run on `st`.
 void k_fill_u8(uint8_t* a, uint8_t v, uint64_t n, hipStream_t st);
@@ -266,14 +283,14 @@ void k_max_deg(const uint32_t* ids, uint64_t n, const uint32_t* deg, uint32_t* o
 void k_max_u32(const uint32_t* a, uint64_t n, uint32_t* out, hipStream_t st);
 void k_gather_u32(const uint32_t* a, const uint32_t* idx, uint64_t n, uint32_t* out, hipStream_t st);
 void k_caps_sum(const uint32_t* deg, uint64_t n, uint32_t thr, unsigned long long* out, hipStream_t st);
-// Ordered worklist queues: ids with f[id] set, ascending, degree <= thr to
-// q0, thr < degree <= thr2 to q1 and above thr2 to q2, as are the ids above
-// thr marked in big (nullable) (thr == 0: all to q0),
-// counts to lens[0..2]; clear resets the flags read.  f holds S bytes
-// rounded up to a multiple of 4; bsum holds 3 * kWlScanBlocks entries.
-void k_wl_queue(uint8_t* f, bool clear, uint64_t S, const uint32_t* deg, uint32_t thr, uint32_t thr2,
-                const uint8_t* big, uint32_t* q0, uint32_t* q1, uint32_t* q2, uint32_t* lens, uint64_t* bsum,
-                hipStream_t st);
+// Ordered worklist queues: ids with f[id] set, ascending, split by degree
+// into kWlQueues queues (queue q takes degrees in (t[q-1], t[q]], the last
+// one the rest; t[0] == 0 puts every id in queue 0, and the ids of queues 1
+// and 2 marked in big go to queue 3), counts to lens[0..3]; clear resets the
+// flags read.  f holds S bytes rounded up to a multiple of 4; bsum holds
+// kWlQueues * kWlScanBlocks entries.
+void k_wl_queue(uint8_t* f, bool clear, uint64_t S, const uint32_t* deg, const WlSplit& sp, const WlQueues& qs,
+                uint32_t* lens, uint64_t* bsum, hipStream_t st);
 void k_modularity_parts(GView g, bool f64, const uint32_t* memb, double* cin, double* ctot, hipStream_t st);
 void k_modularity_sum(const double* cin, const double* ctot, uint64_t n, double M, double R, double* out,
                       hipStream_t st);

@@ -173,13 +173,17 @@ __device__ __forceinline__ void add_counts(unsigned long long* cnt, unsigned lon
   if (me) atomicAdd(&s[CNT_MOVED_EDGES], me);
 }
 
+// The worklist iteration serial (device-resident when the launch sequence
+// is replayed from a captured graph).
+__device__ __forceinline__ uint32_t cur_serial(const MoveArgs& a) { return a.serial_dev ? *a.serial_dev : a.serial; }
+
 // vaff[v] = 1, as a moved vertex u flags its neighbour v (inc/louvain.hxx:602).
 __device__ __forceinline__ void flag_vertex(uint8_t* vaff, uint32_t v, uint32_t u, const MoveArgs& a) {
   __hip_atomic_store(&vaff[v], (uint8_t)1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   // worklist mode: a vertex flagged by a lower id and not visited in this
   // iteration yet joins the next sub-round (its turn in the reference's
   // ascending sweep is still to come, inc/louvain.hxx:602 then :598)
-  if (a.fsame && v > u && __hip_atomic_load(&a.stamp[v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != a.serial)
+  if (a.fsame && v > u && __hip_atomic_load(&a.stamp[v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != cur_serial(a))
     __hip_atomic_store(&a.fsame[v], (uint8_t)1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
@@ -567,7 +571,7 @@ __global__ __launch_bounds__(BLOCK) void move_kernel(MoveArgs a) {
     // One vertex (offset o, degree d, community cu, weight Ku) through the
     // group; every thread of the block calls it.
     auto visit = [&](bool act, uint32_t u, uint64_t o, uint32_t d, uint32_t cu, double Ku) __attribute__((always_inline)) {
-      if (act && a.stamp && lane == 0) st_dev(&a.stamp[u], a.serial);
+      if (act && a.stamp && lane == 0) st_dev(&a.stamp[u], cur_serial(a));
       bool hbm = MODE == MOVE_HBM;
       uint64_t cap = CAP;
       if (act) {
@@ -845,7 +849,7 @@ __global__ __launch_bounds__(kPrepBlock) void hub_prep_kernel(MoveArgs a) {
       a.hb_cap[k] = pc;
       a.hb_chunks[k] = pk;
       if (act[j]) {
-        if (a.stamp) st_dev(&a.stamp[u[j]], a.serial);
+        if (a.stamp) st_dev(&a.stamp[u[j]], cur_serial(a));
         a.hb_cu[k] = cu[j];
         a.hb_sd[k] = sd[j];
         a.hb_vd[k] = 0;
@@ -1049,20 +1053,22 @@ __global__ __launch_bounds__(kBlock) void hub_flag_kernel(MoveArgs a) {
 }
 
 // louvainMoveOmpW's body (inc/louvain.hxx:595-605) for worklist hubs of
-// moderate degree: one block per hub, its neighbour-community table in LDS.
-// The blocks walk the queue a.list (length *a.n_dev) in strides of the grid.
-// Per hub: sum edge weights per neighbour community (self-loops skipped,
-// first position kept for the first-seen tie-break), pick the best positive
-// gain, and unless it is community 0 move there (atomic Sigma_tot) and flag
-// the neighbours; clear the hub's flag and add its gain to el.
+// moderate degree (MIN < deg <= MAXD): one block of BLOCK threads per hub,
+// its neighbour-community table in LDS.  Block b takes queue entries b,
+// b + grid, b + 2 grid, ... of a.list (length *a.n_dev), reading the row,
+// community and K of BLOCK of them at once ahead of their turn (they change
+// only when the hub itself moves).  Per hub: sum edge weights per neighbour
+// community (self-loops skipped, first position kept for the first-seen
+// tie-break), pick the best positive gain, and unless it is community 0
+// move there (atomic Sigma_tot) and flag the neighbours; clear the hub's
+// flag and add its gain to el.
 //
-// The table has nextpow2(2 deg) slots up to kMidSlots.  A hub of more than
-// kMidSlots / 2 edges may have more distinct neighbour communities than
-// that: its distinct count is kept, and past kMidDistinct (the table then
-// still has a free slot for every thread of the block) the hub is left
-// flagged, marked for a full table (hb_big) and sent to the next sub-round,
-// where it runs in the multi-block batch.
-constexpr uint32_t kMidDistinct = kMidSlots - kMidBlock;
+// The table has nextpow2(2 deg) slots up to SLOTS.  A hub of more than
+// SLOTS / 2 edges may have more distinct neighbour communities than that:
+// its distinct count is kept, and past SLOTS - BLOCK (the table then still
+// has a free slot for every thread) the hub is left flagged, marked for a
+// full table (hb_big) and sent to the next sub-round, where it runs in the
+// multi-block batch.
 
 // tab_add_wave with a count of the slots it creates.
 __device__ __forceinline__ void tab_add_wave_counted(uint32_t* key, uint32_t* pos, double* val, uint32_t cap,
@@ -1094,147 +1100,165 @@ __device__ __forceinline__ void tab_add_wave_counted(uint32_t* key, uint32_t* po
   }
 }
 
-template <class WT>
-__global__ __launch_bounds__(kMidBlock) void hub_mid_kernel(MoveArgs a) {
-  __shared__ uint32_t s_key[kMidSlots], s_pos[kMidSlots];
-  __shared__ double s_val[kMidSlots];
-  __shared__ uint32_t s_u, s_d, s_cu, s_made;
-  __shared__ uint64_t s_o;
-  __shared__ double s_ku, s_sd;
-  __shared__ int s_act;
+template <int BLOCK, uint32_t SLOTS, uint32_t MAXD, class WT>
+__global__ __launch_bounds__(BLOCK) void hub_mid_kernel(MoveArgs a) {
+  constexpr uint32_t kDistinct = SLOTS - BLOCK;
+  constexpr int NB = kScanBatch;  // edges per thread loaded at a time
+  constexpr int AHEAD = 64;       // hubs read ahead at a time
+  static_assert(SLOTS > BLOCK, "room for a slot per thread past the distinct limit");
+  static_assert(AHEAD <= BLOCK, "one thread per hub read ahead");
+  __shared__ uint32_t s_key[SLOTS], s_pos[SLOTS];
+  __shared__ double s_val[SLOTS];
+  __shared__ uint32_t p_u[AHEAD], p_d[AHEAD], p_cu[AHEAD];
+  __shared__ uint64_t p_o[AHEAD];
+  __shared__ double p_ku[AHEAD];
+  __shared__ uint32_t s_made;
   const uint64_t n = list_len(a);
+  const uint64_t G = gridDim.x;
   const WT* W = (const WT*)a.g.w;
   double el_acc = 0;
   unsigned long long c_vert = 0, c_edge = 0, c_cand = 0, c_move = 0, c_medge = 0;
-  for (uint64_t k = blockIdx.x; k < n; k += gridDim.x) {
-    // the hub's scalars, read once and shared (flags change under us)
-    if (threadIdx.x == 0) {
-      const uint32_t u = a.list[k];
-      const uint32_t d = a.g.deg[u];
-      int act = d > a.hub_min && d <= kMidHubMax && (!a.g.ex || a.g.ex[u]) && !vertex_skipped(a, u) &&
-                (a.choose_only || (ld_dev(&a.vaff[u]) && (!a.fa || a.fa[u])));
-      s_u = u;
-      s_d = d;
-      s_act = act;
-      s_made = 0;
-      if (act) {
-        if (a.stamp) st_dev(&a.stamp[u], a.serial);
-        const uint32_t cu = ld_dev(&a.vcom[u]);
-        s_cu = cu;
-        s_o = a.g.off[u];
-        s_ku = a.vtot[u];
-        s_sd = ld_dev(&a.ctot[cu]);
+  for (uint64_t base = blockIdx.x; base < n; base += G * AHEAD) {
+    if (threadIdx.x < AHEAD) {  // thread j reads hub base + j G (degree 0: nothing to do)
+      const uint64_t k = base + threadIdx.x * G;
+      uint32_t u = 0, d = 0;
+      if (k < n) {
+        u = a.list[k];
+        d = a.g.deg[u];
+        const bool act = d > a.hub_min && d <= MAXD && (!a.g.ex || a.g.ex[u]) && !vertex_skipped(a, u) &&
+                         (a.choose_only || (ld_dev(&a.vaff[u]) && (!a.fa || a.fa[u])));
+        if (!act) d = 0;
+      }
+      p_u[threadIdx.x] = u;
+      p_d[threadIdx.x] = d;
+      if (d) {
+        p_cu[threadIdx.x] = ld_dev(&a.vcom[u]);
+        p_o[threadIdx.x] = a.g.off[u];
+        p_ku[threadIdx.x] = a.vtot[u];
       }
     }
     __syncthreads();
-    if (!s_act) {
+    const uint64_t J = min((uint64_t)AHEAD, (n - base + G - 1) / G);
+    for (uint64_t j = 0; j < J; ++j) {
+      const uint32_t d = p_d[j];
+      if (!d) continue;  // block-uniform
+      const uint32_t u = p_u[j], cu = p_cu[j];
+      const uint64_t o = p_o[j];
+      const double Ku = p_ku[j];
+      if (threadIdx.x == 0) {
+        if (a.stamp) st_dev(&a.stamp[u], cur_serial(a));
+        s_made = 0;
+      }
+      uint32_t cap = 16;
+      while (cap < 2 * d && cap < SLOTS) cap <<= 1;
+      const bool capped = cap < 2 * d;  // more edges than the table has room for as distinct keys
+      for (uint32_t s = threadIdx.x; s < cap; s += BLOCK) {
+        s_key[s] = kEmpty;
+        s_pos[s] = kNoPos;
+        s_val[s] = 0;
+      }
       __syncthreads();
-      continue;
-    }
-    const uint32_t u = s_u, d = s_d, cu = s_cu;
-    const uint64_t o = s_o;
-    const double Ku = s_ku;
-    uint32_t cap = 16;
-    while (cap < 2 * d && cap < kMidSlots) cap <<= 1;
-    const bool capped = cap < 2 * d;  // more edges than the table has room for as distinct keys
-    for (uint32_t s = threadIdx.x; s < cap; s += kMidBlock) {
-      s_key[s] = kEmpty;
-      s_pos[s] = kNoPos;
-      s_val[s] = 0;
-    }
-    __syncthreads();
-    // kScanBatch rounds of kMidBlock edges at a time: every key and weight
-    // load, then every community gather, then the inserts (a thread makes at
-    // most one new slot per round after reading the distinct count, so a
-    // capped table never fills)
-    bool over = false;
-    for (uint32_t b0 = 0; b0 < d && !over; b0 += kScanBatch * kMidBlock) {  // block-uniform
-      uint32_t v[kScanBatch], c[kScanBatch];
-      double w[kScanBatch];
+      // NB rounds of BLOCK edges at a time: every key and weight load, then
+      // every community gather, then the inserts (a thread makes at most one
+      // new slot per round after reading the distinct count, so a capped
+      // table never fills).  The last batch's keys stay for the flags.
+      bool over = false;
+      uint32_t v[NB];
+      for (uint32_t b0 = 0; b0 < d && !over; b0 += NB * BLOCK) {  // block-uniform
+        uint32_t c[NB];
+        double w[NB];
 #pragma unroll
-      for (int r = 0; r < kScanBatch; ++r) {
-        const uint32_t i = b0 + r * kMidBlock + threadIdx.x;
-        v[r] = i < d ? a.g.keys[o + i] : u;
-        w[r] = i < d ? (double)W[o + i] : 0.0;
-      }
-#pragma unroll
-      for (int r = 0; r < kScanBatch; ++r) c[r] = v[r] != u ? ld_dev(&a.vcom[v[r]]) : 0;
-#pragma unroll
-      for (int r = 0; r < kScanBatch; ++r) {
-        if (b0 + r * kMidBlock < d) {  // block-uniform
-          if (capped && __hip_atomic_load(&s_made, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) >= kMidDistinct)
-            over = true;
-          // self-loops are skipped (louvainScanCommunityW<SELF=false>)
-          tab_add_wave_counted(s_key, s_pos, s_val, cap, !over && v[r] != u, c[r], w[r],
-                               b0 + r * kMidBlock + threadIdx.x, &s_made);
+        for (int r = 0; r < NB; ++r) {
+          const uint32_t i = b0 + r * BLOCK + threadIdx.x;
+          v[r] = i < d ? a.g.keys[o + i] : u;
+          w[r] = i < d ? (double)W[o + i] : 0.0;
         }
+#pragma unroll
+        for (int r = 0; r < NB; ++r) c[r] = v[r] != u ? ld_dev(&a.vcom[v[r]]) : 0;
+#pragma unroll
+        for (int r = 0; r < NB; ++r) {
+          if (b0 + r * BLOCK < d) {  // block-uniform
+            if (capped && __hip_atomic_load(&s_made, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) >= kDistinct)
+              over = true;
+            // self-loops are skipped (louvainScanCommunityW<SELF=false>)
+            tab_add_wave_counted(s_key, s_pos, s_val, cap, !over && v[r] != u, c[r], w[r],
+                                 b0 + r * BLOCK + threadIdx.x, &s_made);
+          }
+        }
+        over = __syncthreads_or(over);
       }
-      over = __syncthreads_or(over);
-    }
-    if (over) {
-      // too many distinct communities for the table: no decision here
-      if (threadIdx.x == 0) {
-        a.hb_big[u] = 1;
-        a.fsame[u] = 1;
-      }
-      __syncthreads();
-      continue;
-    }
-    const double vd = tab_get(s_key, s_val, cap, cu), Sd = s_sd, Mu = vertex_M(a, u);
-    double be = 0;
-    uint32_t bp = kNoPos, bc = 0;
-    for (uint32_t s = threadIdx.x; s < cap; s += kMidBlock) {
-      const uint32_t c = s_key[s];
-      if (c == kEmpty) continue;
-      ++c_cand;
-      if (c == cu || (a.pick_less && c > cu)) continue;
-      const double e = delta_q(s_val[s], vd, Ku, ld_dev(&a.ctot[c]), Sd, Mu, a.R);
-      const uint32_t p = s_pos[s];
-      if (e > 0 && better(e, p, be, bp)) {
-        be = e;
-        bp = p;
-        bc = c;
-      }
-    }
-    block_best<kMidBlock>(be, bp, bc);
-    if (a.choose_only) {
-      if (threadIdx.x == 0) {
-        a.out_c[u] = bc;
-        a.out_e[u] = be;
-      }
-    } else {
-      // a candidate exists and it is not community 0 of u's graph
-      const bool moves = bp != kNoPos && bc != vertex_zero(a, u);
-      if (moves) {
+      if (over) {
+        // too many distinct communities for the table: no decision here
         if (threadIdx.x == 0) {
-          unsafeAtomicAdd(&a.ctot[cu], -Ku);
-          unsafeAtomicAdd(&a.ctot[bc], Ku);
-          st_dev(&a.vcom[u], bc);
-          ++c_move;
-          c_medge += d;
+          a.hb_big[u] = 1;
+          a.fsame[u] = 1;
         }
-        for (uint32_t i = threadIdx.x; i < d; i += kMidBlock) {
-          const uint32_t v = a.g.keys[o + i];
-          if (v != u) flag_vertex(a.vaff, v, u, a);
+        __syncthreads();
+        continue;
+      }
+      const double vd = tab_get(s_key, s_val, cap, cu), Sd = ld_dev(&a.ctot[cu]), Mu = vertex_M(a, u);
+      double be = 0;
+      uint32_t bp = kNoPos, bc = 0;
+      for (uint32_t s = threadIdx.x; s < cap; s += BLOCK) {
+        const uint32_t c = s_key[s];
+        if (c == kEmpty) continue;
+        ++c_cand;
+        if (c == cu || (a.pick_less && c > cu)) continue;
+        const double e = delta_q(s_val[s], vd, Ku, ld_dev(&a.ctot[c]), Sd, Mu, a.R);
+        const uint32_t p = s_pos[s];
+        if (e > 0 && better(e, p, be, bp)) {
+          be = e;
+          bp = p;
+          bc = c;
         }
       }
-      if (threadIdx.x == 0) {
-        st_dev(&a.vaff[u], (uint8_t)0);
-        if (a.gel) unsafeAtomicAdd(&a.gel[a.gid[u]], be);
-        else el_acc += be;
-        ++c_vert;
-        c_edge += d;
+      block_best<BLOCK>(be, bp, bc);
+      if (a.choose_only) {
+        if (threadIdx.x == 0) {
+          a.out_c[u] = bc;
+          a.out_e[u] = be;
+        }
+      } else {
+        // a candidate exists and it is not community 0 of u's graph
+        const bool moves = bp != kNoPos && bc != vertex_zero(a, u);
+        if (moves) {
+          if (threadIdx.x == 0) {
+            unsafeAtomicAdd(&a.ctot[cu], -Ku);
+            unsafeAtomicAdd(&a.ctot[bc], Ku);
+            st_dev(&a.vcom[u], bc);
+            ++c_move;
+            c_medge += d;
+          }
+          if (d <= NB * BLOCK) {  // one batch: its keys are still here
+#pragma unroll
+            for (int r = 0; r < NB; ++r)
+              if (r * BLOCK + threadIdx.x < d && v[r] != u) flag_vertex(a.vaff, v[r], u, a);
+          } else {
+            for (uint32_t i = threadIdx.x; i < d; i += BLOCK) {
+              const uint32_t x = a.g.keys[o + i];
+              if (x != u) flag_vertex(a.vaff, x, u, a);
+            }
+          }
+        }
+        if (threadIdx.x == 0) {
+          st_dev(&a.vaff[u], (uint8_t)0);
+          if (a.gel) unsafeAtomicAdd(&a.gel[a.gid[u]], be);
+          else el_acc += be;
+          ++c_vert;
+          c_edge += d;
+        }
       }
+      __syncthreads();  // the table is reused
     }
-    __syncthreads();  // the table and the shared scalars are reused
+    __syncthreads();  // the read-ahead arrays are reused
   }
   if (a.choose_only) return;
-  const double t = block_sum<kMidBlock>(el_acc);
+  const double t = block_sum<BLOCK>(el_acc);
   if (threadIdx.x == 0 && t != 0) unsafeAtomicAdd(a.el + spread_slot() * kElStride, t);
   if (a.cnt) {
-    unsigned long long v = block_sum_u64<kMidBlock>(c_vert), e = block_sum_u64<kMidBlock>(c_edge);
-    unsigned long long c = block_sum_u64<kMidBlock>(c_cand), m = block_sum_u64<kMidBlock>(c_move);
-    unsigned long long me = block_sum_u64<kMidBlock>(c_medge);
+    unsigned long long v = block_sum_u64<BLOCK>(c_vert), e = block_sum_u64<BLOCK>(c_edge);
+    unsigned long long c = block_sum_u64<BLOCK>(c_cand), m = block_sum_u64<BLOCK>(c_move);
+    unsigned long long me = block_sum_u64<BLOCK>(c_medge);
     if (threadIdx.x == 0) add_counts(a.cnt, v, e, c, m, me);
   }
 }
@@ -1679,7 +1703,6 @@ __global__ __launch_bounds__(kBlock) void caps_sum_kernel(const uint32_t* deg, u
 // the counts, then an ordered scatter; flags are read a 4-byte word per
 // thread.
 constexpr uint64_t kWlTile = 4 * kBlock;
-constexpr int kWlQueues = 3;
 static_assert(kWlScanBlocks % kBlock == 0, "wl_scan_kernel: whole block counts per thread");
 
 __device__ __forceinline__ uint64_t wl_segment(uint64_t S) {
@@ -1688,17 +1711,18 @@ __device__ __forceinline__ uint64_t wl_segment(uint64_t S) {
 }
 
 // Bit j of m[q]: id id0 + j is flagged and belongs to queue q.
-__device__ __forceinline__ bool wl_bits(const uint8_t* f, const uint32_t* deg, uint64_t S, uint32_t thr,
-                                        uint32_t thr2, const uint8_t* big, uint64_t id0,
-                                        uint32_t (&m)[kWlQueues]) {
-  m[0] = m[1] = m[2] = 0;
+__device__ __forceinline__ bool wl_bits(const uint8_t* f, const uint32_t* deg, uint64_t S, WlSplit sp,
+                                        uint64_t id0, uint32_t (&m)[kWlQueues]) {
+#pragma unroll
+  for (int q = 0; q < kWlQueues; ++q) m[q] = 0;
   uint32_t w = reinterpret_cast<const uint32_t*>(f)[id0 >> 2];
   if (!w) return false;
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
     if (id0 + j < S && ((w >> (8 * j)) & 0xFFu)) {
-      const uint32_t d = thr ? deg[id0 + j] : 0;
-      const int q = d <= thr ? 0 : ((d <= thr2 && !(big && big[id0 + j])) ? 1 : 2);
+      const uint32_t d = sp.t[0] ? deg[id0 + j] : 0;
+      int q = d <= sp.t[0] ? 0 : (d <= sp.t[1] ? 1 : (d <= sp.t[2] ? 2 : 3));
+      if (q == 1 || q == 2) q = sp.big && sp.big[id0 + j] ? 3 : q;
       m[q] |= 1u << j;
     }
   }
@@ -1706,13 +1730,12 @@ __device__ __forceinline__ bool wl_bits(const uint8_t* f, const uint32_t* deg, u
 }
 
 __global__ __launch_bounds__(kBlock) void wl_count_kernel(const uint8_t* f, uint64_t S, const uint32_t* deg,
-                                                          uint32_t thr, uint32_t thr2, const uint8_t* big,
-                                                          uint64_t* bsum) {
+                                                          WlSplit sp, uint64_t* bsum) {
   const uint64_t seg = wl_segment(S), lo = blockIdx.x * seg, hi = min(S, lo + seg);
-  unsigned long long c[kWlQueues] = {0, 0, 0};
+  unsigned long long c[kWlQueues] = {};
   for (uint64_t id0 = lo + 4 * (uint64_t)threadIdx.x; id0 < hi; id0 += kWlTile) {
     uint32_t m[kWlQueues];
-    if (!wl_bits(f, deg, S, thr, thr2, big, id0, m)) continue;
+    if (!wl_bits(f, deg, S, sp, id0, m)) continue;
 #pragma unroll
     for (int q = 0; q < kWlQueues; ++q) c[q] += __popc(m[q]);
   }
@@ -1743,25 +1766,22 @@ __global__ __launch_bounds__(kBlock) void wl_scan_kernel(uint64_t* bsum, uint32_
 }
 
 __global__ __launch_bounds__(kBlock) void wl_scatter_kernel(uint8_t* f, int clear, uint64_t S, const uint32_t* deg,
-                                                            uint32_t thr, uint32_t thr2, const uint8_t* big,
-                                                            const uint64_t* bsum, uint32_t* q0, uint32_t* q1,
-                                                            uint32_t* q2) {
+                                                            WlSplit sp, const uint64_t* bsum, WlQueues qs) {
   const uint64_t seg = wl_segment(S), lo = blockIdx.x * seg, hi = min(S, lo + seg);
-  uint32_t* const qs[kWlQueues] = {q0, q1, q2};
   uint64_t r[kWlQueues];
 #pragma unroll
   for (int q = 0; q < kWlQueues; ++q) r[q] = bsum[q * kWlScanBlocks + blockIdx.x];
   for (uint64_t t0 = lo; t0 < hi; t0 += kWlTile) {  // block-uniform trip count
     const uint64_t id0 = t0 + 4 * (uint64_t)threadIdx.x;
-    uint32_t m[kWlQueues] = {0, 0, 0};
-    const bool any = id0 < hi && wl_bits(f, deg, S, thr, thr2, big, id0, m);
+    uint32_t m[kWlQueues] = {};
+    const bool any = id0 < hi && wl_bits(f, deg, S, sp, id0, m);
 #pragma unroll
     for (int q = 0; q < kWlQueues; ++q) {
       uint64_t tq;
       uint64_t e = block_scan_excl<kBlock>(__popc(m[q]), tq);
 #pragma unroll
       for (int j = 0; j < 4; ++j)
-        if (m[q] & (1u << j)) qs[q][r[q] + e++] = (uint32_t)(id0 + j);
+        if (m[q] & (1u << j)) qs.q[q][r[q] + e++] = (uint32_t)(id0 + j);
       r[q] += tq;
     }
     if (clear && any) reinterpret_cast<uint32_t*>(f)[id0 >> 2] = 0;
@@ -1950,9 +1970,17 @@ void k_hub_batch(const MoveArgs& a, hipStream_t st) {
   LV_LAUNCH(hub_apply_kernel, 64, kBlock, st, a);
   if (!a.choose_only) LV_LAUNCH(hub_flag_kernel, grid, kBlock, st, a);
 }
-void k_hub_mid(const MoveArgs& a, hipStream_t st) {
-  if (a.g.w && a.f64_weights) LV_LAUNCH(hub_mid_kernel<double>, kMidGrid, kMidBlock, st, a);
-  else LV_LAUNCH(hub_mid_kernel<float>, kMidGrid, kMidBlock, st, a);
+void k_hub_mid(const MoveArgs& a, int cls, hipStream_t st) {
+  const bool f64 = a.g.w && a.f64_weights;
+  if (cls == 0) {
+    if (f64) LV_LAUNCH((hub_mid_kernel<kSmallHubBlock, kSmallHubSlots, kSmallHubMax, double>), kSmallHubGrid,
+                       kSmallHubBlock, st, a);
+    else LV_LAUNCH((hub_mid_kernel<kSmallHubBlock, kSmallHubSlots, kSmallHubMax, float>), kSmallHubGrid,
+                   kSmallHubBlock, st, a);
+  } else {
+    if (f64) LV_LAUNCH((hub_mid_kernel<kMidBlock, kMidSlots, kMidHubMax, double>), kMidGrid, kMidBlock, st, a);
+    else LV_LAUNCH((hub_mid_kernel<kMidBlock, kMidSlots, kMidHubMax, float>), kMidGrid, kMidBlock, st, a);
+  }
 }
 void k_exists(GView g, const uint32_t* vcom, uint32_t* flags, hipStream_t st) {
   if (g.S) LV_LAUNCH(exists_kernel, grid_for(g.S, kBlock * 4), kBlock, st, g, vcom, flags);
@@ -2028,12 +2056,11 @@ void k_gather_u32(const uint32_t* a, const uint32_t* idx, uint64_t n, uint32_t* 
 void k_caps_sum(const uint32_t* deg, uint64_t n, uint32_t thr, unsigned long long* out, hipStream_t st) {
   if (n) LV_LAUNCH(caps_sum_kernel, grid_for(n, kBlock * 8, kReduceGrid), kBlock, st, deg, n, thr, out);
 }
-void k_wl_queue(uint8_t* f, bool clear, uint64_t S, const uint32_t* deg, uint32_t thr, uint32_t thr2,
-                const uint8_t* big, uint32_t* q0, uint32_t* q1, uint32_t* q2, uint32_t* lens, uint64_t* bsum,
-                hipStream_t st) {
-  LV_LAUNCH(wl_count_kernel, kWlScanBlocks, kBlock, st, f, S, deg, thr, thr2, big, bsum);
+void k_wl_queue(uint8_t* f, bool clear, uint64_t S, const uint32_t* deg, const WlSplit& sp, const WlQueues& qs,
+                uint32_t* lens, uint64_t* bsum, hipStream_t st) {
+  LV_LAUNCH(wl_count_kernel, kWlScanBlocks, kBlock, st, f, S, deg, sp, bsum);
   LV_LAUNCH(wl_scan_kernel, 1, kBlock, st, bsum, lens);
-  LV_LAUNCH(wl_scatter_kernel, kWlScanBlocks, kBlock, st, f, (int)clear, S, deg, thr, thr2, big, bsum, q0, q1, q2);
+  LV_LAUNCH(wl_scatter_kernel, kWlScanBlocks, kBlock, st, f, (int)clear, S, deg, sp, bsum, qs);
 }
 void k_modularity_parts(GView g, bool f64, const uint32_t* memb, double* cin, double* ctot, hipStream_t st) {
   if (!g.S) return;

@@ -35,7 +35,7 @@ PATH_SWEEP, PATH_FRONTIER, PATH_WORKLIST = 0, 1, 2
 EXPORTS = [
     "lv_options_default", "lv_status_string", "lv_context_create", "lv_context_destroy", "lv_last_error",
     "lv_context_set_path", "lv_context_set_schedule", "lv_context_set_pick_less", "lv_context_set_slabs", "lv_context_set_hub_streams", "lv_context_set_hub_threshold",
-    "lv_context_set_profiling", "lv_stats_get",
+    "lv_context_set_profiling", "lv_context_set_graphs", "lv_stats_get",
     "lv_stats_reset", "lv_louvain_static", "lv_louvain_naive_dynamic", "lv_louvain_delta_screening",
     "lv_louvain_dynamic_frontier", "lv_dgraph_create", "lv_dgraph_destroy", "lv_dgraph_span", "lv_dgraph_edges",
     "lv_dbatch_create", "lv_dbatch_destroy", "lv_dstate_create", "lv_dstate_destroy", "lv_dstate_upload",
@@ -211,6 +211,10 @@ class Context:
         self._check(self.lib.lv_context_set_schedule(self.h, C.c_double(stream_fraction), C.c_uint64(min_streams),
                                                      C.c_uint64(max_streams), C.c_uint64(chunk), int(order)),
                     "lv_context_set_schedule")
+
+    def set_graphs(self, on=True):
+        """Replay worklist sub-round groups from captured HIP graphs (default on)."""
+        self._check(self.lib.lv_context_set_graphs(self.h, int(bool(on))), "lv_context_set_graphs")
 
     def set_profiling(self, on=True):
         self._check(self.lib.lv_context_set_profiling(self.h, int(bool(on))), "lv_context_set_profiling")
