@@ -748,6 +748,11 @@ int local_move_worklist(lv_context* c, const Level& lv, uint32_t* vcom, const do
   bool fresh = false;
   a.fsame = c->ws<uint8_t>("wl.same", fbytes, &fresh);
   if (fresh) k_fill_u8(a.fsame, 0, c->ws_bytes("wl.same"), c->st);
+  // per compaction segment: a next-sub-round flag may be set there (all zero
+  // between calls too; a compaction clears the marks of the segments it reads)
+  a.wl_dirty = c->ws<uint8_t>("wl.dirty", kWlScanBlocks, &fresh);
+  if (fresh) k_fill_u8(a.wl_dirty, 0, c->ws_bytes("wl.dirty"), c->st);
+  a.wl_shift = wl_shift(S);
   uint32_t* tq = nullptr;
   uint32_t* tlen = nullptr;
   if (c->tier_order != 2) {
@@ -768,6 +773,8 @@ int local_move_worklist(lv_context* c, const Level& lv, uint32_t* vcom, const do
     sp.t[2] = wl_big_hub(c);
     sp.big = a.hb_big;  // hubs marked for the batch
   }
+  WlSplit sp_same = sp;  // next-sub-round compactions read the marked segments only
+  sp_same.dirty = a.wl_dirty;
   // Sub-rounds are enqueued kSubRounds at a time (their kernels read the
   // queue lengths on the device; an empty round costs its launches only),
   // with one host check of the last compaction per group.  Without
@@ -817,7 +824,7 @@ int local_move_worklist(lv_context* c, const Level& lv, uint32_t* vcom, const do
         launch_move(c, a, k, lv.f64);
       }
     }
-    k_wl_queue(a.fsame, true, S, lv.g.deg, sp, qs, lens, bsum, c->st);
+    k_wl_queue(a.fsame, true, S, lv.g.deg, sp_same, qs, lens, bsum, c->st);
   };
   int l = 0;
   double el = 0;

@@ -57,7 +57,16 @@ constexpr int kWlQueues = 4;
 struct WlSplit {
   uint32_t t[kWlQueues - 1];  // degree bounds of queues 0..2
   const uint8_t* big;         // ids sent to the hub batch (nullable)
+  uint8_t* dirty;             // per segment: a flag may be set (nullable: scan all)
+  uint32_t shift;             // segment of block b: ids [b << shift, (b + 1) << shift)
 };
+// Segment shift of the compaction: 2^shift ids per block (a multiple of
+// the 4 x 256 ids a block reads per tile), at most kWlScanBlocks segments.
+inline uint32_t wl_shift(uint64_t S) {
+  uint32_t sh = 10;
+  while ((S + (1ull << sh) - 1) >> sh > (uint64_t)kWlScanBlocks) ++sh;
+  return sh;
+}
 struct WlQueues {
   uint32_t* q[kWlQueues];
 };
@@ -143,6 +152,8 @@ struct MoveArgs {
   uint32_t ng;
   uint8_t* fsame;              // worklist mode: flagged by a lower id and not yet
                                //   visited this iteration -> next sub-round
+  uint8_t* wl_dirty;           // worklist mode: per compaction segment, fsame may hold a flag
+  uint32_t wl_shift;           //   (segment of id v: v >> wl_shift)
   uint32_t* stamp;             // worklist mode: iteration serial of the last visit
   uint32_t serial;
   const uint32_t* serial_dev;  // if set, the serial is read from here on the device

@@ -173,6 +173,13 @@ __device__ __forceinline__ void add_counts(unsigned long long* cnt, unsigned lon
   if (me) atomicAdd(&s[CNT_MOVED_EDGES], me);
 }
 
+// v joins the next sub-round: its flag, and its compaction segment's
+// dirty mark (a segment without one is skipped by the compaction).
+__device__ __forceinline__ void mark_same(const MoveArgs& a, uint32_t v) {
+  __hip_atomic_store(&a.fsame[v], (uint8_t)1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (a.wl_dirty) a.wl_dirty[v >> a.wl_shift] = 1;
+}
+
 // The worklist iteration serial (device-resident when the launch sequence
 // is replayed from a captured graph).
 __device__ __forceinline__ uint32_t cur_serial(const MoveArgs& a) { return a.serial_dev ? *a.serial_dev : a.serial; }
@@ -184,7 +191,7 @@ __device__ __forceinline__ void flag_vertex(uint8_t* vaff, uint32_t v, uint32_t 
   // iteration yet joins the next sub-round (its turn in the reference's
   // ascending sweep is still to come, inc/louvain.hxx:602 then :598)
   if (a.fsame && v > u && __hip_atomic_load(&a.stamp[v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != cur_serial(a))
-    __hip_atomic_store(&a.fsame[v], (uint8_t)1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    mark_same(a, v);
 }
 
 // ---------------------------------------------------------------- fills
@@ -996,7 +1003,7 @@ __global__ __launch_bounds__(kBlock) void hub_apply_kernel(MoveArgs a) {
       // its capped table overflowed: no decision; it stays flagged, gets a
       // full table and joins the next sub-round
       a.hb_big[u] = 1;
-      a.fsame[u] = 1;
+      mark_same(a, u);
       continue;
     }
     double e = bp == kNoPos ? 0.0 : be;
@@ -1191,7 +1198,7 @@ __global__ __launch_bounds__(BLOCK) void hub_mid_kernel(MoveArgs a) {
         // too many distinct communities for the table: no decision here
         if (threadIdx.x == 0) {
           a.hb_big[u] = 1;
-          a.fsame[u] = 1;
+          mark_same(a, u);
         }
         __syncthreads();
         continue;
@@ -1705,10 +1712,7 @@ __global__ __launch_bounds__(kBlock) void caps_sum_kernel(const uint32_t* deg, u
 constexpr uint64_t kWlTile = 4 * kBlock;
 static_assert(kWlScanBlocks % kBlock == 0, "wl_scan_kernel: whole block counts per thread");
 
-__device__ __forceinline__ uint64_t wl_segment(uint64_t S) {
-  uint64_t seg = (S + gridDim.x - 1) / gridDim.x;
-  return (seg + kWlTile - 1) / kWlTile * kWlTile;
-}
+// ids of block b: [b << shift, (b + 1) << shift)
 
 // Bit j of m[q]: id id0 + j is flagged and belongs to queue q.
 __device__ __forceinline__ bool wl_bits(const uint8_t* f, const uint32_t* deg, uint64_t S, WlSplit sp,
@@ -1731,8 +1735,13 @@ __device__ __forceinline__ bool wl_bits(const uint8_t* f, const uint32_t* deg, u
 
 __global__ __launch_bounds__(kBlock) void wl_count_kernel(const uint8_t* f, uint64_t S, const uint32_t* deg,
                                                           WlSplit sp, uint64_t* bsum) {
-  const uint64_t seg = wl_segment(S), lo = blockIdx.x * seg, hi = min(S, lo + seg);
+  const uint64_t lo = (uint64_t)blockIdx.x << sp.shift, hi = min(S, lo + (1ull << sp.shift));
   unsigned long long c[kWlQueues] = {};
+  if (sp.dirty && !sp.dirty[blockIdx.x]) {  // no flag was set in this segment
+    if (threadIdx.x == 0)
+      for (int q = 0; q < kWlQueues; ++q) bsum[q * kWlScanBlocks + blockIdx.x] = 0;
+    return;
+  }
   for (uint64_t id0 = lo + 4 * (uint64_t)threadIdx.x; id0 < hi; id0 += kWlTile) {
     uint32_t m[kWlQueues];
     if (!wl_bits(f, deg, S, sp, id0, m)) continue;
@@ -1767,7 +1776,12 @@ __global__ __launch_bounds__(kBlock) void wl_scan_kernel(uint64_t* bsum, uint32_
 
 __global__ __launch_bounds__(kBlock) void wl_scatter_kernel(uint8_t* f, int clear, uint64_t S, const uint32_t* deg,
                                                             WlSplit sp, const uint64_t* bsum, WlQueues qs) {
-  const uint64_t seg = wl_segment(S), lo = blockIdx.x * seg, hi = min(S, lo + seg);
+  const uint64_t lo = (uint64_t)blockIdx.x << sp.shift, hi = min(S, lo + (1ull << sp.shift));
+  if (sp.dirty) {  // block-uniform
+    if (!sp.dirty[blockIdx.x]) return;
+    __syncthreads();  // every thread has read the mark
+    if (clear && threadIdx.x == 0) sp.dirty[blockIdx.x] = 0;
+  }
   uint64_t r[kWlQueues];
 #pragma unroll
   for (int q = 0; q < kWlQueues; ++q) r[q] = bsum[q * kWlScanBlocks + blockIdx.x];
@@ -2056,8 +2070,10 @@ void k_gather_u32(const uint32_t* a, const uint32_t* idx, uint64_t n, uint32_t* 
 void k_caps_sum(const uint32_t* deg, uint64_t n, uint32_t thr, unsigned long long* out, hipStream_t st) {
   if (n) LV_LAUNCH(caps_sum_kernel, grid_for(n, kBlock * 8, kReduceGrid), kBlock, st, deg, n, thr, out);
 }
-void k_wl_queue(uint8_t* f, bool clear, uint64_t S, const uint32_t* deg, const WlSplit& sp, const WlQueues& qs,
+void k_wl_queue(uint8_t* f, bool clear, uint64_t S, const uint32_t* deg, const WlSplit& sp_in, const WlQueues& qs,
                 uint32_t* lens, uint64_t* bsum, hipStream_t st) {
+  WlSplit sp = sp_in;
+  sp.shift = wl_shift(S);
   LV_LAUNCH(wl_count_kernel, kWlScanBlocks, kBlock, st, f, S, deg, sp, bsum);
   LV_LAUNCH(wl_scan_kernel, 1, kBlock, st, bsum, lens);
   LV_LAUNCH(wl_scatter_kernel, kWlScanBlocks, kBlock, st, f, (int)clear, S, deg, sp, bsum, qs);
