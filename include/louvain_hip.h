@@ -172,6 +172,12 @@ int lv_context_set_profiling(lv_context* ctx, int on);
  * HIP graph (on by default; off while profiling, which times every launch).
  * No counterpart in the reference: a launch-overhead switch of this build. */
 int lv_context_set_graphs(lv_context* ctx, int on);
+/* Hub neighbour-community tables kept with a resident state (on by default):
+ * from the second worklist DF / DS run of a state on, every vertex of more
+ * than 256 edges is evaluated from its table instead of its edges; batch
+ * edges and moves keep the tables exact.  No counterpart in the reference:
+ * an incremental index of this build. */
+int lv_context_set_hub_tables(lv_context* ctx, int on);
 int lv_stats_get(const lv_context* ctx, lv_stats* out);
 int lv_stats_reset(lv_context* ctx);
 
@@ -254,6 +260,12 @@ int lv_run_batched(lv_context* ctx, int approach, const lv_dgraph* g, const lv_d
 /* Modularity of a device state's membership on a device graph
  * (modularityByOmp, inc/properties.hxx:205-233). */
 int lv_dmodularity(lv_context* ctx, const lv_dgraph* g, const lv_dstate* s, double resolution, double* out);
+/* Diagnostic: rebuild the state's hub neighbour-community tables (see
+ * lv_context_set_hub_tables) from graph g and the state's membership and
+ * compare; *tables = tables compared, *mismatches = those that differ (0 and 0
+ * when the state has no valid tables).  No counterpart in the reference. */
+int lv_dstate_check_hub_tables(lv_context* ctx, const lv_dgraph* g, const lv_dstate* s, uint64_t* tables,
+                               uint64_t* mismatches);
 
 /* Phase entry points: each hot-path function of inc/louvain.hxx on its own,
  * host buffers in and out.  They run the same kernels as lv_run. */

@@ -160,15 +160,18 @@ struct lv_context {
     HIPCHK(hipEventCreateWithFlags(&fork_ev, hipEventDisableTiming));
   }
   // Worklist sub-round groups replayed from captured graphs (one launch
-  // instead of some forty).  A graph is keyed by everything its kernels and
-  // their launch shapes depend on; a key is captured when an earlier call
-  // used it too (a resident graph served again), and the last kGraphCache
-  // graphs are kept.  Other groups run their launches one by one.
+  // instead of some fifty, and no host gaps between them on the device).  A
+  // graph is keyed by everything its kernels and their launch shapes depend
+  // on.  A new key is captured and, when a graph exists already, written
+  // into the most recent one (hipGraphExecUpdate: the same launches with
+  // other arguments), else instantiated -- the latter only for a key an
+  // earlier call used too.  The last kGraphCache graphs are kept.
   static constexpr size_t kGraphCache = 8, kSeenKeys = 32;
   int use_graphs = 1;
+  int use_tables = 1;  // hub neighbour-community tables with resident states
   std::vector<std::pair<std::string, hipGraphExec_t>> graphs;  // most recent last
-  std::vector<std::pair<std::string, uint64_t>> seen;  // (key, call)
-  uint64_t wl_calls = 0;                                // worklist calls so far
+  std::vector<std::pair<std::string, uint64_t>> seen;          // (key, call)
+  uint64_t wl_calls = 0;                                       // worklist calls so far
   // true if the group was launched from a graph; false: enqueue it directly
   template <class F>
   bool replay(const std::string& key, F enqueue) {
@@ -181,14 +184,15 @@ struct lv_context {
       return true;
     }
     auto it = std::find_if(seen.begin(), seen.end(), [&](const auto& e) { return e.first == key; });
-    if (it == seen.end() || it->second == wl_calls) {  // new, or only seen in this call
+    const bool again = it != seen.end() && it->second != wl_calls;
+    if (graphs.empty() && !again) {  // nothing to update, and a key not used before
       if (it == seen.end()) {
         if (seen.size() >= kSeenKeys) seen.erase(seen.begin());
         seen.push_back({key, wl_calls});
       }
       return false;
     }
-    seen.erase(it);
+    if (it != seen.end()) seen.erase(it);
     hipGraph_t g = nullptr;
     HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     try {
@@ -199,6 +203,17 @@ struct lv_context {
       throw;
     }
     HIPCHK(hipStreamEndCapture(st, &g));
+    if (!graphs.empty()) {
+      hipGraphNode_t bad = nullptr;
+      hipGraphExecUpdateResult res;
+      if (hipGraphExecUpdate(graphs.back().second, g, &bad, &res) == hipSuccess) {
+        (void)hipGraphDestroy(g);
+        graphs.back().first = key;
+        HIPCHK(hipGraphLaunch(graphs.back().second, st));
+        return true;
+      }
+      (void)hipGetLastError();  // not updatable: instantiate it
+    }
     hipGraphExec_t ex = nullptr;
     hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
@@ -275,6 +290,26 @@ struct lv_dstate {
   uint32_t* memb = nullptr;
   double* vtot = nullptr;
   double* ctot = nullptr;
+  // Hub neighbour-community tables kept with the state (worklist DF / DS):
+  // valid while every membership change since their build went through the
+  // worklist first pass, which keeps them exact; ht_live is the live edge
+  // count of the graph they describe (a run checks that its graph is that
+  // graph plus its batch).  runs counts the runs since the last upload or
+  // copy: tables are built on the second run of a state.
+  uint64_t* ht_off = nullptr;
+  uint32_t* ht_cap = nullptr;
+  uint64_t ht_slots = 0;              // slab slots allocated
+  unsigned long long* ht_next = nullptr;  // first spare slot (device)
+  uint32_t* ht_key = nullptr;
+  int32_t* ht_cnt = nullptr;
+  double* ht_w = nullptr;
+  bool ht_valid = false;
+  uint64_t ht_live = 0;
+  uint64_t runs = 0;
+  void ht_reset() {
+    ht_valid = false;
+    runs = 0;
+  }
 };
 
 namespace {
@@ -390,6 +425,7 @@ struct Counters {
 constexpr int kTierHubBatch = 6;  // launch_move: hubs through k_hub_batch (stats slot 4)
 constexpr int kTierMidHub = 7;    // launch_move: worklist mid hubs through k_hub_mid (stats slot 3)
 constexpr int kTierSmallHub = 8;  // launch_move: worklist small hubs through k_hub_mid (stats slot 2)
+constexpr int kTierTabHub = 9;    // launch_move: worklist hubs with a table through k_hub_tab (stats slot 1)
 constexpr int kSubRounds = 4;      // worklist sub-rounds enqueued per host check
 constexpr uint64_t kRefChunk = 2048;  // the reference's OpenMP chunk (inc/louvain.hxx:594)
 
@@ -402,6 +438,7 @@ void launch_move(lv_context* c, const MoveArgs& a, int tier, bool f64, hipStream
   if (c->profiling) e0 = c->record(s);
   if (tier == kTierHubBatch) k_hub_batch(a, s);
   else if (tier == kTierMidHub || tier == kTierSmallHub) k_hub_mid(a, tier == kTierMidHub ? 1 : 0, s);
+  else if (tier == kTierTabHub) k_hub_tab(a, s);
   else {
     if (tier == kTierHybrid && a.ctab) k_graph_chunks(a, s);
     k_move(a, tier, f64, s);
@@ -409,6 +446,7 @@ void launch_move(lv_context* c, const MoveArgs& a, int tier, bool f64, hipStream
   const int slot = tier == kTierHubBatch ? 4
                    : tier == kTierMidHub  ? 3
                    : tier == kTierSmallHub ? 2
+                   : tier == kTierTabHub ? 1
                    : (tier == kTierHybrid ? 0 : tier);
   if (c->profiling) c->prof.push_back({slot, {e0, c->record(s)}});
 }
@@ -729,8 +767,15 @@ int local_move(lv_context* c, const Level& lv, uint32_t* vcom, const double* vto
 // two sub-round buffers; qlen[0..3] their lengths.
 int local_move_worklist(lv_context* c, const Level& lv, uint32_t* vcom, const double* vtot, double* ctot,
                         uint8_t* vaff, const uint8_t* fa, double M, double R, int L, double E, Counters& cnt,
-                        Batched* bt = nullptr) {
+                        Batched* bt = nullptr, lv_dstate* ht = nullptr) {
   MoveArgs a = base_args(c, lv, vcom, vtot, ctot, vaff, fa, M, R, true);
+  if (ht && c->tier_order == 2) {  // hubs with a neighbour-community table
+    a.ht_off = ht->ht_off;
+    a.ht_cap = ht->ht_cap;
+    a.ht_key = ht->ht_key;
+    a.ht_cnt = ht->ht_cnt;
+    a.ht_w = ht->ht_w;
+  }
   batched_args(a, bt);
   batched_schedule(c, a, lv, bt);
   if (bt) batched_begin(c, bt);
@@ -772,6 +817,7 @@ int local_move_worklist(lv_context* c, const Level& lv, uint32_t* vcom, const do
     sp.t[1] = std::max(thr, kSmallHubMax);
     sp.t[2] = wl_big_hub(c);
     sp.big = a.hb_big;  // hubs marked for the batch
+    sp.tab = a.ht_cap;  // hubs with a table
   }
   WlSplit sp_same = sp;  // next-sub-round compactions read the marked segments only
   sp_same.dirty = a.wl_dirty;
@@ -787,32 +833,56 @@ int local_move_worklist(lv_context* c, const Level& lv, uint32_t* vcom, const do
     a.serial_dev = serial_dev;
     if (S) c->make_side();
   }
+  // With hub tables and no hub without one, the edge-scanning hub kernels
+  // are not launched (a table dropped during the run sends its hub to them:
+  // they are launched from the next group on).
+  bool scan_hubs = true;
+  if (a.ht_cap) {
+    unsigned long long* nu = c->ws<unsigned long long>("ht.untabled", 1);
+    k_fill_u64(reinterpret_cast<uint64_t*>(nu), 0, 1, c->st);
+    k_untabled(lv.g.deg, a.ht_cap, S, c->hub_min_wl, nu, c->st);
+    scan_hubs = c->read1(nu) != 0;
+  }
   // one sub-round: hubs and the rest, then the next queue
   auto sub_round = [&]() {
     if (c->tier_order == 2) {
-      // id-order mode, three streams at once: the largest hubs (multi-block
-      // batch), the other hubs (a block each, mid then small), and the rest
-      // through the id-order kernel.  Like the reference's threads, each
-      // sees the moves the others have made.
+      // id-order mode, three streams at once: the low-degree vertices
+      // through the id-order kernel, hubs with a table (a wave each) then
+      // the other hubs (a block each, mid then small), and the largest hubs
+      // (multi-block batch).  Like the reference's threads, each sees the
+      // moves the others have made.  The low-degree kernel is enqueued
+      // first: the host's launches of the rest overlap it.
       if (S) {
         c->make_side();
         c->fork();
-        a.list = qs.q[3];
-        a.n_dev = lens + 3;
-        a.cnt = cnt0 + (size_t)4 * CNT_N;
-        launch_move(c, a, kTierHubBatch, lv.f64, c->side[0]);
-        for (int q = 2; q >= 1; --q) {  // stats slots 3 (mid) and 2 (small)
-          a.list = qs.q[q];
-          a.n_dev = lens + q;
-          a.cnt = cnt0 + (size_t)(q + 1) * CNT_N;
-          launch_move(c, a, q == 2 ? kTierMidHub : kTierSmallHub, lv.f64, c->side[1]);
-        }
       }
       a.list = qs.q[0];
       a.n_dev = lens;
       a.cnt = cnt0;
       launch_move(c, a, kTierHybrid, lv.f64);
-      if (S) c->join();
+      if (S) {
+        if (a.ht_cap) {  // stats slot 1
+          a.list = qs.q[4];
+          a.n_dev = lens + 4;
+          a.cnt = cnt0 + (size_t)1 * CNT_N;
+          launch_move(c, a, kTierTabHub, lv.f64, c->side[0]);
+        }
+        if (scan_hubs) {
+          for (int q = 2; q >= 1; --q) {  // stats slots 3 (mid) and 2 (small)
+            a.list = qs.q[q];
+            a.n_dev = lens + q;
+            a.cnt = cnt0 + (size_t)(q + 1) * CNT_N;
+            launch_move(c, a, q == 2 ? kTierMidHub : kTierSmallHub, lv.f64, c->side[0]);
+          }
+          a.list = qs.q[3];
+          a.n_dev = lens + 3;
+          a.cnt = cnt0 + (size_t)4 * CNT_N;
+          launch_move(c, a, kTierHubBatch, lv.f64, c->side[1]);
+        } else {
+          k_carry_hubs(a, qs, lens, c->side[0]);  // tables dropped meanwhile: kept for later
+        }
+        c->join();
+      }
     } else {
       k_fill_u32(tlen, 0, kTiers, c->st);
       k_bin_queue(qs.q[0], lens, std::min<uint64_t>(S, 1u << 19), lv.g.deg, 0, tq, stride, tlen, c->st);
@@ -839,16 +909,16 @@ int local_move_worklist(lv_context* c, const Level& lv, uint32_t* vcom, const do
     else clear_el(c, a.el);
     // the iteration's queue: every flagged id, ascending
     k_wl_queue(vaff, false, S, lv.g.deg, sp, qs, lens, bsum, c->st);
-    std::string key;
-    if (graphs) {
-      k_fill_u32(serial_dev, c->serial, 1, c->st);
-      // what the captured launches depend on (the serial is read on the device)
+    if (graphs) k_fill_u32(serial_dev, c->serial, 1, c->st);
+    // what the captured launches depend on (the serial is read on the device)
+    auto group_key = [&]() {
+      std::string key;
       MoveArgs ka = a;
       ka.serial = 0;
       ka.list = nullptr;
       ka.n_dev = nullptr;
       ka.cnt = cnt0;
-      const int flags[4] = {c->tier_order, lv.f64 ? 1 : 0, (int)lv.maxdeg4, kSubRounds};
+      const int flags[5] = {c->tier_order, lv.f64 ? 1 : 0, (int)lv.maxdeg4, kSubRounds, scan_hubs ? 1 : 0};
       key.append(reinterpret_cast<const char*>(&ka), sizeof ka);
       key.append(reinterpret_cast<const char*>(&qs), sizeof qs);
       key.append(reinterpret_cast<const char*>(&sp), sizeof sp);
@@ -856,18 +926,20 @@ int local_move_worklist(lv_context* c, const Level& lv, uint32_t* vcom, const do
       key.append(reinterpret_cast<const char*>(ptrs), sizeof ptrs);
       key.append(reinterpret_cast<const char*>(flags), sizeof flags);
       key.append(reinterpret_cast<const char*>(&S), sizeof S);
-    }
+      return key;
+    };
     for (;;) {
       auto group = [&] {
         for (int r = 0; r < kSubRounds; ++r) sub_round();
       };
-      if (!graphs || !c->replay(key, group)) group();
+      if (!graphs || !c->replay(group_key(), group)) group();
       c->check();
       uint32_t nq[kWlQueues];
       c->read(nq, lens, kWlQueues);
       uint64_t left = 0;
       for (int q = 0; q < kWlQueues; ++q) left += nq[q];
       if (!left) break;
+      if (nq[1] + nq[2] + nq[3]) scan_hubs = true;
     }
     ++l;
     if (bt) {
@@ -1023,6 +1095,47 @@ void validate_options(const lv_options* o) {
 
 // The pass loop of louvainInvokeOmp (inc/louvain.hxx:1153-1218) for one
 // repeat.  ucom/utot/ctot are the state (membership, K, Sigma_tot).
+// Hub tables of state s for graph x and membership vcom: slots per vertex
+// (a scan of the per-vertex slot counts), the hub list, then one wave per hub.
+void build_hub_tables(lv_context* c, const lv_dgraph* x, lv_dstate* s, const uint32_t* vcom) {
+  const uint64_t S = x->S;
+  if (!s->ht_cap) {
+    s->ht_cap = dalloc<uint32_t>(S);
+    s->ht_off = dalloc<uint64_t>(S + 1);
+  }
+  uint64_t* cap64 = c->ws<uint64_t>("ht.cap64", S + 1);
+  uint8_t* flags = c->ws<uint8_t>("ht.flags", (S + 7) & ~3ull);
+  k_fill_u8(flags, 0, (S + 7) & ~3ull, c->st);
+  k_ht_caps(x->deg, S, s->ht_cap, cap64, flags, c->st);
+  k_fill_u64(cap64 + S, 0, 1, c->st);
+  scan64(c, cap64, s->ht_off, S + 1);
+  const uint64_t used = c->read1(s->ht_off + S);
+  // spare slots for vertices that pass kHtMinDeg edges in later runs
+  const uint64_t slots = used + std::max<uint64_t>(used / 4, (uint64_t)1024 * kHtMaxCap);
+  if (!s->ht_next) s->ht_next = dalloc<unsigned long long>(1);
+  k_fill_u64(reinterpret_cast<uint64_t*>(s->ht_next), used, 1, c->st);
+  if (slots > s->ht_slots) {
+    (void)hipFree(s->ht_key);
+    (void)hipFree(s->ht_cnt);
+    (void)hipFree(s->ht_w);
+    s->ht_key = nullptr;
+    s->ht_cnt = nullptr;
+    s->ht_w = nullptr;
+    s->ht_slots = 0;
+    s->ht_key = dalloc<uint32_t>(slots);
+    s->ht_cnt = dalloc<int32_t>(slots);
+    s->ht_w = dalloc<double>(slots);
+    s->ht_slots = slots;
+  }
+  uint32_t* list = c->ws<uint32_t>("ht.list", std::max<uint64_t>(S, 1));
+  uint32_t* lens = c->ws<uint32_t>("ht.len", kWlQueues);
+  uint64_t* bsum = c->ws<uint64_t>("ht.bsum", (size_t)kWlQueues * kWlScanBlocks);
+  WlQueues qs{};
+  qs.q[0] = list;
+  k_wl_queue(flags, true, S, x->deg, WlSplit{}, qs, lens, bsum, c->st);
+  k_ht_build(x->view(), x->f64, vcom, list, lens, s->ht_off, s->ht_cap, s->ht_key, s->ht_cnt, s->ht_w, c->st);
+}
+
 void invoke(lv_context* c, int approach, const lv_dgraph* x, const lv_dbatch* b, lv_dstate* s, const lv_options* o,
             int& l, int& p, RunTimes& rt, Counters& cnt) {
   const uint64_t S = x->S;
@@ -1061,6 +1174,27 @@ void invoke(lv_context* c, int approach, const lv_dgraph* x, const lv_dbatch* b,
   } else {
     k_update_weights(b->du, b->dw, b->nd, b->iu, b->iw, b->ni, ucom, utot, ctot, c->st);
   }
+  // hub neighbour-community tables of the state: brought up to this graph by
+  // the batch's edges, or built on the second run of a state
+  lv_dstate* ht = nullptr;
+  if (worklist && c->tier_order == 2 && c->use_tables && o->repeat == 1 && approach != LV_STATIC) {
+    if (s->ht_valid && s->ht_live + b->ni - b->nd == x->live) {
+      k_ht_batch(b->du, b->dv, b->dw, b->nd, b->iu, b->iv, b->iw, b->ni, ucom, s->ht_off, s->ht_cap, s->ht_key,
+                 s->ht_cnt, s->ht_w, c->st);
+      // vertices that passed kHtMinDeg edges with this batch: tables from
+      // the spare slots, built from this graph (after the batch deltas)
+      uint32_t* nl = c->ws<uint32_t>("ht.newlist", std::max<uint64_t>(S, 1));
+      uint32_t* nn = c->ws<uint32_t>("ht.newn", 1);
+      k_fill_u32(nn, 0, 1, c->st);
+      k_ht_new_hubs(x->deg, S, s->ht_cap, s->ht_off, s->ht_next, s->ht_slots, nl, nn, c->st);
+      k_ht_build(x->view(), x->f64, ucom, nl, nn, s->ht_off, s->ht_cap, s->ht_key, s->ht_cnt, s->ht_w, c->st);
+      ht = s;
+    } else if (s->runs >= 1) {
+      build_hub_tables(c, x, s, ucom);
+      ht = s;
+    }
+  }
+  s->ht_valid = false;  // until this run ends with its tables intact
   hipEvent_t e1 = c->record();
   // fm: marking
   const uint8_t* fa = nullptr;
@@ -1099,7 +1233,7 @@ void invoke(lv_context* c, int approach, const lv_dgraph* x, const lv_dbatch* b,
     int m;
     hipEvent_t a0 = c->record();
     if (first) {
-      if (worklist) m = local_move_worklist(c, cur, ucom, utot, ctot, vaff, fa, M, R, L, E, cnt);
+      if (worklist) m = local_move_worklist(c, cur, ucom, utot, ctot, vaff, fa, M, R, L, E, cnt, nullptr, ht);
       else m = local_move(c, cur, ucom, utot, ctot, vaff, fa, M, R, L, E, cnt,
                           approach == LV_DYNAMIC_FRONTIER || approach == LV_DELTA_SCREENING);
     } else {
@@ -1141,6 +1275,12 @@ void invoke(lv_context* c, int approach, const lv_dgraph* x, const lv_dbatch* b,
     E /= o->tolerance_drop;
   }
   if (p > 1) k_lookup(ucom, S, vcom, c->st);
+  // the tables stay valid if the membership kept its ids (one pass)
+  if (ht && p == 1) {
+    s->ht_valid = true;
+    s->ht_live = x->live;
+  }
+  ++s->runs;
   hipEvent_t e3 = c->record();
   if (!t1_set) t1 = e3;
   HIPCHK(hipEventSynchronize(e3));
@@ -1164,6 +1304,7 @@ void run(lv_context* c, int approach, const lv_dgraph* x, const lv_dbatch* b, lv
   c->ev_used = 0;
   const uint64_t S = x->S;
   int repeat = o->repeat;
+  if (repeat > 1) s->ht_reset();  // each repeat restarts from the caller's state
   // keep the caller's state for every repeat (louvainSetupInitialsW, inc/louvain.hxx:1240-1249)
   uint32_t* bm = nullptr;
   double *bv = nullptr, *bc = nullptr;
@@ -1484,6 +1625,7 @@ void run_batched(lv_context* c, int approach, const lv_dgraph* x, const lv_dbatc
   for (uint64_t k = 0; k < G; ++k) require(voff[k] <= voff[k + 1], LV_ERR_INVALID, "graph ranges not ascending");
   validate_options(o);
   require(o->repeat == 1, LV_ERR_INVALID, "the batched form runs one repeat");
+  s->ht_reset();  // the batched form moves vertices without hub tables
   HIPCHK(hipSetDevice(c->device));
   c->ev_used = 0;
   c->prof.clear();
@@ -1991,6 +2133,39 @@ int lv_context_set_pick_less(lv_context* c, int iterations) {
   return LV_OK;
 }
 
+int lv_dstate_check_hub_tables(lv_context* c, const lv_dgraph* g, const lv_dstate* s, uint64_t* tables,
+                               uint64_t* mismatches) {
+  if (!c || !g || !s || !tables || !mismatches || g->S != s->S) return LV_ERR_INVALID;
+  return guard(c, [&] {
+    *tables = *mismatches = 0;
+    if (!s->ht_valid) return;
+    const uint64_t S = g->S;
+    uint8_t* flags = c->ws<uint8_t>("htc.flags", (S + 7) & ~3ull);
+    uint32_t* list = c->ws<uint32_t>("htc.list", std::max<uint64_t>(S, 1));
+    uint32_t* lens = c->ws<uint32_t>("htc.len", kWlQueues);
+    uint64_t* bsum = c->ws<uint64_t>("htc.bsum", (size_t)kWlQueues * kWlScanBlocks);
+    unsigned long long* out = c->ws<unsigned long long>("htc.out", 2);
+    k_fill_u64(reinterpret_cast<uint64_t*>(out), 0, 2, c->st);
+    k_fill_u8(flags, 0, (S + 7) & ~3ull, c->st);
+    k_flag_nonzero(s->ht_cap, S, flags, c->st);
+    WlQueues qs{};
+    qs.q[0] = list;
+    k_wl_queue(flags, true, S, g->deg, WlSplit{}, qs, lens, bsum, c->st);
+    k_ht_check(g->view(), g->f64, s->memb, list, lens, s->ht_off, s->ht_cap, s->ht_key, s->ht_cnt, s->ht_w, out,
+               c->st);
+    unsigned long long h[2];
+    c->read(h, out, 2);
+    *tables = h[0];
+    *mismatches = h[1];
+  });
+}
+
+int lv_context_set_hub_tables(lv_context* c, int on) {
+  if (!c) return LV_ERR_INVALID;
+  c->use_tables = on ? 1 : 0;
+  return LV_OK;
+}
+
 int lv_context_set_graphs(lv_context* c, int on) {
   if (!c) return LV_ERR_INVALID;
   c->use_graphs = on ? 1 : 0;
@@ -2151,12 +2326,19 @@ void lv_dstate_destroy(lv_dstate* s) {
   (void)hipFree(s->memb);
   (void)hipFree(s->vtot);
   (void)hipFree(s->ctot);
+  (void)hipFree(s->ht_off);
+  (void)hipFree(s->ht_cap);
+  (void)hipFree(s->ht_key);
+  (void)hipFree(s->ht_cnt);
+  (void)hipFree(s->ht_w);
+  (void)hipFree(s->ht_next);
   delete s;
 }
 int lv_dstate_upload(lv_context* c, lv_dstate* s, const uint32_t* q, const double* qv, const double* qc) {
   if (!c || !s || !q || !qv || !qc) return LV_ERR_INVALID;
   return guard(c, [&] {
     check_membership(q, s->S);
+    s->ht_reset();
     h2d(c, s->memb, q, s->S);
     h2d(c, s->vtot, qv, s->S);
     h2d(c, s->ctot, qc, s->S);
@@ -2175,6 +2357,7 @@ int lv_dstate_download(lv_context* c, const lv_dstate* s, uint32_t* q, double* q
 int lv_dstate_copy(lv_context* c, lv_dstate* dst, const lv_dstate* src) {
   if (!c || !dst || !src || dst->S != src->S) return LV_ERR_INVALID;
   return guard(c, [&] {
+    dst->ht_reset();
     d2d(c, dst->memb, src->memb, src->S);
     d2d(c, dst->vtot, src->vtot, src->S);
     d2d(c, dst->ctot, src->ctot, src->S);

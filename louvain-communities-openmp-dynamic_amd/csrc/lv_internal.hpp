@@ -50,13 +50,29 @@ constexpr uint32_t kMidSlots = 2048;
 constexpr int kMidBlock = 512;
 constexpr int kScanBatch = 4;
 constexpr uint64_t kMidGrid = 512;
+// Hub neighbour-community tables (worklist DF / DS first pass on a resident
+// state): for every vertex of more than kHtMinDeg edges when the tables are
+// built, its neighbours' weight and count per community, in an open-addressed
+// table of min(nextpow2(2 deg), kHtMaxCap) slots (key u32, count i32, weight
+// f64).  Batch edges and every move keep them exact, so such a hub is
+// evaluated from its table -- tens of communities -- instead of its edges.
+// A table that would hold more than kHtMaxDistinct communities is dropped
+// (its hub is evaluated from its edges).
+constexpr uint32_t kHtMinDeg = 256;
+constexpr uint32_t kHtMaxCap = 256;
+constexpr uint32_t kHtMaxDistinct = kHtMaxCap - 64;  // room for a slot per lane of a wave past it
+// slots of a vertex without a table: 0 never built, kHtDropped dropped
+// (too many communities, or no spare slots); a table has kHtMaxCap
+constexpr uint32_t kHtDropped = 1;
+
 // worklist queue compaction: blocks (contiguous id segments) per pass
 constexpr int kWlScanBlocks = 1024;
-// worklist queues: low degree, small hubs, mid hubs, hub batch
-constexpr int kWlQueues = 4;
+// worklist queues: low degree, small hubs, mid hubs, hub batch, tabled hubs
+constexpr int kWlQueues = 5;
 struct WlSplit {
-  uint32_t t[kWlQueues - 1];  // degree bounds of queues 0..2
+  uint32_t t[3];              // degree bounds of queues 0..2
   const uint8_t* big;         // ids sent to the hub batch (nullable)
+  const uint32_t* tab;        // hub table slots per id: hubs with a table go to queue 4 (nullable)
   uint8_t* dirty;             // per segment: a flag may be set (nullable: scan all)
   uint32_t shift;             // segment of block b: ids [b << shift, (b + 1) << shift)
 };
@@ -196,6 +212,12 @@ struct MoveArgs {
   // hb_big (per vertex) and redone with a full table
   uint8_t* hb_over;
   uint8_t* hb_big;
+  // hub neighbour-community tables (nullable ht_cap: none)
+  const uint64_t* ht_off;      // per vertex: first slot of its table
+  uint32_t* ht_cap;            // per vertex: slots (0: no table)
+  uint32_t* ht_key;
+  int32_t* ht_cnt;
+  double* ht_w;
   uint64_t* hb_off;            // hub's row offset and degree
   uint32_t* hb_deg;
   uint32_t* hb_cown;           // list entry owning each edge chunk
@@ -206,6 +228,31 @@ struct MoveArgs {
 // and its edges split over many blocks; the reference's per-vertex steps in
 // separate device passes (table build, best gain, move, neighbour flags).
 void k_hub_batch(const MoveArgs& a, hipStream_t st);
+// Worklist hubs with a neighbour-community table, from a.list (length
+// *a.n_dev), one wave each.
+void k_hub_tab(const MoveArgs& a, hipStream_t st);
+// Hub tables: slots per vertex (cap[u] = min(nextpow2(2 deg), kHtMaxCap) for
+// degree > kHtMinDeg, else 0), hub flags for the build list; the build from
+// the edges and a membership (a table with too many communities is dropped,
+// cap[u] = 0); the batch's edges into the tables (deletions out, insertions
+// in, with the membership before any move).
+void k_ht_caps(const uint32_t* deg, uint64_t S, uint32_t* cap, uint64_t* cap64, uint8_t* flags, hipStream_t st);
+void k_ht_build(GView g, bool f64, const uint32_t* vcom, const uint32_t* list, const uint32_t* n_dev,
+                const uint64_t* off, uint32_t* cap, uint32_t* key, int32_t* cnt, double* w, hipStream_t st);
+void k_flag_nonzero(const uint32_t* a, uint64_t n, uint8_t* flags, hipStream_t st);
+// Entries of worklist queues 1-3 back to the next sub-round; count of the
+// vertices above thr edges without a hub table (added to *out).
+void k_carry_hubs(const MoveArgs& a, const WlQueues& qs, const uint32_t* lens, hipStream_t st);
+void k_ht_new_hubs(const uint32_t* deg, uint64_t S, uint32_t* cap, uint64_t* off, unsigned long long* next,
+                   uint64_t total, uint32_t* list, uint32_t* nlist, hipStream_t st);
+void k_untabled(const uint32_t* deg, const uint32_t* cap, uint64_t S, uint32_t thr, unsigned long long* out,
+                hipStream_t st);
+void k_ht_check(GView g, bool f64, const uint32_t* vcom, const uint32_t* list, const uint32_t* n_dev,
+                const uint64_t* off, uint32_t* cap, uint32_t* key, int32_t* cnt, double* w, unsigned long long* out,
+                hipStream_t st);
+void k_ht_batch(const uint32_t* du, const uint32_t* dv, const float* dw, uint64_t nd, const uint32_t* iu,
+                const uint32_t* iv, const float* iw, uint64_t ni, const uint32_t* vcom, const uint64_t* off,
+                uint32_t* cap, uint32_t* key, int32_t* cnt, double* w, hipStream_t st);
 // Worklist hubs from a.list (length *a.n_dev), one block each with an LDS
 // table: class 0 up to kSmallHubMax edges, class 1 up to kMidHubMax.
 void k_hub_mid(const MoveArgs& a, int cls, hipStream_t st);

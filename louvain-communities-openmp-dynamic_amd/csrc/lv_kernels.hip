@@ -194,6 +194,58 @@ __device__ __forceinline__ void flag_vertex(uint8_t* vaff, uint32_t v, uint32_t 
     mark_same(a, v);
 }
 
+// Hub tables: the slot of community c in the table at off (cap slots),
+// inserted if absent and insert is set; -1 if absent or no room.
+__device__ __forceinline__ int64_t ht_slot(uint32_t* key, uint64_t off, uint32_t cap, uint32_t c, bool insert) {
+  uint32_t s = slot_of(c, cap);
+  for (uint32_t probe = 0; probe < cap; ++probe) {
+    uint32_t k = __hip_atomic_load(&key[off + s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (k == kEmpty) {
+      if (!insert) return -1;
+      k = atomicCAS(&key[off + s], kEmpty, c);
+    }
+    if (k == kEmpty || k == c) return (int64_t)(off + s);
+    s = (s + 1) & (cap - 1);
+  }
+  return -1;
+}
+
+// count n and weight w of community c in x's table; a table with no room
+// for c is dropped (x is then evaluated from its edges).
+__device__ __forceinline__ void ht_add(uint32_t* cap, const uint64_t* off, uint32_t* key, int32_t* cnt, double* wt,
+                                       uint32_t x, uint32_t c, int32_t n, double w) {
+  const uint32_t cx = __hip_atomic_load(&cap[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (cx <= kHtDropped) return;
+  const int64_t s = ht_slot(key, off[x], cx, c, n > 0);
+  if (s < 0) {
+    if (n > 0) __hip_atomic_store(&cap[x], kHtDropped, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return;
+  }
+  atomicAdd(&cnt[s], n);
+  unsafeAtomicAdd(&wt[s], w);
+}
+
+// Neighbour v of x moved from community `from` to `to` over an edge of
+// weight w: x's table follows.
+__device__ __forceinline__ void ht_move(const MoveArgs& a, uint32_t x, double w, uint32_t from, uint32_t to) {
+  if (!a.ht_cap) return;
+  const uint32_t cx = __hip_atomic_load(&a.ht_cap[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (cx <= kHtDropped) return;
+  const uint64_t off = a.ht_off[x];
+  int64_t s = ht_slot(a.ht_key, off, cx, from, false);
+  if (s >= 0) {
+    atomicAdd(&a.ht_cnt[s], -1);
+    unsafeAtomicAdd(&a.ht_w[s], -w);
+  }
+  s = ht_slot(a.ht_key, off, cx, to, true);
+  if (s < 0) {
+    __hip_atomic_store(&a.ht_cap[x], kHtDropped, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return;
+  }
+  atomicAdd(&a.ht_cnt[s], 1);
+  unsafeAtomicAdd(&a.ht_w[s], w);
+}
+
 // ---------------------------------------------------------------- fills
 template <class T>
 __global__ void fill_kernel(T* a, T v, uint64_t n) {
@@ -622,7 +674,10 @@ __global__ __launch_bounds__(BLOCK) void move_kernel(MoveArgs a) {
             }
             for (uint32_t i = lane; i < d; i += G) {
               uint32_t v = a.g.keys[o + i];
-              if (v != u) flag_vertex(a.vaff, v, u, a);
+              if (v != u) {
+                flag_vertex(a.vaff, v, u, a);
+                ht_move(a, v, (double)W[o + i], cu, bc);
+              }
             }
           }
           if (lane == 0) {
@@ -1049,12 +1104,18 @@ __global__ __launch_bounds__(kBlock) void hub_flag_kernel(MoveArgs a) {
     if (a.hb_bpos[k] == kNoPos) continue;  // uniform per block
     const uint32_t u = a.list[k];
     const uint64_t o = a.hb_off[k], d = a.hb_deg[k];
+    // the hub moved from hb_cu[k] to the community the apply pass stored
+    const uint32_t from = a.hb_cu[k], to = a.ht_cap ? ld_dev(&a.vcom[u]) : 0;
     uint64_t i0 = (q - a.hb_chunks[k]) * kHubChunk, i1 = min(d, i0 + kHubChunk);
-    for (uint64_t b = i0; b < i1; b += kBlock) {  // uniform trip count: wave ballots in flag_vertex
+    for (uint64_t b = i0; b < i1; b += kBlock) {
       uint64_t i = b + threadIdx.x;
       bool on = i < i1;
       uint32_t v = on ? a.g.keys[o + i] : u;
-      if (on && v != u) flag_vertex(a.vaff, v, u, a);
+      if (on && v != u) {
+        flag_vertex(a.vaff, v, u, a);
+        if (a.ht_cap) ht_move(a, v, a.f64_weights ? ((const double*)a.g.w)[o + i] : (double)((const float*)a.g.w)[o + i],
+                              from, to);
+      }
     }
   }
 }
@@ -1171,9 +1232,9 @@ __global__ __launch_bounds__(BLOCK) void hub_mid_kernel(MoveArgs a) {
       // table never fills).  The last batch's keys stay for the flags.
       bool over = false;
       uint32_t v[NB];
+      double w[NB];
       for (uint32_t b0 = 0; b0 < d && !over; b0 += NB * BLOCK) {  // block-uniform
         uint32_t c[NB];
-        double w[NB];
 #pragma unroll
         for (int r = 0; r < NB; ++r) {
           const uint32_t i = b0 + r * BLOCK + threadIdx.x;
@@ -1236,14 +1297,20 @@ __global__ __launch_bounds__(BLOCK) void hub_mid_kernel(MoveArgs a) {
             ++c_move;
             c_medge += d;
           }
-          if (d <= NB * BLOCK) {  // one batch: its keys are still here
+          if (d <= NB * BLOCK) {  // one batch: its keys and weights are still here
 #pragma unroll
             for (int r = 0; r < NB; ++r)
-              if (r * BLOCK + threadIdx.x < d && v[r] != u) flag_vertex(a.vaff, v[r], u, a);
+              if (r * BLOCK + threadIdx.x < d && v[r] != u) {
+                flag_vertex(a.vaff, v[r], u, a);
+                ht_move(a, v[r], w[r], cu, bc);
+              }
           } else {
             for (uint32_t i = threadIdx.x; i < d; i += BLOCK) {
               const uint32_t x = a.g.keys[o + i];
-              if (x != u) flag_vertex(a.vaff, x, u, a);
+              if (x != u) {
+                flag_vertex(a.vaff, x, u, a);
+                ht_move(a, x, (double)W[o + i], cu, bc);
+              }
             }
           }
         }
@@ -1267,6 +1334,325 @@ __global__ __launch_bounds__(BLOCK) void hub_mid_kernel(MoveArgs a) {
     unsigned long long c = block_sum_u64<BLOCK>(c_cand), m = block_sum_u64<BLOCK>(c_move);
     unsigned long long me = block_sum_u64<BLOCK>(c_medge);
     if (threadIdx.x == 0) add_counts(a.cnt, v, e, c, m, me);
+  }
+}
+
+// louvainMoveOmpW's body (inc/louvain.hxx:595-605) for hubs with a
+// neighbour-community table, one wave per hub from a.list (length
+// *a.n_dev): the candidates and their weights come from the table (entries
+// with a positive count), the gain is deltaModularity as for every vertex,
+// and a tie for the best gain goes to the tied community that appears first
+// in the hub's row (inc/louvain.hxx:461), found by walking the row.  A hub
+// whose table was dropped since the queue was built joins the next
+// sub-round (it is then queued by degree).
+template <class WT>
+__global__ __launch_bounds__(kBlock) void hub_tab_kernel(MoveArgs a) {
+  constexpr int R = kHtMaxCap / 64;  // table slots per lane
+  const int lane = threadIdx.x % 64;
+  const uint64_t n = list_len(a);
+  const uint64_t nw = (uint64_t)gridDim.x * (kBlock / 64);
+  const WT* W = (const WT*)a.g.w;
+  double el_acc = 0;
+  unsigned long long c_vert = 0, c_edge = 0, c_cand = 0, c_move = 0, c_medge = 0;
+  for (uint64_t k = (uint64_t)blockIdx.x * (kBlock / 64) + threadIdx.x / 64; k < n; k += nw) {  // wave-uniform
+    // the hub's scalars, read by lane 0 (its table may be dropped meanwhile)
+    uint32_t u = 0, d = 0, cap = 0, act = 0;
+    if (lane == 0) {
+      u = a.list[k];
+      d = a.g.deg[u];
+      act = d > a.hub_min && (!a.g.ex || a.g.ex[u]) && !vertex_skipped(a, u) && ld_dev(&a.vaff[u]) &&
+            (!a.fa || a.fa[u]);
+      cap = act ? __hip_atomic_load(&a.ht_cap[u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
+      if (act && cap <= kHtDropped) mark_same(a, u);  // no table any more: evaluated from its edges next
+    }
+    u = __shfl(u, 0, 64);
+    d = __shfl(d, 0, 64);
+    cap = __shfl(cap, 0, 64);
+    if (cap <= kHtDropped) continue;
+    if (lane == 0 && a.stamp) st_dev(&a.stamp[u], cur_serial(a));
+    const uint32_t cu = ld_dev(&a.vcom[u]);
+    const uint64_t o = a.g.off[u], off = a.ht_off[u];
+    const double Ku = a.vtot[u], Sd = ld_dev(&a.ctot[cu]), Mu = vertex_M(a, u);
+    uint32_t ck[R];
+    int32_t cn[R];
+    double cw[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const uint32_t sl = r * 64 + lane;
+      ck[r] = sl < cap ? ld_dev(&a.ht_key[off + sl]) : kEmpty;
+      cn[r] = sl < cap ? ld_dev(&a.ht_cnt[off + sl]) : 0;
+      cw[r] = sl < cap ? ld_dev(&a.ht_w[off + sl]) : 0.0;
+    }
+    double vd = 0;  // weight to the hub's own community
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+      if (ck[r] == cu && cn[r] > 0) vd = cw[r];
+    for (int m = 32; m > 0; m >>= 1) vd += __shfl_xor(vd, m, 64);
+    double e[R], be = 0;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      e[r] = 0;
+      if (ck[r] == kEmpty || cn[r] <= 0) continue;
+      ++c_cand;
+      if (ck[r] == cu || (a.pick_less && ck[r] > cu)) continue;
+      e[r] = delta_q(cw[r], vd, Ku, ld_dev(&a.ctot[ck[r]]), Sd, Mu, a.R);
+      if (e[r] > be) be = e[r];
+    }
+    for (int m = 32; m > 0; m >>= 1) be = fmax(be, __shfl_xor(be, m, 64));
+    uint32_t bc = 0;
+    if (be > 0) {
+      // the communities with the best gain; one: it; several: the first in the row
+      int ties = 0;
+      uint32_t mine = kEmpty;
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+        if (e[r] == be) {
+          ++ties;
+          mine = ck[r];
+        }
+      for (int m = 32; m > 0; m >>= 1) ties += __shfl_xor(ties, m, 64);
+      const unsigned long long hold = __ballot(mine != kEmpty);
+      bc = __shfl(mine, __ffsll((long long)hold) - 1, 64);
+      if (ties > 1) {
+        bool found = false;
+        for (uint32_t b0 = 0; b0 < d && !found; b0 += 64) {  // wave-uniform
+          const uint32_t i = b0 + lane;
+          const uint32_t v = i < d ? a.g.keys[o + i] : u;
+          bool tied = false;
+          uint32_t c = 0;
+          if (v != u) {
+            c = ld_dev(&a.vcom[v]);
+            if (c != cu && !(a.pick_less && c > cu)) {
+              const int64_t sl = ht_slot(a.ht_key, off, cap, c, false);
+              tied = sl >= 0 && ld_dev(&a.ht_cnt[sl]) > 0 &&
+                     delta_q(ld_dev(&a.ht_w[sl]), vd, Ku, ld_dev(&a.ctot[c]), Sd, Mu, a.R) == be;
+            }
+          }
+          const unsigned long long tm = __ballot(tied);
+          if (tm) {
+            bc = __shfl(c, __ffsll((long long)tm) - 1, 64);
+            found = true;
+          }
+        }
+      }
+    }
+    // a candidate exists and it is not community 0 of u's graph
+    if (be > 0 && bc != vertex_zero(a, u)) {
+      if (lane == 0) {
+        unsafeAtomicAdd(&a.ctot[cu], -Ku);
+        unsafeAtomicAdd(&a.ctot[bc], Ku);
+        st_dev(&a.vcom[u], bc);
+        ++c_move;
+        c_medge += d;
+      }
+      for (uint32_t i = lane; i < d; i += 64) {
+        const uint32_t v = a.g.keys[o + i];
+        if (v != u) {
+          flag_vertex(a.vaff, v, u, a);
+          ht_move(a, v, (double)W[o + i], cu, bc);
+        }
+      }
+    }
+    if (lane == 0) {
+      st_dev(&a.vaff[u], (uint8_t)0);
+      if (a.gel) unsafeAtomicAdd(&a.gel[a.gid[u]], be);
+      else el_acc += be;
+      ++c_vert;
+      c_edge += d;
+    }
+  }
+  const double t = block_sum<kBlock>(el_acc);
+  if (threadIdx.x == 0 && t != 0) unsafeAtomicAdd(a.el + spread_slot() * kElStride, t);
+  if (a.cnt) {
+    unsigned long long v = block_sum_u64<kBlock>(c_vert), ed = block_sum_u64<kBlock>(c_edge);
+    unsigned long long c = block_sum_u64<kBlock>(c_cand), m = block_sum_u64<kBlock>(c_move);
+    unsigned long long me = block_sum_u64<kBlock>(c_medge);
+    if (threadIdx.x == 0) add_counts(a.cnt, v, ed, c, m, me);
+  }
+}
+
+// Hubs queued for the edge-scanning kernels (queues 1-3: tables dropped in
+// this run) while those kernels are not launched: back to the next
+// sub-round, so none is lost before the host launches them again.
+__global__ void carry_hubs_kernel(MoveArgs a, WlQueues qs, const uint32_t* lens) {
+  for (int q = 1; q <= 3; ++q) {
+    const uint32_t n = lens[q];
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+      mark_same(a, qs.q[q][i]);
+  }
+}
+
+// Vertices that passed kHtMinDeg edges since the tables were built (no
+// table, none dropped): a table each from the spare slots (bump counter at
+// *next, slots up to total), listed for a build; with no spare slots left,
+// dropped.
+__global__ void ht_new_hubs_kernel(const uint32_t* deg, uint64_t S, uint32_t* cap, uint64_t* off,
+                                   unsigned long long* next, uint64_t total, uint32_t* list, uint32_t* nlist) {
+  for (uint64_t u = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; u < S; u += (uint64_t)gridDim.x * blockDim.x) {
+    if (deg[u] <= kHtMinDeg || cap[u] != 0) continue;
+    const unsigned long long s0 = atomicAdd(next, (unsigned long long)kHtMaxCap);
+    if (s0 + kHtMaxCap > total) {
+      cap[u] = kHtDropped;
+      continue;
+    }
+    off[u] = s0;
+    cap[u] = kHtMaxCap;
+    list[atomicAdd(nlist, 1u)] = (uint32_t)u;
+  }
+}
+
+// Vertices above thr edges without a hub table.
+__global__ __launch_bounds__(kBlock) void untabled_kernel(const uint32_t* deg, const uint32_t* cap, uint64_t S,
+                                                          uint32_t thr, unsigned long long* out) {
+  unsigned long long c = 0;
+  for (uint64_t u = blockIdx.x * (uint64_t)kBlock + threadIdx.x; u < S; u += (uint64_t)gridDim.x * kBlock)
+    c += deg[u] > thr && cap[u] <= kHtDropped;
+  c = block_sum_u64<kBlock>(c);
+  if (threadIdx.x == 0 && c) atomicAdd(out, c);
+}
+
+// Hub tables: per vertex its slots (0 below kHtMinDeg edges) and a flag for
+// the build list.
+__global__ void ht_caps_kernel(const uint32_t* deg, uint64_t S, uint32_t* cap, uint64_t* cap64, uint8_t* flags) {
+  for (uint64_t u = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; u < S; u += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t d = deg[u];
+    uint32_t c = 0;
+    if (d > kHtMinDeg) {
+      c = 64;
+      while (c < 2 * d && c < kHtMaxCap) c <<= 1;
+    }
+    cap[u] = c;
+    cap64[u] = c;
+    flags[u] = c != 0;
+  }
+}
+
+// One wave per listed hub: its neighbours (self-loops skipped) summed per
+// community in an LDS table, then written to its slots; more than
+// kHtMaxDistinct communities drop the table.
+//
+// CHECK: compare instead of write -- every listed vertex that has a table
+// and fits the build gets counted in out[0], and in out[1] if its table
+// differs from the fresh sums (a count, a weight beyond 1e-9 relative, or a
+// community with a positive count that is not a neighbour's).
+template <class WT, bool CHECK>
+__global__ __launch_bounds__(kBlock) void ht_build_kernel(GView g, const uint32_t* vcom, const uint32_t* list,
+                                                          const uint32_t* n_dev, const uint64_t* off, uint32_t* cap,
+                                                          uint32_t* key, int32_t* cnt, double* wt,
+                                                          unsigned long long* out) {
+  constexpr int WPB = kBlock / 64;
+  __shared__ uint32_t s_key[WPB][kHtMaxCap];
+  __shared__ int32_t s_cnt[WPB][kHtMaxCap];
+  __shared__ double s_w[WPB][kHtMaxCap];
+  __shared__ uint32_t s_made[WPB];
+  const int lane = threadIdx.x % 64, wv = threadIdx.x / 64;
+  const uint64_t n = *n_dev;
+  const WT* W = (const WT*)g.w;
+  for (uint64_t k = (uint64_t)blockIdx.x * WPB + wv; k < n; k += (uint64_t)gridDim.x * WPB) {  // wave-uniform
+    const uint32_t u = list[k], c_u = cap[u];
+    if (c_u <= kHtDropped) continue;
+    for (uint32_t sl = lane; sl < c_u; sl += 64) {
+      s_key[wv][sl] = kEmpty;
+      s_cnt[wv][sl] = 0;
+      s_w[wv][sl] = 0;
+    }
+    if (lane == 0) s_made[wv] = 0;
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    const uint64_t o = g.off[u];
+    const uint32_t d = g.deg[u];
+    bool over = false;
+    for (uint32_t b0 = 0; b0 < d && !over; b0 += 64 * kScanRounds) {  // wave-uniform
+      uint32_t v[kScanRounds], c[kScanRounds];
+      double w[kScanRounds];
+#pragma unroll
+      for (int r = 0; r < kScanRounds; ++r) {
+        const uint32_t i = b0 + r * 64 + lane;
+        v[r] = i < d ? g.keys[o + i] : u;
+        w[r] = i < d ? (double)W[o + i] : 0.0;
+      }
+#pragma unroll
+      for (int r = 0; r < kScanRounds; ++r) c[r] = v[r] != u ? vcom[v[r]] : 0;
+#pragma unroll
+      for (int r = 0; r < kScanRounds; ++r) {
+        // each lane makes at most one new slot per round after reading the
+        // distinct count, so the table never fills
+        if (__hip_atomic_load(&s_made[wv], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT) >= kHtMaxDistinct)
+          over = true;
+        const bool ok = !over && v[r] != u;
+        uint32_t sl = slot_of(c[r], c_u);
+        if (ok) {
+          while (true) {
+            const uint32_t kk = atomicCAS(&s_key[wv][sl], kEmpty, c[r]);
+            if (kk == kEmpty || kk == c[r]) {
+              if (kk == kEmpty) atomicAdd(&s_made[wv], 1u);
+              atomicAdd(&s_cnt[wv][sl], 1);
+              unsafeAtomicAdd(&s_w[wv][sl], w[r]);
+              break;
+            }
+            sl = (sl + 1) & (c_u - 1);
+          }
+        }
+      }
+      over = __ballot(over) != 0;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    const uint64_t base = off[u];
+    if constexpr (CHECK) {
+      if (over) continue;  // more communities than a build keeps: not compared
+      bool bad = false;
+      int fresh = 0, kept = 0;
+      for (uint32_t sl = lane; sl < c_u; sl += 64) {
+        const uint32_t fk = s_key[wv][sl];
+        if (fk != kEmpty) {
+          ++fresh;
+          const int64_t t = ht_slot(key, base, c_u, fk, false);
+          const double fw = s_w[wv][sl];
+          if (t < 0 || cnt[t] != s_cnt[wv][sl] || fabs(wt[t] - fw) > 1e-9 * fmax(1.0, fabs(fw))) bad = true;
+        }
+        if (key[base + sl] != kEmpty && cnt[base + sl] > 0) ++kept;
+        if (key[base + sl] != kEmpty && cnt[base + sl] < 0) bad = true;
+      }
+      for (int m = 32; m > 0; m >>= 1) {
+        fresh += __shfl_xor(fresh, m, 64);
+        kept += __shfl_xor(kept, m, 64);
+      }
+      bad = __ballot(bad) != 0 || fresh != kept;
+      if (lane == 0) {
+        atomicAdd(&out[0], 1ull);
+        if (bad) atomicAdd(&out[1], 1ull);
+      }
+      continue;
+    }
+    if (over) {
+      if (lane == 0) cap[u] = kHtDropped;
+      continue;
+    }
+    for (uint32_t sl = lane; sl < c_u; sl += 64) {
+      key[base + sl] = s_key[wv][sl];
+      cnt[base + sl] = s_cnt[wv][sl];
+      wt[base + sl] = s_w[wv][sl];
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+
+// The batch's edges into the tables of their first endpoints (both
+// directions are listed): deletions out, insertions in, by the endpoint's
+// community before any move.
+__global__ void ht_batch_kernel(const uint32_t* du, const uint32_t* dv, const float* dw, uint64_t nd,
+                                const uint32_t* iu, const uint32_t* iv, const float* iw, uint64_t ni,
+                                const uint32_t* vcom, const uint64_t* off, uint32_t* cap, uint32_t* key, int32_t* cnt,
+                                double* wt) {
+  for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < nd + ni;
+       t += (uint64_t)gridDim.x * blockDim.x) {
+    const bool del = t < nd;
+    const uint64_t j = del ? t : t - nd;
+    const uint32_t u = del ? du[j] : iu[j], v = del ? dv[j] : iv[j];
+    if (u == v) continue;
+    const double w = (double)(del ? dw[j] : iw[j]);
+    ht_add(cap, off, key, cnt, wt, u, vcom[v], del ? -1 : 1, del ? -w : w);
   }
 }
 
@@ -1727,6 +2113,7 @@ __device__ __forceinline__ bool wl_bits(const uint8_t* f, const uint32_t* deg, u
       const uint32_t d = sp.t[0] ? deg[id0 + j] : 0;
       int q = d <= sp.t[0] ? 0 : (d <= sp.t[1] ? 1 : (d <= sp.t[2] ? 2 : 3));
       if (q == 1 || q == 2) q = sp.big && sp.big[id0 + j] ? 3 : q;
+      if (q && sp.tab && sp.tab[id0 + j] > kHtDropped) q = 4;
       m[q] |= 1u << j;
     }
   }
@@ -1995,6 +2382,52 @@ void k_hub_mid(const MoveArgs& a, int cls, hipStream_t st) {
     if (f64) LV_LAUNCH((hub_mid_kernel<kMidBlock, kMidSlots, kMidHubMax, double>), kMidGrid, kMidBlock, st, a);
     else LV_LAUNCH((hub_mid_kernel<kMidBlock, kMidSlots, kMidHubMax, float>), kMidGrid, kMidBlock, st, a);
   }
+}
+void k_hub_tab(const MoveArgs& a, hipStream_t st) {
+  if (a.g.w && a.f64_weights) LV_LAUNCH(hub_tab_kernel<double>, 1024, kBlock, st, a);
+  else LV_LAUNCH(hub_tab_kernel<float>, 1024, kBlock, st, a);
+}
+__global__ void flag_nonzero_kernel(const uint32_t* a, uint64_t n, uint8_t* flags) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    flags[i] = a[i] != 0;
+}
+void k_flag_nonzero(const uint32_t* a, uint64_t n, uint8_t* flags, hipStream_t st) {
+  if (n) LV_LAUNCH(flag_nonzero_kernel, grid_for(n, kBlock * 4), kBlock, st, a, n, flags);
+}
+void k_carry_hubs(const MoveArgs& a, const WlQueues& qs, const uint32_t* lens, hipStream_t st) {
+  LV_LAUNCH(carry_hubs_kernel, 64, kBlock, st, a, qs, lens);
+}
+void k_ht_new_hubs(const uint32_t* deg, uint64_t S, uint32_t* cap, uint64_t* off, unsigned long long* next,
+                   uint64_t total, uint32_t* list, uint32_t* nlist, hipStream_t st) {
+  if (S) LV_LAUNCH(ht_new_hubs_kernel, grid_for(S, kBlock * 4), kBlock, st, deg, S, cap, off, next, total, list, nlist);
+}
+void k_untabled(const uint32_t* deg, const uint32_t* cap, uint64_t S, uint32_t thr, unsigned long long* out,
+                hipStream_t st) {
+  if (S) LV_LAUNCH(untabled_kernel, grid_for(S, kBlock * 8, kReduceGrid), kBlock, st, deg, cap, S, thr, out);
+}
+void k_ht_caps(const uint32_t* deg, uint64_t S, uint32_t* cap, uint64_t* cap64, uint8_t* flags, hipStream_t st) {
+  if (S) LV_LAUNCH(ht_caps_kernel, grid_for(S, kBlock * 4), kBlock, st, deg, S, cap, cap64, flags);
+}
+void k_ht_build(GView g, bool f64, const uint32_t* vcom, const uint32_t* list, const uint32_t* n_dev,
+                const uint64_t* off, uint32_t* cap, uint32_t* key, int32_t* cnt, double* w, hipStream_t st) {
+  if (f64) LV_LAUNCH((ht_build_kernel<double, false>), 4096, kBlock, st, g, vcom, list, n_dev, off, cap, key, cnt, w,
+                     nullptr);
+  else LV_LAUNCH((ht_build_kernel<float, false>), 4096, kBlock, st, g, vcom, list, n_dev, off, cap, key, cnt, w,
+                 nullptr);
+}
+void k_ht_check(GView g, bool f64, const uint32_t* vcom, const uint32_t* list, const uint32_t* n_dev,
+                const uint64_t* off, uint32_t* cap, uint32_t* key, int32_t* cnt, double* w, unsigned long long* out,
+                hipStream_t st) {
+  if (f64) LV_LAUNCH((ht_build_kernel<double, true>), 4096, kBlock, st, g, vcom, list, n_dev, off, cap, key, cnt, w,
+                     out);
+  else LV_LAUNCH((ht_build_kernel<float, true>), 4096, kBlock, st, g, vcom, list, n_dev, off, cap, key, cnt, w, out);
+}
+void k_ht_batch(const uint32_t* du, const uint32_t* dv, const float* dw, uint64_t nd, const uint32_t* iu,
+                const uint32_t* iv, const float* iw, uint64_t ni, const uint32_t* vcom, const uint64_t* off,
+                uint32_t* cap, uint32_t* key, int32_t* cnt, double* w, hipStream_t st) {
+  if (nd + ni)
+    LV_LAUNCH(ht_batch_kernel, grid_for(nd + ni, kBlock), kBlock, st, du, dv, dw, nd, iu, iv, iw, ni, vcom, off, cap,
+              key, cnt, w);
 }
 void k_exists(GView g, const uint32_t* vcom, uint32_t* flags, hipStream_t st) {
   if (g.S) LV_LAUNCH(exists_kernel, grid_for(g.S, kBlock * 4), kBlock, st, g, vcom, flags);

@@ -35,7 +35,8 @@ PATH_SWEEP, PATH_FRONTIER, PATH_WORKLIST = 0, 1, 2
 EXPORTS = [
     "lv_options_default", "lv_status_string", "lv_context_create", "lv_context_destroy", "lv_last_error",
     "lv_context_set_path", "lv_context_set_schedule", "lv_context_set_pick_less", "lv_context_set_slabs", "lv_context_set_hub_streams", "lv_context_set_hub_threshold",
-    "lv_context_set_profiling", "lv_context_set_graphs", "lv_stats_get",
+    "lv_context_set_profiling", "lv_context_set_graphs", "lv_context_set_hub_tables",
+    "lv_dstate_check_hub_tables", "lv_stats_get",
     "lv_stats_reset", "lv_louvain_static", "lv_louvain_naive_dynamic", "lv_louvain_delta_screening",
     "lv_louvain_dynamic_frontier", "lv_dgraph_create", "lv_dgraph_destroy", "lv_dgraph_span", "lv_dgraph_edges",
     "lv_dbatch_create", "lv_dbatch_destroy", "lv_dstate_create", "lv_dstate_destroy", "lv_dstate_upload",
@@ -215,6 +216,17 @@ class Context:
     def set_graphs(self, on=True):
         """Replay worklist sub-round groups from captured HIP graphs (default on)."""
         self._check(self.lib.lv_context_set_graphs(self.h, int(bool(on))), "lv_context_set_graphs")
+
+    def set_hub_tables(self, on=True):
+        """Evaluate hubs of resident states from neighbour-community tables (default on)."""
+        self._check(self.lib.lv_context_set_hub_tables(self.h, int(bool(on))), "lv_context_set_hub_tables")
+
+    def check_hub_tables(self, dgraph, dstate):
+        """(tables compared, tables differing from a fresh build) of a state's hub tables."""
+        t, m = C.c_uint64(), C.c_uint64()
+        self._check(self.lib.lv_dstate_check_hub_tables(self.h, dgraph.h, dstate.h, C.byref(t), C.byref(m)),
+                    "lv_dstate_check_hub_tables")
+        return t.value, m.value
 
     def set_profiling(self, on=True):
         self._check(self.lib.lv_context_set_profiling(self.h, int(bool(on))), "lv_context_set_profiling")
