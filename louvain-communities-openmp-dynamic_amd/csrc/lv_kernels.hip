@@ -2096,7 +2096,7 @@ __global__ __launch_bounds__(kBlock) void caps_sum_kernel(const uint32_t* deg, u
 // the counts, then an ordered scatter; flags are read a 4-byte word per
 // thread.
 constexpr uint64_t kWlTile = 4 * kBlock;
-static_assert(kWlScanBlocks % kBlock == 0, "wl_scan_kernel: whole block counts per thread");
+static_assert(kWlScanBlocks % kBlock == 0, "whole segments per block count pass");
 
 // ids of block b: [b << shift, (b + 1) << shift)
 
@@ -2120,15 +2120,25 @@ __device__ __forceinline__ bool wl_bits(const uint8_t* f, const uint32_t* deg, u
   return true;
 }
 
+// Sum over a wave (every lane gets it).
+__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long x) {
+#pragma unroll
+  for (int m = 32; m > 0; m >>= 1) x += __shfl_xor(x, m, 64);
+  return x;
+}
+
+// Per block (segment) and queue: the flagged ids it holds.
 __global__ __launch_bounds__(kBlock) void wl_count_kernel(const uint8_t* f, uint64_t S, const uint32_t* deg,
                                                           WlSplit sp, uint64_t* bsum) {
+  __shared__ unsigned long long s_c[kWlQueues];
   const uint64_t lo = (uint64_t)blockIdx.x << sp.shift, hi = min(S, lo + (1ull << sp.shift));
-  unsigned long long c[kWlQueues] = {};
   if (sp.dirty && !sp.dirty[blockIdx.x]) {  // no flag was set in this segment
-    if (threadIdx.x == 0)
-      for (int q = 0; q < kWlQueues; ++q) bsum[q * kWlScanBlocks + blockIdx.x] = 0;
+    if (threadIdx.x < kWlQueues) bsum[threadIdx.x * kWlScanBlocks + blockIdx.x] = 0;
     return;
   }
+  if (threadIdx.x < kWlQueues) s_c[threadIdx.x] = 0;
+  __syncthreads();
+  unsigned long long c[kWlQueues] = {};
   for (uint64_t id0 = lo + 4 * (uint64_t)threadIdx.x; id0 < hi; id0 += kWlTile) {
     uint32_t m[kWlQueues];
     if (!wl_bits(f, deg, S, sp, id0, m)) continue;
@@ -2137,53 +2147,63 @@ __global__ __launch_bounds__(kBlock) void wl_count_kernel(const uint8_t* f, uint
   }
 #pragma unroll
   for (int q = 0; q < kWlQueues; ++q) {
-    const unsigned long long t = block_sum_u64<kBlock>(c[q]);
-    if (threadIdx.x == 0) bsum[q * kWlScanBlocks + blockIdx.x] = t;
+    const unsigned long long t = wave_sum_u64(c[q]);
+    if (threadIdx.x % 64 == 0 && t) atomicAdd(&s_c[q], t);
   }
+  __syncthreads();
+  if (threadIdx.x < kWlQueues) bsum[threadIdx.x * kWlScanBlocks + blockIdx.x] = s_c[threadIdx.x];
 }
 
-// One block of kBlock threads, kWlScanBlocks / kBlock block counts each.
-__global__ __launch_bounds__(kBlock) void wl_scan_kernel(uint64_t* bsum, uint32_t* lens) {
-  constexpr int P = kWlScanBlocks / kBlock;
-#pragma unroll
-  for (int q = 0; q < kWlQueues; ++q) {
-    uint64_t* a = bsum + q * kWlScanBlocks + (uint64_t)threadIdx.x * P;
-    uint64_t sa = 0;
-    for (int k = 0; k < P; ++k) sa += a[k];
-    uint64_t ta;
-    uint64_t ea = block_scan_excl<kBlock>(sa, ta);
-    for (int k = 0; k < P; ++k) {
-      uint64_t x = a[k];
-      a[k] = ea;
-      ea += x;
-    }
-    if (threadIdx.x == 0) lens[q] = (uint32_t)ta;
-  }
-}
-
+// Block b writes the ids of its segment to every queue from the sum of the
+// counts of blocks 0 .. b-1 (each block adds them up itself; block 0 adds
+// up all of them for the queue lengths).  One block-wide scan per 1024-id
+// tile gives each thread its place in all queues at once: the queues' tile
+// counts (at most 1024 each) are packed 12 bits apart.
 __global__ __launch_bounds__(kBlock) void wl_scatter_kernel(uint8_t* f, int clear, uint64_t S, const uint32_t* deg,
-                                                            WlSplit sp, const uint64_t* bsum, WlQueues qs) {
+                                                            WlSplit sp, const uint64_t* bsum, WlQueues qs,
+                                                            uint32_t* lens) {
+  static_assert(kWlTile <= 4095 && kWlQueues * 12 <= 64, "packed tile counts");
+  __shared__ unsigned long long s_pre[kWlQueues];
   const uint64_t lo = (uint64_t)blockIdx.x << sp.shift, hi = min(S, lo + (1ull << sp.shift));
-  if (sp.dirty) {  // block-uniform
-    if (!sp.dirty[blockIdx.x]) return;
-    __syncthreads();  // every thread has read the mark
-    if (clear && threadIdx.x == 0) sp.dirty[blockIdx.x] = 0;
+  const bool skip = sp.dirty && !sp.dirty[blockIdx.x];  // block-uniform: no flag in this segment
+  if (skip && blockIdx.x != 0) return;
+  if (threadIdx.x < kWlQueues) s_pre[threadIdx.x] = 0;
+  __syncthreads();
+  {
+    const uint32_t nb = blockIdx.x == 0 ? kWlScanBlocks : blockIdx.x;
+    unsigned long long loc[kWlQueues] = {};
+    for (uint32_t j = threadIdx.x; j < nb; j += kBlock)
+#pragma unroll
+      for (int q = 0; q < kWlQueues; ++q) loc[q] += bsum[q * kWlScanBlocks + j];
+#pragma unroll
+    for (int q = 0; q < kWlQueues; ++q) {
+      const unsigned long long t = wave_sum_u64(loc[q]);
+      if (threadIdx.x % 64 == 0 && t) atomicAdd(&s_pre[q], t);
+    }
   }
+  __syncthreads();
   uint64_t r[kWlQueues];
 #pragma unroll
-  for (int q = 0; q < kWlQueues; ++q) r[q] = bsum[q * kWlScanBlocks + blockIdx.x];
+  for (int q = 0; q < kWlQueues; ++q) r[q] = blockIdx.x == 0 ? 0 : s_pre[q];
+  if (blockIdx.x == 0 && threadIdx.x < kWlQueues) lens[threadIdx.x] = (uint32_t)s_pre[threadIdx.x];
+  if (skip) return;
+  if (sp.dirty && clear && threadIdx.x == 0) sp.dirty[blockIdx.x] = 0;
   for (uint64_t t0 = lo; t0 < hi; t0 += kWlTile) {  // block-uniform trip count
     const uint64_t id0 = t0 + 4 * (uint64_t)threadIdx.x;
     uint32_t m[kWlQueues] = {};
     const bool any = id0 < hi && wl_bits(f, deg, S, sp, id0, m);
+    uint64_t packed = 0;
+#pragma unroll
+    for (int q = 0; q < kWlQueues; ++q) packed |= (uint64_t)__popc(m[q]) << (12 * q);
+    uint64_t tot;
+    const uint64_t ex = block_scan_excl<kBlock>(packed, tot);
 #pragma unroll
     for (int q = 0; q < kWlQueues; ++q) {
-      uint64_t tq;
-      uint64_t e = block_scan_excl<kBlock>(__popc(m[q]), tq);
+      uint64_t e = r[q] + ((ex >> (12 * q)) & 0xFFF);
 #pragma unroll
       for (int j = 0; j < 4; ++j)
-        if (m[q] & (1u << j)) qs.q[q][r[q] + e++] = (uint32_t)(id0 + j);
-      r[q] += tq;
+        if (m[q] & (1u << j)) qs.q[q][e++] = (uint32_t)(id0 + j);
+      r[q] += (tot >> (12 * q)) & 0xFFF;
     }
     if (clear && any) reinterpret_cast<uint32_t*>(f)[id0 >> 2] = 0;
   }
@@ -2508,8 +2528,7 @@ void k_wl_queue(uint8_t* f, bool clear, uint64_t S, const uint32_t* deg, const W
   WlSplit sp = sp_in;
   sp.shift = wl_shift(S);
   LV_LAUNCH(wl_count_kernel, kWlScanBlocks, kBlock, st, f, S, deg, sp, bsum);
-  LV_LAUNCH(wl_scan_kernel, 1, kBlock, st, bsum, lens);
-  LV_LAUNCH(wl_scatter_kernel, kWlScanBlocks, kBlock, st, f, (int)clear, S, deg, sp, bsum, qs);
+  LV_LAUNCH(wl_scatter_kernel, kWlScanBlocks, kBlock, st, f, (int)clear, S, deg, sp, bsum, qs, lens);
 }
 void k_modularity_parts(GView g, bool f64, const uint32_t* memb, double* cin, double* ctot, hipStream_t st) {
   if (!g.S) return;
