@@ -582,6 +582,9 @@ def main():
                 "bytes_per_launch": main_k["bytes_per_launch"], "launches": main_k["launches"],
                 "traffic_source": traffic and traffic["source"], "hub_batch": hub_k,
                 "other_kernels": [lines[k] for k in (0, 2, 3) if k != dom]}
+    log("kernel stats per slot: " + "; ".join(
+        f"{k}: launches {st['move_launches'][k]} ms {st['move_ms'][k]:.3f} vertices {st['vertices'][k]} "
+        f"edges {st['edges'][k]} candidates {st['candidates'][k]} moves {st['moves'][k]}" for k in range(5)))
     log(f"roofline: {main_k['kernel']}: {main_k['bytes_per_launch'] / 1e6:.2f} MB algorithmic per launch, "
         f"{main_k['avg_launch_us']:.1f} us -> {main_k['achieved']:.1f} GB/s; hub batch {hub_k['avg_launch_us']:.1f} us, "
         f"{hub_k['achieved']:.1f} GB/s; per launch " + ", ".join(

@@ -775,6 +775,12 @@ int local_move_worklist(lv_context* c, const Level& lv, uint32_t* vcom, const do
     a.ht_key = ht->ht_key;
     a.ht_cnt = ht->ht_cnt;
     a.ht_w = ht->ht_w;
+    // moved tabled hubs, for the grid-wide flag pass (count zeroed by every compaction)
+    a.mv_cap = 4096;
+    a.mv_u = c->ws<uint32_t>("mv.u", a.mv_cap);
+    a.mv_from = c->ws<uint32_t>("mv.from", a.mv_cap);
+    a.mv_to = c->ws<uint32_t>("mv.to", a.mv_cap);
+    a.mv_n = c->ws<uint32_t>("mv.n", 1);
   }
   batched_args(a, bt);
   batched_schedule(c, a, lv, bt);
@@ -818,6 +824,7 @@ int local_move_worklist(lv_context* c, const Level& lv, uint32_t* vcom, const do
     sp.t[2] = wl_big_hub(c);
     sp.big = a.hb_big;  // hubs marked for the batch
     sp.tab = a.ht_cap;  // hubs with a table
+    sp.reset = a.mv_n;  // the moved-hub list starts empty in every sub-round
   }
   WlSplit sp_same = sp;  // next-sub-round compactions read the marked segments only
   sp_same.dirty = a.wl_dirty;
@@ -850,23 +857,23 @@ int local_move_worklist(lv_context* c, const Level& lv, uint32_t* vcom, const do
       // through the id-order kernel, hubs with a table (a wave each) then
       // the other hubs (a block each, mid then small), and the largest hubs
       // (multi-block batch).  Like the reference's threads, each sees the
-      // moves the others have made.  The low-degree kernel is enqueued
-      // first: the host's launches of the rest overlap it.
+      // moves the others have made.
       if (S) {
         c->make_side();
         c->fork();
+        if (a.ht_cap) {  // stats slot 1; its few blocks are resident before the low-degree kernel's many
+          a.list = qs.q[4];
+          a.n_dev = lens + 4;
+          a.cnt = cnt0 + (size_t)1 * CNT_N;
+          launch_move(c, a, kTierTabHub, lv.f64, c->side[0]);
+          k_moved_flag(a, c->side[0]);
+        }
       }
       a.list = qs.q[0];
       a.n_dev = lens;
       a.cnt = cnt0;
       launch_move(c, a, kTierHybrid, lv.f64);
       if (S) {
-        if (a.ht_cap) {  // stats slot 1
-          a.list = qs.q[4];
-          a.n_dev = lens + 4;
-          a.cnt = cnt0 + (size_t)1 * CNT_N;
-          launch_move(c, a, kTierTabHub, lv.f64, c->side[0]);
-        }
         if (scan_hubs) {
           for (int q = 2; q >= 1; --q) {  // stats slots 3 (mid) and 2 (small)
             a.list = qs.q[q];

@@ -74,6 +74,7 @@ struct WlSplit {
   const uint8_t* big;         // ids sent to the hub batch (nullable)
   const uint32_t* tab;        // hub table slots per id: hubs with a table go to queue 4 (nullable)
   uint8_t* dirty;             // per segment: a flag may be set (nullable: scan all)
+  uint32_t* reset;            // zeroed by the compaction (nullable)
   uint32_t shift;             // segment of block b: ids [b << shift, (b + 1) << shift)
 };
 // Segment shift of the compaction: 2^shift ids per block (a multiple of
@@ -218,6 +219,13 @@ struct MoveArgs {
   uint32_t* ht_key;
   int32_t* ht_cnt;
   double* ht_w;
+  // hubs with a table that moved, flagged by a grid-wide pass after their
+  // kernel (nullable mv_n: each wave flags its own)
+  uint32_t* mv_u;
+  uint32_t* mv_from;
+  uint32_t* mv_to;
+  uint32_t* mv_n;
+  uint32_t mv_cap;
   uint64_t* hb_off;            // hub's row offset and degree
   uint32_t* hb_deg;
   uint32_t* hb_cown;           // list entry owning each edge chunk
@@ -243,6 +251,8 @@ void k_flag_nonzero(const uint32_t* a, uint64_t n, uint8_t* flags, hipStream_t s
 // Entries of worklist queues 1-3 back to the next sub-round; count of the
 // vertices above thr edges without a hub table (added to *out).
 void k_carry_hubs(const MoveArgs& a, const WlQueues& qs, const uint32_t* lens, hipStream_t st);
+// Neighbours of the tabled hubs that moved (a.mv_*), flagged over the grid.
+void k_moved_flag(const MoveArgs& a, hipStream_t st);
 void k_ht_new_hubs(const uint32_t* deg, uint64_t S, uint32_t* cap, uint64_t* off, unsigned long long* next,
                    uint64_t total, uint32_t* list, uint32_t* nlist, hipStream_t st);
 void k_untabled(const uint32_t* deg, const uint32_t* cap, uint64_t S, uint32_t thr, unsigned long long* out,

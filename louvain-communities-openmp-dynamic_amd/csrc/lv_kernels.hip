@@ -246,6 +246,61 @@ __device__ __forceinline__ void ht_move(const MoveArgs& a, uint32_t x, double w,
   unsafeAtomicAdd(&a.ht_w[s], w);
 }
 
+// ht_move with x's table size already read (cx > kHtDropped).
+__device__ __forceinline__ void ht_move_cap(const MoveArgs& a, uint32_t x, uint32_t cx, double w, uint32_t from,
+                                            uint32_t to) {
+  const uint64_t off = a.ht_off[x];
+  int64_t s = ht_slot(a.ht_key, off, cx, from, false);
+  if (s >= 0) {
+    atomicAdd(&a.ht_cnt[s], -1);
+    unsafeAtomicAdd(&a.ht_w[s], -w);
+  }
+  s = ht_slot(a.ht_key, off, cx, to, true);
+  if (s < 0) {
+    __hip_atomic_store(&a.ht_cap[x], kHtDropped, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return;
+  }
+  atomicAdd(&a.ht_cnt[s], 1);
+  unsafeAtomicAdd(&a.ht_w[s], w);
+}
+
+// A vertex u that moved from community `from` to `to` (row at o, d edges)
+// flags its neighbours (inc/louvain.hxx:602) and moves their hub tables
+// with it.  Its G lanes take four rounds of G edges at a time: the
+// round's key and weight loads, then its flags with the stamp and table-size
+// loads, then the next-sub-round marks and table updates.
+template <int G, class WT>
+__device__ __forceinline__ void flag_neighbours(const MoveArgs& a, const WT* W, uint32_t u, uint64_t o, uint32_t d,
+                                                int lane, uint32_t from, uint32_t to) {
+  constexpr int R = 4;
+  const uint32_t serial = a.fsame ? cur_serial(a) : 0;
+  for (uint32_t b = 0; b < d; b += R * G) {
+    uint32_t v[R], sv[R], cx[R];
+    double w[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const uint32_t i = b + r * G + lane;
+      v[r] = i < d ? a.g.keys[o + i] : u;
+      w[r] = i < d && a.ht_cap ? (double)W[o + i] : 0.0;
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      sv[r] = serial;
+      cx[r] = 0;
+      if (v[r] == u) continue;
+      __hip_atomic_store(&a.vaff[v[r]], (uint8_t)1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (a.fsame && v[r] > u) sv[r] = __hip_atomic_load(&a.stamp[v[r]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (a.ht_cap) cx[r] = __hip_atomic_load(&a.ht_cap[v[r]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      // flagged by a lower id and not visited this iteration: next sub-round
+      if (sv[r] != serial) mark_same(a, v[r]);
+      if (cx[r] > kHtDropped) ht_move_cap(a, v[r], cx[r], w[r], from, to);
+    }
+  }
+}
+
 // ---------------------------------------------------------------- fills
 template <class T>
 __global__ void fill_kernel(T* a, T v, uint64_t n) {
@@ -672,13 +727,7 @@ __global__ __launch_bounds__(BLOCK) void move_kernel(MoveArgs a) {
               ++c_move;
               c_medge += d;
             }
-            for (uint32_t i = lane; i < d; i += G) {
-              uint32_t v = a.g.keys[o + i];
-              if (v != u) {
-                flag_vertex(a.vaff, v, u, a);
-                ht_move(a, v, (double)W[o + i], cu, bc);
-              }
-            }
+            flag_neighbours<G, WT>(a, W, u, o, d, lane, cu, bc);
           }
           if (lane == 0) {
             st_dev(&a.vaff[u], (uint8_t)0);
@@ -1305,13 +1354,7 @@ __global__ __launch_bounds__(BLOCK) void hub_mid_kernel(MoveArgs a) {
                 ht_move(a, v[r], w[r], cu, bc);
               }
           } else {
-            for (uint32_t i = threadIdx.x; i < d; i += BLOCK) {
-              const uint32_t x = a.g.keys[o + i];
-              if (x != u) {
-                flag_vertex(a.vaff, x, u, a);
-                ht_move(a, x, (double)W[o + i], cu, bc);
-              }
-            }
+            flag_neighbours<BLOCK, WT>(a, W, u, o, d, threadIdx.x, cu, bc);
           }
         }
         if (threadIdx.x == 0) {
@@ -1438,20 +1481,25 @@ __global__ __launch_bounds__(kBlock) void hub_tab_kernel(MoveArgs a) {
     }
     // a candidate exists and it is not community 0 of u's graph
     if (be > 0 && bc != vertex_zero(a, u)) {
+      uint32_t slot = kEmpty;
       if (lane == 0) {
         unsafeAtomicAdd(&a.ctot[cu], -Ku);
         unsafeAtomicAdd(&a.ctot[bc], Ku);
         st_dev(&a.vcom[u], bc);
         ++c_move;
         c_medge += d;
-      }
-      for (uint32_t i = lane; i < d; i += 64) {
-        const uint32_t v = a.g.keys[o + i];
-        if (v != u) {
-          flag_vertex(a.vaff, v, u, a);
-          ht_move(a, v, (double)W[o + i], cu, bc);
+        // its neighbours are flagged by the grid-wide pass after this kernel
+        if (a.mv_n) {
+          slot = atomicAdd(a.mv_n, 1u);
+          if (slot < a.mv_cap) {
+            a.mv_u[slot] = u;
+            a.mv_from[slot] = cu;
+            a.mv_to[slot] = bc;
+          }
         }
       }
+      slot = __shfl(slot, 0, 64);
+      if (!a.mv_n || slot >= a.mv_cap) flag_neighbours<64, WT>(a, W, u, o, d, lane, cu, bc);
     }
     if (lane == 0) {
       st_dev(&a.vaff[u], (uint8_t)0);
@@ -1468,6 +1516,22 @@ __global__ __launch_bounds__(kBlock) void hub_tab_kernel(MoveArgs a) {
     unsigned long long c = block_sum_u64<kBlock>(c_cand), m = block_sum_u64<kBlock>(c_move);
     unsigned long long me = block_sum_u64<kBlock>(c_medge);
     if (threadIdx.x == 0) add_counts(a.cnt, v, ed, c, m, me);
+  }
+}
+
+// Neighbours of the tabled hubs that moved in the kernel before (list
+// a.mv_*, at most a.mv_cap of *a.mv_n): every block takes its share of each
+// hub's row.
+template <class WT>
+__global__ __launch_bounds__(kBlock) void moved_flag_kernel(MoveArgs a) {
+  const uint32_t n = min(*a.mv_n, a.mv_cap);
+  const WT* W = (const WT*)a.g.w;
+  for (uint32_t m = 0; m < n; ++m) {
+    const uint32_t u = a.mv_u[m], d = a.g.deg[u];
+    const uint64_t o = a.g.off[u];
+    const uint32_t share = (d + gridDim.x - 1) / gridDim.x, lo = blockIdx.x * share, hi = min(d, lo + share);
+    if (lo >= hi) continue;
+    flag_neighbours<kBlock, WT>(a, W, u, o + lo, hi - lo, threadIdx.x, a.mv_from[m], a.mv_to[m]);
   }
 }
 
@@ -2186,6 +2250,7 @@ __global__ __launch_bounds__(kBlock) void wl_scatter_kernel(uint8_t* f, int clea
 #pragma unroll
   for (int q = 0; q < kWlQueues; ++q) r[q] = blockIdx.x == 0 ? 0 : s_pre[q];
   if (blockIdx.x == 0 && threadIdx.x < kWlQueues) lens[threadIdx.x] = (uint32_t)s_pre[threadIdx.x];
+  if (blockIdx.x == 0 && threadIdx.x == 0 && sp.reset) *sp.reset = 0;
   if (skip) return;
   if (sp.dirty && clear && threadIdx.x == 0) sp.dirty[blockIdx.x] = 0;
   for (uint64_t t0 = lo; t0 < hi; t0 += kWlTile) {  // block-uniform trip count
@@ -2413,6 +2478,10 @@ __global__ void flag_nonzero_kernel(const uint32_t* a, uint64_t n, uint8_t* flag
 }
 void k_flag_nonzero(const uint32_t* a, uint64_t n, uint8_t* flags, hipStream_t st) {
   if (n) LV_LAUNCH(flag_nonzero_kernel, grid_for(n, kBlock * 4), kBlock, st, a, n, flags);
+}
+void k_moved_flag(const MoveArgs& a, hipStream_t st) {
+  if (a.g.w && a.f64_weights) LV_LAUNCH(moved_flag_kernel<double>, 256, kBlock, st, a);
+  else LV_LAUNCH(moved_flag_kernel<float>, 256, kBlock, st, a);
 }
 void k_carry_hubs(const MoveArgs& a, const WlQueues& qs, const uint32_t* lens, hipStream_t st) {
   LV_LAUNCH(carry_hubs_kernel, 64, kBlock, st, a, qs, lens);
