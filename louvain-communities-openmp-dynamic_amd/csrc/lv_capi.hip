@@ -231,14 +231,14 @@ struct lv_context {
     graphs.clear();
     seen.clear();
   }
-  // side streams wait for the work enqueued on st so far
-  void fork() {
+  // the first n side streams wait for the work enqueued on st so far
+  void fork(int n = 2) {
     HIPCHK(hipEventRecord(fork_ev, st));
-    for (int k = 0; k < 2; ++k) HIPCHK(hipStreamWaitEvent(side[k], fork_ev, 0));
+    for (int k = 0; k < n; ++k) HIPCHK(hipStreamWaitEvent(side[k], fork_ev, 0));
   }
-  // st waits for the work enqueued on the side streams so far
-  void join() {
-    for (int k = 0; k < 2; ++k) {
+  // st waits for the work enqueued on the first n side streams so far
+  void join(int n = 2) {
+    for (int k = 0; k < n; ++k) {
       HIPCHK(hipEventRecord(join_ev[k], side[k]));
       HIPCHK(hipStreamWaitEvent(st, join_ev[k], 0));
     }
@@ -299,7 +299,10 @@ struct lv_dstate {
   uint64_t* ht_off = nullptr;
   uint32_t* ht_cap = nullptr;
   uint64_t ht_slots = 0;              // slab slots allocated
-  unsigned long long* ht_next = nullptr;  // first spare slot (device)
+  unsigned long long* ht_next = nullptr;  // device: first spare slot, tables dropped since the build
+  // some hub may be without a table (tables dropped by the last run's end,
+  // or a build not yet run through a worklist pass)
+  bool ht_scan = true;
   uint32_t* ht_key = nullptr;
   int32_t* ht_cnt = nullptr;
   double* ht_w = nullptr;
@@ -775,6 +778,7 @@ int local_move_worklist(lv_context* c, const Level& lv, uint32_t* vcom, const do
     a.ht_key = ht->ht_key;
     a.ht_cnt = ht->ht_cnt;
     a.ht_w = ht->ht_w;
+    a.ht_ndrop = ht->ht_next + 1;
     // moved tabled hubs, for the grid-wide flag pass (count zeroed by every compaction)
     a.mv_cap = 4096;
     a.mv_u = c->ws<uint32_t>("mv.u", a.mv_cap);
@@ -792,7 +796,8 @@ int local_move_worklist(lv_context* c, const Level& lv, uint32_t* vcom, const do
   // queues: low degree, small hubs, mid hubs, hub batch
   WlQueues qs;
   for (int q = 0; q < kWlQueues; ++q) qs.q[q] = c->ws<uint32_t>("wl.q" + std::to_string(q), stride);
-  uint32_t* lens = c->ws<uint32_t>("wl.len", kWlQueues);
+  // queue lengths, then the tables dropped so far (when there are tables)
+  uint32_t* lens = c->ws<uint32_t>("wl.len", kWlQueues + 1);
   uint64_t* bsum = c->ws<uint64_t>("wl.bsum", (size_t)kWlQueues * kWlScanBlocks);
   // next-sub-round flags: all zero between calls (each compaction resets
   // the flags it reads, and a sub-round loop ends on an empty compaction)
@@ -825,6 +830,7 @@ int local_move_worklist(lv_context* c, const Level& lv, uint32_t* vcom, const do
     sp.big = a.hb_big;  // hubs marked for the batch
     sp.tab = a.ht_cap;  // hubs with a table
     sp.reset = a.mv_n;  // the moved-hub list starts empty in every sub-round
+    sp.ndrop = a.ht_ndrop;
   }
   WlSplit sp_same = sp;  // next-sub-round compactions read the marked segments only
   sp_same.dirty = a.wl_dirty;
@@ -840,16 +846,12 @@ int local_move_worklist(lv_context* c, const Level& lv, uint32_t* vcom, const do
     a.serial_dev = serial_dev;
     if (S) c->make_side();
   }
-  // With hub tables and no hub without one, the edge-scanning hub kernels
-  // are not launched (a table dropped during the run sends its hub to them:
-  // they are launched from the next group on).
-  bool scan_hubs = true;
-  if (a.ht_cap) {
-    unsigned long long* nu = c->ws<unsigned long long>("ht.untabled", 1);
-    k_fill_u64(reinterpret_cast<uint64_t*>(nu), 0, 1, c->st);
-    k_untabled(lv.g.deg, a.ht_cap, S, c->hub_min_wl, nu, c->st);
-    scan_hubs = c->read1(nu) != 0;
-  }
+  // With hub tables and no table dropped by the end of the last run, the
+  // edge-scanning hub kernels are not launched.  A table dropped since (by
+  // this run's batch or a move) sends its hub to them: they are launched from
+  // the next group on, the group after its hub was queued.  Below kHtMinDeg
+  // edges hubs have no table.
+  bool scan_hubs = !a.ht_cap || ht->ht_scan || c->hub_min_wl < kHtMinDeg;
   // one sub-round: hubs and the rest, then the next queue
   auto sub_round = [&]() {
     if (c->tier_order == 2) {
@@ -858,9 +860,10 @@ int local_move_worklist(lv_context* c, const Level& lv, uint32_t* vcom, const do
       // the other hubs (a block each, mid then small), and the largest hubs
       // (multi-block batch).  Like the reference's threads, each sees the
       // moves the others have made.
+      const int sides = scan_hubs ? 2 : 1;  // the hub batch's stream only while it runs
       if (S) {
         c->make_side();
-        c->fork();
+        c->fork(sides);
         if (a.ht_cap) {  // stats slot 1; its few blocks are resident before the low-degree kernel's many
           a.list = qs.q[4];
           a.n_dev = lens + 4;
@@ -888,7 +891,7 @@ int local_move_worklist(lv_context* c, const Level& lv, uint32_t* vcom, const do
         } else {
           k_carry_hubs(a, qs, lens, c->side[0]);  // tables dropped meanwhile: kept for later
         }
-        c->join();
+        c->join(sides);
       }
     } else {
       k_fill_u32(tlen, 0, kTiers, c->st);
@@ -941,8 +944,9 @@ int local_move_worklist(lv_context* c, const Level& lv, uint32_t* vcom, const do
       };
       if (!graphs || !c->replay(group_key(), group)) group();
       c->check();
-      uint32_t nq[kWlQueues];
-      c->read(nq, lens, kWlQueues);
+      uint32_t nq[kWlQueues + 1];
+      c->read(nq, lens, sp.ndrop ? kWlQueues + 1 : kWlQueues);
+      if (sp.ndrop) ht->ht_scan = nq[kWlQueues] != 0;
       uint64_t left = 0;
       for (int q = 0; q < kWlQueues; ++q) left += nq[q];
       if (!left) break;
@@ -1119,8 +1123,10 @@ void build_hub_tables(lv_context* c, const lv_dgraph* x, lv_dstate* s, const uin
   const uint64_t used = c->read1(s->ht_off + S);
   // spare slots for vertices that pass kHtMinDeg edges in later runs
   const uint64_t slots = used + std::max<uint64_t>(used / 4, (uint64_t)1024 * kHtMaxCap);
-  if (!s->ht_next) s->ht_next = dalloc<unsigned long long>(1);
+  if (!s->ht_next) s->ht_next = dalloc<unsigned long long>(2);
   k_fill_u64(reinterpret_cast<uint64_t*>(s->ht_next), used, 1, c->st);
+  k_fill_u64(reinterpret_cast<uint64_t*>(s->ht_next) + 1, 0, 1, c->st);
+  s->ht_scan = true;
   if (slots > s->ht_slots) {
     (void)hipFree(s->ht_key);
     (void)hipFree(s->ht_cnt);
@@ -1140,7 +1146,8 @@ void build_hub_tables(lv_context* c, const lv_dgraph* x, lv_dstate* s, const uin
   WlQueues qs{};
   qs.q[0] = list;
   k_wl_queue(flags, true, S, x->deg, WlSplit{}, qs, lens, bsum, c->st);
-  k_ht_build(x->view(), x->f64, vcom, list, lens, s->ht_off, s->ht_cap, s->ht_key, s->ht_cnt, s->ht_w, c->st);
+  k_ht_build(x->view(), x->f64, vcom, list, lens, s->ht_off, s->ht_cap, s->ht_key, s->ht_cnt, s->ht_w,
+             s->ht_next + 1, c->st);
 }
 
 void invoke(lv_context* c, int approach, const lv_dgraph* x, const lv_dbatch* b, lv_dstate* s, const lv_options* o,
@@ -1187,14 +1194,15 @@ void invoke(lv_context* c, int approach, const lv_dgraph* x, const lv_dbatch* b,
   if (worklist && c->tier_order == 2 && c->use_tables && o->repeat == 1 && approach != LV_STATIC) {
     if (s->ht_valid && s->ht_live + b->ni - b->nd == x->live) {
       k_ht_batch(b->du, b->dv, b->dw, b->nd, b->iu, b->iv, b->iw, b->ni, ucom, s->ht_off, s->ht_cap, s->ht_key,
-                 s->ht_cnt, s->ht_w, c->st);
+                 s->ht_cnt, s->ht_w, s->ht_next + 1, c->st);
       // vertices that passed kHtMinDeg edges with this batch: tables from
       // the spare slots, built from this graph (after the batch deltas)
       uint32_t* nl = c->ws<uint32_t>("ht.newlist", std::max<uint64_t>(S, 1));
       uint32_t* nn = c->ws<uint32_t>("ht.newn", 1);
       k_fill_u32(nn, 0, 1, c->st);
-      k_ht_new_hubs(x->deg, S, s->ht_cap, s->ht_off, s->ht_next, s->ht_slots, nl, nn, c->st);
-      k_ht_build(x->view(), x->f64, ucom, nl, nn, s->ht_off, s->ht_cap, s->ht_key, s->ht_cnt, s->ht_w, c->st);
+      k_ht_new_hubs(b->iu, b->ni, x->deg, s->ht_cap, s->ht_off, s->ht_next, s->ht_slots, nl, nn, c->st);
+      k_ht_build(x->view(), x->f64, ucom, nl, nn, s->ht_off, s->ht_cap, s->ht_key, s->ht_cnt, s->ht_w,
+                 s->ht_next + 1, c->st);
       ht = s;
     } else if (s->runs >= 1) {
       build_hub_tables(c, x, s, ucom);

@@ -75,6 +75,7 @@ struct WlSplit {
   const uint32_t* tab;        // hub table slots per id: hubs with a table go to queue 4 (nullable)
   uint8_t* dirty;             // per segment: a flag may be set (nullable: scan all)
   uint32_t* reset;            // zeroed by the compaction (nullable)
+  const unsigned long long* ndrop;  // copied to lens[kWlQueues], saturated to u32 (nullable)
   uint32_t shift;             // segment of block b: ids [b << shift, (b + 1) << shift)
 };
 // Segment shift of the compaction: 2^shift ids per block (a multiple of
@@ -219,6 +220,7 @@ struct MoveArgs {
   uint32_t* ht_key;
   int32_t* ht_cnt;
   double* ht_w;
+  unsigned long long* ht_ndrop;  // tables dropped since the build (counted once each)
   // hubs with a table that moved, flagged by a grid-wide pass after their
   // kernel (nullable mv_n: each wave flags its own)
   uint32_t* mv_u;
@@ -242,27 +244,29 @@ void k_hub_tab(const MoveArgs& a, hipStream_t st);
 // Hub tables: slots per vertex (cap[u] = min(nextpow2(2 deg), kHtMaxCap) for
 // degree > kHtMinDeg, else 0), hub flags for the build list; the build from
 // the edges and a membership (a table with too many communities is dropped,
-// cap[u] = 0); the batch's edges into the tables (deletions out, insertions
-// in, with the membership before any move).
+// cap[u] = kHtDropped, and counted in *ndrop); the batch's edges into the
+// tables (deletions out, insertions in, with the membership before any move).
 void k_ht_caps(const uint32_t* deg, uint64_t S, uint32_t* cap, uint64_t* cap64, uint8_t* flags, hipStream_t st);
 void k_ht_build(GView g, bool f64, const uint32_t* vcom, const uint32_t* list, const uint32_t* n_dev,
-                const uint64_t* off, uint32_t* cap, uint32_t* key, int32_t* cnt, double* w, hipStream_t st);
+                const uint64_t* off, uint32_t* cap, uint32_t* key, int32_t* cnt, double* w,
+                unsigned long long* ndrop, hipStream_t st);
 void k_flag_nonzero(const uint32_t* a, uint64_t n, uint8_t* flags, hipStream_t st);
-// Entries of worklist queues 1-3 back to the next sub-round; count of the
-// vertices above thr edges without a hub table (added to *out).
+// Entries of worklist queues 1-3 back to the next sub-round.
 void k_carry_hubs(const MoveArgs& a, const WlQueues& qs, const uint32_t* lens, hipStream_t st);
 // Neighbours of the tabled hubs that moved (a.mv_*), flagged over the grid.
 void k_moved_flag(const MoveArgs& a, hipStream_t st);
-void k_ht_new_hubs(const uint32_t* deg, uint64_t S, uint32_t* cap, uint64_t* off, unsigned long long* next,
-                   uint64_t total, uint32_t* list, uint32_t* nlist, hipStream_t st);
-void k_untabled(const uint32_t* deg, const uint32_t* cap, uint64_t S, uint32_t thr, unsigned long long* out,
-                hipStream_t st);
+// Sources of a batch's insertions (iu) that now have more than kHtMinDeg
+// edges and no table: a table each from the spare slots (bump counter
+// next[0], slots up to total), listed for a build; with no spare slot left,
+// dropped (counted in next[1]).
+void k_ht_new_hubs(const uint32_t* iu, uint64_t ni, const uint32_t* deg, uint32_t* cap, uint64_t* off,
+                   unsigned long long* next, uint64_t total, uint32_t* list, uint32_t* nlist, hipStream_t st);
 void k_ht_check(GView g, bool f64, const uint32_t* vcom, const uint32_t* list, const uint32_t* n_dev,
                 const uint64_t* off, uint32_t* cap, uint32_t* key, int32_t* cnt, double* w, unsigned long long* out,
                 hipStream_t st);
 void k_ht_batch(const uint32_t* du, const uint32_t* dv, const float* dw, uint64_t nd, const uint32_t* iu,
                 const uint32_t* iv, const float* iw, uint64_t ni, const uint32_t* vcom, const uint64_t* off,
-                uint32_t* cap, uint32_t* key, int32_t* cnt, double* w, hipStream_t st);
+                uint32_t* cap, uint32_t* key, int32_t* cnt, double* w, unsigned long long* ndrop, hipStream_t st);
 // Worklist hubs from a.list (length *a.n_dev), one block each with an LDS
 // table: class 0 up to kSmallHubMax edges, class 1 up to kMidHubMax.
 void k_hub_mid(const MoveArgs& a, int cls, hipStream_t st);

@@ -210,15 +210,21 @@ __device__ __forceinline__ int64_t ht_slot(uint32_t* key, uint64_t off, uint32_t
   return -1;
 }
 
+// x's table dropped (x is then evaluated from its edges); *ndrop counts
+// each dropped table once.
+__device__ __forceinline__ void ht_drop(uint32_t* cap, uint32_t x, unsigned long long* ndrop) {
+  if (atomicExch(&cap[x], kHtDropped) > kHtDropped && ndrop) atomicAdd(ndrop, 1ull);
+}
+
 // count n and weight w of community c in x's table; a table with no room
-// for c is dropped (x is then evaluated from its edges).
+// for c is dropped.
 __device__ __forceinline__ void ht_add(uint32_t* cap, const uint64_t* off, uint32_t* key, int32_t* cnt, double* wt,
-                                       uint32_t x, uint32_t c, int32_t n, double w) {
+                                       unsigned long long* ndrop, uint32_t x, uint32_t c, int32_t n, double w) {
   const uint32_t cx = __hip_atomic_load(&cap[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   if (cx <= kHtDropped) return;
   const int64_t s = ht_slot(key, off[x], cx, c, n > 0);
   if (s < 0) {
-    if (n > 0) __hip_atomic_store(&cap[x], kHtDropped, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (n > 0) ht_drop(cap, x, ndrop);
     return;
   }
   atomicAdd(&cnt[s], n);
@@ -239,7 +245,7 @@ __device__ __forceinline__ void ht_move(const MoveArgs& a, uint32_t x, double w,
   }
   s = ht_slot(a.ht_key, off, cx, to, true);
   if (s < 0) {
-    __hip_atomic_store(&a.ht_cap[x], kHtDropped, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    ht_drop(a.ht_cap, x, a.ht_ndrop);
     return;
   }
   atomicAdd(&a.ht_cnt[s], 1);
@@ -257,7 +263,7 @@ __device__ __forceinline__ void ht_move_cap(const MoveArgs& a, uint32_t x, uint3
   }
   s = ht_slot(a.ht_key, off, cx, to, true);
   if (s < 0) {
-    __hip_atomic_store(&a.ht_cap[x], kHtDropped, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    ht_drop(a.ht_cap, x, a.ht_ndrop);
     return;
   }
   atomicAdd(&a.ht_cnt[s], 1);
@@ -319,47 +325,116 @@ __global__ void degrees_kernel(const uint64_t* off, uint32_t* deg, uint64_t S) {
 }
 
 // --------------------------------------------- per-vertex row sums (K, 2M)
-// louvainVertexWeightsOmpW (inc/louvain.hxx:210-218) and edgeWeightOmp
-// (inc/properties.hxx:96-106).  Rows of up to 64 edges are summed by one
-// lane in row order (the reference's order); longer rows by the whole wave.
-template <class WT, bool TOTAL>
-__global__ __launch_bounds__(kBlock) void row_sum_kernel(GView g, double* vtot, double* total) {
+// Vertex of this thread in the grid-stride step at `base` (steps of N =
+// the grid's threads): thread t takes base + (t x P mod N), P a prime above
+// any N (a bijection of [0, N)).  R-MAT-like graphs put their long rows at
+// ids with many zero bits, so plain or power-of-two strided assignments give
+// a few waves most of the long rows (each wave reads its long rows one after
+// another); the scrambled one spreads them over the grid.
+__device__ __forceinline__ uint64_t scrambled_vertex(uint64_t base) {
+  const uint64_t N = (uint64_t)gridDim.x * blockDim.x;
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  return base + (t * 2654435761ull) % N;
+}
+
+// louvainVertexWeightsOmpW (inc/louvain.hxx:210-218).  Rows of up to 64
+// edges are summed by one lane in row order (the reference's order); longer
+// rows by the whole wave (lane i sums edges i, i + 64, ... in that order,
+// then a butterfly over the lanes).  Eight loads are issued before their
+// sums, which keep that order.
+template <class WT>
+__global__ __launch_bounds__(kBlock) void row_sum_kernel(GView g, double* vtot) {
   const WT* W = (const WT*)g.w;
-  double acc = 0;
-  int lane = __lane_id();
-  for (uint64_t base = (blockIdx.x * (uint64_t)kBlock) & ~63ull; base < g.S;
-       base += (uint64_t)gridDim.x * kBlock) {
-    uint64_t u = base + (threadIdx.x & ~63u) + lane;
-    bool live = u < g.S && (!g.ex || g.ex[u]);
-    uint64_t o = live ? g.off[u] : 0;
-    uint32_t d = live ? g.deg[u] : 0;
-    bool big = live && d > 64;
+  const int lane = __lane_id();
+  for (uint64_t base = 0; base < g.S; base += (uint64_t)gridDim.x * kBlock) {  // grid-uniform
+    const uint64_t u = scrambled_vertex(base);
+    const bool live = u < g.S && (!g.ex || g.ex[u]);
+    const uint64_t o = live ? g.off[u] : 0;
+    const uint32_t d = live ? g.deg[u] : 0;
+    const bool big = live && d > 64;
     if (live && !big) {
       double s = 0;
-      for (uint32_t i = 0; i < d; ++i) s += (double)W[o + i];
-      if (TOTAL) acc += s;
-      else vtot[u] += s;
+      for (uint32_t i0 = 0; i0 < d; i0 += 8) {
+        double v[8];
+#pragma unroll
+        for (int r = 0; r < 8; ++r) v[r] = i0 + r < d ? (double)W[o + i0 + r] : 0.0;
+#pragma unroll
+        for (int r = 0; r < 8; ++r)
+          if (i0 + r < d) s += v[r];
+      }
+      vtot[u] += s;
     }
     unsigned long long mask = __ballot(big);
     while (mask) {
-      int j = __ffsll((long long)mask) - 1;
+      const int j = __ffsll((long long)mask) - 1;
       mask &= mask - 1;
-      uint64_t oj = __shfl(o, j);
-      uint32_t dj = __shfl(d, j);
+      const uint64_t oj = __shfl(o, j);
+      const uint32_t dj = __shfl(d, j);
       double s = 0;
-      for (uint32_t i = lane; i < dj; i += 64) s += (double)W[oj + i];
+      for (uint32_t i0 = 0; i0 < dj; i0 += 8 * 64) {
+        double v[8];
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+          const uint32_t i = i0 + r * 64 + lane;
+          v[r] = i < dj ? (double)W[oj + i] : 0.0;
+        }
+#pragma unroll
+        for (int r = 0; r < 8; ++r)
+          if (i0 + r * 64 + lane < dj) s += v[r];
+      }
 #pragma unroll
       for (int m = 32; m > 0; m >>= 1) s += __shfl_xor(s, m, 64);
-      if (lane == j) {
-        if (TOTAL) acc += s;
-        else vtot[u] += s;
+      if (lane == j) vtot[u] += s;
+    }
+  }
+}
+
+// edgeWeightOmp (inc/properties.hxx:96-106): 2M, the sum of every live
+// row's weights (in no fixed order, like the reference's reduction).  A
+// wave reads its 64 rows one at a time, lane i taking edge i, eight rows in
+// flight; edges past the 64th of a row eight 64-edge runs at a time.
+template <class WT>
+__global__ __launch_bounds__(kBlock) void edge_weight_kernel(GView g, double* total) {
+  const WT* W = (const WT*)g.w;
+  const int lane = __lane_id();
+  double acc = 0;
+  for (uint64_t base = 0; base < g.S; base += (uint64_t)gridDim.x * kBlock) {  // grid-uniform
+    const uint64_t u = scrambled_vertex(base);
+    const bool live = u < g.S && (!g.ex || g.ex[u]);
+    const uint64_t o = live ? g.off[u] : 0;
+    const uint32_t d = live ? g.deg[u] : 0;
+#pragma unroll
+    for (int j0 = 0; j0 < 64; j0 += 8) {
+      double v[8];
+#pragma unroll
+      for (int r = 0; r < 8; ++r) {
+        const uint64_t oj = __shfl(o, j0 + r);
+        const uint32_t dj = __shfl(d, j0 + r);
+        v[r] = (uint32_t)lane < dj ? (double)W[oj + lane] : 0.0;
+      }
+#pragma unroll
+      for (int r = 0; r < 8; ++r) acc += v[r];
+    }
+    unsigned long long mask = __ballot(d > 64);
+    while (mask) {
+      const int j = __ffsll((long long)mask) - 1;
+      mask &= mask - 1;
+      const uint64_t oj = __shfl(o, j);
+      const uint32_t dj = __shfl(d, j);
+      for (uint32_t i0 = 64; i0 < dj; i0 += 8 * 64) {
+        double v[8];
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+          const uint32_t i = i0 + r * 64 + lane;
+          v[r] = i < dj ? (double)W[oj + i] : 0.0;
+        }
+#pragma unroll
+        for (int r = 0; r < 8; ++r) acc += v[r];
       }
     }
   }
-  if (TOTAL) {
-    double t = block_sum<kBlock>(acc);
-    if (threadIdx.x == 0) unsafeAtomicAdd(total, t);
-  }
+  const double t = block_sum<kBlock>(acc);
+  if (threadIdx.x == 0) unsafeAtomicAdd(total, t);
 }
 
 // louvainInitializeOmpW (inc/louvain.hxx:285-293)
@@ -463,8 +538,8 @@ __global__ void ds_insertions_kernel(const uint32_t* iu, const uint32_t* iv, con
 __global__ __launch_bounds__(kBlock) void ds_sweep_kernel(GView g, const uint32_t* vcom, const uint8_t* neighbors,
                                                           const uint8_t* communities, uint8_t* vertices) {
   int lane = __lane_id();
-  for (uint64_t base = (blockIdx.x * (uint64_t)kBlock) & ~63ull; base < g.S; base += (uint64_t)gridDim.x * kBlock) {
-    uint64_t u = base + (threadIdx.x & ~63u) + lane;
+  for (uint64_t base = 0; base < g.S; base += (uint64_t)gridDim.x * kBlock) {  // grid-uniform
+    uint64_t u = scrambled_vertex(base);
     bool live = u < g.S && (!g.ex || g.ex[u]);
     bool nb = live && neighbors[u];
     if (live && communities[vcom[u]]) vertices[u] = 1;
@@ -1546,33 +1621,27 @@ __global__ void carry_hubs_kernel(MoveArgs a, WlQueues qs, const uint32_t* lens)
   }
 }
 
-// Vertices that passed kHtMinDeg edges since the tables were built (no
-// table, none dropped): a table each from the spare slots (bump counter at
-// *next, slots up to total), listed for a build; with no spare slots left,
-// dropped.
-__global__ void ht_new_hubs_kernel(const uint32_t* deg, uint64_t S, uint32_t* cap, uint64_t* off,
-                                   unsigned long long* next, uint64_t total, uint32_t* list, uint32_t* nlist) {
-  for (uint64_t u = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; u < S; u += (uint64_t)gridDim.x * blockDim.x) {
+// Sources of the batch's insertions that passed kHtMinDeg edges with it (a
+// source's degree changes only with its own entries) and have no table: a
+// table each from the spare slots (bump counter next[0], slots up to total),
+// listed for a build; with no spare slot left, dropped (counted in
+// next[1]).  A source listed many times is claimed once (0 -> kHtDropped).
+__global__ void ht_new_hubs_kernel(const uint32_t* iu, uint64_t ni, const uint32_t* deg, uint32_t* cap,
+                                   uint64_t* off, unsigned long long* next, uint64_t total, uint32_t* list,
+                                   uint32_t* nlist) {
+  for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < ni; t += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t u = iu[t];
     if (deg[u] <= kHtMinDeg || cap[u] != 0) continue;
-    const unsigned long long s0 = atomicAdd(next, (unsigned long long)kHtMaxCap);
+    if (atomicCAS(&cap[u], 0u, kHtDropped) != 0) continue;
+    const unsigned long long s0 = atomicAdd(&next[0], (unsigned long long)kHtMaxCap);
     if (s0 + kHtMaxCap > total) {
-      cap[u] = kHtDropped;
+      atomicAdd(&next[1], 1ull);
       continue;
     }
     off[u] = s0;
-    cap[u] = kHtMaxCap;
-    list[atomicAdd(nlist, 1u)] = (uint32_t)u;
+    __hip_atomic_store(&cap[u], kHtMaxCap, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    list[atomicAdd(nlist, 1u)] = u;
   }
-}
-
-// Vertices above thr edges without a hub table.
-__global__ __launch_bounds__(kBlock) void untabled_kernel(const uint32_t* deg, const uint32_t* cap, uint64_t S,
-                                                          uint32_t thr, unsigned long long* out) {
-  unsigned long long c = 0;
-  for (uint64_t u = blockIdx.x * (uint64_t)kBlock + threadIdx.x; u < S; u += (uint64_t)gridDim.x * kBlock)
-    c += deg[u] > thr && cap[u] <= kHtDropped;
-  c = block_sum_u64<kBlock>(c);
-  if (threadIdx.x == 0 && c) atomicAdd(out, c);
 }
 
 // Hub tables: per vertex its slots (0 below kHtMinDeg edges) and a flag for
@@ -1603,7 +1672,7 @@ template <class WT, bool CHECK>
 __global__ __launch_bounds__(kBlock) void ht_build_kernel(GView g, const uint32_t* vcom, const uint32_t* list,
                                                           const uint32_t* n_dev, const uint64_t* off, uint32_t* cap,
                                                           uint32_t* key, int32_t* cnt, double* wt,
-                                                          unsigned long long* out) {
+                                                          unsigned long long* out, unsigned long long* ndrop) {
   constexpr int WPB = kBlock / 64;
   __shared__ uint32_t s_key[WPB][kHtMaxCap];
   __shared__ int32_t s_cnt[WPB][kHtMaxCap];
@@ -1690,7 +1759,7 @@ __global__ __launch_bounds__(kBlock) void ht_build_kernel(GView g, const uint32_
       continue;
     }
     if (over) {
-      if (lane == 0) cap[u] = kHtDropped;
+      if (lane == 0) ht_drop(cap, u, ndrop);
       continue;
     }
     for (uint32_t sl = lane; sl < c_u; sl += 64) {
@@ -1708,7 +1777,7 @@ __global__ __launch_bounds__(kBlock) void ht_build_kernel(GView g, const uint32_
 __global__ void ht_batch_kernel(const uint32_t* du, const uint32_t* dv, const float* dw, uint64_t nd,
                                 const uint32_t* iu, const uint32_t* iv, const float* iw, uint64_t ni,
                                 const uint32_t* vcom, const uint64_t* off, uint32_t* cap, uint32_t* key, int32_t* cnt,
-                                double* wt) {
+                                double* wt, unsigned long long* ndrop) {
   for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < nd + ni;
        t += (uint64_t)gridDim.x * blockDim.x) {
     const bool del = t < nd;
@@ -1716,7 +1785,7 @@ __global__ void ht_batch_kernel(const uint32_t* du, const uint32_t* dv, const fl
     const uint32_t u = del ? du[j] : iu[j], v = del ? dv[j] : iv[j];
     if (u == v) continue;
     const double w = (double)(del ? dw[j] : iw[j]);
-    ht_add(cap, off, key, cnt, wt, u, vcom[v], del ? -1 : 1, del ? -w : w);
+    ht_add(cap, off, key, cnt, wt, ndrop, u, vcom[v], del ? -1 : 1, del ? -w : w);
   }
 }
 
@@ -2164,13 +2233,13 @@ static_assert(kWlScanBlocks % kBlock == 0, "whole segments per block count pass"
 
 // ids of block b: [b << shift, (b + 1) << shift)
 
-// Bit j of m[q]: id id0 + j is flagged and belongs to queue q.
-__device__ __forceinline__ bool wl_bits(const uint8_t* f, const uint32_t* deg, uint64_t S, WlSplit sp,
-                                        uint64_t id0, uint32_t (&m)[kWlQueues]) {
+// Bit j of m[q]: id id0 + j is flagged and belongs to queue q (w: the flag
+// bytes of ids id0 .. id0 + 3).
+__device__ __forceinline__ void wl_bits(uint32_t w, const uint32_t* deg, uint64_t S, WlSplit sp, uint64_t id0,
+                                        uint32_t (&m)[kWlQueues]) {
 #pragma unroll
   for (int q = 0; q < kWlQueues; ++q) m[q] = 0;
-  uint32_t w = reinterpret_cast<const uint32_t*>(f)[id0 >> 2];
-  if (!w) return false;
+  if (!w) return;
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
     if (id0 + j < S && ((w >> (8 * j)) & 0xFFu)) {
@@ -2181,7 +2250,19 @@ __device__ __forceinline__ bool wl_bits(const uint8_t* f, const uint32_t* deg, u
       m[q] |= 1u << j;
     }
   }
-  return true;
+}
+
+// Tiles of a segment whose flag words a thread loads before it looks at
+// any of them (a segment holds 1, 2, 4 or more tiles).
+constexpr int kWlAhead = 4;
+
+// Flag words of tiles t0 .. t0 + kWlAhead - 1 for this thread (0 past hi).
+__device__ __forceinline__ void wl_words(const uint8_t* f, uint64_t t0, uint64_t hi, uint32_t (&w)[kWlAhead]) {
+#pragma unroll
+  for (int k = 0; k < kWlAhead; ++k) {
+    const uint64_t id0 = t0 + k * kWlTile + 4 * (uint64_t)threadIdx.x;
+    w[k] = id0 < hi ? reinterpret_cast<const uint32_t*>(f)[id0 >> 2] : 0u;
+  }
 }
 
 // Sum over a wave (every lane gets it).
@@ -2203,11 +2284,17 @@ __global__ __launch_bounds__(kBlock) void wl_count_kernel(const uint8_t* f, uint
   if (threadIdx.x < kWlQueues) s_c[threadIdx.x] = 0;
   __syncthreads();
   unsigned long long c[kWlQueues] = {};
-  for (uint64_t id0 = lo + 4 * (uint64_t)threadIdx.x; id0 < hi; id0 += kWlTile) {
-    uint32_t m[kWlQueues];
-    if (!wl_bits(f, deg, S, sp, id0, m)) continue;
+  for (uint64_t t0 = lo; t0 < hi; t0 += kWlAhead * kWlTile) {
+    uint32_t w[kWlAhead];
+    wl_words(f, t0, hi, w);
 #pragma unroll
-    for (int q = 0; q < kWlQueues; ++q) c[q] += __popc(m[q]);
+    for (int k = 0; k < kWlAhead; ++k) {
+      if (!w[k]) continue;
+      uint32_t m[kWlQueues];
+      wl_bits(w[k], deg, S, sp, t0 + k * kWlTile + 4 * (uint64_t)threadIdx.x, m);
+#pragma unroll
+      for (int q = 0; q < kWlQueues; ++q) c[q] += __popc(m[q]);
+    }
   }
 #pragma unroll
   for (int q = 0; q < kWlQueues; ++q) {
@@ -2251,26 +2338,35 @@ __global__ __launch_bounds__(kBlock) void wl_scatter_kernel(uint8_t* f, int clea
   for (int q = 0; q < kWlQueues; ++q) r[q] = blockIdx.x == 0 ? 0 : s_pre[q];
   if (blockIdx.x == 0 && threadIdx.x < kWlQueues) lens[threadIdx.x] = (uint32_t)s_pre[threadIdx.x];
   if (blockIdx.x == 0 && threadIdx.x == 0 && sp.reset) *sp.reset = 0;
+  if (blockIdx.x == 0 && threadIdx.x == 0 && sp.ndrop) lens[kWlQueues] = (uint32_t)min(*sp.ndrop, 0xFFFFFFFFull);
   if (skip) return;
   if (sp.dirty && clear && threadIdx.x == 0) sp.dirty[blockIdx.x] = 0;
-  for (uint64_t t0 = lo; t0 < hi; t0 += kWlTile) {  // block-uniform trip count
-    const uint64_t id0 = t0 + 4 * (uint64_t)threadIdx.x;
-    uint32_t m[kWlQueues] = {};
-    const bool any = id0 < hi && wl_bits(f, deg, S, sp, id0, m);
-    uint64_t packed = 0;
+  for (uint64_t g0 = lo; g0 < hi; g0 += kWlAhead * kWlTile) {  // block-uniform trip counts
+    uint32_t w[kWlAhead];
+    wl_words(f, g0, hi, w);
+    uint32_t m[kWlAhead][kWlQueues];
 #pragma unroll
-    for (int q = 0; q < kWlQueues; ++q) packed |= (uint64_t)__popc(m[q]) << (12 * q);
-    uint64_t tot;
-    const uint64_t ex = block_scan_excl<kBlock>(packed, tot);
+    for (int k = 0; k < kWlAhead; ++k) wl_bits(w[k], deg, S, sp, g0 + k * kWlTile + 4 * (uint64_t)threadIdx.x, m[k]);
 #pragma unroll
-    for (int q = 0; q < kWlQueues; ++q) {
-      uint64_t e = r[q] + ((ex >> (12 * q)) & 0xFFF);
+    for (int k = 0; k < kWlAhead; ++k) {
+      const uint64_t t0 = g0 + k * kWlTile;
+      if (t0 >= hi) break;
+      const uint64_t id0 = t0 + 4 * (uint64_t)threadIdx.x;
+      uint64_t packed = 0;
 #pragma unroll
-      for (int j = 0; j < 4; ++j)
-        if (m[q] & (1u << j)) qs.q[q][e++] = (uint32_t)(id0 + j);
-      r[q] += (tot >> (12 * q)) & 0xFFF;
+      for (int q = 0; q < kWlQueues; ++q) packed |= (uint64_t)__popc(m[k][q]) << (12 * q);
+      uint64_t tot;
+      const uint64_t ex = block_scan_excl<kBlock>(packed, tot);
+#pragma unroll
+      for (int q = 0; q < kWlQueues; ++q) {
+        uint64_t e = r[q] + ((ex >> (12 * q)) & 0xFFF);
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          if (m[k][q] & (1u << j)) qs.q[q][e++] = (uint32_t)(id0 + j);
+        r[q] += (tot >> (12 * q)) & 0xFFF;
+      }
+      if (clear && w[k]) reinterpret_cast<uint32_t*>(f)[id0 >> 2] = 0;
     }
-    if (clear && any) reinterpret_cast<uint32_t*>(f)[id0 >> 2] = 0;
   }
 }
 
@@ -2280,14 +2376,14 @@ template <class WT>
 __global__ __launch_bounds__(kBlock) void modularity_parts_kernel(GView g, const uint32_t* memb, double* cin,
                                                                   double* ctot) {
   const WT* W = (const WT*)g.w;
-  int lane = __lane_id();
-  for (uint64_t base = (blockIdx.x * (uint64_t)kBlock) & ~63ull; base < g.S; base += (uint64_t)gridDim.x * kBlock) {
-    uint64_t u = base + (threadIdx.x & ~63u) + lane;
-    bool live = u < g.S && (!g.ex || g.ex[u]);
-    uint64_t o = live ? g.off[u] : 0;
-    uint32_t d = live ? g.deg[u] : 0;
-    uint32_t c = live ? memb[u] : 0;
-    bool big = live && d > 64;
+  const int lane = __lane_id();
+  for (uint64_t base = 0; base < g.S; base += (uint64_t)gridDim.x * kBlock) {  // grid-uniform
+    const uint64_t u = scrambled_vertex(base);
+    const bool live = u < g.S && (!g.ex || g.ex[u]);
+    const uint64_t o = live ? g.off[u] : 0;
+    const uint32_t d = live ? g.deg[u] : 0;
+    const uint32_t c = live ? memb[u] : 0;
+    const bool big = live && d > 64;
     if (live && !big) {
       double si = 0, st = 0;
       for (uint32_t i = 0; i < d; ++i) {
@@ -2300,16 +2396,26 @@ __global__ __launch_bounds__(kBlock) void modularity_parts_kernel(GView g, const
     }
     unsigned long long mask = __ballot(big);
     while (mask) {
-      int j = __ffsll((long long)mask) - 1;
+      const int j = __ffsll((long long)mask) - 1;
       mask &= mask - 1;
-      uint64_t oj = __shfl(o, j);
-      uint32_t dj = __shfl(d, j);
-      uint32_t cj = __shfl(c, j);
+      const uint64_t oj = __shfl(o, j);
+      const uint32_t dj = __shfl(d, j);
+      const uint32_t cj = __shfl(c, j);
       double si = 0, st = 0;
-      for (uint32_t i = lane; i < dj; i += 64) {
-        double w = (double)W[oj + i];
-        if (memb[g.keys[oj + i]] == cj) si += w;
-        st += w;
+      for (uint32_t i0 = 0; i0 < dj; i0 += 4 * 64) {
+        double w[4];
+        uint32_t k[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const uint32_t i = i0 + r * 64 + lane;
+          w[r] = i < dj ? (double)W[oj + i] : 0.0;
+          k[r] = i < dj ? memb[g.keys[oj + i]] : ~cj;
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          if (k[r] == cj) si += w[r];
+          st += w[r];
+        }
       }
 #pragma unroll
       for (int m = 32; m > 0; m >>= 1) { si += __shfl_xor(si, m, 64); st += __shfl_xor(st, m, 64); }
@@ -2358,14 +2464,14 @@ void k_degrees_from_offsets(const uint64_t* off, uint32_t* deg, uint64_t S, hipS
 void k_vertex_weights(GView g, bool f64, double* vtot, hipStream_t st) {
   if (!g.S) return;
   uint64_t grid = grid_for(g.S, kBlock, 1u << 16);
-  if (f64) LV_LAUNCH((row_sum_kernel<double, false>), grid, kBlock, st, g, vtot, nullptr);
-  else LV_LAUNCH((row_sum_kernel<float, false>), grid, kBlock, st, g, vtot, nullptr);
+  if (f64) LV_LAUNCH((row_sum_kernel<double>), grid, kBlock, st, g, vtot);
+  else LV_LAUNCH((row_sum_kernel<float>), grid, kBlock, st, g, vtot);
 }
 void k_edge_weight(GView g, bool f64, double* out, hipStream_t st) {
   if (!g.S) return;
   uint64_t grid = grid_for(g.S, kBlock, 2048);
-  if (f64) LV_LAUNCH((row_sum_kernel<double, true>), grid, kBlock, st, g, nullptr, out);
-  else LV_LAUNCH((row_sum_kernel<float, true>), grid, kBlock, st, g, nullptr, out);
+  if (f64) LV_LAUNCH((edge_weight_kernel<double>), grid, kBlock, st, g, out);
+  else LV_LAUNCH((edge_weight_kernel<float>), grid, kBlock, st, g, out);
 }
 void k_initialize(GView g, uint32_t* vcom, double* ctot, const double* vtot, hipStream_t st) {
   if (g.S) LV_LAUNCH(initialize_kernel, grid_for(g.S, kBlock * 4), kBlock, st, g, vcom, ctot, vtot);
@@ -2486,37 +2592,36 @@ void k_moved_flag(const MoveArgs& a, hipStream_t st) {
 void k_carry_hubs(const MoveArgs& a, const WlQueues& qs, const uint32_t* lens, hipStream_t st) {
   LV_LAUNCH(carry_hubs_kernel, 64, kBlock, st, a, qs, lens);
 }
-void k_ht_new_hubs(const uint32_t* deg, uint64_t S, uint32_t* cap, uint64_t* off, unsigned long long* next,
-                   uint64_t total, uint32_t* list, uint32_t* nlist, hipStream_t st) {
-  if (S) LV_LAUNCH(ht_new_hubs_kernel, grid_for(S, kBlock * 4), kBlock, st, deg, S, cap, off, next, total, list, nlist);
-}
-void k_untabled(const uint32_t* deg, const uint32_t* cap, uint64_t S, uint32_t thr, unsigned long long* out,
-                hipStream_t st) {
-  if (S) LV_LAUNCH(untabled_kernel, grid_for(S, kBlock * 8, kReduceGrid), kBlock, st, deg, cap, S, thr, out);
+void k_ht_new_hubs(const uint32_t* iu, uint64_t ni, const uint32_t* deg, uint32_t* cap, uint64_t* off,
+                   unsigned long long* next, uint64_t total, uint32_t* list, uint32_t* nlist, hipStream_t st) {
+  if (ni)
+    LV_LAUNCH(ht_new_hubs_kernel, grid_for(ni, kBlock), kBlock, st, iu, ni, deg, cap, off, next, total, list, nlist);
 }
 void k_ht_caps(const uint32_t* deg, uint64_t S, uint32_t* cap, uint64_t* cap64, uint8_t* flags, hipStream_t st) {
   if (S) LV_LAUNCH(ht_caps_kernel, grid_for(S, kBlock * 4), kBlock, st, deg, S, cap, cap64, flags);
 }
 void k_ht_build(GView g, bool f64, const uint32_t* vcom, const uint32_t* list, const uint32_t* n_dev,
-                const uint64_t* off, uint32_t* cap, uint32_t* key, int32_t* cnt, double* w, hipStream_t st) {
+                const uint64_t* off, uint32_t* cap, uint32_t* key, int32_t* cnt, double* w,
+                unsigned long long* ndrop, hipStream_t st) {
   if (f64) LV_LAUNCH((ht_build_kernel<double, false>), 4096, kBlock, st, g, vcom, list, n_dev, off, cap, key, cnt, w,
-                     nullptr);
+                     nullptr, ndrop);
   else LV_LAUNCH((ht_build_kernel<float, false>), 4096, kBlock, st, g, vcom, list, n_dev, off, cap, key, cnt, w,
-                 nullptr);
+                 nullptr, ndrop);
 }
 void k_ht_check(GView g, bool f64, const uint32_t* vcom, const uint32_t* list, const uint32_t* n_dev,
                 const uint64_t* off, uint32_t* cap, uint32_t* key, int32_t* cnt, double* w, unsigned long long* out,
                 hipStream_t st) {
   if (f64) LV_LAUNCH((ht_build_kernel<double, true>), 4096, kBlock, st, g, vcom, list, n_dev, off, cap, key, cnt, w,
-                     out);
-  else LV_LAUNCH((ht_build_kernel<float, true>), 4096, kBlock, st, g, vcom, list, n_dev, off, cap, key, cnt, w, out);
+                     out, nullptr);
+  else LV_LAUNCH((ht_build_kernel<float, true>), 4096, kBlock, st, g, vcom, list, n_dev, off, cap, key, cnt, w, out,
+                 nullptr);
 }
 void k_ht_batch(const uint32_t* du, const uint32_t* dv, const float* dw, uint64_t nd, const uint32_t* iu,
                 const uint32_t* iv, const float* iw, uint64_t ni, const uint32_t* vcom, const uint64_t* off,
-                uint32_t* cap, uint32_t* key, int32_t* cnt, double* w, hipStream_t st) {
+                uint32_t* cap, uint32_t* key, int32_t* cnt, double* w, unsigned long long* ndrop, hipStream_t st) {
   if (nd + ni)
     LV_LAUNCH(ht_batch_kernel, grid_for(nd + ni, kBlock), kBlock, st, du, dv, dw, nd, iu, iv, iw, ni, vcom, off, cap,
-              key, cnt, w);
+              key, cnt, w, ndrop);
 }
 void k_exists(GView g, const uint32_t* vcom, uint32_t* flags, hipStream_t st) {
   if (g.S) LV_LAUNCH(exists_kernel, grid_for(g.S, kBlock * 4), kBlock, st, g, vcom, flags);

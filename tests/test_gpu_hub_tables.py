@@ -106,3 +106,55 @@ def test_tables_off_matches_on(ctx, oracle):
         ctx.set_hub_tables(True)
     _say("  Q with tables", [round(q, 6) for q in qs[True]], "without", [round(q, 6) for q in qs[False]])
     assert max(abs(a - b) for a, b in zip(qs[True], qs[False])) <= Q_TOL
+
+
+def test_table_dropped_by_a_batch(ctx, oracle):
+    """A hub of 300 edges whose table (256 slots) a batch overflows with 300
+    new neighbour communities: the batch drops the table, the run starts
+    without the edge-scanning hub kernels, and the hub, queued for them, is
+    evaluated from its edges a group later.  Tables of the other vertices
+    stay exact; modularity within 1e-3 of the oracle's, K exact."""
+    import lvgraph
+    import lvhip
+    S, hub = 1600, 0
+    # hub - 1..300, those in pairs; 1000..1599 in 300 disconnected pairs
+    near = np.arange(1, 301, dtype=np.uint32)
+    far = np.arange(1000, 1600, dtype=np.uint32)
+    pu = np.concatenate([near[0::2], far[0::2]])
+    pv = np.concatenate([near[1::2], far[1::2]])
+    u = np.concatenate([np.full(300, hub, np.uint32), near, pu, pv])
+    v = np.concatenate([near, np.full(300, hub, np.uint32), pv, pu])
+    x = lvgraph.from_edges(S, u, v)
+
+    def ins(a, b):
+        a, b = np.asarray(a, np.uint32), np.asarray(b, np.uint32)
+        return {"u": np.concatenate([a, b]), "v": np.concatenate([b, a]), "w": np.ones(2 * len(a), np.float32)}
+
+    none = {"u": np.zeros(0, np.uint32), "v": np.zeros(0, np.uint32), "w": np.zeros(0, np.float32)}
+    batches = [(none, ins([1], [4])), (none, ins([5], [8])),  # small batches: runs 0 (no tables) and 1 (build)
+               (none, ins(np.full(300, hub), far[0::2]))]  # the hub meets 300 singleton-pair communities
+    graphs = [x]
+    for d, i in batches:
+        graphs.append(lvgraph.apply_batch(graphs[-1], d, i))
+    s0 = oracle.louvain("static", x)
+    st = ctx.device_state(S)
+    st.upload(s0.membership, s0.vertex_weight, s0.community_weight)
+    for k, ((d, i), g) in enumerate(zip(batches, graphs[1:])):
+        dg, db = ctx.device_graph(g), ctx.device_batch(d, i)
+        prev = st.download()
+        ctx.stats_reset()
+        ctx.run(lvhip.LV_DYNAMIC_FRONTIER, dg, db, st)
+        stats = ctx.stats()
+        t, m = ctx.check_hub_tables(dg, st)
+        q = ctx.dmodularity(dg, st)
+        o = oracle.louvain("dynamic-frontier", g, d, i, *prev)
+        q_ref = oracle.modularity(g, o.membership)
+        _say(f"  run {k}: tables {t} mismatched {m}; small hubs {stats['vertices'][2]}; Q gpu={q:.6f} ref={q_ref:.6f}")
+        assert m == 0, (k, t, m)
+        assert abs(q - q_ref) <= Q_TOL, (k, q, q_ref)
+        assert np.array_equal(st.download()[1], o.vertex_weight), k
+        if k == 1:
+            assert t >= 1  # the hub's table was built
+        if k == 2:
+            assert t == 0  # ... and dropped by the batch
+            assert stats["vertices"][2] >= 1  # the hub went through the block-per-hub kernel
