@@ -57,7 +57,7 @@ def host_info():
     return {"cpu": model, "sockets": len(sockets) or None, "nproc": os.cpu_count(), "affinity": HOST_CORES}
 
 HBM_PEAK_GBPS = 8000.0
-KERNEL_NAME = "move_kernel<64, 64, 512, float, 2>"
+KERNEL_NAME = "move_kernel<256, 64, 512, float, 2>"
 SMALL_HUB_KERNEL = "hub_mid_kernel<256, 2048u, 1024u, float>"
 MID_HUB_KERNEL = "hub_mid_kernel<512, 2048u, 16384u, float>"
 PMC_SUMMARY = os.path.join(ROOT, "profiles", "move_kernel_df_rmat22_pmc.txt")

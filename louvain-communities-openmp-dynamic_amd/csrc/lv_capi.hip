@@ -561,14 +561,15 @@ bool batched_step(lv_context* c, Batched* bt, int L) {
   return any;
 }
 
-// id-order kernel: one HBM region per stream (up to 2^16) holding
-// nextpow2(2 deg) slots for every degree it runs (<= hub_min)
+// id-order kernel: one HBM region per stream (up to 2^16, rounded up to a
+// multiple of the kernel's 4 streams per block) holding nextpow2(2 deg)
+// slots for every degree it runs (<= hub_min)
 void id_order_regions(lv_context* c, MoveArgs& a, const Level& lv) {
   a.gblocks = std::min<uint64_t>(a.max_streams, 1u << 16);
   uint64_t dmax = kHubMin;
   if (c->hub_min > kHubMin) dmax = std::max<uint64_t>(dmax, std::min<uint64_t>(c->hub_min, lv.hubs->maxdeg));
   a.gcap = std::max<uint64_t>(kHybridHbm, next_pow2(2 * dmax));
-  size_t slots = (size_t)a.gcap * a.gblocks;
+  size_t slots = (size_t)a.gcap * ((a.gblocks + 3) & ~3ull);
   a.gkey = c->ws<uint32_t>("g.key", slots);
   a.gpos = c->ws<uint32_t>("g.pos", slots);
   a.gval = c->ws<double>("g.val", slots);
@@ -590,6 +591,10 @@ MoveArgs base_args(lv_context* c, const Level& lv, uint32_t* vcom, const double*
   a.cnt = c->ws<unsigned long long>("cnt", (size_t)kSpread * kCntStride);
   a.min_chunk = 0;
   a.max_streams = level_streams(c, lv.order);
+  // worklist sub-rounds mostly hold a few thousand vertices: no more streams
+  // than the id-order kernel keeps resident at once (a launch with fewer
+  // entries then costs one round of short-lived waves, not several)
+  if (worklist) a.max_streams = std::min<uint64_t>(a.max_streams, kWlStreams);
   a.chunk = c->chunk;
   a.work = c->ws<unsigned long long>("work", 1);
   a.hub_min = worklist ? c->hub_min_wl : c->hub_min;

@@ -67,6 +67,9 @@ constexpr uint32_t kHtDropped = 1;
 
 // worklist queue compaction: blocks (contiguous id segments) per pass
 constexpr int kWlScanBlocks = 1024;
+// streams of the worklist id-order kernel: the waves it keeps resident
+// (256 CUs x 4 SIMDs x 3 waves at its register count)
+constexpr uint64_t kWlStreams = 3072;
 // worklist queues: low degree, small hubs, mid hubs, hub batch, tabled hubs
 constexpr int kWlQueues = 5;
 struct WlSplit {

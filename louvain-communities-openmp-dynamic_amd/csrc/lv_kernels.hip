@@ -731,7 +731,7 @@ __global__ __launch_bounds__(BLOCK) void move_kernel(MoveArgs a) {
     hpos = a.hpos + base;
     hval = a.hval + base;
   } else if constexpr (MODE == MOVE_HYBRID) {
-    uint64_t base = (uint64_t)blockIdx.x * a.gcap;
+    uint64_t base = ((uint64_t)blockIdx.x * GPB + grp) * a.gcap;  // the group's region
     hkey = a.gkey + base;
     hpos = a.gpos + base;
     hval = a.gval + base;
@@ -758,7 +758,18 @@ __global__ __launch_bounds__(BLOCK) void move_kernel(MoveArgs a) {
       clen = ch < nchunks ? a.ctab[2 * ch + 1] - c0 : 0;
     }
     // One vertex (offset o, degree d, community cu, weight Ku) through the
-    // group; every thread of the block calls it.
+    // group; every thread of the block calls it.  Its barriers are the
+    // block's, or the wave's in the id-order kernel (a group per wave, each
+    // with its own LDS table and HBM region: its groups run independently).
+    auto group_sync = [&]() __attribute__((always_inline)) {
+      if constexpr (MODE == MOVE_HYBRID && G == 64) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+      } else {
+        __syncthreads();
+      }
+    };
     auto visit = [&](bool act, uint32_t u, uint64_t o, uint32_t d, uint32_t cu, double Ku) __attribute__((always_inline)) {
       if (act && a.stamp && lane == 0) st_dev(&a.stamp[u], cur_serial(a));
       bool hbm = MODE == MOVE_HBM;
@@ -773,12 +784,12 @@ __global__ __launch_bounds__(BLOCK) void move_kernel(MoveArgs a) {
         if (hbm) table_clear<G>(hkey, hpos, hval, cap, lane);
         else table_clear<G>(lkey, lpos, lval, cap, lane);
       }
-      __syncthreads();
+      group_sync();
       if (act) {
         if (hbm) table_scan<G, WT>(a, W, u, o, d, hkey, hpos, hval, cap, lane);
         else table_scan<G, WT>(a, W, u, o, d, lkey, lpos, lval, cap, lane);
       }
-      __syncthreads();
+      group_sync();
       double be = 0;
       uint32_t bp = kNoPos, bc = 0;
       if (act) {
@@ -813,11 +824,11 @@ __global__ __launch_bounds__(BLOCK) void move_kernel(MoveArgs a) {
           }
         }
       }
-      __syncthreads();
+      group_sync();
     };
     if constexpr (MODE == MOVE_HYBRID) {
       if (a.sparse) {
-        // 64-entry windows (one wave per block): the flags of a window are
+        // 64-entry windows (one wave per group): the flags of a window are
         // read at once and only the flagged entries are visited, in order;
         // after each visit the later flags are read again, so a vertex
         // flagged by an earlier one of the window is still visited in this
@@ -2545,7 +2556,16 @@ static void move_tier(const MoveArgs& a, int tier, hipStream_t st) {
       break;
     }
     default:
-      LV_LAUNCH((move_kernel<64, 64, kHybridLds, WT, MOVE_HYBRID>), grid_for(groups, 1, a.gblocks), 64, st, a);
+      // Many streams with chunks dealt by position: four one-wave groups per
+      // block (a quarter of the workgroups to dispatch; regions for
+      // round_up(gblocks, 4) groups), each still given at most one chunk per
+      // round, so no more than `streams` walk at once.  Chunks handed out
+      // on demand go a block at a time: one wave per block then.
+      if (!a.chunk && groups >= 64)
+        LV_LAUNCH((move_kernel<256, 64, kHybridLds, WT, MOVE_HYBRID>), grid_for(groups, 4, (a.gblocks + 3) / 4), 256,
+                  st, a);
+      else
+        LV_LAUNCH((move_kernel<64, 64, kHybridLds, WT, MOVE_HYBRID>), grid_for(groups, 1, a.gblocks), 64, st, a);
       break;
   }
 }
