@@ -82,9 +82,9 @@ struct WlSplit {
   uint32_t shift;             // segment of block b: ids [b << shift, (b + 1) << shift)
 };
 // Segment shift of the compaction: 2^shift ids per block (a multiple of
-// the 4 x 256 ids a block reads per tile), at most kWlScanBlocks segments.
+// the 16 x 256 ids a block reads per tile), at most kWlScanBlocks segments.
 inline uint32_t wl_shift(uint64_t S) {
-  uint32_t sh = 10;
+  uint32_t sh = 12;
   while ((S + (1ull << sh) - 1) >> sh > (uint64_t)kWlScanBlocks) ++sh;
   return sh;
 }

@@ -2233,46 +2233,62 @@ __global__ __launch_bounds__(kBlock) void caps_sum_kernel(const uint32_t* deg, u
 }
 
 // Ordered worklist queues (inc/louvain.hxx:594-598 visits ids in ascending
-// order): the ids whose flag byte is set, ascending, split by degree into
-// queue 0 (degree <= thr), queue 1 (thr < degree <= thr2) and queue 2
-// (degree > thr2); thr == 0 puts every id in queue 0.  With clear, the flags
-// read are reset.  Counts per contiguous block segment, a one-block scan of
-// the counts, then an ordered scatter; flags are read a 4-byte word per
-// thread.
-constexpr uint64_t kWlTile = 4 * kBlock;
+// order): the ids whose flag byte is set, ascending, split into the
+// kWlQueues queues.  With clear, the flags read are reset.  Counts per
+// contiguous segment (one block each), then an ordered scatter.  A thread
+// reads 16 consecutive flag bytes (one 16-byte load), so a block covers a
+// 4096-id tile per step and needs one block-wide scan per tile.
+constexpr uint64_t kWlTile = 16 * kBlock;
 static_assert(kWlScanBlocks % kBlock == 0, "whole segments per block count pass");
 
-// ids of block b: [b << shift, (b + 1) << shift)
+// ids of block b: [b << shift, (b + 1) << shift), a multiple of kWlTile
 
-// Bit j of m[q]: id id0 + j is flagged and belongs to queue q (w: the flag
-// bytes of ids id0 .. id0 + 3).
-__device__ __forceinline__ void wl_bits(uint32_t w, const uint32_t* deg, uint64_t S, WlSplit sp, uint64_t id0,
-                                        uint32_t (&m)[kWlQueues]) {
+// Flag words of ids id0 .. id0 + 15 (id0 a multiple of 16; 0 at or past hi;
+// the flag array holds whole words up to S).
+__device__ __forceinline__ void wl_words16(const uint8_t* f, uint64_t id0, uint64_t hi, uint32_t (&w)[4]) {
+  if (id0 + 16 <= hi && !((uintptr_t)f & 15)) {
+    const uint4 v = *reinterpret_cast<const uint4*>(f + id0);
+    w[0] = v.x;
+    w[1] = v.y;
+    w[2] = v.z;
+    w[3] = v.w;
+  } else {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) w[k] = id0 + 4 * k < hi ? reinterpret_cast<const uint32_t*>(f)[(id0 >> 2) + k] : 0u;
+  }
+}
+
+// Bit j of m[q]: id id0 + j (j < 16) is flagged and belongs to queue q
+// (queue 0: degree <= t[0]; 1: <= t[1]; 2: <= t[2]; 3: above, or a 1/2 id
+// marked in big; 4: an id of queues 1-3 with a table; t[0] == 0 puts every
+// id in queue 0).  The 16 degrees are read together.
+__device__ __forceinline__ void wl_bits16(const uint32_t (&w)[4], const uint32_t* deg, uint64_t S, const WlSplit& sp,
+                                          uint64_t id0, uint32_t (&m)[kWlQueues]) {
 #pragma unroll
   for (int q = 0; q < kWlQueues; ++q) m[q] = 0;
-  if (!w) return;
+  if (!(w[0] | w[1] | w[2] | w[3])) return;
+  uint32_t d[16];
+  if (sp.t[0] && id0 + 16 <= S && !((uintptr_t)deg & 15)) {
 #pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    if (id0 + j < S && ((w >> (8 * j)) & 0xFFu)) {
-      const uint32_t d = sp.t[0] ? deg[id0 + j] : 0;
-      int q = d <= sp.t[0] ? 0 : (d <= sp.t[1] ? 1 : (d <= sp.t[2] ? 2 : 3));
+    for (int k = 0; k < 4; ++k) {
+      const uint4 v = reinterpret_cast<const uint4*>(deg + id0)[k];
+      d[4 * k] = v.x;
+      d[4 * k + 1] = v.y;
+      d[4 * k + 2] = v.z;
+      d[4 * k + 3] = v.w;
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < 16; ++j) d[j] = sp.t[0] && id0 + j < S ? deg[id0 + j] : 0u;
+  }
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    if (id0 + j < S && ((w[j / 4] >> (8 * (j % 4))) & 0xFFu)) {
+      int q = d[j] <= sp.t[0] ? 0 : (d[j] <= sp.t[1] ? 1 : (d[j] <= sp.t[2] ? 2 : 3));
       if (q == 1 || q == 2) q = sp.big && sp.big[id0 + j] ? 3 : q;
       if (q && sp.tab && sp.tab[id0 + j] > kHtDropped) q = 4;
       m[q] |= 1u << j;
     }
-  }
-}
-
-// Tiles of a segment whose flag words a thread loads before it looks at
-// any of them (a segment holds 1, 2, 4 or more tiles).
-constexpr int kWlAhead = 4;
-
-// Flag words of tiles t0 .. t0 + kWlAhead - 1 for this thread (0 past hi).
-__device__ __forceinline__ void wl_words(const uint8_t* f, uint64_t t0, uint64_t hi, uint32_t (&w)[kWlAhead]) {
-#pragma unroll
-  for (int k = 0; k < kWlAhead; ++k) {
-    const uint64_t id0 = t0 + k * kWlTile + 4 * (uint64_t)threadIdx.x;
-    w[k] = id0 < hi ? reinterpret_cast<const uint32_t*>(f)[id0 >> 2] : 0u;
   }
 }
 
@@ -2295,17 +2311,13 @@ __global__ __launch_bounds__(kBlock) void wl_count_kernel(const uint8_t* f, uint
   if (threadIdx.x < kWlQueues) s_c[threadIdx.x] = 0;
   __syncthreads();
   unsigned long long c[kWlQueues] = {};
-  for (uint64_t t0 = lo; t0 < hi; t0 += kWlAhead * kWlTile) {
-    uint32_t w[kWlAhead];
-    wl_words(f, t0, hi, w);
+  for (uint64_t id0 = lo + 16 * (uint64_t)threadIdx.x; id0 < hi; id0 += kWlTile) {
+    uint32_t w[4];
+    wl_words16(f, id0, hi, w);
+    uint32_t m[kWlQueues];
+    wl_bits16(w, deg, S, sp, id0, m);
 #pragma unroll
-    for (int k = 0; k < kWlAhead; ++k) {
-      if (!w[k]) continue;
-      uint32_t m[kWlQueues];
-      wl_bits(w[k], deg, S, sp, t0 + k * kWlTile + 4 * (uint64_t)threadIdx.x, m);
-#pragma unroll
-      for (int q = 0; q < kWlQueues; ++q) c[q] += __popc(m[q]);
-    }
+    for (int q = 0; q < kWlQueues; ++q) c[q] += __popc(m[q]);
   }
 #pragma unroll
   for (int q = 0; q < kWlQueues; ++q) {
@@ -2318,13 +2330,15 @@ __global__ __launch_bounds__(kBlock) void wl_count_kernel(const uint8_t* f, uint
 
 // Block b writes the ids of its segment to every queue from the sum of the
 // counts of blocks 0 .. b-1 (each block adds them up itself; block 0 adds
-// up all of them for the queue lengths).  One block-wide scan per 1024-id
-// tile gives each thread its place in all queues at once: the queues' tile
-// counts (at most 1024 each) are packed 12 bits apart.
+// up all of them for the queue lengths).  One block-wide scan per 4096-id
+// tile gives each thread its place in queues 0-3 at once (their tile counts,
+// at most 4096 each, packed 13 bits apart); queue 4 takes a second scan in
+// the tiles that hold any of its ids.  Writes stay below S, the queues'
+// capacity.
 __global__ __launch_bounds__(kBlock) void wl_scatter_kernel(uint8_t* f, int clear, uint64_t S, const uint32_t* deg,
                                                             WlSplit sp, const uint64_t* bsum, WlQueues qs,
                                                             uint32_t* lens) {
-  static_assert(kWlTile <= 4095 && kWlQueues * 12 <= 64, "packed tile counts");
+  static_assert(kWlTile < 8192 && 4 * 13 <= 64 && kWlQueues == 5, "packed tile counts");
   __shared__ unsigned long long s_pre[kWlQueues];
   const uint64_t lo = (uint64_t)blockIdx.x << sp.shift, hi = min(S, lo + (1ull << sp.shift));
   const bool skip = sp.dirty && !sp.dirty[blockIdx.x];  // block-uniform: no flag in this segment
@@ -2352,31 +2366,39 @@ __global__ __launch_bounds__(kBlock) void wl_scatter_kernel(uint8_t* f, int clea
   if (blockIdx.x == 0 && threadIdx.x == 0 && sp.ndrop) lens[kWlQueues] = (uint32_t)min(*sp.ndrop, 0xFFFFFFFFull);
   if (skip) return;
   if (sp.dirty && clear && threadIdx.x == 0) sp.dirty[blockIdx.x] = 0;
-  for (uint64_t g0 = lo; g0 < hi; g0 += kWlAhead * kWlTile) {  // block-uniform trip counts
-    uint32_t w[kWlAhead];
-    wl_words(f, g0, hi, w);
-    uint32_t m[kWlAhead][kWlQueues];
+  for (uint64_t t0 = lo; t0 < hi; t0 += kWlTile) {  // block-uniform trip count
+    const uint64_t id0 = t0 + 16 * (uint64_t)threadIdx.x;
+    uint32_t w[4] = {0u, 0u, 0u, 0u};
+    if (id0 < hi) wl_words16(f, id0, hi, w);
+    uint32_t m[kWlQueues];
+    wl_bits16(w, deg, S, sp, id0, m);
+    uint64_t packed = 0;
 #pragma unroll
-    for (int k = 0; k < kWlAhead; ++k) wl_bits(w[k], deg, S, sp, g0 + k * kWlTile + 4 * (uint64_t)threadIdx.x, m[k]);
+    for (int q = 0; q < 4; ++q) packed |= (uint64_t)__popc(m[q]) << (13 * q);
+    uint64_t tot;
+    const uint64_t ex = block_scan_excl<kBlock>(packed, tot);
+    uint64_t ex4 = 0, tot4 = 0;
+    if (__syncthreads_or(m[4] != 0)) ex4 = block_scan_excl<kBlock>((uint64_t)__popc(m[4]), tot4);
 #pragma unroll
-    for (int k = 0; k < kWlAhead; ++k) {
-      const uint64_t t0 = g0 + k * kWlTile;
-      if (t0 >= hi) break;
-      const uint64_t id0 = t0 + 4 * (uint64_t)threadIdx.x;
-      uint64_t packed = 0;
-#pragma unroll
-      for (int q = 0; q < kWlQueues; ++q) packed |= (uint64_t)__popc(m[k][q]) << (12 * q);
-      uint64_t tot;
-      const uint64_t ex = block_scan_excl<kBlock>(packed, tot);
-#pragma unroll
-      for (int q = 0; q < kWlQueues; ++q) {
-        uint64_t e = r[q] + ((ex >> (12 * q)) & 0xFFF);
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-          if (m[k][q] & (1u << j)) qs.q[q][e++] = (uint32_t)(id0 + j);
-        r[q] += (tot >> (12 * q)) & 0xFFF;
+    for (int q = 0; q < kWlQueues; ++q) {
+      uint64_t e = r[q] + (q < 4 ? (ex >> (13 * q)) & 0x1FFF : ex4);
+      uint32_t bits = m[q];
+      while (bits) {
+        const int j = __ffs(bits) - 1;
+        bits &= bits - 1;
+        if (e < S) qs.q[q][e] = (uint32_t)(id0 + j);
+        ++e;
       }
-      if (clear && w[k]) reinterpret_cast<uint32_t*>(f)[id0 >> 2] = 0;
+      r[q] += q < 4 ? (tot >> (13 * q)) & 0x1FFF : tot4;
+    }
+    if (clear && (w[0] | w[1] | w[2] | w[3])) {
+      if (id0 + 16 <= hi && !((uintptr_t)f & 15)) {
+        *reinterpret_cast<uint4*>(f + id0) = make_uint4(0u, 0u, 0u, 0u);
+      } else {
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+          if (w[k]) reinterpret_cast<uint32_t*>(f)[(id0 >> 2) + k] = 0u;
+      }
     }
   }
 }
