@@ -70,7 +70,7 @@ struct lv_context {
   uint32_t serial = 0;          // worklist iteration serial (visit stamps)
   uint64_t stamp_span = ~0ull;
   int profiling = 0;             // time every local-moving launch with events
-  lv_stats stats{};              // accumulated since lv_stats_reset
+  lv_stats stats{};              // accumulated since lv_stats_reset (the work counters: on the device)
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> prof;  // (tier, events) of a run
   std::map<std::string, Buf> bufs;
   void* pinned = nullptr;  // small pinned host buffer for scalar readbacks
@@ -97,6 +97,15 @@ struct lv_context {
     return reinterpret_cast<T*>(b.p);
   }
   size_t ws_bytes(const std::string& name) { return bufs[name].bytes; }
+  // Work counters of the local-moving kernels (kSpread slots of kCntStride),
+  // accumulated on the device since the last lv_stats_reset and read by
+  // lv_stats_get only, so a run reads none of them back.
+  unsigned long long* counters() {
+    bool fresh = false;
+    auto* p = ws<unsigned long long>("cnt", (size_t)kSpread * kCntStride, &fresh);
+    if (fresh) k_fill_u64(reinterpret_cast<uint64_t*>(p), 0, ws_bytes("cnt") / 8, st);
+    return p;
+  }
   void* scan_tmp(size_t bytes) {
     scan_tmp_bytes = std::max(scan_tmp_bytes, bytes);
     return ws<uint8_t>("scan_tmp", scan_tmp_bytes);
@@ -283,6 +292,12 @@ struct lv_dbatch {
   uint64_t nd = 0, ni = 0;
   uint32_t *du = nullptr, *dv = nullptr, *iu = nullptr, *iv = nullptr;
   float *dw = nullptr, *iw = nullptr;
+  // checked once, when the batch is made: every id below id_bound, and
+  // check_flags bit 1 set if a list is unsorted or repeats an entry (bit 0:
+  // an id of 2^32 - 1); a run compares id_bound with its graph's span
+  bool checked = false;
+  uint64_t id_bound = 0;
+  int check_flags = 0;
 };
 
 struct lv_dstate {
@@ -417,10 +432,6 @@ struct Level {
   uint64_t hub_slots;      // hub-batch table slots for all of them
   uint64_t stamp = 0;      // resident graph's degree stamp (0: a super-graph)
   uint64_t slab_slots = 0; // resident graph: hub-batch slots above kMidSlots / 2 edges
-};
-
-struct Counters {
-  unsigned long long v[kTiers][CNT_N] = {};
 };
 
 // Launch one tier of the local-moving kernel, bracketed by events when the
@@ -588,7 +599,7 @@ MoveArgs base_args(lv_context* c, const Level& lv, uint32_t* vcom, const double*
   a.M = M;
   a.R = R;
   a.el = c->ws<double>("el", (size_t)kSpread * kElStride);
-  a.cnt = c->ws<unsigned long long>("cnt", (size_t)kSpread * kCntStride);
+  a.cnt = c->counters();
   a.min_chunk = 0;
   a.max_streams = level_streams(c, lv.order);
   // worklist sub-rounds mostly hold a few thousand vertices: no more streams
@@ -704,15 +715,6 @@ void move_sweep(lv_context* c, MoveArgs a, const Level& lv) {
   }
 }
 
-// Counters of the kSpread slots of dcnt, added into cnt.
-void add_counters(lv_context* c, const unsigned long long* dcnt, Counters& cnt) {
-  std::vector<unsigned long long> h((size_t)kSpread * kCntStride);
-  c->read(h.data(), dcnt, h.size());
-  for (int s = 0; s < kSpread; ++s)
-    for (int t = 0; t < kTiers; ++t)
-      for (int i = 0; i < CNT_N; ++i) cnt.v[t][i] += h[(size_t)s * kCntStride + t * CNT_N + i];
-}
-
 // Zero the spread gain slots of el (kSpread x kElStride doubles).
 void clear_el(lv_context* c, double* el) { k_fill_f64(el, 0, (size_t)kSpread * kElStride, c->st); }
 
@@ -734,14 +736,13 @@ double read_el(lv_context* c, const double* el) {
 // Batched form (bt): every graph iterates under its own tolerance; the
 // per-graph results are in bt->it / bt->el.
 int local_move(lv_context* c, const Level& lv, uint32_t* vcom, const double* vtot, double* ctot, uint8_t* vaff,
-               const uint8_t* fa, double M, double R, int L, double E, Counters& cnt, bool ascending = false,
+               const uint8_t* fa, double M, double R, int L, double E, bool ascending = false,
                Batched* bt = nullptr) {
   MoveArgs a = base_args(c, lv, vcom, vtot, ctot, vaff, fa, M, R);
   batched_args(a, bt);
   a.sparse = c->path == 1;
   if (ascending && !a.chunk) a.chunk = a.sparse ? 64 : 16;
   batched_schedule(c, a, lv, bt);
-  k_fill_u64(reinterpret_cast<uint64_t*>(a.cnt), 0, (size_t)kSpread * kCntStride, c->st);
   unsigned long long* cnt0 = a.cnt;
   int l = 0;
   double el = 0;
@@ -760,7 +761,6 @@ int local_move(lv_context* c, const Level& lv, uint32_t* vcom, const double* vto
     el = read_el(c, a.el);
     if (el <= E) break;
   }
-  add_counters(c, cnt0, cnt);
   return (l > 1 || el) ? l : 0;
 }
 
@@ -774,7 +774,7 @@ int local_move(lv_context* c, const Level& lv, uint32_t* vcom, const double* vto
 // degree tier on the device.  q[0..3]: iteration input / next iteration /
 // two sub-round buffers; qlen[0..3] their lengths.
 int local_move_worklist(lv_context* c, const Level& lv, uint32_t* vcom, const double* vtot, double* ctot,
-                        uint8_t* vaff, const uint8_t* fa, double M, double R, int L, double E, Counters& cnt,
+                        uint8_t* vaff, const uint8_t* fa, double M, double R, int L, double E,
                         Batched* bt = nullptr, lv_dstate* ht = nullptr) {
   MoveArgs a = base_args(c, lv, vcom, vtot, ctot, vaff, fa, M, R, true);
   if (ht && c->tier_order == 2) {  // hubs with a neighbour-community table
@@ -795,7 +795,6 @@ int local_move_worklist(lv_context* c, const Level& lv, uint32_t* vcom, const do
   batched_schedule(c, a, lv, bt);
   if (bt) batched_begin(c, bt);
   a.sparse = 1;  // queue entries read 64 at a time
-  k_fill_u64(reinterpret_cast<uint64_t*>(a.cnt), 0, (size_t)kSpread * kCntStride, c->st);
   unsigned long long* cnt0 = a.cnt;
   const uint64_t S = lv.g.S, stride = std::max<uint64_t>(S, 1), fbytes = (S + 7) & ~3ull;
   // queues: low degree, small hubs, mid hubs, hub batch
@@ -965,7 +964,6 @@ int local_move_worklist(lv_context* c, const Level& lv, uint32_t* vcom, const do
     el = read_el(c, a.el);
     if (el <= E) break;
   }
-  add_counters(c, cnt0, cnt);
   return (l > 1 || el) ? l : 0;
 }
 
@@ -1156,7 +1154,7 @@ void build_hub_tables(lv_context* c, const lv_dgraph* x, lv_dstate* s, const uin
 }
 
 void invoke(lv_context* c, int approach, const lv_dgraph* x, const lv_dbatch* b, lv_dstate* s, const lv_options* o,
-            int& l, int& p, RunTimes& rt, Counters& cnt) {
+            int& l, int& p, RunTimes& rt) {
   const uint64_t S = x->S;
   const double R = o->resolution;
   const int L = o->max_iterations, P = o->max_passes;
@@ -1253,11 +1251,11 @@ void invoke(lv_context* c, int approach, const lv_dgraph* x, const lv_dbatch* b,
     int m;
     hipEvent_t a0 = c->record();
     if (first) {
-      if (worklist) m = local_move_worklist(c, cur, ucom, utot, ctot, vaff, fa, M, R, L, E, cnt, nullptr, ht);
-      else m = local_move(c, cur, ucom, utot, ctot, vaff, fa, M, R, L, E, cnt,
+      if (worklist) m = local_move_worklist(c, cur, ucom, utot, ctot, vaff, fa, M, R, L, E, nullptr, ht);
+      else m = local_move(c, cur, ucom, utot, ctot, vaff, fa, M, R, L, E,
                           approach == LV_DYNAMIC_FRONTIER || approach == LV_DELTA_SCREENING);
     } else {
-      m = local_move(c, cur, vcom, vtot, ctot, vaff, nullptr, M, R, L, E, cnt);
+      m = local_move(c, cur, vcom, vtot, ctot, vaff, nullptr, M, R, L, E);
     }
     lm.push_back({a0, c->record()});
     l += std::max(m, 1);
@@ -1337,7 +1335,6 @@ void run(lv_context* c, int approach, const lv_dgraph* x, const lv_dbatch* b, lv
     d2d(c, bc, s->ctot, S);
   }
   RunTimes rt;
-  Counters cnt;
   int l = 0, p = 0;
   for (int r = 0; r < repeat; ++r) {
     if (r > 0 && approach != LV_STATIC) {
@@ -1347,7 +1344,7 @@ void run(lv_context* c, int approach, const lv_dgraph* x, const lv_dbatch* b, lv
     }
     c->ev_used = 0;
     c->prof.clear();
-    invoke(c, approach, x, b, s, o, l, p, rt, cnt);
+    invoke(c, approach, x, b, s, o, l, p, rt);
     for (auto& pr : c->prof) {
       c->stats.move_ms[pr.first] += elapsed(pr.second.first, pr.second.second);
       c->stats.move_launches[pr.first] += 1;
@@ -1372,14 +1369,6 @@ void run(lv_context* c, int approach, const lv_dgraph* x, const lv_dbatch* b, lv
     if (res->community_weight) d2h(c, res->community_weight, s->ctot, S);
     HIPCHK(hipStreamSynchronize(c->st));
   }
-  // kernel statistics for bandwidth accounting (lv_stats_get)
-  for (int t = 0; t < kTiers; ++t) {
-    c->stats.vertices[t] += cnt.v[t][CNT_VERTICES];
-    c->stats.edges[t] += cnt.v[t][CNT_EDGES];
-    c->stats.candidates[t] += cnt.v[t][CNT_CANDIDATES];
-    c->stats.moves[t] += cnt.v[t][CNT_MOVES];
-    c->stats.moved_edges[t] += cnt.v[t][CNT_MOVED_EDGES];
-  }
   c->stats.runs += 1;
 }
 
@@ -1393,7 +1382,7 @@ void run(lv_context* c, int approach, const lv_dgraph* x, const lv_dbatch* b, lv
 // final ids lie in its own vertex range.
 void invoke_batched(lv_context* c, int approach, const lv_dgraph* x, const lv_dbatch* b, lv_dstate* s,
                     const lv_options* o, const std::vector<uint64_t>& voff, std::vector<int>& l_out,
-                    std::vector<int>& p_out, RunTimes& rt, Counters& cnt) {
+                    std::vector<int>& p_out, RunTimes& rt) {
   const uint64_t S = x->S, G = voff.size() - 1;
   const double R = o->resolution;
   const int L = o->max_iterations, P = o->max_passes;
@@ -1531,11 +1520,11 @@ void invoke_batched(lv_context* c, int approach, const lv_dgraph* x, const lv_db
     batched_level(c, &bt, lo, GN);
     hipEvent_t a0 = c->record();
     if (first) {
-      if (worklist) local_move_worklist(c, cur, ucom, utot, ctot, vaff, nullptr, 0, R, L, 0, cnt, &bt);
-      else local_move(c, cur, ucom, utot, ctot, vaff, nullptr, 0, R, L, 0, cnt,
+      if (worklist) local_move_worklist(c, cur, ucom, utot, ctot, vaff, nullptr, 0, R, L, 0, &bt);
+      else local_move(c, cur, ucom, utot, ctot, vaff, nullptr, 0, R, L, 0,
                       approach == LV_DYNAMIC_FRONTIER, &bt);
     } else {
-      local_move(c, cur, vcom, vtot, ctot, vaff, nullptr, 0, R, L, 0, cnt, false, &bt);
+      local_move(c, cur, vcom, vtot, ctot, vaff, nullptr, 0, R, L, 0, false, &bt);
     }
     lm.push_back({a0, c->record()});
     ++p;
@@ -1650,10 +1639,9 @@ void run_batched(lv_context* c, int approach, const lv_dgraph* x, const lv_dbatc
   c->ev_used = 0;
   c->prof.clear();
   RunTimes rt;
-  Counters cnt;
   std::vector<uint64_t> vo(voff, voff + G + 1);
   std::vector<int> l, p;
-  invoke_batched(c, approach, x, b, s, o, vo, l, p, rt, cnt);
+  invoke_batched(c, approach, x, b, s, o, vo, l, p, rt);
   for (auto& pr : c->prof) {
     c->stats.move_ms[pr.first] += elapsed(pr.second.first, pr.second.second);
     c->stats.move_launches[pr.first] += 1;
@@ -1690,13 +1678,6 @@ void run_batched(lv_context* c, int approach, const lv_dgraph* x, const lv_dbatc
         res[k].membership[u] = (m >= voff[k] && m < voff[k + 1]) ? (uint32_t)(m - voff[k]) : 0u;
       }
     }
-  }
-  for (int t = 0; t < kTiers; ++t) {
-    c->stats.vertices[t] += cnt.v[t][CNT_VERTICES];
-    c->stats.edges[t] += cnt.v[t][CNT_EDGES];
-    c->stats.candidates[t] += cnt.v[t][CNT_CANDIDATES];
-    c->stats.moves[t] += cnt.v[t][CNT_MOVES];
-    c->stats.moved_edges[t] += cnt.v[t][CNT_MOVED_EDGES];
   }
   c->stats.runs += 1;
 }
@@ -1791,6 +1772,33 @@ lv_dgraph* upload_graph(lv_context* c, const lv_graph* g) {
   return d;
 }
 
+// Sortedness, uniqueness and the largest id of a device batch, read back
+// with one synchronisation.
+void check_batch(lv_context* c, lv_dbatch* b) {
+  uint32_t* chk = c->ws<uint32_t>("batch.check", 2);  // flags, largest id
+  k_fill_u32(chk, 0, 2, c->st);
+  int* bad = reinterpret_cast<int*>(chk);
+  k_check_sorted(b->du, b->dv, b->nd, 0xFFFFFFFFu, bad, c->st);
+  k_check_sorted(b->iu, b->iv, b->ni, 0xFFFFFFFFu, bad, c->st);
+  k_max_u32(b->du, b->nd, chk + 1, c->st);
+  k_max_u32(b->dv, b->nd, chk + 1, c->st);
+  k_max_u32(b->iu, b->ni, chk + 1, c->st);
+  k_max_u32(b->iv, b->ni, chk + 1, c->st);
+  uint32_t h[2];
+  c->read(h, chk, 2);
+  b->check_flags = (int)h[0];
+  b->id_bound = b->nd + b->ni ? (uint64_t)h[1] + 1 : 0;
+  b->checked = true;
+}
+
+// The batch's ids lie within span S, its lists sorted and unique (the
+// errors lv_run reports).
+void require_batch(lv_context* c, lv_dbatch* b, uint64_t S) {
+  if (!b->checked) check_batch(c, b);
+  require(!(b->check_flags & 1) && b->id_bound <= S, LV_ERR_INVALID, "batch vertex id out of range");
+  require(!(b->check_flags & 2), LV_ERR_UNSORTED, "batch not sorted by (u, v) or not unique");
+}
+
 lv_dbatch* upload_batch(lv_context* c, const lv_batch* del, const lv_batch* ins, uint64_t S) {
   HIPCHK(hipSetDevice(c->device));
   auto* b = new lv_dbatch();
@@ -1807,17 +1815,8 @@ lv_dbatch* upload_batch(lv_context* c, const lv_batch* del, const lv_batch* ins,
     b->iw = dalloc<float>(b->ni);
     if (b->nd) { h2d(c, b->du, del->u, b->nd); h2d(c, b->dv, del->v, b->nd); h2d(c, b->dw, del->w, b->nd); }
     if (b->ni) { h2d(c, b->iu, ins->u, b->ni); h2d(c, b->iv, ins->v, b->ni); h2d(c, b->iw, ins->w, b->ni); }
-    if (S) {
-      int* bad = c->ws<int>("bad", 1);
-      k_fill_u32(reinterpret_cast<uint32_t*>(bad), 0, 1, c->st);
-      k_check_sorted(b->du, b->dv, b->nd, (uint32_t)S, bad, c->st);
-      int f1 = c->read1(bad);
-      k_fill_u32(reinterpret_cast<uint32_t*>(bad), 0, 1, c->st);
-      k_check_sorted(b->iu, b->iv, b->ni, (uint32_t)S, bad, c->st);
-      int f2 = c->read1(bad);
-      require(!((f1 | f2) & 1), LV_ERR_INVALID, "batch vertex id out of range");
-      require(!((f1 | f2) & 2), LV_ERR_UNSORTED, "batch not sorted by (u, v) or not unique");
-    }
+    check_batch(c, b);  // once: runs compare the largest id with their graph's span
+    if (S) require_batch(c, b, S);
   } catch (...) {
     lv_dbatch_destroy(b);
     throw;
@@ -2199,16 +2198,33 @@ int lv_context_set_profiling(lv_context* c, int on) {
   return LV_OK;
 }
 
-int lv_stats_get(const lv_context* c, lv_stats* out) {
-  if (!c || !out) return LV_ERR_INVALID;
-  *out = c->stats;
-  return LV_OK;
+int lv_stats_get(const lv_context* cc, lv_stats* out) {
+  if (!cc || !out) return LV_ERR_INVALID;
+  auto* c = const_cast<lv_context*>(cc);  // reads the device counters on the context's stream
+  return guard(c, [&] {
+    HIPCHK(hipSetDevice(c->device));
+    *out = c->stats;
+    std::vector<unsigned long long> h((size_t)kSpread * kCntStride);
+    c->read(h.data(), c->counters(), h.size());
+    for (int sl = 0; sl < kSpread; ++sl)
+      for (int t = 0; t < kTiers; ++t) {
+        const unsigned long long* v = &h[(size_t)sl * kCntStride + t * CNT_N];
+        out->vertices[t] += v[CNT_VERTICES];
+        out->edges[t] += v[CNT_EDGES];
+        out->candidates[t] += v[CNT_CANDIDATES];
+        out->moves[t] += v[CNT_MOVES];
+        out->moved_edges[t] += v[CNT_MOVED_EDGES];
+      }
+  });
 }
 
 int lv_stats_reset(lv_context* c) {
   if (!c) return LV_ERR_INVALID;
-  c->stats = lv_stats{};
-  return LV_OK;
+  return guard(c, [&] {
+    HIPCHK(hipSetDevice(c->device));
+    c->stats = lv_stats{};
+    k_fill_u64(reinterpret_cast<uint64_t*>(c->counters()), 0, (size_t)kSpread * kCntStride, c->st);
+  });
 }
 
 int lv_context_set_schedule(lv_context* c, double stream_fraction, uint64_t min_streams, uint64_t max_streams,
@@ -2273,6 +2289,10 @@ int lv_dgraph_update(lv_context* c, lv_dgraph* g, const lv_batch* deletions, con
       b->nd = tidy_list(c, g, deletions, true, b->du, b->dv, b->dw, "up.del");
       b->ni = tidy_list(c, g, insertions, false, b->iu, b->iv, b->iw, "up.ins");
       apply_update(c, g, b);
+      // tidy by construction: sorted, unique, ids of g
+      b->checked = true;
+      b->id_bound = g->S;
+      b->check_flags = 0;
     } catch (...) {
       lv_dbatch_destroy(b);
       throw;
@@ -2389,16 +2409,7 @@ int lv_run(lv_context* c, int approach, const lv_dgraph* g, const lv_dbatch* b, 
            lv_result* res) {
   if (!c) return LV_ERR_INVALID;
   return guard(c, [&] {
-    if (b && g) {
-      // batch ids must lie within the graph (checked once per (batch, graph) use)
-      int* bad = c->ws<int>("bad", 1);
-      k_fill_u32(reinterpret_cast<uint32_t*>(bad), 0, 1, c->st);
-      k_check_sorted(b->du, b->dv, b->nd, (uint32_t)g->S, bad, c->st);
-      k_check_sorted(b->iu, b->iv, b->ni, (uint32_t)g->S, bad, c->st);
-      int f = c->read1(bad);
-      require(!(f & 1), LV_ERR_INVALID, "batch vertex id out of range");
-      require(!(f & 2), LV_ERR_UNSORTED, "batch not sorted by (u, v) or not unique");
-    }
+    if (b && g) require_batch(c, const_cast<lv_dbatch*>(b), g->S);  // checked when the batch was made
     run(c, approach, g, b, s, o, res);
   });
 }
@@ -2407,15 +2418,7 @@ int lv_run_batched(lv_context* c, int approach, const lv_dgraph* g, const lv_dba
                    const uint64_t* voff, uint64_t num_graphs, const lv_options* o, lv_result* res) {
   if (!c) return LV_ERR_INVALID;
   return guard(c, [&] {
-    if (b && g) {
-      int* bad = c->ws<int>("bad", 1);
-      k_fill_u32(reinterpret_cast<uint32_t*>(bad), 0, 1, c->st);
-      k_check_sorted(b->du, b->dv, b->nd, (uint32_t)g->S, bad, c->st);
-      k_check_sorted(b->iu, b->iv, b->ni, (uint32_t)g->S, bad, c->st);
-      int f = c->read1(bad);
-      require(!(f & 1), LV_ERR_INVALID, "batch vertex id out of range");
-      require(!(f & 2), LV_ERR_UNSORTED, "batch not sorted by (u, v) or not unique");
-    }
+    if (b && g) require_batch(c, const_cast<lv_dbatch*>(b), g->S);
     run_batched(c, approach, g, b, s, voff, num_graphs, o, res);
   });
 }

@@ -295,6 +295,29 @@ def test_out_of_range_batch_rejected(ctx):
     assert e.value.status == -1
 
 
+def test_device_batch_checked_against_each_graph(ctx):
+    """A device batch is checked once, when it is made; each run compares its
+    largest id with the run's graph (out of range: -1) and reports an
+    unsorted batch (-4), as the host entry points do."""
+    import lvhip
+    f = F.load("tiny1")
+    y = F.graph(f, "y")
+    S = len(y["off"]) - 1
+    d, i = F.batch(f, "del"), F.batch(f, "ins")
+    dg = ctx.device_graph(y)
+    st = ctx.device_state(S)
+    st.upload(f["static_memb"], f["static_vtot"], f["static_ctot"])
+    far = {"u": np.array([1, S], np.uint32), "v": np.array([S, 1], np.uint32), "w": np.ones(2, np.float32)}
+    with pytest.raises(lvhip.LvError) as e:
+        ctx.run(lvhip.LV_DYNAMIC_FRONTIER, dg, ctx.device_batch(d, far), st)
+    assert e.value.status == -1
+    rev = {k: v[::-1].copy() for k, v in i.items()}
+    with pytest.raises(lvhip.LvError) as e:
+        ctx.run(lvhip.LV_DYNAMIC_FRONTIER, dg, ctx.device_batch(d, rev), st)
+    assert e.value.status == -4
+    ctx.run(lvhip.LV_DYNAMIC_FRONTIER, dg, ctx.device_batch(d, i), st)  # the good batch still runs
+
+
 @pytest.mark.parametrize("name", INTEGER_FIXTURES)
 def test_single_stream_all_passes_bit_exact(ctx, name):
     """One stream, every pass: super-graph rows in the reference's order
