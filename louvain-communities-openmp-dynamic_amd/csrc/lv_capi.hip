@@ -74,6 +74,11 @@ struct lv_context {
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> prof;  // (tier, events) of a run
   std::map<std::string, Buf> bufs;
   void* pinned = nullptr;  // small pinned host buffer for scalar readbacks
+  unsigned long long* tail = nullptr;  // pinned: a run's closing read-back
+  unsigned long long* tail_pin() {
+    if (!tail) HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&tail), 64, hipHostMallocDefault));
+    return tail;
+  }
   size_t pinned_bytes = 0;
   std::vector<hipEvent_t> events;
   size_t ev_used = 0;
@@ -800,8 +805,9 @@ int local_move_worklist(lv_context* c, const Level& lv, uint32_t* vcom, const do
   // queues: low degree, small hubs, mid hubs, hub batch
   WlQueues qs;
   for (int q = 0; q < kWlQueues; ++q) qs.q[q] = c->ws<uint32_t>("wl.q" + std::to_string(q), stride);
-  // queue lengths, then the tables dropped so far (when there are tables)
-  uint32_t* lens = c->ws<uint32_t>("wl.len", kWlQueues + 1);
+  // queue lengths, the tables dropped so far (when there are tables), then
+  // (u32 6..7) the iteration's gain so far as a double: one read per group
+  uint32_t* lens = c->ws<uint32_t>("wl.len", 8);
   uint64_t* bsum = c->ws<uint64_t>("wl.bsum", (size_t)kWlQueues * kWlScanBlocks);
   // next-sub-round flags: all zero between calls (each compaction resets
   // the flags it reads, and a sub-round loop ends on an empty compaction)
@@ -836,6 +842,7 @@ int local_move_worklist(lv_context* c, const Level& lv, uint32_t* vcom, const do
     sp.reset = a.mv_n;  // the moved-hub list starts empty in every sub-round
     sp.ndrop = a.ht_ndrop;
   }
+  if (!bt) sp.el = a.el;  // (the batched form reads its per-graph gains)
   WlSplit sp_same = sp;  // next-sub-round compactions read the marked segments only
   sp_same.dirty = a.wl_dirty;
   // Sub-rounds are enqueued kSubRounds at a time (their kernels read the
@@ -948,9 +955,10 @@ int local_move_worklist(lv_context* c, const Level& lv, uint32_t* vcom, const do
       };
       if (!graphs || !c->replay(group_key(), group)) group();
       c->check();
-      uint32_t nq[kWlQueues + 1];
-      c->read(nq, lens, sp.ndrop ? kWlQueues + 1 : kWlQueues);
+      uint32_t nq[8];
+      c->read(nq, lens, sp.el ? 8 : sp.ndrop ? kWlQueues + 1 : kWlQueues);
       if (sp.ndrop) ht->ht_scan = nq[kWlQueues] != 0;
+      if (sp.el) std::memcpy(&el, nq + 6, sizeof el);
       uint64_t left = 0;
       for (int q = 0; q < kWlQueues; ++q) left += nq[q];
       if (!left) break;
@@ -961,7 +969,7 @@ int local_move_worklist(lv_context* c, const Level& lv, uint32_t* vcom, const do
       if (!batched_step(c, bt, L)) break;
       continue;
     }
-    el = read_el(c, a.el);
+    // (el came with the last group's queue lengths)
     if (el <= E) break;
   }
   return (l > 1 || el) ? l : 0;
@@ -1154,7 +1162,7 @@ void build_hub_tables(lv_context* c, const lv_dgraph* x, lv_dstate* s, const uin
 }
 
 void invoke(lv_context* c, int approach, const lv_dgraph* x, const lv_dbatch* b, lv_dstate* s, const lv_options* o,
-            int& l, int& p, RunTimes& rt) {
+            int& l, int& p, RunTimes& rt, uint64_t& affected) {
   const uint64_t S = x->S;
   const double R = o->resolution;
   const int L = o->max_iterations, P = o->max_passes;
@@ -1301,8 +1309,16 @@ void invoke(lv_context* c, int approach, const lv_dgraph* x, const lv_dbatch* b,
   ++s->runs;
   hipEvent_t e3 = c->record();
   if (!t1_set) t1 = e3;
-  HIPCHK(hipEventSynchronize(e3));
+  // countValueOmp(vaff) of the result (inc/louvain.hxx:1220), after the
+  // timed region, read back with the run's one closing synchronisation
+  unsigned long long* dcount = c->ws<unsigned long long>("affected", 1);
+  k_fill_u64(reinterpret_cast<uint64_t*>(dcount), 0, 1, c->st);
+  k_count_u8(vaff, 1, S, dcount, c->st);
+  unsigned long long* pin = c->tail_pin();
+  HIPCHK(hipMemcpyAsync(pin, dcount, sizeof *pin, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
   c->check();
+  affected = *pin;
   rt.time += elapsed(e0, e3);
   rt.init += elapsed(e0, e1);
   rt.mark += elapsed(e1, e2);
@@ -1336,6 +1352,7 @@ void run(lv_context* c, int approach, const lv_dgraph* x, const lv_dbatch* b, lv
   }
   RunTimes rt;
   int l = 0, p = 0;
+  uint64_t aff = 0;  // affected vertices after the last repeat
   for (int r = 0; r < repeat; ++r) {
     if (r > 0 && approach != LV_STATIC) {
       d2d(c, s->memb, bm, S);
@@ -1344,16 +1361,12 @@ void run(lv_context* c, int approach, const lv_dgraph* x, const lv_dbatch* b, lv
     }
     c->ev_used = 0;
     c->prof.clear();
-    invoke(c, approach, x, b, s, o, l, p, rt);
+    invoke(c, approach, x, b, s, o, l, p, rt, aff);
     for (auto& pr : c->prof) {
       c->stats.move_ms[pr.first] += elapsed(pr.second.first, pr.second.second);
       c->stats.move_launches[pr.first] += 1;
     }
   }
-  unsigned long long* dcount = c->ws<unsigned long long>("affected", 1);
-  k_fill_u64(reinterpret_cast<uint64_t*>(dcount), 0, 1, c->st);
-  k_count_u8(c->ws<uint8_t>("vaff", (S + 7) & ~3ull), 1, S, dcount, c->st);
-  unsigned long long aff = c->read1(dcount);
   if (res) {
     res->iterations = l;
     res->passes = p;
@@ -1367,7 +1380,7 @@ void run(lv_context* c, int approach, const lv_dgraph* x, const lv_dbatch* b, lv
     if (res->membership) d2h(c, res->membership, s->memb, S);
     if (res->vertex_weight) d2h(c, res->vertex_weight, s->vtot, S);
     if (res->community_weight) d2h(c, res->community_weight, s->ctot, S);
-    HIPCHK(hipStreamSynchronize(c->st));
+    if (res->membership || res->vertex_weight || res->community_weight) HIPCHK(hipStreamSynchronize(c->st));
   }
   c->stats.runs += 1;
 }
@@ -2106,6 +2119,7 @@ void lv_context_destroy(lv_context* c) {
   c->drop_graph();
   for (auto e : c->events) (void)hipEventDestroy(e);
   if (c->pinned) (void)hipHostFree(c->pinned);
+  if (c->tail) (void)hipHostFree(c->tail);
   for (int k = 0; k < 2; ++k) {
     if (c->side[k]) {
       (void)hipStreamSynchronize(c->side[k]);

@@ -79,6 +79,7 @@ struct WlSplit {
   uint8_t* dirty;             // per segment: a flag may be set (nullable: scan all)
   uint32_t* reset;            // zeroed by the compaction (nullable)
   const unsigned long long* ndrop;  // copied to lens[kWlQueues], saturated to u32 (nullable)
+  const double* el;            // gain slots summed in slot order into lens[6..7] as a double (nullable)
   uint32_t shift;             // segment of block b: ids [b << shift, (b + 1) << shift)
 };
 // Segment shift of the compaction: 2^shift ids per block (a multiple of

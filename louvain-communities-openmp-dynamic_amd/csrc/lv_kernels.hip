@@ -2364,6 +2364,11 @@ __global__ __launch_bounds__(kBlock) void wl_scatter_kernel(uint8_t* f, int clea
   if (blockIdx.x == 0 && threadIdx.x < kWlQueues) lens[threadIdx.x] = (uint32_t)s_pre[threadIdx.x];
   if (blockIdx.x == 0 && threadIdx.x == 0 && sp.reset) *sp.reset = 0;
   if (blockIdx.x == 0 && threadIdx.x == 0 && sp.ndrop) lens[kWlQueues] = (uint32_t)min(*sp.ndrop, 0xFFFFFFFFull);
+  if (blockIdx.x == 0 && threadIdx.x == 0 && sp.el) {  // the iteration's gain so far, summed as the host does
+    double t = 0;
+    for (int sl = 0; sl < kSpread; ++sl) t += sp.el[(size_t)sl * kElStride];
+    *reinterpret_cast<double*>(lens + 6) = t;
+  }
   if (skip) return;
   if (sp.dirty && clear && threadIdx.x == 0) sp.dirty[blockIdx.x] = 0;
   for (uint64_t t0 = lo; t0 < hi; t0 += kWlTile) {  // block-uniform trip count
