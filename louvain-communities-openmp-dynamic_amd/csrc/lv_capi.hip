@@ -1170,7 +1170,9 @@ void invoke(lv_context* c, int approach, const lv_dgraph* x, const lv_dbatch* b,
   uint32_t* ucom = s->memb;
   double* utot = s->vtot;
   double* ctot = s->ctot;
-  // buffers (allocation and resets are outside the timed region, as in the reference)
+  // buffers (allocation and resets are outside the timed region, as in the
+  // reference); the super-level membership and K (vcom, vtot) are written
+  // by each level's aggregation and initialisation before any read
   uint8_t* vaff = c->ws<uint8_t>("vaff", (S + 7) & ~3ull);
   uint32_t* vcom = c->ws<uint32_t>("vcom", S);
   double* vtot = c->ws<double>("vtot", S);
@@ -1183,8 +1185,6 @@ void invoke(lv_context* c, int approach, const lv_dgraph* x, const lv_dbatch* b,
   bool worklist = c->path == 2 && (approach == LV_DYNAMIC_FRONTIER || approach == LV_DELTA_SCREENING);
   SuperGraph y = super_graph(c, "y", S, x->E), z = super_graph(c, "z", S, x->E);
   k_fill_u8(vaff, 0, (S + 7) & ~3ull, c->st);
-  k_fill_u32(vcom, 0, S, c->st);
-  k_fill_f64(vtot, 0, S, c->st);
   if (approach == LV_STATIC) {
     k_fill_u32(ucom, 0, S, c->st);
     k_fill_f64(utot, 0, S, c->st);

@@ -449,17 +449,34 @@ __global__ void initialize_kernel(GView g, uint32_t* vcom, double* ctot, const d
 // louvainUpdateWeightsFromOmpU (inc/louvain.hxx:374-388): K[u] and
 // Sigma_tot[vcom[u]] minus deletions, plus insertions.  f64 atomics; the
 // reference's per-key batch order only matters for rounding.
-__global__ void update_weights_kernel(const uint32_t* du, const float* dw, uint64_t nd, const uint32_t* iu,
-                                      const float* iw, uint64_t ni, const uint32_t* vcom, double* vtot,
-                                      double* ctot) {
-  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < nd + ni;
-       i += (uint64_t)gridDim.x * blockDim.x) {
-    uint32_t u;
-    double w;
-    if (i < nd) { u = du[i]; w = -(double)dw[i]; }
-    else { u = iu[i - nd]; w = (double)iw[i - nd]; }
-    unsafeAtomicAdd(&vtot[u], w);
-    unsafeAtomicAdd(&ctot[vcom[u]], w);
+// A wave takes 64 consecutive entries of one list (deletions in the first
+// waves, insertions after); a list is sorted by source, so the entries of a
+// source are consecutive lanes: the first lane of each run adds the run's
+// sum (one atomic pair per source and wave instead of per entry -- an R-MAT
+// batch has hundreds of entries on a hub).
+__global__ __launch_bounds__(kBlock) void update_weights_kernel(const uint32_t* du, const float* dw, uint64_t nd,
+                                                                const uint32_t* iu, const float* iw, uint64_t ni,
+                                                                const uint32_t* vcom, double* vtot, double* ctot) {
+  const uint64_t wd = (nd + 63) / 64, wn = wd + (ni + 63) / 64;  // waves of each list
+  const int lane = threadIdx.x % 64;
+  for (uint64_t wv = blockIdx.x * (uint64_t)(kBlock / 64) + threadIdx.x / 64; wv < wn;
+       wv += (uint64_t)gridDim.x * (kBlock / 64)) {  // wave-uniform
+    const bool del = wv < wd;
+    const uint64_t i = (del ? wv : wv - wd) * 64 + lane, n = del ? nd : ni;
+    const bool in = i < n;
+    const uint32_t u = in ? (del ? du[i] : iu[i]) : 0xFFFFFFFFu;
+    double s = in ? (del ? -(double)dw[i] : (double)iw[i]) : 0.0;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {  // s: the sum over this lane's run from here on
+      const double y = __shfl_down(s, d, 64);
+      const uint32_t uy = __shfl_down(u, d, 64);
+      if (lane + d < 64 && uy == u) s += y;
+    }
+    const uint32_t up = __shfl_up(u, 1, 64);
+    if (in && (lane == 0 || up != u)) {
+      unsafeAtomicAdd(&vtot[u], s);
+      unsafeAtomicAdd(&ctot[vcom[u]], s);
+    }
   }
 }
 
@@ -556,11 +573,24 @@ __global__ __launch_bounds__(kBlock) void ds_sweep_kernel(GView g, const uint32_
   }
 }
 
+// Bytes of w equal to v (the zero bytes of w ^ vvvv).
+__device__ __forceinline__ uint32_t bytes_equal(uint32_t w, uint8_t v) {
+  const uint32_t x = w ^ (0x01010101u * v);
+  return 4 - __popc((((x & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | x) & 0x80808080u);
+}
+
+// Entries of a[0 .. n) equal to v, added to *out: 16 bytes per load where a
+// is 16-byte aligned, then the tail byte by byte.
 __global__ __launch_bounds__(kBlock) void count_u8_kernel(const uint8_t* a, uint8_t v, uint64_t n,
                                                           unsigned long long* out) {
   unsigned long long c = 0;
-  for (uint64_t i = blockIdx.x * (uint64_t)kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock)
-    c += a[i] == v;
+  const uint64_t t = blockIdx.x * (uint64_t)kBlock + threadIdx.x, T = (uint64_t)gridDim.x * kBlock;
+  const uint64_t n16 = ((uintptr_t)a & 15) ? 0 : n / 16;
+  for (uint64_t i = t; i < n16; i += T) {
+    const uint4 x = reinterpret_cast<const uint4*>(a)[i];
+    c += bytes_equal(x.x, v) + bytes_equal(x.y, v) + bytes_equal(x.z, v) + bytes_equal(x.w, v);
+  }
+  for (uint64_t i = n16 * 16 + t; i < n; i += T) c += a[i] == v;
   c = block_sum_u64<kBlock>(c);
   if (threadIdx.x == 0 && c) atomicAdd(out, c);
 }
@@ -2517,7 +2547,8 @@ void k_initialize(GView g, uint32_t* vcom, double* ctot, const double* vtot, hip
 void k_update_weights(const uint32_t* du, const float* dw, uint64_t nd, const uint32_t* iu, const float* iw,
                       uint64_t ni, const uint32_t* vcom, double* vtot, double* ctot, hipStream_t st) {
   if (nd + ni)
-    LV_LAUNCH(update_weights_kernel, grid_for(nd + ni, kBlock), kBlock, st, du, dw, nd, iu, iw, ni, vcom, vtot, ctot);
+    LV_LAUNCH(update_weights_kernel, grid_for(nd + ni + 128, kBlock), kBlock, st, du, dw, nd, iu, iw, ni, vcom, vtot,
+              ctot);
 }
 void k_frontier(const uint32_t* du, const uint32_t* dv, uint64_t nd, const uint32_t* iu, const uint32_t* iv,
                 uint64_t ni, const uint32_t* vcom, uint8_t* vaff, hipStream_t st) {
