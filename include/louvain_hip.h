@@ -178,6 +178,8 @@ int lv_context_set_graphs(lv_context* ctx, int on);
  * edges and moves keep the tables exact.  No counterpart in the reference:
  * an incremental index of this build. */
 int lv_context_set_hub_tables(lv_context* ctx, int on);
+/* Counters since the last reset.  The work counters of local-moving are kept
+ * on the device and read here (this waits for the context's stream). */
 int lv_stats_get(const lv_context* ctx, lv_stats* out);
 int lv_stats_reset(lv_context* ctx);
 
