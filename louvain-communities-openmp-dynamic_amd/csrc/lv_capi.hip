@@ -446,6 +446,8 @@ constexpr int kTierMidHub = 7;    // launch_move: worklist mid hubs through k_hu
 constexpr int kTierSmallHub = 8;  // launch_move: worklist small hubs through k_hub_mid (stats slot 2)
 constexpr int kTierTabHub = 9;    // launch_move: worklist hubs with a table through k_hub_tab (stats slot 1)
 constexpr int kSubRounds = 4;      // worklist sub-rounds enqueued per host check
+constexpr int kFirstGroup = 8;     // ... in an iteration's first group (a DF step on an R-MAT graph
+                                   // typically needs 5-8: one host check instead of two)
 constexpr uint64_t kRefChunk = 2048;  // the reference's OpenMP chunk (inc/louvain.hxx:594)
 
 void launch_move(lv_context* c, const MoveArgs& a, int tier, bool f64, hipStream_t s = nullptr) {
@@ -845,7 +847,7 @@ int local_move_worklist(lv_context* c, const Level& lv, uint32_t* vcom, const do
   if (!bt) sp.el = a.el;  // (the batched form reads its per-graph gains)
   WlSplit sp_same = sp;  // next-sub-round compactions read the marked segments only
   sp_same.dirty = a.wl_dirty;
-  // Sub-rounds are enqueued kSubRounds at a time (their kernels read the
+  // Sub-rounds are enqueued kFirstGroup, then kSubRounds at a time (their kernels read the
   // queue lengths on the device; an empty round costs its launches only),
   // with one host check of the last compaction per group.  Without
   // profiling a group is replayed from a captured graph; the iteration
@@ -932,6 +934,7 @@ int local_move_worklist(lv_context* c, const Level& lv, uint32_t* vcom, const do
     k_wl_queue(vaff, false, S, lv.g.deg, sp, qs, lens, bsum, c->st);
     if (graphs) k_fill_u32(serial_dev, c->serial, 1, c->st);
     // what the captured launches depend on (the serial is read on the device)
+    int rounds = kFirstGroup;  // sub-rounds of the next group
     auto group_key = [&]() {
       std::string key;
       MoveArgs ka = a;
@@ -939,7 +942,7 @@ int local_move_worklist(lv_context* c, const Level& lv, uint32_t* vcom, const do
       ka.list = nullptr;
       ka.n_dev = nullptr;
       ka.cnt = cnt0;
-      const int flags[5] = {c->tier_order, lv.f64 ? 1 : 0, (int)lv.maxdeg4, kSubRounds, scan_hubs ? 1 : 0};
+      const int flags[5] = {c->tier_order, lv.f64 ? 1 : 0, (int)lv.maxdeg4, rounds, scan_hubs ? 1 : 0};
       key.append(reinterpret_cast<const char*>(&ka), sizeof ka);
       key.append(reinterpret_cast<const char*>(&qs), sizeof qs);
       key.append(reinterpret_cast<const char*>(&sp), sizeof sp);
@@ -949,9 +952,9 @@ int local_move_worklist(lv_context* c, const Level& lv, uint32_t* vcom, const do
       key.append(reinterpret_cast<const char*>(&S), sizeof S);
       return key;
     };
-    for (;;) {
+    for (;; rounds = kSubRounds) {
       auto group = [&] {
-        for (int r = 0; r < kSubRounds; ++r) sub_round();
+        for (int r = 0; r < rounds; ++r) sub_round();
       };
       if (!graphs || !c->replay(group_key(), group)) group();
       c->check();
