@@ -176,30 +176,38 @@ struct lv_context {
   // Worklist sub-round groups replayed from captured graphs (one launch
   // instead of some fifty, and no host gaps between them on the device).  A
   // graph is keyed by everything its kernels and their launch shapes depend
-  // on.  A new key is captured and, when a graph exists already, written
-  // into the most recent one (hipGraphExecUpdate: the same launches with
-  // other arguments), else instantiated -- the latter only for a key an
+  // on.  A new key is captured and, when a graph of its shape (the same
+  // launches: sub-rounds per group, hub kernels, tables) exists already,
+  // written into the most recent one of them (hipGraphExecUpdate: other
+  // arguments only), else instantiated -- the latter only for a key an
   // earlier call used too.  The last kGraphCache graphs are kept.
   static constexpr size_t kGraphCache = 8, kSeenKeys = 32;
   int use_graphs = 1;
   int use_tables = 1;  // hub neighbour-community tables with resident states
-  std::vector<std::pair<std::string, hipGraphExec_t>> graphs;  // most recent last
+  struct GraphEntry {
+    std::string key;
+    int shape;  // launches of the same shape have the same topology (updatable in place)
+    hipGraphExec_t ex;
+  };
+  std::vector<GraphEntry> graphs;  // most recent last
   std::vector<std::pair<std::string, uint64_t>> seen;          // (key, call)
   uint64_t wl_calls = 0;                                       // worklist calls so far
   // true if the group was launched from a graph; false: enqueue it directly
   template <class F>
-  bool replay(const std::string& key, F enqueue) {
+  bool replay(const std::string& key, int shape, F enqueue) {
     for (size_t k = 0; k < graphs.size(); ++k) {
-      if (graphs[k].first != key) continue;
+      if (graphs[k].key != key) continue;
       auto hit = graphs[k];
       graphs.erase(graphs.begin() + k);
       graphs.push_back(hit);
-      HIPCHK(hipGraphLaunch(hit.second, st));
+      HIPCHK(hipGraphLaunch(hit.ex, st));
       return true;
     }
     auto it = std::find_if(seen.begin(), seen.end(), [&](const auto& e) { return e.first == key; });
     const bool again = it != seen.end() && it->second != wl_calls;
-    if (graphs.empty() && !again) {  // nothing to update, and a key not used before
+    const bool have_shape =
+        std::any_of(graphs.begin(), graphs.end(), [&](const GraphEntry& e) { return e.shape == shape; });
+    if (!have_shape && !again) {  // nothing to update, and a key not used before
       if (it == seen.end()) {
         if (seen.size() >= kSeenKeys) seen.erase(seen.begin());
         seen.push_back({key, wl_calls});
@@ -217,31 +225,36 @@ struct lv_context {
       throw;
     }
     HIPCHK(hipStreamEndCapture(st, &g));
-    if (!graphs.empty()) {
+    for (size_t k = graphs.size(); k-- > 0;) {  // the most recent graph of this shape, updated in place
+      if (graphs[k].shape != shape) continue;
       hipGraphNode_t bad = nullptr;
       hipGraphExecUpdateResult res;
-      if (hipGraphExecUpdate(graphs.back().second, g, &bad, &res) == hipSuccess) {
+      if (hipGraphExecUpdate(graphs[k].ex, g, &bad, &res) == hipSuccess) {
         (void)hipGraphDestroy(g);
-        graphs.back().first = key;
-        HIPCHK(hipGraphLaunch(graphs.back().second, st));
+        auto hit = graphs[k];
+        hit.key = key;
+        graphs.erase(graphs.begin() + k);
+        graphs.push_back(hit);
+        HIPCHK(hipGraphLaunch(hit.ex, st));
         return true;
       }
       (void)hipGetLastError();  // not updatable: instantiate it
+      break;
     }
     hipGraphExec_t ex = nullptr;
     hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     HIPCHK(e);
     if (graphs.size() >= kGraphCache) {
-      (void)hipGraphExecDestroy(graphs.front().second);
+      (void)hipGraphExecDestroy(graphs.front().ex);
       graphs.erase(graphs.begin());
     }
-    graphs.push_back({key, ex});
+    graphs.push_back({key, shape, ex});
     HIPCHK(hipGraphLaunch(ex, st));
     return true;
   }
   void drop_graph() {
-    for (auto& g : graphs) (void)hipGraphExecDestroy(g.second);
+    for (auto& g : graphs) (void)hipGraphExecDestroy(g.ex);
     graphs.clear();
     seen.clear();
   }
@@ -956,7 +969,8 @@ int local_move_worklist(lv_context* c, const Level& lv, uint32_t* vcom, const do
       auto group = [&] {
         for (int r = 0; r < rounds; ++r) sub_round();
       };
-      if (!graphs || !c->replay(group_key(), group)) group();
+      const int shape = rounds * 4 + (scan_hubs ? 2 : 0) + (a.ht_cap ? 1 : 0);
+      if (!graphs || !c->replay(group_key(), shape, group)) group();
       c->check();
       uint32_t nq[8];
       c->read(nq, lens, sp.el ? 8 : sp.ndrop ? kWlQueues + 1 : kWlQueues);
