@@ -881,21 +881,28 @@ int local_move_worklist(lv_context* c, const Level& lv, uint32_t* vcom, const do
   // one sub-round: hubs and the rest, then the next queue
   auto sub_round = [&]() {
     if (c->tier_order == 2) {
-      // id-order mode, three streams at once: the low-degree vertices
-      // through the id-order kernel, hubs with a table (a wave each) then
-      // the other hubs (a block each, mid then small), and the largest hubs
-      // (multi-block batch).  Like the reference's threads, each sees the
-      // moves the others have made.
-      const int sides = scan_hubs ? 2 : 1;  // the hub batch's stream only while it runs
+      // id-order mode: the low-degree vertices through the id-order
+      // kernel, hubs with a table (a wave each), and while some hub has no
+      // table the other hubs (a block each, mid then small) and the largest
+      // (multi-block batch), these on two side streams at once.  Like the
+      // reference's threads, each sees the moves the others have made.
+      // Without the edge-scanning kernels everything runs on the context's
+      // stream (the tabled hubs first): the cross-stream waits cost more
+      // than the overlap gains.
+      const int sides = scan_hubs ? 2 : 0;
+      hipStream_t hs = sides ? c->side[0] : c->st;
       if (S) {
-        c->make_side();
-        c->fork(sides);
-        if (a.ht_cap) {  // stats slot 1; its few blocks are resident before the low-degree kernel's many
+        if (sides) {
+          c->make_side();
+          c->fork(sides);
+          hs = c->side[0];
+        }
+        if (a.ht_cap) {  // stats slot 1
           a.list = qs.q[4];
           a.n_dev = lens + 4;
           a.cnt = cnt0 + (size_t)1 * CNT_N;
-          launch_move(c, a, kTierTabHub, lv.f64, c->side[0]);
-          k_moved_flag(a, c->side[0]);
+          launch_move(c, a, kTierTabHub, lv.f64, hs);
+          k_moved_flag(a, hs);
         }
       }
       a.list = qs.q[0];
@@ -915,9 +922,9 @@ int local_move_worklist(lv_context* c, const Level& lv, uint32_t* vcom, const do
           a.cnt = cnt0 + (size_t)4 * CNT_N;
           launch_move(c, a, kTierHubBatch, lv.f64, c->side[1]);
         } else {
-          k_carry_hubs(a, qs, lens, c->side[0]);  // tables dropped meanwhile: kept for later
+          k_carry_hubs(a, qs, lens, hs);  // tables dropped meanwhile: kept for later
         }
-        c->join(sides);
+        if (sides) c->join(sides);
       }
     } else {
       k_fill_u32(tlen, 0, kTiers, c->st);
