@@ -902,7 +902,7 @@ int local_move_worklist(lv_context* c, const Level& lv, uint32_t* vcom, const do
           a.n_dev = lens + 4;
           a.cnt = cnt0 + (size_t)1 * CNT_N;
           launch_move(c, a, kTierTabHub, lv.f64, hs);
-          k_moved_flag(a, hs);
+          k_moved_flag(a, qs, lens, !scan_hubs, hs);  // with the hubs whose tables dropped, kept for later
         }
       }
       a.list = qs.q[0];
@@ -921,8 +921,6 @@ int local_move_worklist(lv_context* c, const Level& lv, uint32_t* vcom, const do
           a.n_dev = lens + 3;
           a.cnt = cnt0 + (size_t)4 * CNT_N;
           launch_move(c, a, kTierHubBatch, lv.f64, c->side[1]);
-        } else {
-          k_carry_hubs(a, qs, lens, hs);  // tables dropped meanwhile: kept for later
         }
         if (sides) c->join(sides);
       }

@@ -1637,9 +1637,13 @@ __global__ __launch_bounds__(kBlock) void hub_tab_kernel(MoveArgs a) {
 
 // Neighbours of the tabled hubs that moved in the kernel before (list
 // a.mv_*, at most a.mv_cap of *a.mv_n): every block takes its share of each
-// hub's row.
+// hub's row.  With carry, the hubs queued for the edge-scanning kernels
+// (queues 1-3: tables dropped in this run) while those kernels are not
+// launched go back to the next sub-round, so none is lost before the host
+// launches them again.
 template <class WT>
-__global__ __launch_bounds__(kBlock) void moved_flag_kernel(MoveArgs a) {
+__global__ __launch_bounds__(kBlock) void moved_flag_kernel(MoveArgs a, WlQueues qs, const uint32_t* lens,
+                                                            int carry) {
   const uint32_t n = min(*a.mv_n, a.mv_cap);
   const WT* W = (const WT*)a.g.w;
   for (uint32_t m = 0; m < n; ++m) {
@@ -1649,15 +1653,10 @@ __global__ __launch_bounds__(kBlock) void moved_flag_kernel(MoveArgs a) {
     if (lo >= hi) continue;
     flag_neighbours<kBlock, WT>(a, W, u, o + lo, hi - lo, threadIdx.x, a.mv_from[m], a.mv_to[m]);
   }
-}
-
-// Hubs queued for the edge-scanning kernels (queues 1-3: tables dropped in
-// this run) while those kernels are not launched: back to the next
-// sub-round, so none is lost before the host launches them again.
-__global__ void carry_hubs_kernel(MoveArgs a, WlQueues qs, const uint32_t* lens) {
+  if (!carry) return;
   for (int q = 1; q <= 3; ++q) {
-    const uint32_t n = lens[q];
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    const uint32_t nq = lens[q];
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nq; i += gridDim.x * blockDim.x)
       mark_same(a, qs.q[q][i]);
   }
 }
@@ -2663,12 +2662,9 @@ __global__ void flag_nonzero_kernel(const uint32_t* a, uint64_t n, uint8_t* flag
 void k_flag_nonzero(const uint32_t* a, uint64_t n, uint8_t* flags, hipStream_t st) {
   if (n) LV_LAUNCH(flag_nonzero_kernel, grid_for(n, kBlock * 4), kBlock, st, a, n, flags);
 }
-void k_moved_flag(const MoveArgs& a, hipStream_t st) {
-  if (a.g.w && a.f64_weights) LV_LAUNCH(moved_flag_kernel<double>, 256, kBlock, st, a);
-  else LV_LAUNCH(moved_flag_kernel<float>, 256, kBlock, st, a);
-}
-void k_carry_hubs(const MoveArgs& a, const WlQueues& qs, const uint32_t* lens, hipStream_t st) {
-  LV_LAUNCH(carry_hubs_kernel, 64, kBlock, st, a, qs, lens);
+void k_moved_flag(const MoveArgs& a, const WlQueues& qs, const uint32_t* lens, bool carry, hipStream_t st) {
+  if (a.g.w && a.f64_weights) LV_LAUNCH(moved_flag_kernel<double>, 256, kBlock, st, a, qs, lens, (int)carry);
+  else LV_LAUNCH(moved_flag_kernel<float>, 256, kBlock, st, a, qs, lens, (int)carry);
 }
 void k_ht_new_hubs(const uint32_t* iu, uint64_t ni, const uint32_t* deg, uint32_t* cap, uint64_t* off,
                    unsigned long long* next, uint64_t total, uint32_t* list, uint32_t* nlist, hipStream_t st) {
