@@ -2652,8 +2652,12 @@ void k_hub_mid(const MoveArgs& a, int cls, hipStream_t st) {
   }
 }
 void k_hub_tab(const MoveArgs& a, hipStream_t st) {
-  if (a.g.w && a.f64_weights) LV_LAUNCH(hub_tab_kernel<double>, 1024, kBlock, st, a);
-  else LV_LAUNCH(hub_tab_kernel<float>, 1024, kBlock, st, a);
+  // about one wave per tabled hub of a sub-round (their number grows with
+  // the graph: ~750 at 4M vertices); a launch with few hubs then dispatches
+  // few blocks
+  const uint64_t grid = std::min<uint64_t>(1024, std::max<uint64_t>(128, a.g.S >> 14));
+  if (a.g.w && a.f64_weights) LV_LAUNCH(hub_tab_kernel<double>, grid, kBlock, st, a);
+  else LV_LAUNCH(hub_tab_kernel<float>, grid, kBlock, st, a);
 }
 __global__ void flag_nonzero_kernel(const uint32_t* a, uint64_t n, uint8_t* flags) {
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
