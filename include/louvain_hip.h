@@ -137,6 +137,44 @@ int lv_context_set_schedule(lv_context* ctx, double stream_fraction, uint64_t mi
  * vertex to a community with a lower id than its own (no two vertices can
  * then swap communities in one iteration).  Default 0 (off). */
 int lv_context_set_pick_less(lv_context* ctx, int iterations);
+/* Sweeps (order 2): in_sweep = 1 (default) visits every vertex of more than
+ * 4096 edges in its turn of the ascending sweep, as the reference does
+ * (inc/louvain.hxx:594), each with an HBM table of its own; 0 runs all of
+ * them at once before each sweep (the hub kernel).  balance = 1 (default)
+ * cuts spread chunks by work (degree plus a per-vertex cost) rather than by
+ * id count, so a stream whose ids hold the largest hubs is not the last to
+ * finish. */
+int lv_context_set_sweep_hubs(lv_context* ctx, int in_sweep, int balance);
+/* Validated moves of the id-order kernel (order 2): a vertex about to move
+ * reads its neighbours' communities again and, if a concurrent stream moved
+ * one of them while the vertex was being decided, decides again from the
+ * new view (at most twice).  This keeps the decisions on the neighbour view
+ * a sequential sweep would have (inc/louvain.hxx:594-605), which concurrent
+ * streams otherwise lose; one stream never sees a change, so at one stream
+ * the results are unchanged.  0 off, 1 sweeps (every pass except a DF / DS
+ * worklist first pass; default), 2 sweeps and worklists. */
+int lv_context_set_validate(lv_context* ctx, int mode);
+/* Device workspace of the context (grow-only scratch buffers kept between
+ * calls): lv_context_workspace_bytes reports its size; with a limit set
+ * (bytes > 0), a call that would grow it beyond the limit fails with
+ * LV_ERR_NOMEM and leaves its graph and state as they were.  0 removes the
+ * limit.  No counterpart in the reference (host memory, no limit). */
+int lv_context_set_workspace_limit(lv_context* ctx, uint64_t bytes);
+uint64_t lv_context_workspace_bytes(const lv_context* ctx);
+/* Sweep schedules by pass (id-order sweeps, order 2): the first
+ * lead_iterations local-moving iterations of a Static run's first pass run
+ * lead_streams streams over ascending chunks of lead_chunk ids; every pass
+ * after the first runs late_streams streams (chunks of late_chunk ids, 0:
+ * spread) when late_streams is not 0.  The role of the reference's
+ * schedule(dynamic, 2048) loop (inc/louvain.hxx:594) at a chosen width.
+ * Defaults: 0 (off), 16, 2048, 0, 0. */
+int lv_context_set_pass_schedule(lv_context* ctx, int lead_iterations, uint64_t lead_streams, uint64_t lead_chunk,
+                                 uint64_t late_streams, uint64_t late_chunk);
+/* The lead schedule of lv_context_set_pass_schedule over the first `ids` ids
+ * only (0: all ids, the default; lv_context_set_pass_schedule resets it):
+ * a lead iteration sweeps ids [0, ids) under it, then the rest of the ids
+ * under the context's schedule. */
+int lv_context_set_lead_ids(lv_context* ctx, uint64_t ids);
 /* Id-order sweeps (order 2) cut the ids into `slabs` consecutive ranges
  * launched one after another, each seeing every move of the ranges before
  * it; the streams spread within a range.  Default 1. */

@@ -9,6 +9,7 @@
 #include <atomic>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <map>
 #include <stdexcept>
 #include <string>
@@ -57,7 +58,27 @@ struct lv_context {
   uint64_t min_streams = 1;
   uint64_t max_streams = 1ull << 20;
   uint64_t chunk = 0;           // entries per dynamically assigned chunk (0: static spread)
+  // Sweep schedules by pass: the first lead_iters iterations of the first
+  // pass run lead_streams streams over ascending chunks of lead_chunk
+  // entries; passes after the first run late_streams streams (chunks of
+  // late_chunk, 0: spread) when late_streams is set.
+  int lead_iters = 0;
+  uint64_t lead_streams = 16, lead_chunk = 2048;
+  uint64_t lead_ids = 0;  // the lead schedule covers ids [0, lead_ids) (0: all); the rest as usual
+  uint64_t late_streams = 0, late_chunk = 0;
   int pick_less_iters = 0;      // first iterations of each pass that only move to lower ids
+  // validated moves of the id-order kernel: 0 off, 1 sweeps (every pass but
+  // a DF / DS worklist first pass), 2 sweeps and worklists
+  int validate = 1;
+  // sweeps: hubs (degree > kHubMin) visited in their id turn with tables of
+  // their own (1, default) or all at once before each sweep (0); spread
+  // chunks cut by work, degree plus a per-vertex cost (1, default), or by
+  // id count (0)
+  int sweep_hubs = 0;
+  int balance = 0;
+  // diagnostics (environment LV_TRACE set when the context is made): worklist
+  // runs enqueue launch by launch and print every sub-round's queue lengths
+  int trace = 0;
   uint64_t slabs = 1;           // id-order sweeps: consecutive id slabs launched in turn
   uint64_t hub_streams = kHubBlocks;  // hubs processed at once
   uint32_t hub_min = kHubMin;         // sweeps: degrees above run in the hub batch
@@ -85,19 +106,27 @@ struct lv_context {
   size_t scan_tmp_bytes = 0;
 
   // Grow-only named device buffer; contents are unspecified (*fresh is set
-  // when it was just allocated).
+  // when it was just allocated).  The buffers together may not exceed
+  // ws_limit bytes: a growth beyond it fails with LV_ERR_NOMEM and keeps the
+  // buffer as it was.
+  size_t ws_total = 0;
+  size_t ws_limit = ~size_t(0);
   template <class T>
   T* ws(const std::string& name, size_t count, bool* fresh = nullptr) {
     size_t bytes = std::max<size_t>(count * sizeof(T), 256);
     Buf& b = bufs[name];
     if (fresh) *fresh = b.bytes < bytes;
     if (b.bytes < bytes) {
+      size_t want = bytes + bytes / 8;  // headroom for slightly larger later calls
+      if (ws_total - b.bytes + want > ws_limit)
+        throw LvError{LV_ERR_NOMEM, "context workspace limit reached (buffer " + name + ")"};
       if (b.p) HIPCHK(hipFree(b.p));
+      ws_total -= b.bytes;
       b.p = nullptr;
       b.bytes = 0;
-      size_t want = bytes + bytes / 8;  // headroom for slightly larger later calls
       HIPCHK(hipMalloc(&b.p, want));
       b.bytes = want;
+      ws_total += want;
     }
     return reinterpret_cast<T*>(b.p);
   }
@@ -288,6 +317,7 @@ struct lv_dgraph {
   TierLists hubs;   // tier 4 = degree > kHubMin (id-order mode), ascending
   uint64_t hub_slots = 0;
   uint64_t version = 0;  // counts the updates applied to the graph
+  bool broken = false;   // a failed update left the rows inconsistent: the graph is refused
   // process-unique stamp of the current degrees, new at upload and at every
   // update: contexts key what they derive from the degrees on it
   uint64_t stamp = 0;
@@ -316,6 +346,12 @@ struct lv_dbatch {
   bool checked = false;
   uint64_t id_bound = 0;
   int check_flags = 0;
+  // Made by lv_dgraph_update: the graph's stamp before and after the batch
+  // was applied (0: not made by an update), and the stored weight of every
+  // deletion as it was in the graph (the batch keeps the caller's weights,
+  // which the weight update uses, inc/louvain.hxx:374-388).
+  uint64_t from_stamp = 0, to_stamp = 0;
+  float* dws = nullptr;
 };
 
 struct lv_dstate {
@@ -325,10 +361,11 @@ struct lv_dstate {
   double* ctot = nullptr;
   // Hub neighbour-community tables kept with the state (worklist DF / DS):
   // valid while every membership change since their build went through the
-  // worklist first pass, which keeps them exact; ht_live is the live edge
-  // count of the graph they describe (a run checks that its graph is that
-  // graph plus its batch).  runs counts the runs since the last upload or
-  // copy: tables are built on the second run of a state.
+  // worklist first pass, which keeps them exact; ht_stamp is the stamp of the
+  // graph they describe: a run carries them forward only when its batch was
+  // made by lv_dgraph_update from that graph into the run's graph (or the
+  // batch is empty and the graph is that graph).  runs counts the runs since
+  // the last upload or copy: tables are built on the second run of a state.
   uint64_t* ht_off = nullptr;
   uint32_t* ht_cap = nullptr;
   uint64_t ht_slots = 0;              // slab slots allocated
@@ -340,7 +377,7 @@ struct lv_dstate {
   int32_t* ht_cnt = nullptr;
   double* ht_w = nullptr;
   bool ht_valid = false;
-  uint64_t ht_live = 0;
+  uint64_t ht_stamp = 0;
   uint64_t runs = 0;
   void ht_reset() {
     ht_valid = false;
@@ -414,23 +451,6 @@ uint32_t max_degree_tier4(lv_context* c, const TierLists& t, const uint32_t* deg
   return c->read1(out);
 }
 
-// Sum over the hubs (tier 4 of t) of nextpow2(2 deg): the table slots a hub
-// batch over all of them needs.
-uint64_t hub_slot_total(lv_context* c, const TierLists& t, const uint32_t* deg) {
-  uint64_t nh = t.start[5] - t.start[4];
-  if (!nh) return 0;
-  std::vector<uint32_t> ids(nh), d(nh);
-  c->read(ids.data(), t.ids + t.start[4], nh);
-  uint32_t* gd = c->ws<uint32_t>("hub.deg", nh);
-  uint32_t* gi = c->ws<uint32_t>("hub.ids", nh);
-  h2d(c, gi, ids.data(), nh);
-  k_gather_u32(deg, gi, nh, gd, c->st);
-  c->read(d.data(), gd, nh);
-  uint64_t tot = 0;
-  for (uint64_t k = 0; k < nh; ++k) tot += next_pow2(2 * (uint64_t)d[k]);
-  return tot;
-}
-
 // Sum of nextpow2(2 deg) over the degrees above thr (hub-batch table slots).
 uint64_t caps_total(lv_context* c, const uint32_t* deg, uint64_t S, uint32_t thr) {
   unsigned long long* tot = c->ws<unsigned long long>("hb.need", 1);
@@ -474,7 +494,7 @@ void launch_move(lv_context* c, const MoveArgs& a, int tier, bool f64, hipStream
   else if (tier == kTierMidHub || tier == kTierSmallHub) k_hub_mid(a, tier == kTierMidHub ? 1 : 0, s);
   else if (tier == kTierTabHub) k_hub_tab(a, s);
   else {
-    if (tier == kTierHybrid && a.ctab) k_graph_chunks(a, s);
+    if (tier == kTierHybrid && a.ctab && a.gcs) k_graph_chunks(a, s);
     k_move(a, tier, f64, s);
   }
   const int slot = tier == kTierHubBatch ? 4
@@ -698,13 +718,16 @@ MoveArgs base_args(lv_context* c, const Level& lv, uint32_t* vcom, const double*
 }
 
 // One sweep-mode iteration over the tier lists (or, in id-order mode, over
-// all ids ascending through the general HBM-table kernel).
-void move_sweep(lv_context* c, MoveArgs a, const Level& lv) {
+// all ids ascending through the general HBM-table kernel).  core > 0 (id
+// order): ids [0, core) first, by core_streams streams over ascending chunks
+// of core_chunk ids, then the rest under a's schedule.
+void move_sweep(lv_context* c, MoveArgs a, const Level& lv, uint64_t core = 0, uint64_t core_streams = 0,
+                uint64_t core_chunk = 0) {
   unsigned long long* cnt0 = a.cnt;
   if (c->tier_order == 2) {
     // hubs (degree > kHubMin, ascending) first, then every other id ascending
     uint64_t nh = lv.hubs->start[5] - lv.hubs->start[4];
-    if (nh && c->hub_min < lv.hubs->maxdeg) {
+    if (nh && !a.xids && c->hub_min < lv.hubs->maxdeg) {
       a.list = lv.hubs->ids + lv.hubs->start[4];
       a.n = nh;
       if (cnt0) a.cnt = cnt0 + (size_t)4 * CNT_N;
@@ -714,6 +737,20 @@ void move_sweep(lv_context* c, MoveArgs a, const Level& lv) {
     // every move of the slabs before it
     if (cnt0) a.cnt = cnt0;
     a.list = nullptr;
+    if (core && core < lv.g.S) {
+      MoveArgs b = a;
+      b.base = 0;
+      b.n = core;
+      b.max_streams = core_streams;
+      b.chunk = core_chunk;
+      b.ctab = nullptr;
+      launch_move(c, b, kTierHybrid, lv.f64);
+      a.base = core;
+      a.n = lv.g.S - core;
+      a.ctab = nullptr;
+      launch_move(c, a, kTierHybrid, lv.f64);
+      return;
+    }
     uint64_t P = std::max<uint64_t>(1, std::min<uint64_t>(c->slabs, lv.g.S));
     for (uint64_t k = 0; k < P; ++k) {
       uint64_t lo = lv.g.S * k / P, hi = lv.g.S * (k + 1) / P;
@@ -757,12 +794,57 @@ double read_el(lv_context* c, const double* el) {
 // per-graph results are in bt->it / bt->el.
 int local_move(lv_context* c, const Level& lv, uint32_t* vcom, const double* vtot, double* ctot, uint8_t* vaff,
                const uint8_t* fa, double M, double R, int L, double E, bool ascending = false,
-               Batched* bt = nullptr) {
+               Batched* bt = nullptr, int pass = -1) {
   MoveArgs a = base_args(c, lv, vcom, vtot, ctot, vaff, fa, M, R);
   batched_args(a, bt);
   a.sparse = c->path == 1;
   if (ascending && !a.chunk) a.chunk = a.sparse ? 64 : 16;
   batched_schedule(c, a, lv, bt);
+  // hubs in the sweep (a Static run's first pass and every later pass): each
+  // visited in its id turn by its stream, with a table of its own
+  // (nextpow2(2 deg) slots of one slab, hub i's at xoff[i])
+  const uint64_t nhub = lv.hubs ? lv.hubs->start[5] - lv.hubs->start[4] : 0;
+  if (c->tier_order == 2 && c->sweep_hubs && nhub && pass >= 0) {
+    uint64_t* caps = c->ws<uint64_t>("x.caps", nhub + 1);
+    uint64_t* xoff = c->ws<uint64_t>("x.off", nhub + 1);
+    k_list_caps(lv.hubs->ids + lv.hubs->start[4], nhub, lv.g.deg, caps, c->st);
+    scan64(c, caps, xoff, nhub + 1);
+    const uint64_t slots = std::max<uint64_t>(lv.hub_slots, 1);
+    a.xids = lv.hubs->ids + lv.hubs->start[4];
+    a.xoff = xoff;
+    a.nx = nhub;
+    a.xkey = c->ws<uint32_t>("x.key", slots);
+    a.xpos = c->ws<uint32_t>("x.pos", slots);
+    a.xval = c->ws<double>("x.val", slots);
+    a.hub_min = 0xFFFFFFFFu;
+  }
+  // spread chunks cut by work (lv_run sweeps; the batched form keeps its
+  // per-graph chunk table): a table per stream count, made when first used
+  const bool balance = !bt && c->tier_order == 2 && c->balance && c->slabs == 1;
+  uint64_t bal_streams = 0;
+  auto balanced = [&](uint64_t streams) {
+    if (bal_streams != streams) {
+      uint64_t* w = c->ws<uint64_t>("bal.w", lv.g.S + 1);
+      uint64_t* pre = c->ws<uint64_t>("bal.pre", lv.g.S + 1);
+      a.ctab = c->ws<uint64_t>("bal.ctab", 2 * streams);
+      k_visit_work(lv.g.deg, lv.g.S, w, c->st);
+      scan64(c, w, pre, lv.g.S + 1);
+      k_balanced_chunks(pre, lv.g.S, streams, a.ctab, c->st);
+      bal_streams = streams;
+    }
+    a.ctab = c->ws<uint64_t>("bal.ctab", 2 * streams);
+    a.nct = streams;
+  };
+  // per-pass sweep schedules (id-order sweeps of lv_run only): regions for
+  // the most streams any iteration of this pass runs
+  const uint64_t s0 = a.max_streams, ch0 = a.chunk;
+  const bool by_pass = !bt && c->tier_order == 2 && pass >= 0;
+  const bool lead = by_pass && pass == 0 && c->lead_iters > 0;
+  const bool late = by_pass && pass > 0 && c->late_streams > 0;
+  if (lead || late) {
+    a.max_streams = std::max(s0, lead ? c->lead_streams : c->late_streams);
+    id_order_regions(c, a, lv);
+  }
   unsigned long long* cnt0 = a.cnt;
   int l = 0;
   double el = 0;
@@ -771,7 +853,24 @@ int local_move(lv_context* c, const Level& lv, uint32_t* vcom, const double* vto
     if (bt) k_fill_f64(bt->gel, 0, bt->G, c->st);
     else clear_el(c, a.el);
     a.pick_less = l < c->pick_less_iters;
-    move_sweep(c, a, lv);
+    a.validate = c->validate >= 1;
+    uint64_t core = 0;
+    if (lead || late) {
+      const bool lead_now = lead && l < c->lead_iters;
+      if (lead_now && c->lead_ids) {
+        core = c->lead_ids;  // the lead schedule over the first ids only
+        a.max_streams = s0;
+        a.chunk = ch0;
+      } else {
+        a.max_streams = lead_now ? c->lead_streams : late ? c->late_streams : s0;
+        a.chunk = lead_now ? c->lead_chunk : late ? c->late_chunk : ch0;
+      }
+    }
+    if (balance) {
+      if (!a.chunk && !core) balanced(a.max_streams);
+      else a.ctab = nullptr;
+    }
+    move_sweep(c, a, lv, core, c->lead_streams, c->lead_chunk);
     c->check();
     ++l;
     if (bt) {
@@ -946,10 +1045,16 @@ int local_move_worklist(lv_context* c, const Level& lv, uint32_t* vcom, const do
     }
     a.serial = c->serial;
     a.pick_less = l < c->pick_less_iters;
+    a.validate = c->validate >= 2;
     if (bt) k_fill_f64(bt->gel, 0, bt->G, c->st);
     else clear_el(c, a.el);
     // the iteration's queue: every flagged id, ascending
     k_wl_queue(vaff, false, S, lv.g.deg, sp, qs, lens, bsum, c->st);
+    if (c->trace) {
+      uint32_t t[kWlQueues];
+      c->read(t, lens, kWlQueues);
+      std::fprintf(stderr, "[lv trace] iteration %d queues %u %u %u %u %u\n", l, t[0], t[1], t[2], t[3], t[4]);
+    }
     if (graphs) k_fill_u32(serial_dev, c->serial, 1, c->st);
     // what the captured launches depend on (the serial is read on the device)
     int rounds = kFirstGroup;  // sub-rounds of the next group
@@ -972,10 +1077,18 @@ int local_move_worklist(lv_context* c, const Level& lv, uint32_t* vcom, const do
     };
     for (;; rounds = kSubRounds) {
       auto group = [&] {
-        for (int r = 0; r < rounds; ++r) sub_round();
+        for (int r = 0; r < rounds; ++r) {
+          sub_round();
+          if (c->trace) {
+            uint32_t t[kWlQueues];
+            c->read(t, lens, kWlQueues);
+            std::fprintf(stderr, "[lv trace] iteration %d sub-round: next queues %u %u %u %u %u\n", l, t[0], t[1],
+                         t[2], t[3], t[4]);
+          }
+        }
       };
       const int shape = rounds * 4 + (scan_hubs ? 2 : 0) + (a.ht_cap ? 1 : 0);
-      if (!graphs || !c->replay(group_key(), shape, group)) group();
+      if (!graphs || c->trace || !c->replay(group_key(), shape, group)) group();
       c->check();
       uint32_t nq[8];
       c->read(nq, lens, sp.el ? 8 : sp.ndrop ? kWlQueues + 1 : kWlQueues);
@@ -1225,8 +1338,14 @@ void invoke(lv_context* c, int approach, const lv_dgraph* x, const lv_dbatch* b,
   // the batch's edges, or built on the second run of a state
   lv_dstate* ht = nullptr;
   if (worklist && c->tier_order == 2 && c->use_tables && o->repeat == 1 && approach != LV_STATIC) {
-    if (s->ht_valid && s->ht_live + b->ni - b->nd == x->live) {
-      k_ht_batch(b->du, b->dv, b->dw, b->nd, b->iu, b->iv, b->iw, b->ni, ucom, s->ht_off, s->ht_cap, s->ht_key,
+    // the tables describe graph ht_stamp: carried forward when this run's
+    // graph is that graph (empty batch) or that graph with this batch applied
+    // by lv_dgraph_update; anything else rebuilds them
+    const bool same = !b->nd && !b->ni && s->ht_stamp == x->stamp;
+    const bool chained = b->from_stamp && b->from_stamp == s->ht_stamp && b->to_stamp == x->stamp;
+    if (s->ht_valid && (same || chained)) {
+      // deletions leave the tables with the weight the edge had in the graph
+      k_ht_batch(b->du, b->dv, b->dws, b->nd, b->iu, b->iv, b->iw, b->ni, ucom, s->ht_off, s->ht_cap, s->ht_key,
                  s->ht_cnt, s->ht_w, s->ht_next + 1, c->st);
       // vertices that passed kHtMinDeg edges with this batch: tables from
       // the spare slots, built from this graph (after the batch deltas)
@@ -1283,9 +1402,10 @@ void invoke(lv_context* c, int approach, const lv_dgraph* x, const lv_dbatch* b,
     if (first) {
       if (worklist) m = local_move_worklist(c, cur, ucom, utot, ctot, vaff, fa, M, R, L, E, nullptr, ht);
       else m = local_move(c, cur, ucom, utot, ctot, vaff, fa, M, R, L, E,
-                          approach == LV_DYNAMIC_FRONTIER || approach == LV_DELTA_SCREENING);
+                          approach == LV_DYNAMIC_FRONTIER || approach == LV_DELTA_SCREENING, nullptr,
+                          approach == LV_STATIC ? 0 : -1);
     } else {
-      m = local_move(c, cur, vcom, vtot, ctot, vaff, nullptr, M, R, L, E);
+      m = local_move(c, cur, vcom, vtot, ctot, vaff, nullptr, M, R, L, E, false, nullptr, p);
     }
     lm.push_back({a0, c->record()});
     l += std::max(m, 1);
@@ -1314,7 +1434,7 @@ void invoke(lv_context* c, int approach, const lv_dgraph* x, const lv_dbatch* b,
     if (c->tier_order == 2) {
       build_tiers(c, "y.hubs", y.off, y.deg, nullptr, CN, yhubs, nullptr, kHubMin);
       yhubs.maxdeg = max_degree_tier4(c, yhubs, y.deg);
-      yslots = hub_slot_total(c, yhubs, y.deg);
+      yslots = caps_total(c, y.deg, CN, kHubMin);
     } else {
       build_tiers(c, "y.tiers", y.off, y.deg, nullptr, CN, ytiers);
       ytiers.maxdeg = max_degree_tier4(c, ytiers, y.deg);
@@ -1326,7 +1446,7 @@ void invoke(lv_context* c, int approach, const lv_dgraph* x, const lv_dbatch* b,
   // the tables stay valid if the membership kept its ids (one pass)
   if (ht && p == 1) {
     s->ht_valid = true;
-    s->ht_live = x->live;
+    s->ht_stamp = x->stamp;
   }
   ++s->runs;
   hipEvent_t e3 = c->record();
@@ -1355,6 +1475,7 @@ void run(lv_context* c, int approach, const lv_dgraph* x, const lv_dbatch* b, lv
   require(approach >= LV_STATIC && approach <= LV_DYNAMIC_FRONTIER, LV_ERR_INVALID, "unknown approach");
   require(approach == LV_STATIC || b != nullptr, LV_ERR_INVALID, "dynamic approaches need a batch");
   require(s->S == x->S, LV_ERR_INVALID, "state span differs from graph span");
+  require(!x->broken, LV_ERR_INVALID, "graph left inconsistent by a failed update");
   validate_options(o);
   HIPCHK(hipSetDevice(c->device));
   c->ev_used = 0;
@@ -1630,7 +1751,7 @@ void invoke_batched(lv_context* c, int approach, const lv_dgraph* x, const lv_db
     if (c->tier_order == 2) {
       build_tiers(c, "y.hubs", y.off, y.deg, nullptr, CN, yhubs, nullptr, kHubMin);
       yhubs.maxdeg = max_degree_tier4(c, yhubs, y.deg);
-      yslots = hub_slot_total(c, yhubs, y.deg);
+      yslots = caps_total(c, y.deg, CN, kHubMin);
     } else {
       build_tiers(c, "y.tiers", y.off, y.deg, nullptr, CN, ytiers);
       ytiers.maxdeg = max_degree_tier4(c, ytiers, y.deg);
@@ -1666,6 +1787,7 @@ void run_batched(lv_context* c, int approach, const lv_dgraph* x, const lv_dbatc
   require(approach == LV_STATIC || b != nullptr, LV_ERR_INVALID, "dynamic approaches need a batch");
   require(s->S == x->S, LV_ERR_INVALID, "state span differs from graph span");
   require(voff[0] == 0 && voff[G] == x->S, LV_ERR_INVALID, "graph ranges must cover the span from 0");
+  require(!x->broken, LV_ERR_INVALID, "graph left inconsistent by a failed update");
   for (uint64_t k = 0; k < G; ++k) require(voff[k] <= voff[k + 1], LV_ERR_INVALID, "graph ranges not ascending");
   validate_options(o);
   require(o->repeat == 1, LV_ERR_INVALID, "the batched form runs one repeat");
@@ -1797,7 +1919,7 @@ lv_dgraph* upload_graph(lv_context* c, const lv_graph* g) {
     d->hubs.ids = dalloc<uint32_t>(std::max<uint64_t>(d->S, 1));
     build_tiers(c, "g.hubs", d->off, d->deg, d->ex, d->S, d->hubs, d->hubs.ids, kHubMin);
     d->hubs.maxdeg = max_degree_tier4(c, d->hubs, d->deg);
-    d->hub_slots = hub_slot_total(c, d->hubs, d->deg);
+    d->hub_slots = caps_total(c, d->deg, d->S, kHubMin);
     d->slab_slots = caps_total(c, d->deg, d->S, kMidSlots / 2);
     c->check();
   } catch (...) {
@@ -1927,108 +2049,140 @@ uint64_t tidy_list(lv_context* c, const lv_dgraph* d, const lv_batch* b, bool ke
   return m;
 }
 
-// applyBatchUpdateU (inc/batch.hxx:225-236) of a tidy device batch on d.
-void apply_update(lv_context* c, lv_dgraph* d, const lv_dbatch* b) {
+// Scratch of build_tiers for n ids, taken ahead (growth can fail).
+void reserve_tiers(lv_context* c, const std::string& name, uint64_t n) {
+  const uint32_t nb = (uint32_t)std::max<uint64_t>(1, (n + 255) / 256);
+  c->ws<uint32_t>(name + ".cnt", (size_t)kTiers * nb + 1);
+  c->ws<uint32_t>(name + ".base", (size_t)kTiers * nb + 1);
+}
+
+// applyBatchUpdateU (inc/batch.hxx:225-236) of a tidy device batch on d;
+// delw receives the weight every deletion had in the graph.  Every scratch
+// buffer is taken before the graph changes, so a failure up to that point
+// leaves the graph as it was; one after it (a HIP error) marks the graph
+// unusable.  Either way the context's zero-between-calls scratch (per-row
+// degree changes, touched flags) is cleared before the error goes on, and
+// the graph's version and stamp move only when the update completes.
+void apply_update(lv_context* c, lv_dgraph* d, const lv_dbatch* b, float* delw) {
   const uint64_t S = d->S, fb = (S + 7) & ~3ull;
   if (!b->nd && !b->ni) return;
-  ++d->version;
   bool fresh = false;
   int* delta = c->ws<int>("up.delta", S, &fresh);  // zero between calls
   if (fresh) k_fill_u32(reinterpret_cast<uint32_t*>(delta), 0, c->ws_bytes("up.delta") / 4, c->st);
   uint8_t* touched = c->ws<uint8_t>("up.touched", fb, &fresh);  // zero between calls
   if (fresh) k_fill_u8(touched, 0, c->ws_bytes("up.touched"), c->st);
-  k_upd_delta(b->du, b->nd, -1, delta, touched, c->st);
-  k_upd_delta(b->iu, b->ni, +1, delta, touched, c->st);
-  // touched rows, ascending
-  uint32_t* T = c->ws<uint32_t>("up.T", S);
-  uint32_t* lens = c->ws<uint32_t>("up.lens", kWlQueues);
-  uint64_t* bsum = c->ws<uint64_t>("up.bsum", (size_t)kWlQueues * kWlScanBlocks);
-  WlQueues qs{};
-  qs.q[0] = T;
-  k_wl_queue(touched, false, S, d->deg, WlSplit{}, qs, lens, bsum, c->st);  // no degree split
-  const uint64_t R = c->read1(lens);
-  uint64_t* so = c->ws<uint64_t>("up.so", R + 1);
-  uint64_t* sn = c->ws<uint64_t>("up.sn", R + 1);
-  uint64_t* bo = c->ws<uint64_t>("up.bo", R + 1);
-  uint64_t* bn = c->ws<uint64_t>("up.bn", R + 1);
-  int* bad = c->ws<int>("up.bad", 1);
-  k_fill_u32(reinterpret_cast<uint32_t*>(bad), 0, 1, c->st);
-  k_upd_sizes(T, R, d->off, d->deg, delta, so, sn, bad, c->st);
-  scan64(c, so, bo, R + 1);
-  scan64(c, sn, bn, R + 1);
-  const uint64_t total_old = c->read1(bo + R), total_new = c->read1(bn + R);
-  const bool overflow = (c->read1(bad) & 2) != 0;
-  // survivors of the touched rows and their ranks within each row
-  uint32_t* keep = c->ws<uint32_t>("up.keep", total_old + 1);
-  uint32_t* rank = c->ws<uint32_t>("up.rank", total_old + 1);
-  k_fill_u32(keep, 1, total_old, c->st);
-  k_fill_u32(keep + total_old, 0, 1, c->st);
-  float* delw = c->ws<float>("up.delw", std::max<uint64_t>(b->nd, 1));
-  k_upd_tomb(b->du, b->dv, b->nd, T, R, bo, d->off, d->deg, d->keys, static_cast<const float*>(d->w), keep, delw,
-             c->st);
-  scan32(c, keep, rank, total_old + 1);
-  // merged rows in scratch, then into the graph
-  uint32_t* yk = c->ws<uint32_t>("up.yk", std::max<uint64_t>(total_new, 1));
-  float* yw = c->ws<float>("up.yw", std::max<uint64_t>(total_new, 1));
-  k_upd_scatter(T, R, bo, bn, total_old, d->off, d->deg, d->keys, static_cast<const float*>(d->w), keep, rank, b->iu,
-                b->iv, b->iw, b->ni, yk, yw, c->st);
-  if (!overflow) {
-    k_upd_copy_back(T, R, bn, total_new, d->off, yk, yw, d->keys, static_cast<float*>(d->w), c->st);
-  } else {
-    // every row to a new CSR with slack: capacity = degree + max(4, degree / 8)
-    uint64_t* cap = c->ws<uint64_t>("up.cap", S + 1);
-    k_upd_caps(d->deg, delta, S, cap, c->st);
-    uint64_t* noff = dalloc<uint64_t>(S + 1);
+  bool mutating = false;
+  try {
+    k_upd_delta(b->du, b->nd, -1, delta, touched, c->st);
+    k_upd_delta(b->iu, b->ni, +1, delta, touched, c->st);
+    // touched rows, ascending
+    uint32_t* T = c->ws<uint32_t>("up.T", S);
+    uint32_t* lens = c->ws<uint32_t>("up.lens", kWlQueues);
+    uint64_t* bsum = c->ws<uint64_t>("up.bsum", (size_t)kWlQueues * kWlScanBlocks);
+    WlQueues qs{};
+    qs.q[0] = T;
+    k_wl_queue(touched, false, S, d->deg, WlSplit{}, qs, lens, bsum, c->st);  // no degree split
+    const uint64_t R = c->read1(lens);
+    uint64_t* so = c->ws<uint64_t>("up.so", R + 1);
+    uint64_t* sn = c->ws<uint64_t>("up.sn", R + 1);
+    uint64_t* bo = c->ws<uint64_t>("up.bo", R + 1);
+    uint64_t* bn = c->ws<uint64_t>("up.bn", R + 1);
+    int* bad = c->ws<int>("up.bad", 1);
+    k_fill_u32(reinterpret_cast<uint32_t*>(bad), 0, 1, c->st);
+    k_upd_sizes(T, R, d->off, d->deg, delta, so, sn, bad, c->st);
+    scan64(c, so, bo, R + 1);
+    scan64(c, sn, bn, R + 1);
+    const uint64_t total_old = c->read1(bo + R), total_new = c->read1(bn + R);
+    const bool overflow = (c->read1(bad) & 2) != 0;
+    // survivors of the touched rows and their ranks within each row
+    uint32_t* keep = c->ws<uint32_t>("up.keep", total_old + 1);
+    uint32_t* rank = c->ws<uint32_t>("up.rank", total_old + 1);
+    k_fill_u32(keep, 1, total_old, c->st);
+    k_fill_u32(keep + total_old, 0, 1, c->st);
+    k_upd_tomb(b->du, b->dv, b->nd, T, R, bo, d->off, d->deg, d->keys, static_cast<const float*>(d->w), keep, delw,
+               c->st);
+    scan32(c, keep, rank, total_old + 1);
+    // merged rows in scratch, then into the graph
+    uint32_t* yk = c->ws<uint32_t>("up.yk", std::max<uint64_t>(total_new, 1));
+    float* yw = c->ws<float>("up.yw", std::max<uint64_t>(total_new, 1));
+    k_upd_scatter(T, R, bo, bn, total_old, d->off, d->deg, d->keys, static_cast<const float*>(d->w), keep, rank,
+                  b->iu, b->iv, b->iw, b->ni, yk, yw, c->st);
+    // what the steps after the first change need, taken now
+    unsigned long long* order = c->ws<unsigned long long>("up.order", 1);
+    double* m2 = c->ws<double>("up.m2", 1);
+    reserve_tiers(c, "g.tiers", S);
+    reserve_tiers(c, "g.hubs", S);
+    c->ws<unsigned long long>("hb.need", 1);
+    c->ws<uint32_t>("maxdeg", 1);
+    uint64_t* noff = nullptr;
     uint32_t* nkeys = nullptr;
     float* nw = nullptr;
-    try {
-      scan64(c, cap, noff, S + 1);
-      const uint64_t E2 = c->read1(noff + S);
-      nkeys = dalloc<uint32_t>(E2);
-      nw = dalloc<float>(E2);
-      k_upd_copy_rows(S, d->off, d->deg, d->keys, static_cast<const float*>(d->w), touched, noff, nkeys, nw, c->st);
-      k_upd_copy_back(T, R, bn, total_new, noff, yk, yw, nkeys, nw, c->st);
-      c->check();
-      HIPCHK(hipStreamSynchronize(c->st));
-      (void)hipFree(d->off);
-      (void)hipFree(d->keys);
-      (void)hipFree(d->w);
-      d->off = noff;
-      d->keys = nkeys;
-      d->w = nw;
-      d->E = E2;
-    } catch (...) {
-      (void)hipFree(noff);
-      (void)hipFree(nkeys);
-      (void)hipFree(nw);
-      throw;
+    if (overflow) {
+      // every row to a new CSR with slack: capacity = degree + max(4, degree / 8)
+      uint64_t* cap = c->ws<uint64_t>("up.cap", S + 1);
+      k_upd_caps(d->deg, delta, S, cap, c->st);
+      try {
+        noff = dalloc<uint64_t>(S + 1);
+        scan64(c, cap, noff, S + 1);
+        const uint64_t E2 = c->read1(noff + S);
+        nkeys = dalloc<uint32_t>(E2);
+        nw = dalloc<float>(E2);
+        k_upd_copy_rows(S, d->off, d->deg, d->keys, static_cast<const float*>(d->w), touched, noff, nkeys, nw,
+                        c->st);
+        k_upd_copy_back(T, R, bn, total_new, noff, yk, yw, nkeys, nw, c->st);
+        c->check();
+        HIPCHK(hipStreamSynchronize(c->st));
+        mutating = true;
+        (void)hipFree(d->off);
+        (void)hipFree(d->keys);
+        (void)hipFree(d->w);
+        d->off = noff;
+        d->keys = nkeys;
+        d->w = nw;
+        d->E = E2;
+      } catch (...) {
+        if (!mutating) {
+          (void)hipFree(noff);
+          (void)hipFree(nkeys);
+          (void)hipFree(nw);
+        }
+        throw;
+      }
+    } else {
+      mutating = true;
+      k_upd_copy_back(T, R, bn, total_new, d->off, yk, yw, d->keys, static_cast<float*>(d->w), c->st);
     }
+    k_upd_degrees(T, R, delta, d->deg, touched, c->st);  // (clears delta and touched of the rows)
+    if (d->ex) {
+      k_upd_exists(b->iu, b->iv, b->ni, d->ex, c->st);
+      k_fill_u64(reinterpret_cast<uint64_t*>(order), 0, 1, c->st);
+      k_count_u8(d->ex, 1, S, order, c->st);
+      d->order = c->read1(order);
+    }
+    // 2M after the batch: the old total minus the deleted edges' weights plus
+    // the insertions'
+    k_fill_f64(m2, d->M2, 1, c->st);
+    k_upd_wsum(b->iw, b->ni, 1.0, m2, c->st);
+    k_upd_wsum(delw, b->nd, -1.0, m2, c->st);
+    d->M2 = c->read1(m2);
+    d->live = d->live + b->ni - b->nd;
+    // degree lists of the new degrees
+    build_tiers(c, "g.tiers", d->off, d->deg, d->ex, S, d->tiers, d->tiers.ids);
+    d->tiers.maxdeg = max_degree_tier4(c, d->tiers, d->deg);
+    build_tiers(c, "g.hubs", d->off, d->deg, d->ex, S, d->hubs, d->hubs.ids, kHubMin);
+    d->hubs.maxdeg = max_degree_tier4(c, d->hubs, d->deg);
+    d->hub_slots = caps_total(c, d->deg, S, kHubMin);
+    d->slab_slots = caps_total(c, d->deg, S, kMidSlots / 2);
+    c->check();
+  } catch (...) {
+    k_fill_u32(reinterpret_cast<uint32_t*>(delta), 0, c->ws_bytes("up.delta") / 4, c->st);
+    k_fill_u8(touched, 0, c->ws_bytes("up.touched"), c->st);
+    (void)hipStreamSynchronize(c->st);
+    if (mutating) d->broken = true;
+    throw;
   }
-  k_upd_degrees(T, R, delta, d->deg, touched, c->st);
-  if (d->ex) {
-    k_upd_exists(b->iu, b->iv, b->ni, d->ex, c->st);
-    unsigned long long* n = c->ws<unsigned long long>("up.order", 1);
-    k_fill_u64(reinterpret_cast<uint64_t*>(n), 0, 1, c->st);
-    k_count_u8(d->ex, 1, S, n, c->st);
-    d->order = c->read1(n);
-  }
-  // 2M after the batch: the old total minus the deleted edges' weights plus
-  // the insertions'
-  double* m2 = c->ws<double>("up.m2", 1);
-  k_fill_f64(m2, d->M2, 1, c->st);
-  k_upd_wsum(b->iw, b->ni, 1.0, m2, c->st);
-  k_upd_wsum(delw, b->nd, -1.0, m2, c->st);
-  d->M2 = c->read1(m2);
-  d->live = d->live + b->ni - b->nd;
-  // degree lists of the new degrees
-  build_tiers(c, "g.tiers", d->off, d->deg, d->ex, S, d->tiers, d->tiers.ids);
-  d->tiers.maxdeg = max_degree_tier4(c, d->tiers, d->deg);
-  build_tiers(c, "g.hubs", d->off, d->deg, d->ex, S, d->hubs, d->hubs.ids, kHubMin);
-  d->hubs.maxdeg = max_degree_tier4(c, d->hubs, d->deg);
-  d->hub_slots = hub_slot_total(c, d->hubs, d->deg);
-  d->slab_slots = caps_total(c, d->deg, d->S, kMidSlots / 2);
+  ++d->version;
   d->stamp = new_graph_stamp();  // worklist hub-table sizes depend on the degrees
-  c->check();
 }
 
 template <class F>
@@ -2120,6 +2274,7 @@ int lv_context_create(int device, lv_context** out) {
   if (device < 0 || device >= n) return LV_ERR_INVALID;
   auto* c = new lv_context();
   c->device = device;
+  c->trace = std::getenv("LV_TRACE") != nullptr;
   int rc = guard(c, [&] {
     HIPCHK(hipSetDevice(device));
     HIPCHK(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
@@ -2179,6 +2334,45 @@ int lv_context_set_hub_threshold(lv_context* c, uint32_t sweep_degree, uint32_t 
     return LV_ERR_INVALID;
   c->hub_min = sweep_degree;
   c->hub_min_wl = worklist_degree;
+  return LV_OK;
+}
+
+int lv_context_set_workspace_limit(lv_context* c, uint64_t bytes) {
+  if (!c) return LV_ERR_INVALID;
+  c->ws_limit = bytes ? (size_t)bytes : ~size_t(0);
+  return LV_OK;
+}
+
+uint64_t lv_context_workspace_bytes(const lv_context* c) { return c ? (uint64_t)c->ws_total : 0; }
+
+int lv_context_set_pass_schedule(lv_context* c, int lead_iterations, uint64_t lead_streams, uint64_t lead_chunk,
+                                 uint64_t late_streams, uint64_t late_chunk) {
+  if (!c || lead_iterations < 0 || (lead_iterations && (!lead_streams || !lead_chunk))) return LV_ERR_INVALID;
+  c->lead_ids = 0;
+  c->lead_iters = lead_iterations;
+  c->lead_streams = lead_streams;
+  c->lead_chunk = lead_chunk;
+  c->late_streams = late_streams;
+  c->late_chunk = late_chunk;
+  return LV_OK;
+}
+
+int lv_context_set_lead_ids(lv_context* c, uint64_t ids) {
+  if (!c) return LV_ERR_INVALID;
+  c->lead_ids = ids;
+  return LV_OK;
+}
+
+int lv_context_set_sweep_hubs(lv_context* c, int in_sweep, int balance) {
+  if (!c) return LV_ERR_INVALID;
+  c->sweep_hubs = in_sweep ? 1 : 0;
+  c->balance = balance ? 1 : 0;
+  return LV_OK;
+}
+
+int lv_context_set_validate(lv_context* c, int mode) {
+  if (!c || mode < 0 || mode > 2) return LV_ERR_INVALID;
+  c->validate = mode;
   return LV_OK;
 }
 
@@ -2320,11 +2514,15 @@ int lv_dgraph_update(lv_context* c, lv_dgraph* g, const lv_batch* deletions, con
   return guard(c, [&] {
     require(!g->f64, LV_ERR_INVALID, "only input graphs (f32 weights) can be updated");
     HIPCHK(hipSetDevice(c->device));
+    require(!g->broken, LV_ERR_INVALID, "graph left inconsistent by a failed update");
     auto* b = new lv_dbatch();
     try {
       b->nd = tidy_list(c, g, deletions, true, b->du, b->dv, b->dw, "up.del");
       b->ni = tidy_list(c, g, insertions, false, b->iu, b->iv, b->iw, "up.ins");
-      apply_update(c, g, b);
+      b->dws = dalloc<float>(b->nd);
+      b->from_stamp = g->stamp;
+      apply_update(c, g, b, b->dws);
+      b->to_stamp = g->stamp;
       // tidy by construction: sorted, unique, ids of g
       b->checked = true;
       b->id_bound = g->S;
@@ -2342,6 +2540,7 @@ int lv_dgraph_download(lv_context* c, const lv_dgraph* g, uint64_t* offsets, uin
   if (!c || !g || !offsets || !degrees || !keys || !weights) return LV_ERR_INVALID;
   return guard(c, [&] {
     require(!g->f64, LV_ERR_INVALID, "graph has f64 weights");
+    require(!g->broken, LV_ERR_INVALID, "graph left inconsistent by a failed update");
     d2h(c, offsets, g->off, g->S + 1);
     d2h(c, degrees, g->deg, g->S);
     d2h(c, keys, g->keys, g->E);
@@ -2387,6 +2586,7 @@ void lv_dbatch_destroy(lv_dbatch* b) {
   (void)hipFree(b->iu);
   (void)hipFree(b->iv);
   (void)hipFree(b->iw);
+  (void)hipFree(b->dws);
   delete b;
 }
 
@@ -2462,6 +2662,7 @@ int lv_run_batched(lv_context* c, int approach, const lv_dgraph* g, const lv_dba
 int lv_dmodularity(lv_context* c, const lv_dgraph* g, const lv_dstate* s, double R, double* out) {
   if (!c || !g || !s || !out) return LV_ERR_INVALID;
   return guard(c, [&] {
+    require(!g->broken, LV_ERR_INVALID, "graph left inconsistent by a failed update");
     double* cin = c->ws<double>("mod.cin", g->S);
     double* ct = c->ws<double>("mod.ctot", g->S);
     double* o = c->ws<double>("mod.out", 1);

@@ -180,6 +180,8 @@ struct MoveArgs {
   uint32_t serial;
   const uint32_t* serial_dev;  // if set, the serial is read from here on the device
   int pick_less;               // 1: only moves to a lower community id
+  int validate;                // id-order kernel: a vertex about to move re-reads its neighbours'
+                               //   communities and decides again if one changed meanwhile
   uint64_t min_chunk;          // unused (kept zero)
   uint64_t max_streams;        // lane groups launched (streams walking at once)
   uint64_t chunk;              // list entries per dynamically assigned chunk
@@ -236,6 +238,16 @@ struct MoveArgs {
   uint32_t* hb_deg;
   uint32_t* hb_cown;           // list entry owning each edge chunk
   uint32_t* hb_town;           // list entry owning each kHubTabChunk-slot table chunk
+  // id-order sweeps with the hubs in the sweep (xids set): a vertex of more
+  // than kHubMin edges is visited in its turn like any other, with its own
+  // HBM table: hub i of the ascending list xids (nx entries) owns slots
+  // [xoff[i], xoff[i + 1]) of xkey / xpos / xval, nextpow2(2 deg) of them
+  const uint32_t* xids;
+  const uint64_t* xoff;
+  uint64_t nx;
+  uint32_t* xkey;
+  uint32_t* xpos;
+  double* xval;
 };
 
 // Hub batch: every listed vertex (hub) at once, each with its own HBM table
@@ -273,6 +285,16 @@ void k_ht_batch(const uint32_t* du, const uint32_t* dv, const float* dw, uint64_
 // Worklist hubs from a.list (length *a.n_dev), one block each with an LDS
 // table: class 0 up to kSmallHubMax edges, class 1 up to kMidHubMax.
 void k_hub_mid(const MoveArgs& a, int cls, hipStream_t st);
+
+// Sweep chunks balanced by work: `streams` contiguous ranges of the ids
+// [0, S), each with about the same sum of (degree + kVisitWeight); chunk k is
+// ids [ctab[2k], ctab[2k + 1]).  k_visit_work writes each id's work (S + 1
+// entries, the last 0); k_balanced_chunks cuts its exclusive prefix pre.
+constexpr uint64_t kVisitWeight = 8;
+void k_visit_work(const uint32_t* deg, uint64_t S, uint64_t* w, hipStream_t st);
+void k_balanced_chunks(const uint64_t* pre, uint64_t S, uint64_t streams, uint64_t* ctab, hipStream_t st);
+// caps[i] = nextpow2(2 deg[ids[i]]) for i < n, caps[n] = 0.
+void k_list_caps(const uint32_t* ids, uint64_t n, const uint32_t* deg, uint64_t* caps, hipStream_t st);
 
 // Kernel launchers (lv_kernels.hip).  All run on `st`.
 void k_fill_u8(uint8_t* a, uint8_t v, uint64_t n, hipStream_t st);

@@ -665,9 +665,15 @@ __device__ __forceinline__ void table_clear(uint32_t* key, uint32_t* pos, double
 
 constexpr int kScanRounds = 4;
 
+// Order-free digest of the neighbour communities one lane read (its edges
+// of the row, in the same lane-to-edge layout every time): a re-read whose
+// digest differs saw a neighbour move.
+__device__ __forceinline__ uint64_t comm_digest(uint32_t c) { return ((uint64_t)c + 1) * 0x9E3779B97F4A7C15ull; }
+
 template <int G, class WT>
 __device__ __forceinline__ void table_scan(const MoveArgs& a, const WT* W, uint32_t u, uint64_t o, uint32_t d,
-                                           uint32_t* key, uint32_t* pos, double* val, uint64_t cap, int lane) {
+                                           uint32_t* key, uint32_t* pos, double* val, uint64_t cap, int lane,
+                                           uint64_t& dig) {
   // kScanRounds rounds of G edges at a time: their key and weight loads,
   // then their community gathers, then the inserts in round order (the
   // order of a plain per-round loop, so the sums are the same)
@@ -684,9 +690,30 @@ __device__ __forceinline__ void table_scan(const MoveArgs& a, const WT* W, uint3
 #pragma unroll
     for (int r = 0; r < R; ++r) c[r] = v[r] != u ? ld_dev(&a.vcom[v[r]]) : 0;
 #pragma unroll
-    for (int r = 0; r < R; ++r)  // self-loops are skipped (louvainScanCommunityW<SELF=false>)
+    for (int r = 0; r < R; ++r) {  // self-loops are skipped (louvainScanCommunityW<SELF=false>)
       if (v[r] != u) tab_add(key, pos, val, cap, c[r], w[r], b + r * G + lane);
+      dig += comm_digest(c[r]);
+    }
   }
+}
+
+// The digest of the row's neighbour communities read again (table_scan's
+// lane-to-edge layout, no table).
+template <int G>
+__device__ __forceinline__ uint64_t row_digest(const MoveArgs& a, uint32_t u, uint64_t o, uint32_t d, int lane) {
+  constexpr int R = kScanRounds;
+  uint64_t dig = 0;
+  for (uint32_t b = 0; b < d; b += R * G) {
+    uint32_t v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const uint32_t i = b + r * G + lane;
+      v[r] = i < d ? a.g.keys[o + i] : u;
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) dig += comm_digest(v[r] != u ? ld_dev(&a.vcom[v[r]]) : 0);
+  }
+  return dig;
 }
 
 template <int G>
@@ -804,6 +831,8 @@ __global__ __launch_bounds__(BLOCK) void move_kernel(MoveArgs a) {
       if (act && a.stamp && lane == 0) st_dev(&a.stamp[u], cur_serial(a));
       bool hbm = MODE == MOVE_HBM;
       uint64_t cap = CAP;
+      uint32_t *tkey = hkey, *tpos = hpos;
+      double* tval = hval;
       if (act) {
         if constexpr (MODE == MOVE_HYBRID) hbm = 2 * (uint64_t)d > CAP;
         // nextpow2(2 d) slots: of the block's HBM region, or (at least 16)
@@ -811,24 +840,66 @@ __global__ __launch_bounds__(BLOCK) void move_kernel(MoveArgs a) {
         uint64_t want = hbm ? 2 : 16;
         while (want < 2 * (uint64_t)d) want <<= 1;
         cap = hbm ? want : (want < CAP ? want : CAP);
-        if (hbm) table_clear<G>(hkey, hpos, hval, cap, lane);
-        else table_clear<G>(lkey, lpos, lval, cap, lane);
+        if constexpr (MODE == MOVE_HYBRID) {
+          if (a.xids && d > kHubMin) {  // a hub in the sweep: its own table
+            uint64_t lo = 0, hi = a.nx;
+            while (lo < hi) {
+              const uint64_t mid = (lo + hi) / 2;
+              if (a.xids[mid] < u) lo = mid + 1;
+              else hi = mid;
+            }
+            if (lo < a.nx && a.xids[lo] == u) {
+              const uint64_t base = a.xoff[lo];
+              tkey = a.xkey + base;
+              tpos = a.xpos + base;
+              tval = a.xval + base;
+            } else {
+              act = false;  // (every live vertex above kHubMin is listed; never taken)
+            }
+          }
+        }
       }
-      group_sync();
-      if (act) {
-        if (hbm) table_scan<G, WT>(a, W, u, o, d, hkey, hpos, hval, cap, lane);
-        else table_scan<G, WT>(a, W, u, o, d, lkey, lpos, lval, cap, lane);
-      }
-      group_sync();
       double be = 0;
       uint32_t bp = kNoPos, bc = 0;
-      if (act) {
-        const double Mu = vertex_M(a, u);
-        if (hbm) table_best<G>(a, cu, Ku, Mu, hkey, hpos, hval, cap, lane, be, bp, bc, c_cand);
-        else table_best<G>(a, cu, Ku, Mu, lkey, lpos, lval, cap, lane, be, bp, bc, c_cand);
+      // Validated moves (id-order kernel, a.validate): a vertex about to
+      // move reads its neighbours' communities again; if one moved while the
+      // vertex was being decided (a concurrent stream), the decision is
+      // taken again from the new view (at most kRedo times).  One stream
+      // never sees a change, so its visits are the plain ones.
+      constexpr int kRedo = 2;
+      for (int attempt = 0;; ++attempt) {
+        if (act) {
+          if (hbm) table_clear<G>(tkey, tpos, tval, cap, lane);
+          else table_clear<G>(lkey, lpos, lval, cap, lane);
+        }
+        group_sync();
+        uint64_t dig = 0;
+        if (act) {
+          if (hbm) table_scan<G, WT>(a, W, u, o, d, tkey, tpos, tval, cap, lane, dig);
+          else table_scan<G, WT>(a, W, u, o, d, lkey, lpos, lval, cap, lane, dig);
+        }
+        group_sync();
+        be = 0;
+        bp = kNoPos;
+        bc = 0;
+        if (act) {
+          const double Mu = vertex_M(a, u);
+          if (hbm) table_best<G>(a, cu, Ku, Mu, tkey, tpos, tval, cap, lane, be, bp, bc, c_cand);
+          else table_best<G>(a, cu, Ku, Mu, lkey, lpos, lval, cap, lane, be, bp, bc, c_cand);
+        }
+        if constexpr (G <= 64) group_best_shfl<G>(be, bp, bc);
+        else block_best<BLOCK>(be, bp, bc);
+        if constexpr (MODE == MOVE_HYBRID && G == 64) {
+          // (the group is the wave: the redo decision is uniform over it)
+          const bool check = a.validate && act && !a.choose_only && attempt < kRedo && bp != kNoPos &&
+                             bc != vertex_zero(a, u);
+          if (check && __ballot(row_digest<G>(a, u, o, d, lane) != dig)) {
+            group_sync();
+            continue;
+          }
+        }
+        break;
       }
-      if constexpr (G <= 64) group_best_shfl<G>(be, bp, bc);
-      else block_best<BLOCK>(be, bp, bc);
       if (act) {
         if (a.choose_only) {
           if (lane == 0) { a.out_c[u] = bc; a.out_e[u] = be; }
@@ -2510,6 +2581,47 @@ __global__ __launch_bounds__(kBlock) void modularity_sum_kernel(const double* ci
 }  // namespace
 
 // ------------------------------------------------------------ launchers
+// Work of each id for the balanced sweep chunks (S + 1 entries, the last 0).
+__global__ void visit_work_kernel(const uint32_t* deg, uint64_t S, uint64_t* w) {
+  for (uint64_t u = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; u <= S; u += (uint64_t)gridDim.x * blockDim.x)
+    w[u] = u < S ? (uint64_t)deg[u] + kVisitWeight : 0;
+}
+
+// Chunk k of `streams` starts at the first id whose exclusive work prefix
+// reaches k / streams of the total.
+__device__ __forceinline__ uint64_t work_bound(const uint64_t* pre, uint64_t S, uint64_t k, uint64_t streams) {
+  const uint64_t total = pre[S];
+  const uint64_t target = (uint64_t)((unsigned __int128)total * k / streams);
+  uint64_t lo = 0, hi = S;
+  while (lo < hi) {
+    const uint64_t mid = (lo + hi) / 2;
+    if (pre[mid] < target) lo = mid + 1;
+    else hi = mid;
+  }
+  return k >= streams ? S : lo;
+}
+
+__global__ void balanced_chunks_kernel(const uint64_t* pre, uint64_t S, uint64_t streams, uint64_t* ctab) {
+  for (uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; k < streams;
+       k += (uint64_t)gridDim.x * blockDim.x) {
+    ctab[2 * k] = work_bound(pre, S, k, streams);
+    ctab[2 * k + 1] = work_bound(pre, S, k + 1, streams);
+  }
+}
+
+__global__ void list_caps_kernel(const uint32_t* ids, uint64_t n, const uint32_t* deg, uint64_t* caps) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i <= n; i += (uint64_t)gridDim.x * blockDim.x) {
+    if (i == n) {
+      caps[i] = 0;
+      continue;
+    }
+    const uint64_t d = deg[ids[i]];
+    uint64_t c = 2;
+    while (c < 2 * d) c <<= 1;
+    caps[i] = c;
+  }
+}
+
 void k_fill_u8(uint8_t* a, uint8_t v, uint64_t n, hipStream_t st) {
   if (n) LV_LAUNCH(fill_kernel<uint8_t>, grid_for(n, kBlock * 4), kBlock, st, a, v, n);
 }
@@ -2625,6 +2737,15 @@ static void move_tier(const MoveArgs& a, int tier, hipStream_t st) {
         LV_LAUNCH((move_kernel<64, 64, kHybridLds, WT, MOVE_HYBRID>), grid_for(groups, 1, a.gblocks), 64, st, a);
       break;
   }
+}
+void k_visit_work(const uint32_t* deg, uint64_t S, uint64_t* w, hipStream_t st) {
+  LV_LAUNCH(visit_work_kernel, grid_for(S + 1, kBlock * 4), kBlock, st, deg, S, w);
+}
+void k_balanced_chunks(const uint64_t* pre, uint64_t S, uint64_t streams, uint64_t* ctab, hipStream_t st) {
+  if (streams) LV_LAUNCH(balanced_chunks_kernel, grid_for(streams, kBlock), kBlock, st, pre, S, streams, ctab);
+}
+void k_list_caps(const uint32_t* ids, uint64_t n, const uint32_t* deg, uint64_t* caps, hipStream_t st) {
+  LV_LAUNCH(list_caps_kernel, grid_for(n + 1, kBlock), kBlock, st, ids, n, deg, caps);
 }
 void k_move(const MoveArgs& a, int tier, bool f64, hipStream_t st) {
   if (f64) move_tier<double>(a, tier, st);

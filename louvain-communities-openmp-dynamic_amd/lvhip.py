@@ -34,7 +34,7 @@ PATH_SWEEP, PATH_FRONTIER, PATH_WORKLIST = 0, 1, 2
 # Every symbol include/louvain_hip.h declares.
 EXPORTS = [
     "lv_options_default", "lv_status_string", "lv_context_create", "lv_context_destroy", "lv_last_error",
-    "lv_context_set_path", "lv_context_set_schedule", "lv_context_set_pick_less", "lv_context_set_slabs", "lv_context_set_hub_streams", "lv_context_set_hub_threshold",
+    "lv_context_set_path", "lv_context_set_schedule", "lv_context_set_pick_less", "lv_context_set_pass_schedule", "lv_context_set_validate", "lv_context_set_sweep_hubs", "lv_context_set_lead_ids", "lv_context_set_workspace_limit", "lv_context_workspace_bytes", "lv_context_set_slabs", "lv_context_set_hub_streams", "lv_context_set_hub_threshold",
     "lv_context_set_profiling", "lv_context_set_graphs", "lv_context_set_hub_tables",
     "lv_dstate_check_hub_tables", "lv_stats_get",
     "lv_stats_reset", "lv_louvain_static", "lv_louvain_naive_dynamic", "lv_louvain_delta_screening",
@@ -92,6 +92,7 @@ def load_library(path=LIB_PATH):
     lib.lv_dgraph_span.restype = C.c_uint64
     lib.lv_dgraph_edges.restype = C.c_uint64
     lib.lv_dgraph_capacity.restype = C.c_uint64
+    lib.lv_context_workspace_bytes.restype = C.c_uint64
     for name in ("lv_context_destroy", "lv_dgraph_destroy", "lv_dbatch_destroy", "lv_dstate_destroy"):
         getattr(lib, name).restype = None
     return lib
@@ -206,6 +207,29 @@ class Context:
 
     def set_pick_less(self, iterations=0):
         self._check(self.lib.lv_context_set_pick_less(self.h, int(iterations)), "lv_context_set_pick_less")
+
+    def set_lead_ids(self, ids=0):
+        self._check(self.lib.lv_context_set_lead_ids(self.h, C.c_uint64(ids)), "lv_context_set_lead_ids")
+
+    def set_sweep_hubs(self, in_sweep=True, balance=True):
+        self._check(self.lib.lv_context_set_sweep_hubs(self.h, int(bool(in_sweep)), int(bool(balance))),
+                    "lv_context_set_sweep_hubs")
+
+    def set_validate(self, mode=1):
+        """Validated moves: 0 off, 1 sweeps (default), 2 sweeps and worklists."""
+        self._check(self.lib.lv_context_set_validate(self.h, int(mode)), "lv_context_set_validate")
+
+    def set_workspace_limit(self, nbytes=0):
+        self._check(self.lib.lv_context_set_workspace_limit(self.h, C.c_uint64(nbytes)),
+                    "lv_context_set_workspace_limit")
+
+    def workspace_bytes(self):
+        return int(self.lib.lv_context_workspace_bytes(self.h))
+
+    def set_pass_schedule(self, lead_iterations=0, lead_streams=16, lead_chunk=2048, late_streams=0, late_chunk=0):
+        self._check(self.lib.lv_context_set_pass_schedule(self.h, int(lead_iterations), C.c_uint64(lead_streams),
+                                                          C.c_uint64(lead_chunk), C.c_uint64(late_streams),
+                                                          C.c_uint64(late_chunk)), "lv_context_set_pass_schedule")
 
     def set_schedule(self, stream_fraction=0.002, min_streams=1, max_streams=1 << 20, chunk=0, order=2):
         """order: 0 degree tiers lowest first, 1 highest first, 2 ascending ids, no tiers."""

@@ -268,6 +268,35 @@ class RefLib:
             raise RuntimeError(f"ref_louvain failed: {rc}")
         return Result(memb, vt, ct, out)
 
+    def tidy_update(self, g, dels, ins):
+        """tidyBatchUpdateU then applyBatchUpdateOmpU (inc/batch.hxx:202-248):
+        the tidy (deletions, insertions) and the updated graph as a packed CSR
+        dict with its exists mask."""
+        g, a = self._gargs(g)
+        bd, da = self._bargs(dels)
+        bi, ia = self._bargs(ins)
+        nd, ni = len(bd["u"]), len(bi["u"])
+        td = {"u": np.zeros(nd, np.uint32), "v": np.zeros(nd, np.uint32), "w": np.zeros(nd, np.float32)}
+        ti = {"u": np.zeros(ni, np.uint32), "v": np.zeros(ni, np.uint32), "w": np.zeros(ni, np.float32)}
+        tnd, tni = C.c_uint64(0), C.c_uint64(0)
+        S = g["S"]
+        cap = int(g["off"][-1]) + ni
+        yoff = np.zeros(S + 1, np.uint64)
+        yk = np.zeros(max(cap, 1), np.uint32)
+        yw = np.zeros(max(cap, 1), np.float32)
+        yex = np.zeros(S, np.uint8)
+        self.lib.ref_tidy_update.restype = C.c_int64
+        e = self.lib.ref_tidy_update(*a, _p(g["w"]), _p(g["ex"]), *da, *ia,
+                                     _p(td["u"]), _p(td["v"]), _p(td["w"]), C.byref(tnd),
+                                     _p(ti["u"]), _p(ti["v"]), _p(ti["w"]), C.byref(tni),
+                                     _p(yoff), _p(yk), _p(yw), _p(yex), C.c_uint64(cap))
+        if e < 0:
+            raise RuntimeError("ref_tidy_update: edge capacity too small")
+        for d, n in ((td, tnd.value), (ti, tni.value)):
+            for k in d:
+                d[k] = d[k][:n].copy()
+        return td, ti, {"off": yoff, "keys": yk[:e].copy(), "w": yw[:e].copy(), "ex": yex}
+
     def update_weights(self, g, dels, ins, vcom, vtot, ctot):
         g, a = self._gargs(g)
         bd, da = self._bargs(dels)

@@ -261,4 +261,44 @@ void ref_aggregate(uint64_t S, const uint64_t* off, const uint32_t* deg, const u
   for (uint64_t i = 0; i < S; ++i) cedg[i] = cv.edgeKeys[i];
 }
 
+// Batch tidy and update: tidyBatchUpdateU (inc/batch.hxx:202-209) of the raw
+// lists against graph x, then applyBatchUpdateOmpU (inc/batch.hxx:241-248)
+// on a copy of x.  The tidy lists go to tdu/tdv/tdw (room for nd) and
+// tiu/tiv/tiw (room for ni), their sizes to *tnd / *tni; the updated graph to
+// yoff[S+1] / ykeys / yw (room for ycap edges, rows sorted by key) and
+// yex[S].  Ids must be below S.  Returns y's edge count, or -1 if ycap is
+// too small.
+int64_t ref_tidy_update(uint64_t S, const uint64_t* off, const uint32_t* deg, const uint32_t* keys,
+                        const float* w, const uint8_t* ex,
+                        const uint32_t* du, const uint32_t* dv, const float* dw, uint64_t nd,
+                        const uint32_t* iu, const uint32_t* iv, const float* iw, uint64_t ni,
+                        uint32_t* tdu, uint32_t* tdv, float* tdw, uint64_t* tnd,
+                        uint32_t* tiu, uint32_t* tiv, float* tiw, uint64_t* tni,
+                        uint64_t* yoff, uint32_t* ykeys, float* yw, uint8_t* yex, uint64_t ycap) {
+  auto x = build_digraph(S, off, deg, keys, w, ex);
+  auto dels = batch(du, dv, dw, nd);
+  auto inss = batch(iu, iv, iw, ni);
+  tidyBatchUpdateU(dels, inss, x);
+  *tnd = dels.size();
+  *tni = inss.size();
+  for (size_t i = 0; i < dels.size(); ++i) { tdu[i] = get<0>(dels[i]); tdv[i] = get<1>(dels[i]); tdw[i] = get<2>(dels[i]); }
+  for (size_t i = 0; i < inss.size(); ++i) { tiu[i] = get<0>(inss[i]); tiv[i] = get<1>(inss[i]); tiw[i] = get<2>(inss[i]); }
+  auto y = duplicate(x);
+  applyBatchUpdateOmpU(y, dels, inss);
+  uint64_t e = 0;
+  yoff[0] = 0;
+  for (uint64_t u = 0; u < S; ++u) {
+    yex[u] = y.hasVertex(K(u)) ? 1 : 0;
+    bool room = true;
+    if (u < y.span())
+      y.forEachEdge(K(u), [&](K v, V wt) {
+        if (e >= ycap) { room = false; return; }
+        ykeys[e] = v; yw[e] = wt; ++e;
+      });
+    if (!room) return -1;
+    yoff[u + 1] = e;
+  }
+  return int64_t(e);
+}
+
 }  // extern "C"

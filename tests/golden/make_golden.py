@@ -14,7 +14,13 @@ reference's outputs for every function on the hot path:
   the input graph and on its first super-graph, edge weight and modularity
   (properties.hxx:96-106, :178-192).
 
-Usage:  python tests/golden/make_golden.py   (needs /root/reference)
+It also writes update_raw.npz: raw (untidy) batches -- deletions of absent
+edges, insertions of present ones, repeated entries with different weights,
+both directions, shuffled -- on four of the graphs, with the reference's
+tidyBatchUpdateU result (inc/batch.hxx:202-209) and the graph after its
+applyBatchUpdateOmpU (inc/batch.hxx:241-248).
+
+Usage:  python tests/golden/make_golden.py [--update-only]   (needs /root/reference)
 """
 import os
 import sys
@@ -116,6 +122,48 @@ def record(ref, g, dels, ins, opts, prefix, out):
     out[f"{prefix}sg_K"] = ksg
 
 
+def raw_batch(g, rng, n):
+    """An untidy batch on g: present and absent deletions, absent and present
+    insertions, both directions, repeats carrying other weights, shuffled."""
+    u, v, _w = G.edges_of(g)
+    S, E = len(g["off"]) - 1, len(u)
+    pick = rng.integers(0, E, n)
+    du = np.concatenate([u[pick], v[pick], rng.integers(0, S, n // 4), u[pick[: n // 8]]])
+    dv = np.concatenate([v[pick], u[pick], rng.integers(0, S, n // 4), v[pick[: n // 8]]])
+    iu0, iv0 = rng.integers(0, S, n), rng.integers(0, S, n)
+    iu = np.concatenate([iu0, iv0, u[pick[: n // 4]], iu0[: n // 8], iu0[: n // 16]])
+    iv = np.concatenate([iv0, iu0, v[pick[: n // 4]], iv0[: n // 8], iv0[: n // 16]])
+    pd, pi = rng.permutation(len(du)), rng.permutation(len(iu))
+    dels = {"u": du[pd].astype(np.uint32), "v": dv[pd].astype(np.uint32),
+            "w": rng.uniform(0.5, 2.0, len(du)).astype(np.float32)}
+    ins = {"u": iu[pi].astype(np.uint32), "v": iv[pi].astype(np.uint32),
+           "w": rng.uniform(0.5, 2.0, len(iu)).astype(np.float32)}
+    return dels, ins
+
+
+def update_fixtures(ref, graphs):
+    """update_raw.npz: per case the graph, a raw batch, the reference's tidy
+    lists and its updated graph (packed CSR, rows sorted, exists mask)."""
+    out = {}
+    rng = np.random.default_rng(2024)
+    for name, g, n in graphs:
+        dels, ins = raw_batch(g, rng, n)
+        td, ti, y = ref.tidy_update(g, dels, ins)
+        for k, a in g.items():
+            out[f"{name}_x_{k}"] = a
+        for pre, b in (("raw_del", dels), ("raw_ins", ins), ("tidy_del", td), ("tidy_ins", ti)):
+            for k, a in b.items():
+                out[f"{name}_{pre}_{k}"] = a
+        for k, a in y.items():
+            out[f"{name}_y_{k}"] = a
+        print(f"update {name}: raw {len(dels['u'])}/{len(ins['u'])} -> tidy {len(td['u'])}/{len(ti['u'])}, "
+              f"E {int(g['off'][-1])} -> {len(y['keys'])}")
+    out["cases"] = np.array([name for name, _, _ in graphs])
+    path = os.path.join(HERE, "update_raw.npz")
+    np.savez_compressed(path, **out)
+    print(f"update_raw: {os.path.getsize(path)} B")
+
+
 def main():
     ref = RefLib()
     ref.set_threads(1)
@@ -149,6 +197,9 @@ def main():
     ppw = G.symmetric_csr(1500, u[half], v[half], ww)
     dw, iw = G.random_batch(ppw, 25, 100, 13, weight=1.5)
     cases.append(("ppw", ppw, dw, iw, Options()))
+    update_fixtures(ref, [("tiny1", g1, 8), ("pp2k", pp, 200), ("rmat10", rm, 300), ("ppw", ppw, 150)])
+    if "--update-only" in sys.argv:
+        return
     for name, g, d, i, o in cases:
         out = {}
         record(ref, g, d, i, o, "", out)

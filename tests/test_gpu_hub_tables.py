@@ -57,7 +57,9 @@ def _check_run(ctx, oracle, g, dg, db, d, i, st, k, expect_tables):
 
 def test_tables_exact_through_a_chain(ctx, oracle):
     """R-MAT scale 15, six batches, each graph its own resident copy (the
-    reference's driver makes a new graph per batch, main.cxx:105-109)."""
+    reference's driver makes a new graph per batch, main.cxx:105-109): no
+    graph is its predecessor updated in place, so the tables are rebuilt on
+    every run from the second on."""
     graphs, batches = _chain(15, 11, 6, 150, 600)
     s0 = oracle.louvain("static", graphs[0])
     S = len(graphs[0]["off"]) - 1
@@ -139,8 +141,9 @@ def test_table_dropped_by_a_batch(ctx, oracle):
     s0 = oracle.louvain("static", x)
     st = ctx.device_state(S)
     st.upload(s0.membership, s0.vertex_weight, s0.community_weight)
+    dg = ctx.device_graph(x)  # updated in place: the tables are carried from run to run
     for k, ((d, i), g) in enumerate(zip(batches, graphs[1:])):
-        dg, db = ctx.device_graph(g), ctx.device_batch(d, i)
+        db = dg.update(d, i)
         prev = st.download()
         ctx.stats_reset()
         ctx.run(lvhip.LV_DYNAMIC_FRONTIER, dg, db, st)
@@ -158,3 +161,35 @@ def test_table_dropped_by_a_batch(ctx, oracle):
         if k == 2:
             assert t == 0  # ... and dropped by the batch
             assert stats["vertices"][2] >= 1  # the hub went through the block-per-hub kernel
+
+
+def test_tables_rebuilt_when_the_batch_does_not_chain(ctx, oracle):
+    """Tables are carried forward only when the run's batch was made by
+    lv_dgraph_update from the graph they describe into the run's graph.  The
+    same update batch run a second time, and a host-made batch of the same
+    edges, do not chain: the run rebuilds the tables, which equal a fresh
+    build afterwards."""
+    import lvgraph
+    import lvhip
+    graphs, batches = _chain(14, 17, 2, 100, 400)
+    s0 = oracle.louvain("static", graphs[0])
+    S = len(graphs[0]["off"]) - 1
+    dg = ctx.device_graph(graphs[0])
+    st = ctx.device_state(S)
+    st.upload(s0.membership, s0.vertex_weight, s0.community_weight)
+    db0 = dg.update(*batches[0])
+    ctx.run(lvhip.LV_DYNAMIC_FRONTIER, dg, db0, st)  # first run of the state: no tables
+    db1 = dg.update(*batches[1])
+    ctx.run(lvhip.LV_DYNAMIC_FRONTIER, dg, db1, st)  # second: tables built
+    t, m = ctx.check_hub_tables(dg, st)
+    _say(f"  built: tables {t} mismatched {m}")
+    assert t > 0 and m == 0
+    ctx.run(lvhip.LV_DYNAMIC_FRONTIER, dg, db1, st)  # the same batch again: rebuilt
+    t, m = ctx.check_hub_tables(dg, st)
+    _say(f"  same batch again: tables {t} mismatched {m}")
+    assert t > 0 and m == 0
+    hb = ctx.device_batch(*lvgraph.tidy_batch(graphs[1], *batches[1]))
+    ctx.run(lvhip.LV_DYNAMIC_FRONTIER, dg, hb, st)  # a host batch: rebuilt
+    t, m = ctx.check_hub_tables(dg, st)
+    _say(f"  host batch: tables {t} mismatched {m}")
+    assert t > 0 and m == 0

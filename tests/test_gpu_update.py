@@ -130,3 +130,21 @@ def test_update_rejects_ids_beyond_span(ctx):
     with pytest.raises(lvhip.LvError) as e:
         dg.update(None, ins)
     assert e.value.status == -1
+
+
+def test_device_update_matches_reference_vectors(ctx):
+    """lv_dgraph_update against the reference's own tidyBatchUpdateU +
+    applyBatchUpdateOmpU (tests/golden/update_raw.npz; the weight rule for
+    repeated entries is in update_golden.py), on the packed upload and again
+    on its rebuilt copy."""
+    import update_golden as U
+    for name, g, (rd, ri), (td, ti), y in U.load():
+        print("case", name, flush=True)
+        dg = ctx.device_graph(g)
+        tb = dg.update(rd, ri)
+        got_d, got_i = tb.download()
+        rep = U.check_tidy(got_d, td, rd, f"{name} deletions") + U.check_tidy(got_i, ti, ri, f"{name} insertions")
+        U.check_graph(_live(dg.download()), y, ri, name)
+        print(f"  {len(td['u'])} deletions, {len(ti['u'])} insertions, {rep} repeated pairs", flush=True)
+        tb.close()
+        dg.close()
