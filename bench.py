@@ -1,20 +1,25 @@
-"""Benchmark: DF Louvain on R-MAT scale 22 (edge factor 16, a=.57 b=.19 c=.19,
+"""Benchmark: DF Louvain on R-MAT scale 24 (edge factor 16, a=.57 b=.19 c=.19,
 symmetrised, unit weights), consecutive random batches of 1e-4 |E| (80%
 insertions / 20% deletions), seed 7 -- the workload of the headline metric
 "DF-Louvain edges/s & p99 batch latency at dQ<=1e-3 vs plain GPU path".
 
 A step is one Dynamic Frontier run (louvainDynamicFrontierOmp,
-inc/louvain.hxx:1674) on one batch, with graph, batch and state resident in
-HBM when the timed region starts; the state carries from batch to batch as
-in main.cxx:127-135.  value = directed edges of the updated graphs processed
-per second of wall clock over the K timed steps.  The same chain is then run
-on the plain GPU path (sweep over all vertices every iteration), checked
-against the reference on the same inputs (|dQ| <= 1e-3 per batch), and the
-reference's own DF is timed on the host cores (cpu_baseline): its code
-compiled in place (oracle/_ref) where that was built, else the oracle
-restatement.
+inc/louvain.hxx:1674) on one batch, with the graph resident in HBM and
+already updated in place by that batch (lv_dgraph_update, untimed like the
+reference driver's update, main.cxx:104-109) when the timed region starts;
+the state carries from batch to batch as in main.cxx:127-135.  value =
+directed edges of the updated graphs processed per second of the host
+clock around the K timed runs.  The same chain is then run on the frontier
+and the plain GPU path (one sweep launch over all vertices per iteration),
+a few of the timed batches (--cpu-batches, evenly spread) are checked
+against the reference's DF from the same carry-in state (|dQ| <= 1e-3), and
+the reference's own DF on those batches is timed on the host cores
+(cpu_baseline): its code compiled in place (oracle/_ref) where that was
+built, else the oracle restatement.  A bounded serving segment (resident
+R-MAT 20..23 graphs, Poisson arrivals) reports p50 / p99 latency.
 
-    python bench.py [--steps K] [--warmup W] [--scale 22] [--no-cpu] [--no-plain]
+    python bench.py [--steps K] [--warmup W] [--scale 24] [--no-cpu] [--no-plain] [--no-serve]
+    python bench.py --serve | --batched | --sweep     (the other forms)
 """
 import argparse
 import json
@@ -60,7 +65,10 @@ HBM_PEAK_GBPS = 8000.0
 KERNEL_NAME = "move_kernel<256, 64, 512, float, 2>"
 SMALL_HUB_KERNEL = "hub_mid_kernel<256, 2048u, 1024u, float>"
 MID_HUB_KERNEL = "hub_mid_kernel<512, 2048u, 16384u, float>"
-PMC_SUMMARY = os.path.join(ROOT, "profiles", "move_kernel_df_rmat22_pmc.txt")
+def pmc_summary(scale):
+    """The committed FETCH_SIZE / WRITE_SIZE summary of the DF chain at this
+    R-MAT scale."""
+    return os.path.join(ROOT, "profiles", f"move_kernel_df_rmat{scale}_pmc.txt")
 METRIC = "DF-Louvain edges/s & p99 batch latency at dQ<=1e-3 vs plain GPU path"
 
 
@@ -83,48 +91,19 @@ def algorithmic_bytes(st, tier):
             + 20 * st["moves"][tier] + st["moved_edges"][tier])
 
 
-def pmc_traffic(kernel):
+def pmc_traffic(kernel, scale):
     """HBM bytes per launch of `kernel` on this workload, from the committed
-    FETCH_SIZE / WRITE_SIZE summary (two rocprofv3 --pmc runs of this script,
-    each scaled by the counter's reading for a copy of known size); None if
-    the summary is absent."""
-    if not os.path.exists(PMC_SUMMARY):
+    FETCH_SIZE / WRITE_SIZE summary for this scale (two rocprofv3 --pmc runs
+    of this script, each scaled by the counter's reading for a copy of known
+    size); None if there is no summary for the scale."""
+    path = pmc_summary(scale)
+    if not os.path.exists(path):
         return None
-    for line in open(PMC_SUMMARY):
+    for line in open(path):
         parts = line.split("\t")
         if len(parts) >= 2 and parts[0] == "traffic_bytes_per_launch " + kernel:
-            return {"bytes_per_launch": float(parts[1]), "source": os.path.relpath(PMC_SUMMARY, ROOT)}
+            return {"bytes_per_launch": float(parts[1]), "source": os.path.relpath(path, ROOT)}
     return None
-
-
-def run_chain(ctx, lvhip, dgraphs, dbatches, state, state0, W, K, path, profile=False):
-    """Warm up on batches [0, W), time batches [W, W + K); returns wall
-    seconds, per-step algorithm times, edge counts and kernel statistics.
-    With profile, every local-moving launch is bracketed by HIP events on
-    its stream (and launched one by one, not replayed from a graph): the
-    per-kernel durations of the roofline come from such a run, the
-    throughput from one without."""
-    ctx.set_path(path)
-    state.copy_from(state0)
-    for k in range(W):
-        ctx.run(lvhip.LV_DYNAMIC_FRONTIER, dgraphs[k], dbatches[k], state)
-    ctx.stats_reset()
-    ctx.set_profiling(profile)
-    lat, edges, parts = [], 0, []
-    t0 = time.perf_counter()
-    for k in range(W, W + K):
-        r = ctx.run(lvhip.LV_DYNAMIC_FRONTIER, dgraphs[k], dbatches[k], state)
-        lat.append(r.time)
-        parts.append((r.initializationTime, r.markingTime, r.firstPassTime, r.localMoveTime, r.aggregationTime,
-                      r.iterations, r.passes, r.affectedVertices))
-        edges += dgraphs[k].edges
-    t1 = time.perf_counter()
-    ctx.set_profiling(False)
-    m = [sum(x[j] for x in parts) / len(parts) for j in range(8)]
-    log(f"  path {path} per batch (mean): update {m[0]:.3f} ms, marking {m[1]:.3f} ms, first pass {m[2]:.3f} ms, "
-        f"local-moving {m[3]:.3f} ms, aggregation {m[4]:.3f} ms; iterations {m[5]:.1f}, passes {m[6]:.1f}, "
-        f"affected {m[7]:.0f}")
-    return t1 - t0, lat, edges, ctx.stats()
 
 
 def serve_main(args):
@@ -390,18 +369,98 @@ def sweep_main(args):
     print(json.dumps(out), flush=True)
 
 
+def run_chain(ctx, lvhip, g0, batches, state0, W, K, path, profile=False):
+    """The DF chain the way a resident graph serves it: graph 0 uploaded,
+    then per batch the device tidy and in-place update (lv_dgraph_update,
+    untimed, as in the reference's driver, main.cxx:104-109) and one DF run
+    (louvainDynamicFrontierOmp, inc/louvain.hxx:1674) timed by the host
+    clock around lv_run, with the state carried (main.cxx:127-135).  Batches
+    [0, W) warm up.  Returns the run walls (s), the runs' own timed regions
+    (ms), the edges of each updated graph, the update walls (s) and the
+    kernel statistics.  With profile, every local-moving launch is bracketed
+    by HIP events on its stream (launched one by one, not replayed)."""
+    ctx.set_path(path)
+    dg = ctx.device_graph(g0)
+    state = ctx.device_state(dg.span)
+    state.copy_from(state0)
+    walls, lat, edges, upd = [], [], [], []
+    for k in range(W + K):
+        if k == W:
+            ctx.stats_reset()
+            ctx.set_profiling(profile)
+        t0 = time.perf_counter()
+        tb = dg.update(*batches[k])
+        t1 = time.perf_counter()
+        r = ctx.run(lvhip.LV_DYNAMIC_FRONTIER, dg, tb, state)
+        t2 = time.perf_counter()
+        tb.close()
+        if k >= W:
+            walls.append(t2 - t1)
+            upd.append(t1 - t0)
+            lat.append(r.time)
+            edges.append(dg.edges)
+    ctx.set_profiling(False)
+    st = ctx.stats()
+    dg.close()
+    state.close()
+    return walls, lat, edges, upd, st
+
+
+def serve_segment(args, log_prefix="serve"):
+    """A bounded run of the serving form (config 5's shape): resident R-MAT
+    graphs of scales args.serve_scales, two each, Poisson arrivals at
+    args.serve_rate, batches log-uniform in 1e-6..1e-3 |E| (80/20), served by
+    lvserve; returns its summary."""
+    import numpy as np
+
+    import lvgraph
+    import lvserve
+    rng = np.random.default_rng(args.serve_seed)
+    graphs = {}
+    for scale in args.serve_scales:
+        for k in range(2):
+            graphs[f"rmat{scale}_{k}"] = lvgraph.rmat_native(scale, args.edge_factor, 0.57, 0.19, 0.19,
+                                                             args.serve_seed * 100 + scale * 10 + k)
+    names = list(graphs)
+    arrivals = lvserve.poisson_arrivals(args.serve_requests, args.serve_rate, args.serve_seed)
+    reqs = []
+    for k in range(args.serve_requests):
+        name = names[int(rng.integers(len(names)))]
+        g = graphs[name]
+        b = max(1, int(round(10 ** rng.uniform(-6, -3) * (g.edges // 2))))
+        n_ins = int(round(0.8 * b))
+        d, i = g.random_batch(b - n_ins, n_ins, args.serve_seed * 1000003 + k)
+        reqs.append(lvserve.Request(k, name, d, i, float(arrivals[k])))
+    srv = lvserve.Server(graphs, slots=args.serve_slots, memory_budget=args.serve_budget_mib << 20)
+    warm = [lvserve.Request(-1 - j, n, *graphs[n].random_batch(10, 40, 77 + j), 0.0) for j, n in enumerate(names)]
+    srv.run(warm, realtime=False)
+    t = time.perf_counter()
+    done = srv.run(reqs, realtime=True)
+    wall = time.perf_counter() - t
+    s = lvserve.summary(done)
+    s["edges_per_s"] = sum(graphs[r.graph].edges for r in done if r.error is None) / wall
+    s["graphs"] = {n: g.edges for n, g in graphs.items()}
+    log(f"{log_prefix}: {s['requests']} requests over {len(graphs)} resident graphs in {wall:.1f}s: "
+        f"p50 {s['p50_ms']:.2f} ms, p99 {s['p99_ms']:.2f} ms, max {s['max_ms']:.2f} ms, {s['errors']} errors")
+    srv.close()
+    return s
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--scale", type=int, default=22)
+    ap.add_argument("--scale", type=int, default=24)
     ap.add_argument("--edge-factor", type=int, default=16)
     ap.add_argument("--batch-fraction", type=float, default=1e-4)
     ap.add_argument("--seed", type=int, default=7)
-    ap.add_argument("--no-cpu", action="store_true", help="skip the oracle check and CPU baseline")
-    ap.add_argument("--no-plain", action="store_true", help="skip the plain-path comparison")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the reference check and CPU baseline")
+    ap.add_argument("--cpu-batches", type=int, default=3,
+                    help="timed batches the reference also runs (dQ and the CPU baseline), evenly spread")
+    ap.add_argument("--no-plain", action="store_true", help="skip the frontier and plain paths")
     ap.add_argument("--no-static", action="store_true", help="skip the (reported) GPU Static run")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving segment")
     ap.add_argument("--stream-fraction", type=float, default=None,
                     help="streams per live vertex of a level (lv_context_set_schedule)")
     ap.add_argument("--worklist-hub-degree", type=int, default=None,
@@ -412,10 +471,12 @@ def main():
                     help="local-moving path of the timed chain (the others are timed beside it)")
     ap.add_argument("--serve", action="store_true", help="run the serving form instead of the DF chain")
     ap.add_argument("--serve-scales", type=int, nargs="+", default=[20, 21, 22, 23])
-    ap.add_argument("--serve-requests", type=int, default=10000)
+    ap.add_argument("--serve-requests", type=int, default=None,
+                    help="requests (default 10000 with --serve, 2000 for the segment of the default run)")
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--serve-slots", type=int, default=4)
-    ap.add_argument("--serve-budget-mib", type=int, default=1024)
+    ap.add_argument("--serve-budget-mib", type=int, default=65536,
+                    help="device memory of the server: graphs, states, workspaces and batches in flight")
     ap.add_argument("--serve-seed", type=int, default=8)
     ap.add_argument("--batched", action="store_true", help="run the batched form (many graphs at once)")
     ap.add_argument("--batched-graphs", type=int, default=256)
@@ -429,6 +490,7 @@ def main():
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     if args.serve:
+        args.serve_requests = args.serve_requests or 10000
         return serve_main(args)
     if args.batched:
         return batched_main(args)
@@ -445,123 +507,122 @@ def main():
     n_ins = int(round(0.8 * B))
     n_del = B - n_ins
     t = time.time()
-    graphs = [lvgraph.rmat_native(args.scale, args.edge_factor, 0.57, 0.19, 0.19, args.seed)]
-    log(f"rmat scale {args.scale}: S={graphs[0].span} E={graphs[0].edges} ({time.time() - t:.1f}s)")
-    batches = []
-    t = time.time()
-    for k in range(W + K):
-        d, i = graphs[-1].random_batch(n_del, n_ins, args.seed * 1000 + k + 1)
-        batches.append((d, i))
-        graphs.append(graphs[-1].apply(d, i))
-    log(f"{W + K} batches of {B} updates ({n_del} del / {n_ins} ins undirected) applied ({time.time() - t:.1f}s)")
+    H0 = lvgraph.rmat_native(args.scale, args.edge_factor, 0.57, 0.19, 0.19, args.seed)
+    g0 = H0.view()
+    S = H0.span
+    log(f"rmat scale {args.scale}: S={S} E={H0.edges} ({time.time() - t:.1f}s)")
 
     ctx = lvhip.Context(0)
     if args.stream_fraction is not None:
         ctx.set_schedule(stream_fraction=args.stream_fraction)
     if args.worklist_hub_degree is not None:
         ctx.set_hub_threshold(worklist_degree=args.worklist_hub_degree)
-    t = time.time()
-    dgraphs = [ctx.device_graph(g.view()) for g in graphs]
-    dbatches = [ctx.device_batch(d, i) for d, i in batches]
-    S = graphs[0].span
+    dg0 = ctx.device_graph(g0)
     state0 = ctx.device_state(S)
-    state = ctx.device_state(S)
-    log(f"uploaded {len(dgraphs)} graphs + {len(dbatches)} batches ({time.time() - t:.1f}s)")
     # GPU Static Louvain on the original graph (reported), and the starting
     # state of the DF chain: the reference's own Static result, as in
-    # main.cxx:89 (louvainStaticOmp), from the oracle on the host cores.
+    # main.cxx:89 (louvainStaticOmp), on the host cores.
     rs, q_static = None, None
     if not args.no_static:
         t = time.time()
-        rs = ctx.run(lvhip.LV_STATIC, dgraphs[0], None, state0)
-        q_static = ctx.dmodularity(dgraphs[0], state0)
+        rs = ctx.run(lvhip.LV_STATIC, dg0, None, state0)
+        q_static = ctx.dmodularity(dg0, state0)
         log(f"GPU static: Q={q_static:.6f} iters={rs.iterations} passes={rs.passes} time={rs.time:.1f} ms "
             f"(wall {time.time() - t:.1f}s)")
     orc, kind = cpu_checker()
     cores = HOST_CORES
     orc.set_threads(cores)
     t = time.time()
-    r0 = orc.louvain("static", graphs[0].view())
+    r0 = orc.louvain("static", g0)
     state0.upload(r0.membership, r0.vertex_weight, r0.community_weight)
-    q_static_ref = ctx.dmodularity(dgraphs[0], state0)
+    q_static_ref = ctx.dmodularity(dg0, state0)
     log(f"reference static ({kind}, {cores} threads): Q={q_static_ref:.6f} iters={r0.iterations} "
         f"passes={r0.passes} time={r0.time:.1f} ms (wall {time.time() - t:.1f}s)")
+    dg0.close()
 
-    # Quality chain (untimed): the GPU state before every batch, for the oracle.
-    ctx.set_path({"frontier": lvhip.PATH_FRONTIER, "worklist": lvhip.PATH_WORKLIST,
-                  "plain": lvhip.PATH_SWEEP}[args.path])
-    prev, q_gpu, res_gpu = [], [], []
-    if not args.no_cpu or args.log_lines:
-        state.copy_from(state0)
-        for k in range(W + K):
-            prev.append(state.download())
-            res_gpu.append(ctx.run(lvhip.LV_DYNAMIC_FRONTIER, dgraphs[k + 1], dbatches[k], state))
-            q_gpu.append(ctx.dmodularity(dgraphs[k + 1], state))
-    if args.log_lines:
+    # The batches, and the quality chain (untimed): each batch's device
+    # update and DF run, then the reference's DF on the same updated graph
+    # from the same carry-in state (dQ per batch, and the CPU baseline).
+    paths = {"frontier": lvhip.PATH_FRONTIER, "worklist": lvhip.PATH_WORKLIST, "plain": lvhip.PATH_SWEEP}
+    ctx.set_path(paths[args.path])
+    t = time.time()
+    batches, dqs, ctime, cedges = [], [], 0.0, 0
+    logf = open(args.log_lines, "w") if args.log_lines else None
+    if logf:
         # the reference driver's log format (main.cxx:78-84), readable by its process.js
         import lvlog
-        with open(args.log_lines, "w") as fh:
-            fh.write(lvlog.graph_line(f"synthetic/rmat-{args.scale}.txt") + "\n")
-            fh.write(lvlog.order_line(S, graphs[0].edges) + "\n")
-            if rs is not None:
-                fh.write(lvlog.result_line(rs, "louvainStaticHip", 1, q_static) + "\n")
-            for k in range(W + K):
-                fh.write(lvlog.order_line(S, graphs[k + 1].edges, insertions=len(batches[k][1]["u"])) + "\n")
-                fh.write(lvlog.result_line(res_gpu[k], "louvainDynamicFrontierHip", 1, q_gpu[k], 0.0,
-                                           args.batch_fraction) + "\n")
+        logf.write(lvlog.graph_line(f"synthetic/rmat-{args.scale}.txt") + "\n")
+        logf.write(lvlog.order_line(S, H0.edges) + "\n")
+        if rs is not None:
+            logf.write(lvlog.result_line(rs, "louvainStaticHip", 1, q_static) + "\n")
+    dq_state = ctx.device_state(S)
+    dq_state.copy_from(state0)
+    dqg = ctx.device_graph(g0)
+    Hk = H0
+    n_chk = max(1, min(args.cpu_batches, K))
+    checked = {W + int(round(j * (K - 1) / max(n_chk - 1, 1))) for j in range(n_chk)}
+    for k in range(W + K):
+        d, i = Hk.random_batch(n_del, n_ins, args.seed * 1000 + k + 1)
+        batches.append((d, i))
+        check = not args.no_cpu and k in checked
+        prev = dq_state.download() if check else None
+        tb = dqg.update(d, i)
+        r = ctx.run(lvhip.LV_DYNAMIC_FRONTIER, dqg, tb, dq_state)
+        tb.close()
+        q = ctx.dmodularity(dqg, dq_state) if (check or logf) else None
+        if check or logf:
+            Hn = Hk.apply(d, i)
+            if check:
+                ref = orc.louvain("dynamic-frontier", Hn.view(), d, i, *prev)
+                dqs.append(q - orc.modularity(Hn.view(), ref.membership))
+                ctime += ref.time
+                cedges += Hn.edges
+            if logf:
+                logf.write(lvlog.order_line(S, Hn.edges, insertions=len(i["u"])) + "\n")
+                logf.write(lvlog.result_line(r, "louvainDynamicFrontierHip", 1, q, 0.0, args.batch_fraction) + "\n")
+            Hk = Hn
+        else:
+            Hk = Hk.apply(d, i)
+    dqg.close()
+    dq_state.close()
+    if logf:
+        logf.close()
         log(f"log lines written to {args.log_lines}")
+    del Hk
+    log(f"{W + K} batches of {B} updates ({n_del} del / {n_ins} ins undirected), quality chain and reference "
+        f"runs ({time.time() - t:.1f}s)")
 
-    # Timed: the chosen path, then the others beside it.
-    paths = {"frontier": lvhip.PATH_FRONTIER, "worklist": lvhip.PATH_WORKLIST, "plain": lvhip.PATH_SWEEP}
-    dg = dgraphs[1:]
-    wall, lat, edges, _ = run_chain(ctx, lvhip, dg, dbatches, state, state0, W, K, paths[args.path])
-    value = edges / wall
-    log(f"DF {args.path}: {K} steps {wall * 1e3:.2f} ms wall, {value:.4g} edges/s, "
-        f"p50 {percentile(lat, 50):.3f} ms p99 {percentile(lat, 99):.3f} ms")
-    # the same chain again with every local-moving launch timed by events
-    # (launched one by one): the per-kernel durations of the roofline
-    pwall, _, _, st = run_chain(ctx, lvhip, dg, dbatches, state, state0, W, K, paths[args.path], profile=True)
-    log(f"DF {args.path}, profiled (per-launch events, no graph replay): {pwall * 1e3:.2f} ms wall")
+    # Timed: the chosen path, then the others beside it, each over the same chain.
+    walls, lat, edges, upd, _ = run_chain(ctx, lvhip, g0, batches, state0, W, K, paths[args.path])
+    value = sum(edges) / sum(walls)
+    log(f"DF {args.path}: {K} runs {sum(walls) * 1e3:.2f} ms wall, {value:.4g} edges/s, "
+        f"p50 {percentile(walls, 50) * 1e3:.3f} ms p99 {percentile(walls, 99) * 1e3:.3f} ms")
+    # the same chain with every local-moving launch timed by events: the
+    # per-kernel durations of the roofline
+    pw, _, _, _, st = run_chain(ctx, lvhip, g0, batches, state0, W, K, paths[args.path], profile=True)
+    log(f"DF {args.path}, profiled (per-launch events, no graph replay): {sum(pw) * 1e3:.2f} ms wall")
     others = {}
     if not args.no_plain:
         for name, path in paths.items():
             if name == args.path:
                 continue
-            pw, plat, pedges, _ = run_chain(ctx, lvhip, dg, dbatches, state, state0, W, K, path)
-            others[name] = {"value": pedges / pw, "ms_per_step": pw * 1e3 / K, "p50_ms": percentile(plat, 50),
-                            "p99_ms": percentile(plat, 99), "speedup": (pedges / pw) and value / (pedges / pw)}
-            log(f"DF {name}: {pw * 1e3:.2f} ms wall, {others[name]['value']:.4g} edges/s, "
-                f"{args.path} path {others[name]['speedup']:.2f}x faster")
+            ow, olat, oedges, _, _ = run_chain(ctx, lvhip, g0, batches, state0, W, K, path)
+            ov = sum(oedges) / sum(ow)
+            others[name] = {"value": ov, "ms_per_step": sum(ow) * 1e3 / K, "p50_ms": percentile(ow, 50) * 1e3,
+                            "p99_ms": percentile(ow, 99) * 1e3, "speedup": value / ov}
+            log(f"DF {name}: {sum(ow) * 1e3:.2f} ms wall, {ov:.4g} edges/s, {args.path} path "
+                f"{others[name]['speedup']:.2f}x faster")
     ctx.set_path(paths[args.path])
-
-    # End to end: each batch's H2D, device tidy and in-place graph update
-    # (lv_dgraph_update, untimed in the reference), then DF on the updated
-    # resident graph.  The first batch moves the packed upload to a CSR with
-    # slack; it is a warm-up batch.
-    ctx.set_path(paths[args.path])
-    dg_e2e = ctx.device_graph(graphs[0].view())
-    state.copy_from(state0)
-    e2e, upd = [], []
-    for k in range(W + K):
-        t0 = time.perf_counter()
-        tb = dg_e2e.update(*batches[k])
-        t1 = time.perf_counter()
-        ctx.run(lvhip.LV_DYNAMIC_FRONTIER, dg_e2e, tb, state)
-        t2 = time.perf_counter()
-        assert dg_e2e.edges == graphs[k + 1].edges, "device update diverged from the host update"
-        if k >= W:
-            e2e.append((t2 - t0) * 1e3)
-            upd.append((t1 - t0) * 1e3)
+    e2e = [(a + b) * 1e3 for a, b in zip(walls, upd)]
     end_to_end = {"p50_ms": percentile(e2e, 50), "p99_ms": percentile(e2e, 99),
-                  "update_p50_ms": percentile(upd, 50), "update_p99_ms": percentile(upd, 99)}
+                  "update_p50_ms": percentile(upd, 50) * 1e3, "update_p99_ms": percentile(upd, 99) * 1e3}
     log(f"end to end (H2D + device tidy + graph update + DF): p50 {end_to_end['p50_ms']:.3f} ms, "
         f"p99 {end_to_end['p99_ms']:.3f} ms; update alone p50 {end_to_end['update_p50_ms']:.3f} ms")
-    dg_e2e.close()
 
     # Roofline of the dominant local-moving kernel (largest total time).
     # Stats slots on the worklist path: 0 = the id-order kernel (one launch
-    # per sub-round), 2 / 3 = the block-per-hub kernels (small / mid hubs),
-    # 4 = the hub batch (five kernels per launch).
+    # per sub-round), 1 = hubs evaluated from their tables, 2 / 3 = the
+    # block-per-hub kernels (small / mid hubs), 4 = the hub batch.
     def kernel_line(slot, name):
         launches = max(st["move_launches"][slot], 1)
         ms = st["move_ms"][slot]
@@ -575,57 +636,55 @@ def main():
              4: kernel_line(4, "hub batch (hub_prep/insert/best/apply/flag)")}
     dom = max((0, 2, 3), key=lambda k: st["move_ms"][k])
     main_k, hub_k = lines[dom], lines[4]
-    traffic = pmc_traffic(main_k["kernel"])
+    traffic = pmc_traffic(main_k["kernel"], args.scale)
     roofline = {"bound": "hbm", "achieved": main_k["achieved"], "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                 "frac": main_k["achieved"] / HBM_PEAK_GBPS, "traffic": traffic and traffic["bytes_per_launch"],
                 "kernel": main_k["kernel"], "avg_launch_us": main_k["avg_launch_us"],
                 "bytes_per_launch": main_k["bytes_per_launch"], "launches": main_k["launches"],
                 "traffic_source": traffic and traffic["source"], "hub_batch": hub_k,
                 "other_kernels": [lines[k] for k in (0, 2, 3) if k != dom]}
+    visits = sum(st["edges"])  # edges visited by local-moving, every kernel, over the K timed runs
     log("kernel stats per slot: " + "; ".join(
         f"{k}: launches {st['move_launches'][k]} ms {st['move_ms'][k]:.3f} vertices {st['vertices'][k]} "
         f"edges {st['edges'][k]} candidates {st['candidates'][k]} moves {st['moves'][k]}" for k in range(5)))
     log(f"roofline: {main_k['kernel']}: {main_k['bytes_per_launch'] / 1e6:.2f} MB algorithmic per launch, "
-        f"{main_k['avg_launch_us']:.1f} us -> {main_k['achieved']:.1f} GB/s; hub batch {hub_k['avg_launch_us']:.1f} us, "
-        f"{hub_k['achieved']:.1f} GB/s; per launch " + ", ".join(
-            f"{k} {main_k['per_launch'][k]:.0f}/{hub_k['per_launch'][k]:.0f}" for k in main_k["per_launch"]))
+        f"{main_k['avg_launch_us']:.1f} us -> {main_k['achieved']:.1f} GB/s; per launch " + ", ".join(
+            f"{k} {main_k['per_launch'][k]:.0f}" for k in main_k["per_launch"]))
 
-    cpu = None
-    dq_max = None
+    cpu, dq_max = None, None
     if not args.no_cpu:
-        dqs, ctime, cedges = [], 0.0, 0
-        for k in range(W, W + K):
-            q, vt, ct = prev[k]
-            d, i = batches[k]
-            y = graphs[k + 1].view()
-            r = orc.louvain("dynamic-frontier", y, d, i, q, vt, ct)
-            q_ref = orc.modularity(y, r.membership)
-            dqs.append(q_gpu[k] - q_ref)
-            ctime += r.time
-            cedges += graphs[k + 1].edges
         dq_max = max(abs(x) for x in dqs)
         log("dQ per batch (GPU - reference): " + " ".join(f"{x:+.2e}" for x in dqs))
         what = "the reference's DF compiled in place" if kind == "reference" else "oracle DF (C + OpenMP)"
         cpu = {"value": cedges / (ctime * 1e-3), "unit": "edges/s", "cores": cores, "kind": kind,
-               "sample": f"{what} on the {K} timed batches, same inputs and GPU carry-in state, reference "
-                         f"timed region; {ctime:.1f} ms total", "host": host_info()}
+               "sample": f"{what} on {len(dqs)} of the {K} timed batches (evenly spread), same updated "
+                         f"graphs and GPU carry-in states, reference timed region; {ctime:.1f} ms total",
+               "host": host_info()}
         log(f"{kind}: max |dQ| = {dq_max:.3g}; cpu {cpu['value']:.4g} edges/s on {cores} threads")
+
+    serving = None
+    if not args.no_serve:
+        args.serve_requests = args.serve_requests or 2000
+        serving = serve_segment(args)
 
     out = {
         "metric": METRIC, "value": value, "unit": "edges/s", "n_gpus": 1, "steps": K, "warmup": W,
-        "ms_per_step": wall * 1e3 / K, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "ms_per_step": sum(walls) * 1e3 / K, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"DF Louvain, R-MAT scale {args.scale} edge factor {args.edge_factor} "
-                               f"(a=.57 b=.19 c=.19) symmetrised unit weights, batches of "
-                               f"{args.batch_fraction:g}|E| 80/20 insert/delete, seed {args.seed}",
-                   "vertices": S, "edges": graphs[0].edges, "batch_updates": B, "parallelism": "1 GPU"},
-        "p50_batch_ms": percentile(lat, 50), "p99_batch_ms": percentile(lat, 99), "end_to_end": end_to_end,
+                               f"(a=.57 b=.19 c=.19) symmetrised unit weights, consecutive batches of "
+                               f"{args.batch_fraction:g}|E| 80/20 insert/delete applied in place, seed {args.seed}",
+                   "vertices": S, "edges": H0.edges, "batch_updates": B, "parallelism": "1 GPU"},
+        "p50_batch_ms": percentile(walls, 50) * 1e3, "p99_batch_ms": percentile(walls, 99) * 1e3,
+        "algorithm_p50_ms": percentile(lat, 50),
+        "edge_visits_per_s": visits / (sum(pw) if sum(pw) > 0 else 1.0),
+        "end_to_end": end_to_end,
         "dq_max": dq_max, "static": {"gpu_modularity": q_static, "gpu_ms": rs and rs.time,
                                      "reference_modularity": q_static_ref, "reference_ms": r0.time,
                                      "reference_threads": cores},
         "path": args.path, "plain_gpu_path": others.get("plain"),
         "other_gpu_paths": {k: v for k, v in others.items() if k != "plain"},
-        "roofline": roofline, "cpu_baseline": cpu,
+        "roofline": roofline, "cpu_baseline": cpu, "serving": serving,
     }
     print(json.dumps(out), flush=True)
 

@@ -41,7 +41,8 @@ typedef enum lv_status {
   LV_ERR_HIP = -3,       /* a HIP runtime call failed (see lv_last_error) */
   LV_ERR_UNSORTED = -4,  /* batch not sorted by (u, v) / not unique */
   LV_ERR_NODEVICE = -5,  /* no HIP device */
-  LV_ERR_LIMIT = -6      /* input exceeds a documented limit */
+  LV_ERR_LIMIT = -6,     /* input exceeds a documented limit */
+  LV_ERR_PENDING = -7    /* lv_server_poll: the request has not finished yet */
 } lv_status;
 
 typedef enum lv_approach {
@@ -137,22 +138,22 @@ int lv_context_set_schedule(lv_context* ctx, double stream_fraction, uint64_t mi
  * vertex to a community with a lower id than its own (no two vertices can
  * then swap communities in one iteration).  Default 0 (off). */
 int lv_context_set_pick_less(lv_context* ctx, int iterations);
-/* Sweeps (order 2): in_sweep = 1 (default) visits every vertex of more than
- * 4096 edges in its turn of the ascending sweep, as the reference does
- * (inc/louvain.hxx:594), each with an HBM table of its own; 0 runs all of
- * them at once before each sweep (the hub kernel).  balance = 1 (default)
- * cuts spread chunks by work (degree plus a per-vertex cost) rather than by
- * id count, so a stream whose ids hold the largest hubs is not the last to
- * finish. */
-int lv_context_set_sweep_hubs(lv_context* ctx, int in_sweep, int balance);
+/* Sweeps of a Static run's first pass and of every later pass (order 2):
+ * in_sweep = 0 (default) runs every vertex of more than 4096 edges at once
+ * before each sweep (the hub kernel); 1 visits each in its turn of the
+ * ascending sweep, as the reference does (inc/louvain.hxx:594), with an HBM
+ * table of its own.  At one stream, 1 is the reference's sequential visit
+ * order.  With many streams it costs modularity on R-MAT graphs (a hub's
+ * visit takes milliseconds while its neighbours move), so it is off. */
+int lv_context_set_sweep_hubs(lv_context* ctx, int in_sweep);
 /* Validated moves of the id-order kernel (order 2): a vertex about to move
  * reads its neighbours' communities again and, if a concurrent stream moved
  * one of them while the vertex was being decided, decides again from the
  * new view (at most twice).  This keeps the decisions on the neighbour view
  * a sequential sweep would have (inc/louvain.hxx:594-605), which concurrent
  * streams otherwise lose; one stream never sees a change, so at one stream
- * the results are unchanged.  0 off, 1 sweeps (every pass except a DF / DS
- * worklist first pass; default), 2 sweeps and worklists. */
+ * the results are unchanged.  0 off (default), 1 sweeps (every pass except
+ * a DF / DS worklist first pass), 2 sweeps and worklists. */
 int lv_context_set_validate(lv_context* ctx, int mode);
 /* Device workspace of the context (grow-only scratch buffers kept between
  * calls): lv_context_workspace_bytes reports its size; with a limit set
@@ -163,17 +164,20 @@ int lv_context_set_workspace_limit(lv_context* ctx, uint64_t bytes);
 uint64_t lv_context_workspace_bytes(const lv_context* ctx);
 /* Sweep schedules by pass (id-order sweeps, order 2): the first
  * lead_iterations local-moving iterations of a Static run's first pass run
- * lead_streams streams over ascending chunks of lead_chunk ids; every pass
- * after the first runs late_streams streams (chunks of late_chunk ids, 0:
- * spread) when late_streams is not 0.  The role of the reference's
- * schedule(dynamic, 2048) loop (inc/louvain.hxx:594) at a chosen width.
- * Defaults: 0 (off), 16, 2048, 0, 0. */
+ * lead_streams streams over ascending chunks of lead_chunk ids (over the
+ * lead id range, see lv_context_set_lead_ids; the other ids under the wide
+ * schedule); every pass after the first runs late_streams streams (chunks of
+ * late_chunk ids, 0: spread) when late_streams is not 0.  The reference's
+ * schedule(dynamic, 2048) loop (inc/louvain.hxx:594) with 16 threads, where
+ * its visit order decides the communities that form first.
+ * Defaults: 1, 16, 2048, 0, 0. */
 int lv_context_set_pass_schedule(lv_context* ctx, int lead_iterations, uint64_t lead_streams, uint64_t lead_chunk,
                                  uint64_t late_streams, uint64_t late_chunk);
-/* The lead schedule of lv_context_set_pass_schedule over the first `ids` ids
- * only (0: all ids, the default; lv_context_set_pass_schedule resets it):
- * a lead iteration sweeps ids [0, ids) under it, then the rest of the ids
- * under the context's schedule. */
+/* The ids a lead iteration sweeps under the lead schedule: [0, ids), then
+ * the rest under the context's schedule.  0 (default): the id prefix that
+ * holds a third of the edge slots (on R-MAT graphs the dense low-id core,
+ * where the largest communities form), at least 2^18 ids (all of a smaller
+ * graph) and at most 2^20; ~0: every id. */
 int lv_context_set_lead_ids(lv_context* ctx, uint64_t ids);
 /* Id-order sweeps (order 2) cut the ids into `slabs` consecutive ranges
  * launched one after another, each seeing every move of the ranges before
@@ -250,6 +254,10 @@ void lv_dgraph_destroy(lv_dgraph* g);
 uint64_t lv_dgraph_span(const lv_dgraph* g);
 /* Directed edges (sum of the row degrees). */
 uint64_t lv_dgraph_edges(const lv_dgraph* g);
+/* Device bytes a resident graph / state holds (a state's hub
+ * neighbour-community tables included). */
+uint64_t lv_dgraph_device_bytes(const lv_dgraph* g);
+uint64_t lv_dstate_device_bytes(const lv_dstate* s);
 /* Edge slots (offsets[span]): rows may carry slack after lv_dgraph_update. */
 uint64_t lv_dgraph_capacity(const lv_dgraph* g);
 /* Batch tidy and graph update on the device: tidyBatchUpdateU
@@ -342,6 +350,77 @@ int lv_renumber(lv_context* ctx, const lv_graph* g, uint32_t* vcom, uint64_t vco
  * thread timing); the rows as sets are exact. */
 int lv_aggregate(lv_context* ctx, const lv_graph* g, const uint32_t* vcom, uint64_t num_communities,
                  uint64_t* zoff, uint32_t* zdeg, uint32_t* zkeys, double* zw);
+
+/* Serving ---------------------------------------------------------------
+ * A request loop over resident graphs: each graph and its state
+ * (membership, K, Sigma_tot and hub neighbour-community tables) stay in HBM;
+ * a request is a raw update batch for one graph, tidied and applied on the
+ * device (lv_dgraph_update), then Dynamic Frontier Louvain runs from the
+ * graph's state (lv_run) -- the per-batch step of the reference's driver,
+ * main.cxx:101-137, with the state carried as main.cxx:127-135 does.
+ * Continuous batching: `slots` worker threads, each with its own lv_context
+ * (one HIP stream and workspace), take requests as they free up; requests of
+ * one graph run one at a time in submission order, those of different graphs
+ * overlap on the device.  Admission by device memory: a request starts only
+ * while the resident graphs and states, every slot's workspace and the
+ * batches in flight (the request's own included) fit the memory budget, or
+ * when nothing else is in flight.  Each slot's workspace is sized for a graph
+ * when the graph is added (a warm-up run), so serving does not regrow it.
+ * No counterpart in the reference, which runs one batch at a time. */
+typedef struct lv_server lv_server;
+typedef struct lv_server_options {
+  int device;               /* [0] */
+  int slots;                /* worker contexts [4] */
+  uint64_t memory_budget;   /* device bytes for graphs, states, workspaces and batches in flight; 0 = no limit [0] */
+  lv_options options;       /* Louvain options of every run [lv_options_default] */
+  int path;                 /* local-moving path of the slots (lv_context_set_path) [2] */
+  int modularity;           /* compute each request's modularity (lv_dmodularity) [0] */
+  int keep_states;          /* keep each request's resulting state for lv_server_poll [0] */
+} lv_server_options;
+/* A finished request.  result carries the run's scalars; when the server
+ * keeps states and the caller sets result.membership / vertex_weight /
+ * community_weight (span entries each), lv_server_poll fills them with the
+ * graph's state right after the request.  Times are milliseconds on the host
+ * clock since lv_server_create. */
+typedef struct lv_completion {
+  uint64_t ticket;
+  uint32_t graph_id;
+  int status;               /* LV_OK or the request's error */
+  double submitted_ms, started_ms, done_ms;
+  lv_result result;
+  double modularity;        /* NaN unless options.modularity */
+  uint64_t deletions, insertions;  /* sizes of the tidy batch */
+} lv_completion;
+typedef struct lv_server_stats {
+  uint64_t submitted, completed, failed;
+  uint64_t resident_bytes;  /* graphs and states */
+  uint64_t workspace_bytes; /* all slots */
+  uint64_t inflight_bytes;  /* batches of running requests */
+  uint64_t peak_bytes;      /* largest sum of the three at an admission */
+} lv_server_stats;
+void lv_server_options_default(lv_server_options* o);
+int lv_server_create(const lv_server_options* o, lv_server** out);
+/* Waits for every submitted request, then frees everything. */
+void lv_server_destroy(lv_server* s);
+/* A resident graph with its starting state: (q, qvtot, qctot), or NULL for
+ * a Static run on the device.  Returns its id in *graph_id. */
+int lv_server_add_graph(lv_server* s, const lv_graph* g, const uint32_t* q, const double* qvtot, const double* qctot,
+                        uint32_t* graph_id);
+/* Queue a raw batch (copied) for a graph; returns a ticket. */
+int lv_server_submit(lv_server* s, uint32_t graph_id, const lv_batch* deletions, const lv_batch* insertions,
+                     uint64_t* ticket);
+/* The completion of a ticket, waiting up to timeout_ms (0: do not wait,
+ * < 0: until it finishes); LV_ERR_PENDING if it has not finished.  A ticket
+ * is reported once. */
+int lv_server_poll(lv_server* s, uint64_t ticket, int timeout_ms, lv_completion* out);
+/* Wait until every submitted request has finished. */
+int lv_server_drain(lv_server* s);
+/* The current state of a graph (waits until the graph is idle). */
+int lv_server_graph_state(lv_server* s, uint32_t graph_id, uint32_t* q, double* qvtot, double* qctot);
+int lv_server_get_stats(lv_server* s, lv_server_stats* out);
+const char* lv_server_last_error(const lv_server* s);
+/* Milliseconds on the server's clock (the one lv_completion's times use). */
+double lv_server_now_ms(const lv_server* s);
 
 #ifdef __cplusplus
 }

@@ -1,6 +1,6 @@
 """Build the native libraries in place (hipcc for gfx950, g++ for host code).
 
-    liblouvain_hip.so  csrc/lv_kernels.hip + lv_update.hip + lv_capi.hip   (HIP, gfx950)
+    liblouvain_hip.so  csrc/lv_kernels.hip + lv_update.hip + lv_capi.hip + lv_serve.hip   (HIP, gfx950)
     liblvgraph.so      csrc/lvgraph.cpp                          (C++17 + OpenMP)
 
 Usage: python build.py [--quiet]
@@ -34,7 +34,8 @@ def _run(cmd, quiet):
 
 
 def build(quiet=False):
-    hip_src = [os.path.join(HERE, "csrc", f) for f in ("lv_kernels.hip", "lv_update.hip", "lv_capi.hip")]
+    hip_src = [os.path.join(HERE, "csrc", f) for f in ("lv_kernels.hip", "lv_update.hip", "lv_capi.hip",
+                                                          "lv_serve.hip")]
     out = os.path.join(HERE, "liblouvain_hip.so")
     _run([HIPCC, *HIP_FLAGS, f"-I{INC}", "-o", out + ".tmp", *hip_src], quiet)
     os.replace(out + ".tmp", out)

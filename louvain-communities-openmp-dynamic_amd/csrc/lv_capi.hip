@@ -62,20 +62,21 @@ struct lv_context {
   // pass run lead_streams streams over ascending chunks of lead_chunk
   // entries; passes after the first run late_streams streams (chunks of
   // late_chunk, 0: spread) when late_streams is set.
-  int lead_iters = 0;
+  // Defaults (Static): the first iteration sweeps the id prefix that holds a
+  // third of the edges (at most kLeadIdsMax ids) in 16 streams of 2048-id
+  // ascending chunks, the reference's own sweep pattern there; the rest of
+  // that iteration and every later one use the wide schedule.
+  int lead_iters = 1;
   uint64_t lead_streams = 16, lead_chunk = 2048;
-  uint64_t lead_ids = 0;  // the lead schedule covers ids [0, lead_ids) (0: all); the rest as usual
+  uint64_t lead_ids = 0;  // ids under the lead schedule: 0 auto, ~0 all, else [0, lead_ids)
   uint64_t late_streams = 0, late_chunk = 0;
   int pick_less_iters = 0;      // first iterations of each pass that only move to lower ids
-  // validated moves of the id-order kernel: 0 off, 1 sweeps (every pass but
-  // a DF / DS worklist first pass), 2 sweeps and worklists
-  int validate = 1;
-  // sweeps: hubs (degree > kHubMin) visited in their id turn with tables of
-  // their own (1, default) or all at once before each sweep (0); spread
-  // chunks cut by work, degree plus a per-vertex cost (1, default), or by
-  // id count (0)
+  // validated moves of the id-order kernel: 0 off (default), 1 sweeps (every
+  // pass but a DF / DS worklist first pass), 2 sweeps and worklists
+  int validate = 0;
+  // sweeps: hubs (degree > kHubMin) all at once before each sweep (0,
+  // default) or visited in their id turn with tables of their own (1)
   int sweep_hubs = 0;
-  int balance = 0;
   // diagnostics (environment LV_TRACE set when the context is made): worklist
   // runs enqueue launch by launch and print every sub-round's queue lengths
   int trace = 0;
@@ -482,6 +483,12 @@ constexpr int kSubRounds = 4;      // worklist sub-rounds enqueued per host chec
 constexpr int kFirstGroup = 8;     // ... in an iteration's first group (a DF step on an R-MAT graph
                                    // typically needs 5-8: one host check instead of two)
 constexpr uint64_t kRefChunk = 2048;  // the reference's OpenMP chunk (inc/louvain.hxx:594)
+// The automatic lead range: the id prefix holding 1 / kLeadShare of the edge
+// slots, at least kLeadIdsMin ids (the whole of a smaller graph) and at most
+// kLeadIdsMax.
+constexpr uint64_t kLeadIdsMin = 1ull << 18;
+constexpr uint64_t kLeadIdsMax = 1ull << 20;
+constexpr uint64_t kLeadShare = 3;
 
 void launch_move(lv_context* c, const MoveArgs& a, int tier, bool f64, hipStream_t s = nullptr) {
   if (!s) s = c->st;
@@ -797,6 +804,10 @@ int local_move(lv_context* c, const Level& lv, uint32_t* vcom, const double* vto
                Batched* bt = nullptr, int pass = -1) {
   MoveArgs a = base_args(c, lv, vcom, vtot, ctot, vaff, fa, M, R);
   batched_args(a, bt);
+  // the frontier path: 64-id windows, the flagged ids' row data loaded with
+  // the window.  (Worklist-path sweeps check one id at a time: with windows
+  // the wide Static sweeps after the lead iteration oscillate -- 27 to 29
+  // iterations and 2e-3 to 8e-3 less modularity on R-MAT 22.)
   a.sparse = c->path == 1;
   if (ascending && !a.chunk) a.chunk = a.sparse ? 64 : 16;
   batched_schedule(c, a, lv, bt);
@@ -818,29 +829,23 @@ int local_move(lv_context* c, const Level& lv, uint32_t* vcom, const double* vto
     a.xval = c->ws<double>("x.val", slots);
     a.hub_min = 0xFFFFFFFFu;
   }
-  // spread chunks cut by work (lv_run sweeps; the batched form keeps its
-  // per-graph chunk table): a table per stream count, made when first used
-  const bool balance = !bt && c->tier_order == 2 && c->balance && c->slabs == 1;
-  uint64_t bal_streams = 0;
-  auto balanced = [&](uint64_t streams) {
-    if (bal_streams != streams) {
-      uint64_t* w = c->ws<uint64_t>("bal.w", lv.g.S + 1);
-      uint64_t* pre = c->ws<uint64_t>("bal.pre", lv.g.S + 1);
-      a.ctab = c->ws<uint64_t>("bal.ctab", 2 * streams);
-      k_visit_work(lv.g.deg, lv.g.S, w, c->st);
-      scan64(c, w, pre, lv.g.S + 1);
-      k_balanced_chunks(pre, lv.g.S, streams, a.ctab, c->st);
-      bal_streams = streams;
-    }
-    a.ctab = c->ws<uint64_t>("bal.ctab", 2 * streams);
-    a.nct = streams;
-  };
   // per-pass sweep schedules (id-order sweeps of lv_run only): regions for
   // the most streams any iteration of this pass runs
   const uint64_t s0 = a.max_streams, ch0 = a.chunk;
   const bool by_pass = !bt && c->tier_order == 2 && pass >= 0;
   const bool lead = by_pass && pass == 0 && c->lead_iters > 0;
   const bool late = by_pass && pass > 0 && c->late_streams > 0;
+  // ids under the lead schedule: the automatic range, all ids, or the set count
+  uint64_t lead_core = 0;
+  if (lead) {
+    if (c->lead_ids == ~0ull) lead_core = 0;
+    else if (c->lead_ids) lead_core = std::min<uint64_t>(c->lead_ids, lv.g.S);
+    else {
+      uint64_t* at = c->ws<uint64_t>("lead.at", 1);
+      k_offset_bound(lv.g.off, lv.g.S, kLeadShare, at, c->st);
+      lead_core = std::min<uint64_t>(std::max<uint64_t>(c->read1(at), kLeadIdsMin), kLeadIdsMax);
+    }
+  }
   if (lead || late) {
     a.max_streams = std::max(s0, lead ? c->lead_streams : c->late_streams);
     id_order_regions(c, a, lv);
@@ -857,20 +862,17 @@ int local_move(lv_context* c, const Level& lv, uint32_t* vcom, const double* vto
     uint64_t core = 0;
     if (lead || late) {
       const bool lead_now = lead && l < c->lead_iters;
-      if (lead_now && c->lead_ids) {
-        core = c->lead_ids;  // the lead schedule over the first ids only
+      if (lead_now && lead_core && lead_core < lv.g.S) {
+        core = lead_core;  // the lead schedule over the first ids only
         a.max_streams = s0;
         a.chunk = ch0;
       } else {
-        a.max_streams = lead_now ? c->lead_streams : late ? c->late_streams : s0;
+        a.max_streams = lead_now ? std::min(c->lead_streams, s0) : late ? c->late_streams : s0;
         a.chunk = lead_now ? c->lead_chunk : late ? c->late_chunk : ch0;
       }
     }
-    if (balance) {
-      if (!a.chunk && !core) balanced(a.max_streams);
-      else a.ctab = nullptr;
-    }
-    move_sweep(c, a, lv, core, c->lead_streams, c->lead_chunk);
+    // (never more streams than the context's schedule gives the level)
+    move_sweep(c, a, lv, core, std::min(c->lead_streams, s0), c->lead_chunk);
     c->check();
     ++l;
     if (bt) {
@@ -1112,6 +1114,7 @@ int local_move_worklist(lv_context* c, const Level& lv, uint32_t* vcom, const do
 
 // ------------------------------------------------------------ aggregation
 struct SuperGraph {
+  const char* name = nullptr;  // its workspace buffers (kept through swaps)
   uint64_t* off = nullptr;
   uint32_t* deg = nullptr;
   uint32_t* keys = nullptr;
@@ -1121,6 +1124,7 @@ struct SuperGraph {
 SuperGraph super_graph(lv_context* c, const char* name, uint64_t S, uint64_t E) {
   std::string n(name);
   SuperGraph s;
+  s.name = name;
   s.off = c->ws<uint64_t>(n + ".off", S + 1);
   s.deg = c->ws<uint32_t>(n + ".deg", S);
   s.keys = c->ws<uint32_t>(n + ".keys", E);
@@ -1318,7 +1322,12 @@ void invoke(lv_context* c, int approach, const lv_dgraph* x, const lv_dbatch* b,
     dsc = c->ws<uint8_t>("ds.c", S);
   }
   bool worklist = c->path == 2 && (approach == LV_DYNAMIC_FRONTIER || approach == LV_DELTA_SCREENING);
-  SuperGraph y = super_graph(c, "y", S, x->E), z = super_graph(c, "z", S, x->E);
+  // the two super-graphs (gapped CSRs with f64 weights) are taken when a
+  // pass first aggregates, sized to the level: a run that never aggregates
+  // (a DF or DS step at small batch sizes) takes none
+  SuperGraph y, z;
+  y.name = "y";
+  z.name = "z";
   k_fill_u8(vaff, 0, (S + 7) & ~3ull, c->st);
   if (approach == LV_STATIC) {
     k_fill_u32(ucom, 0, S, c->st);
@@ -1421,6 +1430,8 @@ void invoke(lv_context* c, int approach, const lv_dgraph* x, const lv_dbatch* b,
       k_lookup(vcom, cur.g.S, map, c->st);
       k_lookup(ucom, S, vcom, c->st);
     }
+    // (a level's super-graph has at most the input's live edges)
+    z = super_graph(c, z.name, CN, x->live);
     hipEvent_t g0 = c->record();
     aggregate(c, cur, mcur, CN, z, vtot);
     ag.push_back({g0, c->record()});
@@ -1549,7 +1560,9 @@ void invoke_batched(lv_context* c, int approach, const lv_dgraph* x, const lv_db
   uint32_t* vcom = c->ws<uint32_t>("vcom", S);
   double* vtot = c->ws<double>("vtot", S);
   bool worklist = c->path == 2 && approach == LV_DYNAMIC_FRONTIER;
-  SuperGraph y = super_graph(c, "y", S, x->E), z = super_graph(c, "z", S, x->E);
+  SuperGraph y, z;  // taken at the first aggregation, as in invoke()
+  y.name = "y";
+  z.name = "z";
   // per-graph data: graph of every vertex (level 0 and the current level),
   // M from the graph's own edge weights (edgeWeightOmp of each graph)
   uint64_t* dvoff = c->ws<uint64_t>("bt.voff", G + 1);
@@ -1737,6 +1750,7 @@ void invoke_batched(lv_context* c, int approach, const lv_dgraph* x, const lv_db
       k_lookup_active(ucom, S, vcom, gid0, dfin, c->st);
     }
     const uint64_t CN = lo_next[G];
+    z = super_graph(c, z.name, CN, x->live);
     hipEvent_t g0 = c->record();
     Level agl = cur;
     agl.g = act;
@@ -2348,7 +2362,6 @@ uint64_t lv_context_workspace_bytes(const lv_context* c) { return c ? (uint64_t)
 int lv_context_set_pass_schedule(lv_context* c, int lead_iterations, uint64_t lead_streams, uint64_t lead_chunk,
                                  uint64_t late_streams, uint64_t late_chunk) {
   if (!c || lead_iterations < 0 || (lead_iterations && (!lead_streams || !lead_chunk))) return LV_ERR_INVALID;
-  c->lead_ids = 0;
   c->lead_iters = lead_iterations;
   c->lead_streams = lead_streams;
   c->lead_chunk = lead_chunk;
@@ -2358,15 +2371,15 @@ int lv_context_set_pass_schedule(lv_context* c, int lead_iterations, uint64_t le
 }
 
 int lv_context_set_lead_ids(lv_context* c, uint64_t ids) {
+  // (0: automatic, ~0: every id)
   if (!c) return LV_ERR_INVALID;
   c->lead_ids = ids;
   return LV_OK;
 }
 
-int lv_context_set_sweep_hubs(lv_context* c, int in_sweep, int balance) {
+int lv_context_set_sweep_hubs(lv_context* c, int in_sweep) {
   if (!c) return LV_ERR_INVALID;
   c->sweep_hubs = in_sweep ? 1 : 0;
-  c->balance = balance ? 1 : 0;
   return LV_OK;
 }
 
@@ -2502,6 +2515,18 @@ void lv_dgraph_destroy(lv_dgraph* g) {
   (void)hipFree(g->tiers.ids);
   (void)hipFree(g->hubs.ids);
   delete g;
+}
+uint64_t lv_dgraph_device_bytes(const lv_dgraph* g) {
+  if (!g) return 0;
+  const uint64_t w = g->f64 ? 8 : 4;
+  return (g->S + 1) * 8 + g->S * 4 + g->E * (4 + w) + (g->ex ? g->S : 0) + 2 * std::max<uint64_t>(g->S, 1) * 4;
+}
+uint64_t lv_dstate_device_bytes(const lv_dstate* s) {
+  if (!s) return 0;
+  uint64_t b = s->S * (4 + 8 + 8);
+  if (s->ht_cap) b += s->S * 4 + (s->S + 1) * 8;
+  b += s->ht_slots * (4 + 4 + 8);
+  return b;
 }
 uint64_t lv_dgraph_span(const lv_dgraph* g) { return g ? g->S : 0; }
 uint64_t lv_dgraph_edges(const lv_dgraph* g) { return g ? g->live : 0; }

@@ -286,13 +286,9 @@ void k_ht_batch(const uint32_t* du, const uint32_t* dv, const float* dw, uint64_
 // table: class 0 up to kSmallHubMax edges, class 1 up to kMidHubMax.
 void k_hub_mid(const MoveArgs& a, int cls, hipStream_t st);
 
-// Sweep chunks balanced by work: `streams` contiguous ranges of the ids
-// [0, S), each with about the same sum of (degree + kVisitWeight); chunk k is
-// ids [ctab[2k], ctab[2k + 1]).  k_visit_work writes each id's work (S + 1
-// entries, the last 0); k_balanced_chunks cuts its exclusive prefix pre.
-constexpr uint64_t kVisitWeight = 8;
-void k_visit_work(const uint32_t* deg, uint64_t S, uint64_t* w, hipStream_t st);
-void k_balanced_chunks(const uint64_t* pre, uint64_t S, uint64_t streams, uint64_t* ctab, hipStream_t st);
+// *out = the first id u with off[u] >= off[S] / share (a binary search over
+// the offsets, S + 1 entries).
+void k_offset_bound(const uint64_t* off, uint64_t S, uint64_t share, uint64_t* out, hipStream_t st);
 // caps[i] = nextpow2(2 deg[ids[i]]) for i < n, caps[n] = 0.
 void k_list_caps(const uint32_t* ids, uint64_t n, const uint32_t* deg, uint64_t* caps, hipStream_t st);
 

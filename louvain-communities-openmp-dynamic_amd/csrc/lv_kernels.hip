@@ -18,7 +18,8 @@
 // Built with -ffp-contract=off: deltaModularity must evaluate without fused
 // multiply-adds to match the reference's x86-64 arithmetic bit for bit.
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
 
 #include <algorithm>
 
@@ -734,7 +735,10 @@ __device__ __forceinline__ void table_best(const MoveArgs& a, uint32_t cu, doubl
   }
 }
 
-enum { MOVE_LDS = 0, MOVE_HBM = 1, MOVE_HYBRID = 2 };
+// MOVE_HYBRID_X: MOVE_HYBRID with validated moves and hubs visited in the
+// sweep through their own tables (a separate instantiation: the extra state
+// costs the plain kernel a wave per SIMD of occupancy).
+enum { MOVE_LDS = 0, MOVE_HBM = 1, MOVE_HYBRID = 2, MOVE_HYBRID_X = 3 };
 
 // louvainMoveOmpW body (inc/louvain.hxx:595-605): for each affected vertex,
 // sum edge weights per neighbour community (self-loops skipped), choose
@@ -755,8 +759,10 @@ enum { MOVE_LDS = 0, MOVE_HBM = 1, MOVE_HYBRID = 2 };
 //       MOVE_HYBRID one wave per stream: LDS table when 2 deg <= CAP, else the
 //                   block's HBM region of gcap slots; vertices with degree above
 //                   hub_min are left to the MOVE_HBM pre-pass.
-template <int BLOCK, int G, int CAP, class WT, int MODE>
+template <int BLOCK, int G, int CAP, class WT, int MODE_>
 __global__ __launch_bounds__(BLOCK) void move_kernel(MoveArgs a) {
+  constexpr bool EXTRAS = MODE_ == MOVE_HYBRID_X;
+  constexpr int MODE = EXTRAS ? MOVE_HYBRID : MODE_;
   constexpr int GPB = BLOCK / G;
   constexpr int LCAP = MODE == MOVE_HBM ? 1 : CAP;
   __shared__ uint32_t s_key[GPB * LCAP];
@@ -840,7 +846,7 @@ __global__ __launch_bounds__(BLOCK) void move_kernel(MoveArgs a) {
         uint64_t want = hbm ? 2 : 16;
         while (want < 2 * (uint64_t)d) want <<= 1;
         cap = hbm ? want : (want < CAP ? want : CAP);
-        if constexpr (MODE == MOVE_HYBRID) {
+        if constexpr (EXTRAS) {
           if (a.xids && d > kHubMin) {  // a hub in the sweep: its own table
             uint64_t lo = 0, hi = a.nx;
             while (lo < hi) {
@@ -889,7 +895,7 @@ __global__ __launch_bounds__(BLOCK) void move_kernel(MoveArgs a) {
         }
         if constexpr (G <= 64) group_best_shfl<G>(be, bp, bc);
         else block_best<BLOCK>(be, bp, bc);
-        if constexpr (MODE == MOVE_HYBRID && G == 64) {
+        if constexpr (EXTRAS && G == 64) {
           // (the group is the wave: the redo decision is uniform over it)
           const bool check = a.validate && act && !a.choose_only && attempt < kRedo && bp != kNoPos &&
                              bc != vertex_zero(a, u);
@@ -2581,32 +2587,16 @@ __global__ __launch_bounds__(kBlock) void modularity_sum_kernel(const double* ci
 }  // namespace
 
 // ------------------------------------------------------------ launchers
-// Work of each id for the balanced sweep chunks (S + 1 entries, the last 0).
-__global__ void visit_work_kernel(const uint32_t* deg, uint64_t S, uint64_t* w) {
-  for (uint64_t u = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; u <= S; u += (uint64_t)gridDim.x * blockDim.x)
-    w[u] = u < S ? (uint64_t)deg[u] + kVisitWeight : 0;
-}
-
-// Chunk k of `streams` starts at the first id whose exclusive work prefix
-// reaches k / streams of the total.
-__device__ __forceinline__ uint64_t work_bound(const uint64_t* pre, uint64_t S, uint64_t k, uint64_t streams) {
-  const uint64_t total = pre[S];
-  const uint64_t target = (uint64_t)((unsigned __int128)total * k / streams);
+__global__ void offset_bound_kernel(const uint64_t* off, uint64_t S, uint64_t share, uint64_t* out) {
+  if (blockIdx.x || threadIdx.x) return;
+  const uint64_t target = off[S] / share;
   uint64_t lo = 0, hi = S;
   while (lo < hi) {
     const uint64_t mid = (lo + hi) / 2;
-    if (pre[mid] < target) lo = mid + 1;
+    if (off[mid] < target) lo = mid + 1;
     else hi = mid;
   }
-  return k >= streams ? S : lo;
-}
-
-__global__ void balanced_chunks_kernel(const uint64_t* pre, uint64_t S, uint64_t streams, uint64_t* ctab) {
-  for (uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; k < streams;
-       k += (uint64_t)gridDim.x * blockDim.x) {
-    ctab[2 * k] = work_bound(pre, S, k, streams);
-    ctab[2 * k + 1] = work_bound(pre, S, k + 1, streams);
-  }
+  *out = lo;
 }
 
 __global__ void list_caps_kernel(const uint32_t* ids, uint64_t n, const uint32_t* deg, uint64_t* caps) {
@@ -2730,19 +2720,23 @@ static void move_tier(const MoveArgs& a, int tier, hipStream_t st) {
       // round_up(gblocks, 4) groups), each still given at most one chunk per
       // round, so no more than `streams` walk at once.  Chunks handed out
       // on demand go a block at a time: one wave per block then.
-      if (!a.chunk && groups >= 64)
+      if (a.validate || a.xids) {
+        if (!a.chunk && groups >= 64)
+          LV_LAUNCH((move_kernel<256, 64, kHybridLds, WT, MOVE_HYBRID_X>), grid_for(groups, 4, (a.gblocks + 3) / 4),
+                    256, st, a);
+        else
+          LV_LAUNCH((move_kernel<64, 64, kHybridLds, WT, MOVE_HYBRID_X>), grid_for(groups, 1, a.gblocks), 64, st, a);
+      } else if (!a.chunk && groups >= 64) {
         LV_LAUNCH((move_kernel<256, 64, kHybridLds, WT, MOVE_HYBRID>), grid_for(groups, 4, (a.gblocks + 3) / 4), 256,
                   st, a);
-      else
+      } else {
         LV_LAUNCH((move_kernel<64, 64, kHybridLds, WT, MOVE_HYBRID>), grid_for(groups, 1, a.gblocks), 64, st, a);
+      }
       break;
   }
 }
-void k_visit_work(const uint32_t* deg, uint64_t S, uint64_t* w, hipStream_t st) {
-  LV_LAUNCH(visit_work_kernel, grid_for(S + 1, kBlock * 4), kBlock, st, deg, S, w);
-}
-void k_balanced_chunks(const uint64_t* pre, uint64_t S, uint64_t streams, uint64_t* ctab, hipStream_t st) {
-  if (streams) LV_LAUNCH(balanced_chunks_kernel, grid_for(streams, kBlock), kBlock, st, pre, S, streams, ctab);
+void k_offset_bound(const uint64_t* off, uint64_t S, uint64_t share, uint64_t* out, hipStream_t st) {
+  LV_LAUNCH(offset_bound_kernel, 1, 64, st, off, S, share, out);
 }
 void k_list_caps(const uint32_t* ids, uint64_t n, const uint32_t* deg, uint64_t* caps, hipStream_t st) {
   LV_LAUNCH(list_caps_kernel, grid_for(n + 1, kBlock), kBlock, st, ids, n, deg, caps);
@@ -2874,7 +2868,7 @@ void k_member_keys(GView g, const uint32_t* vcom, uint32_t C, uint32_t* key, uin
 }
 hipError_t sort_pairs_u32(const uint32_t* kin, uint32_t* kout, const uint32_t* vin, uint32_t* vout, uint64_t n,
                           int end_bit, void* tmp, size_t& tmp_bytes, hipStream_t st) {
-  return hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, kin, kout, vin, vout, (int)n, 0, end_bit, st);
+  return rocprim::radix_sort_pairs(tmp, tmp_bytes, kin, kout, vin, vout, (size_t)n, 0u, (unsigned)end_bit, st);
 }
 void k_t4_sizes(const uint64_t* yoff, const uint32_t* list, uint64_t n, uint64_t C, uint64_t chunk, uint64_t* caps,
                 uint64_t* chunks, hipStream_t st) {
@@ -2915,10 +2909,10 @@ void k_modularity_sum(const double* cin, const double* ctot, uint64_t n, double 
 }
 
 hipError_t scan_u32(const uint32_t* in, uint32_t* out, uint64_t n, void* tmp, size_t& tmp_bytes, hipStream_t st) {
-  return hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, in, out, (int)n, st);
+  return rocprim::exclusive_scan(tmp, tmp_bytes, in, out, 0u, (size_t)n, rocprim::plus<uint32_t>(), st);
 }
 hipError_t scan_u64(const uint64_t* in, uint64_t* out, uint64_t n, void* tmp, size_t& tmp_bytes, hipStream_t st) {
-  return hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, in, out, (int)n, st);
+  return rocprim::exclusive_scan(tmp, tmp_bytes, in, out, (uint64_t)0, (size_t)n, rocprim::plus<uint64_t>(), st);
 }
 
 }  // namespace lv

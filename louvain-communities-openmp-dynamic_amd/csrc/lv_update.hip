@@ -15,7 +15,8 @@
 // degree fits its capacity; otherwise every row is moved to a new CSR with
 // slack.
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_select.hpp>
 
 #include "lv_internal.hpp"
 
@@ -414,14 +415,14 @@ void k_tidy_keys(GView g, const uint32_t* u, const uint32_t* v, uint64_t n, bool
 }
 hipError_t sort_pairs_u64(const uint64_t* kin, uint64_t* kout, const uint32_t* vin, uint32_t* vout, uint64_t n,
                           void* tmp, size_t& bytes, hipStream_t st) {
-  return hipcub::DeviceRadixSort::SortPairs(tmp, bytes, kin, kout, vin, vout, (int)n, 0, 64, st);
+  return rocprim::radix_sort_pairs(tmp, bytes, kin, kout, vin, vout, (size_t)n, 0u, 64u, st);
 }
 void k_tidy_flags(const uint64_t* key, uint64_t n, uint8_t* flag, hipStream_t st) {
   if (n) LV_UPD_LAUNCH(tidy_flags_kernel, blocks_for(n, kB), kB, st, key, n, flag);
 }
 hipError_t select_flagged_u32(const uint32_t* in, const uint8_t* flags, uint32_t* out, uint32_t* count, uint64_t n,
                               void* tmp, size_t& bytes, hipStream_t st) {
-  return hipcub::DeviceSelect::Flagged(tmp, bytes, in, flags, out, count, (int)n, st);
+  return rocprim::select(tmp, bytes, in, flags, out, count, (size_t)n, st);
 }
 void k_tidy_gather(const uint32_t* sel, uint64_t m, const uint32_t* u, const uint32_t* v, const float* w,
                    uint32_t* ou, uint32_t* ov, float* ow, hipStream_t st) {

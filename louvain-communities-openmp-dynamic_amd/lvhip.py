@@ -30,6 +30,8 @@ LV_STATIC, LV_NAIVE_DYNAMIC, LV_DELTA_SCREENING, LV_DYNAMIC_FRONTIER = 0, 1, 2, 
 APPROACHES = {"static": LV_STATIC, "naive-dynamic": LV_NAIVE_DYNAMIC,
               "delta-screening": LV_DELTA_SCREENING, "dynamic-frontier": LV_DYNAMIC_FRONTIER}
 PATH_SWEEP, PATH_FRONTIER, PATH_WORKLIST = 0, 1, 2
+LV_OK, LV_ERR_INVALID, LV_ERR_NOMEM, LV_ERR_HIP, LV_ERR_UNSORTED, LV_ERR_NODEVICE, LV_ERR_LIMIT, LV_ERR_PENDING = (
+    0, -1, -2, -3, -4, -5, -6, -7)
 
 # Every symbol include/louvain_hip.h declares.
 EXPORTS = [
@@ -45,6 +47,10 @@ EXPORTS = [
     "lv_run_batched",
     "lv_vertex_weights", "lv_update_weights", "lv_affected_frontier", "lv_affected_delta_screening",
     "lv_choose_community", "lv_renumber", "lv_aggregate",
+    "lv_dgraph_device_bytes", "lv_dstate_device_bytes",
+    "lv_server_options_default", "lv_server_create", "lv_server_destroy", "lv_server_add_graph",
+    "lv_server_submit", "lv_server_poll", "lv_server_drain", "lv_server_graph_state", "lv_server_get_stats",
+    "lv_server_last_error", "lv_server_now_ms",
 ]
 
 
@@ -83,6 +89,23 @@ class _Result(C.Structure):
                 ("aggregation_time", C.c_float), ("affected_vertices", C.c_uint64)]
 
 
+class _ServerOptions(C.Structure):
+    _fields_ = [("device", C.c_int32), ("slots", C.c_int32), ("memory_budget", C.c_uint64), ("options", _Options),
+                ("path", C.c_int32), ("modularity", C.c_int32), ("keep_states", C.c_int32)]
+
+
+class _Completion(C.Structure):
+    _fields_ = [("ticket", C.c_uint64), ("graph_id", C.c_uint32), ("status", C.c_int32),
+                ("submitted_ms", C.c_double), ("started_ms", C.c_double), ("done_ms", C.c_double),
+                ("result", _Result), ("modularity", C.c_double), ("deletions", C.c_uint64),
+                ("insertions", C.c_uint64)]
+
+
+class _ServerStats(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in ("submitted", "completed", "failed", "resident_bytes", "workspace_bytes",
+                                          "inflight_bytes", "peak_bytes")]
+
+
 def load_library(path=LIB_PATH):
     if not os.path.exists(path):
         raise ImportError(f"{path} is not built: run `python __graft_entry__.py build` (hipcc, gfx950)")
@@ -93,7 +116,12 @@ def load_library(path=LIB_PATH):
     lib.lv_dgraph_edges.restype = C.c_uint64
     lib.lv_dgraph_capacity.restype = C.c_uint64
     lib.lv_context_workspace_bytes.restype = C.c_uint64
-    for name in ("lv_context_destroy", "lv_dgraph_destroy", "lv_dbatch_destroy", "lv_dstate_destroy"):
+    lib.lv_dgraph_device_bytes.restype = C.c_uint64
+    lib.lv_dstate_device_bytes.restype = C.c_uint64
+    lib.lv_server_last_error.restype = C.c_char_p
+    lib.lv_server_now_ms.restype = C.c_double
+    for name in ("lv_context_destroy", "lv_dgraph_destroy", "lv_dbatch_destroy", "lv_dstate_destroy",
+                 "lv_server_destroy", "lv_server_options_default"):
         getattr(lib, name).restype = None
     return lib
 
@@ -211,12 +239,11 @@ class Context:
     def set_lead_ids(self, ids=0):
         self._check(self.lib.lv_context_set_lead_ids(self.h, C.c_uint64(ids)), "lv_context_set_lead_ids")
 
-    def set_sweep_hubs(self, in_sweep=True, balance=True):
-        self._check(self.lib.lv_context_set_sweep_hubs(self.h, int(bool(in_sweep)), int(bool(balance))),
-                    "lv_context_set_sweep_hubs")
+    def set_sweep_hubs(self, in_sweep=False):
+        self._check(self.lib.lv_context_set_sweep_hubs(self.h, int(bool(in_sweep))), "lv_context_set_sweep_hubs")
 
-    def set_validate(self, mode=1):
-        """Validated moves: 0 off, 1 sweeps (default), 2 sweeps and worklists."""
+    def set_validate(self, mode=0):
+        """Validated moves: 0 off (default), 1 sweeps, 2 sweeps and worklists."""
         self._check(self.lib.lv_context_set_validate(self.h, int(mode)), "lv_context_set_validate")
 
     def set_workspace_limit(self, nbytes=0):
@@ -226,7 +253,7 @@ class Context:
     def workspace_bytes(self):
         return int(self.lib.lv_context_workspace_bytes(self.h))
 
-    def set_pass_schedule(self, lead_iterations=0, lead_streams=16, lead_chunk=2048, late_streams=0, late_chunk=0):
+    def set_pass_schedule(self, lead_iterations=1, lead_streams=16, lead_chunk=2048, late_streams=0, late_chunk=0):
         self._check(self.lib.lv_context_set_pass_schedule(self.h, int(lead_iterations), C.c_uint64(lead_streams),
                                                           C.c_uint64(lead_chunk), C.c_uint64(late_streams),
                                                           C.c_uint64(late_chunk)), "lv_context_set_pass_schedule")

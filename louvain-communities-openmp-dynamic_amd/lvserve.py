@@ -1,21 +1,24 @@
 """Serving loop: resident graphs, DF on update batches as they arrive.
 
-Graphs, memberships, K and Sigma_tot stay in HBM (lv_dgraph / lv_dstate).
-Each request is a raw batch (deletions, insertions) for one graph; the
-worker that takes it tidies and applies it on the device (lv_dgraph_update)
-and runs Dynamic Frontier Louvain from the graph's current state (lv_run),
-the per-batch step of main.cxx:104-135.
+A thin host-side layer over the library's lv_server_* C ABI
+(include/louvain_hip.h).  Graphs, memberships, K, Sigma_tot and the hub
+neighbour-community tables stay in HBM.  Each request is a raw batch
+(deletions, insertions) for one graph; the slot that takes it tidies and
+applies it on the device (lv_dgraph_update) and runs Dynamic Frontier
+Louvain from the graph's current state (lv_run), the per-batch step of
+main.cxx:104-135.
 
-Continuous batching: a fixed set of slots (one lv_context, hence one HIP
-stream, per slot) take requests in arrival order as they become free, so
-requests for different graphs run on the device at the same time.  Requests
-for one graph run one at a time, in arrival order (each batch updates the
-graph the next one applies to).  Admission is by device-memory footprint: a
-request enters a slot only while the bytes of the batches in flight stay
-within the budget.
+Continuous batching happens inside the library: a fixed set of slots (one
+lv_context, hence one HIP stream and workspace, per slot, each on a worker
+thread) take requests in submission order as they become free, so requests
+for different graphs run on the device at the same time; requests for one
+graph run one at a time, in submission order (each batch updates the graph
+the next one applies to).  Admission is by device memory: a request starts
+only while the resident graphs and states, the slots' workspaces and the
+batches in flight fit the budget, or when nothing else runs.  Adding a
+graph runs a warm-up on every slot, so workspaces are sized before serving.
 """
-import heapq
-import threading
+import ctypes as C
 import time
 
 import numpy as np
@@ -25,17 +28,16 @@ import lvhip
 
 class Request:
     __slots__ = ("rid", "graph", "deletions", "insertions", "arrival", "start", "done", "result", "modularity",
-                 "bytes", "error", "state")
+                 "bytes", "error", "state", "ticket")
 
     def __init__(self, rid, graph, deletions, insertions, arrival):
         self.rid, self.graph = rid, graph
         self.deletions, self.insertions = deletions, insertions
         self.arrival = arrival
-        self.start = self.done = None
+        self.start = self.done = self.ticket = None
         self.result = self.modularity = self.error = self.state = None
-        # device bytes while in flight: the raw batch and its tidy copy
-        n = len(deletions["u"]) + len(insertions["u"])
-        self.bytes = 2 * 12 * n
+        # host bytes of the raw batch (u, v, w per entry)
+        self.bytes = 12 * (len(deletions["u"]) + len(insertions["u"]))
 
     @property
     def latency(self):
@@ -43,120 +45,127 @@ class Request:
 
 
 class Server:
-    """Resident graphs served by `slots` workers.  graphs maps a name to a
-    host graph (numpy CSR dict or anything with view()); each starts from a
-    Static Louvain run on the device unless `states` gives (q, K, Sigma_tot)."""
+    """Resident graphs served by `slots` workers of an lv_server.  graphs
+    maps a name to a host graph (numpy CSR dict or anything with view());
+    each starts from a Static Louvain run on the device unless `states`
+    gives (q, K, Sigma_tot).  memory_budget: device bytes for graphs,
+    states, workspaces and batches in flight (None: no limit)."""
 
-    def __init__(self, graphs, slots=4, memory_budget=1 << 30, options=None, device=0, path=None,
+    def __init__(self, graphs, slots=4, memory_budget=None, options=None, device=0, path=None,
                  modularity=False, states=None, keep_states=False):
-        self.slots = [lvhip.Context(device) for _ in range(slots)]
+        self.lib = lvhip._LIB
+        o = lvhip._ServerOptions()
+        self.lib.lv_server_options_default(C.byref(o))
+        o.device, o.slots = int(device), int(slots)
+        o.memory_budget = int(memory_budget or 0)
+        o.options = (options or lvhip.LouvainOptions())._c()
         if path is not None:
-            for c in self.slots:
-                c.set_path(path)
-        self.options = options or lvhip.LouvainOptions()
-        self.budget = memory_budget
-        self.want_modularity = modularity
+            o.path = int(path)
+        o.modularity, o.keep_states = int(bool(modularity)), int(bool(keep_states))
         self.keep_states = keep_states
-        ctx = self.slots[0]
-        self.graphs, self.states = {}, {}
+        self.h = C.c_void_p()
+        rc = self.lib.lv_server_create(C.byref(o), C.byref(self.h))
+        if rc != 0:
+            raise lvhip.LvError(rc, "lv_server_create: " + self.lib.lv_status_string(rc).decode())
+        self.ids, self.spans = {}, {}
         for name, g in graphs.items():
             view = g.view() if hasattr(g, "view") else g
-            dg = ctx.device_graph(view)
-            st = ctx.device_state(dg.span)
-            if states and name in states:
-                st.upload(*states[name])
+            ga = lvhip._GraphArgs(view)
+            gid = C.c_uint32()
+            st = states.get(name) if states else None
+            if st is not None:
+                q, v, c = (lvhip._arr(st[0], np.uint32), lvhip._arr(st[1], np.float64), lvhip._arr(st[2], np.float64))
+                rc = self.lib.lv_server_add_graph(self.h, C.byref(ga.c), lvhip._p(q), lvhip._p(v), lvhip._p(c),
+                                                  C.byref(gid))
             else:
-                ctx.run(lvhip.LV_STATIC, dg, None, st, self.options)
-            self.graphs[name], self.states[name] = dg, st
-        self._lock = threading.Condition()
+                rc = self.lib.lv_server_add_graph(self.h, C.byref(ga.c), None, None, None, C.byref(gid))
+            self._check(rc, f"lv_server_add_graph({name})")
+            self.ids[name], self.spans[name] = gid.value, ga.S
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise lvhip.LvError(rc, f"{what}: {self.lib.lv_server_last_error(self.h).decode()}")
 
     def close(self):
-        for d in list(self.graphs.values()) + list(self.states.values()):
-            d.close()
-        for c in self.slots:
-            c.close()
+        if self.h:
+            self.lib.lv_server_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def now(self):
+        """Seconds on the server's clock."""
+        return self.lib.lv_server_now_ms(self.h) * 1e-3
+
+    def submit(self, r):
+        """Queue one request (its batch is copied); returns its ticket."""
+        d, i = lvhip._BatchArgs(r.deletions), lvhip._BatchArgs(r.insertions)
+        t = C.c_uint64()
+        self._check(self.lib.lv_server_submit(self.h, self.ids[r.graph], C.byref(d.c), C.byref(i.c), C.byref(t)),
+                    "lv_server_submit")
+        r.ticket = t.value
+        return r.ticket
+
+    def poll(self, r, timeout_ms=-1, t0=0.0):
+        """Wait up to timeout_ms (< 0: until done) for a submitted request and
+        fill its start / done times (seconds since t0 on the server's clock),
+        result, modularity, state and error.  False if still pending."""
+        out = lvhip._Completion()
+        state = None
+        if self.keep_states:
+            S = self.spans[r.graph]
+            state = (np.zeros(S, np.uint32), np.zeros(S, np.float64), np.zeros(S, np.float64))
+            out.result.membership, out.result.vertex_weight, out.result.community_weight = (
+                state[0].ctypes.data, state[1].ctypes.data, state[2].ctypes.data)
+        rc = self.lib.lv_server_poll(self.h, C.c_uint64(r.ticket), int(timeout_ms), C.byref(out))
+        if rc == lvhip.LV_ERR_PENDING:
+            return False
+        self._check(rc, "lv_server_poll")
+        r.start, r.done = out.started_ms * 1e-3 - t0, out.done_ms * 1e-3 - t0
+        if out.status != 0:
+            r.error = lvhip.LvError(out.status, self.lib.lv_server_last_error(self.h).decode())
+            return True
+        r.result = lvhip._to_result(out.result, *(state or (None, None, None)))
+        r.modularity = out.modularity if not np.isnan(out.modularity) else None
+        r.state = state
+        return True
 
     def run(self, requests, realtime=True):
         """Serve the requests (arrival times in seconds from the start, in
-        order).  With realtime, a request is released at its arrival time;
-        otherwise all are released at once.  Returns the requests, each with
+        order).  With realtime, a request is submitted at its arrival time;
+        otherwise all are submitted at once.  Returns the requests, each with
         start / done times (seconds from the start), result and, if asked,
-        modularity."""
+        modularity and state."""
         pending = sorted(requests, key=lambda r: (r.arrival, r.rid))
-        waiting = []             # released, not started: (arrival, rid, request)
-        busy_graphs = set()
-        in_flight = [0]
-        state = {"released": 0}
-        t0 = time.perf_counter()
-        now = lambda: time.perf_counter() - t0
-        cv = self._lock
-
-        def take():
-            # oldest released request whose graph is idle and whose batch fits the budget
-            skipped = []
-            got = None
-            while waiting:
-                item = heapq.heappop(waiting)
-                r = item[2]
-                if r.graph in busy_graphs:
-                    skipped.append(item)
-                    continue
-                if in_flight[0] and in_flight[0] + r.bytes > self.budget:
-                    skipped.append(item)
-                    break
-                got = r
-                break
-            for it in skipped:
-                heapq.heappush(waiting, it)
-            return got
-
-        def worker(ctx):
-            while True:
-                with cv:
-                    while True:
-                        r = take()
-                        if r is not None:
-                            break
-                        if state["released"] == len(pending) and not waiting:
-                            return  # everything is taken; the rest is running elsewhere
-                        cv.wait(0.05)
-                    busy_graphs.add(r.graph)
-                    in_flight[0] += r.bytes
-                r.start = now()
-                try:
-                    dg, st = self.graphs[r.graph], self.states[r.graph]
-                    tb = dg.update(r.deletions, r.insertions, ctx=ctx)
-                    r.result = ctx.run(lvhip.LV_DYNAMIC_FRONTIER, dg, tb, st, self.options)
-                    tb.close()
-                    if self.want_modularity:
-                        r.modularity = ctx.dmodularity(dg, st)
-                    if self.keep_states:
-                        r.state = st.download(ctx)
-                except Exception as e:  # reported per request
-                    r.error = e
-                r.done = now()
-                with cv:
-                    busy_graphs.discard(r.graph)
-                    in_flight[0] -= r.bytes
-                    cv.notify_all()
-
-        threads = [threading.Thread(target=worker, args=(c,), daemon=True) for c in self.slots]
-        for t in threads:
-            t.start()
+        t0 = self.now()
         for r in pending:
             if realtime:
-                delay = r.arrival - now()
+                delay = t0 + r.arrival - self.now()
                 if delay > 0:
                     time.sleep(delay)
             else:
-                r.arrival = now()
-            with cv:
-                heapq.heappush(waiting, (r.arrival, r.rid, r))
-                state["released"] += 1
-                cv.notify_all()
-        for t in threads:
-            t.join()
+                r.arrival = self.now() - t0
+            self.submit(r)
+        for r in pending:
+            self.poll(r, -1, t0)
         return pending
+
+    def graph_state(self, name):
+        """(membership, K, Sigma_tot) of a graph once its requests are done."""
+        S = self.spans[name]
+        q, v, c = np.zeros(S, np.uint32), np.zeros(S, np.float64), np.zeros(S, np.float64)
+        self._check(self.lib.lv_server_graph_state(self.h, self.ids[name], lvhip._p(q), lvhip._p(v), lvhip._p(c)),
+                    "lv_server_graph_state")
+        return q, v, c
+
+    def stats(self):
+        st = lvhip._ServerStats()
+        self._check(self.lib.lv_server_get_stats(self.h, C.byref(st)), "lv_server_get_stats")
+        return {f: getattr(st, f) for f, _ in lvhip._ServerStats._fields_}
 
 
 def poisson_arrivals(n, rate, seed):

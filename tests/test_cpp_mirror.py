@@ -100,8 +100,8 @@ def test_mirror_end_to_end(tmp_path, oracle, name):
     res = _read_output(tmp_path / "out.bin", S)
     x, y, d, i = F.graph(f, "x"), F.graph(f, "y"), F.batch(f, "del"), F.batch(f, "ins")
     q = oracle.modularity(x, res[0]["membership"])
-    tol = 0.05 if name.startswith("tiny") else 0.02
-    assert q >= float(f["static_Q"]) - tol
+    # at most 2048 ids: one stream per level, the reference's result at one thread
+    assert q >= float(f["static_Q"]) - 1e-3
     assert np.array_equal(res[0]["vtot"], f["static_vtot"])
     o = lvhip.LouvainOptions()
     for k, approach in ((1, "naive-dynamic"), (2, "delta-screening"), (3, "dynamic-frontier")):

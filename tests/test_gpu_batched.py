@@ -110,6 +110,7 @@ def test_batched_default_schedule_quality(ctx, oracle, approach):
         ref = float(f[key])
         print(name, approach, f"Q batched {q:.6f} reference {ref:.6f}", flush=True)
         if approach == "static":
-            assert q >= ref - (0.05 if name.startswith("tiny") else 0.02), name
+            # each graph has at most 2048 ids: one stream, as it runs alone
+            assert q >= ref - 1e-3, name
         else:
             assert abs(q - ref) <= 1e-3, name
