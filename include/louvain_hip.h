@@ -308,6 +308,13 @@ int lv_run_batched(lv_context* ctx, int approach, const lv_dgraph* g, const lv_d
 /* Modularity of a device state's membership on a device graph
  * (modularityByOmp, inc/properties.hxx:205-233). */
 int lv_dmodularity(lv_context* ctx, const lv_dgraph* g, const lv_dstate* s, double resolution, double* out);
+/* lv_dmodularity with no device allocation, for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops).  No
+ * counterpart in the reference.  Spans up to 2^21 vertices (LV_ERR_LIMIT
+ * above); its scratch is a static device array of 2^22 + 32 doubles (32 MiB)
+ * that the library holds on every GPU it runs on.  Calls that overlap on one
+ * GPU share that array and race. */
+int lv_dmodularity_noalloc(lv_context* ctx, const lv_dgraph* g, const lv_dstate* s, double resolution, double* out);
 /* Diagnostic: rebuild the state's hub neighbour-community tables (see
  * lv_context_set_hub_tables) from graph g and the state's membership and
  * compare; *tables = tables compared, *mismatches = those that differ (0 and 0

@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <map>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -44,11 +45,41 @@ struct Buf {
   size_t bytes = 0;
 };
 
+// Scratch of lv_dmodularity_noalloc: per-community internal weight and
+// total weight (kNoAllocSpan doubles each), then the result.  The library
+// holds this array on every GPU it runs on until the process ends.  Host
+// code never passes the array's name to a kernel or a copy: it uses the
+// device address hipGetSymbolAddress gives for the current device
+// (scratch_address), looked up once per device.
+constexpr uint64_t kNoAllocSpan = 1ull << 21;
+constexpr uint64_t kNoAllocDoubles = 2 * kNoAllocSpan + 32;
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of scratch on `device`, which must be current.
+hipError_t scratch_address(int device, double** out) {
+  *out = nullptr;
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[device] = static_cast<double*>(p);
+  }
+  *out = g_scratch_ptr[device];
+  return hipSuccess;
+}
+
 }  // namespace
 
 struct lv_context {
   int device = 0;
   hipStream_t st = nullptr;
+  double* scratch = nullptr;  // the device's lv_dmodularity_noalloc scratch (its address there)
   std::string err;
   int path = 2;
   // Local-moving streams per launch: clamp(stream_fraction * live vertices of
@@ -2292,6 +2323,7 @@ int lv_context_create(int device, lv_context** out) {
   int rc = guard(c, [&] {
     HIPCHK(hipSetDevice(device));
     HIPCHK(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
+    HIPCHK(scratch_address(device, &c->scratch));
   });
   if (rc != LV_OK) {
     lv_context_destroy(c);
@@ -2697,6 +2729,33 @@ int lv_dmodularity(lv_context* c, const lv_dgraph* g, const lv_dstate* s, double
     k_modularity_parts(g->view(), g->f64, s->memb, cin, ct, c->st);
     k_modularity_sum(cin, ct, g->S, g->M2 / 2, R, o, c->st);
     *out = c->read1(o);
+  });
+}
+
+// Modularity as lv_dmodularity computes it (the same kernels), with no
+// device allocation: the scratch is the library's static device array of
+// this GPU.  Calls on one GPU share that array, so calls that overlap on one
+// GPU race; each call clears what it uses first.
+int lv_dmodularity_noalloc(lv_context* c, const lv_dgraph* g, const lv_dstate* s, double R, double* out) {
+  if (!c || !g || !s || !out) return LV_ERR_INVALID;
+  return guard(c, [&] {
+    require(!g->broken, LV_ERR_INVALID, "graph left inconsistent by a failed update");
+    require(s->S == g->S, LV_ERR_INVALID, "state span differs from graph span");
+    require(g->S <= kNoAllocSpan, LV_ERR_LIMIT,
+            "lv_dmodularity_noalloc: span above its limit of 2^21 vertices (use lv_dmodularity)");
+    require(c->scratch != nullptr, LV_ERR_HIP, "no device scratch address");
+    HIPCHK(hipSetDevice(c->device));
+    double* cin = c->scratch;
+    double* ct = cin + kNoAllocSpan;
+    double* o = ct + kNoAllocSpan;
+    k_fill_f64(cin, 0, g->S, c->st);
+    k_fill_f64(ct, 0, g->S, c->st);
+    k_fill_f64(o, 0, 1, c->st);
+    k_modularity_parts(g->view(), g->f64, s->memb, cin, ct, c->st);
+    k_modularity_sum(cin, ct, g->S, g->M2 / 2, R, o, c->st);
+    HIPCHK(hipMemcpyAsync(out, o, sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    c->check();
   });
 }
 

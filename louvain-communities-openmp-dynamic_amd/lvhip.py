@@ -44,7 +44,7 @@ EXPORTS = [
     "lv_dbatch_create", "lv_dbatch_destroy", "lv_dstate_create", "lv_dstate_destroy", "lv_dstate_upload",
     "lv_dstate_download", "lv_dstate_copy", "lv_run", "lv_dmodularity", "lv_edge_weight", "lv_modularity",
     "lv_dgraph_capacity", "lv_dgraph_update", "lv_dgraph_download", "lv_dbatch_sizes", "lv_dbatch_download",
-    "lv_run_batched",
+    "lv_run_batched", "lv_dmodularity_noalloc",
     "lv_vertex_weights", "lv_update_weights", "lv_affected_frontier", "lv_affected_delta_screening",
     "lv_choose_community", "lv_renumber", "lv_aggregate",
     "lv_dgraph_device_bytes", "lv_dstate_device_bytes",
@@ -447,6 +447,13 @@ class Context:
         self._check(self.lib.lv_run_batched(self.h, int(approach), dgraph.h, dbatch.h if dbatch else None, dstate.h,
                                             _p(voff), C.c_uint64(G), C.byref(oc), res), "lv_run_batched")
         return [_to_result(res[k], *keep[k]) for k in range(G)]
+
+    def dmodularity_noalloc(self, dgraph, dstate, resolution=1.0):
+        """lv_dmodularity_noalloc: no device allocation, spans up to 2^21."""
+        out = C.c_double(0)
+        self._check(self.lib.lv_dmodularity_noalloc(self.h, dgraph.h, dstate.h, C.c_double(resolution),
+                                                    C.byref(out)), "lv_dmodularity_noalloc")
+        return out.value
 
     def dmodularity(self, dgraph, dstate, resolution=1.0):
         out = C.c_double()
