@@ -134,6 +134,14 @@ int lv_context_set_path(lv_context* ctx, int path);
  * Defaults: 0.002, 1, 2^20, 0, 2. */
 int lv_context_set_schedule(lv_context* ctx, double stream_fraction, uint64_t min_streams, uint64_t max_streams,
                             uint64_t chunk, int order);
+/* Worklist sub-rounds (the dynamic approaches' first pass, path 2): the
+ * id-order kernel runs at most `streams` lane groups (1..65536), and with
+ * interleave the sub-round's queue (ascending ids) is dealt out round-robin,
+ * group g taking entries g, g + N, g + 2N, ... in that order, instead of in
+ * N consecutive parts.  A tuning knob with no counterpart in the reference.
+ * Defaults: 3072 (the waves the kernel keeps resident on the chip), 0. */
+int lv_context_set_worklist_schedule(lv_context* ctx, uint64_t streams, int interleave);
+
 /* The first `iterations` local-moving iterations of every pass only move a
  * vertex to a community with a lower id than its own (no two vertices can
  * then swap communities in one iteration).  Default 0 (off). */
@@ -287,6 +295,9 @@ void lv_dstate_destroy(lv_dstate* s);
 int lv_dstate_upload(lv_context* ctx, lv_dstate* s, const uint32_t* q, const double* qvtot, const double* qctot);
 int lv_dstate_download(lv_context* ctx, const lv_dstate* s, uint32_t* q, double* qvtot, double* qctot);
 int lv_dstate_copy(lv_context* ctx, lv_dstate* dst, const lv_dstate* src);
+/* dst = src with its hub neighbour-community tables: dst's next run carries
+ * or builds them as src's would (lv_dstate_copy leaves dst without tables). */
+int lv_dstate_clone(lv_context* ctx, lv_dstate* dst, const lv_dstate* src);
 int lv_run(lv_context* ctx, int approach, const lv_dgraph* g, const lv_dbatch* b, lv_dstate* s,
            const lv_options* o, lv_result* res);
 /* Batched form: G graphs side by side in one device graph (graph k owns
@@ -308,6 +319,53 @@ int lv_run_batched(lv_context* ctx, int approach, const lv_dgraph* g, const lv_d
 /* Modularity of a device state's membership on a device graph
  * (modularityByOmp, inc/properties.hxx:205-233). */
 int lv_dmodularity(lv_context* ctx, const lv_dgraph* g, const lv_dstate* s, double resolution, double* out);
+/* Schedule tuning -----------------------------------------------------------
+ * The degree buckets and schedules of local-moving, chosen by timing: the
+ * worklist hub degree (256 or 512: above it a vertex leaves the id-order
+ * kernel for the hub kernels), the sweep hub degree (4096 or 16384), the
+ * worklist id-order streams (1536, 3072, 6144) and queue layout, and the
+ * streams per live vertex of a level (0.001, 0.002, 0.004).  The
+ * reference's counterpart is its fixed schedule(dynamic, 2048)
+ * (inc/louvain.hxx:594); no counterpart for the tuning itself. */
+typedef struct lv_tune_choice {
+  uint32_t worklist_hub_degree;   /* lv_context_set_hub_threshold's worklist_degree */
+  uint32_t sweep_hub_degree;      /* lv_context_set_hub_threshold's sweep_degree */
+  uint64_t worklist_streams;      /* lv_context_set_worklist_schedule */
+  int32_t interleave;
+  double stream_fraction;         /* lv_context_set_schedule */
+  double ms, default_ms;          /* median host-clock ms of a run: this choice, the defaults */
+  double modularity, default_modularity;  /* after the probe run */
+  int32_t candidates;             /* schedules timed */
+} lv_tune_choice;
+/* Times the candidate schedules of `approach` (ND, DS or DF; the ones its
+ * path uses) on graph g with batch `probe` from state s, `reps` runs each
+ * (1..64; median of the host clock around lv_run), every trial on a copy of
+ * s (hub tables included: s is left as it is, and a probe made by
+ * lv_dgraph_update from the graph of s's tables carries them as a real
+ * batch would).  Keeps the fastest schedule whose modularity is within 1e-3
+ * of the defaults', and the defaults unless it beats them by more than 3%;
+ * caches it under g's shape (approach, log2 span, log2 live edges) and
+ * applies it to the context.  o may be NULL (defaults).  The trials count in
+ * the device work counters of lv_stats_get. */
+int lv_autotune(lv_context* ctx, int approach, const lv_dgraph* g, const lv_dbatch* probe, const lv_dstate* s,
+                const lv_options* o, int reps, lv_tune_choice* out);
+/* on (default): before every run the knobs take the cached choice for the
+ * run's graph shape, or the defaults, and lv_run_batched tunes a shape it
+ * has not seen (on copies of its state, before the run).  Setting a tuned
+ * knob by hand (lv_context_set_schedule's stream_fraction,
+ * lv_context_set_hub_threshold, lv_context_set_worklist_schedule,
+ * lv_context_apply_tuning) to other than its default keeps the knobs as set
+ * until lv_context_set_autotune(ctx, 1) or every such knob is back at its
+ * default.  off: the knobs stay as they are. */
+int lv_context_set_autotune(lv_context* ctx, int on);
+/* The cached choice for g's shape (graphs: 1, or the batched form's graph
+ * count); out->candidates == 0 when none is cached (out then holds the knobs
+ * the next run takes). */
+int lv_context_tuning(lv_context* ctx, int approach, const lv_dgraph* g, uint64_t graphs, lv_tune_choice* out);
+/* Cache a choice (made on another context) for g's shape. */
+int lv_context_add_tuning(lv_context* ctx, int approach, const lv_dgraph* g, const lv_tune_choice* k);
+/* Set the tuned knobs to k's (NULL: the defaults), by hand. */
+int lv_context_apply_tuning(lv_context* ctx, const lv_tune_choice* k);
 /* lv_dmodularity with no device allocation, for callers that cannot
  * allocate (code under HIP graph capture, tight serving loops).  No
  * counterpart in the reference.  Spans up to 2^21 vertices (LV_ERR_LIMIT
@@ -373,6 +431,10 @@ int lv_aggregate(lv_context* ctx, const lv_graph* g, const uint32_t* vcom, uint6
  * batches in flight (the request's own included) fit the memory budget, or
  * when nothing else is in flight.  Each slot's workspace is sized for a graph
  * when the graph is added (a warm-up run), so serving does not regrow it.
+ * With autotune, adding a graph also tunes its DF schedule (lv_autotune on a
+ * copy of the graph and state, with a random probe batch of the requests'
+ * kind: 80% insertions of new edges, 20% deletions of existing ones), and
+ * every slot runs the graph's requests under that choice.
  * No counterpart in the reference, which runs one batch at a time. */
 typedef struct lv_server lv_server;
 typedef struct lv_server_options {
@@ -383,6 +445,8 @@ typedef struct lv_server_options {
   int path;                 /* local-moving path of the slots (lv_context_set_path) [2] */
   int modularity;           /* compute each request's modularity (lv_dmodularity) [0] */
   int keep_states;          /* keep each request's resulting state for lv_server_poll [0] */
+  int autotune;             /* tune each added graph's DF schedule (lv_autotune) on a probe batch [1] */
+  double probe_fraction;    /* probe batch size, a fraction of the graph's undirected edges [1e-4] */
 } lv_server_options;
 /* A finished request.  result carries the run's scalars; when the server
  * keeps states and the caller sets result.membership / vertex_weight /
@@ -425,6 +489,9 @@ int lv_server_drain(lv_server* s);
 /* The current state of a graph (waits until the graph is idle). */
 int lv_server_graph_state(lv_server* s, uint32_t graph_id, uint32_t* q, double* qvtot, double* qctot);
 int lv_server_get_stats(lv_server* s, lv_server_stats* out);
+/* The DF schedule chosen for a graph when it was added (candidates == 0:
+ * not tuned, the defaults). */
+int lv_server_graph_tuning(lv_server* s, uint32_t graph_id, lv_tune_choice* out);
 const char* lv_server_last_error(const lv_server* s);
 /* Milliseconds on the server's clock (the one lv_completion's times use). */
 double lv_server_now_ms(const lv_server* s);

@@ -7,6 +7,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -89,6 +90,16 @@ struct lv_context {
   uint64_t min_streams = 1;
   uint64_t max_streams = 1ull << 20;
   uint64_t chunk = 0;           // entries per dynamically assigned chunk (0: static spread)
+  // worklist sub-rounds: at most wl_streams streams in the id-order kernel,
+  // the queue interleaved over them (lv_context_set_worklist_schedule)
+  uint64_t wl_streams = kWlStreams;
+  int interleave = 0;
+  // schedule tuner: choices by graph shape (lv_autotune), applied before
+  // every run while autotune is on and no tuned knob was set by hand
+  int autotune = 1;
+  bool knobs_hand = false;  // a tuned knob set by hand: the knobs stay as set
+  bool tuning = false;      // inside a tuning: trial runs keep the candidate's knobs
+  std::map<std::string, lv_tune_choice> tuned;
   // Sweep schedules by pass: the first lead_iters iterations of the first
   // pass run lead_streams streams over ascending chunks of lead_chunk
   // entries; passes after the first run late_streams streams (chunks of
@@ -683,7 +694,7 @@ MoveArgs base_args(lv_context* c, const Level& lv, uint32_t* vcom, const double*
   // worklist sub-rounds mostly hold a few thousand vertices: no more streams
   // than the id-order kernel keeps resident at once (a launch with fewer
   // entries then costs one round of short-lived waves, not several)
-  if (worklist) a.max_streams = std::min<uint64_t>(a.max_streams, kWlStreams);
+  if (worklist) a.max_streams = std::min<uint64_t>(a.max_streams, c->wl_streams);
   a.chunk = c->chunk;
   a.work = c->ws<unsigned long long>("work", 1);
   a.hub_min = worklist ? c->hub_min_wl : c->hub_min;
@@ -947,8 +958,9 @@ int local_move_worklist(lv_context* c, const Level& lv, uint32_t* vcom, const do
   batched_schedule(c, a, lv, bt);
   if (bt) batched_begin(c, bt);
   a.sparse = 1;  // queue entries read 64 at a time
+  a.interleave = c->interleave;
   unsigned long long* cnt0 = a.cnt;
-  const uint64_t S = lv.g.S, stride = std::max<uint64_t>(S, 1), fbytes = (S + 7) & ~3ull;
+  const uint64_t S = lv.g.S, stride = std::max<uint64_t>(S, 1), fwords = (S >> 5) + 2;
   // queues: low degree, small hubs, mid hubs, hub batch
   WlQueues qs;
   for (int q = 0; q < kWlQueues; ++q) qs.q[q] = c->ws<uint32_t>("wl.q" + std::to_string(q), stride);
@@ -959,8 +971,8 @@ int local_move_worklist(lv_context* c, const Level& lv, uint32_t* vcom, const do
   // next-sub-round flags: all zero between calls (each compaction resets
   // the flags it reads, and a sub-round loop ends on an empty compaction)
   bool fresh = false;
-  a.fsame = c->ws<uint8_t>("wl.same", fbytes, &fresh);
-  if (fresh) k_fill_u8(a.fsame, 0, c->ws_bytes("wl.same"), c->st);
+  a.fsame = c->ws<uint32_t>("wl.same", fwords, &fresh);
+  if (fresh) k_fill_u32(a.fsame, 0, c->ws_bytes("wl.same") / 4, c->st);
   // per compaction segment: a next-sub-round flag may be set there (all zero
   // between calls too; a compaction clears the marks of the segments it reads)
   a.wl_dirty = c->ws<uint8_t>("wl.dirty", kWlScanBlocks, &fresh);
@@ -1067,7 +1079,7 @@ int local_move_worklist(lv_context* c, const Level& lv, uint32_t* vcom, const do
         launch_move(c, a, k, lv.f64);
       }
     }
-    k_wl_queue(a.fsame, true, S, lv.g.deg, sp_same, qs, lens, bsum, c->st);
+    k_wl_queue_bits(a.fsame, true, S, lv.g.deg, sp_same, qs, lens, bsum, c->st);
   };
   int l = 0;
   double el = 0;
@@ -1511,6 +1523,11 @@ void invoke(lv_context* c, int approach, const lv_dgraph* x, const lv_dbatch* b,
   for (auto& pr : ag) rt.aggr += elapsed(pr.first, pr.second);
 }
 
+lv_tune_choice tune(lv_context* c, int approach, const lv_dgraph* x, const lv_dbatch* b, const lv_dstate* s,
+                    const lv_options* o, int reps, const std::vector<uint64_t>* voff);
+void apply_tuned(lv_context* c, int approach, const lv_dgraph* x, uint64_t G);
+std::string shape_key(const lv_context* c, int approach, const lv_dgraph* x, uint64_t G);
+
 void run(lv_context* c, int approach, const lv_dgraph* x, const lv_dbatch* b, lv_dstate* s, const lv_options* o,
          lv_result* res) {
   require(c && x && s, LV_ERR_INVALID, "null context, graph or state");
@@ -1520,6 +1537,7 @@ void run(lv_context* c, int approach, const lv_dgraph* x, const lv_dbatch* b, lv
   require(!x->broken, LV_ERR_INVALID, "graph left inconsistent by a failed update");
   validate_options(o);
   HIPCHK(hipSetDevice(c->device));
+  apply_tuned(c, approach, x, 1);
   c->ev_used = 0;
   const uint64_t S = x->S;
   int repeat = o->repeat;
@@ -1838,10 +1856,15 @@ void run_batched(lv_context* c, int approach, const lv_dgraph* x, const lv_dbatc
   require(o->repeat == 1, LV_ERR_INVALID, "the batched form runs one repeat");
   s->ht_reset();  // the batched form moves vertices without hub tables
   HIPCHK(hipSetDevice(c->device));
+  std::vector<uint64_t> vo(voff, voff + G + 1);
+  // a shape seen for the first time is tuned first (on clones of s)
+  if (c->autotune && !c->tuning && !c->knobs_hand && approach != LV_STATIC &&
+      !c->tuned.count(shape_key(c, approach, x, G)))
+    tune(c, approach, x, b, s, o, 3, &vo);
+  apply_tuned(c, approach, x, G);
   c->ev_used = 0;
   c->prof.clear();
   RunTimes rt;
-  std::vector<uint64_t> vo(voff, voff + G + 1);
   std::vector<int> l, p;
   invoke_batched(c, approach, x, b, s, o, vo, l, p, rt);
   for (auto& pr : c->prof) {
@@ -1882,6 +1905,249 @@ void run_batched(lv_context* c, int approach, const lv_dgraph* x, const lv_dbatc
     }
   }
   c->stats.runs += 1;
+}
+
+// ------------------------------------------------------------ schedule tuner
+// Modularity of s's membership on g (modularityByOmp, inc/properties.hxx:205-233).
+double dmodularity(lv_context* c, const lv_dgraph* g, const lv_dstate* s, double R) {
+  double* cin = c->ws<double>("mod.cin", g->S);
+  double* ct = c->ws<double>("mod.ctot", g->S);
+  double* o = c->ws<double>("mod.out", 1);
+  k_fill_f64(cin, 0, g->S, c->st);
+  k_fill_f64(ct, 0, g->S, c->st);
+  k_fill_f64(o, 0, 1, c->st);
+  k_modularity_parts(g->view(), g->f64, s->memb, cin, ct, c->st);
+  k_modularity_sum(cin, ct, g->S, g->M2 / 2, R, o, c->st);
+  return c->read1(o);
+}
+
+// The knobs a tuning chooses: the worklist and sweep hub degrees (the
+// degree bucket between the id-order kernel and the hub kernels), the
+// worklist id-order streams and queue layout, and the streams per live
+// vertex of a level.  The reference's counterpart is its fixed
+// schedule(dynamic, 2048) (inc/louvain.hxx:594).
+lv_tune_choice default_knobs() {
+  lv_tune_choice k{};
+  k.worklist_hub_degree = 256;
+  k.sweep_hub_degree = kHubMin;
+  k.worklist_streams = kWlStreams;
+  k.interleave = 0;
+  k.stream_fraction = 0.002;
+  return k;
+}
+bool same_knobs(const lv_tune_choice& a, const lv_tune_choice& b) {
+  return a.worklist_hub_degree == b.worklist_hub_degree && a.sweep_hub_degree == b.sweep_hub_degree &&
+         a.worklist_streams == b.worklist_streams && a.interleave == b.interleave &&
+         a.stream_fraction == b.stream_fraction;
+}
+lv_tune_choice knobs_of(const lv_context* c) {
+  lv_tune_choice k = default_knobs();
+  k.worklist_hub_degree = c->hub_min_wl;
+  k.sweep_hub_degree = c->hub_min;
+  k.worklist_streams = c->wl_streams;
+  k.interleave = c->interleave;
+  k.stream_fraction = c->stream_fraction;
+  return k;
+}
+void apply_knobs(lv_context* c, const lv_tune_choice& k) {
+  c->hub_min_wl = k.worklist_hub_degree;
+  c->hub_min = k.sweep_hub_degree;
+  c->wl_streams = k.worklist_streams;
+  c->interleave = k.interleave;
+  c->stream_fraction = k.stream_fraction;
+}
+// A public setter of a tuned knob: from the defaults (leaving automatic
+// mode) or from the knobs set by hand so far; back in automatic mode when
+// every tuned knob is at its default again.
+template <class F>
+void set_by_hand(lv_context* c, F set) {
+  if (!c->knobs_hand) apply_knobs(c, default_knobs());
+  set();
+  c->knobs_hand = !same_knobs(knobs_of(c), default_knobs());
+}
+
+// Graph shape: approach, floor(log2) of the span and of the live edges, and
+// for the batched form the graphs' count (log2 + 1).
+std::string shape_key(const lv_context* c, int approach, const lv_dgraph* x, uint64_t G) {
+  auto lg = [](uint64_t v) {
+    int b = 0;
+    while (v >>= 1) ++b;
+    return b;
+  };
+  return std::to_string(approach) + "/" + std::to_string(c->path) + "/" + std::to_string(lg(std::max<uint64_t>(x->S, 1))) +
+         "/" + std::to_string(lg(std::max<uint64_t>(x->live, 1))) + "/" + std::to_string(G > 1 ? lg(G) + 1 : 0);
+}
+
+// Candidates: single-graph worklist approaches (DF, DS on path 2) the
+// worklist hub degree, id-order streams and queue layout; single-graph
+// sweeps (ND, or any approach off path 2) the sweep hub degree and the
+// streams per live vertex; the batched form the streams per live vertex,
+// the worklist hub degree and the queue layout.  The defaults first.
+std::vector<lv_tune_choice> tune_candidates(const lv_context* c, int approach, bool batched) {
+  std::vector<lv_tune_choice> v{default_knobs()};
+  const bool worklist = c->path == 2 && (approach == LV_DYNAMIC_FRONTIER || approach == LV_DELTA_SCREENING);
+  auto add = [&](const lv_tune_choice& k) {
+    for (const auto& e : v)
+      if (same_knobs(e, k)) return;
+    v.push_back(k);
+  };
+  if (batched) {
+    for (double f : {0.001, 0.002, 0.004})
+      for (uint32_t h : {256u, 512u})
+        for (int il : {0, 1}) {
+          lv_tune_choice k = default_knobs();
+          k.stream_fraction = f;
+          k.worklist_hub_degree = h;
+          k.interleave = il;
+          add(k);
+        }
+  } else if (worklist) {
+    for (uint32_t h : {256u, 512u})
+      for (uint64_t st : {1536ull, 3072ull, 6144ull})
+        for (int il : {0, 1}) {
+          lv_tune_choice k = default_knobs();
+          k.worklist_hub_degree = h;
+          k.worklist_streams = st;
+          k.interleave = il;
+          add(k);
+        }
+  } else {
+    for (uint32_t h : {(uint32_t)kHubMin, 16384u})
+      for (double f : {0.001, 0.002, 0.004}) {
+        lv_tune_choice k = default_knobs();
+        k.sweep_hub_degree = h;
+        k.stream_fraction = f;
+        add(k);
+      }
+  }
+  return v;
+}
+
+// s's membership, K, Sigma_tot and (when valid) hub tables into t, which
+// takes table buffers as it needs; t then stands where s stands (its next
+// run carries or builds the tables as s's would).
+void clone_state(lv_context* c, lv_dstate* t, const lv_dstate* s) {
+  require(t->S == s->S, LV_ERR_INVALID, "state spans differ");
+  const uint64_t S = s->S;
+  d2d(c, t->memb, s->memb, S);
+  d2d(c, t->vtot, s->vtot, S);
+  d2d(c, t->ctot, s->ctot, S);
+  t->ht_reset();
+  if (s->ht_valid && s->ht_cap && s->ht_next && s->ht_slots) {
+    if (!t->ht_cap) {
+      t->ht_cap = dalloc<uint32_t>(S);
+      t->ht_off = dalloc<uint64_t>(S + 1);
+    }
+    if (!t->ht_next) t->ht_next = dalloc<unsigned long long>(2);
+    if (t->ht_slots < s->ht_slots) {
+      (void)hipFree(t->ht_key);
+      (void)hipFree(t->ht_cnt);
+      (void)hipFree(t->ht_w);
+      t->ht_key = nullptr;
+      t->ht_cnt = nullptr;
+      t->ht_w = nullptr;
+      t->ht_slots = 0;
+      t->ht_key = dalloc<uint32_t>(s->ht_slots);
+      t->ht_cnt = dalloc<int32_t>(s->ht_slots);
+      t->ht_w = dalloc<double>(s->ht_slots);
+      t->ht_slots = s->ht_slots;
+    }
+    d2d(c, t->ht_cap, s->ht_cap, S);
+    d2d(c, t->ht_off, s->ht_off, S + 1);
+    d2d(c, t->ht_next, s->ht_next, 2);
+    d2d(c, t->ht_key, s->ht_key, s->ht_slots);
+    d2d(c, t->ht_cnt, s->ht_cnt, s->ht_slots);
+    d2d(c, t->ht_w, s->ht_w, s->ht_slots);
+    t->ht_valid = true;
+    t->ht_stamp = s->ht_stamp;
+    t->ht_scan = s->ht_scan;
+  }
+  t->runs = s->runs;
+}
+
+void run(lv_context* c, int approach, const lv_dgraph* x, const lv_dbatch* b, lv_dstate* s, const lv_options* o,
+         lv_result* res);
+
+// Times every candidate schedule of `approach` on graph x with batch b from
+// state s (each trial on a clone of s: s is left as it was), `reps` runs
+// each, median of the host clock around the run; keeps the fastest whose
+// modularity is within 1e-3 of the defaults', and the defaults unless it is
+// faster than them by more than 3% (run-to-run noise).  The choice is
+// cached under the graph's shape and applied to the context.
+lv_tune_choice tune(lv_context* c, int approach, const lv_dgraph* x, const lv_dbatch* b, const lv_dstate* s,
+                    const lv_options* o, int reps, const std::vector<uint64_t>* voff) {
+  struct Restore {  // (a failed tuning leaves the knobs as they were)
+    lv_context* c;
+    lv_stats st;
+    lv_tune_choice knobs;
+    bool done = false;
+    ~Restore() {
+      c->tuning = false;
+      c->stats = st;
+      if (!done) apply_knobs(c, knobs);
+    }
+  } restore{c, c->stats, knobs_of(c)};
+  c->tuning = true;
+  lv_dstate* t = nullptr;
+  {
+    const int rc = lv_dstate_create(c, s->S, &t);
+    require(rc == LV_OK, rc, "tuning state: " + c->err);
+  }
+  struct Free {
+    lv_dstate* t;
+    ~Free() { lv_dstate_destroy(t); }
+  } free_t{t};
+  const auto cands = tune_candidates(c, approach, voff != nullptr);
+  auto measure = [&](const lv_tune_choice& k, double& q) {
+    apply_knobs(c, k);
+    std::vector<double> ms;
+    for (int r = 0; r < reps; ++r) {
+      clone_state(c, t, s);
+      HIPCHK(hipStreamSynchronize(c->st));
+      const auto t0 = std::chrono::steady_clock::now();
+      if (voff) run_batched(c, approach, x, b, t, voff->data(), voff->size() - 1, o, nullptr);
+      else run(c, approach, x, b, t, o, nullptr);
+      ms.push_back(std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+    }
+    q = dmodularity(c, x, t, o->resolution);
+    std::sort(ms.begin(), ms.end());
+    return ms[ms.size() / 2];
+  };
+  double qd = 0;
+  const double md = measure(cands[0], qd);
+  lv_tune_choice best = cands[0];
+  double bm = md, bq = qd;
+  for (size_t k = 1; k < cands.size(); ++k) {
+    double qk = 0;
+    const double mk = measure(cands[k], qk);
+    if (std::fabs(qk - qd) <= 1e-3 && mk < bm) {
+      best = cands[k];
+      bm = mk;
+      bq = qk;
+    }
+  }
+  if (bm > 0.97 * md) {
+    best = cands[0];
+    bm = md;
+    bq = qd;
+  }
+  best.ms = bm;
+  best.default_ms = md;
+  best.modularity = bq;
+  best.default_modularity = qd;
+  best.candidates = (int32_t)cands.size();
+  apply_knobs(c, best);
+  c->tuned[shape_key(c, approach, x, voff ? voff->size() - 1 : 1)] = best;
+  restore.done = true;
+  return best;
+}
+
+// Before a run: the tuned choice for the graph's shape, or the defaults,
+// unless tuning is off or a tuned knob was set by hand.
+void apply_tuned(lv_context* c, int approach, const lv_dgraph* x, uint64_t G) {
+  if (!c->autotune || c->tuning || c->knobs_hand) return;
+  const auto it = c->tuned.find(shape_key(c, approach, x, G));
+  apply_knobs(c, it != c->tuned.end() ? it->second : default_knobs());
 }
 
 // ------------------------------------------------------------ uploads
@@ -2378,8 +2644,10 @@ int lv_context_set_hub_streams(lv_context* c, uint64_t streams) {
 int lv_context_set_hub_threshold(lv_context* c, uint32_t sweep_degree, uint32_t worklist_degree) {
   if (!c || sweep_degree < kHubMin || worklist_degree < kHybridLds / 2 || worklist_degree > kHubMin)
     return LV_ERR_INVALID;
-  c->hub_min = sweep_degree;
-  c->hub_min_wl = worklist_degree;
+  set_by_hand(c, [&] {
+    c->hub_min = sweep_degree;
+    c->hub_min_wl = worklist_degree;
+  });
   return LV_OK;
 }
 
@@ -2505,13 +2773,93 @@ int lv_stats_reset(lv_context* c) {
 int lv_context_set_schedule(lv_context* c, double stream_fraction, uint64_t min_streams, uint64_t max_streams,
                             uint64_t chunk, int hubs_first) {
   if (!c || !(stream_fraction > 0) || min_streams < 1 || max_streams < min_streams) return LV_ERR_INVALID;
+  if (hubs_first < 0 || hubs_first > 2) return LV_ERR_INVALID;
   c->chunk = chunk;
-  c->stream_fraction = stream_fraction;
+  set_by_hand(c, [&] { c->stream_fraction = stream_fraction; });
   c->min_streams = min_streams;
   c->max_streams = max_streams;
-  if (hubs_first < 0 || hubs_first > 2) return LV_ERR_INVALID;
   c->tier_order = hubs_first;
   return LV_OK;
+}
+
+int lv_context_set_worklist_schedule(lv_context* c, uint64_t streams, int interleave) {
+  if (!c || streams < 1 || streams > (1u << 16) || interleave < 0 || interleave > 1) return LV_ERR_INVALID;
+  set_by_hand(c, [&] {
+    c->wl_streams = streams;
+    c->interleave = interleave;
+  });
+  return LV_OK;
+}
+
+int lv_context_set_autotune(lv_context* c, int on) {
+  if (!c || on < 0 || on > 1) return LV_ERR_INVALID;
+  c->autotune = on;
+  if (on && c->knobs_hand) {
+    apply_knobs(c, default_knobs());
+    c->knobs_hand = false;
+  }
+  return LV_OK;
+}
+
+int lv_context_apply_tuning(lv_context* c, const lv_tune_choice* k) {
+  if (!c) return LV_ERR_INVALID;
+  if (k && (k->worklist_hub_degree < kHybridLds / 2 || k->worklist_hub_degree > kHubMin ||
+            k->sweep_hub_degree < kHubMin || k->worklist_streams < 1 || k->worklist_streams > (1u << 16) ||
+            k->interleave < 0 || k->interleave > 1 || !(k->stream_fraction > 0)))
+    return LV_ERR_INVALID;
+  set_by_hand(c, [&] { apply_knobs(c, k ? *k : default_knobs()); });
+  return LV_OK;
+}
+
+int lv_context_add_tuning(lv_context* c, int approach, const lv_dgraph* g, const lv_tune_choice* k) {
+  if (!c || !g || !k || approach < LV_STATIC || approach > LV_DYNAMIC_FRONTIER) return LV_ERR_INVALID;
+  c->tuned[shape_key(c, approach, g, 1)] = *k;
+  return LV_OK;
+}
+
+int lv_context_tuning(lv_context* c, int approach, const lv_dgraph* g, uint64_t graphs, lv_tune_choice* out) {
+  if (!c || !g || !out || graphs < 1) return LV_ERR_INVALID;
+  const auto it = c->tuned.find(shape_key(c, approach, g, graphs));
+  if (it != c->tuned.end()) {
+    *out = it->second;
+  } else {
+    *out = knobs_of(c);
+    if (c->autotune && !c->knobs_hand) {
+      const lv_tune_choice d = default_knobs();
+      *out = d;
+    }
+    out->candidates = 0;
+  }
+  return LV_OK;
+}
+
+int lv_dstate_clone(lv_context* c, lv_dstate* dst, const lv_dstate* src) {
+  if (!c || !dst || !src || dst->S != src->S) return LV_ERR_INVALID;
+  return guard(c, [&] {
+    clone_state(c, dst, src);
+    HIPCHK(hipStreamSynchronize(c->st));
+  });
+}
+
+int lv_autotune(lv_context* c, int approach, const lv_dgraph* g, const lv_dbatch* probe, const lv_dstate* s,
+                const lv_options* o, int reps, lv_tune_choice* out) {
+  if (!c || !g || !s) return LV_ERR_INVALID;
+  return guard(c, [&] {
+    require(approach >= LV_NAIVE_DYNAMIC && approach <= LV_DYNAMIC_FRONTIER, LV_ERR_INVALID,
+            "lv_autotune tunes the dynamic approaches (Static's modularity moves with its schedule)");
+    require(probe != nullptr, LV_ERR_INVALID, "dynamic approaches need a batch");
+    require(s->S == g->S, LV_ERR_INVALID, "state span differs from graph span");
+    require(reps >= 1 && reps <= 64, LV_ERR_INVALID, "reps must be 1..64");
+    lv_options od;
+    lv_options_default(&od);
+    const lv_options* oo = o ? o : &od;
+    validate_options(oo);
+    require(oo->repeat == 1, LV_ERR_INVALID, "tuning runs one repeat");
+    require_batch(c, const_cast<lv_dbatch*>(probe), g->S);
+    c->knobs_hand = false;
+    const lv_tune_choice k = tune(c, approach, g, probe, s, oo, reps, nullptr);
+    if (out) *out = k;
+  });
 }
 
 int lv_louvain_static(lv_context* c, const lv_graph* x, const lv_options* o, lv_result* res) {

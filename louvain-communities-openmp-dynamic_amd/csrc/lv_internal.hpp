@@ -172,8 +172,8 @@ struct MoveArgs {
   const uint64_t* glo;
   const uint64_t* gcs;
   uint32_t ng;
-  uint8_t* fsame;              // worklist mode: flagged by a lower id and not yet
-                               //   visited this iteration -> next sub-round
+  uint32_t* fsame;             // worklist mode, bit v: flagged by a lower id and not
+                               //   yet visited this iteration -> next sub-round
   uint8_t* wl_dirty;           // worklist mode: per compaction segment, fsame may hold a flag
   uint32_t wl_shift;           //   (segment of id v: v >> wl_shift)
   uint32_t* stamp;             // worklist mode: iteration serial of the last visit
@@ -186,6 +186,8 @@ struct MoveArgs {
   uint64_t max_streams;        // lane groups launched (streams walking at once)
   uint64_t chunk;              // list entries per dynamically assigned chunk
   int sparse;                  // id-order kernel: scan 64-entry windows by ballot, visit flagged only
+  int interleave;              // id-order kernel, chunk 0, no ctab: group g takes entries g, g + N,
+                               //   g + 2N, ... of the list (N groups), in that order
   unsigned long long* work;    // chunk counter, zero before each launch
   // hub kernel (tier 4) HBM hashtables: one region of hcap slots per block
   uint32_t* hkey;
@@ -384,6 +386,9 @@ void k_caps_sum(const uint32_t* deg, uint64_t n, uint32_t thr, unsigned long lon
 // kWlQueues * kWlScanBlocks entries.
 void k_wl_queue(uint8_t* f, bool clear, uint64_t S, const uint32_t* deg, const WlSplit& sp, const WlQueues& qs,
                 uint32_t* lens, uint64_t* bsum, hipStream_t st);
+// k_wl_queue over a bitmap (bit v of u32 word v >> 5; (S >> 5) + 2 words).
+void k_wl_queue_bits(uint32_t* bits, bool clear, uint64_t S, const uint32_t* deg, const WlSplit& sp,
+                     const WlQueues& qs, uint32_t* lens, uint64_t* bsum, hipStream_t st);
 void k_modularity_parts(GView g, bool f64, const uint32_t* memb, double* cin, double* ctot, hipStream_t st);
 void k_modularity_sum(const double* cin, const double* ctot, uint64_t n, double M, double R, double* out,
                       hipStream_t st);

@@ -174,10 +174,10 @@ __device__ __forceinline__ void add_counts(unsigned long long* cnt, unsigned lon
   if (me) atomicAdd(&s[CNT_MOVED_EDGES], me);
 }
 
-// v joins the next sub-round: its flag, and its compaction segment's
-// dirty mark (a segment without one is skipped by the compaction).
+// v joins the next sub-round: its bit, and its compaction segment's dirty
+// mark (a segment without one is skipped by the compaction).
 __device__ __forceinline__ void mark_same(const MoveArgs& a, uint32_t v) {
-  __hip_atomic_store(&a.fsame[v], (uint8_t)1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_fetch_or(&a.fsame[v >> 5], 1u << (v & 31), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   if (a.wl_dirty) a.wl_dirty[v >> a.wl_shift] = 1;
 }
 
@@ -775,11 +775,15 @@ __global__ __launch_bounds__(BLOCK) void move_kernel(MoveArgs a) {
   // a.chunk > 0: chunks of a.chunk entries handed out in ascending order;
   // a.chunk == 0: ceil(n / streams) entries per chunk, chunk g to group g,
   // so the groups walk far-apart parts of the list at the same time.
+  // a.interleave (chunk 0, no ctab): min(n, streams) groups, group g taking
+  // entries g, g + N, g + 2N, ... -- an ascending list whose low entries
+  // are the heavy ones (R-MAT ids) is then spread evenly over the groups.
   const bool dyn = a.chunk != 0;
+  const bool ilv = a.interleave && !dyn && !a.ctab;
   const uint64_t streams = a.max_streams ? a.max_streams : 1;
   const uint64_t chunk = dyn ? a.chunk : ((n + streams - 1) / streams ? (n + streams - 1) / streams : 1);
   // a.ctab (id-order kernel, one group per block): chunk bounds per chunk
-  const uint64_t nchunks = a.ctab ? a.nct : (n + chunk - 1) / chunk;
+  const uint64_t nchunks = a.ctab ? a.nct : ilv ? (n < streams ? n : streams) : (n + chunk - 1) / chunk;
   uint64_t next = (uint64_t)blockIdx.x * GPB;
   const WT* W = (const WT*)a.g.w;
   // LDS table of this group; HBM region of this block
@@ -815,10 +819,15 @@ __global__ __launch_bounds__(BLOCK) void move_kernel(MoveArgs a) {
     if (cb >= nchunks) break;
     const uint64_t ch = cb + grp;
     // entries [c0, c0 + clen) of the list
-    uint64_t c0 = ch * chunk, clen = chunk;
+    // (interleaved: entries ch + j * nchunks, j < clen)
+    uint64_t c0 = ch * chunk, clen = chunk, step = 1;
     if (a.ctab) {
       c0 = ch < nchunks ? a.ctab[2 * ch] : 0;
       clen = ch < nchunks ? a.ctab[2 * ch + 1] - c0 : 0;
+    } else if (ilv) {
+      c0 = ch;
+      step = nchunks;
+      clen = ch < nchunks ? (n - ch + nchunks - 1) / nchunks : 0;
     }
     // One vertex (offset o, degree d, community cu, weight Ku) through the
     // group; every thread of the block calls it.  Its barriers are the
@@ -943,7 +952,7 @@ __global__ __launch_bounds__(BLOCK) void move_kernel(MoveArgs a) {
         for (uint64_t j0 = 0; j0 < clen; j0 += 64) {
           // (a flagged entry's offset, degree, community and K are loaded
           // with the window; its community changes only when it is visited)
-          const uint64_t idx = c0 + j0 + lane;
+          const uint64_t idx = c0 + (j0 + lane) * step;
           const bool in = ch < nchunks && j0 + lane < clen && idx < n;
           uint32_t ul = 0, dl = 0, cl = 0;
           uint64_t ol = 0;
@@ -975,7 +984,7 @@ __global__ __launch_bounds__(BLOCK) void move_kernel(MoveArgs a) {
       }
     }
     for (uint64_t j = 0; j < clen; ++j) {
-      const uint64_t idx = c0 + j;
+      const uint64_t idx = c0 + j * step;
       bool act = false;
       uint32_t u = 0;
       if (ch < nchunks && idx < n) {
@@ -2364,15 +2373,67 @@ __device__ __forceinline__ void wl_words16(const uint8_t* f, uint64_t id0, uint6
   }
 }
 
-// Bit j of m[q]: id id0 + j (j < 16) is flagged and belongs to queue q
-// (queue 0: degree <= t[0]; 1: <= t[1]; 2: <= t[2]; 3: above, or a 1/2 id
-// marked in big; 4: an id of queues 1-3 with a table; t[0] == 0 puts every
-// id in queue 0).  The 16 degrees are read together.
-__device__ __forceinline__ void wl_bits16(const uint32_t (&w)[4], const uint32_t* deg, uint64_t S, const WlSplit& sp,
+// Bit j: id id0 + j is flagged (id0 a multiple of 16; 0 at or past hi).
+// Byte flags (one per id, whole words up to S) or a bitmap (bit v of u32
+// word v >> 5; its 16-bit halves read here).
+template <bool BITS>
+__device__ __forceinline__ uint32_t wl_mask16(const void* f, uint64_t id0, uint64_t hi) {
+  if constexpr (BITS) {
+    if (id0 >= hi) return 0u;
+    uint32_t m = reinterpret_cast<const uint16_t*>(f)[id0 >> 4];
+    if (id0 + 16 > hi) m &= (1u << (hi - id0)) - 1u;
+    return m;
+  } else {
+    uint32_t w[4];
+    wl_words16(static_cast<const uint8_t*>(f), id0, hi, w);
+    uint32_t m = 0;
+#pragma unroll
+    for (int j = 0; j < 16; ++j)
+      if ((w[j / 4] >> (8 * (j % 4))) & 0xFFu) m |= 1u << j;
+    return m;
+  }
+}
+
+// Clear the flags of ids id0 .. id0 + 15 (below hi) whose mask was f16.
+template <bool BITS>
+__device__ __forceinline__ void wl_clear16(void* f, uint64_t id0, uint64_t hi, uint32_t f16) {
+  if (!f16) return;
+  if constexpr (BITS) {
+    reinterpret_cast<uint16_t*>(f)[id0 >> 4] = 0;
+  } else {
+    uint8_t* b = static_cast<uint8_t*>(f);
+    if (id0 + 16 <= hi && !((uintptr_t)b & 15)) {
+      *reinterpret_cast<uint4*>(b + id0) = make_uint4(0u, 0u, 0u, 0u);
+    } else {
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        if ((f16 >> (4 * k)) & 0xFu) reinterpret_cast<uint32_t*>(b)[(id0 >> 2) + k] = 0u;
+    }
+  }
+}
+
+// Bit j of m[q]: id id0 + j (j < 16) is flagged (bit j of f16) and belongs
+// to queue q (queue 0: degree <= t[0]; 1: <= t[1]; 2: <= t[2]; 3: above, or
+// a 1/2 id marked in big; 4: an id of queues 1-3 with a table; t[0] == 0
+// puts every id in queue 0).  The 16 degrees are read together when more
+// than 4 of the ids are flagged, one at a time otherwise.
+__device__ __forceinline__ void wl_bits16(uint32_t f16, const uint32_t* deg, uint64_t S, const WlSplit& sp,
                                           uint64_t id0, uint32_t (&m)[kWlQueues]) {
 #pragma unroll
   for (int q = 0; q < kWlQueues; ++q) m[q] = 0;
-  if (!(w[0] | w[1] | w[2] | w[3])) return;
+  if (!f16) return;
+  if (__popc(f16) <= 4) {
+    for (uint32_t bits = f16; bits; bits &= bits - 1) {
+      const int j = __ffs(bits) - 1;
+      if (id0 + j >= S) continue;
+      const uint32_t dj = sp.t[0] ? deg[id0 + j] : 0u;
+      int q = dj <= sp.t[0] ? 0 : (dj <= sp.t[1] ? 1 : (dj <= sp.t[2] ? 2 : 3));
+      if (q == 1 || q == 2) q = sp.big && sp.big[id0 + j] ? 3 : q;
+      if (q && sp.tab && sp.tab[id0 + j] > kHtDropped) q = 4;
+      m[q] |= 1u << j;
+    }
+    return;
+  }
   uint32_t d[16];
   if (sp.t[0] && id0 + 16 <= S && !((uintptr_t)deg & 15)) {
 #pragma unroll
@@ -2389,7 +2450,7 @@ __device__ __forceinline__ void wl_bits16(const uint32_t (&w)[4], const uint32_t
   }
 #pragma unroll
   for (int j = 0; j < 16; ++j) {
-    if (id0 + j < S && ((w[j / 4] >> (8 * (j % 4))) & 0xFFu)) {
+    if (id0 + j < S && ((f16 >> j) & 1u)) {
       int q = d[j] <= sp.t[0] ? 0 : (d[j] <= sp.t[1] ? 1 : (d[j] <= sp.t[2] ? 2 : 3));
       if (q == 1 || q == 2) q = sp.big && sp.big[id0 + j] ? 3 : q;
       if (q && sp.tab && sp.tab[id0 + j] > kHtDropped) q = 4;
@@ -2406,7 +2467,8 @@ __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long x)
 }
 
 // Per block (segment) and queue: the flagged ids it holds.
-__global__ __launch_bounds__(kBlock) void wl_count_kernel(const uint8_t* f, uint64_t S, const uint32_t* deg,
+template <bool BITS>
+__global__ __launch_bounds__(kBlock) void wl_count_kernel(const void* f, uint64_t S, const uint32_t* deg,
                                                           WlSplit sp, uint64_t* bsum) {
   __shared__ unsigned long long s_c[kWlQueues];
   const uint64_t lo = (uint64_t)blockIdx.x << sp.shift, hi = min(S, lo + (1ull << sp.shift));
@@ -2418,10 +2480,8 @@ __global__ __launch_bounds__(kBlock) void wl_count_kernel(const uint8_t* f, uint
   __syncthreads();
   unsigned long long c[kWlQueues] = {};
   for (uint64_t id0 = lo + 16 * (uint64_t)threadIdx.x; id0 < hi; id0 += kWlTile) {
-    uint32_t w[4];
-    wl_words16(f, id0, hi, w);
     uint32_t m[kWlQueues];
-    wl_bits16(w, deg, S, sp, id0, m);
+    wl_bits16(wl_mask16<BITS>(f, id0, hi), deg, S, sp, id0, m);
 #pragma unroll
     for (int q = 0; q < kWlQueues; ++q) c[q] += __popc(m[q]);
   }
@@ -2441,7 +2501,8 @@ __global__ __launch_bounds__(kBlock) void wl_count_kernel(const uint8_t* f, uint
 // at most 4096 each, packed 13 bits apart); queue 4 takes a second scan in
 // the tiles that hold any of its ids.  Writes stay below S, the queues'
 // capacity.
-__global__ __launch_bounds__(kBlock) void wl_scatter_kernel(uint8_t* f, int clear, uint64_t S, const uint32_t* deg,
+template <bool BITS>
+__global__ __launch_bounds__(kBlock) void wl_scatter_kernel(void* f, int clear, uint64_t S, const uint32_t* deg,
                                                             WlSplit sp, const uint64_t* bsum, WlQueues qs,
                                                             uint32_t* lens) {
   static_assert(kWlTile < 8192 && 4 * 13 <= 64 && kWlQueues == 5, "packed tile counts");
@@ -2479,10 +2540,9 @@ __global__ __launch_bounds__(kBlock) void wl_scatter_kernel(uint8_t* f, int clea
   if (sp.dirty && clear && threadIdx.x == 0) sp.dirty[blockIdx.x] = 0;
   for (uint64_t t0 = lo; t0 < hi; t0 += kWlTile) {  // block-uniform trip count
     const uint64_t id0 = t0 + 16 * (uint64_t)threadIdx.x;
-    uint32_t w[4] = {0u, 0u, 0u, 0u};
-    if (id0 < hi) wl_words16(f, id0, hi, w);
+    const uint32_t f16 = wl_mask16<BITS>(f, id0, hi);
     uint32_t m[kWlQueues];
-    wl_bits16(w, deg, S, sp, id0, m);
+    wl_bits16(f16, deg, S, sp, id0, m);
     uint64_t packed = 0;
 #pragma unroll
     for (int q = 0; q < 4; ++q) packed |= (uint64_t)__popc(m[q]) << (13 * q);
@@ -2502,15 +2562,7 @@ __global__ __launch_bounds__(kBlock) void wl_scatter_kernel(uint8_t* f, int clea
       }
       r[q] += q < 4 ? (tot >> (13 * q)) & 0x1FFF : tot4;
     }
-    if (clear && (w[0] | w[1] | w[2] | w[3])) {
-      if (id0 + 16 <= hi && !((uintptr_t)f & 15)) {
-        *reinterpret_cast<uint4*>(f + id0) = make_uint4(0u, 0u, 0u, 0u);
-      } else {
-#pragma unroll
-        for (int k = 0; k < 4; ++k)
-          if (w[k]) reinterpret_cast<uint32_t*>(f)[(id0 >> 2) + k] = 0u;
-      }
-    }
+    if (clear) wl_clear16<BITS>(f, id0, hi, f16);
   }
 }
 
@@ -2894,8 +2946,15 @@ void k_wl_queue(uint8_t* f, bool clear, uint64_t S, const uint32_t* deg, const W
                 uint32_t* lens, uint64_t* bsum, hipStream_t st) {
   WlSplit sp = sp_in;
   sp.shift = wl_shift(S);
-  LV_LAUNCH(wl_count_kernel, kWlScanBlocks, kBlock, st, f, S, deg, sp, bsum);
-  LV_LAUNCH(wl_scatter_kernel, kWlScanBlocks, kBlock, st, f, (int)clear, S, deg, sp, bsum, qs, lens);
+  LV_LAUNCH(wl_count_kernel<false>, kWlScanBlocks, kBlock, st, f, S, deg, sp, bsum);
+  LV_LAUNCH(wl_scatter_kernel<false>, kWlScanBlocks, kBlock, st, f, (int)clear, S, deg, sp, bsum, qs, lens);
+}
+void k_wl_queue_bits(uint32_t* bits, bool clear, uint64_t S, const uint32_t* deg, const WlSplit& sp_in,
+                     const WlQueues& qs, uint32_t* lens, uint64_t* bsum, hipStream_t st) {
+  WlSplit sp = sp_in;
+  sp.shift = wl_shift(S);
+  LV_LAUNCH(wl_count_kernel<true>, kWlScanBlocks, kBlock, st, bits, S, deg, sp, bsum);
+  LV_LAUNCH(wl_scatter_kernel<true>, kWlScanBlocks, kBlock, st, bits, (int)clear, S, deg, sp, bsum, qs, lens);
 }
 void k_modularity_parts(GView g, bool f64, const uint32_t* memb, double* cin, double* ctot, hipStream_t st) {
   if (!g.S) return;

@@ -40,6 +40,7 @@ struct Graph {
   uint64_t span = 0;
   uint64_t bytes = 0;  // graph + state, as of the last request on it
   bool busy = false;
+  lv_tune_choice tuning{};  // its DF schedule (candidates 0: the defaults)
 };
 
 struct Request {
@@ -57,6 +58,51 @@ struct Finished {
   std::vector<uint32_t> memb;
   std::vector<double> vtot, ctot;
 };
+
+uint64_t splitmix64(uint64_t& x) {
+  uint64_t z = (x += 0x9E3779B97F4A7C15ull);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+// A raw probe batch for graph g: b undirected updates (80% insertions of
+// random id pairs, 20% deletions of random existing edges), both directions
+// of each, as the requests bring them.
+void probe_batch(const lv_graph* g, double fraction, uint64_t seed, std::vector<uint32_t>& du,
+                 std::vector<uint32_t>& dv, std::vector<float>& dw, std::vector<uint32_t>& iu,
+                 std::vector<uint32_t>& iv, std::vector<float>& iw) {
+  const uint64_t S = g->span, E = S ? g->offsets[S] : 0;
+  if (!S || !E) return;
+  const uint64_t b = std::max<uint64_t>(1, (uint64_t)std::llround(fraction * (double)(E / 2)));
+  const uint64_t ni = (uint64_t)std::llround(0.8 * (double)b), nd = b - ni;
+  uint64_t x = seed * 0x2545F4914F6CDD1Dull + 1;
+  for (uint64_t k = 0, tries = 0; k < nd && tries < 16 * nd; ++tries) {
+    const uint64_t i = splitmix64(x) % E;
+    const uint64_t u = (uint64_t)(std::upper_bound(g->offsets, g->offsets + S + 1, i) - g->offsets) - 1;
+    if (u >= S || (g->degrees && i - g->offsets[u] >= g->degrees[u]) || (g->exists && !g->exists[u])) continue;
+    const uint32_t v = g->keys[i];
+    const float w = g->weights ? g->weights[i] : (float)g->weights_f64[i];
+    du.push_back((uint32_t)u);
+    dv.push_back(v);
+    dw.push_back(w);
+    du.push_back(v);
+    dv.push_back((uint32_t)u);
+    dw.push_back(w);
+    ++k;
+  }
+  for (uint64_t k = 0, tries = 0; k < ni && tries < 16 * ni; ++tries) {
+    const uint32_t u = (uint32_t)(splitmix64(x) % S), v = (uint32_t)(splitmix64(x) % S);
+    if (g->exists && (!g->exists[u] || !g->exists[v])) continue;
+    iu.push_back(u);
+    iv.push_back(v);
+    iw.push_back(1.0f);
+    iu.push_back(v);
+    iv.push_back(u);
+    iw.push_back(1.0f);
+    ++k;
+  }
+}
 
 lv_batch as_batch(const std::vector<uint32_t>& u, const std::vector<uint32_t>& v, const std::vector<float>& w) {
   lv_batch b;
@@ -143,11 +189,13 @@ void serve_one(lv_server* s, int k, Request* r, Graph& g, Finished& f) {
   lv_result res;
   std::memset(&res, 0, sizeof res);
   if (r->warm_slot >= 0) {
-    // size the slot's workspace for this graph: a run with no batch on a
-    // copy of the state (the graph's state keeps its own run count)
+    // the graph's DF schedule on this slot, then its workspace sized for
+    // the graph: a run with no batch on a copy of the state (the graph's
+    // state keeps its own run count)
     lv_dstate* tmp = nullptr;
     lv_dbatch* none = nullptr;
-    rc = lv_dstate_create(c, g.span, &tmp);
+    if (g.tuning.candidates) rc = lv_context_add_tuning(c, LV_DYNAMIC_FRONTIER, g.g, &g.tuning);
+    if (rc == LV_OK) rc = lv_dstate_create(c, g.span, &tmp);
     if (rc == LV_OK) rc = lv_dstate_copy(c, tmp, g.s);
     if (rc == LV_OK) rc = lv_dbatch_create(c, nullptr, nullptr, &none);
     if (rc == LV_OK) rc = lv_run(c, LV_DYNAMIC_FRONTIER, g.g, none, tmp, &s->o.options, &res);
@@ -175,6 +223,38 @@ void serve_one(lv_server* s, int k, Request* r, Graph& g, Finished& f) {
     std::lock_guard<std::mutex> lock(s->mu);
     s->err = std::string("request ") + std::to_string(r->ticket) + ": " + lv_last_error(c);
   }
+}
+
+// The DF schedule of a graph being added (ctrl held): a copy of the graph
+// and its state, two runs with no batch (the second builds the hub tables,
+// as a graph's second request does), the probe batch applied by
+// lv_dgraph_update, then lv_autotune on the copy.
+int tune_graph(lv_server* s, const lv_graph* host, Graph& G, uint64_t seed) {
+  lv_context* c = s->ctrl;
+  lv_dgraph* gc = nullptr;
+  lv_dstate* sc = nullptr;
+  lv_dbatch *none = nullptr, *tb = nullptr;
+  std::vector<uint32_t> du, dv, iu, iv;
+  std::vector<float> dw, iw;
+  probe_batch(host, s->o.probe_fraction, seed, du, dv, dw, iu, iv, iw);
+  lv_result res;
+  std::memset(&res, 0, sizeof res);
+  int rc = lv_dgraph_create(c, host, &gc);
+  if (rc == LV_OK) rc = lv_dstate_create(c, G.span, &sc);
+  if (rc == LV_OK) rc = lv_dstate_copy(c, sc, G.s);
+  if (rc == LV_OK) rc = lv_dbatch_create(c, nullptr, nullptr, &none);
+  for (int k = 0; k < 2 && rc == LV_OK; ++k) rc = lv_run(c, LV_DYNAMIC_FRONTIER, gc, none, sc, &s->o.options, &res);
+  const lv_batch del = as_batch(du, dv, dw), ins = as_batch(iu, iv, iw);
+  if (rc == LV_OK) rc = lv_dgraph_update(c, gc, &del, &ins, &tb);
+  lv_tune_choice k{};
+  if (rc == LV_OK && (!du.empty() || !iu.empty()))
+    rc = lv_autotune(c, LV_DYNAMIC_FRONTIER, gc, tb, sc, &s->o.options, 3, &k);
+  if (rc == LV_OK) G.tuning = k;
+  lv_dbatch_destroy(tb);
+  lv_dbatch_destroy(none);
+  lv_dstate_destroy(sc);
+  lv_dgraph_destroy(gc);
+  return rc;
 }
 
 }  // namespace
@@ -232,14 +312,19 @@ void lv_server_options_default(lv_server_options* o) {
   o->path = 2;
   o->modularity = 0;
   o->keep_states = 0;
+  o->autotune = 1;
+  o->probe_fraction = 1e-4;
 }
 
 int lv_server_create(const lv_server_options* o, lv_server** out) {
-  if (!o || !out || o->slots < 1 || o->slots > 64 || o->path < 0 || o->path > 2) return LV_ERR_INVALID;
+  if (!o || !out || o->slots < 1 || o->slots > 64 || o->path < 0 || o->path > 2 || o->autotune < 0 ||
+      o->autotune > 1 || !(o->probe_fraction > 0) || o->probe_fraction > 1)
+    return LV_ERR_INVALID;
   *out = nullptr;
   auto* s = new lv_server();
   s->o = *o;
   int rc = lv_context_create(o->device, &s->ctrl);
+  if (rc == LV_OK) rc = lv_context_set_path(s->ctrl, o->path);
   for (int k = 0; rc == LV_OK && k < o->slots; ++k) {
     lv_context* c = nullptr;
     rc = lv_context_create(o->device, &c);
@@ -298,6 +383,7 @@ int lv_server_add_graph(lv_server* s, const lv_graph* g, const uint32_t* q, cons
       rc = lv_run(s->ctrl, LV_STATIC, G.g, nullptr, G.s, &s->o.options, &res);
     }
   }
+  if (rc == LV_OK && s->o.autotune) rc = tune_graph(s, g, G, (uint64_t)s->graphs.size() + 1);
   if (rc != LV_OK) {
     std::lock_guard<std::mutex> lock(s->mu);
     s->err = lv_last_error(s->ctrl);
@@ -407,6 +493,14 @@ int lv_server_graph_state(lv_server* s, uint32_t graph_id, uint32_t* q, double* 
   if (rc != LV_OK) s->err = lv_last_error(s->ctrl);
   s->cv_work.notify_all();
   return rc;
+}
+
+int lv_server_graph_tuning(lv_server* s, uint32_t graph_id, lv_tune_choice* out) {
+  if (!s || !out) return LV_ERR_INVALID;
+  std::lock_guard<std::mutex> lock(s->mu);
+  if (graph_id >= s->graphs.size()) return LV_ERR_INVALID;
+  *out = s->graphs[graph_id].tuning;
+  return LV_OK;
 }
 
 int lv_server_get_stats(lv_server* s, lv_server_stats* out) {

@@ -36,7 +36,7 @@ LV_OK, LV_ERR_INVALID, LV_ERR_NOMEM, LV_ERR_HIP, LV_ERR_UNSORTED, LV_ERR_NODEVIC
 # Every symbol include/louvain_hip.h declares.
 EXPORTS = [
     "lv_options_default", "lv_status_string", "lv_context_create", "lv_context_destroy", "lv_last_error",
-    "lv_context_set_path", "lv_context_set_schedule", "lv_context_set_pick_less", "lv_context_set_pass_schedule", "lv_context_set_validate", "lv_context_set_sweep_hubs", "lv_context_set_lead_ids", "lv_context_set_workspace_limit", "lv_context_workspace_bytes", "lv_context_set_slabs", "lv_context_set_hub_streams", "lv_context_set_hub_threshold",
+    "lv_context_set_path", "lv_context_set_schedule", "lv_context_set_worklist_schedule", "lv_context_set_pick_less", "lv_context_set_pass_schedule", "lv_context_set_validate", "lv_context_set_sweep_hubs", "lv_context_set_lead_ids", "lv_context_set_workspace_limit", "lv_context_workspace_bytes", "lv_context_set_slabs", "lv_context_set_hub_streams", "lv_context_set_hub_threshold",
     "lv_context_set_profiling", "lv_context_set_graphs", "lv_context_set_hub_tables",
     "lv_dstate_check_hub_tables", "lv_stats_get",
     "lv_stats_reset", "lv_louvain_static", "lv_louvain_naive_dynamic", "lv_louvain_delta_screening",
@@ -50,7 +50,9 @@ EXPORTS = [
     "lv_dgraph_device_bytes", "lv_dstate_device_bytes",
     "lv_server_options_default", "lv_server_create", "lv_server_destroy", "lv_server_add_graph",
     "lv_server_submit", "lv_server_poll", "lv_server_drain", "lv_server_graph_state", "lv_server_get_stats",
-    "lv_server_last_error", "lv_server_now_ms",
+    "lv_server_last_error", "lv_server_now_ms", "lv_server_graph_tuning",
+    "lv_autotune", "lv_context_set_autotune", "lv_context_add_tuning", "lv_context_apply_tuning", "lv_context_tuning",
+    "lv_dstate_clone",
 ]
 
 
@@ -91,7 +93,22 @@ class _Result(C.Structure):
 
 class _ServerOptions(C.Structure):
     _fields_ = [("device", C.c_int32), ("slots", C.c_int32), ("memory_budget", C.c_uint64), ("options", _Options),
-                ("path", C.c_int32), ("modularity", C.c_int32), ("keep_states", C.c_int32)]
+                ("path", C.c_int32), ("modularity", C.c_int32), ("keep_states", C.c_int32), ("autotune", C.c_int32),
+                ("probe_fraction", C.c_double)]
+
+
+class _TuneChoice(C.Structure):
+    _fields_ = [("worklist_hub_degree", C.c_uint32), ("sweep_hub_degree", C.c_uint32),
+                ("worklist_streams", C.c_uint64), ("interleave", C.c_int32), ("stream_fraction", C.c_double),
+                ("ms", C.c_double), ("default_ms", C.c_double), ("modularity", C.c_double),
+                ("default_modularity", C.c_double), ("candidates", C.c_int32)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+    @classmethod
+    def of(cls, d):
+        return cls(*[d[f] if d and f in d else 0 for f, _ in cls._fields_])
 
 
 class _Completion(C.Structure):
@@ -263,6 +280,46 @@ class Context:
         self._check(self.lib.lv_context_set_schedule(self.h, C.c_double(stream_fraction), C.c_uint64(min_streams),
                                                      C.c_uint64(max_streams), C.c_uint64(chunk), int(order)),
                     "lv_context_set_schedule")
+
+    def set_worklist_schedule(self, streams=3072, interleave=False):
+        """Worklist sub-rounds: at most `streams` groups in the id-order kernel,
+        the queue dealt out round-robin (interleave) or in consecutive parts."""
+        self._check(self.lib.lv_context_set_worklist_schedule(self.h, C.c_uint64(streams), int(bool(interleave))),
+                    "lv_context_set_worklist_schedule")
+
+    def autotune(self, approach, dgraph, dbatch, dstate, o=None, reps=3):
+        """lv_autotune: time the candidate schedules on clones of dstate with
+        batch dbatch; the choice (a dict) is cached for dgraph's shape and applied."""
+        if isinstance(approach, str):
+            approach = APPROACHES[approach]
+        oc = (o or LouvainOptions())._c()
+        out = _TuneChoice()
+        self._check(self.lib.lv_autotune(self.h, int(approach), dgraph.h, dbatch.h, dstate.h, C.byref(oc), int(reps),
+                                         C.byref(out)), "lv_autotune")
+        return out.as_dict()
+
+    def set_autotune(self, on=True):
+        self._check(self.lib.lv_context_set_autotune(self.h, int(bool(on))), "lv_context_set_autotune")
+
+    def apply_tuning(self, choice=None):
+        """Set the tuned knobs by hand to a choice (dict), or to the defaults."""
+        k = _TuneChoice.of(choice) if choice else None
+        self._check(self.lib.lv_context_apply_tuning(self.h, C.byref(k) if k else None), "lv_context_apply_tuning")
+
+    def add_tuning(self, approach, dgraph, choice):
+        if isinstance(approach, str):
+            approach = APPROACHES[approach]
+        k = _TuneChoice.of(choice)
+        self._check(self.lib.lv_context_add_tuning(self.h, int(approach), dgraph.h, C.byref(k)), "lv_context_add_tuning")
+
+    def tuning(self, approach, dgraph, graphs=1):
+        """The cached choice for dgraph's shape (candidates 0: none cached)."""
+        if isinstance(approach, str):
+            approach = APPROACHES[approach]
+        out = _TuneChoice()
+        self._check(self.lib.lv_context_tuning(self.h, int(approach), dgraph.h, C.c_uint64(graphs), C.byref(out)),
+                    "lv_context_tuning")
+        return out.as_dict()
 
     def set_graphs(self, on=True):
         """Replay worklist sub-round groups from captured HIP graphs (default on)."""
@@ -562,6 +619,10 @@ class DeviceState:
 
     def copy_from(self, other):
         self.ctx._check(self.ctx.lib.lv_dstate_copy(self.ctx.h, self.h, other.h), "lv_dstate_copy")
+
+    def clone_from(self, other):
+        """lv_dstate_clone: other's state with its hub tables."""
+        self.ctx._check(self.ctx.lib.lv_dstate_clone(self.ctx.h, self.h, other.h), "lv_dstate_clone")
 
     def close(self):
         if self.h:

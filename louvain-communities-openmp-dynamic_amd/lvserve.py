@@ -52,7 +52,7 @@ class Server:
     states, workspaces and batches in flight (None: no limit)."""
 
     def __init__(self, graphs, slots=4, memory_budget=None, options=None, device=0, path=None,
-                 modularity=False, states=None, keep_states=False):
+                 modularity=False, states=None, keep_states=False, autotune=True, probe_fraction=1e-4):
         self.lib = lvhip._LIB
         o = lvhip._ServerOptions()
         self.lib.lv_server_options_default(C.byref(o))
@@ -62,6 +62,7 @@ class Server:
         if path is not None:
             o.path = int(path)
         o.modularity, o.keep_states = int(bool(modularity)), int(bool(keep_states))
+        o.autotune, o.probe_fraction = int(bool(autotune)), float(probe_fraction)
         self.keep_states = keep_states
         self.h = C.c_void_p()
         rc = self.lib.lv_server_create(C.byref(o), C.byref(self.h))
@@ -161,6 +162,13 @@ class Server:
         self._check(self.lib.lv_server_graph_state(self.h, self.ids[name], lvhip._p(q), lvhip._p(v), lvhip._p(c)),
                     "lv_server_graph_state")
         return q, v, c
+
+    def tuning(self, name):
+        """The DF schedule chosen for a graph when it was added (a dict;
+        candidates 0: not tuned)."""
+        k = lvhip._TuneChoice()
+        self._check(self.lib.lv_server_graph_tuning(self.h, self.ids[name], C.byref(k)), "lv_server_graph_tuning")
+        return k.as_dict()
 
     def stats(self):
         st = lvhip._ServerStats()

@@ -1,0 +1,199 @@
+"""Schedule tuner (lv_autotune, lv_context_set_autotune): on a resident
+R-MAT 20 graph and a probe batch made by lv_dgraph_update, the choice is
+never slower than the defaults beyond noise when timed independently on
+clones of the same state, and its DF result stays within 1e-3 of the
+reference's (the oracle, from the same carry-in state); the choice is
+cached per graph shape, knobs set by hand win until autotune is turned on
+again; lv_run_batched tunes a shape on its first call; the server tunes
+every graph it adds."""
+import time
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+Q_TOL = 1e-3
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    import lvhip
+    c = lvhip.Context(0)
+    yield c
+    c.close()
+
+
+@pytest.fixture(scope="module")
+def chain(ctx):
+    """R-MAT 20 from the reference's Static result, two batches of 1e-4|E|
+    (the second builds the hub tables), then the probe batch applied."""
+    import lvgraph
+    import lvhip
+    from oracle import Oracle
+    H = lvgraph.rmat_native(20, 16, 0.57, 0.19, 0.19, 11)
+    o = Oracle()
+    o.set_threads(16)
+    r0 = o.louvain("static", H.view())
+    dg = ctx.device_graph(H.view())
+    st = ctx.device_state(H.span)
+    st.upload(r0.membership, r0.vertex_weight, r0.community_weight)
+    B = int(round(1e-4 * (H.edges // 2)))
+    n_ins = int(round(0.8 * B))
+    for k in range(2):
+        d, i = H.random_batch(B - n_ins, n_ins, 1100 + k)
+        tb = dg.update(d, i)
+        ctx.run(lvhip.LV_DYNAMIC_FRONTIER, dg, tb, st)
+        tb.close()
+        H = H.apply(d, i)
+    prev = st.download()
+    d, i = H.random_batch(B - n_ins, n_ins, 1107)
+    tb = dg.update(d, i)
+    H = H.apply(d, i)
+    yield H, dg, tb, st, prev
+    for h in (tb, dg, st):
+        h.close()
+
+
+def test_tuned_schedule_never_slower_and_keeps_modularity(ctx, chain):
+    import lvhip
+    from oracle import Oracle
+    H, dg, tb, st, prev = chain
+    k = ctx.autotune("dynamic-frontier", dg, tb, st, reps=5)
+    _say(f"choice: hub degree {k['worklist_hub_degree']} streams {k['worklist_streams']} interleave "
+         f"{k['interleave']}; {k['ms']:.3f} ms vs defaults {k['default_ms']:.3f} ms over {k['candidates']} schedules")
+    assert k["candidates"] == 12
+    assert k["ms"] <= k["default_ms"]
+    assert abs(k["modularity"] - k["default_modularity"]) <= Q_TOL
+    assert ctx.tuning("dynamic-frontier", dg)["candidates"] == 12  # cached for the shape
+    # independent timing, interleaved, on clones of the same state
+    ctx.set_autotune(False)
+    clone = ctx.device_state(st.span)
+    times = {"default": [], "tuned": []}
+    try:
+        for rep in range(7):
+            for mode in ("default", "tuned"):
+                ctx.apply_tuning(None if mode == "default" else k)
+                clone.clone_from(st)
+                t0 = time.perf_counter()
+                ctx.run(lvhip.LV_DYNAMIC_FRONTIER, dg, tb, clone)
+                times[mode].append((time.perf_counter() - t0) * 1e3)
+        q = ctx.dmodularity(dg, clone)
+        memb, vt, _ = clone.download()
+    finally:
+        ctx.apply_tuning(None)
+        ctx.set_autotune(True)
+    md, mt = np.median(times["default"]), np.median(times["tuned"])
+    _say(f"independent medians: defaults {md:.3f} ms, tuned {mt:.3f} ms")
+    assert mt <= md * 1.10 + 0.05
+    o = Oracle()
+    o.set_threads(1)
+    td, ti = tb.download()
+    ref = o.louvain("dynamic-frontier", H.view(), td, ti, *prev)
+    q_ref = o.modularity(H.view(), ref.membership)
+    _say(f"tuned DF: Q {q:.6f} reference {q_ref:.6f}")
+    assert abs(q - q_ref) <= Q_TOL
+    assert np.array_equal(vt, ref.vertex_weight)
+    clone.close()
+
+
+def test_knobs_set_by_hand_win_until_autotune(ctx):
+    import lvgraph
+    g = lvgraph.planted_partition(3000, 8, 12, 0.3, 5)
+    dg = ctx.device_graph(g)
+    try:
+        assert ctx.tuning("dynamic-frontier", dg)["candidates"] == 0  # an unseen shape: the defaults
+        assert ctx.tuning("dynamic-frontier", dg)["worklist_hub_degree"] == 256
+        ctx.set_hub_threshold(4096, 512)
+        assert ctx.tuning("dynamic-frontier", dg)["worklist_hub_degree"] == 512
+        ctx.set_hub_threshold()  # every tuned knob at its default: automatic again
+        assert ctx.tuning("dynamic-frontier", dg)["worklist_hub_degree"] == 256
+        ctx.set_worklist_schedule(1536, True)
+        assert ctx.tuning("dynamic-frontier", dg)["worklist_streams"] == 1536
+        ctx.set_autotune(True)
+        t = ctx.tuning("dynamic-frontier", dg)
+        assert (t["worklist_streams"], t["interleave"]) == (3072, 0)
+    finally:
+        ctx.set_worklist_schedule()
+        ctx.set_hub_threshold()
+        dg.close()
+
+
+def test_batched_form_tunes_a_shape_on_its_first_call():
+    import lvgraph
+    import lvhip
+    from oracle import Oracle
+    c = lvhip.Context(0)
+    o = Oracle()
+    o.set_threads(16)
+    try:
+        xs, ys, dels, inss, states0 = [], [], [], [], []
+        for k, n in enumerate((3000, 9000, 20000, 40000, 6000, 15000)):
+            g = lvgraph.planted_partition(n, max(2, n // 1000), 16, 0.3, 4000 + k)
+            b = max(1, int(round(1e-3 * len(g["keys"]) / 2)))
+            d, i = lvgraph.random_batch(g, b - int(round(0.8 * b)), int(round(0.8 * b)), 4100 + k)
+            td, ti = lvgraph.tidy_batch(g, d, i)
+            r = o.louvain("static", g)
+            xs.append(g)
+            ys.append(lvgraph.apply_batch(g, d, i))
+            dels.append(td)
+            inss.append(ti)
+            states0.append((r.membership, r.vertex_weight, r.community_weight))
+        _, voff = lvhip.union_graphs(xs)
+        uy, _ = lvhip.union_graphs(ys)
+        dy = c.device_graph(uy)
+        db = c.device_batch(lvhip.union_batch(dels, voff), lvhip.union_batch(inss, voff))
+        ds = c.device_state(int(voff[-1]))
+        assert c.tuning("dynamic-frontier", dy, len(xs))["candidates"] == 0
+        ds.upload(*lvhip.union_state(states0, voff))
+        t0 = time.perf_counter()
+        res = c.run_batched("dynamic-frontier", dy, db, ds, voff)
+        t1 = time.perf_counter()
+        k = c.tuning("dynamic-frontier", dy, len(xs))
+        assert k["candidates"] == 12
+        ds.upload(*lvhip.union_state(states0, voff))
+        t2 = time.perf_counter()
+        res2 = c.run_batched("dynamic-frontier", dy, db, ds, voff)
+        t3 = time.perf_counter()
+        _say(f"batched: first call {1e3 * (t1 - t0):.1f} ms (tuning), second {1e3 * (t3 - t2):.1f} ms; choice "
+             f"stream fraction {k['stream_fraction']} hub degree {k['worklist_hub_degree']} interleave {k['interleave']}")
+        assert t3 - t2 < t1 - t0
+        for g in range(len(xs)):
+            band = []
+            for t in (1, 16):
+                o.set_threads(t)
+                ref = o.louvain("dynamic-frontier", ys[g], dels[g], inss[g], *states0[g])
+                band.append(o.modularity(ys[g], ref.membership))
+            for r in (res[g], res2[g]):
+                q = o.modularity(ys[g], r.membership)
+                assert min(band) - Q_TOL <= q <= max(band) + Q_TOL, (g, q, band)
+                assert np.array_equal(r.vertexWeight, ref.vertex_weight), g
+        for h in (dy, db, ds):
+            h.close()
+    finally:
+        c.close()
+
+
+def test_server_tunes_each_added_graph():
+    import lvgraph
+    import lvserve
+    graphs = {"rmat16": lvgraph.rmat_native(16, 16, 0.57, 0.19, 0.19, 61)}
+    srv = lvserve.Server(graphs, slots=2, modularity=True)
+    try:
+        k = srv.tuning("rmat16")
+        _say(f"server choice: {k}")
+        assert k["candidates"] == 12
+        g = graphs["rmat16"]
+        reqs = []
+        for j in range(6):
+            d, i = g.random_batch(20, 80, 6100 + j)
+            reqs.append(lvserve.Request(j, "rmat16", d, i, 0.0))
+            g = g.apply(d, i)
+        srv.run(reqs, realtime=False)
+        assert all(r.error is None for r in reqs)
+    finally:
+        srv.close()

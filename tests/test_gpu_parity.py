@@ -457,3 +457,46 @@ def test_single_stream_hubs_in_the_sweep_bit_exact(ctx, oracle):
     assert np.array_equal(r.membership, ref.membership)
     assert np.array_equal(r.vertexWeight, ref.vertex_weight)
     assert (r.iterations, r.passes) == (ref.iterations, ref.passes)
+
+
+def test_single_stream_repeat_bit_exact(ctx):
+    """repeat = 3 (inc/louvain.hxx:1240-1249: every repeat starts from the
+    caller's state; the result is the last repeat's, the times the means):
+    at one stream each approach gives the membership, K, Sigma_tot,
+    iterations and passes of the reference run with repeat = 3, and those of
+    a repeat = 1 run."""
+    import lvhip
+    from oracle import Options, Oracle
+    f = F.load("pp2k")
+    x, y, d, i = _g(f)
+    orc = Oracle()
+    orc.set_threads(1)
+    ctx.set_schedule(1e-12, 1, 1, 16, 2)
+    ctx.set_hub_threshold(1 << 30, 4096)
+    ctx.set_path(0)
+    try:
+        for approach in ("static", "naive-dynamic", "delta-screening", "dynamic-frontier"):
+            got = {}
+            for rep in (1, 3):
+                o = lvhip.LouvainOptions(repeat=rep)
+                oo = Options(repeat=rep)
+                if approach == "static":
+                    r = ctx.louvain(approach, x, o=o)
+                    ref = orc.louvain(approach, x, options=oo)
+                else:
+                    r = ctx.louvain(approach, y, d, i, f["static_memb"], f["static_vtot"], f["static_ctot"], o)
+                    ref = orc.louvain(approach, y, d, i, f["static_memb"], f["static_vtot"], f["static_ctot"], oo)
+                assert np.array_equal(r.membership, ref.membership), (approach, rep)
+                assert np.array_equal(r.vertexWeight, ref.vertex_weight), (approach, rep)
+                live = np.unique(r.membership)
+                assert np.array_equal(r.communityWeight[live], ref.community_weight[live]), (approach, rep)
+                assert (r.iterations, r.passes) == (ref.iterations, ref.passes), (approach, rep)
+                assert r.time > 0
+                got[rep] = r
+            assert np.array_equal(got[1].membership, got[3].membership), approach
+            _say(f"  {approach}: repeat 3 it={got[3].iterations} passes={got[3].passes} "
+                 f"time {got[3].time:.3f} ms (repeat 1: {got[1].time:.3f} ms)")
+    finally:
+        ctx.set_schedule()
+        ctx.set_hub_threshold()
+        ctx.set_path(2)

@@ -269,3 +269,27 @@ def test_planted_mistake_is_caught(mutant):
                 caught = True
                 break
         assert caught, f"{mutant} was not detected by the golden vectors"
+
+
+@pytest.mark.parametrize("approach", ["static", "naive-dynamic", "delta-screening", "dynamic-frontier"])
+def test_repeat_matches_compiled_reference(oracle, approach):
+    """repeat = 3 (inc/louvain.hxx:1240-1249) against the reference compiled
+    in place (oracle/_ref), one thread: same membership, K, Sigma_tot,
+    iterations and passes."""
+    import oracle as O
+    if not os.path.exists(O.REF_SO):
+        pytest.skip("oracle/_ref is not built (the reference sources are not here)")
+    ref = O.RefLib()
+    ref.set_threads(1)
+    f = F.load("pp2k")
+    x, y, d, i = _graphs(f)
+    o = O.Options(repeat=3)
+    if approach == "static":
+        a, b = oracle.louvain(approach, x, options=o), ref.louvain(approach, x, options=o)
+    else:
+        st = (f["static_memb"], f["static_vtot"], f["static_ctot"])
+        a, b = oracle.louvain(approach, y, d, i, *st, o), ref.louvain(approach, y, d, i, *st, o)
+    assert np.array_equal(a.membership, b.membership)
+    assert np.array_equal(a.vertex_weight, b.vertex_weight)
+    assert np.array_equal(a.community_weight, b.community_weight)
+    assert (a.iterations, a.passes) == (b.iterations, b.passes)
