@@ -174,6 +174,7 @@ def batched_main(args):
     one DF update of 1e-3 |E| each (80/20), run together by lv_run_batched
     over the union of the graphs: every launch covers all of them, each
     graph keeping its own M, tolerance, iterations, passes and streams.
+    The previous states are the reference's Static result of each graph.
     Graphs, batches and the previous states are resident in HBM when the
     timed region starts; each step re-uploads the previous states first
     (untimed), since a run updates them."""
@@ -210,10 +211,21 @@ def batched_main(args):
     dx = ctx.device_graph(ux)
     dsx = ctx.device_state(int(voff[-1]))
     static = ctx.run_batched("static", dx, None, dsx, voff)
-    states0 = [(r.membership, r.vertexWeight, r.communityWeight) for r in static]
     dx.close()
     dsx.close()
     log(f"Static of every graph on the GPU, batched ({static[0].time:.1f} ms; {time.time() - t:.1f}s with upload)")
+    # the DF carry-in: the reference's own Static result of each graph, as
+    # in main.cxx:89 (louvainStaticOmp) on the host cores
+    orc, kind = cpu_checker()
+    orc.set_threads(HOST_CORES)
+    t = time.time()
+    states0 = []
+    for g in xs:
+        r = orc.louvain("static", g)
+        states0.append((r.membership, r.vertex_weight, r.community_weight))
+    log(f"reference Static of every graph ({kind}, {HOST_CORES} threads, {time.time() - t:.1f}s)")
+    if args.no_tune:
+        ctx.set_autotune(False)  # the default schedule (lv_run_batched tunes a new shape otherwise)
     uy, voff_y = lvhip.union_graphs(ys)
     assert np.array_equal(voff, voff_y)
     dy = ctx.device_graph(uy)
@@ -239,7 +251,6 @@ def batched_main(args):
     cpu = None
     dq = over = None
     if not args.no_cpu:
-        orc, kind = cpu_checker()
         cores = HOST_CORES
         # The reference's own DF result moves with its thread count (on a
         # weak start, one thread may run several iterations and passes where

@@ -303,11 +303,13 @@ int lv_run(lv_context* ctx, int approach, const lv_dgraph* g, const lv_dbatch* b
 /* Batched form: G graphs side by side in one device graph (graph k owns
  * vertex ids [voff[k], voff[k+1]), no edges between graphs), one batch and
  * one state over all of them (graph k's community ids within its range).
- * Static, ND or DF (not DS) runs on every graph at once -- each kernel
- * launch covers all graphs -- with each graph's own M, tolerance, iteration
- * count, pass count and aggregation decision, as lv_run on that graph alone
- * would (inc/louvain.hxx:1120-1221 per graph), and in as many concurrent
- * streams as it would get alone.  A graph that finishes leaves the later
+ * Static, ND, DS or DF runs on every graph at once -- each kernel launch
+ * covers all graphs -- with each graph's own M (DS's insertion gains too),
+ * tolerance, iteration count, pass count and aggregation decision, as
+ * lv_run on that graph alone would (inc/louvain.hxx:1120-1221 per graph),
+ * in no more concurrent streams than it would get alone, nor than its ids
+ * fill 2048-id chunks (the reference's schedule(dynamic, 2048),
+ * inc/louvain.hxx:594).  A graph that finishes leaves the later
  * levels.  res holds num_graphs results with graph-local membership ids
  * (arrays optional); the state keeps the union ids, each graph's re-based
  * into its own range, ready for the next batch.  community_weight holds

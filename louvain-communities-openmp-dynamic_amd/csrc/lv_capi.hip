@@ -605,13 +605,18 @@ void batched_args(MoveArgs& a, const Batched* bt) {
 }
 
 // Level schedule: graph k walks its own id range in level_streams(its live
-// vertices) streams, as it would alone (so a small graph is one stream
-// beside a large one); a finished graph walks none.
+// vertices) streams, and in no more streams than its ids fill reference
+// chunks (the schedule(dynamic, 2048) of inc/louvain.hxx:594 runs at most
+// that many threads on it at once), so a graph of a few thousand vertices
+// is one stream beside a large one; a finished graph walks none.
+uint64_t graph_streams(const lv_context* c, uint64_t order) {
+  return std::max<uint64_t>(1, std::min(level_streams(c, order), order / kRefChunk));
+}
 void batched_level(lv_context* c, Batched* bt, const std::vector<uint64_t>& lo, const std::vector<uint64_t>& GN) {
   const uint64_t G = bt->G;
   bt->hlo = lo;
   bt->hcs.assign(G + 1, 0);
-  for (uint64_t k = 0; k < G; ++k) bt->hcs[k + 1] = bt->hcs[k] + (bt->done[k] ? 0 : level_streams(c, GN[k]));
+  for (uint64_t k = 0; k < G; ++k) bt->hcs[k + 1] = bt->hcs[k] + (bt->done[k] ? 0 : graph_streams(c, GN[k]));
   bt->nct = bt->hcs[G];
   bt->glo = c->ws<uint64_t>("bt.glo", G + 1);
   bt->gcs = c->ws<uint64_t>("bt.gcs", G + 1);
@@ -1608,7 +1613,13 @@ void invoke_batched(lv_context* c, int approach, const lv_dgraph* x, const lv_db
   uint8_t* vaff = c->ws<uint8_t>("vaff", (S + 7) & ~3ull);
   uint32_t* vcom = c->ws<uint32_t>("vcom", S);
   double* vtot = c->ws<double>("vtot", S);
-  bool worklist = c->path == 2 && approach == LV_DYNAMIC_FRONTIER;
+  bool worklist = c->path == 2 && (approach == LV_DYNAMIC_FRONTIER || approach == LV_DELTA_SCREENING);
+  uint8_t *dsv = nullptr, *dsn = nullptr, *dsc = nullptr;
+  if (approach == LV_DELTA_SCREENING) {
+    dsv = c->ws<uint8_t>("ds.v", S);
+    dsn = c->ws<uint8_t>("ds.n", S);
+    dsc = c->ws<uint8_t>("ds.c", S);
+  }
   SuperGraph y, z;  // taken at the first aggregation, as in invoke()
   y.name = "y";
   z.name = "z";
@@ -1686,8 +1697,30 @@ void invoke_batched(lv_context* c, int approach, const lv_dgraph* x, const lv_db
     k_update_weights(b->du, b->dw, b->nd, b->iu, b->iw, b->ni, ucom, utot, ctot, c->st);
   }
   hipEvent_t e1 = c->record();
-  if (approach == LV_DYNAMIC_FRONTIER) k_frontier(b->du, b->dv, b->nd, b->iu, b->iv, b->ni, ucom, vaff, c->st);
-  else k_fill_u8(vaff, 1, S, c->st);
+  // marking as in invoke(); delta-screening's insertion gains take each
+  // source's M from its own graph
+  const uint8_t* fa = nullptr;
+  if (approach == LV_DYNAMIC_FRONTIER) {
+    k_frontier(b->du, b->dv, b->nd, b->iu, b->iv, b->ni, ucom, vaff, c->st);
+  } else if (approach == LV_DELTA_SCREENING) {
+    k_fill_u8(dsv, 0, S, c->st);
+    k_fill_u8(dsn, 0, S, c->st);
+    k_fill_u8(dsc, 0, S, c->st);
+    k_ds_deletions(b->du, b->dv, b->nd, ucom, dsv, dsn, dsc, c->st);
+    if (b->ni) {
+      uint32_t* tkey = c->ws<uint32_t>("ds.tkey", 2 * b->ni);
+      double* tval = c->ws<double>("ds.tval", 2 * b->ni);
+      uint32_t* tlist = c->ws<uint32_t>("ds.tlist", b->ni);
+      k_fill_u32(tkey, kEmpty, 2 * b->ni, c->st);
+      k_ds_insertions(b->iu, b->iv, b->iw, b->ni, ucom, utot, ctot, 0.0, R, tkey, tval, tlist, dsv, dsn, dsc, c->st,
+                      gid0, bt.gM);
+    }
+    k_ds_sweep(x->view(), ucom, dsn, dsc, dsv, c->st);
+    d2d(c, vaff, dsv, S);
+    fa = dsv;
+  } else {
+    k_fill_u8(vaff, 1, S, c->st);
+  }
   hipEvent_t e2 = c->record();
   hipEvent_t t1 = e2;
   bool t1_set = false;
@@ -1738,9 +1771,9 @@ void invoke_batched(lv_context* c, int approach, const lv_dgraph* x, const lv_db
     batched_level(c, &bt, lo, GN);
     hipEvent_t a0 = c->record();
     if (first) {
-      if (worklist) local_move_worklist(c, cur, ucom, utot, ctot, vaff, nullptr, 0, R, L, 0, &bt);
-      else local_move(c, cur, ucom, utot, ctot, vaff, nullptr, 0, R, L, 0,
-                      approach == LV_DYNAMIC_FRONTIER, &bt);
+      if (worklist) local_move_worklist(c, cur, ucom, utot, ctot, vaff, fa, 0, R, L, 0, &bt);
+      else local_move(c, cur, ucom, utot, ctot, vaff, fa, 0, R, L, 0,
+                      approach == LV_DYNAMIC_FRONTIER || approach == LV_DELTA_SCREENING, &bt);
     } else {
       local_move(c, cur, vcom, vtot, ctot, vaff, nullptr, 0, R, L, 0, false, &bt);
     }
@@ -1845,8 +1878,7 @@ void invoke_batched(lv_context* c, int approach, const lv_dgraph* x, const lv_db
 void run_batched(lv_context* c, int approach, const lv_dgraph* x, const lv_dbatch* b, lv_dstate* s,
                  const uint64_t* voff, uint64_t G, const lv_options* o, lv_result* res) {
   require(c && x && s && voff && G > 0, LV_ERR_INVALID, "null context, graph, state or graph ranges");
-  require(approach == LV_STATIC || approach == LV_NAIVE_DYNAMIC || approach == LV_DYNAMIC_FRONTIER, LV_ERR_INVALID,
-          "the batched form runs Static, Naive-dynamic and Dynamic Frontier");
+  require(approach >= LV_STATIC && approach <= LV_DYNAMIC_FRONTIER, LV_ERR_INVALID, "unknown approach");
   require(approach == LV_STATIC || b != nullptr, LV_ERR_INVALID, "dynamic approaches need a batch");
   require(s->S == x->S, LV_ERR_INVALID, "state span differs from graph span");
   require(voff[0] == 0 && voff[G] == x->S, LV_ERR_INVALID, "graph ranges must cover the span from 0");
@@ -1921,6 +1953,36 @@ double dmodularity(lv_context* c, const lv_dgraph* g, const lv_dstate* s, double
   return c->read1(o);
 }
 
+// Modularity of each graph of a union (graph k on ids [voff[k], voff[k+1]),
+// each with its own M, as after run_batched): the per-community terms of
+// the union summed per graph.
+std::vector<double> dmodularity_groups(lv_context* c, const lv_dgraph* g, const lv_dstate* s, double R,
+                                       const std::vector<uint64_t>& voff) {
+  const uint64_t S = g->S, G = voff.size() - 1;
+  const auto& M2 = g->groups.M2;
+  require(g->groups.voff == voff && M2.size() == G, LV_ERR_INVALID, "per-graph weights not computed for these ranges");
+  double* cin = c->ws<double>("mod.cin", S);
+  double* ct = c->ws<double>("mod.ctot", S);
+  uint64_t* dvoff = c->ws<uint64_t>("mod.voff", G + 1);
+  uint32_t* gid = c->ws<uint32_t>("mod.gid", S);
+  double* gM = c->ws<double>("mod.gM", G);
+  double* o = c->ws<double>("mod.gout", G);
+  std::vector<double> hM(G);
+  for (uint64_t k = 0; k < G; ++k) hM[k] = M2[k] / 2;
+  h2d(c, dvoff, voff.data(), G + 1);
+  h2d(c, gM, hM.data(), G);
+  k_gid_fill(dvoff, G, S, gid, c->st);
+  k_fill_f64(cin, 0, S, c->st);
+  k_fill_f64(ct, 0, S, c->st);
+  k_fill_f64(o, 0, G, c->st);
+  k_modularity_parts(g->view(), g->f64, s->memb, cin, ct, c->st);
+  k_group_modularity(cin, ct, gid, gM, S, R, c->st);
+  k_group_sum(cin, gid, S, o, c->st);
+  std::vector<double> q(G);
+  c->read(q.data(), o, G);
+  return q;
+}
+
 // The knobs a tuning chooses: the worklist and sweep hub degrees (the
 // degree bucket between the id-order kernel and the hub kernels), the
 // worklist id-order streams and queue layout, and the streams per live
@@ -1992,7 +2054,9 @@ std::vector<lv_tune_choice> tune_candidates(const lv_context* c, int approach, b
     v.push_back(k);
   };
   if (batched) {
-    for (double f : {0.001, 0.002, 0.004})
+    // (a graph's streams are capped at its 2048-id chunks, which the default
+    // fraction reaches: the other candidate runs fewer, longer streams)
+    for (double f : {0.002, 0.00025})
       for (uint32_t h : {256u, 512u})
         for (int il : {0, 1}) {
           lv_tune_choice k = default_knobs();
@@ -2071,7 +2135,8 @@ void run(lv_context* c, int approach, const lv_dgraph* x, const lv_dbatch* b, lv
 // Times every candidate schedule of `approach` on graph x with batch b from
 // state s (each trial on a clone of s: s is left as it was), `reps` runs
 // each, median of the host clock around the run; keeps the fastest whose
-// modularity is within 1e-3 of the defaults', and the defaults unless it is
+// modularity (for the batched form, every graph's) is within 1e-3 of the
+// defaults', and the defaults unless it is
 // faster than them by more than 3% (run-to-run noise).  The choice is
 // cached under the graph's shape and applied to the context.
 lv_tune_choice tune(lv_context* c, int approach, const lv_dgraph* x, const lv_dbatch* b, const lv_dstate* s,
@@ -2098,7 +2163,9 @@ lv_tune_choice tune(lv_context* c, int approach, const lv_dgraph* x, const lv_db
     ~Free() { lv_dstate_destroy(t); }
   } free_t{t};
   const auto cands = tune_candidates(c, approach, voff != nullptr);
-  auto measure = [&](const lv_tune_choice& k, double& q) {
+  // quality of a trial: the modularity of the whole graph, and for the
+  // batched form that of each graph
+  auto measure = [&](const lv_tune_choice& k, double& q, std::vector<double>& qg) {
     apply_knobs(c, k);
     std::vector<double> ms;
     for (int r = 0; r < reps; ++r) {
@@ -2110,17 +2177,26 @@ lv_tune_choice tune(lv_context* c, int approach, const lv_dgraph* x, const lv_db
       ms.push_back(std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
     }
     q = dmodularity(c, x, t, o->resolution);
+    if (voff) qg = dmodularity_groups(c, x, t, o->resolution, *voff);
     std::sort(ms.begin(), ms.end());
     return ms[ms.size() / 2];
   };
+  // within 1e-3 of the defaults: the whole graph, and every graph of a union
+  auto close = [](double qa, double qb, const std::vector<double>& ga, const std::vector<double>& gb) {
+    if (std::fabs(qa - qb) > 1e-3 || ga.size() != gb.size()) return false;
+    for (size_t k = 0; k < ga.size(); ++k)
+      if (std::fabs(ga[k] - gb[k]) > 1e-3) return false;
+    return true;
+  };
   double qd = 0;
-  const double md = measure(cands[0], qd);
+  std::vector<double> gd, gk;
+  const double md = measure(cands[0], qd, gd);
   lv_tune_choice best = cands[0];
   double bm = md, bq = qd;
   for (size_t k = 1; k < cands.size(); ++k) {
     double qk = 0;
-    const double mk = measure(cands[k], qk);
-    if (std::fabs(qk - qd) <= 1e-3 && mk < bm) {
+    const double mk = measure(cands[k], qk, gk);
+    if (close(qk, qd, gk, gd) && mk < bm) {
       best = cands[k];
       bm = mk;
       bq = qk;

@@ -312,7 +312,8 @@ void k_ds_deletions(const uint32_t* du, const uint32_t* dv, uint64_t nd, const u
                     uint8_t* neighbors, uint8_t* communities, hipStream_t st);
 void k_ds_insertions(const uint32_t* iu, const uint32_t* iv, const float* iw, uint64_t ni, const uint32_t* vcom,
                      const double* vtot, const double* ctot, double M, double R, uint32_t* tkey, double* tval,
-                     uint32_t* tlist, uint8_t* vertices, uint8_t* neighbors, uint8_t* communities, hipStream_t st);
+                     uint32_t* tlist, uint8_t* vertices, uint8_t* neighbors, uint8_t* communities, hipStream_t st,
+                     const uint32_t* gid = nullptr, const double* gM = nullptr);
 void k_ds_sweep(GView g, const uint32_t* vcom, const uint8_t* neighbors, const uint8_t* communities,
                 uint8_t* vertices, hipStream_t st);
 void k_count_u8(const uint8_t* a, uint8_t v, uint64_t n, unsigned long long* out, hipStream_t st);
@@ -401,6 +402,8 @@ void k_modularity_sum(const double* cin, const double* ctot, uint64_t n, double 
 // level to their vertex range, Sigma_tot into ftot.
 void k_gid_fill(const uint64_t* voff, uint64_t G, uint64_t S, uint32_t* gid, hipStream_t st);
 void k_group_sum(const double* x, const uint32_t* gid, uint64_t S, double* out, hipStream_t st);
+void k_group_modularity(double* cin, const double* ctot, const uint32_t* gid, const double* gM, uint64_t S, double R,
+                        hipStream_t st);
 void k_gid_next(const uint32_t* vcom, const uint32_t* gid, const uint32_t* map, const uint8_t* ex, uint64_t S,
                 uint32_t* gid_next, hipStream_t st);
 void k_active_mask(const uint32_t* gid, uint64_t SL, const uint8_t* ex, const uint8_t* done, uint8_t* mask,

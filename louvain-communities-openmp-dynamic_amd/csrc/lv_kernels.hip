@@ -512,13 +512,16 @@ __global__ void ds_deletions_kernel(const uint32_t* du, const uint32_t* dv, uint
 // scanned sequentially exactly as the reference does: communities listed in
 // first-seen order, sums in batch order, strict '>' from 0.  The group at
 // batch index i uses table slots [2i, 2i + 2 len) and list slots [i, i + len).
+// A union of graphs (gid set) takes each source's M from its own graph.
 __global__ void ds_insertions_kernel(const uint32_t* iu, const uint32_t* iv, const float* iw, uint64_t ni,
                                      const uint32_t* vcom, const double* vtot, const double* ctot, double M, double R,
                                      uint32_t* tkey, double* tval, uint32_t* tlist, uint8_t* vertices,
-                                     uint8_t* neighbors, uint8_t* communities) {
+                                     uint8_t* neighbors, uint8_t* communities, const uint32_t* gid,
+                                     const double* gM) {
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < ni; i += (uint64_t)gridDim.x * blockDim.x) {
     uint32_t u = iu[i];
     if (i > 0 && iu[i - 1] == u) continue;  // not a group head
+    const double Mu = gid ? gM[gid[u]] : M;
     uint64_t len = 1;
     while (i + len < ni && iu[i + len] == u) ++len;
     uint64_t cap = 2 * len;
@@ -542,7 +545,7 @@ __global__ void ds_insertions_kernel(const uint32_t* iu, const uint32_t* iv, con
       if (c == d) continue;
       uint64_t s = ((uint64_t)(c * 2654435761u)) % cap;
       while (key[s] != c) s = (s + 1) % cap;
-      double e = delta_q(val[s], 0.0, vtot[u], ctot[c], ctot[d], M, R);
+      double e = delta_q(val[s], 0.0, vtot[u], ctot[c], ctot[d], Mu, R);
       if (e > emax) { emax = e; cmax = c; }
     }
     if (emax <= 0) continue;
@@ -2715,10 +2718,11 @@ void k_ds_deletions(const uint32_t* du, const uint32_t* dv, uint64_t nd, const u
 }
 void k_ds_insertions(const uint32_t* iu, const uint32_t* iv, const float* iw, uint64_t ni, const uint32_t* vcom,
                      const double* vtot, const double* ctot, double M, double R, uint32_t* tkey, double* tval,
-                     uint32_t* tlist, uint8_t* vertices, uint8_t* neighbors, uint8_t* communities, hipStream_t st) {
+                     uint32_t* tlist, uint8_t* vertices, uint8_t* neighbors, uint8_t* communities, hipStream_t st,
+                     const uint32_t* gid, const double* gM) {
   if (ni)
     LV_LAUNCH(ds_insertions_kernel, grid_for(ni, kBlock), kBlock, st, iu, iv, iw, ni, vcom, vtot, ctot, M, R, tkey,
-              tval, tlist, vertices, neighbors, communities);
+              tval, tlist, vertices, neighbors, communities, gid, gM);
 }
 void k_ds_sweep(GView g, const uint32_t* vcom, const uint8_t* neighbors, const uint8_t* communities,
                 uint8_t* vertices, hipStream_t st) {

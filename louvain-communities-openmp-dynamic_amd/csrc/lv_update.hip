@@ -284,6 +284,19 @@ __global__ void group_sum_kernel(const double* x, const uint32_t* gid, uint64_t 
   }
 }
 
+// Per-community modularity terms of a union of graphs, in place: community
+// c (a vertex id, so in the range of graph gid[c]) gets
+// cin[c] / 2M - R (ctot[c] / 2M)^2 with its own graph's M = gM[gid[c]]
+// (modularityByOmp, inc/properties.hxx:205-233, per graph).
+__global__ void group_modularity_kernel(double* cin, const double* ctot, const uint32_t* gid, const double* gM,
+                                        uint64_t S, double R) {
+  for (uint64_t c = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; c < S; c += (uint64_t)gridDim.x * blockDim.x) {
+    const double m = gM[gid[c]];
+    const double x = m > 0 ? ctot[c] / (2 * m) : 0.0;
+    cin[c] = m > 0 ? cin[c] / (2 * m) - R * x * x : 0.0;
+  }
+}
+
 // Graph of the next level's super-vertices: gid_next[map[vcom[u]]] = gid[u].
 __global__ void gid_next_kernel(const uint32_t* vcom, const uint32_t* gid, const uint32_t* map, const uint8_t* ex,
                                 uint64_t S, uint32_t* gid_next) {
@@ -382,6 +395,10 @@ void k_gid_fill(const uint64_t* voff, uint64_t G, uint64_t S, uint32_t* gid, hip
 }
 void k_group_sum(const double* x, const uint32_t* gid, uint64_t S, double* out, hipStream_t st) {
   if (S) LV_UPD_LAUNCH(group_sum_kernel, blocks_for(S, kB), kB, st, x, gid, S, out);
+}
+void k_group_modularity(double* cin, const double* ctot, const uint32_t* gid, const double* gM, uint64_t S, double R,
+                        hipStream_t st) {
+  if (S) LV_UPD_LAUNCH(group_modularity_kernel, blocks_for(S, kB), kB, st, cin, ctot, gid, gM, S, R);
 }
 void k_gid_next(const uint32_t* vcom, const uint32_t* gid, const uint32_t* map, const uint8_t* ex, uint64_t S,
                 uint32_t* gid_next, hipStream_t st) {
