@@ -174,11 +174,12 @@ uint64_t lv_context_workspace_bytes(const lv_context* ctx);
  * lead_iterations local-moving iterations of a Static run's first pass run
  * lead_streams streams over ascending chunks of lead_chunk ids (over the
  * lead id range, see lv_context_set_lead_ids; the other ids under the wide
- * schedule); every pass after the first runs late_streams streams (chunks of
- * late_chunk ids, 0: spread) when late_streams is not 0.  The reference's
- * schedule(dynamic, 2048) loop (inc/louvain.hxx:594) with 16 threads, where
- * its visit order decides the communities that form first.
- * Defaults: 1, 16, 2048, 0, 0. */
+ * schedule); every pass after the first runs late_streams streams over
+ * chunks of late_chunk ids handed out in ascending order (0: spread; at most
+ * the streams the level's schedule gives) when late_streams is not 0.  The
+ * reference's schedule(dynamic, 2048) loop (inc/louvain.hxx:594) with 16
+ * threads, where its visit order decides the communities that form first.
+ * Defaults: 1, 16, 2048, 256, 2048. */
 int lv_context_set_pass_schedule(lv_context* ctx, int lead_iterations, uint64_t lead_streams, uint64_t lead_chunk,
                                  uint64_t late_streams, uint64_t late_chunk);
 /* The ids a lead iteration sweeps under the lead schedule: [0, ids), then

@@ -102,16 +102,22 @@ struct lv_context {
   std::map<std::string, lv_tune_choice> tuned;
   // Sweep schedules by pass: the first lead_iters iterations of the first
   // pass run lead_streams streams over ascending chunks of lead_chunk
-  // entries; passes after the first run late_streams streams (chunks of
-  // late_chunk, 0: spread) when late_streams is set.
-  // Defaults (Static): the first iteration sweeps the id prefix that holds a
+  // entries; passes after the first run late_streams streams over chunks of
+  // late_chunk entries handed out in ascending order (0: spread; no more
+  // streams than the level's schedule gives) when late_streams is set.
+  // Defaults: Static's first iteration sweeps the id prefix that holds a
   // third of the edges (at most kLeadIdsMax ids) in 16 streams of 2048-id
-  // ascending chunks, the reference's own sweep pattern there; the rest of
-  // that iteration and every later one use the wide schedule.
+  // ascending chunks, the reference's own sweep pattern there, and the rest
+  // of the first pass runs the wide schedule; later passes walk 2048-id
+  // chunks in ascending order in 256 streams.  (Measured on R-MAT 22 Static:
+  // spread chunks in the later passes gave 0.0660-0.0679, ascending ones
+  // 0.0678-0.0689 over 5 runs each of 64, 256 and 1024 streams; the same
+  // schedule over the first pass's later iterations gained nothing and took
+  // 15-40% longer.)
   int lead_iters = 1;
   uint64_t lead_streams = 16, lead_chunk = 2048;
   uint64_t lead_ids = 0;  // ids under the lead schedule: 0 auto, ~0 all, else [0, lead_ids)
-  uint64_t late_streams = 0, late_chunk = 0;
+  uint64_t late_streams = 256, late_chunk = 2048;
   int pick_less_iters = 0;      // first iterations of each pass that only move to lower ids
   // validated moves of the id-order kernel: 0 off (default), 1 sweeps (every
   // pass but a DF / DS worklist first pass), 2 sweeps and worklists
@@ -671,9 +677,13 @@ bool batched_step(lv_context* c, Batched* bt, int L) {
 // slots for every degree it runs (<= hub_min)
 void id_order_regions(lv_context* c, MoveArgs& a, const Level& lv) {
   a.gblocks = std::min<uint64_t>(a.max_streams, 1u << 16);
-  uint64_t dmax = kHubMin;
-  if (c->hub_min > kHubMin) dmax = std::max<uint64_t>(dmax, std::min<uint64_t>(c->hub_min, lv.hubs->maxdeg));
-  a.gcap = std::max<uint64_t>(kHybridHbm, next_pow2(2 * dmax));
+  // the largest degree the kernel visits with a region: its hub threshold
+  // (a worklist's is at most kHubMin), or above kHubMin the level's largest
+  // hub (hubs in the sweep have tables of their own)
+  const uint64_t hm = a.xids ? kHubMin : a.hub_min;
+  uint64_t dmax = std::min<uint64_t>(hm, kHubMin);
+  if (hm > kHubMin) dmax = std::max<uint64_t>(kHubMin, std::min<uint64_t>(hm, lv.hubs->maxdeg));
+  a.gcap = std::max<uint64_t>(dmax >= kHubMin ? kHybridHbm : kHybridLds, next_pow2(2 * dmax));
   size_t slots = (size_t)a.gcap * ((a.gblocks + 3) & ~3ull);
   a.gkey = c->ws<uint32_t>("g.key", slots);
   a.gpos = c->ws<uint32_t>("g.pos", slots);
@@ -914,7 +924,7 @@ int local_move(lv_context* c, const Level& lv, uint32_t* vcom, const double* vto
         a.max_streams = s0;
         a.chunk = ch0;
       } else {
-        a.max_streams = lead_now ? std::min(c->lead_streams, s0) : late ? c->late_streams : s0;
+        a.max_streams = lead_now ? std::min(c->lead_streams, s0) : late ? std::min(c->late_streams, s0) : s0;
         a.chunk = lead_now ? c->lead_chunk : late ? c->late_chunk : ch0;
       }
     }
@@ -962,7 +972,7 @@ int local_move_worklist(lv_context* c, const Level& lv, uint32_t* vcom, const do
   batched_args(a, bt);
   batched_schedule(c, a, lv, bt);
   if (bt) batched_begin(c, bt);
-  a.sparse = 1;  // queue entries read 64 at a time
+  a.sparse = 2;  // queue entries read a window at a time, their flags once
   a.interleave = c->interleave;
   unsigned long long* cnt0 = a.cnt;
   const uint64_t S = lv.g.S, stride = std::max<uint64_t>(S, 1), fwords = (S >> 5) + 2;

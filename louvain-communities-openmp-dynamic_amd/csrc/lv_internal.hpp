@@ -185,7 +185,11 @@ struct MoveArgs {
   uint64_t min_chunk;          // unused (kept zero)
   uint64_t max_streams;        // lane groups launched (streams walking at once)
   uint64_t chunk;              // list entries per dynamically assigned chunk
-  int sparse;                  // id-order kernel: scan 64-entry windows by ballot, visit flagged only
+  int sparse;                  // id-order kernel: scan windows of a group's lanes by ballot, visit
+                               //   flagged only: 1 reading each later flag again after a visit
+                               //   (sweeps), 2 reading the flags once per window (worklist
+                               //   queues: an entry is flagged when queued and only its own
+                               //   visit clears its flag)
   int interleave;              // id-order kernel, chunk 0, no ctab: group g takes entries g, g + N,
                                //   g + 2N, ... of the list (N groups), in that order
   unsigned long long* work;    // chunk counter, zero before each launch

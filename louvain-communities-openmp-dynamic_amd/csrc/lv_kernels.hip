@@ -253,22 +253,47 @@ __device__ __forceinline__ void ht_move(const MoveArgs& a, uint32_t x, double w,
   unsafeAtomicAdd(&a.ht_w[s], w);
 }
 
-// ht_move with x's table size already read (cx > kHtDropped).
-__device__ __forceinline__ void ht_move_cap(const MoveArgs& a, uint32_t x, uint32_t cx, double w, uint32_t from,
-                                            uint32_t to) {
-  const uint64_t off = a.ht_off[x];
-  int64_t s = ht_slot(a.ht_key, off, cx, from, false);
-  if (s >= 0) {
-    atomicAdd(&a.ht_cnt[s], -1);
-    unsafeAtomicAdd(&a.ht_w[s], -w);
+// ht_move with x's table size (cx > kHtDropped) and first slot read: the
+// slots of `from` and `to` are searched together, one key load of each per
+// probe step (linear probing from each one's home slot, at most cx steps).
+__device__ __forceinline__ void ht_move_cap(const MoveArgs& a, uint32_t x, uint32_t cx, uint64_t off, double w,
+                                            uint32_t from, uint32_t to) {
+  uint32_t sf = slot_of(from, cx), st = slot_of(to, cx);
+  int64_t pf = -1, pt = -1;
+  bool df = false, dt = false;
+  for (uint32_t probe = 0; probe < cx && !(df && dt); ++probe) {
+    const uint32_t kf = df ? 0u : __hip_atomic_load(&a.ht_key[off + sf], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    uint32_t kt = dt ? 0u : __hip_atomic_load(&a.ht_key[off + st], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (!df) {
+      if (kf == from) {
+        pf = (int64_t)(off + sf);
+        df = true;
+      } else if (kf == kEmpty) {
+        df = true;
+      } else {
+        sf = (sf + 1) & (cx - 1);
+      }
+    }
+    if (!dt) {
+      if (kt == kEmpty) kt = atomicCAS(&a.ht_key[off + st], kEmpty, to);
+      if (kt == kEmpty || kt == to) {
+        pt = (int64_t)(off + st);
+        dt = true;
+      } else {
+        st = (st + 1) & (cx - 1);
+      }
+    }
   }
-  s = ht_slot(a.ht_key, off, cx, to, true);
-  if (s < 0) {
+  if (pf >= 0) {
+    atomicAdd(&a.ht_cnt[pf], -1);
+    unsafeAtomicAdd(&a.ht_w[pf], -w);
+  }
+  if (pt < 0) {
     ht_drop(a.ht_cap, x, a.ht_ndrop);
     return;
   }
-  atomicAdd(&a.ht_cnt[s], 1);
-  unsafeAtomicAdd(&a.ht_w[s], w);
+  atomicAdd(&a.ht_cnt[pt], 1);
+  unsafeAtomicAdd(&a.ht_w[pt], w);
 }
 
 // A vertex u that moved from community `from` to `to` (row at o, d edges)
@@ -283,6 +308,7 @@ __device__ __forceinline__ void flag_neighbours(const MoveArgs& a, const WT* W, 
   const uint32_t serial = a.fsame ? cur_serial(a) : 0;
   for (uint32_t b = 0; b < d; b += R * G) {
     uint32_t v[R], sv[R], cx[R];
+    uint64_t ox[R];
     double w[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
@@ -294,16 +320,22 @@ __device__ __forceinline__ void flag_neighbours(const MoveArgs& a, const WT* W, 
     for (int r = 0; r < R; ++r) {
       sv[r] = serial;
       cx[r] = 0;
+      ox[r] = 0;
       if (v[r] == u) continue;
       __hip_atomic_store(&a.vaff[v[r]], (uint8_t)1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       if (a.fsame && v[r] > u) sv[r] = __hip_atomic_load(&a.stamp[v[r]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      if (a.ht_cap) cx[r] = __hip_atomic_load(&a.ht_cap[v[r]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (a.ht_cap) {
+        // (the table's first slot is read with its size: a table's slots
+        // never move, so the load is good whatever the size says)
+        cx[r] = __hip_atomic_load(&a.ht_cap[v[r]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        ox[r] = a.ht_off[v[r]];
+      }
     }
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       // flagged by a lower id and not visited this iteration: next sub-round
       if (sv[r] != serial) mark_same(a, v[r]);
-      if (cx[r] > kHtDropped) ht_move_cap(a, v[r], cx[r], w[r], from, to);
+      if (cx[r] > kHtDropped) ht_move_cap(a, v[r], cx[r], ox[r], w[r], from, to);
     }
   }
 }
@@ -720,21 +752,37 @@ __device__ __forceinline__ uint64_t row_digest(const MoveArgs& a, uint32_t u, ui
   return dig;
 }
 
+constexpr int kBestBatch = 2;
+
 template <int G>
 __device__ __forceinline__ void table_best(const MoveArgs& a, uint32_t cu, double Ku, double M, const uint32_t* key,
                                            const uint32_t* pos, const double* val, uint64_t cap, int lane, double& be,
                                            uint32_t& bp, uint32_t& bc, unsigned long long& c_cand) {
-  const double vd = tab_get(key, val, cap, cu);
   const double Sd = ld_dev(&a.ctot[cu]);
-  for (uint64_t s = lane; s < cap; s += G) {
-    uint32_t c = key[s];
-    if (c == kEmpty) continue;
-    ++c_cand;
-    if (c == cu) continue;
-    if (a.pick_less && c > cu) continue;
-    double e = delta_q(val[s], vd, Ku, ld_dev(&a.ctot[c]), Sd, M, a.R);
-    uint32_t p = pos[s];
-    if (e > 0 && better(e, p, be, bp)) { be = e; bp = p; bc = c; }
+  const double vd = tab_get(key, val, cap, cu);
+  // kBestBatch of the lane's slots at a time: their Sigma_tot gathers are
+  // issued together (the best is order-free: larger gain, then earlier
+  // first position)
+  constexpr int B = kBestBatch;
+  for (uint64_t s0 = 0; s0 < cap; s0 += (uint64_t)B * G) {
+    uint32_t c[B];
+    double t[B];
+#pragma unroll
+    for (int r = 0; r < B; ++r) {
+      const uint64_t s = s0 + (uint64_t)r * G + lane;
+      c[r] = s < cap ? key[s] : kEmpty;
+      c_cand += c[r] != kEmpty;
+      if (c[r] == cu || (a.pick_less && c[r] > cu)) c[r] = kEmpty;
+      t[r] = c[r] != kEmpty ? ld_dev(&a.ctot[c[r]]) : 0.0;
+    }
+#pragma unroll
+    for (int r = 0; r < B; ++r) {
+      if (c[r] == kEmpty) continue;
+      const uint64_t s = s0 + (uint64_t)r * G + lane;
+      const double e = delta_q(val[s], vd, Ku, t[r], Sd, M, a.R);
+      const uint32_t p = pos[s];
+      if (e > 0 && better(e, p, be, bp)) { be = e; bp = p; bc = c[r]; }
+    }
   }
 }
 
@@ -834,10 +882,11 @@ __global__ __launch_bounds__(BLOCK) void move_kernel(MoveArgs a) {
     }
     // One vertex (offset o, degree d, community cu, weight Ku) through the
     // group; every thread of the block calls it.  Its barriers are the
-    // block's, or the wave's in the id-order kernel (a group per wave, each
-    // with its own LDS table and HBM region: its groups run independently).
+    // block's, or the wave's in the id-order kernel (one or more groups per
+    // wave, each with its own LDS table and HBM region: the groups run
+    // independently).
     auto group_sync = [&]() __attribute__((always_inline)) {
-      if constexpr (MODE == MOVE_HYBRID && G == 64) {
+      if constexpr (MODE == MOVE_HYBRID && G <= 64) {
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
@@ -947,12 +996,21 @@ __global__ __launch_bounds__(BLOCK) void move_kernel(MoveArgs a) {
     };
     if constexpr (MODE == MOVE_HYBRID) {
       if (a.sparse) {
-        // 64-entry windows (one wave per group): the flags of a window are
-        // read at once and only the flagged entries are visited, in order;
-        // after each visit the later flags are read again, so a vertex
-        // flagged by an earlier one of the window is still visited in this
-        // sweep (inc/louvain.hxx:602 then :598).
-        for (uint64_t j0 = 0; j0 < clen; j0 += 64) {
+        // G-entry windows (an entry per lane of the group): the flags of a
+        // window are read at once and only the flagged entries are visited,
+        // in order.  Sweeps (sparse 1) read the later flags again after
+        // each visit, so a vertex flagged by an earlier one of the window is
+        // still visited in this sweep (inc/louvain.hxx:602 then :598); a
+        // worklist queue (sparse 2) holds flagged entries whose flags only
+        // their own visits clear, and a flag set again meanwhile belongs to
+        // the next sub-round or iteration, so its flags are read once.
+        const int gsh = (int)(threadIdx.x % 64) / G * G;  // the group's first lane in its wave
+        auto group_mask = [&](bool p) __attribute__((always_inline)) {
+          const unsigned long long m = __ballot(p);
+          if constexpr (G == 64) return m;
+          else return (m >> gsh) & ((1ull << G) - 1);
+        };
+        for (uint64_t j0 = 0; j0 < clen; j0 += G) {
           // (a flagged entry's offset, degree, community and K are loaded
           // with the window; its community changes only when it is visited)
           const uint64_t idx = c0 + (j0 + lane) * step;
@@ -968,7 +1026,8 @@ __global__ __launch_bounds__(BLOCK) void move_kernel(MoveArgs a) {
                  !vertex_skipped(a, ul);
           }
           bool live = ok && (a.choose_only || ld_dev(&a.vaff[ul]));
-          unsigned long long mask = __ballot(live);
+          const bool once = a.choose_only || a.sparse == 2;
+          unsigned long long mask = group_mask(live);
           while (mask) {
             if (live && !pre) {
               ol = a.g.off[ul];
@@ -977,10 +1036,10 @@ __global__ __launch_bounds__(BLOCK) void move_kernel(MoveArgs a) {
               pre = true;
             }
             const int b = __ffsll((long long)mask) - 1;
-            visit(true, (uint32_t)__shfl(ul, b, 64), (uint64_t)__shfl(ol, b, 64), (uint32_t)__shfl(dl, b, 64),
-                  (uint32_t)__shfl(cl, b, 64), __shfl(kl, b, 64));
-            live = ok && lane > b && (a.choose_only || ld_dev(&a.vaff[ul]));
-            mask = __ballot(live);
+            visit(true, (uint32_t)__shfl(ul, b, G), (uint64_t)__shfl(ol, b, G), (uint32_t)__shfl(dl, b, G),
+                  (uint32_t)__shfl(cl, b, G), __shfl(kl, b, G));
+            live = once ? live && lane > b : ok && lane > b && ld_dev(&a.vaff[ul]);
+            mask = group_mask(live);
           }
         }
         continue;

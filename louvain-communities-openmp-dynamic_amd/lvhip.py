@@ -270,7 +270,7 @@ class Context:
     def workspace_bytes(self):
         return int(self.lib.lv_context_workspace_bytes(self.h))
 
-    def set_pass_schedule(self, lead_iterations=1, lead_streams=16, lead_chunk=2048, late_streams=0, late_chunk=0):
+    def set_pass_schedule(self, lead_iterations=1, lead_streams=16, lead_chunk=2048, late_streams=256, late_chunk=2048):
         self._check(self.lib.lv_context_set_pass_schedule(self.h, int(lead_iterations), C.c_uint64(lead_streams),
                                                           C.c_uint64(lead_chunk), C.c_uint64(late_streams),
                                                           C.c_uint64(late_chunk)), "lv_context_set_pass_schedule")
