@@ -63,6 +63,7 @@ def host_info():
 
 HBM_PEAK_GBPS = 8000.0
 KERNEL_NAME = "move_kernel<256, 64, 512, float, 2>"
+FUSED_KERNEL = "wl_fused_kernel<float>"
 SMALL_HUB_KERNEL = "hub_mid_kernel<256, 2048u, 1024u, float>"
 MID_HUB_KERNEL = "hub_mid_kernel<512, 2048u, 16384u, float>"
 def pmc_summary(scale):
@@ -86,8 +87,11 @@ def algorithmic_bytes(st, tier):
     visited edge (key, weight, community gather), 29 B per processed vertex
     (offset, degree, community, K, own Sigma_tot, flag), 8 B per candidate
     community (Sigma_tot gather), 20 B per move (two Sigma_tot updates and
-    the membership store) and 1 B per flagged neighbour."""
-    return (12 * st["edges"][tier] + 29 * st["vertices"][tier] + 8 * st["candidates"][tier]
+    the membership store) and 1 B per flagged neighbour.  The hubs evaluated
+    from their neighbour-community tables (slot 1) read table slots instead
+    of edges: 16 B each (key, count, weight), counted in their "edges"."""
+    per_edge = 16 if tier == 1 else 12
+    return (per_edge * st["edges"][tier] + 29 * st["vertices"][tier] + 8 * st["candidates"][tier]
             + 20 * st["moves"][tier] + st["moved_edges"][tier])
 
 
@@ -676,17 +680,30 @@ def main():
              4: kernel_line(4, "hub batch (hub_prep/insert/best/apply/flag)")}
     dom = max((0, 2, 3), key=lambda k: st["move_ms"][k])
     main_k, hub_k = lines[dom], lines[4]
+    if st["fused_launches"] and st["fused_ms"] > max(st["move_ms"][k] for k in (0, 2, 3)):
+        # sub-rounds whose id-order kernel and tabled hubs ran as one launch
+        # (wl_fused_kernel): its bytes are those of slots 0 and 1 (which also
+        # count any sub-round that ran them apart: move_launches of slot 0)
+        fl, fms = st["fused_launches"], st["fused_ms"]
+        b = algorithmic_bytes(st, 0) + algorithmic_bytes(st, 1)
+        main_k = {"kernel": FUSED_KERNEL, "achieved": b / (fms * 1e-3) / 1e9, "avg_launch_us": fms * 1e3 / fl,
+                  "bytes_per_launch": b / fl, "launches": fl, "unfused_id_order_launches": st["move_launches"][0],
+                  "per_launch": {k: (st[k][0] + st[k][1]) / fl for k in ("vertices", "edges", "candidates", "moves")}}
+        dom = None
     traffic = pmc_traffic(main_k["kernel"], args.scale)
     roofline = {"bound": "hbm", "achieved": main_k["achieved"], "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                 "frac": main_k["achieved"] / HBM_PEAK_GBPS, "traffic": traffic and traffic["bytes_per_launch"],
                 "kernel": main_k["kernel"], "avg_launch_us": main_k["avg_launch_us"],
                 "bytes_per_launch": main_k["bytes_per_launch"], "launches": main_k["launches"],
                 "traffic_source": traffic and traffic["source"], "hub_batch": hub_k,
-                "other_kernels": [lines[k] for k in (0, 2, 3) if k != dom]}
-    visits = sum(st["edges"])  # edges visited by local-moving, every kernel, over the K timed runs
+                "other_kernels": [lines[k] for k in (0, 2, 3) if k != dom and st["move_launches"][k]]}
+    # edges visited by local-moving over the K timed runs: every kernel but
+    # the tabled hubs, which read their tables instead
+    visits = sum(st["edges"]) - st["edges"][1]
     log("kernel stats per slot: " + "; ".join(
         f"{k}: launches {st['move_launches'][k]} ms {st['move_ms'][k]:.3f} vertices {st['vertices'][k]} "
-        f"edges {st['edges'][k]} candidates {st['candidates'][k]} moves {st['moves'][k]}" for k in range(5)))
+        f"edges {st['edges'][k]} candidates {st['candidates'][k]} moves {st['moves'][k]}" for k in range(5)) +
+        f"; fused launches {st['fused_launches']} ms {st['fused_ms']:.3f}")
     log(f"roofline: {main_k['kernel']}: {main_k['bytes_per_launch'] / 1e6:.2f} MB algorithmic per launch, "
         f"{main_k['avg_launch_us']:.1f} us -> {main_k['achieved']:.1f} GB/s; per launch " + ", ".join(
             f"{k} {main_k['per_launch'][k]:.0f}" for k in main_k["per_launch"]))

@@ -217,6 +217,11 @@ typedef struct lv_stats {
   uint64_t moves[5];
   uint64_t moved_edges[5];
   uint64_t runs;
+  /* worklist sub-rounds whose id-order kernel and tabled hubs ran as one
+   * launch: their time (profiling) and count; their work counts in slots 0
+   * and 1 */
+  double fused_ms;
+  uint64_t fused_launches;
 } lv_stats;
 int lv_context_set_profiling(lv_context* ctx, int on);
 /* Worklist sub-round groups (the DF / DS first pass) replayed from a captured

@@ -560,6 +560,19 @@ void launch_move(lv_context* c, const MoveArgs& a, int tier, bool f64, hipStream
   if (c->profiling) c->prof.push_back({slot, {e0, c->record(s)}});
 }
 
+// A timed launch into the context's statistics: a local-moving stats slot,
+// or kSlotFused (the fused worklist launch, id-order kernel and tabled hubs).
+constexpr int kSlotFused = 5;
+void add_prof(lv_context* c, int slot, float ms) {
+  if (slot == kSlotFused) {
+    c->stats.fused_ms += ms;
+    c->stats.fused_launches += 1;
+  } else {
+    c->stats.move_ms[slot] += ms;
+    c->stats.move_launches[slot] += 1;
+  }
+}
+
 // Worklist queues: degrees above hub_min_wl are hubs; up to this one they
 // run a block each (LDS table), above it in the multi-block hub batch.
 uint32_t wl_big_hub(const lv_context* c) { return std::max<uint32_t>(c->hub_min_wl, kMidHubMax); }
@@ -1046,28 +1059,37 @@ int local_move_worklist(lv_context* c, const Level& lv, uint32_t* vcom, const do
       // (multi-block batch), these on two side streams at once.  Like the
       // reference's threads, each sees the moves the others have made.
       // Without the edge-scanning kernels everything runs on the context's
-      // stream (the tabled hubs first): the cross-stream waits cost more
-      // than the overlap gains.
+      // stream: the tabled hubs and the id-order kernel in one launch, then
+      // the flags of the tabled hubs that moved (the cross-stream waits of
+      // side streams cost more than their overlap gains).
       const int sides = scan_hubs ? 2 : 0;
       hipStream_t hs = sides ? c->side[0] : c->st;
-      if (S) {
+      MoveArgs ah = a;  // the tabled hubs (stats slot 1)
+      ah.list = qs.q[4];
+      ah.n_dev = lens + 4;
+      ah.cnt = cnt0 + (size_t)1 * CNT_N;
+      a.list = qs.q[0];
+      a.n_dev = lens;
+      a.cnt = cnt0;
+      // Every hub has a table: the tabled hubs and the id-order kernel in one
+      // launch, then the flags of the hubs that moved
+      hipEvent_t f0 = nullptr;
+      if (c->profiling && S && a.ht_cap && !sides) f0 = c->record();
+      const bool fused = S && a.ht_cap && !sides && k_wl_fused(a, ah, lv.f64, c->st);
+      if (f0 && fused) c->prof.push_back({kSlotFused, {f0, c->record()}});
+      if (fused) k_moved_flag(ah, qs, lens, true, c->st);  // with the hubs whose tables dropped, kept for later
+      if (S && !fused) {
         if (sides) {
           c->make_side();
           c->fork(sides);
           hs = c->side[0];
         }
-        if (a.ht_cap) {  // stats slot 1
-          a.list = qs.q[4];
-          a.n_dev = lens + 4;
-          a.cnt = cnt0 + (size_t)1 * CNT_N;
-          launch_move(c, a, kTierTabHub, lv.f64, hs);
-          k_moved_flag(a, qs, lens, !scan_hubs, hs);  // with the hubs whose tables dropped, kept for later
+        if (a.ht_cap) {
+          launch_move(c, ah, kTierTabHub, lv.f64, hs);
+          k_moved_flag(ah, qs, lens, !scan_hubs, hs);  // with the hubs whose tables dropped, kept for later
         }
       }
-      a.list = qs.q[0];
-      a.n_dev = lens;
-      a.cnt = cnt0;
-      launch_move(c, a, kTierHybrid, lv.f64);
+      if (!fused) launch_move(c, a, kTierHybrid, lv.f64);
       if (S) {
         if (scan_hubs) {
           for (int q = 2; q >= 1; --q) {  // stats slots 3 (mid) and 2 (small)
@@ -1358,8 +1380,21 @@ void build_hub_tables(lv_context* c, const lv_dgraph* x, lv_dstate* s, const uin
              s->ht_next + 1, c->st);
 }
 
+// Static re-optimises the whole graph from singletons, and its modularity
+// falls with the number of streams moving vertices at once: on R-MAT 24
+// (33,554 streams by the stream fraction) four runs averaged 0.0600, with
+// at most 8192 streams 0.0615 (the reference: 0.0612-0.0614), in 2.1 s
+// instead of 1.5 s.  The other approaches keep the context's schedule.
+constexpr uint64_t kStaticStreams = 8192;
+
 void invoke(lv_context* c, int approach, const lv_dgraph* x, const lv_dbatch* b, lv_dstate* s, const lv_options* o,
             int& l, int& p, RunTimes& rt, uint64_t& affected) {
+  struct Keep {
+    lv_context* c;
+    uint64_t max_streams;
+    ~Keep() { c->max_streams = max_streams; }
+  } keep{c, c->max_streams};
+  if (approach == LV_STATIC) c->max_streams = std::min<uint64_t>(c->max_streams, kStaticStreams);
   const uint64_t S = x->S;
   const double R = o->resolution;
   const int L = o->max_iterations, P = o->max_passes;
@@ -1580,10 +1615,7 @@ void run(lv_context* c, int approach, const lv_dgraph* x, const lv_dbatch* b, lv
     c->ev_used = 0;
     c->prof.clear();
     invoke(c, approach, x, b, s, o, l, p, rt, aff);
-    for (auto& pr : c->prof) {
-      c->stats.move_ms[pr.first] += elapsed(pr.second.first, pr.second.second);
-      c->stats.move_launches[pr.first] += 1;
-    }
+    for (auto& pr : c->prof) add_prof(c, pr.first, elapsed(pr.second.first, pr.second.second));
   }
   if (res) {
     res->iterations = l;
@@ -1909,10 +1941,7 @@ void run_batched(lv_context* c, int approach, const lv_dgraph* x, const lv_dbatc
   RunTimes rt;
   std::vector<int> l, p;
   invoke_batched(c, approach, x, b, s, o, vo, l, p, rt);
-  for (auto& pr : c->prof) {
-    c->stats.move_ms[pr.first] += elapsed(pr.second.first, pr.second.second);
-    c->stats.move_launches[pr.first] += 1;
-  }
+  for (auto& pr : c->prof) add_prof(c, pr.first, elapsed(pr.second.first, pr.second.second));
   const uint64_t S = x->S;
   unsigned long long* aff = c->ws<unsigned long long>("bt.aff", G);
   k_range_count(c->ws<uint8_t>("vaff", (S + 7) & ~3ull), c->ws<uint64_t>("bt.voff", G + 1), G, aff, c->st);

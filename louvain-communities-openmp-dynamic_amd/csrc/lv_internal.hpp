@@ -276,6 +276,11 @@ void k_flag_nonzero(const uint32_t* a, uint64_t n, uint8_t* flags, hipStream_t s
 // Neighbours of the tabled hubs that moved (a.mv_*), flagged over the grid;
 // with carry, the entries of worklist queues 1-3 back to the next sub-round.
 void k_moved_flag(const MoveArgs& a, const WlQueues& qs, const uint32_t* lens, bool carry, hipStream_t st);
+// A worklist sub-round's id-order kernel (a, queue 0) and tabled hubs (h,
+// queue 4) in one launch; false (nothing launched) for schedules it does
+// not cover (chunks handed out in order, validated moves, hubs in the
+// sweep, per-graph chunk tables, fewer than 64 streams).
+bool k_wl_fused(const MoveArgs& a, const MoveArgs& h, bool f64, hipStream_t st);
 // Sources of a batch's insertions (iu) that now have more than kHtMinDeg
 // edges and no table: a table each from the spare slots (bump counter
 // next[0], slots up to total), listed for a build; with no spare slot left,

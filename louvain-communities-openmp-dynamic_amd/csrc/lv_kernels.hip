@@ -810,8 +810,11 @@ enum { MOVE_LDS = 0, MOVE_HBM = 1, MOVE_HYBRID = 2, MOVE_HYBRID_X = 3 };
 //       MOVE_HYBRID one wave per stream: LDS table when 2 deg <= CAP, else the
 //                   block's HBM region of gcap slots; vertices with degree above
 //                   hub_min are left to the MOVE_HBM pre-pass.
+//
+// The body runs as block `bid` of `nblk` (a launch of its own, or the
+// first blocks of the fused worklist launch).
 template <int BLOCK, int G, int CAP, class WT, int MODE_>
-__global__ __launch_bounds__(BLOCK) void move_kernel(MoveArgs a) {
+__device__ __forceinline__ void move_body(const MoveArgs& a, uint32_t bid, uint32_t nblk) {
   constexpr bool EXTRAS = MODE_ == MOVE_HYBRID_X;
   constexpr int MODE = EXTRAS ? MOVE_HYBRID : MODE_;
   constexpr int GPB = BLOCK / G;
@@ -835,7 +838,7 @@ __global__ __launch_bounds__(BLOCK) void move_kernel(MoveArgs a) {
   const uint64_t chunk = dyn ? a.chunk : ((n + streams - 1) / streams ? (n + streams - 1) / streams : 1);
   // a.ctab (id-order kernel, one group per block): chunk bounds per chunk
   const uint64_t nchunks = a.ctab ? a.nct : ilv ? (n < streams ? n : streams) : (n + chunk - 1) / chunk;
-  uint64_t next = (uint64_t)blockIdx.x * GPB;
+  uint64_t next = (uint64_t)bid * GPB;
   const WT* W = (const WT*)a.g.w;
   // LDS table of this group; HBM region of this block
   uint32_t* lkey = s_key + grp * LCAP;
@@ -844,12 +847,12 @@ __global__ __launch_bounds__(BLOCK) void move_kernel(MoveArgs a) {
   uint32_t *hkey = nullptr, *hpos = nullptr;
   double* hval = nullptr;
   if constexpr (MODE == MOVE_HBM) {
-    uint64_t base = (uint64_t)blockIdx.x * a.hcap;
+    uint64_t base = (uint64_t)bid * a.hcap;
     hkey = a.hkey + base;
     hpos = a.hpos + base;
     hval = a.hval + base;
   } else if constexpr (MODE == MOVE_HYBRID) {
-    uint64_t base = ((uint64_t)blockIdx.x * GPB + grp) * a.gcap;  // the group's region
+    uint64_t base = ((uint64_t)bid * GPB + grp) * a.gcap;  // the group's region
     hkey = a.gkey + base;
     hpos = a.gpos + base;
     hval = a.gval + base;
@@ -865,7 +868,7 @@ __global__ __launch_bounds__(BLOCK) void move_kernel(MoveArgs a) {
       cb = s_take;
       __syncthreads();
     } else {
-      next += (uint64_t)gridDim.x * GPB;
+      next += (uint64_t)nblk * GPB;
     }
     if (cb >= nchunks) break;
     const uint64_t ch = cb + grp;
@@ -1077,6 +1080,11 @@ __global__ __launch_bounds__(BLOCK) void move_kernel(MoveArgs a) {
     unsigned long long me = block_sum_u64<BLOCK>(c_medge);
     if (threadIdx.x == 0) add_counts(a.cnt, v, e, c, m, me);
   }
+}
+
+template <int BLOCK, int G, int CAP, class WT, int MODE_>
+__global__ __launch_bounds__(BLOCK) void move_kernel(MoveArgs a) {
+  move_body<BLOCK, G, CAP, WT, MODE_>(a, blockIdx.x, gridDim.x);
 }
 
 // ------------------------------------------------------- hub batches
@@ -1652,16 +1660,18 @@ __global__ __launch_bounds__(BLOCK) void hub_mid_kernel(MoveArgs a) {
 // in the hub's row (inc/louvain.hxx:461), found by walking the row.  A hub
 // whose table was dropped since the queue was built joins the next
 // sub-round (it is then queued by degree).
+// (as block `bid` of `nblk`: a launch of its own, or the last blocks of the
+// fused worklist launch)
 template <class WT>
-__global__ __launch_bounds__(kBlock) void hub_tab_kernel(MoveArgs a) {
+__device__ __forceinline__ void hub_tab_body(const MoveArgs& a, uint32_t bid, uint32_t nblk) {
   constexpr int R = kHtMaxCap / 64;  // table slots per lane
   const int lane = threadIdx.x % 64;
   const uint64_t n = list_len(a);
-  const uint64_t nw = (uint64_t)gridDim.x * (kBlock / 64);
+  const uint64_t nw = (uint64_t)nblk * (kBlock / 64);
   const WT* W = (const WT*)a.g.w;
   double el_acc = 0;
   unsigned long long c_vert = 0, c_edge = 0, c_cand = 0, c_move = 0, c_medge = 0;
-  for (uint64_t k = (uint64_t)blockIdx.x * (kBlock / 64) + threadIdx.x / 64; k < n; k += nw) {  // wave-uniform
+  for (uint64_t k = (uint64_t)bid * (kBlock / 64) + threadIdx.x / 64; k < n; k += nw) {  // wave-uniform
     // the hub's scalars, read by lane 0 (its table may be dropped meanwhile)
     uint32_t u = 0, d = 0, cap = 0, act = 0;
     if (lane == 0) {
@@ -1770,7 +1780,7 @@ __global__ __launch_bounds__(kBlock) void hub_tab_kernel(MoveArgs a) {
       if (a.gel) unsafeAtomicAdd(&a.gel[a.gid[u]], be);
       else el_acc += be;
       ++c_vert;
-      c_edge += d;
+      c_edge += cap;  // (its "edges": the table slots read, 16 B each)
     }
   }
   const double t = block_sum<kBlock>(el_acc);
@@ -1781,6 +1791,21 @@ __global__ __launch_bounds__(kBlock) void hub_tab_kernel(MoveArgs a) {
     unsigned long long me = block_sum_u64<kBlock>(c_medge);
     if (threadIdx.x == 0) add_counts(a.cnt, v, ed, c, m, me);
   }
+}
+
+template <class WT>
+__global__ __launch_bounds__(kBlock) void hub_tab_kernel(MoveArgs a) {
+  hub_tab_body<WT>(a, blockIdx.x, gridDim.x);
+}
+
+// One worklist sub-round in one launch when every hub has a table: blocks
+// [0, nlo) run the id-order kernel over queue 0 (a, four one-wave streams a
+// block), the others evaluate the tabled hubs of queue 4 (h), side by side
+// as the reference's threads visit hubs and other vertices at once.
+template <class WT>
+__global__ __launch_bounds__(kBlock) void wl_fused_kernel(MoveArgs a, MoveArgs h, uint32_t nlo) {
+  if (blockIdx.x < nlo) move_body<kBlock, 64, kHybridLds, WT, MOVE_HYBRID>(a, blockIdx.x, nlo);
+  else hub_tab_body<WT>(h, blockIdx.x - nlo, gridDim.x - nlo);
 }
 
 // Neighbours of the tabled hubs that moved in the kernel before (list
@@ -2881,13 +2906,27 @@ void k_hub_mid(const MoveArgs& a, int cls, hipStream_t st) {
     else LV_LAUNCH((hub_mid_kernel<kMidBlock, kMidSlots, kMidHubMax, float>), kMidGrid, kMidBlock, st, a);
   }
 }
+// about one wave per tabled hub of a sub-round (their number grows with the
+// graph: ~750 at 4M vertices); a launch with few hubs then dispatches few
+// blocks
+static uint64_t hub_tab_grid(const MoveArgs& a) {
+  return std::min<uint64_t>(1024, std::max<uint64_t>(128, a.g.S >> 14));
+}
 void k_hub_tab(const MoveArgs& a, hipStream_t st) {
-  // about one wave per tabled hub of a sub-round (their number grows with
-  // the graph: ~750 at 4M vertices); a launch with few hubs then dispatches
-  // few blocks
-  const uint64_t grid = std::min<uint64_t>(1024, std::max<uint64_t>(128, a.g.S >> 14));
+  const uint64_t grid = hub_tab_grid(a);
   if (a.g.w && a.f64_weights) LV_LAUNCH(hub_tab_kernel<double>, grid, kBlock, st, a);
   else LV_LAUNCH(hub_tab_kernel<float>, grid, kBlock, st, a);
+}
+bool k_wl_fused(const MoveArgs& a, const MoveArgs& h, bool f64, hipStream_t st) {
+  // the id-order part as move_tier launches it with spread chunks and four
+  // streams a block; other schedules take the separate launches
+  const uint64_t streams = a.max_streams ? a.max_streams : 1;
+  if (a.chunk || a.validate || a.xids || a.ctab || streams < 64) return false;
+  const uint64_t nlo = grid_for(streams, 4, (a.gblocks + 3) / 4);
+  const uint64_t grid = nlo + hub_tab_grid(h);
+  if (f64) LV_LAUNCH(wl_fused_kernel<double>, grid, kBlock, st, a, h, (uint32_t)nlo);
+  else LV_LAUNCH(wl_fused_kernel<float>, grid, kBlock, st, a, h, (uint32_t)nlo);
+  return true;
 }
 __global__ void flag_nonzero_kernel(const uint32_t* a, uint64_t n, uint8_t* flags) {
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)

@@ -80,7 +80,8 @@ class _Batch(C.Structure):
 class _Stats(C.Structure):
     _fields_ = [("move_ms", C.c_double * 5), ("move_launches", C.c_uint64 * 5), ("vertices", C.c_uint64 * 5),
                 ("edges", C.c_uint64 * 5), ("candidates", C.c_uint64 * 5), ("moves", C.c_uint64 * 5),
-                ("moved_edges", C.c_uint64 * 5), ("runs", C.c_uint64)]
+                ("moved_edges", C.c_uint64 * 5), ("runs", C.c_uint64), ("fused_ms", C.c_double),
+                ("fused_launches", C.c_uint64)]
 
 
 class _Result(C.Structure):
@@ -346,7 +347,8 @@ class Context:
         """Local-moving kernel statistics per degree tier (lv_stats)."""
         st = _Stats()
         self._check(self.lib.lv_stats_get(self.h, C.byref(st)), "lv_stats_get")
-        return {f: (list(getattr(st, f)) if f != "runs" else st.runs) for f, _ in _Stats._fields_}
+        return {f: (list(getattr(st, f)) if isinstance(getattr(st, f), C.Array) else getattr(st, f))
+                for f, _ in _Stats._fields_}
 
     def _check(self, rc, what):
         if rc != 0:

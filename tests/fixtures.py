@@ -1,4 +1,5 @@
-"""Loading helpers for the golden vectors in tests/golden/*.npz."""
+"""Loading helpers for the golden vectors in tests/golden/*.npz, and the
+bar of the DF tests against the reference at several thread counts."""
 import os
 
 import numpy as np
@@ -40,3 +41,16 @@ def rows_as_maps(off, deg, keys, w):
         o, d = int(off[c]), int(deg[c])
         out.append(dict(zip(keys[o:o + d].tolist(), w[o:o + d].tolist())))
     return out
+
+
+def within_band(q, band, tol):
+    """DF against the reference's results at several thread counts from the
+    same carry-in state: not below the lowest by more than tol, and, where
+    those results agree with each other within tol, not above the highest by
+    more than tol.  (Where they do not -- a small graph on which one thread
+    count runs a second pass and another stops after one -- the reference
+    gives no single result to hold the GPU's above.)"""
+    lo, hi = min(band), max(band)
+    if q < lo - tol:
+        return False
+    return hi - lo > tol or q <= hi + tol

@@ -11,6 +11,8 @@ import time
 import numpy as np
 import pytest
 
+import fixtures as F
+
 pytestmark = pytest.mark.gpu
 
 Q_TOL = 1e-3
@@ -170,7 +172,7 @@ def test_batched_form_tunes_a_shape_on_its_first_call():
                 band.append(o.modularity(ys[g], ref.membership))
             for r in (res[g], res2[g]):
                 q = o.modularity(ys[g], r.membership)
-                assert min(band) - Q_TOL <= q <= max(band) + Q_TOL, (g, q, band)
+                assert F.within_band(q, band, Q_TOL), (g, q, band)
                 assert np.array_equal(r.vertexWeight, ref.vertex_weight), g
         for h in (dy, db, ds):
             h.close()
