@@ -250,6 +250,29 @@ def batched_main(args):
     ps = [r.passes for r in res]
     log(f"batched DF over {G} graphs: {wall * 1e3:.2f} ms per run (median of {len(walls)}; device events "
         f"{np.median(gpu_ms):.2f} ms), iterations {min(it)}..{max(it)}, passes {min(ps)}..{max(ps)}")
+    # the same runs with every local-moving launch timed by events: the
+    # roofline of the dominant kernel (the id-order kernel over the union,
+    # slot 0; the block-per-hub kernels and the hub batch beside it)
+    ctx.stats_reset()
+    ctx.set_profiling(True)
+    for step in range(args.steps):
+        ds.upload(*u0)
+        ctx.run_batched("dynamic-frontier", dy, db, ds, voff, copy_back=False)
+    ctx.set_profiling(False)
+    st = ctx.stats()
+    dom = max(range(5), key=lambda k: st["move_ms"][k])
+    launches = max(st["move_launches"][dom], 1)
+    b = algorithmic_bytes(st, dom)
+    ms = st["move_ms"][dom]
+    gbps = b / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+    names = {0: "move_kernel<256, 64, 512, float, 2>", 2: SMALL_HUB_KERNEL, 3: MID_HUB_KERNEL,
+             4: "hub batch (hub_prep/insert/best/apply/flag)"}
+    roofline = {"bound": "hbm", "achieved": gbps, "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": gbps / HBM_PEAK_GBPS,
+                "traffic": None, "kernel": names.get(dom, str(dom)), "avg_launch_us": ms * 1e3 / launches,
+                "bytes_per_launch": b / launches, "launches": launches,
+                "per_slot_ms": st["move_ms"], "per_slot_launches": st["move_launches"]}
+    log(f"roofline: {roofline['kernel']}: {b / launches / 1e6:.2f} MB algorithmic per launch, "
+        f"{roofline['avg_launch_us']:.1f} us -> {gbps:.1f} GB/s; per-slot ms {[round(x, 2) for x in st['move_ms']]}")
     ds.upload(*u0)
     res = ctx.run_batched("dynamic-frontier", dy, db, ds, voff)
     cpu = None
@@ -291,7 +314,7 @@ def batched_main(args):
            "config": {"workload": f"{G} planted-partition graphs in one union, n log-uniform 1k-200k, avg degree "
                                   f"U[8,32], inter-block U[0.2,0.4], one DF update of 1e-3|E| each; seed "
                                   f"{args.batched_seed}", "vertices": S_tot, "edges": E_tot},
-           "cpu_baseline": cpu}
+           "roofline": roofline, "cpu_baseline": cpu}
     for d in (dy, db, ds):
         d.close()
     ctx.close()

@@ -313,9 +313,8 @@ int lv_run(lv_context* ctx, int approach, const lv_dgraph* g, const lv_dbatch* b
  * covers all graphs -- with each graph's own M (DS's insertion gains too),
  * tolerance, iteration count, pass count and aggregation decision, as
  * lv_run on that graph alone would (inc/louvain.hxx:1120-1221 per graph),
- * in no more concurrent streams than it would get alone, nor than its ids
- * fill 2048-id chunks (the reference's schedule(dynamic, 2048),
- * inc/louvain.hxx:594).  A graph that finishes leaves the later
+ * and in as many concurrent streams as it would get alone.  A graph that
+ * finishes leaves the later
  * levels.  res holds num_graphs results with graph-local membership ids
  * (arrays optional); the state keeps the union ids, each graph's re-based
  * into its own range, ready for the next batch.  community_weight holds

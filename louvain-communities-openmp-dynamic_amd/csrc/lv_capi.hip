@@ -624,18 +624,13 @@ void batched_args(MoveArgs& a, const Batched* bt) {
 }
 
 // Level schedule: graph k walks its own id range in level_streams(its live
-// vertices) streams, and in no more streams than its ids fill reference
-// chunks (the schedule(dynamic, 2048) of inc/louvain.hxx:594 runs at most
-// that many threads on it at once), so a graph of a few thousand vertices
-// is one stream beside a large one; a finished graph walks none.
-uint64_t graph_streams(const lv_context* c, uint64_t order) {
-  return std::max<uint64_t>(1, std::min(level_streams(c, order), order / kRefChunk));
-}
+// vertices) streams, as it would alone (so a small graph is one stream
+// beside a large one); a finished graph walks none.
 void batched_level(lv_context* c, Batched* bt, const std::vector<uint64_t>& lo, const std::vector<uint64_t>& GN) {
   const uint64_t G = bt->G;
   bt->hlo = lo;
   bt->hcs.assign(G + 1, 0);
-  for (uint64_t k = 0; k < G; ++k) bt->hcs[k + 1] = bt->hcs[k] + (bt->done[k] ? 0 : graph_streams(c, GN[k]));
+  for (uint64_t k = 0; k < G; ++k) bt->hcs[k + 1] = bt->hcs[k] + (bt->done[k] ? 0 : level_streams(c, GN[k]));
   bt->nct = bt->hcs[G];
   bt->glo = c->ws<uint64_t>("bt.glo", G + 1);
   bt->gcs = c->ws<uint64_t>("bt.gcs", G + 1);
@@ -2093,9 +2088,7 @@ std::vector<lv_tune_choice> tune_candidates(const lv_context* c, int approach, b
     v.push_back(k);
   };
   if (batched) {
-    // (a graph's streams are capped at its 2048-id chunks, which the default
-    // fraction reaches: the other candidate runs fewer, longer streams)
-    for (double f : {0.002, 0.00025})
+    for (double f : {0.001, 0.002, 0.004})
       for (uint32_t h : {256u, 512u})
         for (int il : {0, 1}) {
           lv_tune_choice k = default_knobs();

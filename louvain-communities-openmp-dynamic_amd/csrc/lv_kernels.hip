@@ -2440,10 +2440,14 @@ __global__ __launch_bounds__(kBlock) void caps_sum_kernel(const uint32_t* deg, u
 // contiguous segment (one block each), then an ordered scatter.  A thread
 // reads 16 consecutive flag bytes (one 16-byte load), so a block covers a
 // 4096-id tile per step and needs one block-wide scan per tile.
-constexpr uint64_t kWlTile = 16 * kBlock;
+// TB threads a block: a tile of 16 TB ids.  The launchers take TB =
+// min(512, segment / 16), so a segment of up to 8192 ids is one tile (one
+// round of loads, one scan); the tiles of a larger segment follow each other.
 static_assert(kWlScanBlocks % kBlock == 0, "whole segments per block count pass");
+template <int TB>
+constexpr int wl_count_bits() { return TB <= 256 ? 13 : TB <= 512 ? 14 : 15; }  // 16 TB fits
 
-// ids of block b: [b << shift, (b + 1) << shift), a multiple of kWlTile
+// ids of block b: [b << shift, (b + 1) << shift), a multiple of the tile
 
 // Flag words of ids id0 .. id0 + 15 (id0 a multiple of 16; 0 at or past hi;
 // the flag array holds whole words up to S).
@@ -2554,9 +2558,9 @@ __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long x)
 }
 
 // Per block (segment) and queue: the flagged ids it holds.
-template <bool BITS>
-__global__ __launch_bounds__(kBlock) void wl_count_kernel(const void* f, uint64_t S, const uint32_t* deg,
-                                                          WlSplit sp, uint64_t* bsum) {
+template <bool BITS, int TB>
+__global__ __launch_bounds__(TB) void wl_count_kernel(const void* f, uint64_t S, const uint32_t* deg,
+                                                      WlSplit sp, uint64_t* bsum) {
   __shared__ unsigned long long s_c[kWlQueues];
   const uint64_t lo = (uint64_t)blockIdx.x << sp.shift, hi = min(S, lo + (1ull << sp.shift));
   if (sp.dirty && !sp.dirty[blockIdx.x]) {  // no flag was set in this segment
@@ -2566,7 +2570,7 @@ __global__ __launch_bounds__(kBlock) void wl_count_kernel(const void* f, uint64_
   if (threadIdx.x < kWlQueues) s_c[threadIdx.x] = 0;
   __syncthreads();
   unsigned long long c[kWlQueues] = {};
-  for (uint64_t id0 = lo + 16 * (uint64_t)threadIdx.x; id0 < hi; id0 += kWlTile) {
+  for (uint64_t id0 = lo + 16 * (uint64_t)threadIdx.x; id0 < hi; id0 += 16 * (uint64_t)TB) {
     uint32_t m[kWlQueues];
     wl_bits16(wl_mask16<BITS>(f, id0, hi), deg, S, sp, id0, m);
 #pragma unroll
@@ -2588,11 +2592,13 @@ __global__ __launch_bounds__(kBlock) void wl_count_kernel(const void* f, uint64_
 // at most 4096 each, packed 13 bits apart); queue 4 takes a second scan in
 // the tiles that hold any of its ids.  Writes stay below S, the queues'
 // capacity.
-template <bool BITS>
-__global__ __launch_bounds__(kBlock) void wl_scatter_kernel(void* f, int clear, uint64_t S, const uint32_t* deg,
-                                                            WlSplit sp, const uint64_t* bsum, WlQueues qs,
-                                                            uint32_t* lens) {
-  static_assert(kWlTile < 8192 && 4 * 13 <= 64 && kWlQueues == 5, "packed tile counts");
+template <bool BITS, int TB>
+__global__ __launch_bounds__(TB) void wl_scatter_kernel(void* f, int clear, uint64_t S, const uint32_t* deg,
+                                                        WlSplit sp, const uint64_t* bsum, WlQueues qs,
+                                                        uint32_t* lens) {
+  constexpr int CB = wl_count_bits<TB>();
+  constexpr uint64_t CM = (1ull << CB) - 1;
+  static_assert(16 * TB < (1 << CB) && 4 * CB <= 64 && kWlQueues == 5, "packed tile counts");
   __shared__ unsigned long long s_pre[kWlQueues];
   const uint64_t lo = (uint64_t)blockIdx.x << sp.shift, hi = min(S, lo + (1ull << sp.shift));
   const bool skip = sp.dirty && !sp.dirty[blockIdx.x];  // block-uniform: no flag in this segment
@@ -2602,7 +2608,7 @@ __global__ __launch_bounds__(kBlock) void wl_scatter_kernel(void* f, int clear, 
   {
     const uint32_t nb = blockIdx.x == 0 ? kWlScanBlocks : blockIdx.x;
     unsigned long long loc[kWlQueues] = {};
-    for (uint32_t j = threadIdx.x; j < nb; j += kBlock)
+    for (uint32_t j = threadIdx.x; j < nb; j += TB)
 #pragma unroll
       for (int q = 0; q < kWlQueues; ++q) loc[q] += bsum[q * kWlScanBlocks + j];
 #pragma unroll
@@ -2625,21 +2631,21 @@ __global__ __launch_bounds__(kBlock) void wl_scatter_kernel(void* f, int clear, 
   }
   if (skip) return;
   if (sp.dirty && clear && threadIdx.x == 0) sp.dirty[blockIdx.x] = 0;
-  for (uint64_t t0 = lo; t0 < hi; t0 += kWlTile) {  // block-uniform trip count
+  for (uint64_t t0 = lo; t0 < hi; t0 += 16 * (uint64_t)TB) {  // block-uniform trip count
     const uint64_t id0 = t0 + 16 * (uint64_t)threadIdx.x;
     const uint32_t f16 = wl_mask16<BITS>(f, id0, hi);
     uint32_t m[kWlQueues];
     wl_bits16(f16, deg, S, sp, id0, m);
     uint64_t packed = 0;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) packed |= (uint64_t)__popc(m[q]) << (13 * q);
+    for (int q = 0; q < 4; ++q) packed |= (uint64_t)__popc(m[q]) << (CB * q);
     uint64_t tot;
-    const uint64_t ex = block_scan_excl<kBlock>(packed, tot);
+    const uint64_t ex = block_scan_excl<TB>(packed, tot);
     uint64_t ex4 = 0, tot4 = 0;
-    if (__syncthreads_or(m[4] != 0)) ex4 = block_scan_excl<kBlock>((uint64_t)__popc(m[4]), tot4);
+    if (__syncthreads_or(m[4] != 0)) ex4 = block_scan_excl<TB>((uint64_t)__popc(m[4]), tot4);
 #pragma unroll
     for (int q = 0; q < kWlQueues; ++q) {
-      uint64_t e = r[q] + (q < 4 ? (ex >> (13 * q)) & 0x1FFF : ex4);
+      uint64_t e = r[q] + (q < 4 ? (ex >> (CB * q)) & CM : ex4);
       uint32_t bits = m[q];
       while (bits) {
         const int j = __ffs(bits) - 1;
@@ -2647,7 +2653,7 @@ __global__ __launch_bounds__(kBlock) void wl_scatter_kernel(void* f, int clear, 
         if (e < S) qs.q[q][e] = (uint32_t)(id0 + j);
         ++e;
       }
-      r[q] += q < 4 ? (tot >> (13 * q)) & 0x1FFF : tot4;
+      r[q] += q < 4 ? (tot >> (CB * q)) & CM : tot4;
     }
     if (clear) wl_clear16<BITS>(f, id0, hi, f16);
   }
@@ -3044,19 +3050,31 @@ void k_gather_u32(const uint32_t* a, const uint32_t* idx, uint64_t n, uint32_t* 
 void k_caps_sum(const uint32_t* deg, uint64_t n, uint32_t thr, unsigned long long* out, hipStream_t st) {
   if (n) LV_LAUNCH(caps_sum_kernel, grid_for(n, kBlock * 8, kReduceGrid), kBlock, st, deg, n, thr, out);
 }
+// The two compaction kernels with min(512, segment / 16) threads a block:
+// a segment of up to 8192 ids is one tile.  (1024 threads would cover 16384
+// ids a tile, but the scatter kernel then spills registers.)
+template <bool BITS>
+static void wl_launch(void* f, bool clear, uint64_t S, const uint32_t* deg, const WlSplit& sp, const WlQueues& qs,
+                      uint32_t* lens, uint64_t* bsum, hipStream_t st) {
+  if (sp.shift >= 13) {
+    LV_LAUNCH((wl_count_kernel<BITS, 512>), kWlScanBlocks, 512, st, f, S, deg, sp, bsum);
+    LV_LAUNCH((wl_scatter_kernel<BITS, 512>), kWlScanBlocks, 512, st, f, (int)clear, S, deg, sp, bsum, qs, lens);
+  } else {
+    LV_LAUNCH((wl_count_kernel<BITS, 256>), kWlScanBlocks, 256, st, f, S, deg, sp, bsum);
+    LV_LAUNCH((wl_scatter_kernel<BITS, 256>), kWlScanBlocks, 256, st, f, (int)clear, S, deg, sp, bsum, qs, lens);
+  }
+}
 void k_wl_queue(uint8_t* f, bool clear, uint64_t S, const uint32_t* deg, const WlSplit& sp_in, const WlQueues& qs,
                 uint32_t* lens, uint64_t* bsum, hipStream_t st) {
   WlSplit sp = sp_in;
   sp.shift = wl_shift(S);
-  LV_LAUNCH(wl_count_kernel<false>, kWlScanBlocks, kBlock, st, f, S, deg, sp, bsum);
-  LV_LAUNCH(wl_scatter_kernel<false>, kWlScanBlocks, kBlock, st, f, (int)clear, S, deg, sp, bsum, qs, lens);
+  wl_launch<false>(f, clear, S, deg, sp, qs, lens, bsum, st);
 }
 void k_wl_queue_bits(uint32_t* bits, bool clear, uint64_t S, const uint32_t* deg, const WlSplit& sp_in,
                      const WlQueues& qs, uint32_t* lens, uint64_t* bsum, hipStream_t st) {
   WlSplit sp = sp_in;
   sp.shift = wl_shift(S);
-  LV_LAUNCH(wl_count_kernel<true>, kWlScanBlocks, kBlock, st, bits, S, deg, sp, bsum);
-  LV_LAUNCH(wl_scatter_kernel<true>, kWlScanBlocks, kBlock, st, bits, (int)clear, S, deg, sp, bsum, qs, lens);
+  wl_launch<true>(bits, clear, S, deg, sp, qs, lens, bsum, st);
 }
 void k_modularity_parts(GView g, bool f64, const uint32_t* memb, double* cin, double* ctot, hipStream_t st) {
   if (!g.S) return;

@@ -156,7 +156,7 @@ def test_batched_form_tunes_a_shape_on_its_first_call():
         res = c.run_batched("dynamic-frontier", dy, db, ds, voff)
         t1 = time.perf_counter()
         k = c.tuning("dynamic-frontier", dy, len(xs))
-        assert k["candidates"] == 8
+        assert k["candidates"] == 12
         ds.upload(*lvhip.union_state(states0, voff))
         t2 = time.perf_counter()
         res2 = c.run_batched("dynamic-frontier", dy, db, ds, voff)

@@ -180,12 +180,12 @@ def test_config4_batched_256_graphs(ctx):
     result (main.cxx:89 then :127-135); K exact.  Modularity against the
     reference's 1- and 16-thread results from the same carry-in state
     (fixtures.within_band): no graph below them by more than 1e-3, and at
-    most 1% of the graphs above them by more than 1e-3.  (A graph of a few
-    thousand vertices runs as one stream; its sub-rounds visit the vertices
+    most 1% of the graphs above them by more than 1e-3.  (On graphs of a few
+    thousand vertices the visit order alone -- a few streams where the
+    reference runs one or two threads, and sub-rounds that visit the vertices
     flagged during a sweep after the rest of the sweep rather than in their
-    id turn, and on such small graphs that order alone has moved the result
-    up by 1.6e-3 and 2e-3 on single graphs.)  The batched Static itself is
-    checked by test_gpu_batched."""
+    id turn -- has moved single graphs' results up by 1.6e-3 and 2e-3.)  The
+    batched Static itself is checked by test_gpu_batched."""
     import lvgraph
     import lvhip
     from oracle import Oracle
