@@ -66,10 +66,10 @@ KERNEL_NAME = "move_kernel<256, 64, 512, float, 2>"
 FUSED_KERNEL = "wl_fused_kernel<float>"
 SMALL_HUB_KERNEL = "hub_mid_kernel<256, 2048u, 1024u, float>"
 MID_HUB_KERNEL = "hub_mid_kernel<512, 2048u, 16384u, float>"
-def pmc_summary(scale):
+def pmc_summary(kernel, scale):
     """The committed FETCH_SIZE / WRITE_SIZE summary of the DF chain at this
-    R-MAT scale."""
-    return os.path.join(ROOT, "profiles", f"move_kernel_df_rmat{scale}_pmc.txt")
+    R-MAT scale for this kernel (named by the kernel's template name)."""
+    return os.path.join(ROOT, "profiles", f"{kernel.split('<')[0]}_df_rmat{scale}_pmc.txt")
 METRIC = "DF-Louvain edges/s & p99 batch latency at dQ<=1e-3 vs plain GPU path"
 
 
@@ -100,7 +100,7 @@ def pmc_traffic(kernel, scale):
     FETCH_SIZE / WRITE_SIZE summary for this scale (two rocprofv3 --pmc runs
     of this script, each scaled by the counter's reading for a copy of known
     size); None if there is no summary for the scale."""
-    path = pmc_summary(scale)
+    path = pmc_summary(kernel, scale)
     if not os.path.exists(path):
         return None
     for line in open(path):

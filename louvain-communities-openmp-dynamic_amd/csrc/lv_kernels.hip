@@ -308,7 +308,6 @@ __device__ __forceinline__ void flag_neighbours(const MoveArgs& a, const WT* W, 
   const uint32_t serial = a.fsame ? cur_serial(a) : 0;
   for (uint32_t b = 0; b < d; b += R * G) {
     uint32_t v[R], sv[R], cx[R];
-    uint64_t ox[R];
     double w[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
@@ -320,22 +319,16 @@ __device__ __forceinline__ void flag_neighbours(const MoveArgs& a, const WT* W, 
     for (int r = 0; r < R; ++r) {
       sv[r] = serial;
       cx[r] = 0;
-      ox[r] = 0;
       if (v[r] == u) continue;
       __hip_atomic_store(&a.vaff[v[r]], (uint8_t)1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       if (a.fsame && v[r] > u) sv[r] = __hip_atomic_load(&a.stamp[v[r]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      if (a.ht_cap) {
-        // (the table's first slot is read with its size: a table's slots
-        // never move, so the load is good whatever the size says)
-        cx[r] = __hip_atomic_load(&a.ht_cap[v[r]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        ox[r] = a.ht_off[v[r]];
-      }
+      if (a.ht_cap) cx[r] = __hip_atomic_load(&a.ht_cap[v[r]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       // flagged by a lower id and not visited this iteration: next sub-round
       if (sv[r] != serial) mark_same(a, v[r]);
-      if (cx[r] > kHtDropped) ht_move_cap(a, v[r], cx[r], ox[r], w[r], from, to);
+      if (cx[r] > kHtDropped) ht_move_cap(a, v[r], cx[r], a.ht_off[v[r]], w[r], from, to);
     }
   }
 }
