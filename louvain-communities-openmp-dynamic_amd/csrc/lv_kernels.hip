@@ -1802,22 +1802,48 @@ __global__ __launch_bounds__(kBlock) void wl_fused_kernel(MoveArgs a, MoveArgs h
 }
 
 // Neighbours of the tabled hubs that moved in the kernel before (list
-// a.mv_*, at most a.mv_cap of *a.mv_n): every block takes its share of each
-// hub's row.  With carry, the hubs queued for the edge-scanning kernels
-// (queues 1-3: tables dropped in this run) while those kernels are not
-// launched go back to the next sub-round, so none is lost before the host
-// launches them again.
+// a.mv_*, at most a.mv_cap of *a.mv_n): the rows, cut into chunks of one
+// block round (4 kBlock edges), are spread over the grid together, so a
+// list of many small hubs costs about what one hub does.  With carry, the
+// hubs queued for the edge-scanning kernels (queues 1-3: tables dropped in
+// this run) while those kernels are not launched go back to the next
+// sub-round, so none is lost before the host launches them again.
 template <class WT>
 __global__ __launch_bounds__(kBlock) void moved_flag_kernel(MoveArgs a, WlQueues qs, const uint32_t* lens,
                                                             int carry) {
+  constexpr uint32_t CH = 4 * kBlock;  // edges per chunk
+  __shared__ uint64_t s_first[kBlock];  // first chunk of each entry of the batch
+  __shared__ uint32_t s_u[kBlock], s_d[kBlock];
   const uint32_t n = min(*a.mv_n, a.mv_cap);
   const WT* W = (const WT*)a.g.w;
-  for (uint32_t m = 0; m < n; ++m) {
-    const uint32_t u = a.mv_u[m], d = a.g.deg[u];
-    const uint64_t o = a.g.off[u];
-    const uint32_t share = (d + gridDim.x - 1) / gridDim.x, lo = blockIdx.x * share, hi = min(d, lo + share);
-    if (lo >= hi) continue;
-    flag_neighbours<kBlock, WT>(a, W, u, o + lo, hi - lo, threadIdx.x, a.mv_from[m], a.mv_to[m]);
+  for (uint32_t m0 = 0; m0 < n; m0 += kBlock) {  // block-uniform: kBlock entries at a time
+    const uint32_t m = m0 + threadIdx.x;
+    uint32_t u = 0, d = 0;
+    if (m < n) {
+      u = a.mv_u[m];
+      d = a.g.deg[u];
+    }
+    uint64_t chunks;
+    const uint64_t first = block_scan_excl<kBlock>((d + CH - 1) / CH, chunks);
+    s_first[threadIdx.x] = first;
+    s_u[threadIdx.x] = u;
+    s_d[threadIdx.x] = d;
+    __syncthreads();
+    const uint32_t nb = min(n - m0, (uint32_t)kBlock);
+    for (uint64_t k = blockIdx.x; k < chunks; k += gridDim.x) {  // block-uniform
+      // the entry holding chunk k: the last with first <= k (entries
+      // without edges hold no chunk)
+      uint32_t lo = 0, hi = nb;
+      while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (s_first[mid] <= k) lo = mid;
+        else hi = mid;
+      }
+      const uint32_t e = (uint32_t)(k - s_first[lo]) * CH, du = s_d[lo], uu = s_u[lo];
+      flag_neighbours<kBlock, WT>(a, W, uu, a.g.off[uu] + e, min(du - e, CH), threadIdx.x, a.mv_from[m0 + lo],
+                                  a.mv_to[m0 + lo]);
+    }
+    __syncthreads();
   }
   if (!carry) return;
   for (int q = 1; q <= 3; ++q) {
