@@ -975,7 +975,9 @@ int local_move_worklist(lv_context* c, const Level& lv, uint32_t* vcom, const do
     a.mv_u = c->ws<uint32_t>("mv.u", a.mv_cap);
     a.mv_from = c->ws<uint32_t>("mv.from", a.mv_cap);
     a.mv_to = c->ws<uint32_t>("mv.to", a.mv_cap);
-    a.mv_n = c->ws<uint32_t>("mv.n", 1);
+    a.mv_d = c->ws<uint32_t>("mv.d", a.mv_cap);
+    a.mv_o = c->ws<uint64_t>("mv.o", a.mv_cap);
+    a.mv_n = c->ws<uint32_t>("mv.n", 3);  // entries, pieces reserved, pieces taken
   }
   batched_args(a, bt);
   batched_schedule(c, a, lv, bt);
@@ -984,6 +986,13 @@ int local_move_worklist(lv_context* c, const Level& lv, uint32_t* vcom, const do
   a.interleave = c->interleave;
   unsigned long long* cnt0 = a.cnt;
   const uint64_t S = lv.g.S, stride = std::max<uint64_t>(S, 1), fwords = (S >> 5) + 2;
+  // moved-hub pieces of the fused launch (every descriptor pending between launches)
+  uint32_t* mv_piece = nullptr;
+  if (a.mv_n) {
+    bool fresh_p = false;
+    mv_piece = c->ws<uint32_t>("mv.piece", kMvPieces, &fresh_p);
+    if (fresh_p) k_fill_u32(mv_piece, kMvPending, kMvPieces, c->st);
+  }
   // queues: low degree, small hubs, mid hubs, hub batch
   WlQueues qs;
   for (int q = 0; q < kWlQueues; ++q) qs.q[q] = c->ws<uint32_t>("wl.q" + std::to_string(q), stride);
@@ -1060,6 +1069,8 @@ int local_move_worklist(lv_context* c, const Level& lv, uint32_t* vcom, const do
       const int sides = scan_hubs ? 2 : 0;
       hipStream_t hs = sides ? c->side[0] : c->st;
       MoveArgs ah = a;  // the tabled hubs (stats slot 1)
+      ah.mv_piece = mv_piece;  // (their moves' rows flagged inside the fused launch)
+      ah.mv_pcap = kMvPieces;
       ah.list = qs.q[4];
       ah.n_dev = lens + 4;
       ah.cnt = cnt0 + (size_t)1 * CNT_N;
@@ -1070,9 +1081,9 @@ int local_move_worklist(lv_context* c, const Level& lv, uint32_t* vcom, const do
       // launch, then the flags of the hubs that moved
       hipEvent_t f0 = nullptr;
       if (c->profiling && S && a.ht_cap && !sides) f0 = c->record();
-      const bool fused = S && a.ht_cap && !sides && k_wl_fused(a, ah, lv.f64, c->st);
+      const bool fused = S && a.ht_cap && !sides && k_wl_fused(a, ah, qs, lens, lv.f64, c->st);
       if (f0 && fused) c->prof.push_back({kSlotFused, {f0, c->record()}});
-      if (fused) k_moved_flag(ah, qs, lens, true, c->st);  // with the hubs whose tables dropped, kept for later
+      ah.mv_piece = nullptr;  // (the separate launches: a flag pass reads the entries)
       if (S && !fused) {
         if (sides) {
           c->make_side();

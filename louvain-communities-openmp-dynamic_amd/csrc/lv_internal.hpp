@@ -60,6 +60,13 @@ constexpr uint64_t kMidGrid = 512;
 // (its hub is evaluated from its edges).
 constexpr uint32_t kHtMinDeg = 256;
 constexpr uint32_t kHtMaxCap = 256;
+// moved-hub pieces of the fused worklist launch: edges a piece, pieces a
+// sub-round (more: the hub's wave flags its row itself), the descriptor of a
+// reserved piece not yet written, a piece without work
+constexpr uint32_t kMvChunk = 1024;
+constexpr uint32_t kMvPieces = 16384;
+constexpr uint32_t kMvPending = 0xFFFFFFFFu;
+constexpr uint32_t kMvSkip = 0xFFFFFFFEu;
 constexpr uint32_t kHtMaxDistinct = kHtMaxCap - 64;  // room for a slot per lane of a wave past it
 // slots of a vertex without a table: 0 never built, kHtDropped dropped
 // (too many communities, or no spare slots); a table has kHtMaxCap
@@ -77,7 +84,7 @@ struct WlSplit {
   const uint8_t* big;         // ids sent to the hub batch (nullable)
   const uint32_t* tab;        // hub table slots per id: hubs with a table go to queue 4 (nullable)
   uint8_t* dirty;             // per segment: a flag may be set (nullable: scan all)
-  uint32_t* reset;            // zeroed by the compaction (nullable)
+  uint32_t* reset;            // three words zeroed by the compaction (nullable)
   const unsigned long long* ndrop;  // copied to lens[kWlQueues], saturated to u32 (nullable)
   const double* el;            // gain slots summed in slot order into lens[6..7] as a double (nullable)
   uint32_t shift;             // segment of block b: ids [b << shift, (b + 1) << shift)
@@ -238,6 +245,14 @@ struct MoveArgs {
   uint32_t* mv_u;
   uint32_t* mv_from;
   uint32_t* mv_to;
+  uint32_t* mv_d;              //   (each hub's degree and row offset)
+  uint64_t* mv_o;
+  // fused worklist launch (nullable: a flag pass after the kernel reads the
+  // entries): each moved hub's row in pieces of kMvChunk edges, taken by the
+  // launch's blocks when their own work is done; mv_n[1] pieces reserved,
+  // mv_n[2] taken; piece k: entry mv_piece[k] & 4095, part mv_piece[k] >> 12
+  uint32_t* mv_piece;
+  uint32_t mv_pcap;
   uint32_t* mv_n;
   uint32_t mv_cap;
   uint64_t* hb_off;            // hub's row offset and degree
@@ -277,10 +292,13 @@ void k_flag_nonzero(const uint32_t* a, uint64_t n, uint8_t* flags, hipStream_t s
 // with carry, the entries of worklist queues 1-3 back to the next sub-round.
 void k_moved_flag(const MoveArgs& a, const WlQueues& qs, const uint32_t* lens, bool carry, hipStream_t st);
 // A worklist sub-round's id-order kernel (a, queue 0) and tabled hubs (h,
-// queue 4) in one launch; false (nothing launched) for schedules it does
-// not cover (chunks handed out in order, validated moves, hubs in the
+// queue 4) in one launch, which also flags the neighbours of the tabled hubs
+// that move (h.mv_piece set) and marks the entries of queues 1-3 (qs,
+// lens) for the next sub-round; false (nothing launched) for schedules it
+// does not cover (chunks handed out in order, validated moves, hubs in the
 // sweep, per-graph chunk tables, fewer than 64 streams).
-bool k_wl_fused(const MoveArgs& a, const MoveArgs& h, bool f64, hipStream_t st);
+bool k_wl_fused(const MoveArgs& a, const MoveArgs& h, const WlQueues& qs, const uint32_t* lens, bool f64,
+                hipStream_t st);
 // Sources of a batch's insertions (iu) that now have more than kHtMinDeg
 // edges and no table: a table each from the spare slots (bump counter
 // next[0], slots up to total), listed for a build; with no spare slot left,

@@ -1656,7 +1656,8 @@ __global__ __launch_bounds__(BLOCK) void hub_mid_kernel(MoveArgs a) {
 // (as block `bid` of `nblk`: a launch of its own, or the last blocks of the
 // fused worklist launch)
 template <class WT>
-__device__ __forceinline__ void hub_tab_body(const MoveArgs& a, uint32_t bid, uint32_t nblk) {
+__device__ __forceinline__ void hub_tab_body(const MoveArgs& a, uint32_t bid, uint32_t nblk,
+                                             uint32_t& mv_made) {
   constexpr int R = kHtMaxCap / 64;  // table slots per lane
   const int lane = threadIdx.x % 64;
   const uint64_t n = list_len(a);
@@ -1748,7 +1749,7 @@ __device__ __forceinline__ void hub_tab_body(const MoveArgs& a, uint32_t bid, ui
     }
     // a candidate exists and it is not community 0 of u's graph
     if (be > 0 && bc != vertex_zero(a, u)) {
-      uint32_t slot = kEmpty;
+      uint32_t slot = kEmpty, pb = 0;
       if (lane == 0) {
         unsafeAtomicAdd(&a.ctot[cu], -Ku);
         unsafeAtomicAdd(&a.ctot[bc], Ku);
@@ -1760,13 +1761,32 @@ __device__ __forceinline__ void hub_tab_body(const MoveArgs& a, uint32_t bid, ui
           slot = atomicAdd(a.mv_n, 1u);
           if (slot < a.mv_cap) {
             a.mv_u[slot] = u;
-            a.mv_from[slot] = cu;
-            a.mv_to[slot] = bc;
+            st_dev(&a.mv_u[slot], u);
+            st_dev(&a.mv_from[slot], cu);
+            st_dev(&a.mv_to[slot], bc);
+            st_dev(&a.mv_d[slot], d);
+            st_dev(&a.mv_o[slot], o);
+            if (a.mv_piece) {
+              __builtin_amdgcn_s_waitcnt(0);  // (the entry is complete before its pieces exist)
+              pb = atomicAdd(a.mv_n + 1, (d + kMvChunk - 1) / kMvChunk);
+            }
           }
         }
       }
       slot = __shfl(slot, 0, 64);
-      if (!a.mv_n || slot >= a.mv_cap) flag_neighbours<64, WT>(a, W, u, o, d, lane, cu, bc);
+      pb = __shfl(pb, 0, 64);
+      // its row's pieces, for the blocks of this launch to take; with no
+      // room for all of them, none (those below the cap are written empty)
+      // and the wave flags the row itself
+      const uint32_t np = (d + kMvChunk - 1) / kMvChunk;
+      const bool listed = a.mv_n && slot < a.mv_cap;
+      const bool own = !listed || (a.mv_piece && (uint64_t)pb + np > a.mv_pcap);
+      if (listed && a.mv_piece) {
+        for (uint32_t j = lane; j < np && pb + j < a.mv_pcap; j += 64)
+          st_dev(&a.mv_piece[pb + j], own ? kMvSkip : (slot | j << 12));
+        if (lane == 0 && !own) mv_made = 1;  // this block takes pieces before it ends
+      }
+      if (own) flag_neighbours<64, WT>(a, W, u, o, d, lane, cu, bc);
     }
     if (lane == 0) {
       st_dev(&a.vaff[u], (uint8_t)0);
@@ -1788,7 +1808,63 @@ __device__ __forceinline__ void hub_tab_body(const MoveArgs& a, uint32_t bid, ui
 
 template <class WT>
 __global__ __launch_bounds__(kBlock) void hub_tab_kernel(MoveArgs a) {
-  hub_tab_body<WT>(a, blockIdx.x, gridDim.x);
+  __shared__ uint32_t s_made;
+  hub_tab_body<WT>(a, blockIdx.x, gridDim.x, s_made);
+}
+
+// The pieces of the moved hubs' rows (h.mv_*), taken one at a time by the
+// block until none is reserved: every block of the fused launch whose waves
+// reserved pieces calls it when its own work is done, so every piece is
+// taken before the launch ends (by its own block, or another that was
+// draining then).  A taken piece's descriptor is cleared for the next launch.
+// (Counters and descriptors are device-coherent atomics; a descriptor is
+// written right after its reservation by a running wave, and the wait for
+// it is bounded all the same.)
+template <class WT>
+__device__ __forceinline__ void drain_moved(const MoveArgs& h) {
+  __shared__ uint32_t s_k, s_u, s_d, s_from, s_to;
+  __shared__ uint64_t s_o;
+  const WT* W = (const WT*)h.g.w;
+  for (;;) {
+    if (threadIdx.x == 0) {
+      uint32_t k = kMvPending, t = atomicAdd(h.mv_n + 2, 0u);
+      for (;;) {
+        const uint32_t n = min(atomicAdd(h.mv_n + 1, 0u), h.mv_pcap);
+        if (t >= n) break;
+        const uint32_t old = atomicCAS(h.mv_n + 2, t, t + 1);
+        if (old == t) {
+          k = t;
+          break;
+        }
+        t = old;
+      }
+      uint32_t dsc = kMvPending;
+      if (k != kMvPending) {
+        uint32_t spin = 0;
+        while ((dsc = ld_dev(&h.mv_piece[k])) == kMvPending && ++spin < (1u << 20)) __builtin_amdgcn_s_sleep(1);
+        st_dev(&h.mv_piece[k], kMvPending);
+        if (dsc != kMvPending && dsc != kMvSkip) {
+          const uint32_t m = dsc & 4095u;
+          s_u = ld_dev(&h.mv_u[m]);
+          s_d = ld_dev(&h.mv_d[m]);
+          s_o = ld_dev(&h.mv_o[m]) + (uint64_t)(dsc >> 12) * kMvChunk;
+          s_from = ld_dev(&h.mv_from[m]);
+          s_to = ld_dev(&h.mv_to[m]);
+          const uint32_t e = (dsc >> 12) * kMvChunk;
+          s_d = e < s_d ? min(s_d - e, kMvChunk) : 0u;  // (edges of this piece)
+        } else {
+          s_d = 0;
+        }
+      }
+      s_k = k;
+    }
+    __syncthreads();
+    const uint32_t k = s_k, u = s_u, d = s_d, from = s_from, to = s_to;
+    const uint64_t o = s_o;
+    __syncthreads();
+    if (k == kMvPending) break;
+    if (d) flag_neighbours<kBlock, WT>(h, W, u, o, d, threadIdx.x, from, to);
+  }
 }
 
 // One worklist sub-round in one launch when every hub has a table: blocks
@@ -1796,9 +1872,25 @@ __global__ __launch_bounds__(kBlock) void hub_tab_kernel(MoveArgs a) {
 // block), the others evaluate the tabled hubs of queue 4 (h), side by side
 // as the reference's threads visit hubs and other vertices at once.
 template <class WT>
-__global__ __launch_bounds__(kBlock) void wl_fused_kernel(MoveArgs a, MoveArgs h, uint32_t nlo) {
-  if (blockIdx.x < nlo) move_body<kBlock, 64, kHybridLds, WT, MOVE_HYBRID>(a, blockIdx.x, nlo);
-  else hub_tab_body<WT>(h, blockIdx.x - nlo, gridDim.x - nlo);
+__global__ __launch_bounds__(kBlock) void wl_fused_kernel(MoveArgs a, MoveArgs h, uint32_t nlo, WlQueues qs,
+                                                          const uint32_t* lens) {
+  __shared__ uint32_t s_made;  // a wave of this block reserved moved-hub pieces
+  if (threadIdx.x == 0) s_made = 0;
+  __syncthreads();
+  if (blockIdx.x < nlo) {
+    move_body<kBlock, 64, kHybridLds, WT, MOVE_HYBRID>(a, blockIdx.x, nlo);
+  } else {
+    hub_tab_body<WT>(h, blockIdx.x - nlo, gridDim.x - nlo, s_made);
+    // the hubs queued for the edge-scanning kernels (queues 1-3: tables
+    // dropped), which are not launched with this one, go back to the next
+    // sub-round, so none is lost before the host launches them again
+    const uint64_t nt = (uint64_t)(gridDim.x - nlo) * kBlock;
+    for (int q = 1; q <= 3; ++q)
+      for (uint64_t i = (uint64_t)(blockIdx.x - nlo) * kBlock + threadIdx.x; i < lens[q]; i += nt)
+        mark_same(h, qs.q[q][i]);
+  }
+  __syncthreads();
+  if (h.mv_piece && s_made) drain_moved<WT>(h);
 }
 
 // Neighbours of the tabled hubs that moved in the kernel before (list
@@ -1812,8 +1904,8 @@ template <class WT>
 __global__ __launch_bounds__(kBlock) void moved_flag_kernel(MoveArgs a, WlQueues qs, const uint32_t* lens,
                                                             int carry) {
   constexpr uint32_t CH = 4 * kBlock;  // edges per chunk
-  __shared__ uint64_t s_first[kBlock];  // first chunk of each entry of the batch
-  __shared__ uint32_t s_u[kBlock], s_d[kBlock];
+  __shared__ uint64_t s_first[kBlock], s_o[kBlock];  // first chunk of each entry of the batch, its row
+  __shared__ uint32_t s_u[kBlock], s_d[kBlock], s_from[kBlock], s_to[kBlock];
   const uint32_t n = min(*a.mv_n, a.mv_cap);
   const WT* W = (const WT*)a.g.w;
   for (uint32_t m0 = 0; m0 < n; m0 += kBlock) {  // block-uniform: kBlock entries at a time
@@ -1821,7 +1913,10 @@ __global__ __launch_bounds__(kBlock) void moved_flag_kernel(MoveArgs a, WlQueues
     uint32_t u = 0, d = 0;
     if (m < n) {
       u = a.mv_u[m];
-      d = a.g.deg[u];
+      d = a.mv_d[m];
+      s_o[threadIdx.x] = a.mv_o[m];
+      s_from[threadIdx.x] = a.mv_from[m];
+      s_to[threadIdx.x] = a.mv_to[m];
     }
     uint64_t chunks;
     const uint64_t first = block_scan_excl<kBlock>((d + CH - 1) / CH, chunks);
@@ -1839,9 +1934,8 @@ __global__ __launch_bounds__(kBlock) void moved_flag_kernel(MoveArgs a, WlQueues
         if (s_first[mid] <= k) lo = mid;
         else hi = mid;
       }
-      const uint32_t e = (uint32_t)(k - s_first[lo]) * CH, du = s_d[lo], uu = s_u[lo];
-      flag_neighbours<kBlock, WT>(a, W, uu, a.g.off[uu] + e, min(du - e, CH), threadIdx.x, a.mv_from[m0 + lo],
-                                  a.mv_to[m0 + lo]);
+      const uint32_t e = (uint32_t)(k - s_first[lo]) * CH, du = s_d[lo];
+      flag_neighbours<kBlock, WT>(a, W, s_u[lo], s_o[lo] + e, min(du - e, CH), threadIdx.x, s_from[lo], s_to[lo]);
     }
     __syncthreads();
   }
@@ -2641,7 +2735,7 @@ __global__ __launch_bounds__(TB) void wl_scatter_kernel(void* f, int clear, uint
 #pragma unroll
   for (int q = 0; q < kWlQueues; ++q) r[q] = blockIdx.x == 0 ? 0 : s_pre[q];
   if (blockIdx.x == 0 && threadIdx.x < kWlQueues) lens[threadIdx.x] = (uint32_t)s_pre[threadIdx.x];
-  if (blockIdx.x == 0 && threadIdx.x == 0 && sp.reset) *sp.reset = 0;
+  if (blockIdx.x == 0 && threadIdx.x == 0 && sp.reset) sp.reset[0] = sp.reset[1] = sp.reset[2] = 0;
   if (blockIdx.x == 0 && threadIdx.x == 0 && sp.ndrop) lens[kWlQueues] = (uint32_t)min(*sp.ndrop, 0xFFFFFFFFull);
   if (blockIdx.x == 0 && threadIdx.x == 0 && sp.el) {  // the iteration's gain so far, summed as the host does
     double t = 0;
@@ -2942,15 +3036,16 @@ void k_hub_tab(const MoveArgs& a, hipStream_t st) {
   if (a.g.w && a.f64_weights) LV_LAUNCH(hub_tab_kernel<double>, grid, kBlock, st, a);
   else LV_LAUNCH(hub_tab_kernel<float>, grid, kBlock, st, a);
 }
-bool k_wl_fused(const MoveArgs& a, const MoveArgs& h, bool f64, hipStream_t st) {
+bool k_wl_fused(const MoveArgs& a, const MoveArgs& h, const WlQueues& qs, const uint32_t* lens, bool f64,
+                hipStream_t st) {
   // the id-order part as move_tier launches it with spread chunks and four
   // streams a block; other schedules take the separate launches
   const uint64_t streams = a.max_streams ? a.max_streams : 1;
   if (a.chunk || a.validate || a.xids || a.ctab || streams < 64) return false;
   const uint64_t nlo = grid_for(streams, 4, (a.gblocks + 3) / 4);
   const uint64_t grid = nlo + hub_tab_grid(h);
-  if (f64) LV_LAUNCH(wl_fused_kernel<double>, grid, kBlock, st, a, h, (uint32_t)nlo);
-  else LV_LAUNCH(wl_fused_kernel<float>, grid, kBlock, st, a, h, (uint32_t)nlo);
+  if (f64) LV_LAUNCH(wl_fused_kernel<double>, grid, kBlock, st, a, h, (uint32_t)nlo, qs, lens);
+  else LV_LAUNCH(wl_fused_kernel<float>, grid, kBlock, st, a, h, (uint32_t)nlo, qs, lens);
   return true;
 }
 __global__ void flag_nonzero_kernel(const uint32_t* a, uint64_t n, uint8_t* flags) {
