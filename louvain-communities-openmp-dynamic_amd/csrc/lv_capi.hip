@@ -528,6 +528,8 @@ constexpr int kTierMidHub = 7;    // launch_move: worklist mid hubs through k_hu
 constexpr int kTierSmallHub = 8;  // launch_move: worklist small hubs through k_hub_mid (stats slot 2)
 constexpr int kTierTabHub = 9;    // launch_move: worklist hubs with a table through k_hub_tab (stats slot 1)
 constexpr int kSubRounds = 4;      // worklist sub-rounds enqueued per host check
+constexpr int kShortGroup = 2;     // ... once no more than kShortLeft vertices are queued (the cascade
+constexpr uint64_t kShortLeft = 64;  // then ends within a sub-round or two; an empty one costs its launches)
 constexpr int kFirstGroup = 8;     // ... in an iteration's first group (a DF step on an R-MAT graph
                                    // typically needs 5-8: one host check instead of two)
 constexpr uint64_t kRefChunk = 2048;  // the reference's OpenMP chunk (inc/louvain.hxx:594)
@@ -1163,7 +1165,8 @@ int local_move_worklist(lv_context* c, const Level& lv, uint32_t* vcom, const do
       key.append(reinterpret_cast<const char*>(&S), sizeof S);
       return key;
     };
-    for (;; rounds = kSubRounds) {
+    uint64_t left = 0;
+    for (;; rounds = left <= kShortLeft ? kShortGroup : kSubRounds) {
       auto group = [&] {
         for (int r = 0; r < rounds; ++r) {
           sub_round();
@@ -1182,7 +1185,7 @@ int local_move_worklist(lv_context* c, const Level& lv, uint32_t* vcom, const do
       c->read(nq, lens, sp.el ? 8 : sp.ndrop ? kWlQueues + 1 : kWlQueues);
       if (sp.ndrop) ht->ht_scan = nq[kWlQueues] != 0;
       if (sp.el) std::memcpy(&el, nq + 6, sizeof el);
-      uint64_t left = 0;
+      left = 0;
       for (int q = 0; q < kWlQueues; ++q) left += nq[q];
       if (!left) break;
       if (nq[1] + nq[2] + nq[3]) scan_hubs = true;

@@ -852,6 +852,7 @@ __device__ __forceinline__ void move_body(const MoveArgs& a, uint32_t bid, uint3
   }
   double el_acc = 0;
   unsigned long long c_vert = 0, c_edge = 0, c_cand = 0, c_move = 0, c_medge = 0;
+  bool worked = false;  // (block-uniform: the block took a chunk)
   for (;;) {
     // the block takes GPB consecutive chunks (one per group)
     uint64_t cb = next;
@@ -864,6 +865,7 @@ __device__ __forceinline__ void move_body(const MoveArgs& a, uint32_t bid, uint3
       next += (uint64_t)nblk * GPB;
     }
     if (cb >= nchunks) break;
+    worked = true;
     const uint64_t ch = cb + grp;
     // entries [c0, c0 + clen) of the list
     // (interleaved: entries ch + j * nchunks, j < clen)
@@ -1063,6 +1065,7 @@ __device__ __forceinline__ void move_body(const MoveArgs& a, uint32_t bid, uint3
       visit(act, u, o, d, cu, Ku);
     }
   }
+  if (!worked) return;  // nothing to add up (a block past a short list)
   if (!a.choose_only) {
     double t = block_sum<BLOCK>(el_acc);
     if (threadIdx.x == 0 && t != 0) unsafeAtomicAdd(a.el + spread_slot() * kElStride, t);
@@ -1663,6 +1666,7 @@ __device__ __forceinline__ void hub_tab_body(const MoveArgs& a, uint32_t bid, ui
   const uint64_t n = list_len(a);
   const uint64_t nw = (uint64_t)nblk * (kBlock / 64);
   const WT* W = (const WT*)a.g.w;
+  if ((uint64_t)bid * (kBlock / 64) >= n) return;  // (block-uniform: no hub for this block)
   double el_acc = 0;
   unsigned long long c_vert = 0, c_edge = 0, c_cand = 0, c_move = 0, c_medge = 0;
   for (uint64_t k = (uint64_t)bid * (kBlock / 64) + threadIdx.x / 64; k < n; k += nw) {  // wave-uniform
