@@ -498,6 +498,9 @@ def main():
                     help="timed batches the reference also runs (dQ and the CPU baseline), evenly spread")
     ap.add_argument("--no-plain", action="store_true", help="skip the frontier and plain paths")
     ap.add_argument("--no-static", action="store_true", help="skip the (reported) GPU Static run")
+    ap.add_argument("--static-runs", type=int, default=3,
+                    help="GPU and reference Static runs reported (their mean; the DF chain starts from the "
+                         "reference's first)")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving segment")
     ap.add_argument("--no-tune", action="store_true", help="run the default schedule (no lv_autotune)")
     ap.add_argument("--stream-fraction", type=float, default=None,
@@ -561,22 +564,33 @@ def main():
     # GPU Static Louvain on the original graph (reported), and the starting
     # state of the DF chain: the reference's own Static result, as in
     # main.cxx:89 (louvainStaticOmp), on the host cores.
-    rs, q_static = None, None
+    # (Static's outcome moves from run to run on the GPU with its concurrent
+    # streams, and in the reference with its threads' schedule: both are run
+    # --static-runs times and reported by their mean, with every run listed)
+    rs, q_static, static_runs = None, None, []
     if not args.no_static:
-        t = time.time()
-        rs = ctx.run(lvhip.LV_STATIC, dg0, None, state0)
-        q_static = ctx.dmodularity(dg0, state0)
-        log(f"GPU static: Q={q_static:.6f} iters={rs.iterations} passes={rs.passes} time={rs.time:.1f} ms "
-            f"(wall {time.time() - t:.1f}s)")
+        for k in range(max(1, args.static_runs)):
+            t = time.time()
+            rs = ctx.run(lvhip.LV_STATIC, dg0, None, state0)
+            static_runs.append((ctx.dmodularity(dg0, state0), rs.time))
+            log(f"GPU static: Q={static_runs[-1][0]:.6f} iters={rs.iterations} passes={rs.passes} "
+                f"time={rs.time:.1f} ms (wall {time.time() - t:.1f}s)")
+        q_static = sum(q for q, _ in static_runs) / len(static_runs)
     orc, kind = cpu_checker()
     cores = HOST_CORES
     orc.set_threads(cores)
-    t = time.time()
-    r0 = orc.louvain("static", g0)
+    ref_runs = []
+    for k in range(max(1, args.static_runs if not args.no_static else 1)):
+        t = time.time()
+        r = orc.louvain("static", g0)
+        state0.upload(r.membership, r.vertex_weight, r.community_weight)
+        ref_runs.append((ctx.dmodularity(dg0, state0), r.time))
+        log(f"reference static ({kind}, {cores} threads): Q={ref_runs[-1][0]:.6f} iters={r.iterations} "
+            f"passes={r.passes} time={r.time:.1f} ms (wall {time.time() - t:.1f}s)")
+        if k == 0:
+            r0 = r  # the DF chain's carry-in state
     state0.upload(r0.membership, r0.vertex_weight, r0.community_weight)
-    q_static_ref = ctx.dmodularity(dg0, state0)
-    log(f"reference static ({kind}, {cores} threads): Q={q_static_ref:.6f} iters={r0.iterations} "
-        f"passes={r0.passes} time={r0.time:.1f} ms (wall {time.time() - t:.1f}s)")
+    q_static_ref = sum(q for q, _ in ref_runs) / len(ref_runs)
     dg0.close()
 
     # The batches, and the quality chain (untimed): each batch's device
@@ -593,7 +607,7 @@ def main():
         logf.write(lvlog.graph_line(f"synthetic/rmat-{args.scale}.txt") + "\n")
         logf.write(lvlog.order_line(S, H0.edges) + "\n")
         if rs is not None:
-            logf.write(lvlog.result_line(rs, "louvainStaticHip", 1, q_static) + "\n")
+            logf.write(lvlog.result_line(rs, "louvainStaticHip", 1, static_runs[-1][0]) + "\n")
     dq_state = ctx.device_state(S)
     dq_state.copy_from(state0)
     dqg = ctx.device_graph(g0)
@@ -759,9 +773,13 @@ def main():
         "algorithm_p50_ms": percentile(lat, 50),
         "edge_visits_per_s": visits / (sum(pw) if sum(pw) > 0 else 1.0),
         "end_to_end": end_to_end,
-        "dq_max": dq_max, "static": {"gpu_modularity": q_static, "gpu_ms": rs and rs.time,
-                                     "reference_modularity": q_static_ref, "reference_ms": r0.time,
-                                     "reference_threads": cores},
+        "dq_max": dq_max, "static": {"gpu_modularity": q_static,
+                                     "gpu_ms": static_runs and sum(t for _, t in static_runs) / len(static_runs),
+                                     "gpu_runs": [q for q, _ in static_runs],
+                                     "reference_modularity": q_static_ref,
+                                     "reference_ms": sum(t for _, t in ref_runs) / len(ref_runs),
+                                     "reference_runs": [q for q, _ in ref_runs], "reference_threads": cores,
+                                     "statistic": "mean of the runs"},
         "path": args.path, "plain_gpu_path": others.get("plain"),
         "other_gpu_paths": {k: v for k, v in others.items() if k != "plain"},
         "roofline": roofline, "cpu_baseline": cpu, "serving": serving, "tuning": tuning,
