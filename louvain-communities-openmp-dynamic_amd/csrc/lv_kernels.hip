@@ -1874,7 +1874,9 @@ __device__ __forceinline__ void drain_moved(const MoveArgs& h) {
 // One worklist sub-round in one launch when every hub has a table: blocks
 // [0, nlo) run the id-order kernel over queue 0 (a, four one-wave streams a
 // block), the others evaluate the tabled hubs of queue 4 (h), side by side
-// as the reference's threads visit hubs and other vertices at once.
+// as the reference's threads visit hubs and other vertices at once; a block
+// whose hub waves moved hubs then flags the neighbours of moved hubs, a piece
+// of a row at a time, until no piece is left.
 template <class WT>
 __global__ __launch_bounds__(kBlock) void wl_fused_kernel(MoveArgs a, MoveArgs h, uint32_t nlo, WlQueues qs,
                                                           const uint32_t* lens) {
