@@ -364,6 +364,7 @@ struct lv_dgraph {
   double M2 = 0;  // 2M, edgeWeightOmp
   TierLists tiers;  // five degree tiers
   TierLists hubs;   // tier 4 = degree > kHubMin (id-order mode), ascending
+  uint32_t maxdeg = ~0u;  // largest degree above kTierMax[2] (0: none), with the tiers
   uint64_t hub_slots = 0;
   uint64_t version = 0;  // counts the updates applied to the graph
   bool broken = false;   // a failed update left the rows inconsistent: the graph is refused
@@ -491,12 +492,12 @@ void build_tiers(lv_context* c, const std::string& name, const uint64_t* off, co
   for (int k = 0; k <= kTiers; ++k) t.start[k] = st[k];
 }
 
-uint32_t max_degree_tier4(lv_context* c, const TierLists& t, const uint32_t* deg) {
-  uint64_t n4 = t.start[5] - t.start[4];
+uint32_t max_degree_tier4(lv_context* c, const TierLists& t, const uint32_t* deg, int from = 4) {
+  uint64_t n4 = t.start[5] - t.start[from];
   if (!n4) return 0;
   uint32_t* out = c->ws<uint32_t>("maxdeg", 1);
   k_fill_u32(out, 0, 1, c->st);
-  k_max_deg(t.ids + t.start[4], n4, deg, out, c->st);
+  k_max_deg(t.ids + t.start[from], n4, deg, out, c->st);
   return c->read1(out);
 }
 
@@ -519,6 +520,7 @@ struct Level {
   uint64_t hub_slots;      // hub-batch table slots for all of them
   uint64_t stamp = 0;      // resident graph's degree stamp (0: a super-graph)
   uint64_t slab_slots = 0; // resident graph: hub-batch slots above kMidSlots / 2 edges
+  uint32_t maxdeg = ~0u;   // resident graph: its largest degree above kTierMax[2] (0: none); ~0 unknown
 };
 
 // Launch one tier of the local-moving kernel, bracketed by events when the
@@ -1056,6 +1058,9 @@ int local_move_worklist(lv_context* c, const Level& lv, uint32_t* vcom, const do
   // the next group on, the group after its hub was queued.  Below kHtMinDeg
   // edges hubs have no table.
   bool scan_hubs = !a.ht_cap || ht->ht_scan || c->hub_min_wl < kHtMinDeg;
+  // a graph without a vertex above the hub degree has no hub to queue: no
+  // hub kernel is launched, nor a side stream forked for one
+  if (lv.maxdeg <= c->hub_min_wl) scan_hubs = false;
   // one sub-round: hubs and the rest, then the next queue
   auto sub_round = [&]() {
     if (c->tier_order == 2) {
@@ -1500,7 +1505,7 @@ void invoke(lv_context* c, int approach, const lv_dgraph* x, const lv_dbatch* b,
   hipEvent_t t1 = e2;
   bool t1_set = false;
   Level lv0{x->view(), x->f64, &x->tiers, x->order, x->tiers.maxdeg, &x->hubs, x->hub_slots, x->stamp,
-            x->slab_slots};
+            x->slab_slots, x->maxdeg};
   Level cur = lv0;
   TierLists ytiers, yhubs;
   uint64_t ySpan = 0;
@@ -1776,7 +1781,7 @@ void invoke_batched(lv_context* c, int approach, const lv_dgraph* x, const lv_db
   hipEvent_t t1 = e2;
   bool t1_set = false;
   Level cur{x->view(), x->f64, &x->tiers, x->order, x->tiers.maxdeg, &x->hubs, x->hub_slots, x->stamp,
-            x->slab_slots};
+            x->slab_slots, x->maxdeg};
   TierLists ytiers, yhubs;
   uint32_t* gidcur = gid0;
   uint64_t levelS = S;
@@ -2349,6 +2354,7 @@ lv_dgraph* upload_graph(lv_context* c, const lv_graph* g) {
     d->tiers.ids = dalloc<uint32_t>(std::max<uint64_t>(d->S, 1));
     build_tiers(c, "g.tiers", d->off, d->deg, d->ex, d->S, d->tiers, d->tiers.ids);
     d->tiers.maxdeg = max_degree_tier4(c, d->tiers, d->deg);
+    d->maxdeg = max_degree_tier4(c, d->tiers, d->deg, 3);
     d->hubs.ids = dalloc<uint32_t>(std::max<uint64_t>(d->S, 1));
     build_tiers(c, "g.hubs", d->off, d->deg, d->ex, d->S, d->hubs, d->hubs.ids, kHubMin);
     d->hubs.maxdeg = max_degree_tier4(c, d->hubs, d->deg);
@@ -2602,6 +2608,7 @@ void apply_update(lv_context* c, lv_dgraph* d, const lv_dbatch* b, float* delw) 
     // degree lists of the new degrees
     build_tiers(c, "g.tiers", d->off, d->deg, d->ex, S, d->tiers, d->tiers.ids);
     d->tiers.maxdeg = max_degree_tier4(c, d->tiers, d->deg);
+    d->maxdeg = max_degree_tier4(c, d->tiers, d->deg, 3);
     build_tiers(c, "g.hubs", d->off, d->deg, d->ex, S, d->hubs, d->hubs.ids, kHubMin);
     d->hubs.maxdeg = max_degree_tier4(c, d->hubs, d->deg);
     d->hub_slots = caps_total(c, d->deg, S, kHubMin);
