@@ -1035,6 +1035,14 @@ int local_move_worklist(lv_context* c, const Level& lv, uint32_t* vcom, const do
     sp.big = a.hb_big;  // hubs marked for the batch
     sp.tab = a.ht_cap;  // hubs with a table
     sp.reset = a.mv_n;  // the moved-hub list starts empty in every sub-round
+    sp.nreset = 3;
+    if (a.ctab && !a.mv_n) {  // (the batched form, which keeps no hub tables)
+      // the id-order kernel walks the chunks that hold entries, packed
+      // first by the chunk-table pass after each compaction
+      a.nct_dev = c->ws<uint32_t>("bt.nct", 1);
+      sp.reset = a.nct_dev;
+      sp.nreset = 1;
+    }
     sp.ndrop = a.ht_ndrop;
   }
   if (!bt) sp.el = a.el;  // (the batched form reads its per-graph gains)
@@ -1245,14 +1253,11 @@ void aggregate(lv_context* c, const Level& lv, const uint32_t* vcom, uint64_t C,
   const GView& g = lv.g;
   uint64_t GN = lv.order;
   uint32_t* coff = c->ws<uint32_t>("ag.coff", C + 1);
-  uint32_t* cnt = c->ws<uint32_t>("ag.cnt", C + 1);
   uint64_t* mdeg = c->ws<uint64_t>("ag.mdeg", GN + 1);
   uint64_t* mscan = c->ws<uint64_t>("ag.mscan", GN + 1);
-  k_fill_u32(cnt, 0, C + 1, c->st);
-  k_count_members(g, vcom, cnt, c->st);
-  scan32(c, cnt, coff, C + 1);
   // members of each community in ascending id order: a stable sort of
-  // (community, id) over the level's ids, the absent ones (key C) last
+  // (community, id) over the level's ids, the absent ones (key C) last;
+  // the community offsets are read off the sorted keys
   uint32_t* mk = c->ws<uint32_t>("ag.mk", g.S);
   uint32_t* mk2 = c->ws<uint32_t>("ag.mk2", g.S);
   uint32_t* mid = c->ws<uint32_t>("ag.mid", g.S);
@@ -1267,6 +1272,7 @@ void aggregate(lv_context* c, const Level& lv, const uint32_t* vcom, uint64_t C,
     bytes = c->scan_tmp_bytes;
     HIPCHK(sort_pairs_u32(mk, mk2, mid, cedg, g.S, bits, tmp, bytes, c->st));
   }
+  k_member_bounds(mk2, g.S, (uint32_t)C, coff, c->st);
   k_member_degrees(cedg, GN, g, mdeg, c->st);
   scan64(c, mdeg, mscan, GN + 1);
   k_gather_yoff(coff, mscan, C, z.off, c->st);

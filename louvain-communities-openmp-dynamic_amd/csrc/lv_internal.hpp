@@ -84,7 +84,8 @@ struct WlSplit {
   const uint8_t* big;         // ids sent to the hub batch (nullable)
   const uint32_t* tab;        // hub table slots per id: hubs with a table go to queue 4 (nullable)
   uint8_t* dirty;             // per segment: a flag may be set (nullable: scan all)
-  uint32_t* reset;            // three words zeroed by the compaction (nullable)
+  uint32_t* reset;            // nreset words (at most 3) zeroed by the compaction (nullable)
+  uint32_t nreset;
   const unsigned long long* ndrop;  // copied to lens[kWlQueues], saturated to u32 (nullable)
   const double* el;            // gain slots summed in slot order into lens[6..7] as a double (nullable)
   uint32_t shift;             // segment of block b: ids [b << shift, (b + 1) << shift)
@@ -174,11 +175,15 @@ struct MoveArgs {
   // graph's share of the list cut into its own streams, as it would be alone.
   // Chunk t of nct covers list entries [ctab[2t], ctab[2t+1]) of graph k,
   // gcs[k] <= t < gcs[k+1]; graph k holds level ids [glo[k], glo[k+1]).
+  // With nct_dev set the chunks that hold entries are packed first, in no
+  // set order, and *nct_dev counts them (a word the compaction before the
+  // table zeroes): the other chunks would be walked by no one.
   uint64_t* ctab;
   uint64_t nct;
   const uint64_t* glo;
   const uint64_t* gcs;
   uint32_t ng;
+  uint32_t* nct_dev;
   uint32_t* fsame;             // worklist mode, bit v: flagged by a lower id and not
                                //   yet visited this iteration -> next sub-round
   uint8_t* wl_dirty;           // worklist mode: per compaction segment, fsame may hold a flag
@@ -353,7 +358,9 @@ void k_check_edges(const uint64_t* off, const uint32_t* keys, uint64_t S, uint64
 void k_move(const MoveArgs& a, int tier, bool f64, hipStream_t st);
 void k_exists(GView g, const uint32_t* vcom, uint32_t* flags, hipStream_t st);
 void k_lookup(uint32_t* a, uint64_t n, const uint32_t* map, hipStream_t st);
-void k_count_members(GView g, const uint32_t* vcom, uint32_t* cnt, hipStream_t st);
+// Community offsets from the level's ids sorted by community (keys in [0, C],
+// C for an absent id): coff[c] = the number of keys below c, c = 0..C.
+void k_member_bounds(const uint32_t* sorted_keys, uint64_t n, uint32_t C, uint32_t* coff, hipStream_t st);
 void k_member_degrees(const uint32_t* cedg, uint64_t n, GView g, uint64_t* mdeg, hipStream_t st);
 void k_gather_yoff(const uint32_t* coff, const uint64_t* mscan, uint64_t C, uint64_t* yoff, hipStream_t st);
 // Degree partition of ids 0..n-1 (live only), ascending within each part:
@@ -444,7 +451,8 @@ void k_range_count(const uint8_t* f, const uint64_t* voff, uint64_t G, unsigned 
 // The chunk table a.ctab (a.nct chunks) of the list the id-order kernel is
 // about to walk (a.list with a.n_dev / a.n entries, or ids a.base.. when
 // a.list is null): graph k's entries in a.gcs[k+1] - a.gcs[k] chunks of
-// ceil(len / streams) entries.
+// ceil(len / streams) entries (packed, with a.nct_dev, to the chunks that
+// hold entries).
 void k_graph_chunks(const MoveArgs& a, hipStream_t st);
 
 // Device batch tidy and graph update (lv_update.hip).

@@ -830,7 +830,8 @@ __device__ __forceinline__ void move_body(const MoveArgs& a, uint32_t bid, uint3
   const uint64_t streams = a.max_streams ? a.max_streams : 1;
   const uint64_t chunk = dyn ? a.chunk : ((n + streams - 1) / streams ? (n + streams - 1) / streams : 1);
   // a.ctab (id-order kernel, one group per block): chunk bounds per chunk
-  const uint64_t nchunks = a.ctab ? a.nct : ilv ? (n < streams ? n : streams) : (n + chunk - 1) / chunk;
+  const uint64_t nchunks = a.ctab ? (a.nct_dev ? (uint64_t)*a.nct_dev : a.nct)
+                          : ilv ? (n < streams ? n : streams) : (n + chunk - 1) / chunk;
   uint64_t next = (uint64_t)bid * GPB;
   const WT* W = (const WT*)a.g.w;
   // LDS table of this group; HBM region of this block
@@ -2158,13 +2159,16 @@ __device__ __forceinline__ uint32_t wave_inc(uint32_t* x, bool on, uint32_t c) {
   return on ? atomicAdd(&x[c], 1u) : 0;
 }
 
-// (block-uniform trip counts: the waves of a block stay together for the
-// wave-wide steps)
-__global__ void count_members_kernel(GView g, const uint32_t* vcom, uint32_t* cnt) {
-  for (uint64_t b = blockIdx.x * (uint64_t)blockDim.x; b < g.S; b += (uint64_t)gridDim.x * blockDim.x) {
-    const uint64_t u = b + threadIdx.x;
-    const bool on = u < g.S && (!g.ex || g.ex[u]);
-    wave_inc(cnt, on, on ? vcom[u] : 0);
+// Community offsets (the exclusive scan of the member counts that
+// louvainCommunityVerticesOmpW fills, inc/louvain.hxx:749-767) read off the
+// ids sorted by community: coff[c] is the first position whose key is at
+// least c.  Position i writes the c in (key[i-1], key[i]]; position n stands
+// for a key C past the last, so every c in 0..C is written once.
+__global__ void member_bounds_kernel(const uint32_t* sk, uint64_t n, uint32_t C, uint32_t* coff) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i <= n; i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t hi = i < n ? min(sk[i], C) : C;
+    const int64_t lo = i ? (int64_t)min(sk[i - 1], C) : -1;
+    for (int64_t c = lo + 1; c <= (int64_t)hi; ++c) coff[c] = (uint32_t)i;
   }
 }
 
@@ -2741,7 +2745,7 @@ __global__ __launch_bounds__(TB) void wl_scatter_kernel(void* f, int clear, uint
 #pragma unroll
   for (int q = 0; q < kWlQueues; ++q) r[q] = blockIdx.x == 0 ? 0 : s_pre[q];
   if (blockIdx.x == 0 && threadIdx.x < kWlQueues) lens[threadIdx.x] = (uint32_t)s_pre[threadIdx.x];
-  if (blockIdx.x == 0 && threadIdx.x == 0 && sp.reset) sp.reset[0] = sp.reset[1] = sp.reset[2] = 0;
+  if (blockIdx.x == 0 && threadIdx.x < sp.nreset && sp.reset) sp.reset[threadIdx.x] = 0;
   if (blockIdx.x == 0 && threadIdx.x == 0 && sp.ndrop) lens[kWlQueues] = (uint32_t)min(*sp.ndrop, 0xFFFFFFFFull);
   if (blockIdx.x == 0 && threadIdx.x == 0 && sp.el) {  // the iteration's gain so far, summed as the host does
     double t = 0;
@@ -3102,8 +3106,8 @@ void k_exists(GView g, const uint32_t* vcom, uint32_t* flags, hipStream_t st) {
 void k_lookup(uint32_t* a, uint64_t n, const uint32_t* map, hipStream_t st) {
   if (n) LV_LAUNCH(lookup_kernel, grid_for(n, kBlock * 4), kBlock, st, a, n, map);
 }
-void k_count_members(GView g, const uint32_t* vcom, uint32_t* cnt, hipStream_t st) {
-  if (g.S) LV_LAUNCH(count_members_kernel, grid_for(g.S, kBlock * 4), kBlock, st, g, vcom, cnt);
+void k_member_bounds(const uint32_t* sorted_keys, uint64_t n, uint32_t C, uint32_t* coff, hipStream_t st) {
+  LV_LAUNCH(member_bounds_kernel, grid_for(n + 1, kBlock * 4), kBlock, st, sorted_keys, n, C, coff);
 }
 void k_member_degrees(const uint32_t* cedg, uint64_t n, GView g, uint64_t* mdeg, hipStream_t st) {
   LV_LAUNCH(member_degrees_kernel, grid_for(n + 1, kBlock * 4), kBlock, st, cedg, n, g, mdeg);

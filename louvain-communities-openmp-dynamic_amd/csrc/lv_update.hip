@@ -362,13 +362,8 @@ __global__ __launch_bounds__(kB) void range_count_kernel(const uint8_t* f, const
   }
 }
 
-// Chunk t of the batched id-order schedule: graph k owns chunks
-// gcs[k] .. gcs[k+1]-1 and list entries [b0, b1) (ids glo[k] .. glo[k+1]-1;
-// an ascending list is searched, an id range is offset by base); its entries
-// are cut as the kernel cuts a list alone: ceil(len / streams) per chunk.
-__global__ void graph_chunks_kernel(MoveArgs a) {
-  const uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
-  if (t >= a.nct) return;
+// List entries [e0, e1) of chunk t (t < a.nct) of the batched id-order schedule.
+__device__ __forceinline__ void chunk_bounds(const MoveArgs& a, uint64_t t, uint64_t& e0, uint64_t& e1) {
   const uint64_t n = a.n_dev ? (uint64_t)*a.n_dev : a.n;
   const uint64_t k = owner_of(a.gcs, a.ng, t);
   auto bound = [&](uint64_t id) -> uint64_t {  // first entry with id >= `id`
@@ -384,8 +379,40 @@ __global__ void graph_chunks_kernel(MoveArgs a) {
   const uint64_t b0 = bound(a.glo[k]), b1 = bound(a.glo[k + 1]);
   const uint64_t len = b1 - b0, s = a.gcs[k + 1] - a.gcs[k], j = t - a.gcs[k];
   const uint64_t chunk = max<uint64_t>(1, (len + s - 1) / s);
-  a.ctab[2 * t] = b0 + min(len, j * chunk);
-  a.ctab[2 * t + 1] = b0 + min(len, (j + 1) * chunk);
+  e0 = b0 + min(len, j * chunk);
+  e1 = b0 + min(len, (j + 1) * chunk);
+}
+
+// Chunk t of the batched id-order schedule: graph k owns chunks
+// gcs[k] .. gcs[k+1]-1 and list entries [b0, b1) (ids glo[k] .. glo[k+1]-1;
+// an ascending list is searched, an id range is offset by base); its entries
+// are cut as the kernel cuts a list alone: ceil(len / streams) per chunk.
+// With a.nct_dev the chunks that hold entries take the first places of the
+// table, a wave's at a time (whole waves run the kernel: a.nct_dev counts them).
+__global__ void graph_chunks_kernel(MoveArgs a) {
+  const uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  if (a.nct_dev) {
+    uint64_t e0 = 0, e1 = 0;
+    if (t < a.nct) chunk_bounds(a, t, e0, e1);
+    const bool has = e1 > e0;
+    const unsigned long long m = __ballot(has);
+    if (!m) return;
+    const int lane = __lane_id(), lead = __ffsll((long long)m) - 1;
+    uint32_t base = 0;
+    if (lane == lead) base = atomicAdd(a.nct_dev, (uint32_t)__popcll(m));
+    base = __shfl(base, lead, 64);
+    if (has) {
+      const uint64_t p = base + (uint64_t)__popcll(m & ((1ull << lane) - 1));
+      a.ctab[2 * p] = e0;
+      a.ctab[2 * p + 1] = e1;
+    }
+    return;
+  }
+  if (t >= a.nct) return;
+  uint64_t e0 = 0, e1 = 0;
+  chunk_bounds(a, t, e0, e1);
+  a.ctab[2 * t] = e0;
+  a.ctab[2 * t + 1] = e1;
 }
 
 }  // namespace
