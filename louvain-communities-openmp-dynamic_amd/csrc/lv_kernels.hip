@@ -851,7 +851,12 @@ __device__ __forceinline__ void move_body(const MoveArgs& a, uint32_t bid, uint3
     hpos = a.gpos + base;
     hval = a.gval + base;
   }
+  // gains: the block's in el_acc; in the batched form (a.gel) el_acc holds
+  // the group's gains in graph g_cur so far, added to the graph's gel slot
+  // when the group leaves the graph or its chunk (one atomic per stretch of
+  // visits, not per visit: all of a graph's visits add to one word)
   double el_acc = 0;
+  uint32_t g_cur = 0;
   unsigned long long c_vert = 0, c_edge = 0, c_cand = 0, c_move = 0, c_medge = 0;
   bool worked = false;  // (block-uniform: the block took a chunk)
   for (;;) {
@@ -984,8 +989,15 @@ __device__ __forceinline__ void move_body(const MoveArgs& a, uint32_t bid, uint3
           }
           if (lane == 0) {
             st_dev(&a.vaff[u], (uint8_t)0);
-            if (a.gel) unsafeAtomicAdd(&a.gel[a.gid[u]], be);
-            else el_acc += be;
+            if (a.gel) {
+              const uint32_t k = a.gid[u];
+              if (k != g_cur) {
+                if (el_acc != 0) unsafeAtomicAdd(&a.gel[g_cur], el_acc);
+                g_cur = k;
+                el_acc = 0;
+              }
+            }
+            el_acc += be;
             ++c_vert;
             c_edge += d;
           }
@@ -1041,6 +1053,10 @@ __device__ __forceinline__ void move_body(const MoveArgs& a, uint32_t bid, uint3
             mask = group_mask(live);
           }
         }
+        if (a.gel && el_acc != 0) {
+          unsafeAtomicAdd(&a.gel[g_cur], el_acc);
+          el_acc = 0;
+        }
         continue;
       }
     }
@@ -1064,6 +1080,10 @@ __device__ __forceinline__ void move_body(const MoveArgs& a, uint32_t bid, uint3
         Ku = a.vtot[u];
       }
       visit(act, u, o, d, cu, Ku);
+    }
+    if (a.gel && el_acc != 0) {
+      unsafeAtomicAdd(&a.gel[g_cur], el_acc);
+      el_acc = 0;
     }
   }
   if (!worked) return;  // nothing to add up (a block past a short list)
