@@ -341,22 +341,27 @@ __global__ void active_mask_kernel(const uint32_t* gid, uint64_t SL, const uint8
     mask[v] = (!ex || ex[v]) && !done[gid[v]];
 }
 
-// Flagged entries of each graph range [voff[k], voff[k+1]): one block per
-// graph, a block-wide sum, one store.
+// Flagged entries of each graph range [voff[k], voff[k+1]): block b takes
+// slice b % kRangeSlices of graph b / kRangeSlices (every kRangeSlices-th
+// kB-entry tile of the range), sums it and adds the sum to out[k] (zeroed
+// by the launcher), so a large graph's range is read by many blocks.
+constexpr uint64_t kRangeSlices = 32;
 __global__ __launch_bounds__(kB) void range_count_kernel(const uint8_t* f, const uint64_t* voff, uint64_t G,
                                                          unsigned long long* out) {
   __shared__ unsigned long long s_part[kB / 64];
   const int lane = threadIdx.x % 64, wv = threadIdx.x / 64;
-  for (uint64_t k = blockIdx.x; k < G; k += gridDim.x) {
+  for (uint64_t b = blockIdx.x; b < G * kRangeSlices; b += gridDim.x) {  // block-uniform
+    const uint64_t k = b / kRangeSlices, j = b % kRangeSlices;
+    const uint64_t lo = voff[k], hi = voff[k + 1];
     unsigned long long n = 0;
-    for (uint64_t u = voff[k] + threadIdx.x; u < voff[k + 1]; u += kB) n += f[u] != 0;
+    for (uint64_t u = lo + j * kB + threadIdx.x; u < hi; u += kRangeSlices * kB) n += f[u] != 0;
     for (int m = 32; m > 0; m >>= 1) n += __shfl_xor(n, m, 64);
     if (lane == 0) s_part[wv] = n;
     __syncthreads();
     if (threadIdx.x == 0) {
       unsigned long long t = 0;
       for (int i = 0; i < kB / 64; ++i) t += s_part[i];
-      out[k] = t;
+      if (t) atomicAdd(&out[k], t);
     }
     __syncthreads();
   }
@@ -446,7 +451,9 @@ void k_active_mask(const uint32_t* gid, uint64_t SL, const uint8_t* ex, const ui
   if (SL) LV_UPD_LAUNCH(active_mask_kernel, blocks_for(SL, kB), kB, st, gid, SL, ex, done, mask);
 }
 void k_range_count(const uint8_t* f, const uint64_t* voff, uint64_t G, unsigned long long* out, hipStream_t st) {
-  if (G) LV_UPD_LAUNCH(range_count_kernel, blocks_for(G, 1), kB, st, f, voff, G, out);
+  if (!G) return;
+  k_fill_u64(reinterpret_cast<uint64_t*>(out), 0, G, st);
+  LV_UPD_LAUNCH(range_count_kernel, blocks_for(G * kRangeSlices, 1), kB, st, f, voff, G, out);
 }
 
 void k_graph_chunks(const MoveArgs& a, hipStream_t st) {
