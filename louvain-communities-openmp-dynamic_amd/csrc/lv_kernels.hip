@@ -806,9 +806,15 @@ enum { MOVE_LDS = 0, MOVE_HBM = 1, MOVE_HYBRID = 2, MOVE_HYBRID_X = 3 };
 //
 // The body runs as block `bid` of `nblk` (a launch of its own, or the
 // first blocks of the fused worklist launch).
-template <int BLOCK, int G, int CAP, class WT, int MODE_>
+// BT: the batched form's per-graph state (a.gid and what it indexes) is
+// read; without it a.gid must be null (one graph: its registers are spared).
+template <int BLOCK, int G, int CAP, class WT, int MODE_, bool BT = true>
 __device__ __forceinline__ void move_body(const MoveArgs& a, uint32_t bid, uint32_t nblk) {
   constexpr bool EXTRAS = MODE_ == MOVE_HYBRID_X;
+  // the batched form's graph of each entry read with it, its M and
+  // community-0 id loaded ahead of their use (the validated kernel, short of
+  // registers, looks them up where used)
+  constexpr bool EARLY = BT && !EXTRAS;
   constexpr int MODE = EXTRAS ? MOVE_HYBRID : MODE_;
   constexpr int GPB = BLOCK / G;
   constexpr int LCAP = MODE == MOVE_HBM ? 1 : CAP;
@@ -898,8 +904,19 @@ __device__ __forceinline__ void move_body(const MoveArgs& a, uint32_t bid, uint3
         __syncthreads();
       }
     };
-    auto visit = [&](bool act, uint32_t u, uint64_t o, uint32_t d, uint32_t cu, double Ku) __attribute__((always_inline)) {
+    // (gu: u's graph in the batched form, read with the entry; its M and
+    // community-0 id are loaded at once, ahead of their use)
+    auto visit = [&](bool act, uint32_t u, uint64_t o, uint32_t d, uint32_t cu, double Ku,
+                     uint32_t gu) __attribute__((always_inline)) {
       if (act && a.stamp && lane == 0) st_dev(&a.stamp[u], cur_serial(a));
+      double Mu = a.M;
+      uint32_t zu = 0;
+      if constexpr (EARLY) {
+        if (act && a.gid) {
+          Mu = a.gM[gu];
+          zu = a.gzero[gu];
+        }
+      }
       bool hbm = MODE == MOVE_HBM;
       uint64_t cap = CAP;
       uint32_t *tkey = hkey, *tpos = hpos;
@@ -954,7 +971,7 @@ __device__ __forceinline__ void move_body(const MoveArgs& a, uint32_t bid, uint3
         bp = kNoPos;
         bc = 0;
         if (act) {
-          const double Mu = vertex_M(a, u);
+          if constexpr (EXTRAS) Mu = vertex_M(a, u);
           if (hbm) table_best<G>(a, cu, Ku, Mu, tkey, tpos, tval, cap, lane, be, bp, bc, c_cand);
           else table_best<G>(a, cu, Ku, Mu, lkey, lpos, lval, cap, lane, be, bp, bc, c_cand);
         }
@@ -977,7 +994,7 @@ __device__ __forceinline__ void move_body(const MoveArgs& a, uint32_t bid, uint3
         } else {
           // a candidate exists (bp set) and it is not community 0 of u's
           // graph (with no candidate bc is 0, as the reference's (0, 0))
-          if (bp != kNoPos && bc != vertex_zero(a, u)) {
+          if (bp != kNoPos && bc != (EXTRAS ? vertex_zero(a, u) : zu)) {
             if (lane == 0) {
               unsafeAtomicAdd(&a.ctot[cu], -Ku);
               unsafeAtomicAdd(&a.ctot[bc], Ku);
@@ -989,8 +1006,8 @@ __device__ __forceinline__ void move_body(const MoveArgs& a, uint32_t bid, uint3
           }
           if (lane == 0) {
             st_dev(&a.vaff[u], (uint8_t)0);
-            if (a.gel) {
-              const uint32_t k = a.gid[u];
+            if (BT && a.gel) {
+              const uint32_t k = EARLY ? gu : a.gid[u];
               if (k != g_cur) {
                 if (el_acc != 0) unsafeAtomicAdd(&a.gel[g_cur], el_acc);
                 g_cur = k;
@@ -1026,15 +1043,16 @@ __device__ __forceinline__ void move_body(const MoveArgs& a, uint32_t bid, uint3
           // with the window; its community changes only when it is visited)
           const uint64_t idx = c0 + (j0 + lane) * step;
           const bool in = ch < nchunks && j0 + lane < clen && idx < n;
-          uint32_t ul = 0, dl = 0, cl = 0;
+          uint32_t ul = 0, dl = 0, cl = 0, gl = 0;
           uint64_t ol = 0;
           double kl = 0;
           bool ok = false, pre = false;
           if (in) {
             ul = a.list ? a.list[idx] : (uint32_t)(a.base + idx);
             dl = a.g.deg[ul];
+            if (EARLY && a.gid) gl = a.gid[ul];
             ok = (!a.g.ex || a.g.ex[ul]) && (a.choose_only || !a.fa || a.fa[ul]) && dl <= a.hub_min &&
-                 !vertex_skipped(a, ul);
+                 !(EARLY ? a.gid && a.gmask[gl] : BT && vertex_skipped(a, ul));
           }
           bool live = ok && (a.choose_only || ld_dev(&a.vaff[ul]));
           const bool once = a.choose_only || a.sparse == 2;
@@ -1048,12 +1066,12 @@ __device__ __forceinline__ void move_body(const MoveArgs& a, uint32_t bid, uint3
             }
             const int b = __ffsll((long long)mask) - 1;
             visit(true, (uint32_t)__shfl(ul, b, G), (uint64_t)__shfl(ol, b, G), (uint32_t)__shfl(dl, b, G),
-                  (uint32_t)__shfl(cl, b, G), __shfl(kl, b, G));
+                  (uint32_t)__shfl(cl, b, G), __shfl(kl, b, G), (uint32_t)__shfl(gl, b, G));
             live = once ? live && lane > b : ok && lane > b && ld_dev(&a.vaff[ul]);
             mask = group_mask(live);
           }
         }
-        if (a.gel && el_acc != 0) {
+        if (BT && a.gel && el_acc != 0) {
           unsafeAtomicAdd(&a.gel[g_cur], el_acc);
           el_acc = 0;
         }
@@ -1063,12 +1081,13 @@ __device__ __forceinline__ void move_body(const MoveArgs& a, uint32_t bid, uint3
     for (uint64_t j = 0; j < clen; ++j) {
       const uint64_t idx = c0 + j * step;
       bool act = false;
-      uint32_t u = 0;
+      uint32_t u = 0, gu = 0;
       if (ch < nchunks && idx < n) {
         u = a.list ? a.list[idx] : (uint32_t)(a.base + idx);
         act = (!a.g.ex || a.g.ex[u]) && (a.choose_only || (ld_dev(&a.vaff[u]) && (!a.fa || a.fa[u])));
         if constexpr (MODE == MOVE_HYBRID) act = act && a.g.deg[u] <= a.hub_min;
-        act = act && !vertex_skipped(a, u);
+        if (EARLY && a.gid) gu = a.gid[u];
+        act = act && !(EARLY ? a.gid && a.gmask[gu] : BT && vertex_skipped(a, u));
       }
       uint64_t o = 0;
       uint32_t d = 0, cu = 0;
@@ -1079,9 +1098,9 @@ __device__ __forceinline__ void move_body(const MoveArgs& a, uint32_t bid, uint3
         cu = ld_dev(&a.vcom[u]);
         Ku = a.vtot[u];
       }
-      visit(act, u, o, d, cu, Ku);
+      visit(act, u, o, d, cu, Ku, gu);
     }
-    if (a.gel && el_acc != 0) {
+    if (BT && a.gel && el_acc != 0) {
       unsafeAtomicAdd(&a.gel[g_cur], el_acc);
       el_acc = 0;
     }
@@ -1102,6 +1121,19 @@ __device__ __forceinline__ void move_body(const MoveArgs& a, uint32_t bid, uint3
 template <int BLOCK, int G, int CAP, class WT, int MODE_>
 __global__ __launch_bounds__(BLOCK) void move_kernel(MoveArgs a) {
   move_body<BLOCK, G, CAP, WT, MODE_>(a, blockIdx.x, gridDim.x);
+}
+
+// The batched form's id-order kernel, held to three waves a SIMD (its
+// per-graph lookups would otherwise take it past 168 registers, to two).
+template <int BLOCK, int G, int CAP, class WT, int MODE_>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3))) void move_bt_kernel(MoveArgs a) {
+  move_body<BLOCK, G, CAP, WT, MODE_>(a, blockIdx.x, gridDim.x);
+}
+
+// The id-order kernel of one graph (a.gid null).
+template <int BLOCK, int G, int CAP, class WT, int MODE_>
+__global__ __launch_bounds__(BLOCK) void move_one_kernel(MoveArgs a) {
+  move_body<BLOCK, G, CAP, WT, MODE_, false>(a, blockIdx.x, gridDim.x);
 }
 
 // ------------------------------------------------------- hub batches
@@ -1905,7 +1937,7 @@ __global__ __launch_bounds__(kBlock) void wl_fused_kernel(MoveArgs a, MoveArgs h
   if (threadIdx.x == 0) s_made = 0;
   __syncthreads();
   if (blockIdx.x < nlo) {
-    move_body<kBlock, 64, kHybridLds, WT, MOVE_HYBRID>(a, blockIdx.x, nlo);
+    move_body<kBlock, 64, kHybridLds, WT, MOVE_HYBRID, false>(a, blockIdx.x, nlo);
   } else {
     hub_tab_body<WT>(h, blockIdx.x - nlo, gridDim.x - nlo, s_made);
     // the hubs queued for the edge-scanning kernels (queues 1-3: tables
@@ -3015,9 +3047,12 @@ static void move_tier(const MoveArgs& a, int tier, hipStream_t st) {
                     256, st, a);
         else
           LV_LAUNCH((move_kernel<64, 64, kHybridLds, WT, MOVE_HYBRID_X>), grid_for(groups, 1, a.gblocks), 64, st, a);
+      } else if (!a.chunk && groups >= 64 && !a.gid) {
+        LV_LAUNCH((move_one_kernel<256, 64, kHybridLds, WT, MOVE_HYBRID>), grid_for(groups, 4, (a.gblocks + 3) / 4),
+                  256, st, a);
       } else if (!a.chunk && groups >= 64) {
-        LV_LAUNCH((move_kernel<256, 64, kHybridLds, WT, MOVE_HYBRID>), grid_for(groups, 4, (a.gblocks + 3) / 4), 256,
-                  st, a);
+        LV_LAUNCH((move_bt_kernel<256, 64, kHybridLds, WT, MOVE_HYBRID>), grid_for(groups, 4, (a.gblocks + 3) / 4),
+                  256, st, a);
       } else {
         LV_LAUNCH((move_kernel<64, 64, kHybridLds, WT, MOVE_HYBRID>), grid_for(groups, 1, a.gblocks), 64, st, a);
       }
@@ -3071,7 +3106,7 @@ bool k_wl_fused(const MoveArgs& a, const MoveArgs& h, const WlQueues& qs, const 
   // the id-order part as move_tier launches it with spread chunks and four
   // streams a block; other schedules take the separate launches
   const uint64_t streams = a.max_streams ? a.max_streams : 1;
-  if (a.chunk || a.validate || a.xids || a.ctab || streams < 64) return false;
+  if (a.chunk || a.validate || a.xids || a.ctab || a.gid || streams < 64) return false;  // (one graph)
   const uint64_t nlo = grid_for(streams, 4, (a.gblocks + 3) / 4);
   const uint64_t grid = nlo + hub_tab_grid(h);
   if (f64) LV_LAUNCH(wl_fused_kernel<double>, grid, kBlock, st, a, h, (uint32_t)nlo, qs, lens);
