@@ -239,6 +239,12 @@ struct lv_context {
     HIPCHK(hipEventRecord(e, s ? s : st));
     return e;
   }
+  // The start event of a timed launch: a 40 us hold first, so the device
+  // meets the event only once the launch behind it is queued too.
+  hipEvent_t record_start(hipStream_t s = nullptr) {
+    k_hold(40, s ? s : st);
+    return record(s);
+  }
   // Side streams of the worklist sub-rounds (created on first use, default
   // priority): the hub kernels run there beside the low-degree kernel on st.
   hipStream_t side[2] = {nullptr, nullptr};
@@ -548,7 +554,7 @@ void launch_move(lv_context* c, const MoveArgs& a, int tier, bool f64, hipStream
   // spread chunks do not use it)
   if (tier != kTierHubBatch && a.chunk) k_fill_u64(reinterpret_cast<uint64_t*>(a.work), 0, 1, s);
   hipEvent_t e0 = nullptr;
-  if (c->profiling) e0 = c->record(s);
+  if (c->profiling) e0 = c->record_start(s);
   if (tier == kTierHubBatch) k_hub_batch(a, s);
   else if (tier == kTierMidHub || tier == kTierSmallHub) k_hub_mid(a, tier == kTierMidHub ? 1 : 0, s);
   else if (tier == kTierTabHub) k_hub_tab(a, s);
@@ -1095,7 +1101,7 @@ int local_move_worklist(lv_context* c, const Level& lv, uint32_t* vcom, const do
       // Every hub has a table: the tabled hubs and the id-order kernel in one
       // launch, then the flags of the hubs that moved
       hipEvent_t f0 = nullptr;
-      if (c->profiling && S && a.ht_cap && !sides) f0 = c->record();
+      if (c->profiling && S && a.ht_cap && !sides) f0 = c->record_start();
       const bool fused = S && a.ht_cap && !sides && k_wl_fused(a, ah, qs, lens, lv.f64, c->st);
       if (f0 && fused) c->prof.push_back({kSlotFused, {f0, c->record()}});
       ah.mv_piece = nullptr;  // (the separate launches: a flag pass reads the entries)

@@ -293,6 +293,7 @@ void k_ht_build(GView g, bool f64, const uint32_t* vcom, const uint32_t* list, c
                 const uint64_t* off, uint32_t* cap, uint32_t* key, int32_t* cnt, double* w,
                 unsigned long long* ndrop, hipStream_t st);
 void k_flag_nonzero(const uint32_t* a, uint64_t n, uint8_t* flags, hipStream_t st);
+void k_hold(uint32_t us, hipStream_t st);
 // Neighbours of the tabled hubs that moved (a.mv_*), flagged over the grid;
 // with carry, the entries of worklist queues 1-3 back to the next sub-round.
 void k_moved_flag(const MoveArgs& a, const WlQueues& qs, const uint32_t* lens, bool carry, hipStream_t st);

@@ -3117,6 +3117,16 @@ __global__ void flag_nonzero_kernel(const uint32_t* a, uint64_t n, uint8_t* flag
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
     flags[i] = a[i] != 0;
 }
+// One wave that waits on the device's 100 MHz wall clock for `us`
+// microseconds (bounded: the clock always advances).  Put before the start
+// event of a timed launch, it lets the host write the event, the kernel and
+// the end event into the queue before the device reaches them, so the pair
+// of events brackets the kernel and not the host's launch latency.
+__global__ void hold_kernel(uint64_t ticks) {
+  const uint64_t t0 = wall_clock64();
+  while (wall_clock64() - t0 < ticks) __builtin_amdgcn_s_sleep(16);
+}
+void k_hold(uint32_t us, hipStream_t st) { LV_LAUNCH(hold_kernel, 1, 64, st, (uint64_t)us * 100); }
 void k_flag_nonzero(const uint32_t* a, uint64_t n, uint8_t* flags, hipStream_t st) {
   if (n) LV_LAUNCH(flag_nonzero_kernel, grid_for(n, kBlock * 4), kBlock, st, a, n, flags);
 }
