@@ -498,7 +498,7 @@ def main():
                     help="timed batches the reference also runs (dQ and the CPU baseline), evenly spread")
     ap.add_argument("--no-plain", action="store_true", help="skip the frontier and plain paths")
     ap.add_argument("--no-static", action="store_true", help="skip the (reported) GPU Static run")
-    ap.add_argument("--static-runs", type=int, default=3,
+    ap.add_argument("--static-runs", type=int, default=5,
                     help="GPU and reference Static runs reported (their mean; the DF chain starts from the "
                          "reference's first)")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving segment")
